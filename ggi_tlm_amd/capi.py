@@ -1,0 +1,252 @@
+"""ctypes binding of the C ABI declared in include/tlmscn.h.
+
+The same signature table binds any library exporting the ABI under a
+prefix ("tlm_" for the HIP engine, libtlmscn.so).  Python is only the host
+shell here; every hot-path operation runs in the library.  There is no
+fallback: if the engine library is missing, :func:`engine_library` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from .problem import Tables
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ENGINE_SO = os.path.join(HERE, "libtlmscn.so")
+
+FORM_PLAIN_FP64 = 0
+FORM_PRECISION_FP32 = 1
+
+
+class TlmGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("n_timesteps", C.c_int32), ("dl", C.c_double), ("dt", C.c_double),
+                ("reduced_c_factor", C.c_double),
+                ("r_xmin", C.c_double), ("r_xmax", C.c_double), ("r_ymin", C.c_double),
+                ("r_ymax", C.c_double), ("r_zmin", C.c_double), ("r_zmax", C.c_double)]
+
+
+_p = C.c_void_p
+_i32 = C.c_int32
+_h = C.c_void_p
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "create": (C.c_int, [C.POINTER(TlmGrid), _i32, C.POINTER(C.c_void_p)]),
+    "destroy": (None, [_h]),
+    "last_error": (C.c_char_p, [_h]),
+    "device_bytes": (C.c_int64, [_h]),
+    "set_cell_codes": (C.c_int, [_h, _p, _i32, _p, _p, _p, _p, _p]),
+    "set_face_codes": (C.c_int, [_h, _p, _i32, _p, _p, _p, _p]),
+    "set_volume_materials": (C.c_int, [_h, _i32, _p]),
+    "set_surface_materials": (C.c_int, [_h, _i32, _p]),
+    "set_pml": (C.c_int, [_h, _i32, _p, _i32, _p]),
+    "set_excitation_functions": (C.c_int, [_h, _i32, _p, _p]),
+    "set_point_sources": (C.c_int, [_h, _i32, _p, _i32, _i32]),
+    "set_huygens_surface": (C.c_int, [_h, _i32, C.c_double, _p, _p, _i32, _p]),
+    "set_output_points": (C.c_int, [_h, _i32, _p, _p]),
+    "set_dft_frequencies": (C.c_int, [_h, _i32, _p]),
+    "upload_links": (C.c_int, [_h, _p]),
+    "download_links": (C.c_int, [_h, _p]),
+    "excitation": (C.c_int, [_h, _i32]),
+    "scatter": (C.c_int, [_h, _i32]),
+    "connect": (C.c_int, [_h, _i32]),
+    "outer_boundary": (C.c_int, [_h]),
+    "step": (C.c_int, [_h, _i32, _i32]),
+    "synchronize": (C.c_int, [_h]),
+    "set_output_capacity": (C.c_int, [_h, _i32]),
+    "read_cell_outputs": (C.c_int, [_h, _p, _i32, C.POINTER(_i32), C.POINTER(_i32)]),
+    "read_dft": (C.c_int, [_h, _p]),
+}
+
+EXPORTED = tuple("tlm_" + n for n in SIGNATURES)
+
+
+class TlmError(RuntimeError):
+    pass
+
+
+class Library:
+    """A loaded library exporting the ABI under ``prefix``."""
+
+    def __init__(self, path: str, prefix: str = "tlm_", extra=()):
+        if not os.path.exists(path):
+            raise FileNotFoundError(
+                f"{path} is not built; run __graft_entry__.build() (no CPU fallback exists)")
+        self.path = path
+        self.prefix = prefix
+        self.dll = C.CDLL(path)
+        self.fn = {}
+        for name, (res, args) in list(SIGNATURES.items()) + list(extra):
+            f = getattr(self.dll, prefix + name)
+            f.restype = res
+            f.argtypes = args
+            self.fn[name] = f
+
+
+_engine: Optional[Library] = None
+
+
+def engine_library() -> Library:
+    """The HIP engine (in-tree libtlmscn.so)."""
+    global _engine
+    if _engine is None:
+        _engine = Library(ENGINE_SO, "tlm_")
+    return _engine
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def grid_of(t: Tables) -> TlmGrid:
+    r = t.reflection
+    return TlmGrid(t.nx, t.ny, t.nz, t.n_timesteps, t.dl, t.dt, t.reduced_c_factor,
+                   r[0], r[1], r[2], r[3], r[4], r[5])
+
+
+class Handle:
+    """One solve loaded into a library (engine or oracle), mirroring the
+    reference's per-step call sequence (run_TLM.F90:191-295)."""
+
+    def __init__(self, lib: Library, tables: Tables, form: int = FORM_PLAIN_FP64,
+                 output_capacity: Optional[int] = None):
+        self.lib = lib
+        self.t = tables
+        self._keep = []
+        h = C.c_void_p()
+        g = grid_of(tables)
+        self._check(lib.fn["create"](C.byref(g), form, C.byref(h)), None)
+        self.h = h
+        try:
+            self._load(tables, output_capacity)
+        except Exception:
+            self.close()
+            raise
+
+    def _check(self, rc, h="self"):
+        if rc != 0:
+            msg = ""
+            hh = self.h if h == "self" else h
+            if hh:
+                m = self.lib.fn["last_error"](hh)
+                msg = m.decode() if m else ""
+            raise TlmError(f"{self.lib.prefix}call failed (status {rc}): {msg}")
+
+    def _arr(self, a, dtype):
+        a = np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel(order="F")) if a is not None else None
+        self._keep.append(a)
+        return a
+
+    def _load(self, t: Tables, cap):
+        f = self.lib.fn
+        i32 = np.int32
+        if cap is not None:
+            self._check(f["set_output_capacity"](self.h, int(cap)))
+        self._check(f["set_volume_materials"](self.h, len(t.volume_materials),
+                                               _ptr(np.ascontiguousarray(t.volume_materials))))
+        self._check(f["set_surface_materials"](self.h, len(t.surface_materials),
+                                                _ptr(self._arr(t.surface_materials, i32))))
+        self._check(f["set_cell_codes"](
+            self.h, _ptr(self._arr(t.cell_code, i32)), t.n_special_cells,
+            _ptr(self._arr(t.cell_to_material, i32)), _ptr(self._arr(t.cell_to_pml, i32)),
+            _ptr(self._arr(t.cell_to_excitation, i32)), _ptr(self._arr(t.cell_to_output, i32)),
+            _ptr(self._arr(t.cell_to_cable, i32))))
+        self._check(f["set_face_codes"](
+            self.h, _ptr(self._arr(t.face_code, i32)), t.n_special_faces,
+            _ptr(self._arr(t.face_to_material, i32)), _ptr(self._arr(t.face_to_excitation, i32)),
+            _ptr(self._arr(t.face_to_output, i32)), _ptr(self._arr(t.face_to_cable, i32))))
+        # materials again so that filter storage sees the cell codes
+        self._check(f["set_volume_materials"](self.h, len(t.volume_materials),
+                                               _ptr(np.ascontiguousarray(t.volume_materials))))
+        self._check(f["set_pml"](self.h, len(t.pml_params), _ptr(np.ascontiguousarray(t.pml_params)),
+                                 len(t.pml_cell_param), _ptr(self._arr(t.pml_cell_param, i32))))
+        nf = t.value.shape[1] if t.value.ndim == 2 else 0
+        self._check(f["set_excitation_functions"](self.h, nf, _ptr(self._arr(t.value, np.float64)),
+                                                   _ptr(self._arr(t.value_face, np.float64))))
+        self._check(f["set_point_sources"](self.h, len(t.point_sources),
+                                            _ptr(np.ascontiguousarray(t.point_sources)),
+                                            t.n_cell_excitation_rows, t.n_face_excitation_rows))
+        if len(t.huygens_faces):
+            self._check(f["set_huygens_surface"](
+                self.h, t.huygens_function, t.huygens_offset_min,
+                _ptr(self._arr(t.huygens_Ei, np.float64)), _ptr(self._arr(t.huygens_Hi, np.float64)),
+                len(t.huygens_faces), _ptr(np.ascontiguousarray(t.huygens_faces))))
+        self._check(f["set_dft_frequencies"](self.h, len(t.dft_frequencies),
+                                              _ptr(self._arr(t.dft_frequencies, np.float64))))
+        self._check(f["set_output_points"](self.h, len(t.output_rows),
+                                            _ptr(self._arr(t.output_rows, i32)),
+                                            _ptr(self._arr(t.output_components, i32))))
+
+    # ---- reference call sequence ---------------------------------------
+    def excitation(self, timestep):
+        self._check(self.lib.fn["excitation"](self.h, int(timestep)))
+
+    def scatter(self, timestep):
+        self._check(self.lib.fn["scatter"](self.h, int(timestep)))
+
+    def connect(self, timestep):
+        self._check(self.lib.fn["connect"](self.h, int(timestep)))
+
+    def outer_boundary(self):
+        self._check(self.lib.fn["outer_boundary"](self.h))
+
+    def step(self, first_timestep, n_steps):
+        self._check(self.lib.fn["step"](self.h, int(first_timestep), int(n_steps)))
+
+    def synchronize(self):
+        self._check(self.lib.fn["synchronize"](self.h))
+
+    def device_bytes(self) -> int:
+        return int(self.lib.fn["device_bytes"](self.h))
+
+    def upload_links(self, V):
+        V = np.asfortranarray(V, dtype=np.float64)
+        self._check(self.lib.fn["upload_links"](self.h, _ptr(V)))
+
+    def download_links(self, V=None):
+        t = self.t
+        if V is None:
+            V = np.zeros((12, t.nx + 2, t.ny + 2, t.nz), dtype=np.float64, order="F")
+        self._check(self.lib.fn["download_links"](self.h, _ptr(V)))
+        return V
+
+    def read_cell_outputs(self, max_steps=None):
+        """Recorded cell_output_field rows: array (steps, n_output_cells, 6)
+        and the first timestep."""
+        rows = self.t.n_output_cells
+        cap = max_steps if max_steps is not None else self.t.n_timesteps
+        out = np.zeros((max(cap, 1), max(rows, 1), 6), dtype=np.float64)
+        n = C.c_int32(0)
+        first = C.c_int32(0)
+        self._check(self.lib.fn["read_cell_outputs"](self.h, _ptr(out), int(cap), C.byref(n), C.byref(first)))
+        return out[: n.value, :rows], first.value
+
+    def read_dft(self):
+        npnt, nfreq = len(self.t.output_rows), len(self.t.dft_frequencies)
+        out = np.zeros((max(npnt, 1), max(nfreq, 1), 2), dtype=np.float64)
+        self._check(self.lib.fn["read_dft"](self.h, _ptr(out)))
+        return out[:npnt, :nfreq, 0] + 1j * out[:npnt, :nfreq, 1]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fn["destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def probe_series(cell_outputs: np.ndarray, tables: Tables) -> np.ndarray:
+    """(steps, n_points): the output-point values picked from recorded
+    cell_output_field rows (cell_output_point, output_points.F90:545-552)."""
+    rows = tables.output_rows - 1
+    comps = tables.output_components - 1
+    return cell_outputs[:, rows, comps]

@@ -1,0 +1,236 @@
+/*
+ * tlmscn.h -- C ABI of the MI355X (gfx950) SCN scatter/connect engine.
+ *
+ * One handle holds one TLM solve resident in HBM.  The entry points replace
+ * the argument-less Fortran hot-path subroutines of GGI_TLM v2.1.7 and take
+ * the reference's own module tables (copied in; the caller keeps ownership).
+ * Every table uses the reference's conventions: 1-based row numbers, scan
+ * order with x fastest, Fortran column-major for 2-D tables.
+ *
+ * Reference call sites replaced (SRC/TLM_SOLVER/run_TLM.F90):
+ *   CALL excitation()      :242  -> tlm_excitation
+ *   CALL scatter()         :244  -> tlm_scatter
+ *   CALL cell_output()     :246  -> tlm_read_cell_outputs (batched read)
+ *   CALL connect()         :277  -> tlm_connect
+ *   CALL outer_boundary()  :283  -> tlm_outer_boundary
+ *   the whole loop body    :191-295 -> tlm_step (fused fast path)
+ *
+ * Error convention: every function returns 0 (TLM_OK) on success and a
+ * nonzero TLM_ERR_* code otherwise; tlm_last_error() returns the message.
+ * The reference's convention (write(*,*) msg; STOP 1) is applied by the
+ * Fortran wrapper module on a nonzero status.
+ *
+ * Threading: one handle per solve, driven from one host thread; each handle
+ * owns one HIP stream.  Calls on one handle are not re-entrant.
+ */
+#ifndef TLMSCN_H
+#define TLMSCN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TLM_OK 0
+#define TLM_ERR_ARG 1         /* bad argument or inconsistent table */
+#define TLM_ERR_STATE 2       /* call out of order (e.g. step before tables) */
+#define TLM_ERR_HIP 3         /* HIP runtime failure */
+#define TLM_ERR_UNSUPPORTED 4 /* feature that stays on the host path */
+#define TLM_ERR_NOMEM 5       /* device allocation failed */
+
+/* Numerical forms.  PLAIN: FP64 link planes, bit-faithful to the reference
+ * sequential build's operation order.  PRECISION: FP32 link planes; filter
+ * states, PML accumulators and the probe DFT stay FP64. */
+#define TLM_FORM_PLAIN_FP64 0
+#define TLM_FORM_PRECISION_FP32 1
+
+/* Largest Zfilter order accepted (a and b polynomials of order <= this). */
+#define TLM_MAX_FILTER_ORDER 8
+
+/* Volume material types (SRC/TLM_MODULES/TLM_material_modules.F90:72-76). */
+#define TLM_VMAT_PEC 1
+#define TLM_VMAT_PMC 2
+#define TLM_VMAT_DISPERSIVE 3
+
+/* Surface material types (TLM_material_modules.F90:153-169).  Only PEC, PMC
+ * and FREE_SPACE faces are updated on the device; the others (thin layers,
+ * diodes, switches, spice ports) stay on the host path. */
+#define TLM_SMAT_PEC 1
+#define TLM_SMAT_PMC 2
+#define TLM_SMAT_FREE_SPACE 4
+
+typedef struct tlm_handle tlm_handle;
+
+/* TLM_general::{nx,ny,nz,n_timesteps,dl,dt,reduced_c_factor,R_*}
+ * (SRC/TLM_MODULES/TLM_Modules.F90); one z-slab covering 1..nz. */
+typedef struct tlm_grid {
+  int32_t nx, ny, nz;
+  int32_t n_timesteps; /* length of the excitation function tables */
+  double dl, dt;
+  double reduced_c_factor;
+  double r_xmin, r_xmax, r_ymin, r_ymax, r_zmin, r_zmax;
+} tlm_grid;
+
+/* One row of TLM_volume_materials::volume_material_list with the fields the
+ * scatter reads (scatter.F90:286-405).  The Zfilters are the slow parts
+ * after Z_fast_slow_decomposition (filter_subroutines.F90:1145-1199): a and
+ * b coefficients of z^-1, indices 0..order, both of the same order. */
+typedef struct tlm_volume_material {
+  int32_t type;
+  int32_t eps_filter_exists;
+  int32_t mu_filter_exists;
+  int32_t eps_order;
+  int32_t mu_order;
+  int32_t pad_;
+  double Ys_eps_f, Ge, Zs_mu_f, Rm;
+  double eps_a[TLM_MAX_FILTER_ORDER + 1];
+  double eps_b[TLM_MAX_FILTER_ORDER + 1];
+  double mu_a[TLM_MAX_FILTER_ORDER + 1];
+  double mu_b[TLM_MAX_FILTER_ORDER + 1];
+} tlm_volume_material;
+
+/* PML_module::PML_material_type without the d_x,d_y,d_z depth fields
+ * (SRC/TLM_MODULES/PML_module.F90:102-114). */
+typedef struct tlm_pml_parameter {
+  double sx, sy, sz;
+  double ax, ay, az;
+  double exp_x, exp_y, exp_z;
+  double Csx, Csy, Csz;
+  double Csx2, Csy2, Csz2;
+  double Csy_sz, Csz_sx, Csx_sy;
+} tlm_pml_parameter;
+
+/* One entry of TLM_excitation::excitation_points, reduced to what
+ * Point_excitation (SRC/EXCITATION/excitation_points.F90:231-335) uses.
+ * at_face=0: cell-centre source, row = cell_excitation_field_number.
+ * at_face=1: face source, row = face_excitation_field_number. */
+typedef struct tlm_point_source {
+  int32_t function;    /* excitation function number, 1-based */
+  int32_t component;   /* 1..12: Ex..Hz, Jx..Jz, Mx..Mz */
+  int32_t source_type; /* 0 hard, 1 soft */
+  int32_t at_face;
+  int32_t row;
+} tlm_point_source;
+
+/* One face of TLM_excitation::huygens_surface
+ * (SRC/EXCITATION/huygens_surface.F90:734-848). */
+typedef struct tlm_huygens_face {
+  int32_t row;        /* face_excitation_field_number_list entry; 0 = none */
+  int32_t nx, ny, nz; /* outward normal, one of them +-1 */
+  double offset;      /* k . r of the face centre */
+} tlm_huygens_face;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+int tlm_create(const tlm_grid* grid, int32_t form, tlm_handle** out);
+void tlm_destroy(tlm_handle* h);
+const char* tlm_last_error(const tlm_handle* h);
+/* Device bytes held by the handle (link planes, states, tables). */
+int64_t tlm_device_bytes(const tlm_handle* h);
+
+/* ---- tables (set once, after tlm_create, before the first step) -------- */
+
+/* mesh::cell_centre_update_code(1:nx*ny*nz) and the cell_update_code_to_*
+ * lookup tables (SRC/TLM_MODULES/Mesh_Modules.F90:139-150, built by
+ * set_cell_update_codes, SRC/TLM_SOLVER/set_update_codes.F90:38-363, and
+ * allocate_volume_material_filter_data, SRC/MATERIALS/
+ * TLM_volume_materials.F90:401-518).  to_material is the Fortran (n,2)
+ * array, column-major.  to_cable must be all zero on the device path. */
+int tlm_set_cell_codes(tlm_handle* h, const int32_t* cell_centre_update_code,
+                       int32_t n_special_cells, const int32_t* to_material,
+                       const int32_t* to_pml, const int32_t* to_excitation,
+                       const int32_t* to_output, const int32_t* to_cable);
+
+/* mesh::face_update_code(1:number_of_face_codes) and face_update_code_to_*
+ * (set_update_codes.F90:383-791).  to_material is the Fortran (n,3) array,
+ * column-major; column 1 is the signed surface material number. */
+int tlm_set_face_codes(tlm_handle* h, const int32_t* face_update_code,
+                       int32_t n_special_faces, const int32_t* to_material,
+                       const int32_t* to_excitation, const int32_t* to_output,
+                       const int32_t* to_cable);
+
+/* TLM_volume_materials::volume_material_list(1:n). */
+int tlm_set_volume_materials(tlm_handle* h, int32_t n,
+                             const tlm_volume_material* materials);
+
+/* TLM_surface_materials::surface_material_list(1:n)%type. */
+int tlm_set_surface_materials(tlm_handle* h, int32_t n, const int32_t* type);
+
+/* PML_module::PML_parameters(1:n_params) and
+ * PML_cell_data(1:n_cells)%PML_parameter_array_pos
+ * (SRC/MATERIALS/TLM_PML_volume_materials.F90:305-357). */
+int tlm_set_pml(tlm_handle* h, int32_t n_params,
+                const tlm_pml_parameter* params, int32_t n_cells,
+                const int32_t* cell_param);
+
+/* excitation_functions(f)%value(1:n_timesteps) and %value_face, stored as
+ * Fortran (n_timesteps, n_functions) arrays, column-major
+ * (SRC/EXCITATION/excitation.F90:155-559). */
+int tlm_set_excitation_functions(tlm_handle* h, int32_t n_functions,
+                                 const double* value,
+                                 const double* value_face);
+
+/* Point sources in the reference's excitation_points order.
+ * n_cell_rows / n_face_rows = total_number_excitation_cells / _faces. */
+int tlm_set_point_sources(tlm_handle* h, int32_t n,
+                          const tlm_point_source* sources,
+                          int32_t n_cell_rows, int32_t n_face_rows);
+
+/* TLM_excitation::huygens_surface: function number, offset_min, Ei, Hi and
+ * the face list (set_huygens_surface_data, huygens_surface.F90:247-715). */
+int tlm_set_huygens_surface(tlm_handle* h, int32_t function,
+                            double offset_min, const double* Ei,
+                            const double* Hi, int32_t n_faces,
+                            const tlm_huygens_face* faces);
+
+/* Cell-centre output points: row = output_points(i)%cell_output_field_number,
+ * component = output_points(i)%field_component (1..6)
+ * (SRC/OUTPUT/output_points.F90:143-237). */
+int tlm_set_output_points(tlm_handle* h, int32_t n, const int32_t* row,
+                          const int32_t* component);
+
+/* On-the-fly probe DFT frequencies (Hz).  Accumulates, for each output
+ * point p and frequency f, sum_n v_p(n) * exp(-j 2 pi f t_n) * dt with
+ * t_n = (n-1) dt, the cell-output time of step n, in FP64. */
+int tlm_set_dft_frequencies(tlm_handle* h, int32_t n, const double* freq);
+
+/* ---- link state ---------------------------------------------------------- */
+
+/* mesh::V(1:12, 0:nx+1, 0:ny+1, 1:nz), FP64, Fortran layout.  Only the
+ * interior cells are read or written; the x/y halo is left untouched. */
+int tlm_upload_links(tlm_handle* h, const double* V);
+int tlm_download_links(tlm_handle* h, double* V);
+
+/* ---- stepping (reference call order) ------------------------------------- */
+int tlm_excitation(tlm_handle* h, int32_t timestep);
+int tlm_scatter(tlm_handle* h, int32_t timestep);
+int tlm_connect(tlm_handle* h, int32_t timestep);
+int tlm_outer_boundary(tlm_handle* h);
+
+/* Fused fast path: n_steps complete loop bodies starting at first_timestep,
+ * equivalent to excitation, scatter, cell_output, connect, outer_boundary
+ * for each step. */
+int tlm_step(tlm_handle* h, int32_t first_timestep, int32_t n_steps);
+
+/* Wait for all queued work of the handle. */
+int tlm_synchronize(tlm_handle* h);
+
+/* ---- outputs ------------------------------------------------------------- */
+
+/* Drain recorded cell_output_field rows: out receives up to max_steps
+ * blocks of n_output_cells*6 doubles, row-major (step, cell row, component);
+ * *n_read gets the number of steps written, *first_step their first
+ * timestep.  Recording is bounded by the ring capacity set here
+ * (tlm_set_output_capacity, default 4096 steps). */
+int tlm_set_output_capacity(tlm_handle* h, int32_t steps);
+int tlm_read_cell_outputs(tlm_handle* h, double* out, int32_t max_steps,
+                          int32_t* n_read, int32_t* first_step);
+
+/* DFT accumulators: re_im[(p*n_freq + k)*2 + {0,1}]. */
+int tlm_read_dft(tlm_handle* h, double* re_im);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TLMSCN_H */

@@ -1,0 +1,38 @@
+"""Python access to the CPU oracle (test infrastructure only).
+
+Only tests/, the smoke check and bench.py's CPU-baseline leg use this.  The
+oracle library (liboracle_scn.so, built from scn_oracle.c by the Makefile
+here) exports the engine's C ABI under the prefix ``oracle_``, so the same
+host binding drives both.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle_scn.so")
+
+_lib = None
+
+
+def library():
+    from ggi_tlm_amd.capi import Library
+
+    global _lib
+    if _lib is None:
+        _lib = Library(ORACLE_SO, "oracle_",
+                       extra=[("read_face_outputs", (C.c_int, [C.c_void_p, C.c_void_p]))])
+    return _lib
+
+
+def handle(tables, form=0, output_capacity=None):
+    from ggi_tlm_amd.capi import Handle
+
+    return Handle(library(), tables, form, output_capacity)
+
+
+def build():
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", HERE], check=True)

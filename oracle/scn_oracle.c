@@ -1,0 +1,1042 @@
+/*
+ * scn_oracle.c -- CPU oracle for the SCN scatter/connect hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see scn_oracle.h).  A single-threaded FP64
+ * restatement of the GGI_TLM v2.1.7 loop body, kept in the reference's data
+ * layout (V(1:12,0:nx+1,0:ny+1,1:nz), column-major, 12 links of a cell
+ * contiguous) and scan order (x fastest), with every expression written in
+ * the reference's evaluation order.  Build with -ffp-contract=off so no
+ * multiply-add is fused.
+ *
+ * Reference files followed (paths under GGI_TLM/SRC):
+ *   TLM_SOLVER/scatter.F90:216-585        scatter, free space and Thevenin
+ *   TLM_SOLVER/scatter.F90:606-661        add_shunt_admittance / series
+ *   TLM_SOLVER/scatter_PML.F90:22-313     PML cell update (stages 1-13)
+ *   FILTER/filter_subroutines.F90:510-575 evaluate_Zfilter / timeshift
+ *   TLM_SOLVER/connect.F90:180-214        face loop
+ *   TLM_SOLVER/connect_normal_to_{x,y,z}.F90  face update, sources, outputs
+ *   TLM_SOLVER/outer_boundary.F90:71-131  reflection at the mesh boundary
+ *   EXCITATION/excitation.F90:578-626     per-step excitation reset
+ *   EXCITATION/excitation_points.F90:231-335  point sources
+ *   EXCITATION/huygens_surface.F90:734-848    Huygens face sources
+ *   EXCITATION/excitation.F90:645-690     get_interpolated_excitation_value
+ *   OUTPUT/output_points.F90:463-612      cell output capture
+ */
+#include "scn_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* SRC/CONSTANTS/constants.F90:27-39 */
+static const double PI = 3.141592653589793;
+static const double Z0 = 376.73031346177;
+static const double C0 = 2.99792458e8;
+#define Y0 (1.0 / Z0)
+
+/* Port numbering, SRC/TLM_MODULES/Mesh_Modules.F90:83-94 (1-based). */
+enum {
+  Vy_xmin = 1, Vz_xmin, Vy_xmax, Vz_xmax, Vx_ymin, Vz_ymin,
+  Vx_ymax, Vz_ymax, Vx_zmin, Vy_zmin, Vx_zmax, Vy_zmax
+};
+/* Field components, SRC/TLM_MODULES/TLM_Modules.F90:164-176 (1-based). */
+enum { Ex = 1, Ey, Ez, Hx, Hy, Hz, Jx, Jy, Jz, Mx, My, Mz };
+
+#define NW (TLM_MAX_FILTER_ORDER + 1)
+
+struct oracle_handle {
+  tlm_grid g;
+  int nx, ny, nz;
+  char err[512];
+  double* V; /* 12*(nx+2)*(ny+2)*nz */
+  /* cell codes */
+  int32_t* cell_code;
+  int n_special;
+  int32_t *c_mat, *c_filt, *c_pml, *c_exc, *c_out;
+  int n_out_rows, n_exc_rows;
+  /* face codes */
+  int32_t* face_code;
+  int n_faces, n_special_faces;
+  int32_t *f_mat, *f_exc, *f_out;
+  int n_face_out_rows;
+  /* materials */
+  int n_vmat;
+  tlm_volume_material* vmat;
+  int n_smat;
+  int32_t* smat;
+  /* Zfilter states w(0:order), per filter number (1-based) */
+  int n_filters;
+  double *eps_w, *mu_w;
+  int32_t *eps_ord, *mu_ord;
+  /* PML */
+  int n_pml_params, n_pml_cells;
+  tlm_pml_parameter* pml_par;
+  int32_t* pml_cell_param;
+  double* pml_state; /* per PML cell: 27 words, see PS_* */
+  /* excitation */
+  int n_functions;
+  double *value, *value_face;
+  int n_ps;
+  tlm_point_source* ps;
+  int n_cell_rows, n_face_rows;
+  double* cell_exc_field; /* (row, 1:9) row-major [row*9 + c-1] */
+  int32_t* cell_exc_type;  /* (row, 1:6) */
+  double* face_exc_field;  /* (row, side, 1:12) [(row*2+side)*12 + c-1] */
+  int32_t* face_exc_type;  /* (row, side, 1:6) */
+  int huy_function;
+  double huy_offset_min, Ei[3], Hi[3];
+  int n_huy;
+  tlm_huygens_face* huy;
+  /* outputs */
+  double* cell_out; /* (row, 1:6) */
+  double* face_out; /* (row, side, 1:6) */
+  int n_points;
+  int32_t *pt_row, *pt_comp;
+  int n_freq;
+  double* freq;
+  double* dft; /* (point, freq, 2) */
+  int cap, ring_count, ring_first;
+  double* ring; /* cap blocks of n_out_rows*6 */
+};
+
+/* PML_data_type words (PML_module.F90:117-129). */
+enum {
+  PS_Vxt, PS_Vyt, PS_Vzt, PS_Ix, PS_Iy, PS_Iz, PS_Ixt, PS_Iyt, PS_Izt,
+  PS_Vi, /* 12 words */
+  PS_Vyxt = PS_Vi + 12, PS_Vzxt, PS_Vxyt, PS_Vzyt, PS_Vyzt, PS_Vxzt, PS_N
+};
+
+static int fail(oracle_handle* h, int code, const char* msg) {
+  snprintf(h->err, sizeof h->err, "%s", msg);
+  return code;
+}
+
+static inline double* cellV(oracle_handle* h, int cx, int cy, int cz) {
+  /* V(1,cx,cy,cz) with cx in 0..nx+1, cy in 0..ny+1, cz in 1..nz */
+  size_t i = (size_t)cx + (size_t)(h->nx + 2) * ((size_t)cy + (size_t)(h->ny + 2) * (size_t)(cz - 1));
+  return h->V + 12 * i - 1; /* so that p[port] with port 1..12 works */
+}
+
+int oracle_create(const tlm_grid* g, int32_t form, oracle_handle** out) {
+  if (!g || !out || g->nx < 1 || g->ny < 1 || g->nz < 1) return TLM_ERR_ARG;
+  oracle_handle* h = (oracle_handle*)calloc(1, sizeof *h);
+  if (!h) return TLM_ERR_NOMEM;
+  (void)form; /* the oracle is FP64 throughout */
+  h->g = *g;
+  h->nx = g->nx; h->ny = g->ny; h->nz = g->nz;
+  size_t n = (size_t)12 * (g->nx + 2) * (g->ny + 2) * g->nz;
+  h->V = (double*)calloc(n, sizeof(double));
+  if (!h->V) { free(h); return TLM_ERR_NOMEM; }
+  h->cap = 4096;
+  *out = h;
+  return TLM_OK;
+}
+
+void oracle_destroy(oracle_handle* h) {
+  if (!h) return;
+  free(h->V); free(h->cell_code); free(h->c_mat); free(h->c_filt); free(h->c_pml);
+  free(h->c_exc); free(h->c_out); free(h->face_code); free(h->f_mat); free(h->f_exc);
+  free(h->f_out); free(h->vmat); free(h->smat); free(h->eps_w); free(h->mu_w);
+  free(h->eps_ord); free(h->mu_ord); free(h->pml_par); free(h->pml_cell_param);
+  free(h->pml_state); free(h->value); free(h->value_face); free(h->ps);
+  free(h->cell_exc_field); free(h->cell_exc_type); free(h->face_exc_field);
+  free(h->face_exc_type); free(h->huy); free(h->cell_out); free(h->face_out);
+  free(h->pt_row); free(h->pt_comp); free(h->freq); free(h->dft); free(h->ring);
+  free(h);
+}
+
+const char* oracle_last_error(const oracle_handle* h) { return h ? h->err : "null handle"; }
+int64_t oracle_device_bytes(const oracle_handle* h) { (void)h; return 0; }
+
+static int alloc_filters(oracle_handle* h);
+
+static int32_t* dup_i32(const int32_t* p, size_t n) {
+  int32_t* q = (int32_t*)malloc((n ? n : 1) * sizeof(int32_t));
+  if (q && p && n) memcpy(q, p, n * sizeof(int32_t));
+  else if (q) memset(q, 0, (n ? n : 1) * sizeof(int32_t));
+  return q;
+}
+
+int oracle_set_cell_codes(oracle_handle* h, const int32_t* code, int32_t n_special,
+                          const int32_t* to_material, const int32_t* to_pml,
+                          const int32_t* to_excitation, const int32_t* to_output,
+                          const int32_t* to_cable) {
+  size_t N = (size_t)h->nx * h->ny * h->nz;
+  for (int32_t i = 0; i < n_special; i++)
+    if (to_cable && to_cable[i]) return fail(h, TLM_ERR_UNSUPPORTED, "cable cells stay on the host path");
+  h->cell_code = dup_i32(code, N);
+  h->n_special = n_special;
+  h->c_mat = dup_i32(to_material, n_special);
+  h->c_filt = dup_i32(to_material ? to_material + n_special : NULL, n_special);
+  h->c_pml = dup_i32(to_pml, n_special);
+  h->c_exc = dup_i32(to_excitation, n_special);
+  h->c_out = dup_i32(to_output, n_special);
+  int no = 0, ne = 0;
+  for (int32_t i = 0; i < n_special; i++) { no += h->c_out[i] != 0; ne += h->c_exc[i] != 0; }
+  h->n_out_rows = no;
+  h->n_exc_rows = ne;
+  free(h->cell_out);
+  h->cell_out = (double*)calloc((size_t)(no ? no : 1) * 6, sizeof(double));
+  return alloc_filters(h);
+}
+
+int oracle_set_face_codes(oracle_handle* h, const int32_t* code, int32_t n_special,
+                          const int32_t* to_material, const int32_t* to_excitation,
+                          const int32_t* to_output, const int32_t* to_cable) {
+  int nx = h->nx, ny = h->ny, nz = h->nz;
+  size_t nf = (size_t)(nx - 1) * ny * nz + (size_t)nx * (ny - 1) * nz + (size_t)nx * ny * (nz - 1);
+  for (int32_t i = 0; i < n_special; i++)
+    if (to_cable && to_cable[i]) return fail(h, TLM_ERR_UNSUPPORTED, "cable faces stay on the host path");
+  h->n_faces = (int)nf;
+  h->face_code = dup_i32(code, nf);
+  h->n_special_faces = n_special;
+  h->f_mat = dup_i32(to_material, n_special);
+  h->f_exc = dup_i32(to_excitation, n_special);
+  h->f_out = dup_i32(to_output, n_special);
+  int no = 0;
+  for (int32_t i = 0; i < n_special; i++) no += h->f_out[i] != 0;
+  h->n_face_out_rows = no;
+  free(h->face_out);
+  h->face_out = (double*)calloc((size_t)(no ? no : 1) * 12, sizeof(double));
+  return TLM_OK;
+}
+
+/* Filter storage follows allocate_volume_material_filter_data
+ * (TLM_volume_materials.F90:401-518): three filters per dispersive cell,
+ * each with w(0:order) zeroed (allocate_Zfilter_response,
+ * FILTER/filter_functions.F90:261-286). */
+static int alloc_filters(oracle_handle* h) {
+  if (!h->cell_code || !h->vmat) return TLM_OK;
+  int maxf = 0;
+  for (int i = 0; i < h->n_special; i++)
+    if (h->c_mat[i] && h->c_filt[i] + 2 > maxf) maxf = h->c_filt[i] + 2;
+  h->n_filters = maxf;
+  free(h->eps_w); free(h->mu_w); free(h->eps_ord); free(h->mu_ord);
+  h->eps_w = (double*)calloc((size_t)(maxf + 1) * NW, sizeof(double));
+  h->mu_w = (double*)calloc((size_t)(maxf + 1) * NW, sizeof(double));
+  h->eps_ord = (int32_t*)calloc((size_t)(maxf + 1), sizeof(int32_t));
+  h->mu_ord = (int32_t*)calloc((size_t)(maxf + 1), sizeof(int32_t));
+  for (int i = 0; i < h->n_special; i++) {
+    int m = h->c_mat[i];
+    if (!m) continue;
+    if (m < 1 || m > h->n_vmat) return fail(h, TLM_ERR_ARG, "material number out of range");
+    const tlm_volume_material* vm = &h->vmat[m - 1];
+    if (vm->type != TLM_VMAT_DISPERSIVE) continue;
+    int f = h->c_filt[i];
+    if (f < 1) return fail(h, TLM_ERR_ARG, "dispersive cell without filter number");
+    for (int k = 0; k < 3; k++) {
+      h->eps_ord[f + k] = vm->eps_filter_exists ? vm->eps_order : 0;
+      h->mu_ord[f + k] = vm->mu_filter_exists ? vm->mu_order : 0;
+    }
+  }
+  return TLM_OK;
+}
+
+int oracle_set_volume_materials(oracle_handle* h, int32_t n, const tlm_volume_material* m) {
+  free(h->vmat);
+  h->n_vmat = n;
+  h->vmat = (tlm_volume_material*)malloc((size_t)(n ? n : 1) * sizeof *m);
+  if (n) memcpy(h->vmat, m, (size_t)n * sizeof *m);
+  for (int i = 0; i < n; i++) {
+    if (m[i].eps_order < 0 || m[i].eps_order > TLM_MAX_FILTER_ORDER ||
+        m[i].mu_order < 0 || m[i].mu_order > TLM_MAX_FILTER_ORDER)
+      return fail(h, TLM_ERR_ARG, "filter order out of range");
+  }
+  return alloc_filters(h);
+}
+
+int oracle_set_surface_materials(oracle_handle* h, int32_t n, const int32_t* type) {
+  for (int i = 0; i < n; i++)
+    if (type[i] != TLM_SMAT_PEC && type[i] != TLM_SMAT_PMC && type[i] != TLM_SMAT_FREE_SPACE)
+      return fail(h, TLM_ERR_UNSUPPORTED, "surface material type stays on the host path");
+  free(h->smat);
+  h->n_smat = n;
+  h->smat = dup_i32(type, n);
+  return TLM_OK;
+}
+
+int oracle_set_pml(oracle_handle* h, int32_t n_params, const tlm_pml_parameter* p,
+                   int32_t n_cells, const int32_t* cell_param) {
+  free(h->pml_par); free(h->pml_cell_param); free(h->pml_state);
+  h->n_pml_params = n_params;
+  h->pml_par = (tlm_pml_parameter*)malloc((size_t)(n_params ? n_params : 1) * sizeof *p);
+  if (n_params) memcpy(h->pml_par, p, (size_t)n_params * sizeof *p);
+  h->n_pml_cells = n_cells;
+  h->pml_cell_param = dup_i32(cell_param, n_cells);
+  h->pml_state = (double*)calloc((size_t)(n_cells ? n_cells : 1) * PS_N, sizeof(double));
+  return TLM_OK;
+}
+
+int oracle_set_excitation_functions(oracle_handle* h, int32_t n_functions,
+                                    const double* value, const double* value_face) {
+  size_t n = (size_t)n_functions * h->g.n_timesteps;
+  free(h->value); free(h->value_face);
+  h->n_functions = n_functions;
+  h->value = (double*)malloc((n ? n : 1) * sizeof(double));
+  h->value_face = (double*)malloc((n ? n : 1) * sizeof(double));
+  if (n) { memcpy(h->value, value, n * sizeof(double)); memcpy(h->value_face, value_face, n * sizeof(double)); }
+  return TLM_OK;
+}
+
+int oracle_set_point_sources(oracle_handle* h, int32_t n, const tlm_point_source* s,
+                             int32_t n_cell_rows, int32_t n_face_rows) {
+  free(h->ps); free(h->cell_exc_field); free(h->cell_exc_type);
+  free(h->face_exc_field); free(h->face_exc_type);
+  h->n_ps = n;
+  h->ps = (tlm_point_source*)malloc((size_t)(n ? n : 1) * sizeof *s);
+  if (n) memcpy(h->ps, s, (size_t)n * sizeof *s);
+  h->n_cell_rows = n_cell_rows;
+  h->n_face_rows = n_face_rows;
+  h->cell_exc_field = (double*)calloc((size_t)(n_cell_rows ? n_cell_rows : 1) * 9, sizeof(double));
+  h->cell_exc_type = (int32_t*)malloc((size_t)(n_cell_rows ? n_cell_rows : 1) * 6 * sizeof(int32_t));
+  h->face_exc_field = (double*)calloc((size_t)(n_face_rows ? n_face_rows : 1) * 24, sizeof(double));
+  h->face_exc_type = (int32_t*)malloc((size_t)(n_face_rows ? n_face_rows : 1) * 12 * sizeof(int32_t));
+  /* set_cell_update_codes / set_face_update_codes initialise the types soft */
+  for (int i = 0; i < (n_cell_rows ? n_cell_rows : 1) * 6; i++) h->cell_exc_type[i] = 1;
+  for (int i = 0; i < (n_face_rows ? n_face_rows : 1) * 12; i++) h->face_exc_type[i] = 1;
+  for (int i = 0; i < n; i++) {
+    if (s[i].function < 1 || s[i].function > h->n_functions) return fail(h, TLM_ERR_ARG, "point source function out of range");
+    if (s[i].component < 1 || s[i].component > (s[i].at_face ? 12 : 9)) return fail(h, TLM_ERR_ARG, "point source component out of range");
+    int rows = s[i].at_face ? n_face_rows : n_cell_rows;
+    if (s[i].row < 1 || s[i].row > rows) return fail(h, TLM_ERR_ARG, "point source row out of range");
+  }
+  return TLM_OK;
+}
+
+int oracle_set_huygens_surface(oracle_handle* h, int32_t function, double offset_min,
+                               const double* Ei, const double* Hi, int32_t n_faces,
+                               const tlm_huygens_face* faces) {
+  free(h->huy);
+  h->huy_function = function;
+  h->huy_offset_min = offset_min;
+  for (int i = 0; i < 3; i++) { h->Ei[i] = Ei[i]; h->Hi[i] = Hi[i]; }
+  h->n_huy = n_faces;
+  h->huy = (tlm_huygens_face*)malloc((size_t)(n_faces ? n_faces : 1) * sizeof *faces);
+  if (n_faces) memcpy(h->huy, faces, (size_t)n_faces * sizeof *faces);
+  if (n_faces && (function < 1 || function > h->n_functions)) return fail(h, TLM_ERR_ARG, "Huygens function out of range");
+  for (int i = 0; i < n_faces; i++)
+    if (faces[i].row < 0 || faces[i].row > h->n_face_rows) return fail(h, TLM_ERR_ARG, "Huygens face row out of range");
+  return TLM_OK;
+}
+
+int oracle_set_output_points(oracle_handle* h, int32_t n, const int32_t* row, const int32_t* comp) {
+  free(h->pt_row); free(h->pt_comp); free(h->dft);
+  h->n_points = n;
+  h->pt_row = dup_i32(row, n);
+  h->pt_comp = dup_i32(comp, n);
+  for (int i = 0; i < n; i++)
+    if (row[i] < 1 || row[i] > h->n_out_rows || comp[i] < 1 || comp[i] > 6)
+      return fail(h, TLM_ERR_ARG, "output point row/component out of range");
+  h->dft = (double*)calloc((size_t)(n ? n : 1) * (h->n_freq ? h->n_freq : 1) * 2, sizeof(double));
+  return TLM_OK;
+}
+
+int oracle_set_dft_frequencies(oracle_handle* h, int32_t n, const double* f) {
+  free(h->freq); free(h->dft);
+  h->n_freq = n;
+  h->freq = (double*)malloc((size_t)(n ? n : 1) * sizeof(double));
+  if (n) memcpy(h->freq, f, (size_t)n * sizeof(double));
+  h->dft = (double*)calloc((size_t)(h->n_points ? h->n_points : 1) * (n ? n : 1) * 2, sizeof(double));
+  return TLM_OK;
+}
+
+int oracle_upload_links(oracle_handle* h, const double* V) {
+  size_t n = (size_t)12 * (h->nx + 2) * (h->ny + 2) * h->nz;
+  memcpy(h->V, V, n * sizeof(double));
+  return TLM_OK;
+}
+
+int oracle_download_links(oracle_handle* h, double* V) {
+  for (int cz = 1; cz <= h->nz; cz++)
+    for (int cy = 1; cy <= h->ny; cy++)
+      for (int cx = 1; cx <= h->nx; cx++) {
+        size_t i = (size_t)cx + (size_t)(h->nx + 2) * ((size_t)cy + (size_t)(h->ny + 2) * (size_t)(cz - 1));
+        memcpy(V + 12 * i, h->V + 12 * i, 12 * sizeof(double));
+      }
+  return TLM_OK;
+}
+
+/* ---- excitation ---------------------------------------------------------- */
+
+/* get_interpolated_excitation_value, EXCITATION/excitation.F90:645-690.
+ * NINT rounds half away from zero, as C round() does. */
+static double interpolated_value(oracle_handle* h, int timestep, double offset, int function) {
+  double dt = h->g.dt;
+  int nts = h->g.n_timesteps;
+  double t_offset = (offset - h->huy_offset_min) / (C0 / h->g.reduced_c_factor);
+  double local_time = ((double)(timestep - 1)) * dt - t_offset;
+  int timestep1 = (int)round(local_time / dt);
+  int timestep2 = timestep1 + 1;
+  double delta_t = local_time - (double)timestep1 * dt;
+  const double* vf = h->value_face + (size_t)(function - 1) * nts;
+  double value1 = (timestep1 >= 1 && timestep1 <= nts) ? vf[timestep1 - 1] : 0.0;
+  double value2 = (timestep2 >= 1 && timestep2 <= nts) ? vf[timestep2 - 1] : 0.0;
+  return value1 + delta_t * (value2 - value1) / dt;
+}
+
+int oracle_excitation(oracle_handle* h, int32_t timestep) {
+  if (timestep < 1 || timestep > h->g.n_timesteps) return fail(h, TLM_ERR_ARG, "timestep out of range");
+  int nts = h->g.n_timesteps;
+  /* excitation.F90:611-612: reset all excitation fields */
+  if (h->cell_exc_field) memset(h->cell_exc_field, 0, (size_t)(h->n_cell_rows ? h->n_cell_rows : 1) * 9 * sizeof(double));
+  if (h->face_exc_field) memset(h->face_exc_field, 0, (size_t)(h->n_face_rows ? h->n_face_rows : 1) * 24 * sizeof(double));
+  /* Point_excitation */
+  for (int i = 0; i < h->n_ps; i++) {
+    const tlm_point_source* s = &h->ps[i];
+    int f = s->function, c = s->component, r = s->row;
+    if (!s->at_face) {
+      double value = h->value[(size_t)(f - 1) * nts + (timestep - 1)];
+      h->cell_exc_field[(size_t)(r - 1) * 9 + (c - 1)] = h->cell_exc_field[(size_t)(r - 1) * 9 + (c - 1)] + value;
+      if (c <= 6) h->cell_exc_type[(size_t)(r - 1) * 6 + (c - 1)] = s->source_type;
+    } else {
+      double value = h->value_face[(size_t)(f - 1) * nts + (timestep - 1)];
+      for (int side = 0; side < 2; side++) {
+        double* fe = &h->face_exc_field[((size_t)(r - 1) * 2 + side) * 12];
+        fe[c - 1] = fe[c - 1] + value;
+        if (c <= 6) h->face_exc_type[((size_t)(r - 1) * 2 + side) * 6 + (c - 1)] = s->source_type;
+      }
+    }
+  }
+  /* Huygens_surface_excitation, huygens_surface.F90:760-842 */
+  for (int i = 0; i < h->n_huy; i++) {
+    const tlm_huygens_face* hf = &h->huy[i];
+    double value = interpolated_value(h, timestep, hf->offset, h->huy_function);
+    double normx = (double)hf->nx, normy = (double)hf->ny, normz = (double)hf->nz;
+    double Js[3], Ms[3];
+    Js[0] = (normy * value * (h->Hi[2]) - normz * value * (h->Hi[1]));
+    Js[1] = (normz * value * (h->Hi[0]) - normx * value * (h->Hi[2]));
+    Js[2] = (normx * value * (h->Hi[1]) - normy * value * (h->Hi[0]));
+    Ms[0] = -(normy * value * (h->Ei[2]) - normz * value * (h->Ei[1]));
+    Ms[1] = -(normz * value * (h->Ei[0]) - normx * value * (h->Ei[2]));
+    Ms[2] = -(normx * value * (h->Ei[1]) - normy * value * (h->Ei[0]));
+    if (hf->row == 0) continue; /* face not on an interior connect face */
+    for (int side = 0; side < 2; side++) {
+      double* fe = &h->face_exc_field[((size_t)(hf->row - 1) * 2 + side) * 12];
+      for (int k = 0; k < 3; k++) fe[Jx - 1 + k] = fe[Jx - 1 + k] + Js[k];
+      for (int k = 0; k < 3; k++) fe[Mx - 1 + k] = fe[Mx - 1 + k] + Ms[k];
+    }
+  }
+  return TLM_OK;
+}
+
+/* ---- scatter ------------------------------------------------------------- */
+
+/* evaluate_Zfilter, FILTER/filter_subroutines.F90:510-540 */
+static double evaluate_Zfilter(const double* a, const double* b, double* w, int order, double G) {
+  double W0 = G;
+  for (int i = 1; i <= order; i++) W0 = W0 - b[i] * w[i];
+  W0 = W0 / b[0];
+  w[0] = W0;
+  double F0 = 0.0;
+  for (int i = 0; i <= order; i++) F0 = F0 + a[i] * w[i];
+  return F0;
+}
+
+/* timeshift_Zfilter, filter_subroutines.F90:557-575 */
+static void timeshift_Zfilter(double* w, int order) {
+  for (int i = order; i >= 1; i--) w[i] = w[i - 1];
+}
+
+/* add_shunt_admittance / add_series_impedance, scatter.F90:606-661 */
+static void add_shunt(double* V1, double* Y1, double V2, double Y2) {
+  double V = (*V1 * *Y1 + V2 * Y2) / (*Y1 + Y2);
+  double Y = *Y1 + Y2;
+  *V1 = V;
+  *Y1 = Y;
+}
+static void add_series(double* V1, double* Z1, double V2, double Z2) {
+  double V = *V1 + V2;
+  double Z = *Z1 + Z2;
+  *V1 = V;
+  *Z1 = Z;
+}
+
+/* The 12-port scatter, scatter.F90:240-272 (and :517-547). */
+static inline void scatter_ports(double* v, double Vx, double Vy, double Vz,
+                                 double Ix, double Iy, double Iz) {
+  double V_min, V_max;
+  V_min = v[Vx_ymin]; V_max = v[Vx_ymax];
+  v[Vx_ymin] = Vx - Z0 * Iz - V_max;
+  v[Vx_ymax] = Vx + Z0 * Iz - V_min;
+  V_min = v[Vx_zmin]; V_max = v[Vx_zmax];
+  v[Vx_zmin] = Vx + Z0 * Iy - V_max;
+  v[Vx_zmax] = Vx - Z0 * Iy - V_min;
+  V_min = v[Vy_zmin]; V_max = v[Vy_zmax];
+  v[Vy_zmin] = Vy - Z0 * Ix - V_max;
+  v[Vy_zmax] = Vy + Z0 * Ix - V_min;
+  V_min = v[Vy_xmin]; V_max = v[Vy_xmax];
+  v[Vy_xmin] = Vy + Z0 * Iz - V_max;
+  v[Vy_xmax] = Vy - Z0 * Iz - V_min;
+  V_min = v[Vz_xmin]; V_max = v[Vz_xmax];
+  v[Vz_xmin] = Vz - Z0 * Iy - V_max;
+  v[Vz_xmax] = Vz + Z0 * Iy - V_min;
+  V_min = v[Vz_ymin]; V_max = v[Vz_ymax];
+  v[Vz_ymin] = Vz + Z0 * Ix - V_max;
+  v[Vz_ymax] = Vz - Z0 * Ix - V_min;
+}
+
+/* PML port aliases, PML_module.F90:60-71 */
+#define Vxny 1
+#define Vxnz 2
+#define Vxpy 3
+#define Vxpz 4
+#define Vynx 5
+#define Vynz 6
+#define Vypx 7
+#define Vypz 8
+#define Vznx 9
+#define Vzny 10
+#define Vzpx 11
+#define Vzpy 12
+
+/* scatter_PML.F90:22-313, the free-space PML cell update. */
+static void scatter_pml_cell(oracle_handle* h, double* v, int PML_cell, int output_number,
+                             int* output_cell_number) {
+  const double dl = h->g.dl;
+  /* STAGE 1 */
+  const tlm_pml_parameter* P = &h->pml_par[h->pml_cell_param[PML_cell - 1] - 1];
+  double ax = P->ax, ay = P->ay, az = P->az;
+  double exp_xi = P->exp_x, exp_yi = P->exp_y, exp_zi = P->exp_z;
+  double exp_xr = P->exp_x, exp_yr = P->exp_y, exp_zr = P->exp_z;
+  double Csx = P->Csx, Csy = P->Csy, Csz = P->Csz;
+  double Csx2 = P->Csx2, Csy2 = P->Csy2, Csz2 = P->Csz2;
+  double Csy_sz = P->Csy_sz, Csz_sx = P->Csz_sx, Csx_sy = P->Csx_sy;
+  double* S = &h->pml_state[(size_t)(PML_cell - 1) * PS_N];
+  /* STAGE 2 */
+  double last_Ix = S[PS_Ix], last_Iy = S[PS_Iy], last_Iz = S[PS_Iz];
+  double last_Vi[13];
+  for (int i = 1; i <= 12; i++) last_Vi[i] = S[PS_Vi + i - 1];
+  double last_Vxt = S[PS_Vxt], last_Vyt = S[PS_Vyt], last_Vzt = S[PS_Vzt];
+  double last_Vyxt = S[PS_Vyxt], last_Vzxt = S[PS_Vzxt], last_Vxyt = S[PS_Vxyt];
+  double last_Vzyt = S[PS_Vzyt], last_Vyzt = S[PS_Vyzt], last_Vxzt = S[PS_Vxzt];
+  /* STAGE 3 */
+  v[Vynx] = v[Vynx] * exp_yi; v[Vypx] = v[Vypx] * exp_yi;
+  v[Vznx] = v[Vznx] * exp_zi; v[Vzpx] = v[Vzpx] * exp_zi;
+  v[Vxny] = v[Vxny] * exp_xi; v[Vxpy] = v[Vxpy] * exp_xi;
+  v[Vzny] = v[Vzny] * exp_zi; v[Vzpy] = v[Vzpy] * exp_zi;
+  v[Vxnz] = v[Vxnz] * exp_xi; v[Vxpz] = v[Vxpz] * exp_xi;
+  v[Vynz] = v[Vynz] * exp_yi; v[Vypz] = v[Vypz] * exp_yi;
+  /* STAGE 4 */
+  double Vx = (v[Vynx] + v[Vypx] + v[Vznx] + v[Vzpx]) / 2.0;
+  double Vy = (v[Vxny] + v[Vxpy] + v[Vzny] + v[Vzpy]) / 2.0;
+  double Vz = (v[Vxnz] + v[Vxpz] + v[Vynz] + v[Vypz]) / 2.0;
+  double Ix = (v[Vypz] - v[Vzpy] - v[Vynz] + v[Vzny]) / (2.0 * Z0);
+  double Iy = (v[Vzpx] - v[Vxpz] - v[Vznx] + v[Vxnz]) / (2.0 * Z0);
+  double Iz = (v[Vxpy] - v[Vypx] - v[Vxny] + v[Vynx]) / (2.0 * Z0);
+  /* STAGE 5 */
+  double Vx_PML_shunt = ax * (-(last_Vi[Vznx] + last_Vi[Vzpx] + last_Vi[Vynx] + last_Vi[Vypx]) / 2.0
+                              + Csy * ((v[Vznx] + v[Vzpx]) + (last_Vi[Vznx] + last_Vi[Vzpx]))
+                              + Csz * ((v[Vynx] + v[Vypx]) + (last_Vi[Vynx] + last_Vi[Vypx]))
+                              + Csy_sz * last_Vxt);
+  double Vy_PML_shunt = ay * (-(last_Vi[Vxny] + last_Vi[Vxpy] + last_Vi[Vzny] + last_Vi[Vzpy]) / 2.0
+                              + Csz * ((v[Vxny] + v[Vxpy]) + (last_Vi[Vxny] + last_Vi[Vxpy]))
+                              + Csx * ((v[Vzny] + v[Vzpy]) + (last_Vi[Vzny] + last_Vi[Vzpy]))
+                              + Csz_sx * last_Vyt);
+  double Vz_PML_shunt = az * (-(last_Vi[Vynz] + last_Vi[Vypz] + last_Vi[Vxnz] + last_Vi[Vxpz]) / 2.0
+                              + Csx * ((v[Vynz] + v[Vypz]) + (last_Vi[Vynz] + last_Vi[Vypz]))
+                              + Csy * ((v[Vxnz] + v[Vxpz]) + (last_Vi[Vxnz] + last_Vi[Vxpz]))
+                              + Csx_sy * last_Vzt);
+  /* STAGE 6 */
+  double Vxt = ax * Vx + Vx_PML_shunt;
+  double Vyt = ay * Vy + Vy_PML_shunt;
+  double Vzt = az * Vz + Vz_PML_shunt;
+  /* STAGE 7a */
+  double Ixt = 0.0, Iyt = 0.0, Izt = 0.0;
+  if (output_number != 0) {
+    double last_Ixt = S[PS_Ixt], last_Iyt = S[PS_Iyt], last_Izt = S[PS_Izt];
+    Ixt = ax * (Ix - last_Ix + Csx2 * (Ix + last_Ix) + Csy_sz * last_Ixt);
+    Iyt = ay * (Iy - last_Iy + Csy2 * (Iy + last_Iy) + Csz_sx * last_Iyt);
+    Izt = az * (Iz - last_Iz + Csz2 * (Iz + last_Iz) + Csx_sy * last_Izt);
+  }
+  /* STAGE 7 */
+  double Vxy_PML_series = az * (Z0 * (-last_Iz + Csy2 * (Iz + last_Iz)) + Csx_sy * last_Vxyt);
+  double Vyx_PML_series = az * (Z0 * (-last_Iz + Csx2 * (Iz + last_Iz)) + Csx_sy * last_Vyxt);
+  double Vzx_PML_series = ay * (Z0 * (-last_Iy + Csx2 * (Iy + last_Iy)) + Csz_sx * last_Vzxt);
+  double Vxz_PML_series = ay * (Z0 * (-last_Iy + Csz2 * (Iy + last_Iy)) + Csz_sx * last_Vxzt);
+  double Vyz_PML_series = ax * (Z0 * (-last_Ix + Csz2 * (Ix + last_Ix)) + Csy_sz * last_Vyzt);
+  double Vzy_PML_series = ax * (Z0 * (-last_Ix + Csy2 * (Ix + last_Ix)) + Csy_sz * last_Vzyt);
+  /* STAGE 8 */
+  double Vxyt = az * Iz * Z0 + Vxy_PML_series;
+  double Vyxt = az * Iz * Z0 + Vyx_PML_series;
+  double Vzxt = ay * Iy * Z0 + Vzx_PML_series;
+  double Vxzt = ay * Iy * Z0 + Vxz_PML_series;
+  double Vyzt = ax * Ix * Z0 + Vyz_PML_series;
+  double Vzyt = ax * Ix * Z0 + Vzy_PML_series;
+  /* STAGE 9 */
+  S[PS_Ix] = Ix; S[PS_Iy] = Iy; S[PS_Iz] = Iz;
+  for (int i = 1; i <= 12; i++) S[PS_Vi + i - 1] = v[i];
+  S[PS_Vxt] = Vxt; S[PS_Vyt] = Vyt; S[PS_Vzt] = Vzt;
+  if (output_number != 0) { S[PS_Ixt] = Ixt; S[PS_Iyt] = Iyt; S[PS_Izt] = Izt; }
+  S[PS_Vxyt] = Vxyt; S[PS_Vyxt] = Vyxt;
+  S[PS_Vzxt] = Vzxt; S[PS_Vxzt] = Vxzt;
+  S[PS_Vzyt] = Vzyt; S[PS_Vyzt] = Vyzt;
+  /* STAGE 10 */
+  double Vynx_PML = Vx_PML_shunt - Vyx_PML_series;
+  double Vypx_PML = Vx_PML_shunt + Vyx_PML_series;
+  double Vznx_PML = Vx_PML_shunt + Vzx_PML_series;
+  double Vzpx_PML = Vx_PML_shunt - Vzx_PML_series;
+  double Vxny_PML = Vy_PML_shunt + Vxy_PML_series;
+  double Vxpy_PML = Vy_PML_shunt - Vxy_PML_series;
+  double Vzny_PML = Vy_PML_shunt - Vzy_PML_series;
+  double Vzpy_PML = Vy_PML_shunt + Vzy_PML_series;
+  double Vynz_PML = Vz_PML_shunt + Vyz_PML_series;
+  double Vypz_PML = Vz_PML_shunt - Vyz_PML_series;
+  double Vxnz_PML = Vz_PML_shunt - Vxz_PML_series;
+  double Vxpz_PML = Vz_PML_shunt + Vxz_PML_series;
+  /* STAGE 11 */
+  double V_max, V_min;
+  V_max = v[Vypx]; V_min = v[Vynx];
+  v[Vynx] = ax * Vx - az * Z0 * Iz - V_max + Vynx_PML;
+  v[Vypx] = ax * Vx + az * Z0 * Iz - V_min + Vypx_PML;
+  V_max = v[Vzpx]; V_min = v[Vznx];
+  v[Vznx] = ax * Vx + ay * Z0 * Iy - V_max + Vznx_PML;
+  v[Vzpx] = ax * Vx - ay * Z0 * Iy - V_min + Vzpx_PML;
+  V_max = v[Vzpy]; V_min = v[Vzny];
+  v[Vzny] = ay * Vy - ax * Z0 * Ix - V_max + Vzny_PML;
+  v[Vzpy] = ay * Vy + ax * Z0 * Ix - V_min + Vzpy_PML;
+  V_max = v[Vxpy]; V_min = v[Vxny];
+  v[Vxny] = ay * Vy + az * Z0 * Iz - V_max + Vxny_PML;
+  v[Vxpy] = ay * Vy - az * Z0 * Iz - V_min + Vxpy_PML;
+  V_max = v[Vxpz]; V_min = v[Vxnz];
+  v[Vxnz] = az * Vz - ay * Z0 * Iy - V_max + Vxnz_PML;
+  v[Vxpz] = az * Vz + ay * Z0 * Iy - V_min + Vxpz_PML;
+  V_max = v[Vypz]; V_min = v[Vynz];
+  v[Vynz] = az * Vz + ax * Z0 * Ix - V_max + Vynz_PML;
+  v[Vypz] = az * Vz - ax * Z0 * Ix - V_min + Vypz_PML;
+  /* STAGE 12 */
+  if (output_number != 0) {
+    *output_cell_number += 1;
+    double* fo = &h->cell_out[(size_t)(*output_cell_number - 1) * 6];
+    fo[0] = -Vxt / dl; fo[1] = -Vyt / dl; fo[2] = -Vzt / dl;
+    fo[3] = Ixt / dl; fo[4] = Iyt / dl; fo[5] = Izt / dl;
+  }
+  /* STAGE 13 */
+  v[Vynx] = v[Vynx] * exp_yr; v[Vypx] = v[Vypx] * exp_yr;
+  v[Vznx] = v[Vznx] * exp_zr; v[Vzpx] = v[Vzpx] * exp_zr;
+  v[Vxny] = v[Vxny] * exp_xr; v[Vxpy] = v[Vxpy] * exp_xr;
+  v[Vzny] = v[Vzny] * exp_zr; v[Vzpy] = v[Vzpy] * exp_zr;
+  v[Vxnz] = v[Vxnz] * exp_xr; v[Vxpz] = v[Vxpz] * exp_xr;
+  v[Vynz] = v[Vynz] * exp_yr; v[Vypz] = v[Vypz] * exp_yr;
+}
+
+static void record_outputs(oracle_handle* h, int timestep) {
+  /* cell_output(): copy this step's cell_output_field into the ring and
+   * accumulate the probe DFT at time=(timestep-1)*dt (run_TLM.F90:240). */
+  int rows = h->n_out_rows;
+  if (!h->ring && rows) h->ring = (double*)malloc((size_t)h->cap * rows * 6 * sizeof(double));
+  if (rows) {
+    if (h->ring_count == 0) h->ring_first = timestep;
+    if (h->ring_count < h->cap) {
+      memcpy(h->ring + (size_t)h->ring_count * rows * 6, h->cell_out, (size_t)rows * 6 * sizeof(double));
+      h->ring_count++;
+    }
+  }
+  if (h->n_freq && h->n_points) {
+    double dt = h->g.dt;
+    double time = (double)(timestep - 1) * dt;
+    for (int p = 0; p < h->n_points; p++) {
+      double v = h->cell_out[(size_t)(h->pt_row[p] - 1) * 6 + (h->pt_comp[p] - 1)];
+      for (int k = 0; k < h->n_freq; k++) {
+        double ph = 2.0 * PI * h->freq[k] * time;
+        double* acc = &h->dft[((size_t)p * h->n_freq + k) * 2];
+        acc[0] += v * cos(ph) * dt;
+        acc[1] -= v * sin(ph) * dt;
+      }
+    }
+  }
+}
+
+int oracle_scatter(oracle_handle* h, int32_t timestep) {
+  const double dl = h->g.dl;
+  const double Z04 = Z0 * 4.0, Y04 = Y0 * 4.0;
+  int nx = h->nx, ny = h->ny, nz = h->nz;
+  int cell_number = 0, output_cell_number = 0, excitation_cell_number = 0;
+  if (!h->cell_code) return fail(h, TLM_ERR_STATE, "cell codes not set");
+  for (int cz = 1; cz <= nz; cz++)
+    for (int cy = 1; cy <= ny; cy++)
+      for (int cx = 1; cx <= nx; cx++) {
+        cell_number++;
+        double* v = cellV(h, cx, cy, cz);
+        int code = h->cell_code[cell_number - 1];
+        if (code == 0) {
+          double Vx = (v[Vx_ymin] + v[Vx_ymax] + v[Vx_zmin] + v[Vx_zmax]) / 2.0;
+          double Vy = (v[Vy_xmin] + v[Vy_xmax] + v[Vy_zmin] + v[Vy_zmax]) / 2.0;
+          double Vz = (v[Vz_xmin] + v[Vz_xmax] + v[Vz_ymin] + v[Vz_ymax]) / 2.0;
+          double Ix = (v[Vz_ymax] - v[Vy_zmax] - v[Vz_ymin] + v[Vy_zmin]) / (2.0 * Z0);
+          double Iy = (v[Vx_zmax] - v[Vz_xmax] - v[Vx_zmin] + v[Vz_xmin]) / (2.0 * Z0);
+          double Iz = (v[Vy_xmax] - v[Vx_ymax] - v[Vy_xmin] + v[Vx_ymin]) / (2.0 * Z0);
+          scatter_ports(v, Vx, Vy, Vz, Ix, Iy, Iz);
+          continue;
+        }
+        int special = code;
+        int PML_cell = h->c_pml[special - 1];
+        int material_number = h->c_mat[special - 1];
+        int material_type = material_number ? h->vmat[material_number - 1].type : 0;
+        int output_number = h->c_out[special - 1];
+        if (PML_cell != 0) {
+          if (material_type == TLM_VMAT_PEC || material_type == TLM_VMAT_PMC)
+            return fail(h, TLM_ERR_ARG, "PEC/PMC volume material in PML");
+          if (material_type == TLM_VMAT_DISPERSIVE)
+            return fail(h, TLM_ERR_UNSUPPORTED, "material inside the PML is not supported");
+          scatter_pml_cell(h, v, PML_cell, output_number, &output_cell_number);
+          continue;
+        }
+        int excitation_number = h->c_exc[special - 1];
+        const tlm_volume_material* vm = material_number ? &h->vmat[material_number - 1] : NULL;
+        double VEx = (v[Vx_ymin] + v[Vx_ymax] + v[Vx_zmin] + v[Vx_zmax]) / 2.0;
+        double VEy = (v[Vy_xmin] + v[Vy_xmax] + v[Vy_zmin] + v[Vy_zmax]) / 2.0;
+        double VEz = (v[Vz_xmin] + v[Vz_xmax] + v[Vz_ymin] + v[Vz_ymax]) / 2.0;
+        double Yx = Y04, Yy = Y04, Yz = Y04;
+        double VHx = 2.0 * (v[Vz_ymax] - v[Vy_zmax] - v[Vz_ymin] + v[Vy_zmin]);
+        double VHy = 2.0 * (v[Vx_zmax] - v[Vz_xmax] - v[Vx_zmin] + v[Vz_xmin]);
+        double VHz = 2.0 * (v[Vy_xmax] - v[Vx_ymax] - v[Vy_xmin] + v[Vx_ymin]);
+        double Zx = Z04, Zy = Z04, Zz = Z04;
+        double Vox = 0, Voy = 0, Voz = 0, Ysx = 0, Ysy = 0, Ysz = 0;
+        int filter_number = 0;
+        if (material_type == TLM_VMAT_PEC) {
+          VEx = 0.0; VEy = 0.0; VEz = 0.0;
+        } else if (material_type == TLM_VMAT_PMC) {
+          VHx = 0.0; VHy = 0.0; VHz = 0.0;
+        } else if (material_type == TLM_VMAT_DISPERSIVE) {
+          filter_number = h->c_filt[special - 1];
+          if (vm->eps_filter_exists) {
+            Ysx = vm->Ys_eps_f; Ysy = vm->Ys_eps_f; Ysz = vm->Ys_eps_f;
+            int o = vm->eps_order;
+            double* w;
+            w = &h->eps_w[(size_t)(filter_number) * NW];
+            timeshift_Zfilter(w, o);
+            Vox = evaluate_Zfilter(vm->eps_a, vm->eps_b, w, o, 0.0);
+            w = &h->eps_w[(size_t)(filter_number + 1) * NW];
+            timeshift_Zfilter(w, o);
+            Voy = evaluate_Zfilter(vm->eps_a, vm->eps_b, w, o, 0.0);
+            w = &h->eps_w[(size_t)(filter_number + 2) * NW];
+            timeshift_Zfilter(w, o);
+            Voz = evaluate_Zfilter(vm->eps_a, vm->eps_b, w, o, 0.0);
+            add_shunt(&VEx, &Yx, Vox, Ysx);
+            add_shunt(&VEy, &Yy, Voy, Ysy);
+            add_shunt(&VEz, &Yz, Voz, Ysz);
+          }
+          double Gex = vm->Ge, Gey = vm->Ge, Gez = vm->Ge;
+          add_shunt(&VEx, &Yx, 0.0, Gex);
+          add_shunt(&VEy, &Yy, 0.0, Gey);
+          add_shunt(&VEz, &Yz, 0.0, Gez);
+          if (vm->mu_filter_exists) {
+            double Zsx = vm->Zs_mu_f, Zsy = vm->Zs_mu_f, Zsz = vm->Zs_mu_f;
+            int o = vm->mu_order;
+            double* w;
+            double Vsx, Vsy, Vsz;
+            w = &h->mu_w[(size_t)(filter_number) * NW];
+            timeshift_Zfilter(w, o);
+            Vsx = evaluate_Zfilter(vm->mu_a, vm->mu_b, w, o, 0.0);
+            w = &h->mu_w[(size_t)(filter_number + 1) * NW];
+            timeshift_Zfilter(w, o);
+            Vsy = evaluate_Zfilter(vm->mu_a, vm->mu_b, w, o, 0.0);
+            w = &h->mu_w[(size_t)(filter_number + 2) * NW];
+            timeshift_Zfilter(w, o);
+            Vsz = evaluate_Zfilter(vm->mu_a, vm->mu_b, w, o, 0.0);
+            add_series(&VHx, &Zx, Vsx, Zsx);
+            add_series(&VHy, &Zy, Vsy, Zsy);
+            add_series(&VHz, &Zz, Vsz, Zsz);
+          }
+          double Rmx = vm->Rm, Rmy = vm->Rm, Rmz = vm->Rm;
+          add_series(&VHx, &Zx, 0.0, Rmx);
+          add_series(&VHy, &Zy, 0.0, Rmy);
+          add_series(&VHz, &Zz, 0.0, Rmz);
+        }
+        double Vx = VEx, Vy = VEy, Vz = VEz;
+        double Ix = VHx / Zx, Iy = VHy / Zy, Iz = VHz / Zz;
+        if (excitation_number != 0) {
+          excitation_cell_number++;
+          const double* field = &h->cell_exc_field[(size_t)(excitation_cell_number - 1) * 9];
+          const int32_t* hsf = &h->cell_exc_type[(size_t)(excitation_cell_number - 1) * 6];
+          double Isx = field[Jx - 1] * dl * dl;
+          double Isy = field[Jy - 1] * dl * dl;
+          double Isz = field[Jz - 1] * dl * dl;
+          Vx = (double)hsf[0] * Vx - field[Ex - 1] * dl / 2.0 + Isx / Yx;
+          Vy = (double)hsf[1] * Vy - field[Ey - 1] * dl / 2.0 + Isy / Yy;
+          Vz = (double)hsf[2] * Vz - field[Ez - 1] * dl / 2.0 + Isz / Yz;
+          Ix = (double)hsf[3] * Ix + field[Hx - 1] * dl / 2.0;
+          Iy = (double)hsf[4] * Iy + field[Hy - 1] * dl / 2.0;
+          Iz = (double)hsf[5] * Iz + field[Hz - 1] * dl / 2.0;
+        }
+        if (output_number != 0) {
+          output_cell_number++;
+          double* fo = &h->cell_out[(size_t)(output_cell_number - 1) * 6];
+          fo[0] = -Vx / dl; fo[1] = -Vy / dl; fo[2] = -Vz / dl;
+          fo[3] = Ix / dl; fo[4] = Iy / dl; fo[5] = Iz / dl;
+        }
+        if (material_type == TLM_VMAT_DISPERSIVE) {
+          if (vm->eps_filter_exists) {
+            double Iox = (Vx - Vox) * Ysx;
+            double Ioy = (Vy - Voy) * Ysy;
+            double Ioz = (Vz - Voz) * Ysz;
+            int o = vm->eps_order;
+            evaluate_Zfilter(vm->eps_a, vm->eps_b, &h->eps_w[(size_t)(filter_number) * NW], o, Iox);
+            evaluate_Zfilter(vm->eps_a, vm->eps_b, &h->eps_w[(size_t)(filter_number + 1) * NW], o, Ioy);
+            evaluate_Zfilter(vm->eps_a, vm->eps_b, &h->eps_w[(size_t)(filter_number + 2) * NW], o, Ioz);
+          }
+          if (vm->mu_filter_exists) {
+            int o = vm->mu_order;
+            evaluate_Zfilter(vm->mu_a, vm->mu_b, &h->mu_w[(size_t)(filter_number) * NW], o, -Ix);
+            evaluate_Zfilter(vm->mu_a, vm->mu_b, &h->mu_w[(size_t)(filter_number + 1) * NW], o, -Iy);
+            evaluate_Zfilter(vm->mu_a, vm->mu_b, &h->mu_w[(size_t)(filter_number + 2) * NW], o, -Iz);
+          }
+        }
+        scatter_ports(v, Vx, Vy, Vz, Ix, Iy, Iz);
+      }
+  record_outputs(h, timestep);
+  return TLM_OK;
+}
+
+/* ---- connect ------------------------------------------------------------- */
+
+/* One special face, connect_normal_to_{x,y,z}.F90 lines 12-361.
+ * axis 0/1/2 = x/y/z normal.  pa/pb are the two polarisation ports on the
+ * min side (cell c) and qa/qb the matching ports on the max side (cell c-1):
+ * x: (Vy_xmin,Vz_xmin)/(Vy_xmax,Vz_xmax); y: (Vx_ymin,Vz_ymin)/(Vx_ymax,Vz_ymax);
+ * z: (Vx_zmin,Vy_zmin)/(Vx_zmax,Vy_zmax). */
+static int connect_special_face(oracle_handle* h, int axis, double* vmin_cell, double* vmax_cell,
+                                int code, int* excitation_face_number, int* output_face_number) {
+  static const int pa[3] = {Vy_xmin, Vx_ymin, Vx_zmin}, pb[3] = {Vz_xmin, Vz_ymin, Vy_zmin};
+  static const int qa[3] = {Vy_xmax, Vx_ymax, Vx_zmax}, qb[3] = {Vz_xmax, Vz_ymax, Vy_zmax};
+  const double dl = h->g.dl;
+  int m = h->f_mat[code - 1];
+  int material_number = m < 0 ? -m : m;
+  int material_type = material_number ? h->smat[material_number - 1] : 0;
+  int output_number = h->f_out[code - 1];
+  int excitation_number = h->f_exc[code - 1];
+  if (excitation_number < 0) excitation_number = -excitation_number;
+  double* A = &vmin_cell[pa[axis]]; /* min-side first polarisation  */
+  double* B = &vmin_cell[pb[axis]]; /* min-side second polarisation */
+  double* C = &vmax_cell[qa[axis]]; /* max-side first polarisation  */
+  double* D = &vmax_cell[qb[axis]]; /* max-side second polarisation */
+  double a_in = *A, b_in = *B, c_in = *C, d_in = *D; /* pre-connect values */
+  double a_min, a_max, b_min, b_max;
+  if (material_type == 0 || material_type == TLM_SMAT_FREE_SPACE) {
+    a_min = *A + *C; a_max = a_min;
+    b_min = *B + *D; b_max = b_min;
+  } else if (material_type == TLM_SMAT_PEC) {
+    a_min = 0.0; a_max = 0.0; b_min = 0.0; b_max = 0.0;
+  } else if (material_type == TLM_SMAT_PMC) {
+    a_min = 2.0 * *A; a_max = 2.0 * *C;
+    b_min = 2.0 * *B; b_max = 2.0 * *D;
+  } else {
+    return fail(h, TLM_ERR_UNSUPPORTED, "surface material type stays on the host path");
+  }
+  if (excitation_number != 0) {
+    *excitation_face_number += 1;
+    const double* fmin = &h->face_exc_field[((size_t)(*excitation_face_number - 1) * 2 + 0) * 12];
+    const double* fmax = &h->face_exc_field[((size_t)(*excitation_face_number - 1) * 2 + 1) * 12];
+    /* hard_source_factor_max takes side 1's types (connect_normal_to_x.F90:294) */
+    const int32_t* hmin = &h->face_exc_type[((size_t)(*excitation_face_number - 1) * 2 + 0) * 6];
+    const int32_t* hmax = hmin;
+    const double* Jsmin = fmin + 6; const double* Jsmax = fmax + 6;
+    const double* Msmin = fmin + 9; const double* Msmax = fmax + 9;
+    if (axis == 0) { /* connect_normal_to_x.F90:300-304: (a,b)=(y,z) */
+      a_min = a_min * (double)hmin[Ey - 1] - fmin[Ey - 1] * dl - fmin[Hz - 1] * Z0 * dl + Jsmin[1] * dl * Z0 / 2.0 + Msmin[2] * dl / 2.0;
+      a_max = a_max * (double)hmax[Ey - 1] - fmax[Ey - 1] * dl + fmax[Hz - 1] * Z0 * dl + Jsmax[1] * dl * Z0 / 2.0 - Msmax[2] * dl / 2.0;
+      b_min = b_min * (double)hmin[Ez - 1] - fmin[Ez - 1] * dl + fmin[Hy - 1] * Z0 * dl + Jsmin[2] * dl * Z0 / 2.0 - Msmin[1] * dl / 2.0;
+      b_max = b_max * (double)hmax[Ez - 1] - fmax[Ez - 1] * dl - fmax[Hy - 1] * Z0 * dl + Jsmax[2] * dl * Z0 / 2.0 + Msmax[1] * dl / 2.0;
+    } else if (axis == 1) { /* connect_normal_to_y.F90:306-310: (a,b)=(x,z) */
+      a_min = a_min * (double)hmin[Ex - 1] - fmin[Ex - 1] * dl + fmin[Hz - 1] * Z0 * dl + Jsmin[0] * dl * Z0 / 2.0 - Msmin[2] * dl / 2.0;
+      a_max = a_max * (double)hmax[Ex - 1] - fmax[Ex - 1] * dl - fmax[Hz - 1] * Z0 * dl + Jsmax[0] * dl * Z0 / 2.0 + Msmax[2] * dl / 2.0;
+      b_min = b_min * (double)hmin[Ez - 1] - fmin[Ez - 1] * dl - fmin[Hx - 1] * Z0 * dl + Jsmin[2] * dl * Z0 / 2.0 + Msmin[0] * dl / 2.0;
+      b_max = b_max * (double)hmax[Ez - 1] - fmax[Ez - 1] * dl + fmax[Hx - 1] * Z0 * dl + Jsmax[2] * dl * Z0 / 2.0 - Msmax[0] * dl / 2.0;
+    } else { /* connect_normal_to_z.F90:303-307: (a,b)=(x,y) */
+      a_min = a_min * (double)hmin[Ex - 1] - fmin[Ex - 1] * dl - fmin[Hy - 1] * Z0 * dl + Jsmin[0] * dl * Z0 / 2.0 + Msmin[1] * dl / 2.0;
+      a_max = a_max * (double)hmax[Ex - 1] - fmax[Ex - 1] * dl + fmax[Hy - 1] * Z0 * dl + Jsmax[0] * dl * Z0 / 2.0 - Msmax[1] * dl / 2.0;
+      b_min = b_min * (double)hmin[Ey - 1] - fmin[Ey - 1] * dl + fmin[Hx - 1] * Z0 * dl + Jsmin[1] * dl * Z0 / 2.0 - Msmin[0] * dl / 2.0;
+      b_max = b_max * (double)hmax[Ey - 1] - fmax[Ey - 1] * dl - fmax[Hx - 1] * Z0 * dl + Jsmax[1] * dl * Z0 / 2.0 + Msmax[0] * dl / 2.0;
+    }
+  }
+  /* complete the connection */
+  *C = a_max - *C;
+  *A = a_min - *A;
+  *D = b_max - *D;
+  *B = b_min - *B;
+  if (output_number != 0) {
+    *output_face_number += 1;
+    double* o1 = &h->face_out[((size_t)(*output_face_number - 1) * 2 + 0) * 6];
+    double* o2 = &h->face_out[((size_t)(*output_face_number - 1) * 2 + 1) * 6];
+    /* side 1: positive = post-connect, negative = pre-connect;
+     * side 2: positive = pre-connect, negative = post-connect. */
+    double ap1 = *A, an1 = a_in, bp1 = *B, bn1 = b_in;
+    double ap2 = c_in, an2 = *C, bp2 = d_in, bn2 = *D;
+    double* o[2] = {o1, o2};
+    double ap[2] = {ap1, ap2}, an[2] = {an1, an2}, bp[2] = {bp1, bp2}, bn[2] = {bn1, bn2};
+    for (int s = 0; s < 2; s++) {
+      double* f = o[s];
+      if (axis == 0) { /* connect_normal_to_x.F90:334-355 */
+        f[0] = 0.0;
+        f[1] = -(ap[s] + an[s]) / dl;
+        f[2] = -(bp[s] + bn[s]) / dl;
+        f[3] = 0.0;
+        f[4] = (bp[s] - bn[s]) / (Z0 * dl);
+        f[5] = -(ap[s] - an[s]) / (Z0 * dl);
+      } else if (axis == 1) { /* connect_normal_to_y.F90:342-362 */
+        f[0] = -(ap[s] + an[s]) / dl;
+        f[1] = 0.0;
+        f[2] = -(bp[s] + bn[s]) / dl;
+        f[3] = -(bp[s] - bn[s]) / (Z0 * dl);
+        f[4] = 0.0;
+        f[5] = (ap[s] - an[s]) / (Z0 * dl);
+      } else { /* connect_normal_to_z.F90:337-361 */
+        f[0] = -(ap[s] + an[s]) / dl;
+        f[1] = -(bp[s] + bn[s]) / dl;
+        f[2] = 0.0;
+        f[3] = (bp[s] - bn[s]) / (Z0 * dl);
+        f[4] = -(ap[s] - an[s]) / (Z0 * dl);
+        f[5] = 0.0;
+      }
+    }
+  }
+  return TLM_OK;
+}
+
+int oracle_connect(oracle_handle* h, int32_t timestep) {
+  (void)timestep;
+  int nx = h->nx, ny = h->ny, nz = h->nz;
+  int face_number = 0, excitation_face_number = 0, output_face_number = 0;
+  if (!h->face_code) return fail(h, TLM_ERR_STATE, "face codes not set");
+  for (int cz = 1; cz <= nz; cz++)
+    for (int cy = 1; cy <= ny; cy++)
+      for (int cx = 1; cx <= nx; cx++) {
+        double* v = cellV(h, cx, cy, cz);
+        if (cx != 1) {
+          face_number++;
+          double* w = cellV(h, cx - 1, cy, cz);
+          int code = h->face_code[face_number - 1];
+          if (code == 0) { /* connect_normal_to_x.F90:4-10 */
+            double Vy = v[Vy_xmin] + w[Vy_xmax];
+            v[Vy_xmin] = Vy - v[Vy_xmin];
+            w[Vy_xmax] = Vy - w[Vy_xmax];
+            double Vz = v[Vz_xmin] + w[Vz_xmax];
+            v[Vz_xmin] = Vz - v[Vz_xmin];
+            w[Vz_xmax] = Vz - w[Vz_xmax];
+          } else {
+            int rc = connect_special_face(h, 0, v, w, code, &excitation_face_number, &output_face_number);
+            if (rc) return rc;
+          }
+        }
+        if (cy != 1) {
+          face_number++;
+          double* w = cellV(h, cx, cy - 1, cz);
+          int code = h->face_code[face_number - 1];
+          if (code == 0) {
+            double Vx = v[Vx_ymin] + w[Vx_ymax];
+            v[Vx_ymin] = Vx - v[Vx_ymin];
+            w[Vx_ymax] = Vx - w[Vx_ymax];
+            double Vz = v[Vz_ymin] + w[Vz_ymax];
+            v[Vz_ymin] = Vz - v[Vz_ymin];
+            w[Vz_ymax] = Vz - w[Vz_ymax];
+          } else {
+            int rc = connect_special_face(h, 1, v, w, code, &excitation_face_number, &output_face_number);
+            if (rc) return rc;
+          }
+        }
+        if (cz != 1) {
+          face_number++;
+          double* w = cellV(h, cx, cy, cz - 1);
+          int code = h->face_code[face_number - 1];
+          if (code == 0) {
+            double Vx = v[Vx_zmin] + w[Vx_zmax];
+            v[Vx_zmin] = Vx - v[Vx_zmin];
+            w[Vx_zmax] = Vx - w[Vx_zmax];
+            double Vy = v[Vy_zmin] + w[Vy_zmax];
+            v[Vy_zmin] = Vy - v[Vy_zmin];
+            w[Vy_zmax] = Vy - w[Vy_zmax];
+          } else {
+            int rc = connect_special_face(h, 2, v, w, code, &excitation_face_number, &output_face_number);
+            if (rc) return rc;
+          }
+        }
+      }
+  return TLM_OK;
+}
+
+/* outer_boundary.F90:71-131 (SEQ: rank 0 owns both z faces). */
+int oracle_outer_boundary(oracle_handle* h) {
+  int nx = h->nx, ny = h->ny, nz = h->nz;
+  const tlm_grid* g = &h->g;
+  for (int cz = 1; cz <= nz; cz++)
+    for (int cy = 1; cy <= ny; cy++) {
+      double* v = cellV(h, 1, cy, cz);
+      v[Vy_xmin] = v[Vy_xmin] * g->r_xmin;
+      v[Vz_xmin] = v[Vz_xmin] * g->r_xmin;
+      v = cellV(h, nx, cy, cz);
+      v[Vy_xmax] = v[Vy_xmax] * g->r_xmax;
+      v[Vz_xmax] = v[Vz_xmax] * g->r_xmax;
+    }
+  for (int cz = 1; cz <= nz; cz++)
+    for (int cx = 1; cx <= nx; cx++) {
+      double* v = cellV(h, cx, 1, cz);
+      v[Vx_ymin] = v[Vx_ymin] * g->r_ymin;
+      v[Vz_ymin] = v[Vz_ymin] * g->r_ymin;
+      v = cellV(h, cx, ny, cz);
+      v[Vx_ymax] = v[Vx_ymax] * g->r_ymax;
+      v[Vz_ymax] = v[Vz_ymax] * g->r_ymax;
+    }
+  for (int cy = 1; cy <= ny; cy++)
+    for (int cx = 1; cx <= nx; cx++) {
+      double* v = cellV(h, cx, cy, 1);
+      v[Vx_zmin] = v[Vx_zmin] * g->r_zmin;
+      v[Vy_zmin] = v[Vy_zmin] * g->r_zmin;
+    }
+  for (int cy = 1; cy <= ny; cy++)
+    for (int cx = 1; cx <= nx; cx++) {
+      double* v = cellV(h, cx, cy, nz);
+      v[Vx_zmax] = v[Vx_zmax] * g->r_zmax;
+      v[Vy_zmax] = v[Vy_zmax] * g->r_zmax;
+    }
+  return TLM_OK;
+}
+
+int oracle_step(oracle_handle* h, int32_t first_timestep, int32_t n_steps) {
+  for (int32_t t = first_timestep; t < first_timestep + n_steps; t++) {
+    int rc;
+    if ((rc = oracle_excitation(h, t))) return rc;
+    if ((rc = oracle_scatter(h, t))) return rc;
+    if ((rc = oracle_connect(h, t))) return rc;
+    if ((rc = oracle_outer_boundary(h))) return rc;
+  }
+  return TLM_OK;
+}
+
+int oracle_synchronize(oracle_handle* h) { (void)h; return TLM_OK; }
+
+int oracle_set_output_capacity(oracle_handle* h, int32_t steps) {
+  if (steps < 1) return fail(h, TLM_ERR_ARG, "capacity must be positive");
+  free(h->ring);
+  h->ring = NULL;
+  h->cap = steps;
+  h->ring_count = 0;
+  return TLM_OK;
+}
+
+int oracle_read_cell_outputs(oracle_handle* h, double* out, int32_t max_steps,
+                             int32_t* n_read, int32_t* first_step) {
+  int rows = h->n_out_rows;
+  int n = h->ring_count < max_steps ? h->ring_count : max_steps;
+  if (n > 0) memcpy(out, h->ring, (size_t)n * rows * 6 * sizeof(double));
+  if (n_read) *n_read = n;
+  if (first_step) *first_step = h->ring_first;
+  if (n < h->ring_count) {
+    memmove(h->ring, h->ring + (size_t)n * rows * 6, (size_t)(h->ring_count - n) * rows * 6 * sizeof(double));
+    h->ring_first += n;
+  }
+  h->ring_count -= n;
+  return TLM_OK;
+}
+
+int oracle_read_dft(oracle_handle* h, double* re_im) {
+  size_t n = (size_t)h->n_points * h->n_freq * 2;
+  if (n) memcpy(re_im, h->dft, n * sizeof(double));
+  return TLM_OK;
+}
+
+int oracle_read_face_outputs(oracle_handle* h, double* out) {
+  size_t n = (size_t)h->n_face_out_rows * 12;
+  if (n) memcpy(out, h->face_out, n * sizeof(double));
+  return TLM_OK;
+}
