@@ -1,0 +1,52 @@
+"""Build recipe of the engine library (hipcc, gfx950) and the Fortran binding.
+
+    python -m ggi_tlm_amd.build
+
+compiles csrc/tlmscn.hip into ggi_tlm_amd/libtlmscn.so in-tree.  The file is
+compiled with -ffp-contract=off: the plain FP64 form must not fuse multiply
+and add, so that it reproduces the reference's source-order arithmetic bit
+for bit.  Packed FP32 instructions are disabled for all device code.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SO = os.path.join(HERE, "libtlmscn.so")
+SRC = os.path.join(HERE, "csrc", "tlmscn.hip")
+ARCH = os.environ.get("TLM_OFFLOAD_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+
+def hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def build_engine(verbose: bool = False) -> str:
+    tmp = SO + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), SRC, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libtlmscn.so")
+    os.replace(tmp, SO)
+    return SO
+
+
+if __name__ == "__main__":
+    print(build_engine(verbose=True))

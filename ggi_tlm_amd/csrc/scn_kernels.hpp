@@ -1,0 +1,747 @@
+// scn_kernels.hpp -- gfx950 kernels of the SCN scatter/connect hot path.
+//
+// Layout in HBM: twelve structure-of-arrays link planes per buffer,
+// plane p holding port p+1 of every cell, cell index c = x + nx*(y + ny*z)
+// (x fastest, as the reference's scan order).  Two buffers ping-pong: a
+// step reads the previous step's scattered voltages from one buffer, pulls
+// each cell's incident voltages across its six faces (the reference's
+// connect and outer_boundary, SRC/TLM_SOLVER/connect.F90:180-214 and
+// outer_boundary.F90:71-131), scatters (scatter.F90:216-585) and writes the
+// new scattered voltages to the other buffer.  Each link therefore crosses
+// HBM once per step in each direction; the neighbour reads hit the caches.
+//
+// Cell classes run as separate, wavefront-uniform kernels:
+//   bulk    free-space cells with six ordinary faces (bitmask-selected)
+//   pml     PML cells (scatter_PML.F90), FP64 state in SoA arrays
+//   generic every other special cell: materials, point sources, probes,
+//           and cells touching a special face (PEC/PMC/free-space surface,
+//           face sources), following the Thevenin path of scatter.F90.
+//
+// Template parameters: T is the link storage type (double: plain form,
+// float: precision form); PULL selects between reading incident voltages
+// directly (state uploaded after a connect) and pulling them from the
+// scattered planes.  In the plain form every expression keeps the
+// reference's operation order and the file is compiled with
+// -ffp-contract=off, so results are bit-identical to the reference's
+// source-order FP64 arithmetic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tlmscn.h"
+
+namespace scn {
+
+constexpr double kZ0 = 376.73031346177;   // SRC/CONSTANTS/constants.F90:29
+constexpr double kY0 = 1.0 / kZ0;          // constants.F90:31
+constexpr double kC0 = 2.99792458e8;       // constants.F90:39
+constexpr double kPI = 3.141592653589793;  // constants.F90:27
+constexpr int NW = TLM_MAX_FILTER_ORDER + 1;
+
+// 0-based ports (Mesh_Modules.F90:83-94 minus one)
+enum {
+  Vy_xmin = 0, Vz_xmin, Vy_xmax, Vz_xmax, Vx_ymin, Vz_ymin,
+  Vx_ymax, Vz_ymax, Vx_zmin, Vy_zmin, Vx_zmax, Vy_zmax
+};
+
+// PML state words per cell (PML_module.F90:117-129), SoA: state[w*n + i]
+enum {
+  PS_Vxt, PS_Vyt, PS_Vzt, PS_Ix, PS_Iy, PS_Iz,
+  PS_Vi,  // 12 words
+  PS_Vyxt = PS_Vi + 12, PS_Vzxt, PS_Vxyt, PS_Vzyt, PS_Vyzt, PS_Vxzt,
+  PS_N    // 24 words streamed by every PML cell
+};
+// Ixt,Iyt,Izt live in a separate array, touched only by output cells.
+
+struct Dev {
+  int nx, ny, nz;
+  long long N;        // cells
+  long long P;        // plane stride (elements)
+  double dl, dt, c_red;  // c_red = c0 / reduced_c_factor
+  int n_timesteps;
+  double R[6];        // xmin,xmax,ymin,ymax,zmin,zmax
+  const uint32_t* mask;  // 1 = special cell (not bulk)
+  // pml class
+  int n_pml;
+  const int32_t* pml_cell;
+  const int32_t* pml_state_idx;  // PML_cell - 1
+  const int32_t* pml_par;        // 0-based
+  const int32_t* pml_out;        // output row (0-based) or -1
+  const int32_t* pml_fd;         // face descriptor or -1
+  const tlm_pml_parameter* pml_params;
+  double* pml_state;             // PS_N * n_pml_state
+  double* pml_It;                // 3 * n_pml_state (Ixt,Iyt,Izt)
+  long long n_pml_state;
+  // generic class
+  int n_gen;
+  const int32_t* gen_cell;
+  const int32_t* gen_code;   // reference code (0 = free cell beside a special face)
+  const int32_t* gen_mat;    // material number (1-based, 0 = none)
+  const int32_t* gen_filt;   // filter number (1-based)
+  const int32_t* gen_exc;    // cell excitation row (0-based) or -1
+  const int32_t* gen_out;    // output row (0-based) or -1
+  const int32_t* gen_fd;     // face descriptor or -1
+  const tlm_volume_material* vmat;
+  double* eps_w;             // NW * n_filt, SoA by word
+  double* mu_w;
+  long long n_filt;
+  // special faces
+  const int32_t* fd;         // 6 special-face codes per descriptor (0 = ordinary)
+  const int32_t* sf_mtype;   // per special face code (1-based index): surface material type
+  const int32_t* sf_exc;     // excitation row (0-based) or -1
+  // excitation
+  const double* value;       // (n_timesteps, n_functions) column-major
+  const double* value_face;
+  const int32_t* cex_start;  // CSR over cell excitation rows -> point sources
+  const int32_t* cex_src;
+  const int32_t* cex_type;   // 6 per cell row
+  const int32_t* fex_start;  // CSR over face excitation rows -> point sources
+  const int32_t* fex_src;
+  const int32_t* fex_type;   // 6 per face row (side 1 types)
+  const int32_t* fhy_start;  // CSR over face excitation rows -> Huygens faces
+  const int32_t* fhy_face;
+  const tlm_point_source* ps;
+  const tlm_huygens_face* hf;
+  int huy_function;
+  double huy_offset_min;
+  double Ei[3], Hi[3];
+  // outputs
+  int n_out_rows;
+  double* ring;              // slots of n_out_rows*6
+};
+
+// ---------------------------------------------------------------------------
+// excitation values on the fly
+// ---------------------------------------------------------------------------
+
+// get_interpolated_excitation_value, SRC/EXCITATION/excitation.F90:645-690.
+// NINT rounds half away from zero, as round() does.
+__device__ inline double huygens_value(const Dev& d, int timestep, double offset) {
+  double dt = d.dt;
+  double t_offset = (offset - d.huy_offset_min) / d.c_red;
+  double local_time = ((double)(timestep - 1)) * dt - t_offset;
+  int t1 = (int)round(local_time / dt);
+  int t2 = t1 + 1;
+  double delta_t = local_time - (double)t1 * dt;
+  const double* vf = d.value_face + (long long)(d.huy_function - 1) * d.n_timesteps;
+  double v1 = (t1 >= 1 && t1 <= d.n_timesteps) ? vf[t1 - 1] : 0.0;
+  double v2 = (t2 >= 1 && t2 <= d.n_timesteps) ? vf[t2 - 1] : 0.0;
+  return v1 + delta_t * (v2 - v1) / dt;
+}
+
+// face_excitation_field(row, side, 1:12) at `timestep`: zero, then the point
+// sources in order (excitation_points.F90:231-335), then the Huygens faces
+// (huygens_surface.F90:760-842).  Every contribution lands on both sides,
+// so one 12-vector serves both.
+__device__ inline void face_excitation(const Dev& d, int row, int timestep, double f[12]) {
+  for (int k = 0; k < 12; k++) f[k] = 0.0;
+  for (int s = d.fex_start[row]; s < d.fex_start[row + 1]; s++) {
+    const tlm_point_source& p = d.ps[d.fex_src[s]];
+    double v = d.value_face[(long long)(p.function - 1) * d.n_timesteps + (timestep - 1)];
+    f[p.component - 1] = f[p.component - 1] + v;
+  }
+  for (int s = d.fhy_start[row]; s < d.fhy_start[row + 1]; s++) {
+    const tlm_huygens_face& h = d.hf[d.fhy_face[s]];
+    double value = huygens_value(d, timestep, h.offset);
+    double nx = (double)h.nx, ny = (double)h.ny, nz = (double)h.nz;
+    double Js0 = (ny * value * (d.Hi[2]) - nz * value * (d.Hi[1]));
+    double Js1 = (nz * value * (d.Hi[0]) - nx * value * (d.Hi[2]));
+    double Js2 = (nx * value * (d.Hi[1]) - ny * value * (d.Hi[0]));
+    double Ms0 = -(ny * value * (d.Ei[2]) - nz * value * (d.Ei[1]));
+    double Ms1 = -(nz * value * (d.Ei[0]) - nx * value * (d.Ei[2]));
+    double Ms2 = -(nx * value * (d.Ei[1]) - ny * value * (d.Ei[0]));
+    f[6] = f[6] + Js0; f[7] = f[7] + Js1; f[8] = f[8] + Js2;
+    f[9] = f[9] + Ms0; f[10] = f[10] + Ms1; f[11] = f[11] + Ms2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// incident voltages (connect + outer boundary of the previous step)
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ inline double ld(const T* __restrict__ L, long long P, int port, long long c) {
+  return (double)L[(long long)port * P + c];
+}
+
+// Ordinary face: S = V_min + V_max; incident = S - own
+// (connect_normal_to_x.F90:4-10).  The precision form takes the exact
+// swap, incident = neighbour, which has no rounding.
+template <bool PLAIN>
+__device__ inline double free_face(double own, double nb) {
+  if (PLAIN) {
+    double S = own + nb;
+    return S - own;
+  }
+  return nb;
+}
+
+// Special face (connect_normal_to_{x,y,z}.F90:12-314) seen from one side.
+// axis 0/1/2; is_min: the cell owns the face as its min face (ports
+// V?_xmin of cell cx in the reference's naming), else it is the max-side
+// cell cx-1.  own_a/own_b: this cell's scattered values of the two
+// polarisations (a,b) = (y,z) | (x,z) | (x,y); nb_a/nb_b: the other side's.
+template <bool PLAIN>
+__device__ inline void special_face(const Dev& d, int code, int axis, bool is_min,
+                                    double own_a, double own_b, double nb_a, double nb_b,
+                                    int ts_connect, double& inc_a, double& inc_b) {
+  int mtype = d.sf_mtype[code];
+  double min_a = is_min ? own_a : nb_a, max_a = is_min ? nb_a : own_a;
+  double min_b = is_min ? own_b : nb_b, max_b = is_min ? nb_b : own_b;
+  double a_min, a_max, b_min, b_max;
+  if (mtype == TLM_SMAT_PEC) {
+    a_min = 0.0; a_max = 0.0; b_min = 0.0; b_max = 0.0;
+  } else if (mtype == TLM_SMAT_PMC) {
+    a_min = 2.0 * min_a; a_max = 2.0 * max_a;
+    b_min = 2.0 * min_b; b_max = 2.0 * max_b;
+  } else {  // none or FREE_SPACE
+    a_min = min_a + max_a; a_max = a_min;
+    b_min = min_b + max_b; b_max = b_min;
+  }
+  int row = d.sf_exc[code];
+  if (row >= 0) {
+    double f[12];
+    face_excitation(d, row, ts_connect, f);
+    const int32_t* h = d.fex_type + 6 * (long long)row;  // side 1 types for both sides
+    const double dl = d.dl, Z0 = kZ0;
+    const double* Js = f + 6;
+    const double* Ms = f + 9;
+    if (axis == 0) {
+      a_min = a_min * (double)h[1] - f[1] * dl - f[5] * Z0 * dl + Js[1] * dl * Z0 / 2.0 + Ms[2] * dl / 2.0;
+      a_max = a_max * (double)h[1] - f[1] * dl + f[5] * Z0 * dl + Js[1] * dl * Z0 / 2.0 - Ms[2] * dl / 2.0;
+      b_min = b_min * (double)h[2] - f[2] * dl + f[4] * Z0 * dl + Js[2] * dl * Z0 / 2.0 - Ms[1] * dl / 2.0;
+      b_max = b_max * (double)h[2] - f[2] * dl - f[4] * Z0 * dl + Js[2] * dl * Z0 / 2.0 + Ms[1] * dl / 2.0;
+    } else if (axis == 1) {
+      a_min = a_min * (double)h[0] - f[0] * dl + f[5] * Z0 * dl + Js[0] * dl * Z0 / 2.0 - Ms[2] * dl / 2.0;
+      a_max = a_max * (double)h[0] - f[0] * dl - f[5] * Z0 * dl + Js[0] * dl * Z0 / 2.0 + Ms[2] * dl / 2.0;
+      b_min = b_min * (double)h[2] - f[2] * dl - f[3] * Z0 * dl + Js[2] * dl * Z0 / 2.0 + Ms[0] * dl / 2.0;
+      b_max = b_max * (double)h[2] - f[2] * dl + f[3] * Z0 * dl + Js[2] * dl * Z0 / 2.0 - Ms[0] * dl / 2.0;
+    } else {
+      a_min = a_min * (double)h[0] - f[0] * dl - f[4] * Z0 * dl + Js[0] * dl * Z0 / 2.0 + Ms[1] * dl / 2.0;
+      a_max = a_max * (double)h[0] - f[0] * dl + f[4] * Z0 * dl + Js[0] * dl * Z0 / 2.0 - Ms[1] * dl / 2.0;
+      b_min = b_min * (double)h[1] - f[1] * dl + f[3] * Z0 * dl + Js[1] * dl * Z0 / 2.0 - Ms[0] * dl / 2.0;
+      b_max = b_max * (double)h[1] - f[1] * dl - f[3] * Z0 * dl + Js[1] * dl * Z0 / 2.0 + Ms[0] * dl / 2.0;
+    }
+  }
+  if (is_min) {
+    inc_a = a_min - own_a;
+    inc_b = b_min - own_b;
+  } else {
+    inc_a = a_max - own_a;
+    inc_b = b_max - own_b;
+  }
+  (void)PLAIN;
+}
+
+struct PullArgs {
+  bool apply_r;    // outer_boundary pending
+  int ts_connect;  // timestep of the pending connect (face sources)
+};
+
+// Incident voltages of cell c = (x,y,z) in v[12].  fdi: face descriptor
+// (-1 = six ordinary faces).
+template <typename T, bool PLAIN>
+__device__ inline void pull_cell(const Dev& d, const T* __restrict__ L, long long c, int x, int y, int z,
+                                 int fdi, const PullArgs& a, double v[12]) {
+  const long long P = d.P;
+  const long long sx = 1, sy = d.nx, sz = (long long)d.nx * d.ny;
+  double o[12];
+#pragma unroll
+  for (int p = 0; p < 12; p++) o[p] = ld(L, P, p, c);
+  int fc[6] = {0, 0, 0, 0, 0, 0};
+  if (fdi >= 0) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) fc[k] = d.fd[6 * fdi + k];
+  }
+  // per face side: (own ports a,b), (neighbour ports a,b), offset, boundary, R
+  const int pa[6] = {Vy_xmin, Vy_xmax, Vx_ymin, Vx_ymax, Vx_zmin, Vx_zmax};
+  const int pb[6] = {Vz_xmin, Vz_xmax, Vz_ymin, Vz_ymax, Vy_zmin, Vy_zmax};
+  const int qa[6] = {Vy_xmax, Vy_xmin, Vx_ymax, Vx_ymin, Vx_zmax, Vx_zmin};
+  const int qb[6] = {Vz_xmax, Vz_xmin, Vz_ymax, Vz_ymin, Vy_zmax, Vy_zmin};
+  const long long off[6] = {-sx, sx, -sy, sy, -sz, sz};
+  const bool bnd[6] = {x == 0, x == d.nx - 1, y == 0, y == d.ny - 1, z == 0, z == d.nz - 1};
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    double oa = o[pa[s]], ob = o[pb[s]];
+    if (bnd[s]) {
+      double R = a.apply_r ? d.R[s] : 1.0;
+      v[pa[s]] = oa * R;
+      v[pb[s]] = ob * R;
+      continue;
+    }
+    double na = ld(L, P, qa[s], c + off[s]);
+    double nb = ld(L, P, qb[s], c + off[s]);
+    if (fc[s] == 0) {
+      v[pa[s]] = free_face<PLAIN>(oa, na);
+      v[pb[s]] = free_face<PLAIN>(ob, nb);
+    } else {
+      double ia, ib;
+      special_face<PLAIN>(d, fc[s], s >> 1, (s & 1) == 0, oa, ob, na, nb, a.ts_connect, ia, ib);
+      v[pa[s]] = ia;
+      v[pb[s]] = ib;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// scatter pieces
+// ---------------------------------------------------------------------------
+
+// 12-port scatter (scatter.F90:240-272 and :517-547), in place on v.
+__device__ inline void scatter_ports(double v[12], double Vx, double Vy, double Vz,
+                                     double Ix, double Iy, double Iz) {
+  const double Z0 = kZ0;
+  double m, M;
+  m = v[Vx_ymin]; M = v[Vx_ymax];
+  v[Vx_ymin] = Vx - Z0 * Iz - M;
+  v[Vx_ymax] = Vx + Z0 * Iz - m;
+  m = v[Vx_zmin]; M = v[Vx_zmax];
+  v[Vx_zmin] = Vx + Z0 * Iy - M;
+  v[Vx_zmax] = Vx - Z0 * Iy - m;
+  m = v[Vy_zmin]; M = v[Vy_zmax];
+  v[Vy_zmin] = Vy - Z0 * Ix - M;
+  v[Vy_zmax] = Vy + Z0 * Ix - m;
+  m = v[Vy_xmin]; M = v[Vy_xmax];
+  v[Vy_xmin] = Vy + Z0 * Iz - M;
+  v[Vy_xmax] = Vy - Z0 * Iz - m;
+  m = v[Vz_xmin]; M = v[Vz_xmax];
+  v[Vz_xmin] = Vz - Z0 * Iy - M;
+  v[Vz_xmax] = Vz + Z0 * Iy - m;
+  m = v[Vz_ymin]; M = v[Vz_ymax];
+  v[Vz_ymin] = Vz + Z0 * Ix - M;
+  v[Vz_ymax] = Vz - Z0 * Ix - m;
+}
+
+// Free-space cell (scatter.F90:222-272).
+__device__ inline void scatter_free_plain(double v[12]) {
+  const double Z0 = kZ0;
+  double Vx = (v[Vx_ymin] + v[Vx_ymax] + v[Vx_zmin] + v[Vx_zmax]) / 2.0;
+  double Vy = (v[Vy_xmin] + v[Vy_xmax] + v[Vy_zmin] + v[Vy_zmax]) / 2.0;
+  double Vz = (v[Vz_xmin] + v[Vz_xmax] + v[Vz_ymin] + v[Vz_ymax]) / 2.0;
+  double Ix = (v[Vz_ymax] - v[Vy_zmax] - v[Vz_ymin] + v[Vy_zmin]) / (2.0 * Z0);
+  double Iy = (v[Vx_zmax] - v[Vz_xmax] - v[Vx_zmin] + v[Vz_xmin]) / (2.0 * Z0);
+  double Iz = (v[Vy_xmax] - v[Vx_ymax] - v[Vy_xmin] + v[Vx_ymin]) / (2.0 * Z0);
+  scatter_ports(v, Vx, Vy, Vz, Ix, Iy, Iz);
+}
+
+// Precision form of the same update: Z0*I is the half curl exactly in real
+// arithmetic, so the division by 2 Z0 and multiplication by Z0 are folded.
+__device__ inline void scatter_free_fast(double v[12]) {
+  double Vx = (v[Vx_ymin] + v[Vx_ymax] + v[Vx_zmin] + v[Vx_zmax]) * 0.5;
+  double Vy = (v[Vy_xmin] + v[Vy_xmax] + v[Vy_zmin] + v[Vy_zmax]) * 0.5;
+  double Vz = (v[Vz_xmin] + v[Vz_xmax] + v[Vz_ymin] + v[Vz_ymax]) * 0.5;
+  double Ix = (v[Vz_ymax] - v[Vy_zmax] - v[Vz_ymin] + v[Vy_zmin]) * 0.5;
+  double Iy = (v[Vx_zmax] - v[Vz_xmax] - v[Vx_zmin] + v[Vz_xmin]) * 0.5;
+  double Iz = (v[Vy_xmax] - v[Vx_ymax] - v[Vy_xmin] + v[Vx_ymin]) * 0.5;
+  double m, M;
+  m = v[Vx_ymin]; M = v[Vx_ymax];
+  v[Vx_ymin] = Vx - Iz - M;  v[Vx_ymax] = Vx + Iz - m;
+  m = v[Vx_zmin]; M = v[Vx_zmax];
+  v[Vx_zmin] = Vx + Iy - M;  v[Vx_zmax] = Vx - Iy - m;
+  m = v[Vy_zmin]; M = v[Vy_zmax];
+  v[Vy_zmin] = Vy - Ix - M;  v[Vy_zmax] = Vy + Ix - m;
+  m = v[Vy_xmin]; M = v[Vy_xmax];
+  v[Vy_xmin] = Vy + Iz - M;  v[Vy_xmax] = Vy - Iz - m;
+  m = v[Vz_xmin]; M = v[Vz_xmax];
+  v[Vz_xmin] = Vz - Iy - M;  v[Vz_xmax] = Vz + Iy - m;
+  m = v[Vz_ymin]; M = v[Vz_ymax];
+  v[Vz_ymin] = Vz + Ix - M;  v[Vz_ymax] = Vz - Ix - m;
+}
+
+template <typename T>
+__device__ inline void store_cell(T* __restrict__ L, long long P, long long c, const double v[12]) {
+#pragma unroll
+  for (int p = 0; p < 12; p++) L[(long long)p * P + c] = (T)v[p];
+}
+
+// evaluate_Zfilter / timeshift_Zfilter (FILTER/filter_subroutines.F90:510-575)
+// on SoA state w[k*n + f].
+__device__ inline double zf_evaluate(const double* a, const double* b, double* w, long long n,
+                                     long long f, int order, double G) {
+  double W0 = G;
+  for (int i = 1; i <= order; i++) W0 = W0 - b[i] * w[i * n + f];
+  W0 = W0 / b[0];
+  w[f] = W0;
+  double F0 = 0.0;
+  for (int i = 0; i <= order; i++) F0 = F0 + a[i] * w[i * n + f];
+  return F0;
+}
+__device__ inline void zf_timeshift(double* w, long long n, long long f, int order) {
+  for (int i = order; i >= 1; i--) w[i * n + f] = w[(i - 1) * n + f];
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+// Bulk free-space cells.  Block (64,4): one wavefront per x-row segment.
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_bulk(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                             PullArgs a) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y * blockDim.y + threadIdx.y;
+  int z = blockIdx.z;
+  if (x >= d.nx || y >= d.ny) return;
+  long long c = x + (long long)d.nx * (y + (long long)d.ny * z);
+  if ((d.mask[c >> 5] >> (c & 31)) & 1u) return;
+  double v[12];
+  if (PULL) {
+    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, -1, a, v);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
+  }
+  if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
+  store_cell(Lout, d.P, c, v);
+}
+
+// PML cells: scatter_PML.F90 stages 1-13.
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_pml(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                            PullArgs a, int slot) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n_pml) return;
+  const long long c = d.pml_cell[i];
+  const int nxy = d.nx * d.ny;
+  const int z = (int)(c / nxy);
+  const int rem = (int)(c - (long long)z * nxy);
+  const int y = rem / d.nx, x = rem - y * d.nx;
+  double v[12];
+  if (PULL) {
+    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, d.pml_fd[i], a, v);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
+  }
+  const double Z0 = kZ0, dl = d.dl;
+  const tlm_pml_parameter& Pp = d.pml_params[d.pml_par[i]];
+  const double ax = Pp.ax, ay = Pp.ay, az = Pp.az;
+  const double exp_x = Pp.exp_x, exp_y = Pp.exp_y, exp_z = Pp.exp_z;
+  const double Csx = Pp.Csx, Csy = Pp.Csy, Csz = Pp.Csz;
+  const double Csx2 = Pp.Csx2, Csy2 = Pp.Csy2, Csz2 = Pp.Csz2;
+  const double Csy_sz = Pp.Csy_sz, Csz_sx = Pp.Csz_sx, Csx_sy = Pp.Csx_sy;
+  const long long n = d.n_pml_state;
+  const long long si = d.pml_state_idx[i];
+  double* S = d.pml_state;
+  // STAGE 2 (state reads)
+  const double last_Ix = S[PS_Ix * n + si], last_Iy = S[PS_Iy * n + si], last_Iz = S[PS_Iz * n + si];
+  double lv[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) lv[k] = S[(PS_Vi + k) * n + si];
+  const double last_Vxt = S[PS_Vxt * n + si], last_Vyt = S[PS_Vyt * n + si], last_Vzt = S[PS_Vzt * n + si];
+  const double last_Vyxt = S[PS_Vyxt * n + si], last_Vzxt = S[PS_Vzxt * n + si];
+  const double last_Vxyt = S[PS_Vxyt * n + si], last_Vzyt = S[PS_Vzyt * n + si];
+  const double last_Vyzt = S[PS_Vyzt * n + si], last_Vxzt = S[PS_Vxzt * n + si];
+  // PML port aliases (PML_module.F90:60-71), 0-based
+  enum { Vxny = 0, Vxnz, Vxpy, Vxpz, Vynx, Vynz, Vypx, Vypz, Vznx, Vzny, Vzpx, Vzpy };
+  // STAGE 3
+  v[Vynx] = v[Vynx] * exp_y; v[Vypx] = v[Vypx] * exp_y;
+  v[Vznx] = v[Vznx] * exp_z; v[Vzpx] = v[Vzpx] * exp_z;
+  v[Vxny] = v[Vxny] * exp_x; v[Vxpy] = v[Vxpy] * exp_x;
+  v[Vzny] = v[Vzny] * exp_z; v[Vzpy] = v[Vzpy] * exp_z;
+  v[Vxnz] = v[Vxnz] * exp_x; v[Vxpz] = v[Vxpz] * exp_x;
+  v[Vynz] = v[Vynz] * exp_y; v[Vypz] = v[Vypz] * exp_y;
+  // STAGE 4
+  const double Vx = (v[Vynx] + v[Vypx] + v[Vznx] + v[Vzpx]) / 2.0;
+  const double Vy = (v[Vxny] + v[Vxpy] + v[Vzny] + v[Vzpy]) / 2.0;
+  const double Vz = (v[Vxnz] + v[Vxpz] + v[Vynz] + v[Vypz]) / 2.0;
+  const double Ix = (v[Vypz] - v[Vzpy] - v[Vynz] + v[Vzny]) / (2.0 * Z0);
+  const double Iy = (v[Vzpx] - v[Vxpz] - v[Vznx] + v[Vxnz]) / (2.0 * Z0);
+  const double Iz = (v[Vxpy] - v[Vypx] - v[Vxny] + v[Vynx]) / (2.0 * Z0);
+  // STAGE 5
+  const double Vx_sh = ax * (-(lv[Vznx] + lv[Vzpx] + lv[Vynx] + lv[Vypx]) / 2.0
+                             + Csy * ((v[Vznx] + v[Vzpx]) + (lv[Vznx] + lv[Vzpx]))
+                             + Csz * ((v[Vynx] + v[Vypx]) + (lv[Vynx] + lv[Vypx]))
+                             + Csy_sz * last_Vxt);
+  const double Vy_sh = ay * (-(lv[Vxny] + lv[Vxpy] + lv[Vzny] + lv[Vzpy]) / 2.0
+                             + Csz * ((v[Vxny] + v[Vxpy]) + (lv[Vxny] + lv[Vxpy]))
+                             + Csx * ((v[Vzny] + v[Vzpy]) + (lv[Vzny] + lv[Vzpy]))
+                             + Csz_sx * last_Vyt);
+  const double Vz_sh = az * (-(lv[Vynz] + lv[Vypz] + lv[Vxnz] + lv[Vxpz]) / 2.0
+                             + Csx * ((v[Vynz] + v[Vypz]) + (lv[Vynz] + lv[Vypz]))
+                             + Csy * ((v[Vxnz] + v[Vxpz]) + (lv[Vxnz] + lv[Vxpz]))
+                             + Csx_sy * last_Vzt);
+  // STAGE 6
+  const double Vxt = ax * Vx + Vx_sh;
+  const double Vyt = ay * Vy + Vy_sh;
+  const double Vzt = az * Vz + Vz_sh;
+  // STAGE 7a
+  const int orow = d.pml_out[i];
+  double Ixt = 0.0, Iyt = 0.0, Izt = 0.0;
+  if (orow >= 0) {
+    const double lIxt = d.pml_It[0 * n + si], lIyt = d.pml_It[1 * n + si], lIzt = d.pml_It[2 * n + si];
+    Ixt = ax * (Ix - last_Ix + Csx2 * (Ix + last_Ix) + Csy_sz * lIxt);
+    Iyt = ay * (Iy - last_Iy + Csy2 * (Iy + last_Iy) + Csz_sx * lIyt);
+    Izt = az * (Iz - last_Iz + Csz2 * (Iz + last_Iz) + Csx_sy * lIzt);
+  }
+  // STAGE 7
+  const double Vxy_se = az * (Z0 * (-last_Iz + Csy2 * (Iz + last_Iz)) + Csx_sy * last_Vxyt);
+  const double Vyx_se = az * (Z0 * (-last_Iz + Csx2 * (Iz + last_Iz)) + Csx_sy * last_Vyxt);
+  const double Vzx_se = ay * (Z0 * (-last_Iy + Csx2 * (Iy + last_Iy)) + Csz_sx * last_Vzxt);
+  const double Vxz_se = ay * (Z0 * (-last_Iy + Csz2 * (Iy + last_Iy)) + Csz_sx * last_Vxzt);
+  const double Vyz_se = ax * (Z0 * (-last_Ix + Csz2 * (Ix + last_Ix)) + Csy_sz * last_Vyzt);
+  const double Vzy_se = ax * (Z0 * (-last_Ix + Csy2 * (Ix + last_Ix)) + Csy_sz * last_Vzyt);
+  // STAGE 8
+  const double Vxyt = az * Iz * Z0 + Vxy_se;
+  const double Vyxt = az * Iz * Z0 + Vyx_se;
+  const double Vzxt = ay * Iy * Z0 + Vzx_se;
+  const double Vxzt = ay * Iy * Z0 + Vxz_se;
+  const double Vyzt = ax * Ix * Z0 + Vyz_se;
+  const double Vzyt = ax * Ix * Z0 + Vzy_se;
+  // STAGE 9
+  S[PS_Ix * n + si] = Ix; S[PS_Iy * n + si] = Iy; S[PS_Iz * n + si] = Iz;
+#pragma unroll
+  for (int k = 0; k < 12; k++) S[(PS_Vi + k) * n + si] = v[k];
+  S[PS_Vxt * n + si] = Vxt; S[PS_Vyt * n + si] = Vyt; S[PS_Vzt * n + si] = Vzt;
+  if (orow >= 0) {
+    d.pml_It[0 * n + si] = Ixt; d.pml_It[1 * n + si] = Iyt; d.pml_It[2 * n + si] = Izt;
+  }
+  S[PS_Vxyt * n + si] = Vxyt; S[PS_Vyxt * n + si] = Vyxt;
+  S[PS_Vzxt * n + si] = Vzxt; S[PS_Vxzt * n + si] = Vxzt;
+  S[PS_Vzyt * n + si] = Vzyt; S[PS_Vyzt * n + si] = Vyzt;
+  // STAGE 10
+  const double Vynx_P = Vx_sh - Vyx_se, Vypx_P = Vx_sh + Vyx_se;
+  const double Vznx_P = Vx_sh + Vzx_se, Vzpx_P = Vx_sh - Vzx_se;
+  const double Vxny_P = Vy_sh + Vxy_se, Vxpy_P = Vy_sh - Vxy_se;
+  const double Vzny_P = Vy_sh - Vzy_se, Vzpy_P = Vy_sh + Vzy_se;
+  const double Vynz_P = Vz_sh + Vyz_se, Vypz_P = Vz_sh - Vyz_se;
+  const double Vxnz_P = Vz_sh - Vxz_se, Vxpz_P = Vz_sh + Vxz_se;
+  // STAGE 11
+  double M, m;
+  M = v[Vypx]; m = v[Vynx];
+  v[Vynx] = ax * Vx - az * Z0 * Iz - M + Vynx_P;
+  v[Vypx] = ax * Vx + az * Z0 * Iz - m + Vypx_P;
+  M = v[Vzpx]; m = v[Vznx];
+  v[Vznx] = ax * Vx + ay * Z0 * Iy - M + Vznx_P;
+  v[Vzpx] = ax * Vx - ay * Z0 * Iy - m + Vzpx_P;
+  M = v[Vzpy]; m = v[Vzny];
+  v[Vzny] = ay * Vy - ax * Z0 * Ix - M + Vzny_P;
+  v[Vzpy] = ay * Vy + ax * Z0 * Ix - m + Vzpy_P;
+  M = v[Vxpy]; m = v[Vxny];
+  v[Vxny] = ay * Vy + az * Z0 * Iz - M + Vxny_P;
+  v[Vxpy] = ay * Vy - az * Z0 * Iz - m + Vxpy_P;
+  M = v[Vxpz]; m = v[Vxnz];
+  v[Vxnz] = az * Vz - ay * Z0 * Iy - M + Vxnz_P;
+  v[Vxpz] = az * Vz + ay * Z0 * Iy - m + Vxpz_P;
+  M = v[Vypz]; m = v[Vynz];
+  v[Vynz] = az * Vz + ax * Z0 * Ix - M + Vynz_P;
+  v[Vypz] = az * Vz - ax * Z0 * Ix - m + Vypz_P;
+  // STAGE 12
+  if (orow >= 0) {
+    double* fo = d.ring + ((long long)slot * d.n_out_rows + orow) * 6;
+    fo[0] = -Vxt / dl; fo[1] = -Vyt / dl; fo[2] = -Vzt / dl;
+    fo[3] = Ixt / dl; fo[4] = Iyt / dl; fo[5] = Izt / dl;
+  }
+  // STAGE 13
+  v[Vynx] = v[Vynx] * exp_y; v[Vypx] = v[Vypx] * exp_y;
+  v[Vznx] = v[Vznx] * exp_z; v[Vzpx] = v[Vzpx] * exp_z;
+  v[Vxny] = v[Vxny] * exp_x; v[Vxpy] = v[Vxpy] * exp_x;
+  v[Vzny] = v[Vzny] * exp_z; v[Vzpy] = v[Vzpy] * exp_z;
+  v[Vxnz] = v[Vxnz] * exp_x; v[Vxpz] = v[Vxpz] * exp_x;
+  v[Vynz] = v[Vynz] * exp_y; v[Vypz] = v[Vypz] * exp_y;
+  store_cell(Lout, d.P, c, v);
+}
+
+// Generic special cells: scatter.F90:274-547 (Thevenin node), or the
+// free-space branch for code 0 cells that only touch a special face.
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_generic(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                PullArgs a, int timestep, int slot) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n_gen) return;
+  const long long c = d.gen_cell[i];
+  const int nxy = d.nx * d.ny;
+  const int z = (int)(c / nxy);
+  const int rem = (int)(c - (long long)z * nxy);
+  const int y = rem / d.nx, x = rem - y * d.nx;
+  double v[12];
+  if (PULL) {
+    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, d.gen_fd[i], a, v);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
+  }
+  if (d.gen_code[i] == 0) {
+    if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
+    store_cell(Lout, d.P, c, v);
+    return;
+  }
+  const double Z0 = kZ0, dl = d.dl;
+  const double Z04 = Z0 * 4.0, Y04 = kY0 * 4.0;
+  const int mnum = d.gen_mat[i];
+  const tlm_volume_material* vm = mnum ? &d.vmat[mnum - 1] : nullptr;
+  const int mtype = vm ? vm->type : 0;
+  double VEx = (v[Vx_ymin] + v[Vx_ymax] + v[Vx_zmin] + v[Vx_zmax]) / 2.0;
+  double VEy = (v[Vy_xmin] + v[Vy_xmax] + v[Vy_zmin] + v[Vy_zmax]) / 2.0;
+  double VEz = (v[Vz_xmin] + v[Vz_xmax] + v[Vz_ymin] + v[Vz_ymax]) / 2.0;
+  double Yx = Y04, Yy = Y04, Yz = Y04;
+  double VHx = 2.0 * (v[Vz_ymax] - v[Vy_zmax] - v[Vz_ymin] + v[Vy_zmin]);
+  double VHy = 2.0 * (v[Vx_zmax] - v[Vz_xmax] - v[Vx_zmin] + v[Vz_xmin]);
+  double VHz = 2.0 * (v[Vy_xmax] - v[Vx_ymax] - v[Vy_xmin] + v[Vx_ymin]);
+  double Zx = Z04, Zy = Z04, Zz = Z04;
+  double Vox = 0.0, Voy = 0.0, Voz = 0.0, Ys = 0.0;
+  const long long nf = d.n_filt;
+  const long long f = d.gen_filt[i];
+  if (mtype == TLM_VMAT_PEC) {
+    VEx = 0.0; VEy = 0.0; VEz = 0.0;
+  } else if (mtype == TLM_VMAT_PMC) {
+    VHx = 0.0; VHy = 0.0; VHz = 0.0;
+  } else if (mtype == TLM_VMAT_DISPERSIVE) {
+    if (vm->eps_filter_exists) {
+      Ys = vm->Ys_eps_f;
+      const int o = vm->eps_order;
+      zf_timeshift(d.eps_w, nf, f, o);
+      Vox = zf_evaluate(vm->eps_a, vm->eps_b, d.eps_w, nf, f, o, 0.0);
+      zf_timeshift(d.eps_w, nf, f + 1, o);
+      Voy = zf_evaluate(vm->eps_a, vm->eps_b, d.eps_w, nf, f + 1, o, 0.0);
+      zf_timeshift(d.eps_w, nf, f + 2, o);
+      Voz = zf_evaluate(vm->eps_a, vm->eps_b, d.eps_w, nf, f + 2, o, 0.0);
+      // add_shunt_admittance (scatter.F90:606-626)
+      VEx = (VEx * Yx + Vox * Ys) / (Yx + Ys); Yx = Yx + Ys;
+      VEy = (VEy * Yy + Voy * Ys) / (Yy + Ys); Yy = Yy + Ys;
+      VEz = (VEz * Yz + Voz * Ys) / (Yz + Ys); Yz = Yz + Ys;
+    }
+    const double Ge = vm->Ge;
+    VEx = (VEx * Yx + 0.0 * Ge) / (Yx + Ge); Yx = Yx + Ge;
+    VEy = (VEy * Yy + 0.0 * Ge) / (Yy + Ge); Yy = Yy + Ge;
+    VEz = (VEz * Yz + 0.0 * Ge) / (Yz + Ge); Yz = Yz + Ge;
+    if (vm->mu_filter_exists) {
+      const double Zs = vm->Zs_mu_f;
+      const int o = vm->mu_order;
+      zf_timeshift(d.mu_w, nf, f, o);
+      const double Vsx = zf_evaluate(vm->mu_a, vm->mu_b, d.mu_w, nf, f, o, 0.0);
+      zf_timeshift(d.mu_w, nf, f + 1, o);
+      const double Vsy = zf_evaluate(vm->mu_a, vm->mu_b, d.mu_w, nf, f + 1, o, 0.0);
+      zf_timeshift(d.mu_w, nf, f + 2, o);
+      const double Vsz = zf_evaluate(vm->mu_a, vm->mu_b, d.mu_w, nf, f + 2, o, 0.0);
+      // add_series_impedance (scatter.F90:641-661)
+      VHx = VHx + Vsx; Zx = Zx + Zs;
+      VHy = VHy + Vsy; Zy = Zy + Zs;
+      VHz = VHz + Vsz; Zz = Zz + Zs;
+    }
+    const double Rm = vm->Rm;
+    VHx = VHx + 0.0; Zx = Zx + Rm;
+    VHy = VHy + 0.0; Zy = Zy + Rm;
+    VHz = VHz + 0.0; Zz = Zz + Rm;
+  }
+  double Vx = VEx, Vy = VEy, Vz = VEz;
+  double Ix = VHx / Zx, Iy = VHy / Zy, Iz = VHz / Zz;
+  const int er = d.gen_exc[i];
+  if (er >= 0) {
+    // cell_excitation_field(row,1:9) at this timestep (excitation_points.F90:262-275)
+    double fl[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int s = d.cex_start[er]; s < d.cex_start[er + 1]; s++) {
+      const tlm_point_source& p = d.ps[d.cex_src[s]];
+      double val = d.value[(long long)(p.function - 1) * d.n_timesteps + (timestep - 1)];
+      fl[p.component - 1] = fl[p.component - 1] + val;
+    }
+    const int32_t* h = d.cex_type + 6 * (long long)er;
+    const double Isx = fl[6] * dl * dl, Isy = fl[7] * dl * dl, Isz = fl[8] * dl * dl;
+    Vx = (double)h[0] * Vx - fl[0] * dl / 2.0 + Isx / Yx;
+    Vy = (double)h[1] * Vy - fl[1] * dl / 2.0 + Isy / Yy;
+    Vz = (double)h[2] * Vz - fl[2] * dl / 2.0 + Isz / Yz;
+    Ix = (double)h[3] * Ix + fl[3] * dl / 2.0;
+    Iy = (double)h[4] * Iy + fl[4] * dl / 2.0;
+    Iz = (double)h[5] * Iz + fl[5] * dl / 2.0;
+  }
+  const int orow = d.gen_out[i];
+  if (orow >= 0) {
+    double* fo = d.ring + ((long long)slot * d.n_out_rows + orow) * 6;
+    fo[0] = -Vx / dl; fo[1] = -Vy / dl; fo[2] = -Vz / dl;
+    fo[3] = Ix / dl; fo[4] = Iy / dl; fo[5] = Iz / dl;
+  }
+  if (mtype == TLM_VMAT_DISPERSIVE) {
+    if (vm->eps_filter_exists) {
+      const double Ysx = vm->Ys_eps_f;
+      const int o = vm->eps_order;
+      zf_evaluate(vm->eps_a, vm->eps_b, d.eps_w, nf, f, o, (Vx - Vox) * Ysx);
+      zf_evaluate(vm->eps_a, vm->eps_b, d.eps_w, nf, f + 1, o, (Vy - Voy) * Ysx);
+      zf_evaluate(vm->eps_a, vm->eps_b, d.eps_w, nf, f + 2, o, (Vz - Voz) * Ysx);
+    }
+    if (vm->mu_filter_exists) {
+      const int o = vm->mu_order;
+      zf_evaluate(vm->mu_a, vm->mu_b, d.mu_w, nf, f, o, -Ix);
+      zf_evaluate(vm->mu_a, vm->mu_b, d.mu_w, nf, f + 1, o, -Iy);
+      zf_evaluate(vm->mu_a, vm->mu_b, d.mu_w, nf, f + 2, o, -Iz);
+    }
+  }
+  scatter_ports(v, Vx, Vy, Vz, Ix, Iy, Iz);
+  store_cell(Lout, d.P, c, v);
+}
+
+// Connect only (no scatter): incident voltages into Lout, used to hand the
+// state back after a connect.  Bulk variant over all cells; the class
+// lists then rewrite their own cells with special-face pulls.
+template <typename T, bool PLAIN>
+__global__ __launch_bounds__(256) void k_connect_bulk(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                     PullArgs a) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  int y = blockIdx.y * blockDim.y + threadIdx.y;
+  int z = blockIdx.z;
+  if (x >= d.nx || y >= d.ny) return;
+  long long c = x + (long long)d.nx * (y + (long long)d.ny * z);
+  double v[12];
+  pull_cell<T, PLAIN>(d, Lin, c, x, y, z, -1, a, v);
+  store_cell(Lout, d.P, c, v);
+}
+
+template <typename T, bool PLAIN>
+__global__ __launch_bounds__(256) void k_connect_list(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                     PullArgs a, const int32_t* cells, const int32_t* fds, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long c = cells[i];
+  const int nxy = d.nx * d.ny;
+  const int z = (int)(c / nxy);
+  const int rem = (int)(c - (long long)z * nxy);
+  const int y = rem / d.nx, x = rem - y * d.nx;
+  double v[12];
+  pull_cell<T, PLAIN>(d, Lin, c, x, y, z, fds[i], a, v);
+  store_cell(Lout, d.P, c, v);
+}
+
+// Probe DFT: acc(p,k) += v_p(n) exp(-j 2 pi f_k t_n) dt, t_n = (n-1) dt.
+__global__ void k_dft(const double* __restrict__ ring, int n_out_rows, int slot, int timestep,
+                      const int32_t* __restrict__ prow, const int32_t* __restrict__ pcomp, int n_points,
+                      const double* __restrict__ freq, int n_freq, double dt, double* __restrict__ acc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_points * n_freq) return;
+  int p = i / n_freq, k = i - p * n_freq;
+  double v = ring[((long long)slot * n_out_rows + prow[p]) * 6 + pcomp[p]];
+  double time = (double)(timestep - 1) * dt;
+  double ph = 2.0 * kPI * freq[k] * time;
+  double s, co;
+  sincos(ph, &s, &co);
+  acc[2 * i] += v * co * dt;
+  acc[2 * i + 1] -= v * s * dt;
+}
+
+// Fortran V(1:12,0:nx+1,0:ny+1,1:nz) <-> planes.
+template <typename T>
+__global__ void k_aos_to_planes(const double* __restrict__ V, T* __restrict__ L, int nx, int ny, int nz, long long P) {
+  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long N = (long long)nx * ny * nz;
+  if (c >= N) return;
+  int x = (int)(c % nx);
+  long long r = c / nx;
+  int y = (int)(r % ny);
+  int z = (int)(r / ny);
+  long long fi = 12 * ((long long)(x + 1) + (long long)(nx + 2) * ((long long)(y + 1) + (long long)(ny + 2) * z));
+  for (int p = 0; p < 12; p++) L[p * P + c] = (T)V[fi + p];
+}
+
+template <typename T>
+__global__ void k_planes_to_aos(const T* __restrict__ L, double* __restrict__ V, int nx, int ny, int nz, long long P) {
+  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long N = (long long)nx * ny * nz;
+  if (c >= N) return;
+  int x = (int)(c % nx);
+  long long r = c / nx;
+  int y = (int)(r % ny);
+  int z = (int)(r / ny);
+  long long fi = 12 * ((long long)(x + 1) + (long long)(nx + 2) * ((long long)(y + 1) + (long long)(ny + 2) * z));
+  for (int p = 0; p < 12; p++) V[fi + p] = (double)L[p * P + c];
+}
+
+}  // namespace scn

@@ -1,0 +1,853 @@
+// tlmscn.hip -- C ABI of the gfx950 SCN scatter/connect engine
+// (declarations and reference mapping in include/tlmscn.h).
+//
+// Host side: validates the reference tables, derives the device layout
+// (link planes, bulk bitmask, PML and generic class lists, special-face
+// descriptors, excitation CSR tables), owns one HIP stream per handle and
+// drives the step kernels of scn_kernels.hpp.
+//
+// Link-state machine.  The reference keeps one array V that scatter and
+// connect update in place.  Here a buffer holds either incident voltages
+// (after an upload) or the scattered voltages of the last scatter; connect
+// and outer_boundary are recorded as pending and folded into the next
+// scatter's pull.  A download materialises the pending connect first, so
+// the caller always sees exactly the reference's V.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "scn_kernels.hpp"
+
+using scn::Dev;
+using scn::PullArgs;
+
+namespace {
+
+enum class Kind { INCIDENT, SCATTERED };
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct tlm_handle {
+  tlm_grid g{};
+  int form = TLM_FORM_PLAIN_FP64;
+  std::string err;
+  int nx = 0, ny = 0, nz = 0;
+  long long N = 0, P = 0;
+  hipStream_t stream = nullptr;
+
+  // host tables (reference layout)
+  bool cells_set = false, faces_set = false;
+  std::vector<int32_t> cell_code, c_mat, c_filt, c_pml, c_exc, c_out;
+  int n_special = 0;
+  std::vector<int32_t> face_code, f_mat, f_exc, f_out;
+  int n_special_faces = 0;
+  std::vector<tlm_volume_material> vmat;
+  std::vector<int32_t> smat;
+  std::vector<tlm_pml_parameter> pml_par;
+  std::vector<int32_t> pml_cell_param;
+  int n_functions = 0;
+  std::vector<double> value, value_face;
+  std::vector<tlm_point_source> ps;
+  int n_cell_rows = 0, n_face_rows = 0;
+  int huy_function = 0;
+  double huy_offset_min = 0.0, Ei[3] = {0, 0, 0}, Hi[3] = {0, 0, 0};
+  std::vector<tlm_huygens_face> huy;
+  std::vector<int32_t> pt_row, pt_comp;
+  std::vector<double> freq;
+
+  // device
+  bool finalized = false;
+  std::vector<DevBuf> bufs;
+  void* planes[2] = {nullptr, nullptr};
+  int cur = 0;
+  Dev d{};
+  int n_gen = 0, n_pml = 0;
+  const int32_t* d_gen_cell = nullptr;
+  const int32_t* d_gen_fd = nullptr;
+  const int32_t* d_pml_cell = nullptr;
+  const int32_t* d_pml_fd = nullptr;
+  int32_t* d_prow = nullptr;
+  int32_t* d_pcomp = nullptr;
+  double* d_freq = nullptr;
+  double* d_dft = nullptr;
+  int64_t dev_bytes = 0;
+
+  // state machine
+  Kind kind = Kind::INCIDENT;
+  bool pend_connect = false, pend_boundary = false;
+  int pend_ts = 0;
+
+  // output ring
+  int cap = 4096;
+  int ring_used = 0;         // slots holding unread steps on the device
+  int ring_first_step = 0;   // timestep of slot 0
+  std::vector<double> host_out;  // drained rows awaiting read
+  int host_first = 0, host_steps = 0;
+};
+
+namespace {
+
+int fail(tlm_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+int hip_fail(tlm_handle* h, hipError_t e, const char* what) {
+  return fail(h, TLM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(h, expr)                                   \
+  do {                                                     \
+    hipError_t e_ = (expr);                                \
+    if (e_ != hipSuccess) return hip_fail((h), e_, #expr); \
+  } while (0)
+
+template <typename V>
+int dev_upload(tlm_handle* h, const std::vector<V>& v, const V** out) {
+  size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(V);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) return fail(h, TLM_ERR_NOMEM, "hipMalloc failed for a table");
+  h->bufs.push_back({p, bytes});
+  h->dev_bytes += (int64_t)bytes;
+  if (!v.empty()) HIP_TRY(h, hipMemcpy(p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
+  *out = (const V*)p;
+  return TLM_OK;
+}
+
+int dev_alloc(tlm_handle* h, size_t bytes, void** out) {
+  void* p = nullptr;
+  bytes = std::max<size_t>(bytes, 8);
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) return fail(h, TLM_ERR_NOMEM, "hipMalloc failed (" + std::to_string(bytes) + " bytes)");
+  h->bufs.push_back({p, bytes});
+  h->dev_bytes += (int64_t)bytes;
+  *out = p;
+  return TLM_OK;
+}
+
+size_t link_size(const tlm_handle* h) { return h->form == TLM_FORM_PLAIN_FP64 ? sizeof(double) : sizeof(float); }
+
+// Build every device table from the reference tables.
+int finalize(tlm_handle* h) {
+  if (h->finalized) return TLM_OK;
+  if (!h->cells_set) return fail(h, TLM_ERR_STATE, "tlm_set_cell_codes has not been called");
+  if (!h->faces_set) return fail(h, TLM_ERR_STATE, "tlm_set_face_codes has not been called");
+  const int nx = h->nx, ny = h->ny, nz = h->nz;
+  const long long N = h->N;
+  int rc;
+
+  // ---- special faces: locate each special face code in scan order -------
+  // face_info[code] = (cell index of the face's min-side cell cx, axis)
+  std::vector<long long> sf_cell(h->n_special_faces + 1, -1);
+  std::vector<int> sf_axis(h->n_special_faces + 1, -1);
+  if (h->n_special_faces > 0) {
+    long long fnum = 0;
+    int found = 0;
+    for (int cz = 0; cz < nz && found < h->n_special_faces; cz++)
+      for (int cy = 0; cy < ny && found < h->n_special_faces; cy++)
+        for (int cx = 0; cx < nx; cx++) {
+          long long c = cx + (long long)nx * (cy + (long long)ny * cz);
+          for (int ax = 0; ax < 3; ax++) {
+            bool has = (ax == 0 && cx != 0) || (ax == 1 && cy != 0) || (ax == 2 && cz != 0);
+            if (!has) continue;
+            int code = h->face_code[fnum++];
+            if (code != 0) {
+              if (code < 1 || code > h->n_special_faces) return fail(h, TLM_ERR_ARG, "face code out of range");
+              sf_cell[code] = c;
+              sf_axis[code] = ax;
+              found++;
+            }
+          }
+        }
+  }
+  // surface material type, excitation and output rows per special face
+  std::vector<int32_t> sf_mtype(h->n_special_faces + 1, 0), sf_exc(h->n_special_faces + 1, -1);
+  {
+    int er = 0;
+    // excitation rows follow the face scan order = code order
+    for (int code = 1; code <= h->n_special_faces; code++) {
+      int m = h->f_mat[code - 1];
+      int mn = m < 0 ? -m : m;
+      if (mn) {
+        if (mn > (int)h->smat.size()) return fail(h, TLM_ERR_ARG, "surface material number out of range");
+        sf_mtype[code] = h->smat[mn - 1];
+      }
+      if (h->f_exc[code - 1] != 0) sf_exc[code] = er++;
+    }
+    if (er != h->n_face_rows) return fail(h, TLM_ERR_ARG, "face excitation rows do not match the face codes");
+  }
+  // descriptors: per special cell, the special-face codes of its 6 faces
+  // (xmin, xmax, ymin, ymax, zmin, zmax).  Key: cell index.
+  struct FdEntry { long long cell; int side; int code; };
+  std::vector<FdEntry> fe;
+  fe.reserve(2 * (size_t)h->n_special_faces);
+  for (int code = 1; code <= h->n_special_faces; code++) {
+    long long c = sf_cell[code];
+    if (c < 0) return fail(h, TLM_ERR_ARG, "special face code not present in face_update_code");
+    int ax = sf_axis[code];
+    long long off = ax == 0 ? 1 : (ax == 1 ? nx : (long long)nx * ny);
+    fe.push_back({c, 2 * ax, code});            // min face of cell c
+    fe.push_back({c - off, 2 * ax + 1, code});  // max face of the cell below
+  }
+  std::sort(fe.begin(), fe.end(), [](const FdEntry& a, const FdEntry& b) { return a.cell < b.cell; });
+  std::vector<int32_t> fd;         // 6 per descriptor
+  std::vector<long long> fd_cell;  // sorted cells with descriptors
+  for (size_t i = 0; i < fe.size();) {
+    size_t j = i;
+    int32_t codes[6] = {0, 0, 0, 0, 0, 0};
+    while (j < fe.size() && fe[j].cell == fe[i].cell) { codes[fe[j].side] = fe[j].code; j++; }
+    fd_cell.push_back(fe[i].cell);
+    for (int k = 0; k < 6; k++) fd.push_back(codes[k]);
+    i = j;
+  }
+  auto fd_of = [&](long long c) -> int32_t {
+    auto it = std::lower_bound(fd_cell.begin(), fd_cell.end(), c);
+    if (it != fd_cell.end() && *it == c) return (int32_t)(it - fd_cell.begin());
+    return -1;
+  };
+
+  // ---- cell classes ------------------------------------------------------
+  std::vector<uint32_t> mask((N + 31) / 32, 0u);
+  std::vector<int32_t> pml_cell, pml_sidx, pml_par, pml_out, pml_fd;
+  std::vector<int32_t> gen_cell, gen_code, gen_mat, gen_filt, gen_exc, gen_out, gen_fd;
+  long long max_filt = 0;
+  int out_rank = 0, exc_rank = 0;
+  size_t fdi = 0;  // walks fd_cell in scan order
+  int n_pml_state = (int)h->pml_cell_param.size();
+  for (long long c = 0; c < N; c++) {
+    int code = h->cell_code[c];
+    bool has_fd = fdi < fd_cell.size() && fd_cell[fdi] == c;
+    int32_t fdc = has_fd ? (int32_t)fdi : -1;
+    if (has_fd) fdi++;
+    if (code == 0 && !has_fd) continue;
+    mask[c >> 5] |= 1u << (c & 31);
+    if (code < 0 || code > h->n_special) return fail(h, TLM_ERR_ARG, "cell code out of range");
+    int s = code - 1;
+    int pmlc = code ? h->c_pml[s] : 0;
+    int orow = (code && h->c_out[s]) ? out_rank++ : -1;
+    if (pmlc != 0) {
+      if (pmlc < 1 || pmlc > n_pml_state) return fail(h, TLM_ERR_ARG, "PML cell number out of range");
+      int m = h->c_mat[s];
+      if (m) {
+        if (m > (int)h->vmat.size()) return fail(h, TLM_ERR_ARG, "material number out of range");
+        int t = h->vmat[m - 1].type;
+        if (t == TLM_VMAT_PEC) return fail(h, TLM_ERR_ARG, "ERROR: PEC volume material in PML");
+        if (t == TLM_VMAT_PMC) return fail(h, TLM_ERR_ARG, "ERROR: PMC volume material in PML");
+        return fail(h, TLM_ERR_UNSUPPORTED, "material inside the PML is not supported on the device path");
+      }
+      if (h->c_exc[s]) return fail(h, TLM_ERR_ARG, "Excitation intersects PML");
+      int par = h->pml_cell_param[pmlc - 1];
+      if (par < 1 || par > (int)h->pml_par.size()) return fail(h, TLM_ERR_ARG, "PML parameter index out of range");
+      pml_cell.push_back((int32_t)c);
+      pml_sidx.push_back(pmlc - 1);
+      pml_par.push_back(par - 1);
+      pml_out.push_back(orow);
+      pml_fd.push_back(fdc);
+    } else {
+      int m = code ? h->c_mat[s] : 0;
+      int f = code ? h->c_filt[s] : 0;
+      if (m) {
+        if (m > (int)h->vmat.size()) return fail(h, TLM_ERR_ARG, "material number out of range");
+        if (h->vmat[m - 1].type == TLM_VMAT_DISPERSIVE) {
+          if (f < 1) return fail(h, TLM_ERR_ARG, "dispersive cell without filter number");
+          max_filt = std::max<long long>(max_filt, (long long)f + 3);
+        }
+      }
+      int er = (code && h->c_exc[s]) ? exc_rank++ : -1;
+      gen_cell.push_back((int32_t)c);
+      gen_code.push_back(code);
+      gen_mat.push_back(m);
+      gen_filt.push_back(f);
+      gen_exc.push_back(er);
+      gen_out.push_back(orow);
+      gen_fd.push_back(fdc);
+    }
+  }
+  if (N > 0x7fffffffLL) return fail(h, TLM_ERR_UNSUPPORTED, "more than 2^31 cells per handle");
+  if (exc_rank != h->n_cell_rows)
+    return fail(h, TLM_ERR_ARG, "cell excitation rows do not match the cell codes");
+  const int n_out_rows = out_rank;
+
+  // ---- excitation CSR ---------------------------------------------------
+  std::vector<int32_t> cex_start(h->n_cell_rows + 1, 0), cex_src, fex_start(h->n_face_rows + 1, 0), fex_src;
+  std::vector<int32_t> cex_type(6 * (size_t)std::max(h->n_cell_rows, 1), 1), fex_type(6 * (size_t)std::max(h->n_face_rows, 1), 1);
+  {
+    std::vector<std::vector<int32_t>> cl(h->n_cell_rows), fl(h->n_face_rows);
+    for (size_t i = 0; i < h->ps.size(); i++) {
+      const tlm_point_source& p = h->ps[i];
+      if (p.at_face) {
+        fl[p.row - 1].push_back((int32_t)i);
+        if (p.component <= 6) fex_type[6 * (p.row - 1) + p.component - 1] = p.source_type;
+      } else {
+        cl[p.row - 1].push_back((int32_t)i);
+        if (p.component <= 6) cex_type[6 * (p.row - 1) + p.component - 1] = p.source_type;
+      }
+    }
+    for (int r = 0; r < h->n_cell_rows; r++) {
+      cex_start[r + 1] = cex_start[r] + (int)cl[r].size();
+      cex_src.insert(cex_src.end(), cl[r].begin(), cl[r].end());
+    }
+    for (int r = 0; r < h->n_face_rows; r++) {
+      fex_start[r + 1] = fex_start[r] + (int)fl[r].size();
+      fex_src.insert(fex_src.end(), fl[r].begin(), fl[r].end());
+    }
+  }
+  std::vector<int32_t> fhy_start(h->n_face_rows + 1, 0), fhy_face;
+  {
+    std::vector<std::vector<int32_t>> hl(h->n_face_rows);
+    for (size_t i = 0; i < h->huy.size(); i++)
+      if (h->huy[i].row > 0) hl[h->huy[i].row - 1].push_back((int32_t)i);
+    for (int r = 0; r < h->n_face_rows; r++) {
+      fhy_start[r + 1] = fhy_start[r] + (int)hl[r].size();
+      fhy_face.insert(fhy_face.end(), hl[r].begin(), hl[r].end());
+    }
+  }
+
+  // ---- device allocations -------------------------------------------------
+  Dev& d = h->d;
+  d.nx = nx; d.ny = ny; d.nz = nz; d.N = N; d.P = h->P;
+  d.dl = h->g.dl; d.dt = h->g.dt; d.c_red = scn::kC0 / h->g.reduced_c_factor;
+  d.n_timesteps = h->g.n_timesteps;
+  d.R[0] = h->g.r_xmin; d.R[1] = h->g.r_xmax; d.R[2] = h->g.r_ymin;
+  d.R[3] = h->g.r_ymax; d.R[4] = h->g.r_zmin; d.R[5] = h->g.r_zmax;
+  const size_t ls = link_size(h);
+  for (int b = 0; b < 2; b++) {
+    if ((rc = dev_alloc(h, 12 * (size_t)h->P * ls, &h->planes[b]))) return rc;
+    HIP_TRY(h, hipMemsetAsync(h->planes[b], 0, 12 * (size_t)h->P * ls, h->stream));
+  }
+  if ((rc = dev_upload(h, mask, &d.mask))) return rc;
+  h->n_pml = d.n_pml = (int)pml_cell.size();
+  if ((rc = dev_upload(h, pml_cell, &d.pml_cell))) return rc;
+  if ((rc = dev_upload(h, pml_sidx, &d.pml_state_idx))) return rc;
+  if ((rc = dev_upload(h, pml_par, &d.pml_par))) return rc;
+  if ((rc = dev_upload(h, pml_out, &d.pml_out))) return rc;
+  if ((rc = dev_upload(h, pml_fd, &d.pml_fd))) return rc;
+  h->d_pml_cell = d.pml_cell;
+  h->d_pml_fd = d.pml_fd;
+  if ((rc = dev_upload(h, h->pml_par, &d.pml_params))) return rc;
+  d.n_pml_state = std::max(n_pml_state, 1);
+  {
+    void* p;
+    if ((rc = dev_alloc(h, (size_t)scn::PS_N * d.n_pml_state * sizeof(double), &p))) return rc;
+    HIP_TRY(h, hipMemsetAsync(p, 0, (size_t)scn::PS_N * d.n_pml_state * sizeof(double), h->stream));
+    d.pml_state = (double*)p;
+    if ((rc = dev_alloc(h, (size_t)3 * d.n_pml_state * sizeof(double), &p))) return rc;
+    HIP_TRY(h, hipMemsetAsync(p, 0, (size_t)3 * d.n_pml_state * sizeof(double), h->stream));
+    d.pml_It = (double*)p;
+  }
+  h->n_gen = d.n_gen = (int)gen_cell.size();
+  if ((rc = dev_upload(h, gen_cell, &d.gen_cell))) return rc;
+  if ((rc = dev_upload(h, gen_code, &d.gen_code))) return rc;
+  if ((rc = dev_upload(h, gen_mat, &d.gen_mat))) return rc;
+  if ((rc = dev_upload(h, gen_filt, &d.gen_filt))) return rc;
+  if ((rc = dev_upload(h, gen_exc, &d.gen_exc))) return rc;
+  if ((rc = dev_upload(h, gen_out, &d.gen_out))) return rc;
+  if ((rc = dev_upload(h, gen_fd, &d.gen_fd))) return rc;
+  h->d_gen_cell = d.gen_cell;
+  h->d_gen_fd = d.gen_fd;
+  if ((rc = dev_upload(h, h->vmat, &d.vmat))) return rc;
+  d.n_filt = std::max<long long>(max_filt, 1);
+  {
+    void* p;
+    size_t fb = (size_t)scn::NW * d.n_filt * sizeof(double);
+    if ((rc = dev_alloc(h, fb, &p))) return rc;
+    HIP_TRY(h, hipMemsetAsync(p, 0, fb, h->stream));
+    d.eps_w = (double*)p;
+    if ((rc = dev_alloc(h, fb, &p))) return rc;
+    HIP_TRY(h, hipMemsetAsync(p, 0, fb, h->stream));
+    d.mu_w = (double*)p;
+  }
+  if ((rc = dev_upload(h, fd, &d.fd))) return rc;
+  if ((rc = dev_upload(h, sf_mtype, &d.sf_mtype))) return rc;
+  if ((rc = dev_upload(h, sf_exc, &d.sf_exc))) return rc;
+  if ((rc = dev_upload(h, h->value, &d.value))) return rc;
+  if ((rc = dev_upload(h, h->value_face, &d.value_face))) return rc;
+  if ((rc = dev_upload(h, cex_start, &d.cex_start))) return rc;
+  if ((rc = dev_upload(h, cex_src, &d.cex_src))) return rc;
+  if ((rc = dev_upload(h, cex_type, &d.cex_type))) return rc;
+  if ((rc = dev_upload(h, fex_start, &d.fex_start))) return rc;
+  if ((rc = dev_upload(h, fex_src, &d.fex_src))) return rc;
+  if ((rc = dev_upload(h, fex_type, &d.fex_type))) return rc;
+  if ((rc = dev_upload(h, fhy_start, &d.fhy_start))) return rc;
+  if ((rc = dev_upload(h, fhy_face, &d.fhy_face))) return rc;
+  if ((rc = dev_upload(h, h->ps, &d.ps))) return rc;
+  if ((rc = dev_upload(h, h->huy, &d.hf))) return rc;
+  d.huy_function = h->huy_function;
+  d.huy_offset_min = h->huy_offset_min;
+  for (int k = 0; k < 3; k++) { d.Ei[k] = h->Ei[k]; d.Hi[k] = h->Hi[k]; }
+  d.n_out_rows = n_out_rows;
+  {
+    void* p;
+    if ((rc = dev_alloc(h, (size_t)h->cap * std::max(n_out_rows, 1) * 6 * sizeof(double), &p))) return rc;
+    d.ring = (double*)p;
+  }
+  // probes and DFT
+  for (size_t i = 0; i < h->pt_row.size(); i++)
+    if (h->pt_row[i] < 1 || h->pt_row[i] > n_out_rows || h->pt_comp[i] < 1 || h->pt_comp[i] > 6)
+      return fail(h, TLM_ERR_ARG, "output point row/component out of range");
+  {
+    std::vector<int32_t> r0(h->pt_row.size()), c0(h->pt_comp.size());
+    for (size_t i = 0; i < r0.size(); i++) { r0[i] = h->pt_row[i] - 1; c0[i] = h->pt_comp[i] - 1; }
+    const int32_t* pr;
+    if ((rc = dev_upload(h, r0, &pr))) return rc;
+    h->d_prow = (int32_t*)pr;
+    if ((rc = dev_upload(h, c0, &pr))) return rc;
+    h->d_pcomp = (int32_t*)pr;
+    const double* fq;
+    if ((rc = dev_upload(h, h->freq, &fq))) return rc;
+    h->d_freq = (double*)fq;
+    void* p;
+    size_t nb = std::max<size_t>(1, h->pt_row.size() * h->freq.size()) * 2 * sizeof(double);
+    if ((rc = dev_alloc(h, nb, &p))) return rc;
+    HIP_TRY(h, hipMemsetAsync(p, 0, nb, h->stream));
+    h->d_dft = (double*)p;
+  }
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->finalized = true;
+  return TLM_OK;
+}
+
+// Move filled ring slots to the host staging vector.
+int drain_ring(tlm_handle* h) {
+  if (h->ring_used == 0) return TLM_OK;
+  const int rows = h->d.n_out_rows;
+  size_t n = (size_t)h->ring_used * rows * 6;
+  size_t old = h->host_out.size();
+  if (h->host_steps == 0) h->host_first = h->ring_first_step;
+  h->host_out.resize(old + n);
+  HIP_TRY(h, hipMemcpyAsync(h->host_out.data() + old, h->d.ring, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->host_steps += h->ring_used;
+  h->ring_used = 0;
+  return TLM_OK;
+}
+
+template <typename T, bool PLAIN>
+int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, int slot) {
+  const Dev& d = h->d;
+  const T* Lin = (const T*)h->planes[h->cur];
+  T* Lout = (T*)h->planes[h->cur ^ 1];
+  dim3 bb(64, 4, 1);
+  dim3 gb((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
+  if (pull) hipLaunchKernelGGL((scn::k_bulk<T, PLAIN, true>), gb, bb, 0, h->stream, d, Lin, Lout, a);
+  else hipLaunchKernelGGL((scn::k_bulk<T, PLAIN, false>), gb, bb, 0, h->stream, d, Lin, Lout, a);
+  if (d.n_pml) {
+    dim3 g((d.n_pml + 255) / 256);
+    if (pull) hipLaunchKernelGGL((scn::k_pml<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot);
+    else hipLaunchKernelGGL((scn::k_pml<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot);
+  }
+  if (d.n_gen) {
+    dim3 g((d.n_gen + 255) / 256);
+    if (pull) hipLaunchKernelGGL((scn::k_generic<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, timestep, slot);
+    else hipLaunchKernelGGL((scn::k_generic<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, timestep, slot);
+  }
+  return TLM_OK;
+}
+
+template <typename T, bool PLAIN>
+int launch_connect(tlm_handle* h, const PullArgs& a) {
+  const Dev& d = h->d;
+  const T* Lin = (const T*)h->planes[h->cur];
+  T* Lout = (T*)h->planes[h->cur ^ 1];
+  dim3 bb(64, 4, 1);
+  dim3 gb((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
+  hipLaunchKernelGGL((scn::k_connect_bulk<T, PLAIN>), gb, bb, 0, h->stream, d, Lin, Lout, a);
+  if (h->n_pml)
+    hipLaunchKernelGGL((scn::k_connect_list<T, PLAIN>), dim3((h->n_pml + 255) / 256), dim3(256), 0, h->stream, d, Lin,
+                       Lout, a, h->d_pml_cell, h->d_pml_fd, h->n_pml);
+  if (h->n_gen)
+    hipLaunchKernelGGL((scn::k_connect_list<T, PLAIN>), dim3((h->n_gen + 255) / 256), dim3(256), 0, h->stream, d, Lin,
+                       Lout, a, h->d_gen_cell, h->d_gen_fd, h->n_gen);
+  return TLM_OK;
+}
+
+// One scatter at `timestep`, consuming any pending connect.
+int do_scatter(tlm_handle* h, int timestep) {
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  if (timestep < 1 || timestep > h->g.n_timesteps) return fail(h, TLM_ERR_ARG, "timestep out of range");
+  bool pull;
+  if (h->kind == Kind::INCIDENT) {
+    pull = false;
+  } else {
+    if (!h->pend_connect) return fail(h, TLM_ERR_STATE, "scatter called twice without connect");
+    pull = true;
+  }
+  PullArgs a{h->pend_boundary, h->pend_ts};
+  if (h->d.n_out_rows > 0 && h->ring_used == h->cap)
+    if ((rc = drain_ring(h))) return rc;
+  int slot = h->ring_used;
+  if (h->form == TLM_FORM_PLAIN_FP64) rc = launch_scatter<double, true>(h, timestep, pull, a, slot);
+  else rc = launch_scatter<float, false>(h, timestep, pull, a, slot);
+  if (rc) return rc;
+  HIP_TRY(h, hipGetLastError());
+  if (h->d.n_out_rows > 0) {
+    if (h->ring_used == 0) h->ring_first_step = timestep;
+    h->ring_used++;
+    int np = (int)h->pt_row.size(), nf = (int)h->freq.size();
+    if (np && nf)
+      hipLaunchKernelGGL(scn::k_dft, dim3((np * nf + 255) / 256), dim3(256), 0, h->stream, h->d.ring, h->d.n_out_rows,
+                         slot, timestep, h->d_prow, h->d_pcomp, np, h->d_freq, nf, h->g.dt, h->d_dft);
+  }
+  h->cur ^= 1;
+  h->kind = Kind::SCATTERED;
+  h->pend_connect = false;
+  h->pend_boundary = false;
+  return TLM_OK;
+}
+
+// Make the buffer hold incident voltages (apply any pending connect).
+int materialize(tlm_handle* h) {
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  if (h->kind == Kind::INCIDENT) return TLM_OK;
+  if (!h->pend_connect) return TLM_OK;  // scattered state is what V holds
+  PullArgs a{h->pend_boundary, h->pend_ts};
+  if (h->form == TLM_FORM_PLAIN_FP64) rc = launch_connect<double, true>(h, a);
+  else rc = launch_connect<float, false>(h, a);
+  if (rc) return rc;
+  HIP_TRY(h, hipGetLastError());
+  h->cur ^= 1;
+  h->kind = Kind::INCIDENT;
+  h->pend_connect = false;
+  h->pend_boundary = false;
+  return TLM_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" {
+
+int tlm_create(const tlm_grid* g, int32_t form, tlm_handle** out) {
+  if (!g || !out) return TLM_ERR_ARG;
+  *out = nullptr;
+  if (g->nx < 1 || g->ny < 1 || g->nz < 1 || g->n_timesteps < 1 || !(g->dl > 0) || !(g->dt > 0))
+    return TLM_ERR_ARG;
+  if (form != TLM_FORM_PLAIN_FP64 && form != TLM_FORM_PRECISION_FP32) return TLM_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return TLM_ERR_HIP;
+  tlm_handle* h = new (std::nothrow) tlm_handle();
+  if (!h) return TLM_ERR_NOMEM;
+  h->g = *g;
+  h->form = form;
+  h->nx = g->nx; h->ny = g->ny; h->nz = g->nz;
+  h->N = (long long)g->nx * g->ny * g->nz;
+  h->P = (h->N + 63) / 64 * 64;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return TLM_ERR_HIP;
+  }
+  *out = h;
+  return TLM_OK;
+}
+
+void tlm_destroy(tlm_handle* h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto& b : h->bufs) hipFree(b.p);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+const char* tlm_last_error(const tlm_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int64_t tlm_device_bytes(const tlm_handle* h) { return h ? h->dev_bytes : 0; }
+
+int tlm_set_cell_codes(tlm_handle* h, const int32_t* code, int32_t n_special, const int32_t* to_material,
+                       const int32_t* to_pml, const int32_t* to_excitation, const int32_t* to_output,
+                       const int32_t* to_cable) {
+  if (!h) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  if (!code || n_special < 0) return fail(h, TLM_ERR_ARG, "bad cell code table");
+  h->cell_code.assign(code, code + h->N);
+  h->n_special = n_special;
+  auto cp = [&](const int32_t* src, std::vector<int32_t>& dst) {
+    if (src) dst.assign(src, src + n_special); else dst.assign(n_special, 0);
+  };
+  cp(to_material, h->c_mat);
+  if (to_material) h->c_filt.assign(to_material + n_special, to_material + 2 * (size_t)n_special);
+  else h->c_filt.assign(n_special, 0);
+  cp(to_pml, h->c_pml);
+  cp(to_excitation, h->c_exc);
+  cp(to_output, h->c_out);
+  if (to_cable)
+    for (int i = 0; i < n_special; i++)
+      if (to_cable[i]) return fail(h, TLM_ERR_UNSUPPORTED, "cable cells stay on the host path");
+  h->cells_set = true;
+  return TLM_OK;
+}
+
+int tlm_set_face_codes(tlm_handle* h, const int32_t* code, int32_t n_special, const int32_t* to_material,
+                       const int32_t* to_excitation, const int32_t* to_output, const int32_t* to_cable) {
+  if (!h) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  long long nf = (long long)(h->nx - 1) * h->ny * h->nz + (long long)h->nx * (h->ny - 1) * h->nz +
+                 (long long)h->nx * h->ny * (h->nz - 1);
+  if (n_special < 0 || (nf > 0 && !code)) return fail(h, TLM_ERR_ARG, "bad face code table");
+  if (code) h->face_code.assign(code, code + nf); else h->face_code.assign(nf, 0);
+  h->n_special_faces = n_special;
+  auto cp = [&](const int32_t* src, std::vector<int32_t>& dst) {
+    if (src) dst.assign(src, src + n_special); else dst.assign(n_special, 0);
+  };
+  cp(to_material, h->f_mat);
+  cp(to_excitation, h->f_exc);
+  cp(to_output, h->f_out);
+  if (to_cable)
+    for (int i = 0; i < n_special; i++)
+      if (to_cable[i]) return fail(h, TLM_ERR_UNSUPPORTED, "cable faces stay on the host path");
+  h->faces_set = true;
+  return TLM_OK;
+}
+
+int tlm_set_volume_materials(tlm_handle* h, int32_t n, const tlm_volume_material* m) {
+  if (!h || n < 0 || (n > 0 && !m)) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  for (int i = 0; i < n; i++) {
+    if (m[i].type < 1 || m[i].type > 3) return fail(h, TLM_ERR_ARG, "unknown volume material type");
+    if (m[i].eps_order < 0 || m[i].eps_order > TLM_MAX_FILTER_ORDER || m[i].mu_order < 0 ||
+        m[i].mu_order > TLM_MAX_FILTER_ORDER)
+      return fail(h, TLM_ERR_ARG, "filter order out of range");
+    if ((m[i].eps_filter_exists && m[i].eps_b[0] == 0.0) || (m[i].mu_filter_exists && m[i].mu_b[0] == 0.0))
+      return fail(h, TLM_ERR_ARG, "filter b0 is zero");
+  }
+  h->vmat.assign(m, m + n);
+  return TLM_OK;
+}
+
+int tlm_set_surface_materials(tlm_handle* h, int32_t n, const int32_t* type) {
+  if (!h || n < 0 || (n > 0 && !type)) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  for (int i = 0; i < n; i++)
+    if (type[i] != TLM_SMAT_PEC && type[i] != TLM_SMAT_PMC && type[i] != TLM_SMAT_FREE_SPACE)
+      return fail(h, TLM_ERR_UNSUPPORTED, "surface material type stays on the host path");
+  h->smat.assign(type, type + n);
+  return TLM_OK;
+}
+
+int tlm_set_pml(tlm_handle* h, int32_t n_params, const tlm_pml_parameter* p, int32_t n_cells,
+                const int32_t* cell_param) {
+  if (!h || n_params < 0 || n_cells < 0 || (n_params && !p) || (n_cells && !cell_param)) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  h->pml_par.assign(p, p + n_params);
+  h->pml_cell_param.assign(cell_param, cell_param + n_cells);
+  return TLM_OK;
+}
+
+int tlm_set_excitation_functions(tlm_handle* h, int32_t n_functions, const double* value, const double* value_face) {
+  if (!h || n_functions < 0 || (n_functions && (!value || !value_face))) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  size_t n = (size_t)n_functions * h->g.n_timesteps;
+  h->n_functions = n_functions;
+  h->value.assign(value, value + n);
+  h->value_face.assign(value_face, value_face + n);
+  return TLM_OK;
+}
+
+int tlm_set_point_sources(tlm_handle* h, int32_t n, const tlm_point_source* s, int32_t n_cell_rows,
+                          int32_t n_face_rows) {
+  if (!h || n < 0 || (n && !s) || n_cell_rows < 0 || n_face_rows < 0) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  for (int i = 0; i < n; i++) {
+    if (s[i].function < 1 || s[i].function > h->n_functions)
+      return fail(h, TLM_ERR_ARG, "point source function out of range");
+    if (s[i].component < 1 || s[i].component > (s[i].at_face ? 12 : 9))
+      return fail(h, TLM_ERR_ARG, "point source component out of range");
+    if (s[i].source_type != 0 && s[i].source_type != 1) return fail(h, TLM_ERR_ARG, "bad source type");
+    int rows = s[i].at_face ? n_face_rows : n_cell_rows;
+    if (s[i].row < 1 || s[i].row > rows) return fail(h, TLM_ERR_ARG, "point source row out of range");
+  }
+  h->ps.assign(s, s + n);
+  h->n_cell_rows = n_cell_rows;
+  h->n_face_rows = n_face_rows;
+  return TLM_OK;
+}
+
+int tlm_set_huygens_surface(tlm_handle* h, int32_t function, double offset_min, const double* Ei, const double* Hi,
+                            int32_t n_faces, const tlm_huygens_face* faces) {
+  if (!h || n_faces < 0 || (n_faces && (!faces || !Ei || !Hi))) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  if (n_faces && (function < 1 || function > h->n_functions)) return fail(h, TLM_ERR_ARG, "Huygens function out of range");
+  for (int i = 0; i < n_faces; i++) {
+    if (faces[i].row < 0 || faces[i].row > h->n_face_rows) return fail(h, TLM_ERR_ARG, "Huygens face row out of range");
+    int s = abs(faces[i].nx) + abs(faces[i].ny) + abs(faces[i].nz);
+    if (s != 1) return fail(h, TLM_ERR_ARG, "Huygens face normal must be a unit axis vector");
+  }
+  h->huy_function = function;
+  h->huy_offset_min = offset_min;
+  for (int k = 0; k < 3; k++) { h->Ei[k] = Ei ? Ei[k] : 0.0; h->Hi[k] = Hi ? Hi[k] : 0.0; }
+  h->huy.assign(faces, faces + n_faces);
+  return TLM_OK;
+}
+
+int tlm_set_output_points(tlm_handle* h, int32_t n, const int32_t* row, const int32_t* component) {
+  if (!h || n < 0 || (n && (!row || !component))) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  h->pt_row.assign(row, row + n);
+  h->pt_comp.assign(component, component + n);
+  return TLM_OK;
+}
+
+int tlm_set_dft_frequencies(tlm_handle* h, int32_t n, const double* f) {
+  if (!h || n < 0 || (n && !f)) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  h->freq.assign(f, f + n);
+  return TLM_OK;
+}
+
+int tlm_set_output_capacity(tlm_handle* h, int32_t steps) {
+  if (!h) return TLM_ERR_ARG;
+  if (steps < 1) return fail(h, TLM_ERR_ARG, "capacity must be positive");
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "capacity is fixed after the first step");
+  h->cap = steps;
+  return TLM_OK;
+}
+
+int tlm_upload_links(tlm_handle* h, const double* V) {
+  if (!h || !V) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  size_t n = (size_t)12 * (h->nx + 2) * (h->ny + 2) * h->nz;
+  void* tmp = nullptr;
+  HIP_TRY(h, hipMalloc(&tmp, n * sizeof(double)));
+  hipError_t e = hipMemcpyAsync(tmp, V, n * sizeof(double), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) {
+    unsigned g = (unsigned)((h->N + 255) / 256);
+    if (h->form == TLM_FORM_PLAIN_FP64)
+      hipLaunchKernelGGL(scn::k_aos_to_planes<double>, dim3(g), dim3(256), 0, h->stream, (const double*)tmp,
+                         (double*)h->planes[h->cur], h->nx, h->ny, h->nz, h->P);
+    else
+      hipLaunchKernelGGL(scn::k_aos_to_planes<float>, dim3(g), dim3(256), 0, h->stream, (const double*)tmp,
+                         (float*)h->planes[h->cur], h->nx, h->ny, h->nz, h->P);
+    e = hipStreamSynchronize(h->stream);
+  }
+  hipFree(tmp);
+  if (e != hipSuccess) return hip_fail(h, e, "upload_links");
+  h->kind = Kind::INCIDENT;
+  h->pend_connect = h->pend_boundary = false;
+  return TLM_OK;
+}
+
+int tlm_download_links(tlm_handle* h, double* V) {
+  if (!h || !V) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = materialize(h))) return rc;
+  size_t n = (size_t)12 * (h->nx + 2) * (h->ny + 2) * h->nz;
+  void* tmp = nullptr;
+  HIP_TRY(h, hipMalloc(&tmp, n * sizeof(double)));
+  // keep the caller's halo: copy it in first, overwrite the interior
+  hipError_t e = hipMemcpyAsync(tmp, V, n * sizeof(double), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) {
+    unsigned g = (unsigned)((h->N + 255) / 256);
+    if (h->form == TLM_FORM_PLAIN_FP64)
+      hipLaunchKernelGGL(scn::k_planes_to_aos<double>, dim3(g), dim3(256), 0, h->stream,
+                         (const double*)h->planes[h->cur], (double*)tmp, h->nx, h->ny, h->nz, h->P);
+    else
+      hipLaunchKernelGGL(scn::k_planes_to_aos<float>, dim3(g), dim3(256), 0, h->stream,
+                         (const float*)h->planes[h->cur], (double*)tmp, h->nx, h->ny, h->nz, h->P);
+    e = hipMemcpyAsync(V, tmp, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipFree(tmp);
+  if (e != hipSuccess) return hip_fail(h, e, "download_links");
+  return TLM_OK;
+}
+
+int tlm_excitation(tlm_handle* h, int32_t timestep) {
+  if (!h) return TLM_ERR_ARG;
+  if (timestep < 1 || timestep > h->g.n_timesteps) return fail(h, TLM_ERR_ARG, "timestep out of range");
+  // Excitation values are evaluated inside the scatter and connect kernels
+  // from the function tables at their own timestep.
+  return finalize(h);
+}
+
+int tlm_scatter(tlm_handle* h, int32_t timestep) {
+  if (!h) return TLM_ERR_ARG;
+  return do_scatter(h, timestep);
+}
+
+int tlm_connect(tlm_handle* h, int32_t timestep) {
+  if (!h) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  if (h->kind != Kind::SCATTERED || h->pend_connect)
+    return fail(h, TLM_ERR_STATE, "connect must follow a scatter");
+  if (timestep < 1 || timestep > h->g.n_timesteps) return fail(h, TLM_ERR_ARG, "timestep out of range");
+  h->pend_connect = true;
+  h->pend_ts = timestep;
+  return TLM_OK;
+}
+
+int tlm_outer_boundary(tlm_handle* h) {
+  if (!h) return TLM_ERR_ARG;
+  if (!h->pend_connect || h->pend_boundary)
+    return fail(h, TLM_ERR_STATE, "outer_boundary must follow a connect");
+  h->pend_boundary = true;
+  return TLM_OK;
+}
+
+int tlm_step(tlm_handle* h, int32_t first_timestep, int32_t n_steps) {
+  if (!h) return TLM_ERR_ARG;
+  if (n_steps < 0 || first_timestep < 1 || first_timestep + n_steps - 1 > h->g.n_timesteps)
+    return fail(h, TLM_ERR_ARG, "timestep range out of range");
+  int rc;
+  for (int t = first_timestep; t < first_timestep + n_steps; t++) {
+    if ((rc = do_scatter(h, t))) return rc;
+    h->pend_connect = true;
+    h->pend_ts = t;
+    h->pend_boundary = true;
+  }
+  return TLM_OK;
+}
+
+int tlm_synchronize(tlm_handle* h) {
+  if (!h) return TLM_ERR_ARG;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return TLM_OK;
+}
+
+int tlm_read_cell_outputs(tlm_handle* h, double* out, int32_t max_steps, int32_t* n_read, int32_t* first_step) {
+  if (!h || max_steps < 0 || (max_steps && !out)) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  if ((rc = drain_ring(h))) return rc;
+  const int rows = h->d.n_out_rows;
+  int n = std::min(h->host_steps, (int)max_steps);
+  if (n > 0 && rows > 0) std::memcpy(out, h->host_out.data(), (size_t)n * rows * 6 * sizeof(double));
+  if (n_read) *n_read = n;
+  if (first_step) *first_step = h->host_first;
+  if (n > 0) {
+    h->host_out.erase(h->host_out.begin(), h->host_out.begin() + (size_t)n * rows * 6);
+    h->host_steps -= n;
+    h->host_first += n;
+  }
+  return TLM_OK;
+}
+
+int tlm_read_dft(tlm_handle* h, double* re_im) {
+  if (!h || !re_im) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  size_t n = h->pt_row.size() * h->freq.size() * 2;
+  if (n) {
+    HIP_TRY(h, hipMemcpyAsync(re_im, h->d_dft, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+  }
+  return TLM_OK;
+}
+
+}  // extern "C"

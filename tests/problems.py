@@ -1,0 +1,179 @@
+"""Problem decks shared by the tests.
+
+The reference decks (TEST_DATA/*/PROBLEM_SPECIFICATION_FILES/*.inp) are
+re-expressed with ggi_tlm_amd.problem; the synthetic ones exercise every
+hot-path class at sizes the oracle runs in well under a second.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ggi_tlm_amd.constants import FACE_XMAX, FACE_XMIN, FACE_YMIN, FACE_ZMAX, FACE_ZMIN
+from ggi_tlm_amd.filters import Sfilter, debye_lorentz_eps
+from ggi_tlm_amd.problem import (PML, Box, ExcitationFunction, HuygensBox, OutputPoint,
+                                 PointSource, Problem, Sphere, VolumeMaterial, point_to_cell)
+
+
+def mesh_dl(lo, hi, n):
+    """set_mesh_parameters.F90:66-69: the mean of the three cell sizes."""
+    d = [(hi[i] - lo[i]) / float(n[i]) for i in range(3)]
+    return (d[0] + d[1] + d[2]) / 3.0
+
+
+# ---- reference decks ----------------------------------------------------------
+
+def cavity(eps=None, mu=None):
+    """TEST_DATA/CAVITY_FREE_SPACE (and CAVITY_EPSR=2 / CAVITY_MUR=2 with a
+    filling material): 10x20x30 PEC cavity, soft Ex Gaussian source."""
+    mm = (-0.5, -1.0, -1.5)
+    dl = mesh_dl(mm, (0.5, 1.0, 1.5), (10, 20, 30))
+    p = Problem(nx=10, ny=20, nz=30, dl=dl, simulation_time=1e-6, mesh_min=mm, reflection=(-1.0,) * 6,
+                functions=[ExcitationFunction("gaussian", (1.0, 0.25e-9, 1e-9))],
+                sources=[PointSource(point_to_cell(mm, dl, (-0.25, -0.45, 0.15)), "ex", 1)],
+                outputs=[OutputPoint(point_to_cell(mm, dl, (0.15, -0.75, -0.15)), "ex")])
+    if eps is not None or mu is not None:
+        p.materials = [VolumeMaterial("dispersive", eps or Sfilter(1e8, [1.0], [1.0]),
+                                      mu or Sfilter(1e8, [1.0], [1.0]))]
+        p.material_regions = [(Box(1, 10, 1, 20, 1, 30), 1)]
+    return p
+
+
+def material_test2():
+    """TEST_DATA/MATERIAL_TEST2: 60^3, mur/epsr Lorentz block, soft Ex source.
+    Cells (42,31,31) source and (18,31,31) probe as the reference's model
+    builder places the deck's points."""
+    mm = (-3e-3,) * 3
+    dl = mesh_dl(mm, (3e-3,) * 3, (60, 60, 60))
+    lor = Sfilter(1e10, [2.0, 3.1827e-1, 4.0518e-1], [1.0, 3.1827e-1, 4.0518e-1])
+    return Problem(nx=60, ny=60, nz=60, dl=dl, simulation_time=10e-10, mesh_min=mm, reflection=(0.0,) * 6,
+                   materials=[VolumeMaterial("dispersive", lor, lor)],
+                   material_regions=[(Box(26, 35, 26, 35, 26, 35), 1)],
+                   functions=[ExcitationFunction("gaussian", (1.0, 1e-10, 3e-10))],
+                   sources=[PointSource((42, 31, 31), "ex", 1)],
+                   outputs=[OutputPoint((18, 31, 31), "ex")])
+
+
+def huygens_surface():
+    """TEST_DATA/HUYGENS_SURFACE: 40^3, R=0, Huygens box -1..1 m, oblique k.
+    The deck's output point (0,0,0) sits on a cell corner; the reference's
+    model builder puts it in cell (21,21,21), with dl = 0.1."""
+    mm = (-2.0,) * 3
+    dl = 0.1
+    c = (21, 21, 21)
+    return Problem(nx=40, ny=40, nz=40, dl=dl, simulation_time=2e-8, mesh_min=mm, reflection=(0.0,) * 6,
+                   functions=[ExcitationFunction("gaussian", (1.0, 2e-9, 6e-9))],
+                   huygens=HuygensBox(Box(11, 30, 11, 30, 11, 30), 1, 54.73, 45.0, 1.0, 0.5, side=1),
+                   outputs=[OutputPoint(c, k) for k in ("ex", "ey", "ez", "hx", "hy", "hz")])
+
+
+def tem_pml():
+    """TEST_DATA/TEM_PML_TEST: 1x1x200 TEM cell, 20-cell PML at zmax, one
+    Huygens face at z = 0.1 m."""
+    mm = (-0.05, -0.05, 0.0)
+    dl = 0.1
+    c1 = point_to_cell(mm, dl, (0, 0, 9.95))
+    c2 = point_to_cell(mm, dl, (0, 0, 10.05))
+    return Problem(nx=1, ny=1, nz=200, dl=dl, simulation_time=30e-8, mesh_min=mm,
+                   reflection=(1.0, 1.0, -1.0, -1.0, 0.0, -1.0),
+                   pml=PML((0, 0, 0, 0, 0, 20), (1e-1, 1e-1, 1e-1, 1e-1, 1e-1, 1e-4), 2),
+                   functions=[ExcitationFunction("gaussian", (1.0, 2e-9, 8e-9))],
+                   huygens=HuygensBox(Box(1, 1, 1, 1, 2, 1), 1, 0.0, 90.0, 1.0, 0.0, side=1,
+                                      faces=[(1, 1, 1, FACE_ZMAX)]),
+                   outputs=[OutputPoint(c1, "ey"), OutputPoint(c1, "hx"), OutputPoint(c2, "ey"),
+                            OutputPoint(c2, "hx")])
+
+
+# ---- synthetic decks ------------------------------------------------------------
+
+def free_cube(n=16, steps=60, r=-1.0, dft=()):
+    """Plain free-space cube with a centred soft Ez source and three probes
+    (the 64^3 correctness workload, shrunk)."""
+    dl = 0.01
+    c = n // 2
+    return Problem(nx=n, ny=n, nz=n, dl=dl, n_timesteps=steps, reflection=(r,) * 6,
+                   functions=[ExcitationFunction("gaussian", (1.0, 20 * dl / (2 * 2.99792458e8),
+                                                              80 * dl / (2 * 2.99792458e8)))],
+                   sources=[PointSource((c, c, c), "ez", 1)],
+                   outputs=[OutputPoint((c + 2, c - 1, c + 3), k) for k in ("ex", "ey", "ez")] +
+                           [OutputPoint((2, 3, 4), "hz")],
+                   dft_frequencies=dft)
+
+
+def pml_sphere(n=24, steps=80, npml=4, sphere=True, huygens=True, sigma=0.01):
+    """Shrunk 256^3 + PML + dielectric sphere + x-directed plane wave."""
+    dl = 5e-3
+    dt = dl / (2 * 2.99792458e8)
+    p = Problem(nx=n, ny=n, nz=n, dl=dl, n_timesteps=steps, reflection=(-1.0,) * 6,
+                pml=PML((npml,) * 6, (1e-4,) * 6, 2),
+                functions=[ExcitationFunction("gaussian", (1.0, 6 * dt, 24 * dt))],
+                outputs=[OutputPoint((npml + 2, n // 2, n // 2 + 1), "ez"),
+                         OutputPoint((n - npml - 3, n // 2 - 1, n // 2), "ez"),
+                         OutputPoint((n // 2, npml + 3, n // 2), "hy"),
+                         OutputPoint((2, n // 2, n // 2), "ez"),      # inside the PML
+                         OutputPoint((n // 2, n // 2, 1), "hx")])     # inside the PML
+    if sphere:
+        p.materials = [VolumeMaterial("dispersive", Sfilter.constant(4.0), Sfilter.constant(1.0), sigma_e=sigma)]
+        p.material_regions = [(Sphere(n / 2 + 1.3, n / 2 - 0.7, n / 2 + 0.4, n / 6), 1)]
+        p.outputs.append(OutputPoint((n // 2 + 1, n // 2, n // 2), "ez"))
+    if huygens:
+        b = npml + 2
+        p.huygens = HuygensBox(Box(b, n - b + 1, b, n - b + 1, b, n - b + 1), 1, 90.0, 0.0, -1.0, 0.0, side=1)
+    return p
+
+
+def dispersive_blocks(n=20, steps=80, seed=3):
+    """Shrunk 2-pole dispersive load with PML: Debye and Lorentz blocks."""
+    rng = np.random.default_rng(seed)
+    dl = 1e-3
+    dt = dl / (2 * 2.99792458e8)
+    npml = 3
+    mats, regions = [], []
+    for k in range(4):
+        einf = float(rng.uniform(2, 6))
+        f1, f2 = (float(10 ** rng.uniform(8, 10)) for _ in range(2))
+        r1, r2 = (float(rng.uniform(0.5, 5)) for _ in range(2))
+        if k % 2 == 0:
+            eps = debye_lorentz_eps(einf, [("debye", r1, 2 * np.pi * f1), ("debye", r2, 2 * np.pi * f2)])
+        else:
+            eps = debye_lorentz_eps(einf, [("lorentz", r1, 2 * np.pi * f1, 0.3)])
+        mats.append(VolumeMaterial("dispersive", eps, Sfilter.constant(1.0), sigma_e=float(rng.uniform(0, 0.1))))
+        lo = rng.integers(npml + 1, n - npml - 5, size=3)
+        e = rng.integers(2, 5, size=3)
+        regions.append((Box(int(lo[0]), int(lo[0] + e[0]), int(lo[1]), int(lo[1] + e[1]),
+                            int(lo[2]), int(lo[2] + e[2])), k + 1))
+    return Problem(nx=n, ny=n, nz=n, dl=dl, n_timesteps=steps, reflection=(-1.0,) * 6,
+                   pml=PML((npml,) * 6, (1e-4,) * 6, 2), materials=mats, material_regions=regions,
+                   functions=[ExcitationFunction("gaussian", (1.0, 5 * dt, 20 * dt))],
+                   sources=[PointSource((n // 2, n // 2, n // 2), "ez", 1)],
+                   outputs=[OutputPoint((n // 2 + 2, n // 2 + 1, n // 2 - 2), "ez"),
+                            OutputPoint((n // 2 - 3, n // 2, n // 2 + 1), "ex")],
+                   dft_frequencies=np.logspace(8, 10, 7))
+
+
+def special_mix(n=14, steps=50):
+    """PEC and PMC volume cells, surface PEC/PMC/free-space faces, hard and
+    current-density point sources, a face point source and face outputs."""
+    dl = 0.01
+    dt = dl / (2 * 2.99792458e8)
+    faces_pec = np.array([[4, j, k, FACE_XMIN] for j in range(3, 9) for k in range(3, 9)])
+    faces_pmc = np.array([[i, 10, k, FACE_YMIN] for i in range(4, 9) for k in range(5, 10)])
+    faces_free = np.array([[i, j, 11, FACE_ZMAX] for i in range(5, 8) for j in range(5, 8)])
+    return Problem(nx=n, ny=n, nz=n, dl=dl, n_timesteps=steps, reflection=(-1.0, 0.0, 0.5, -1.0, 1.0, -0.3),
+                   materials=[VolumeMaterial("pec"), VolumeMaterial("pmc"),
+                              VolumeMaterial("dispersive", Sfilter.constant(2.5), Sfilter.constant(1.7),
+                                             sigma_e=0.02, sigma_m=3.0)],
+                   material_regions=[(Box(9, 10, 9, 10, 2, 4), 1), (Box(2, 3, 11, 12, 9, 11), 2),
+                                     (Box(6, 8, 2, 4, 6, 8), 3)],
+                   surface_materials=["pec", "pmc", "free_space"],
+                   surface_faces=[(faces_pec, 1), (faces_pmc, 2), (faces_free, 3)],
+                   functions=[ExcitationFunction("gaussian", (1.0, 4 * dt, 16 * dt)),
+                              ExcitationFunction("gaussian_sinusoid", (0.5, 6 * dt, 20 * dt, 3e9, 10.0)),
+                              ExcitationFunction("step", (0.2, 10 * dt))],
+                   sources=[PointSource((7, 7, 7), "ey", 1),
+                            PointSource((7, 7, 7), "jz", 2),
+                            PointSource((11, 6, 5), "hx", 3, soft=False),
+                            PointSource((5, 9, 6), "ez", 2, point="xmax"),
+                            PointSource((7, 8, 7), "ex", 1, soft=False)],
+                   outputs=[OutputPoint((8, 8, 8), k) for k in ("ex", "ey", "ez", "hx", "hy", "hz")] +
+                           [OutputPoint((9, 9, 3), "ex"), OutputPoint((2, 11, 10), "hx"),
+                            OutputPoint((7, 3, 7), "ez")])

@@ -1,0 +1,119 @@
+"""HIP engine vs the CPU oracle, through the C ABI.
+
+Plain FP64 form: bit-identical cell outputs and link state after every
+checked step (the reference's operation order is kept and multiply-add
+fusion is off).  Precision form: FP32 link planes, compared with the FP64
+oracle by relative L2 per probe, tolerance 1e-5 as the north-star bar.
+"""
+import numpy as np
+import pytest
+
+import problems
+from ggi_tlm_amd import capi
+from ggi_tlm_amd.capi import FORM_PLAIN_FP64, FORM_PRECISION_FP32, probe_series
+from ggi_tlm_amd.problem import build_tables
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "free_cube": lambda: problems.free_cube(16, 60, dft=np.linspace(1e8, 3e9, 5)),
+    "free_cube_odd": lambda: problems.free_cube(13, 41, r=0.3),
+    "cavity_free_space": lambda: problems.cavity(),
+    "tem_pml": problems.tem_pml,
+    "pml_sphere": problems.pml_sphere,
+    "dispersive_blocks": problems.dispersive_blocks,
+    "special_mix": problems.special_mix,
+    "huygens_surface": problems.huygens_surface,
+}
+
+
+def run(lib_handle, t, steps):
+    lib_handle.step(1, steps)
+    out, first = lib_handle.read_cell_outputs()
+    return out
+
+
+def engine(t, form):
+    print(f"[engine] {t.nx}x{t.ny}x{t.nz} form={form} special={t.n_special_cells} "
+          f"faces={t.n_special_faces}", flush=True)
+    return capi.Handle(capi.engine_library(), t, form, output_capacity=64)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_plain_bitwise(name, oracle_lib):
+    from oracle import oracle
+
+    t = build_tables(CASES[name]())
+    steps = min(t.n_timesteps, 400)
+    o = oracle.handle(t, output_capacity=steps)
+    g = engine(t, FORM_PLAIN_FP64)
+    ro = run(o, t, steps)
+    rg = run(g, t, steps)
+    assert ro.shape == rg.shape
+    assert np.array_equal(ro, rg), np.max(np.abs(ro - rg))
+    Vo = o.download_links()
+    Vg = g.download_links()
+    assert np.array_equal(Vo[:, 1:-1, 1:-1, :], Vg[:, 1:-1, 1:-1, :])
+    if len(t.dft_frequencies) and len(t.output_rows):
+        do, dg = o.read_dft(), g.read_dft()
+        scale = np.max(np.abs(do))
+        assert np.max(np.abs(do - dg)) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("name", ["pml_sphere", "special_mix", "dispersive_blocks"])
+def test_split_calls_match_fused_step(name, oracle_lib):
+    """excitation/scatter/connect/outer_boundary one by one (the reference's
+    call sites) == tlm_step, with downloads in between."""
+    from oracle import oracle
+
+    t = build_tables(CASES[name]())
+    steps = 25
+    o = oracle.handle(t, output_capacity=steps)
+    g = engine(t, FORM_PLAIN_FP64)
+    for ts in range(1, steps + 1):
+        for h in (o, g):
+            h.excitation(ts)
+            h.scatter(ts)
+            h.connect(ts)
+            h.outer_boundary()
+        if ts in (1, 7, steps):
+            assert np.array_equal(o.download_links()[:, 1:-1, 1:-1], g.download_links()[:, 1:-1, 1:-1])
+    assert np.array_equal(o.read_cell_outputs()[0], g.read_cell_outputs()[0])
+
+
+def test_upload_then_step(oracle_lib):
+    from oracle import oracle
+
+    t = build_tables(problems.pml_sphere(steps=40))
+    rng = np.random.default_rng(7)
+    V = np.zeros((12, t.nx + 2, t.ny + 2, t.nz), order="F")
+    V[:, 1:-1, 1:-1, :] = rng.standard_normal((12, t.nx, t.ny, t.nz))
+    o = oracle.handle(t, output_capacity=40)
+    g = engine(t, FORM_PLAIN_FP64)
+    o.upload_links(V)
+    g.upload_links(V)
+    o.step(1, 40)
+    g.step(1, 40)
+    assert np.array_equal(o.download_links()[:, 1:-1, 1:-1], g.download_links()[:, 1:-1, 1:-1])
+    assert np.array_equal(o.read_cell_outputs()[0], g.read_cell_outputs()[0])
+
+
+@pytest.mark.parametrize("name", ["free_cube", "pml_sphere", "dispersive_blocks", "special_mix", "huygens_surface"])
+def test_precision_form(name, oracle_lib):
+    from oracle import oracle
+
+    t = build_tables(CASES[name]())
+    steps = min(t.n_timesteps, 400)
+    o = oracle.handle(t, output_capacity=steps)
+    g = engine(t, FORM_PRECISION_FP32)
+    so = probe_series(run(o, t, steps), t)
+    sg = probe_series(run(g, t, steps), t)
+    # A probe whose FP64 signal is zero by symmetry (only round-off) has no
+    # meaningful relative error; its norm is floored at 1e-3 of the largest
+    # probe norm of the solve.
+    floor = 1e-3 * max(np.linalg.norm(so[:, k]) for k in range(so.shape[1]))
+    for k in range(so.shape[1]):
+        den = max(np.linalg.norm(so[:, k]), floor)
+        rel = np.linalg.norm(sg[:, k] - so[:, k]) / den
+        print(f"probe {k}: relative L2 {rel:.3e}")
+        assert rel < 1e-5, (name, k, rel)
