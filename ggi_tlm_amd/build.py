@@ -48,5 +48,24 @@ def build_engine(verbose: bool = False) -> str:
     return SO
 
 
+COPY_SRC = os.path.join(HERE, "csrc", "copy_probe.hip")
+COPY_BIN = os.path.join(HERE, "copy_probe")
+
+
+def build_copy_probe(verbose: bool = False) -> str:
+    """HBM calibration program (plain copy of known size) used when scaling
+    rocprof FETCH_SIZE / WRITE_SIZE and as the practical bandwidth ceiling."""
+    cmd = [hipcc(), "-O3", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", COPY_SRC, "-o", COPY_BIN]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building copy_probe")
+    return COPY_BIN
+
+
 if __name__ == "__main__":
     print(build_engine(verbose=True))
+    print(build_copy_probe(verbose=True))

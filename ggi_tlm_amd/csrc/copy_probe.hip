@@ -1,0 +1,48 @@
+// copy_probe.hip -- calibration program for HBM measurements.
+//
+// Streams a plain float4 copy of a known number of bytes, so that rocprof's
+// FETCH_SIZE / WRITE_SIZE counters can be scaled (bytes the counter reports
+// for a copy of known size), and prints the copy bandwidth as the practical
+// HBM ceiling of the box.  Usage: copy_probe [MiB=1024] [reps=20]
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a, float4* __restrict__ b, long long n) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) b[i] = a[i];
+}
+
+int main(int argc, char** argv) {
+  long long mib = argc > 1 ? atoll(argv[1]) : 1024;
+  int reps = argc > 2 ? atoi(argv[2]) : 20;
+  long long bytes = mib << 20;
+  long long n = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    fprintf(stderr, "hipMalloc failed\n");
+    return 1;
+  }
+  hipMemset(a, 0, bytes);
+  hipMemset(b, 0, bytes);
+  int blocks = 256 * 32;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+  hipDeviceSynchronize();
+  hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+  hipEventRecord(e1, 0);
+  hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  double per = ms / reps;
+  printf("{\"copy_bytes_read\": %lld, \"copy_bytes_written\": %lld, \"avg_ms\": %.5f, \"GB_s\": %.1f}\n", bytes, bytes,
+         per, 2.0 * bytes / (per * 1e-3) / 1e9);
+  hipFree(a);
+  hipFree(b);
+  return 0;
+}

@@ -87,6 +87,12 @@ struct tlm_handle {
   bool pend_connect = false, pend_boundary = false;
   int pend_ts = 0;
 
+  // profiling: event pairs per kernel class, resolved on read
+  bool profiling = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  int64_t prof_launches[TLM_N_KCLASS] = {0, 0, 0, 0};
+  double prof_ms[TLM_N_KCLASS] = {0, 0, 0, 0};
+
   // output ring
   int cap = 4096;
   int ring_used = 0;         // slots holding unread steps on the device
@@ -137,6 +143,39 @@ int dev_alloc(tlm_handle* h, size_t bytes, void** out) {
 }
 
 size_t link_size(const tlm_handle* h) { return h->form == TLM_FORM_PLAIN_FP64 ? sizeof(double) : sizeof(float); }
+
+// Profiling brackets (no-ops unless enabled).
+struct ProfScope {
+  tlm_handle* h;
+  int cls;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(tlm_handle* hh, int c) : h(hh), cls(c) {
+    if (!h->profiling) return;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    hipEventRecord(a, h->stream);
+  }
+  ~ProfScope() {
+    if (!h->profiling) return;
+    hipEventRecord(b, h->stream);
+    h->ev.push_back({cls, {a, b}});
+  }
+};
+
+int resolve_profile(tlm_handle* h) {
+  if (h->ev.empty()) return TLM_OK;
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  for (auto& e : h->ev) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e.second.first, e.second.second);
+    h->prof_ms[e.first] += ms;
+    h->prof_launches[e.first] += 1;
+    hipEventDestroy(e.second.first);
+    hipEventDestroy(e.second.second);
+  }
+  h->ev.clear();
+  return TLM_OK;
+}
 
 // Build every device table from the reference tables.
 int finalize(tlm_handle* h) {
@@ -440,14 +479,19 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
   T* Lout = (T*)h->planes[h->cur ^ 1];
   dim3 bb(64, 4, 1);
   dim3 gb((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
-  if (pull) hipLaunchKernelGGL((scn::k_bulk<T, PLAIN, true>), gb, bb, 0, h->stream, d, Lin, Lout, a);
-  else hipLaunchKernelGGL((scn::k_bulk<T, PLAIN, false>), gb, bb, 0, h->stream, d, Lin, Lout, a);
+  {
+    ProfScope ps(h, TLM_KCLASS_BULK);
+    if (pull) hipLaunchKernelGGL((scn::k_bulk<T, PLAIN, true>), gb, bb, 0, h->stream, d, Lin, Lout, a);
+    else hipLaunchKernelGGL((scn::k_bulk<T, PLAIN, false>), gb, bb, 0, h->stream, d, Lin, Lout, a);
+  }
   if (d.n_pml) {
+    ProfScope ps(h, TLM_KCLASS_PML);
     dim3 g((d.n_pml + 255) / 256);
     if (pull) hipLaunchKernelGGL((scn::k_pml<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot);
     else hipLaunchKernelGGL((scn::k_pml<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot);
   }
   if (d.n_gen) {
+    ProfScope ps(h, TLM_KCLASS_GENERIC);
     dim3 g((d.n_gen + 255) / 256);
     if (pull) hipLaunchKernelGGL((scn::k_generic<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, timestep, slot);
     else hipLaunchKernelGGL((scn::k_generic<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, timestep, slot);
@@ -496,6 +540,7 @@ int do_scatter(tlm_handle* h, int timestep) {
     if (h->ring_used == 0) h->ring_first_step = timestep;
     h->ring_used++;
     int np = (int)h->pt_row.size(), nf = (int)h->freq.size();
+    ProfScope ps(h, TLM_KCLASS_DFT);
     if (np && nf)
       hipLaunchKernelGGL(scn::k_dft, dim3((np * nf + 255) / 256), dim3(256), 0, h->stream, h->d.ring, h->d.n_out_rows,
                          slot, timestep, h->d_prow, h->d_pcomp, np, h->d_freq, nf, h->g.dt, h->d_dft);
@@ -559,6 +604,10 @@ int tlm_create(const tlm_grid* g, int32_t form, tlm_handle** out) {
 void tlm_destroy(tlm_handle* h) {
   if (!h) return;
   if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto& e : h->ev) {
+    hipEventDestroy(e.second.first);
+    hipEventDestroy(e.second.second);
+  }
   for (auto& b : h->bufs) hipFree(b.p);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -847,6 +896,59 @@ int tlm_read_dft(tlm_handle* h, double* re_im) {
     HIP_TRY(h, hipMemcpyAsync(re_im, h->d_dft, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
   }
+  return TLM_OK;
+}
+
+int tlm_set_profiling(tlm_handle* h, int32_t enable) {
+  if (!h) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = resolve_profile(h))) return rc;
+  h->profiling = enable != 0;
+  return TLM_OK;
+}
+
+int tlm_read_kernel_times(tlm_handle* h, int64_t* launches, double* ms) {
+  if (!h || !launches || !ms) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = resolve_profile(h))) return rc;
+  for (int k = 0; k < TLM_N_KCLASS; k++) {
+    launches[k] = h->prof_launches[k];
+    ms[k] = h->prof_ms[k];
+    h->prof_launches[k] = 0;
+    h->prof_ms[k] = 0.0;
+  }
+  return TLM_OK;
+}
+
+int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
+  if (!h || !cells || !bytes) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  const int64_t sl = (int64_t)link_size(h);
+  const int64_t n_bulk = h->N - h->n_pml - h->n_gen;
+  // bulk: 12 links read + 12 written, plus its bit of the class mask
+  cells[TLM_KCLASS_BULK] = n_bulk;
+  bytes[TLM_KCLASS_BULK] = n_bulk * 24 * sl + (h->N + 7) / 8;
+  // PML: links, 24 FP64 state words read and written, 5 int32 list fields
+  cells[TLM_KCLASS_PML] = h->n_pml;
+  bytes[TLM_KCLASS_PML] = (int64_t)h->n_pml * (24 * sl + 2 * 8 * scn::PS_N + 5 * 4);
+  // generic: links, 7 int32 list fields, filter words of dispersive cells
+  int64_t gb = (int64_t)h->n_gen * (24 * sl + 7 * 4);
+  {
+    std::vector<int32_t> gm(h->n_gen);
+    if (h->n_gen) hipMemcpy(gm.data(), h->d.gen_mat, gm.size() * sizeof(int32_t), hipMemcpyDeviceToHost);
+    for (int32_t m : gm) {
+      if (!m) continue;
+      const tlm_volume_material& vm = h->vmat[m - 1];
+      if (vm.type != TLM_VMAT_DISPERSIVE) continue;
+      int64_t words = (vm.eps_filter_exists ? vm.eps_order + 1 : 0) + (vm.mu_filter_exists ? vm.mu_order + 1 : 0);
+      gb += 3 * 2 * 8 * words;
+    }
+  }
+  cells[TLM_KCLASS_GENERIC] = h->n_gen;
+  bytes[TLM_KCLASS_GENERIC] = gb;
+  cells[TLM_KCLASS_DFT] = (int64_t)h->pt_row.size() * (int64_t)h->freq.size();
+  bytes[TLM_KCLASS_DFT] = cells[TLM_KCLASS_DFT] * 2 * 2 * 8;
   return TLM_OK;
 }
 

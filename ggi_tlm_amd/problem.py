@@ -73,9 +73,10 @@ class Sphere:
     radius: float
 
     def mask(self, nx, ny, nz):
-        z, y, x = np.meshgrid(np.arange(1, nz + 1) - 0.5, np.arange(1, ny + 1) - 0.5,
-                              np.arange(1, nx + 1) - 0.5, indexing="ij")
-        return (x - self.cx) ** 2 + (y - self.cy) ** 2 + (z - self.cz) ** 2 <= self.radius ** 2
+        x = (np.arange(1, nx + 1) - 0.5 - self.cx) ** 2
+        y = (np.arange(1, ny + 1) - 0.5 - self.cy) ** 2
+        z = (np.arange(1, nz + 1) - 0.5 - self.cz) ** 2
+        return (x[None, None, :] + y[None, :, None] + z[:, None, None]) <= self.radius ** 2
 
 
 @dataclass

@@ -229,6 +229,27 @@ int tlm_read_cell_outputs(tlm_handle* h, double* out, int32_t max_steps,
 /* DFT accumulators: re_im[(p*n_freq + k)*2 + {0,1}]. */
 int tlm_read_dft(tlm_handle* h, double* re_im);
 
+/* ---- measurement (no counterpart in the reference) ----------------------- */
+
+/* Kernel classes timed by the profiling hooks. */
+#define TLM_KCLASS_BULK 0
+#define TLM_KCLASS_PML 1
+#define TLM_KCLASS_GENERIC 2
+#define TLM_KCLASS_DFT 3
+#define TLM_N_KCLASS 4
+
+/* When enabled, every kernel launch of a step is bracketed by HIP events
+ * recorded on the handle's own stream.  tlm_read_kernel_times waits for the
+ * stream and returns, per class, the launch count and the summed duration
+ * (ms) since the previous read, then resets them. */
+int tlm_set_profiling(tlm_handle* h, int32_t enable);
+int tlm_read_kernel_times(tlm_handle* h, int64_t* launches, double* ms);
+
+/* Per-class cell counts and algorithmic HBM bytes per launch (links read
+ * and written, FP64 state words read and written, per-cell indices), the
+ * figures the roofline fraction is computed from. */
+int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1040,3 +1040,16 @@ int oracle_read_face_outputs(oracle_handle* h, double* out) {
   if (n) memcpy(out, h->face_out, n * sizeof(double));
   return TLM_OK;
 }
+
+/* Measurement hooks of the engine ABI; the oracle has no kernels to time. */
+int oracle_set_profiling(oracle_handle* h, int32_t enable) { (void)h; (void)enable; return TLM_OK; }
+int oracle_read_kernel_times(oracle_handle* h, int64_t* launches, double* ms) {
+  (void)h;
+  for (int k = 0; k < TLM_N_KCLASS; k++) { launches[k] = 0; ms[k] = 0.0; }
+  return TLM_OK;
+}
+int oracle_kernel_bytes(oracle_handle* h, int64_t* cells, int64_t* bytes) {
+  (void)h;
+  for (int k = 0; k < TLM_N_KCLASS; k++) { cells[k] = 0; bytes[k] = 0; }
+  return TLM_OK;
+}
