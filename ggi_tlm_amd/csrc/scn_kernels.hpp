@@ -375,25 +375,132 @@ __device__ inline void zf_timeshift(double* w, long long n, long long f, int ord
 // kernels
 // ---------------------------------------------------------------------------
 
-// Bulk free-space cells.  Block (64,4): one wavefront per x-row segment.
+// Tile schedule of the bulk kernel: 64 x BY cell tiles of an xy-plane, each
+// block walking ZC consecutive planes.  Hardware dispatches workgroup b to
+// XCD b % 8; the logical tile of workgroup b is chosen so that each XCD
+// receives one contiguous range of logical tiles (x fastest, then y, then
+// z-chunk), which keeps the x/y halo rows of neighbouring tiles in the same
+// XCD's L2.
+struct BulkMap {
+  int nbx, nby, nchunks, zc, nblocks, xcd_remap;
+};
+
+constexpr int kBulkBY = 4;
+
+template <typename T>
+__device__ inline T shfl_up1(T v) { return __shfl_up(v, 1, 64); }
+template <typename T>
+__device__ inline T shfl_down1(T v) { return __shfl_down(v, 1, 64); }
+
+// Bulk free-space cells, marching along z.  Per cell and plane:
+//   own 12 ports       prefetched one plane ahead (coalesced along x)
+//   zmin-face ports    the previous plane's own Vx_zmax, Vy_zmax (registers)
+//   zmax-face ports    part of the next plane's prefetch
+//   x-face ports       neighbouring lanes (cross-lane moves), loads only at
+//                      the wavefront edges
+//   y-face ports       loads of the adjacent rows (shared through L1/L2 with
+//                      the block's other waves and the neighbouring tiles)
+// Special cells (bitmask) keep the register pipeline but are not written.
 template <typename T, bool PLAIN, bool PULL>
 __global__ __launch_bounds__(256) void k_bulk(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                             PullArgs a) {
-  int x = blockIdx.x * blockDim.x + threadIdx.x;
-  int y = blockIdx.y * blockDim.y + threadIdx.y;
-  int z = blockIdx.z;
-  if (x >= d.nx || y >= d.ny) return;
-  long long c = x + (long long)d.nx * (y + (long long)d.ny * z);
-  if ((d.mask[c >> 5] >> (c & 31)) & 1u) return;
-  double v[12];
-  if (PULL) {
-    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, -1, a, v);
-  } else {
-#pragma unroll
-    for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
+                                             PullArgs a, BulkMap m) {
+  int b = blockIdx.x, L = b;
+  if (m.xcd_remap) {
+    const int per = m.nblocks >> 3;
+    L = (b & 7) * per + (b >> 3);
   }
-  if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
-  store_cell(Lout, d.P, c, v);
+  const int bx = L % m.nbx;
+  const int t = L / m.nbx;
+  const int by = t % m.nby;
+  const int chunk = t / m.nby;
+  const int lane = threadIdx.x;
+  const int x = bx * 64 + lane;
+  const int y = by * kBulkBY + threadIdx.y;
+  const int z0 = chunk * m.zc;
+  const int z1 = min(z0 + m.zc, d.nz);
+  const bool valid = x < d.nx && y < d.ny;
+  const long long P = d.P;
+  const long long sxy = (long long)d.nx * d.ny;
+  const long long c0 = valid ? x + (long long)d.nx * y : 0;  // offset inside a plane
+  const double Rx0 = a.apply_r ? d.R[0] : 1.0, Rx1 = a.apply_r ? d.R[1] : 1.0;
+  const double Ry0 = a.apply_r ? d.R[2] : 1.0, Ry1 = a.apply_r ? d.R[3] : 1.0;
+  const double Rz0 = a.apply_r ? d.R[4] : 1.0, Rz1 = a.apply_r ? d.R[5] : 1.0;
+
+  T cur[12];
+  T pza = 0, pzb = 0;  // Vx_zmax, Vy_zmax of plane z-1
+#pragma unroll
+  for (int p = 0; p < 12; p++) cur[p] = valid ? Lin[p * P + c0 + z0 * sxy] : (T)0;
+  if (PULL && valid && z0 > 0) {
+    pza = Lin[Vx_zmax * P + c0 + (z0 - 1) * sxy];
+    pzb = Lin[Vy_zmax * P + c0 + (z0 - 1) * sxy];
+  }
+  for (int z = z0; z < z1; z++) {
+    const long long c = c0 + z * sxy;
+    T nxt[12];
+    const bool has_next = z + 1 < z1;
+#pragma unroll
+    for (int p = 0; p < 12; p++) nxt[p] = (T)0;
+    if (valid) {
+      if (has_next) {
+#pragma unroll
+        for (int p = 0; p < 12; p++) nxt[p] = Lin[p * P + c + sxy];
+      } else if (PULL && z + 1 < d.nz) {
+        nxt[Vx_zmin] = Lin[Vx_zmin * P + c + sxy];
+        nxt[Vy_zmin] = Lin[Vy_zmin * P + c + sxy];
+      }
+    }
+    double v[12];
+    if (PULL) {
+      // x neighbours from the adjacent lanes (all lanes take part)
+      T xa = shfl_up1(cur[Vy_xmax]), xb = shfl_up1(cur[Vz_xmax]);    // cell x-1
+      T xc = shfl_down1(cur[Vy_xmin]), xd = shfl_down1(cur[Vz_xmin]);  // cell x+1
+      if (valid) {
+        if (lane == 0 && x > 0) {
+          xa = Lin[Vy_xmax * P + c - 1];
+          xb = Lin[Vz_xmax * P + c - 1];
+        }
+        if (lane == 63 && x + 1 < d.nx) {
+          xc = Lin[Vy_xmin * P + c + 1];
+          xd = Lin[Vz_xmin * P + c + 1];
+        }
+        T ya = 0, yb = 0, yc = 0, yd = 0;
+        if (y > 0) {
+          ya = Lin[Vx_ymax * P + c - d.nx];
+          yb = Lin[Vz_ymax * P + c - d.nx];
+        }
+        if (y + 1 < d.ny) {
+          yc = Lin[Vx_ymin * P + c + d.nx];
+          yd = Lin[Vz_ymin * P + c + d.nx];
+        }
+        double o[12];
+#pragma unroll
+        for (int p = 0; p < 12; p++) o[p] = (double)cur[p];
+        if (x == 0) { v[Vy_xmin] = o[Vy_xmin] * Rx0; v[Vz_xmin] = o[Vz_xmin] * Rx0; }
+        else { v[Vy_xmin] = free_face<PLAIN>(o[Vy_xmin], (double)xa); v[Vz_xmin] = free_face<PLAIN>(o[Vz_xmin], (double)xb); }
+        if (x == d.nx - 1) { v[Vy_xmax] = o[Vy_xmax] * Rx1; v[Vz_xmax] = o[Vz_xmax] * Rx1; }
+        else { v[Vy_xmax] = free_face<PLAIN>(o[Vy_xmax], (double)xc); v[Vz_xmax] = free_face<PLAIN>(o[Vz_xmax], (double)xd); }
+        if (y == 0) { v[Vx_ymin] = o[Vx_ymin] * Ry0; v[Vz_ymin] = o[Vz_ymin] * Ry0; }
+        else { v[Vx_ymin] = free_face<PLAIN>(o[Vx_ymin], (double)ya); v[Vz_ymin] = free_face<PLAIN>(o[Vz_ymin], (double)yb); }
+        if (y == d.ny - 1) { v[Vx_ymax] = o[Vx_ymax] * Ry1; v[Vz_ymax] = o[Vz_ymax] * Ry1; }
+        else { v[Vx_ymax] = free_face<PLAIN>(o[Vx_ymax], (double)yc); v[Vz_ymax] = free_face<PLAIN>(o[Vz_ymax], (double)yd); }
+        if (z == 0) { v[Vx_zmin] = o[Vx_zmin] * Rz0; v[Vy_zmin] = o[Vy_zmin] * Rz0; }
+        else { v[Vx_zmin] = free_face<PLAIN>(o[Vx_zmin], (double)pza); v[Vy_zmin] = free_face<PLAIN>(o[Vy_zmin], (double)pzb); }
+        if (z == d.nz - 1) { v[Vx_zmax] = o[Vx_zmax] * Rz1; v[Vy_zmax] = o[Vy_zmax] * Rz1; }
+        else { v[Vx_zmax] = free_face<PLAIN>(o[Vx_zmax], (double)nxt[Vx_zmin]); v[Vy_zmax] = free_face<PLAIN>(o[Vy_zmax], (double)nxt[Vy_zmin]); }
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < 12; p++) v[p] = (double)cur[p];
+    }
+    if (valid && !((d.mask[c >> 5] >> (c & 31)) & 1u)) {
+      if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
+      store_cell(Lout, P, c, v);
+    }
+    pza = cur[Vx_zmax];
+    pzb = cur[Vy_zmax];
+#pragma unroll
+    for (int p = 0; p < 12; p++) cur[p] = nxt[p];
+  }
 }
 
 // PML cells: scatter_PML.F90 stages 1-13.

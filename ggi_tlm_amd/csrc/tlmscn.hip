@@ -472,17 +472,33 @@ int drain_ring(tlm_handle* h) {
   return TLM_OK;
 }
 
+// z-chunk length: enough blocks to fill 256 CUs several times over while
+// keeping the chunk-boundary re-read (one plane in zc) small.
+scn::BulkMap bulk_map(const tlm_handle* h) {
+  scn::BulkMap m;
+  m.nbx = (h->nx + 63) / 64;
+  m.nby = (h->ny + scn::kBulkBY - 1) / scn::kBulkBY;
+  long long tiles = (long long)m.nbx * m.nby;
+  int zc = 32;
+  while (zc > 4 && tiles * ((h->nz + zc - 1) / zc) < 2048) zc >>= 1;
+  m.zc = std::min(zc, h->nz);
+  m.nchunks = (h->nz + m.zc - 1) / m.zc;
+  m.nblocks = (int)(tiles * m.nchunks);
+  m.xcd_remap = (m.nblocks % 8 == 0) ? 1 : 0;
+  return m;
+}
+
 template <typename T, bool PLAIN>
 int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, int slot) {
   const Dev& d = h->d;
   const T* Lin = (const T*)h->planes[h->cur];
   T* Lout = (T*)h->planes[h->cur ^ 1];
-  dim3 bb(64, 4, 1);
-  dim3 gb((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
   {
     ProfScope ps(h, TLM_KCLASS_BULK);
-    if (pull) hipLaunchKernelGGL((scn::k_bulk<T, PLAIN, true>), gb, bb, 0, h->stream, d, Lin, Lout, a);
-    else hipLaunchKernelGGL((scn::k_bulk<T, PLAIN, false>), gb, bb, 0, h->stream, d, Lin, Lout, a);
+    scn::BulkMap m = bulk_map(h);
+    dim3 bb(64, scn::kBulkBY, 1);
+    if (pull) hipLaunchKernelGGL((scn::k_bulk<T, PLAIN, true>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+    else hipLaunchKernelGGL((scn::k_bulk<T, PLAIN, false>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
   }
   if (d.n_pml) {
     ProfScope ps(h, TLM_KCLASS_PML);
@@ -540,10 +556,11 @@ int do_scatter(tlm_handle* h, int timestep) {
     if (h->ring_used == 0) h->ring_first_step = timestep;
     h->ring_used++;
     int np = (int)h->pt_row.size(), nf = (int)h->freq.size();
-    ProfScope ps(h, TLM_KCLASS_DFT);
-    if (np && nf)
+    if (np && nf) {
+      ProfScope ps(h, TLM_KCLASS_DFT);
       hipLaunchKernelGGL(scn::k_dft, dim3((np * nf + 255) / 256), dim3(256), 0, h->stream, h->d.ring, h->d.n_out_rows,
                          slot, timestep, h->d_prow, h->d_pcomp, np, h->d_freq, nf, h->g.dt, h->d_dft);
+    }
   }
   h->cur ^= 1;
   h->kind = Kind::SCATTERED;

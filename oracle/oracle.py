@@ -22,7 +22,9 @@ def library():
     global _lib
     if _lib is None:
         _lib = Library(ORACLE_SO, "oracle_",
-                       extra=[("read_face_outputs", (C.c_int, [C.c_void_p, C.c_void_p]))])
+                       extra=[("read_face_outputs", (C.c_int, [C.c_void_p, C.c_void_p])),
+                              ("zfilter_sequence", (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32,
+                                                              C.c_void_p, C.c_int32, C.c_void_p]))])
     return _lib
 
 
@@ -36,3 +38,29 @@ def build():
     import subprocess
 
     subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def zfilter_sequence(a, b, G):
+    """Outputs of the oracle's filter recursion for the input sequence G."""
+    import numpy as np
+
+    order = len(a) - 1
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    G = np.ascontiguousarray(G, dtype=np.float64)
+    out = np.zeros(len(G))
+    p = lambda x: x.ctypes.data_as(C.c_void_p)
+    rc = library().fn["zfilter_sequence"](p(a), p(b), order, p(G), len(G), p(out))
+    assert rc == 0
+    return out
+
+
+def face_outputs(h):
+    """face_output_field(row, side, 1:6) after the last connect."""
+    import numpy as np
+
+    n = h.t.n_output_faces
+    out = np.zeros((max(n, 1), 2, 6))
+    rc = library().fn["read_face_outputs"](h.h, out.ctypes.data_as(C.c_void_p))
+    assert rc == 0
+    return out[:n]

@@ -1053,3 +1053,18 @@ int oracle_kernel_bytes(oracle_handle* h, int64_t* cells, int64_t* bytes) {
   for (int k = 0; k < TLM_N_KCLASS; k++) { cells[k] = 0; bytes[k] = 0; }
   return TLM_OK;
 }
+
+/* Test hook: the per-step filter sequence of a dispersive cell component
+ * (timeshift, history-only evaluate, evaluate with the input) applied to
+ * G[0..n-1]; out[i] is the filter output after step i. */
+int oracle_zfilter_sequence(const double* a, const double* b, int32_t order, const double* G,
+                            int32_t n, double* out) {
+  double w[NW] = {0};
+  if (order < 0 || order > TLM_MAX_FILTER_ORDER) return TLM_ERR_ARG;
+  for (int i = 0; i < n; i++) {
+    timeshift_Zfilter(w, order);
+    (void)evaluate_Zfilter(a, b, w, order, 0.0);
+    out[i] = evaluate_Zfilter(a, b, w, order, G[i]);
+  }
+  return TLM_OK;
+}

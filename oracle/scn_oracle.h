@@ -65,6 +65,8 @@ int oracle_kernel_bytes(oracle_handle* h, int64_t* cells, int64_t* bytes);
 /* face_output_field(row, side, 1:6) of the last connect, row-major
  * (row, side, component); n_rows = total_number_output_faces. */
 int oracle_read_face_outputs(oracle_handle* h, double* out);
+int oracle_zfilter_sequence(const double* a, const double* b, int32_t order, const double* G,
+                            int32_t n, double* out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,295 @@
+"""Pins of the oracle that do not rely on the reference's golden files:
+closed forms, invariants, library routines and degenerate cases, each
+chosen so that a plausible restatement error (dropped term, wrong sign or
+port, transposed pair) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import problems
+from ggi_tlm_amd.capi import probe_series
+from ggi_tlm_amd.constants import C0, EPS0
+from ggi_tlm_amd.filters import (Sfilter, bilinear_s_to_z, debye_lorentz_eps, electric_stub_impedance,
+                                 fast_slow, magnetic_stub_impedance, sfilter_response, zfilter_response)
+from ggi_tlm_amd.problem import (PML, Box, ExcitationFunction, OutputPoint, PointSource, Problem,
+                                 VolumeMaterial, build_tables, excitation_tables)
+
+
+def oracle_handle(t, cap=None):
+    from oracle import oracle
+
+    return oracle.handle(t, output_capacity=cap or t.n_timesteps)
+
+
+def links(h):
+    return h.download_links()[:, 1:-1, 1:-1, :]
+
+
+# ---- scatter matrix ------------------------------------------------------------
+
+def scatter_matrix():
+    """Columns: scattered voltages of one free-space cell for unit incident
+    voltages, read through a 1x1x1 mesh with R = 0 (so the connect and
+    boundary leave the scattered values untouched)."""
+    p = Problem(nx=1, ny=1, nz=1, dl=0.01, n_timesteps=2, reflection=(0.0,) * 6)
+    t = build_tables(p)
+    S = np.zeros((12, 12))
+    for i in range(12):
+        h = oracle_handle(t)
+        V = np.zeros((12, 3, 3, 1), order="F")
+        V[i, 1, 1, 0] = 1.0
+        h.upload_links(V)
+        h.excitation(1)
+        h.scatter(1)
+        S[:, i] = links(h)[:, 0, 0, 0]
+    return S
+
+
+def test_scatter_matrix_orthogonal_and_symmetric(oracle_lib):
+    S = scatter_matrix()
+    assert np.allclose(S.T @ S, np.eye(12), atol=1e-15)   # lossless node
+    assert np.allclose(S, S.T, atol=1e-15)                 # reciprocity
+    assert np.allclose(np.diag(S), 0.0)                     # no port reflects into itself
+    # each row: two entries 1/2, one -1/2 ... (SCN: 4 nonzero, magnitude 1/2)
+    assert np.allclose(np.sort(np.abs(S), axis=1)[:, -4:], 0.5, atol=1e-15)
+    assert np.allclose(np.sort(np.abs(S), axis=1)[:, :-4], 0.0)
+
+
+def test_uniform_field_eigenvector(oracle_lib):
+    """A uniform E_x (all four x-polarised links equal) is a +1 eigenvector
+    of the node: the cell-centre voltage equals the link voltage, no current."""
+    S = scatter_matrix()
+    v = np.zeros(12)
+    v[[4, 6, 8, 10]] = 1.0  # Vx_ymin, Vx_ymax, Vx_zmin, Vx_zmax
+    assert np.allclose(S @ v, v, atol=1e-15)
+
+
+# ---- energy ----------------------------------------------------------------------
+
+def test_energy_conserved_lossless_cavity(oracle_lib):
+    """R = +-1, no loss, source switched off: sum V^2 over all links is
+    invariant (orthogonal scatter, connect a permutation)."""
+    p = problems.free_cube(12, 260, r=-1.0)
+    p.reflection = (-1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
+    t = build_tables(p)
+    h = oracle_handle(t)
+    h.step(1, 200)  # Gaussian: delay 80 dt, width 20 dt, cut off beyond 5 widths (step 181)
+    e0 = np.sum(links(h) ** 2)
+    h.step(201, 60)
+    e1 = np.sum(links(h) ** 2)
+    assert e0 > 0
+    assert abs(e1 - e0) / e0 < 1e-13
+
+
+def test_pml_absorbs_energy(oracle_lib):
+    """A zero-mean pulse radiated inside a PML-lined PEC box loses its energy
+    (a pulse with a DC part would leave a static field behind)."""
+    p = problems.pml_sphere(20, 200, npml=4, sphere=False, huygens=False)
+    dt = p.dt
+    p.functions = [ExcitationFunction("gaussian_sinusoid", (1.0, 6 * dt, 24 * dt, 1.0 / (10 * dt), 0.0))]
+    p.sources = [PointSource((10, 10, 10), "ez", 1)]
+    p.outputs = [OutputPoint((10, 10, 10), "ez")]
+    t = build_tables(p)
+    h = oracle_handle(t)
+    h.step(1, 60)
+    e0 = np.sum(links(h) ** 2)
+    h.step(61, 140)
+    e1 = np.sum(links(h) ** 2)
+    assert e1 < 2e-2 * e0, (e0, e1)  # a lossless box keeps all of it (test above)
+
+
+# ---- symmetry ----------------------------------------------------------------------
+
+def test_mirror_symmetry_of_centred_source(oracle_lib):
+    n = 11
+    p = Problem(nx=n, ny=n, nz=n, dl=0.01, n_timesteps=40, reflection=(-1.0,) * 6,
+                functions=[ExcitationFunction("gaussian", (1.0, 5 * 0.01 / (2 * C0), 20 * 0.01 / (2 * C0)))],
+                sources=[PointSource((6, 6, 6), "ez", 1)],
+                outputs=[OutputPoint((6 + 3, 6, 6 + 1), "ez"), OutputPoint((6 - 3, 6, 6 + 1), "ez"),
+                         OutputPoint((6, 6 + 3, 6 - 2), "ez"), OutputPoint((6, 6 - 3, 6 - 2), "ez"),
+                         OutputPoint((6 + 2, 6 + 1, 6), "hy"), OutputPoint((6 - 2, 6 + 1, 6), "hy")])
+    t = build_tables(p)
+    h = oracle_handle(t)
+    h.step(1, 40)
+    out, _ = h.read_cell_outputs()
+    s = probe_series(out, t)
+    scale = np.abs(s).max()
+    assert np.abs(s).max() > 0
+    assert np.max(np.abs(s[:, 0] - s[:, 1])) < 1e-13 * scale
+    assert np.max(np.abs(s[:, 2] - s[:, 3])) < 1e-13 * scale
+    assert np.max(np.abs(s[:, 4] + s[:, 5])) < 1e-13 * scale  # Hy is odd in x
+
+
+# ---- degenerate materials and PML -----------------------------------------------------
+
+def _series(p, steps):
+    t = build_tables(p)
+    h = oracle_handle(t)
+    h.step(1, steps)
+    out, _ = h.read_cell_outputs()
+    return probe_series(out, t)
+
+
+def test_free_space_material_equals_free_space(oracle_lib):
+    """eps_r = mu_r = 1, sigma = 0 goes through the Thevenin path and must
+    reproduce the free-space branch up to rounding."""
+    base = problems.free_cube(12, 60)
+    mat = problems.free_cube(12, 60)
+    mat.materials = [VolumeMaterial("dispersive", Sfilter.constant(1.0), Sfilter.constant(1.0))]
+    mat.material_regions = [(Box(3, 9, 3, 9, 3, 9), 1)]
+    a, b = _series(base, 60), _series(mat, 60)
+    assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(a)
+
+
+def test_zero_conductivity_pml_equals_free_space(oracle_lib):
+    """R_pml = 1 gives sigma = 0: the PML update reduces to free space."""
+    base = problems.free_cube(14, 70)
+    base.reflection = (0.0,) * 6
+    pml = problems.free_cube(14, 70)
+    pml.reflection = (0.0,) * 6
+    pml.pml = PML((3,) * 6, (1.0,) * 6, 2)
+    pml.outputs = [OutputPoint((9, 6, 10), "ez"), OutputPoint((9, 6, 10), "hx"), OutputPoint((2, 7, 7), "ez"),
+                   OutputPoint((7, 7, 13), "hy")]
+    base.outputs = list(pml.outputs)
+    a, b = _series(base, 70), _series(pml, 70)
+    assert np.linalg.norm(a) > 0
+    assert np.linalg.norm(a - b) <= 1e-11 * np.linalg.norm(a)
+
+
+def test_pec_and_pmc_cells(oracle_lib):
+    t = build_tables(problems.special_mix())
+    h = oracle_handle(t, 50)
+    h.step(1, 50)
+    out, _ = h.read_cell_outputs()
+    s = probe_series(out, t)
+    assert np.all(s[:, 6] == 0.0)  # Ex in a PEC cell
+    assert np.all(s[:, 7] == 0.0)  # Hx in a PMC cell
+    assert np.abs(s[:, 8]).max() > 0
+
+
+def test_pec_face_zero_tangential_e(oracle_lib):
+    """A PEC surface face leaves zero tangential E on both of its sides."""
+    from oracle import oracle
+
+    from ggi_tlm_amd.constants import FACE_XMIN
+    p = problems.free_cube(10, 30)
+    p.surface_materials = ["pec"]
+    p.surface_faces = [(np.array([[6, 5, 5, FACE_XMIN]]), 1)]
+    p.outputs.append(OutputPoint((6, 5, 5), "ey", point="xmin"))
+    t = build_tables(p)
+    h = oracle_handle(t, 30)
+    h.step(1, 30)
+    fo = oracle.face_outputs(h)
+    assert fo.shape == (1, 2, 6)
+    assert np.all(fo[0, :, 1:3] == 0.0)  # Ey, Ez both sides
+    assert np.abs(fo[0, :, 4:6]).max() > 0  # surface current -> tangential H
+
+
+# ---- filters ------------------------------------------------------------------------
+
+@pytest.mark.parametrize("eps", [
+    Sfilter.constant(4.0),
+    Sfilter(1e10, [2.0, 3.1827e-1, 4.0518e-1], [1.0, 3.1827e-1, 4.0518e-1]),
+    debye_lorentz_eps(3.0, [("debye", 2.0, 2 * np.pi * 1e9), ("debye", 1.0, 2 * np.pi * 5e9)]),
+    debye_lorentz_eps(2.5, [("lorentz", 1.5, 2 * np.pi * 3e9, 0.2)]),
+])
+def test_bilinear_frequency_identity(eps):
+    """H_z(exp(j w T)) = H_s(j (2/T) tan(w T / 2)) for the stub filters."""
+    dl = 1e-3
+    T = dl / (2 * C0)
+    zs = electric_stub_impedance(eps, EPS0 * dl)
+    z = bilinear_s_to_z(zs, T)
+    f = np.logspace(7, 10.5, 25)
+    warped = (2 / T) * np.tan(2 * np.pi * f * T / 2) / (2 * np.pi)
+    hz = zfilter_response(z, f, T)
+    hs = sfilter_response(zs, warped)
+    assert np.all(np.abs(hz - hs) <= 1e-8 * np.abs(hs))
+    # the s filter itself is 1/(s C (eps_r - 1))
+    er = sfilter_response(eps, f)
+    expect = 1.0 / (1j * 2 * np.pi * f * EPS0 * dl * (er - 1.0))
+    assert np.allclose(sfilter_response(zs, f), expect, rtol=1e-10)
+
+
+def test_fast_slow_split():
+    zs = electric_stub_impedance(Sfilter.constant(4.0), EPS0 * 1e-3)
+    z = bilinear_s_to_z(zs, 1e-3 / (2 * C0))
+    f, slow = fast_slow(z)
+    assert slow.a[0] == 0.0
+    assert f == z.a[0] / z.b[0]
+    # constant eps_r: integrator 1/(s C), C = 3 eps0 dl -> f_fast = T/(2C)
+    assert math.isclose(f, (1e-3 / (2 * C0)) / (2 * 3 * EPS0 * 1e-3), rel_tol=1e-12)
+
+
+def test_magnetic_stub_is_inductor():
+    mu = Sfilter.constant(3.0)
+    zs = magnetic_stub_impedance(mu, 1e-6)
+    f = np.array([1e8, 1e9])
+    assert np.allclose(sfilter_response(zs, f), 1j * 2 * np.pi * f * 1e-6 * 2.0, rtol=1e-12)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_zfilter_recursion_is_lfilter(order):
+    """The oracle's timeshift / evaluate sequence is the IIR filter a/b
+    (scipy.signal.lfilter), the history-only evaluation changing nothing."""
+    from scipy.signal import lfilter
+
+    from oracle import oracle
+    rng = np.random.default_rng(order)
+    b = np.concatenate([[1.0], rng.uniform(-0.3, 0.3, order)])
+    a = rng.uniform(-1, 1, order + 1)
+    G = rng.standard_normal(64)
+    assert np.allclose(oracle.zfilter_sequence(a, b, G), lfilter(a, b, G), rtol=1e-12, atol=1e-14)
+
+
+# ---- excitation, DFT, PML profile -----------------------------------------------------
+
+def test_gaussian_table_closed_form():
+    dt = 1e-11
+    v, vf = excitation_tables([ExcitationFunction("gaussian", (2.0, 5 * dt, 30 * dt))], 100, dt)
+    t = np.arange(100) * dt
+    ref = 2.0 * np.exp(-((t - 30 * dt) / (5 * dt)) ** 2)
+    ref[np.abs(t - 30 * dt) >= 25 * dt] = 0.0
+    assert np.allclose(v[:, 0], ref, rtol=1e-15, atol=0)
+    t2 = t + dt / 2
+    ref2 = 2.0 * np.exp(-((t2 - 30 * dt) / (5 * dt)) ** 2)
+    ref2[np.abs(t2 - 30 * dt) >= 25 * dt] = 0.0
+    assert np.allclose(vf[:, 0], ref2, rtol=1e-15, atol=0)
+    assert v[5, 0] == 0.0 and v[6, 0] > 0.0  # |t-delay| < 5 width is strict
+
+
+def test_probe_dft_matches_numpy(oracle_lib):
+    freqs = np.linspace(2e8, 4e9, 9)
+    t = build_tables(problems.free_cube(12, 80, dft=freqs))
+    h = oracle_handle(t)
+    h.step(1, 80)
+    out, _ = h.read_cell_outputs()
+    s = probe_series(out, t)
+    tn = np.arange(80) * t.dt
+    X = np.array([[np.sum(s[:, p] * np.exp(-1j * 2 * np.pi * f * tn) * t.dt) for f in freqs]
+                  for p in range(s.shape[1])])
+    D = h.read_dft()
+    assert np.allclose(D, X, rtol=1e-11, atol=1e-11 * np.abs(X).max())
+
+
+def test_pml_profile_meets_target_reflection():
+    """Normal-incidence reflection of the graded PML, exp(-(2/c) sum sigma dl)
+    with sigma normalised by eps0, equals the requested R."""
+    for R, order, npml in ((1e-4, 2, 8), (1e-3, 3, 5)):
+        p = Problem(nx=3 * npml, ny=4, nz=4, dl=5e-3, n_timesteps=1, reflection=(0.0,) * 6,
+                    pml=PML((npml, npml, 0, 0, 0, 0), (R,) * 6, order))
+        t = build_tables(p)
+        sx = sorted(set(float(s) for s in t.pml_params["sx"] if s > 0))  # one side's layers
+        round_trip = 2 * sum(sx) * p.dl / C0
+        assert len(sx) == npml
+        assert math.isclose(math.exp(-round_trip), R, rel_tol=1e-10)
+
+
+def test_table_ranks_scan_order():
+    t = build_tables(problems.special_mix())
+    codes = t.cell_code[t.cell_code > 0]
+    assert np.array_equal(codes, np.arange(1, t.n_special_cells + 1))
+    fc = t.face_code[t.face_code > 0]
+    assert np.array_equal(fc, np.arange(1, t.n_special_faces + 1))
+    assert t.n_face_excitation_rows == 1 and t.n_cell_excitation_rows == 3  # (7,7,7) twice, (11,6,5), (7,8,7)
