@@ -66,6 +66,35 @@ def build_copy_probe(verbose: bool = False) -> str:
     return COPY_BIN
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_BIN = os.path.join(FORTRAN_DIR, "tlm_gpu_cavity")
+
+
+def amdflang() -> str:
+    for c in (os.environ.get("FC_AMD"), "/opt/rocm/bin/amdflang", shutil.which("amdflang")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("amdflang not found")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The iso_c_binding module and the Fortran host driver, linked against
+    the engine library (rpath to this directory)."""
+    mod_dir = os.path.join(FORTRAN_DIR, "_build")
+    os.makedirs(mod_dir, exist_ok=True)
+    cmd = [amdflang(), "-O2", "-cpp", "-J", mod_dir,
+           os.path.join(FORTRAN_DIR, "tlm_gpu_iface.F90"), os.path.join(FORTRAN_DIR, "tlm_gpu_cavity.F90"),
+           "-L", HERE, "-ltlmscn", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_BIN]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=mod_dir)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("amdflang failed building the Fortran driver")
+    return FORTRAN_BIN
+
+
 if __name__ == "__main__":
     print(build_engine(verbose=True))
     print(build_copy_probe(verbose=True))
+    print(build_fortran(verbose=True))

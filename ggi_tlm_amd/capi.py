@@ -61,12 +61,13 @@ SIGNATURES = {
     "set_output_capacity": (C.c_int, [_h, _i32]),
     "read_cell_outputs": (C.c_int, [_h, _p, _i32, C.POINTER(_i32), C.POINTER(_i32)]),
     "read_dft": (C.c_int, [_h, _p]),
+    "set_schedule": (C.c_int, [_h, _i32, _i32, _i32, _i32]),
     "set_profiling": (C.c_int, [_h, _i32]),
     "read_kernel_times": (C.c_int, [_h, _p, _p]),
     "kernel_bytes": (C.c_int, [_h, _p, _p]),
 }
 
-KERNEL_CLASSES = ("bulk", "pml", "generic", "dft")
+KERNEL_CLASSES = ("bulk", "pml", "generic", "dft", "material")
 
 EXPORTED = tuple("tlm_" + n for n in SIGNATURES)
 
@@ -237,20 +238,23 @@ class Handle:
         self._check(self.lib.fn["read_dft"](self.h, _ptr(out)))
         return out[:npnt, :nfreq, 0] + 1j * out[:npnt, :nfreq, 1]
 
+    def set_schedule(self, variant: int, rows: int, zc: int, xcd_remap: bool = True):
+        self._check(self.lib.fn["set_schedule"](self.h, int(variant), int(rows), int(zc), 1 if xcd_remap else 0))
+
     def set_profiling(self, on: bool):
         self._check(self.lib.fn["set_profiling"](self.h, 1 if on else 0))
 
     def read_kernel_times(self):
         """{class: (launches, total_ms)} since the previous read."""
-        n = np.zeros(4, dtype=np.int64)
-        ms = np.zeros(4, dtype=np.float64)
+        n = np.zeros(len(KERNEL_CLASSES), dtype=np.int64)
+        ms = np.zeros(len(KERNEL_CLASSES), dtype=np.float64)
         self._check(self.lib.fn["read_kernel_times"](self.h, _ptr(n), _ptr(ms)))
         return {k: (int(n[i]), float(ms[i])) for i, k in enumerate(KERNEL_CLASSES)}
 
     def kernel_bytes(self):
         """{class: (cells, algorithmic bytes per launch)}."""
-        c = np.zeros(4, dtype=np.int64)
-        b = np.zeros(4, dtype=np.int64)
+        c = np.zeros(len(KERNEL_CLASSES), dtype=np.int64)
+        b = np.zeros(len(KERNEL_CLASSES), dtype=np.int64)
         self._check(self.lib.fn["kernel_bytes"](self.h, _ptr(c), _ptr(b)))
         return {k: (int(c[i]), int(b[i])) for i, k in enumerate(KERNEL_CLASSES)}
 

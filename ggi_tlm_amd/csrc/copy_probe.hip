@@ -15,6 +15,24 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a, floa
   for (; i < n; i += stride) b[i] = a[i];
 }
 
+// one float4 per thread, no loop
+__global__ __launch_bounds__(256) void k_copy1(const float4* __restrict__ a, float4* __restrict__ b, long long n) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+// twelve float planes in, twelve out, one cell per thread: the access
+// pattern of the link planes without neighbour reads
+__global__ __launch_bounds__(256) void k_planes12(const float* __restrict__ a, float* __restrict__ b, long long P) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  float v[12];
+#pragma unroll
+  for (int p = 0; p < 12; p++) v[p] = a[p * P + i];
+#pragma unroll
+  for (int p = 0; p < 12; p++) b[p * P + i] = v[(p + 1) % 12];
+}
+
 int main(int argc, char** argv) {
   long long mib = argc > 1 ? atoll(argv[1]) : 1024;
   int reps = argc > 2 ? atoi(argv[2]) : 20;
@@ -42,6 +60,31 @@ int main(int argc, char** argv) {
   double per = ms / reps;
   printf("{\"copy_bytes_read\": %lld, \"copy_bytes_written\": %lld, \"avg_ms\": %.5f, \"GB_s\": %.1f}\n", bytes, bytes,
          per, 2.0 * bytes / (per * 1e-3) / 1e9);
+  // one float4 per thread
+  {
+    int g1 = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_copy1, dim3(g1), dim3(256), 0, 0, a, b, n);
+    hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy1, dim3(g1), dim3(256), 0, 0, a, b, n);
+    hipEventRecord(e1, 0);
+    hipEventSynchronize(e1);
+    hipEventElapsedTime(&ms, e0, e1);
+    per = ms / reps;
+    printf("{\"kernel\": \"copy1\", \"avg_ms\": %.5f, \"GB_s\": %.1f}\n", per, 2.0 * bytes / (per * 1e-3) / 1e9);
+  }
+  // twelve planes
+  {
+    long long P = bytes / 4 / 12;
+    int g1 = (int)((P + 255) / 256);
+    hipLaunchKernelGGL(k_planes12, dim3(g1), dim3(256), 0, 0, (const float*)a, (float*)b, P);
+    hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_planes12, dim3(g1), dim3(256), 0, 0, (const float*)a, (float*)b, P);
+    hipEventRecord(e1, 0);
+    hipEventSynchronize(e1);
+    hipEventElapsedTime(&ms, e0, e1);
+    per = ms / reps;
+    printf("{\"kernel\": \"planes12\", \"avg_ms\": %.5f, \"GB_s\": %.1f}\n", per, 2.0 * 48 * P / (per * 1e-3) / 1e9);
+  }
   hipFree(a);
   hipFree(b);
   return 0;

@@ -45,14 +45,43 @@ enum {
   Vx_ymax, Vz_ymax, Vx_zmin, Vy_zmin, Vx_zmax, Vy_zmax
 };
 
-// PML state words per cell (PML_module.F90:117-129), SoA: state[w*n + i]
+// PML state words per cell (PML_module.F90:117-129), SoA: state[w*n + i].
+// Plain form: the reference's 24 words, Vi(12) kept port by port.
 enum {
   PS_Vxt, PS_Vyt, PS_Vzt, PS_Ix, PS_Iy, PS_Iz,
   PS_Vi,  // 12 words
   PS_Vyxt = PS_Vi + 12, PS_Vzxt, PS_Vxyt, PS_Vzyt, PS_Vyzt, PS_Vxzt,
   PS_N    // 24 words streamed by every PML cell
 };
+// Precision form: the previous step's incident voltages Vi enter the update
+// only as the six pair sums (Vznx+Vzpx), (Vynx+Vypx), (Vxny+Vxpy),
+// (Vzny+Vzpy), (Vynz+Vypz), (Vxnz+Vxpz) (scatter_PML.F90:127-141), so those
+// are kept instead, in FP64: 18 words per cell.
+enum {
+  PC_Vxt, PC_Vyt, PC_Vzt, PC_Ix, PC_Iy, PC_Iz,
+  PC_Szx, PC_Syx, PC_Sxy, PC_Szy, PC_Syz, PC_Sxz,
+  PC_Vyxt, PC_Vzxt, PC_Vxyt, PC_Vzyt, PC_Vyzt, PC_Vxzt,
+  PC_N
+};
 // Ixt,Iyt,Izt live in a separate array, touched only by output cells.
+
+// Packed per-cell word of the PML and material class lists.
+constexpr int kCellBits = 28;
+constexpr int32_t kCellMask = (1 << kCellBits) - 1;
+constexpr int32_t kFlagOut = 1 << 28;  // cell has an output row
+constexpr int32_t kFlagFd = 1 << 29;   // cell has a special-face descriptor
+
+// Value for key in a sorted (key, value) table, -1 if absent.
+__device__ inline int32_t sorted_lookup(const int2* tab, int n, int32_t key) {
+  int lo = 0, hi = n - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    int k = tab[mid].x;
+    if (k == key) return tab[mid].y;
+    if (k < key) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
 
 struct Dev {
   int nx, ny, nz;
@@ -64,15 +93,22 @@ struct Dev {
   const uint32_t* mask;  // 1 = special cell (not bulk)
   // pml class
   int n_pml;
-  const int32_t* pml_cell;
-  const int32_t* pml_state_idx;  // PML_cell - 1
-  const int32_t* pml_par;        // 0-based
-  const int32_t* pml_out;        // output row (0-based) or -1
-  const int32_t* pml_fd;         // face descriptor or -1
+  const int32_t* pml_cell;       // cell | kFlagOut | kFlagFd
+  const int32_t* pml_state_idx;  // PML_cell - 1, or null when it equals the list position
+  const uint16_t* pml_par;       // 0-based parameter row
+  const int2* pml_out_tab;       // (list position, output row), sorted
+  int n_pml_out;
+  const int2* pml_fd_tab;        // (list position, face descriptor), sorted
+  int n_pml_fd;
   const tlm_pml_parameter* pml_params;
-  double* pml_state;             // PS_N * n_pml_state
+  double* pml_state;             // PS_N (plain) or PC_N (precision) words * n_pml_state
   double* pml_It;                // 3 * n_pml_state (Ixt,Iyt,Izt)
   long long n_pml_state;
+  // material class
+  int n_mat;
+  const int32_t* mat_cell;
+  const uint16_t* mat_num;   // material number (1-based)
+  const int32_t* mat_filt;   // filter number (1-based), 0 when not dispersive
   // generic class
   int n_gen;
   const int32_t* gen_cell;
@@ -375,131 +411,153 @@ __device__ inline void zf_timeshift(double* w, long long n, long long f, int ord
 // kernels
 // ---------------------------------------------------------------------------
 
-// Tile schedule of the bulk kernel: 64 x BY cell tiles of an xy-plane, each
-// block walking ZC consecutive planes.  Hardware dispatches workgroup b to
-// XCD b % 8; the logical tile of workgroup b is chosen so that each XCD
-// receives one contiguous range of logical tiles (x fastest, then y, then
-// z-chunk), which keeps the x/y halo rows of neighbouring tiles in the same
-// XCD's L2.
+// Tile schedule of the bulk kernels: 64 x by cell tiles of an xy-plane.
+// Hardware dispatches workgroup b to XCD b % 8; with xcd_remap the logical
+// tile of workgroup b is chosen so that each XCD receives one contiguous
+// range of logical tiles (x fastest, then y, then z or z-chunk), so the
+// neighbour planes and rows a tile reads were fetched into the same XCD's
+// L2 by the tiles just before it.
 struct BulkMap {
-  int nbx, nby, nchunks, zc, nblocks, xcd_remap;
+  int nbx, nby, by;     // tiles in x and y, tile height (rows)
+  int nchunks, zc;      // z-chunks of zc planes (zc = 1: one plane per block)
+  int nblocks, xcd_remap;
 };
 
-constexpr int kBulkBY = 4;
-
-template <typename T>
-__device__ inline T shfl_up1(T v) { return __shfl_up(v, 1, 64); }
-template <typename T>
-__device__ inline T shfl_down1(T v) { return __shfl_down(v, 1, 64); }
-
-// Bulk free-space cells, marching along z.  Per cell and plane:
-//   own 12 ports       prefetched one plane ahead (coalesced along x)
-//   zmin-face ports    the previous plane's own Vx_zmax, Vy_zmax (registers)
-//   zmax-face ports    part of the next plane's prefetch
-//   x-face ports       neighbouring lanes (cross-lane moves), loads only at
-//                      the wavefront edges
-//   y-face ports       loads of the adjacent rows (shared through L1/L2 with
-//                      the block's other waves and the neighbouring tiles)
-// Special cells (bitmask) keep the register pipeline but are not written.
-template <typename T, bool PLAIN, bool PULL>
-__global__ __launch_bounds__(256) void k_bulk(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                             PullArgs a, BulkMap m) {
+__device__ inline void bulk_tile(const BulkMap& m, int& bx, int& byi, int& chunk) {
   int b = blockIdx.x, L = b;
   if (m.xcd_remap) {
     const int per = m.nblocks >> 3;
     L = (b & 7) * per + (b >> 3);
   }
-  const int bx = L % m.nbx;
+  bx = L % m.nbx;
   const int t = L / m.nbx;
-  const int by = t % m.nby;
-  const int chunk = t / m.nby;
-  const int lane = threadIdx.x;
-  const int x = bx * 64 + lane;
-  const int y = by * kBulkBY + threadIdx.y;
-  const int z0 = chunk * m.zc;
-  const int z1 = min(z0 + m.zc, d.nz);
-  const bool valid = x < d.nx && y < d.ny;
-  const long long P = d.P;
-  const long long sxy = (long long)d.nx * d.ny;
-  const long long c0 = valid ? x + (long long)d.nx * y : 0;  // offset inside a plane
+  byi = t % m.nby;
+  chunk = t / m.nby;
+}
+
+// Incident voltages of a bulk cell from its own scattered values o[] and the
+// neighbours' facing ports (x-1, x+1, y-1, y+1, z-1, z+1 pairs).
+template <bool PLAIN>
+__device__ inline void bulk_incident(const Dev& d, const PullArgs& a, int x, int y, int z, const double o[12],
+                                     double xa, double xb, double xc, double xd, double ya, double yb,
+                                     double yc, double yd, double za, double zb, double zc_, double zd,
+                                     double v[12]) {
   const double Rx0 = a.apply_r ? d.R[0] : 1.0, Rx1 = a.apply_r ? d.R[1] : 1.0;
   const double Ry0 = a.apply_r ? d.R[2] : 1.0, Ry1 = a.apply_r ? d.R[3] : 1.0;
   const double Rz0 = a.apply_r ? d.R[4] : 1.0, Rz1 = a.apply_r ? d.R[5] : 1.0;
+  if (x == 0) { v[Vy_xmin] = o[Vy_xmin] * Rx0; v[Vz_xmin] = o[Vz_xmin] * Rx0; }
+  else { v[Vy_xmin] = free_face<PLAIN>(o[Vy_xmin], xa); v[Vz_xmin] = free_face<PLAIN>(o[Vz_xmin], xb); }
+  if (x == d.nx - 1) { v[Vy_xmax] = o[Vy_xmax] * Rx1; v[Vz_xmax] = o[Vz_xmax] * Rx1; }
+  else { v[Vy_xmax] = free_face<PLAIN>(o[Vy_xmax], xc); v[Vz_xmax] = free_face<PLAIN>(o[Vz_xmax], xd); }
+  if (y == 0) { v[Vx_ymin] = o[Vx_ymin] * Ry0; v[Vz_ymin] = o[Vz_ymin] * Ry0; }
+  else { v[Vx_ymin] = free_face<PLAIN>(o[Vx_ymin], ya); v[Vz_ymin] = free_face<PLAIN>(o[Vz_ymin], yb); }
+  if (y == d.ny - 1) { v[Vx_ymax] = o[Vx_ymax] * Ry1; v[Vz_ymax] = o[Vz_ymax] * Ry1; }
+  else { v[Vx_ymax] = free_face<PLAIN>(o[Vx_ymax], yc); v[Vz_ymax] = free_face<PLAIN>(o[Vz_ymax], yd); }
+  if (z == 0) { v[Vx_zmin] = o[Vx_zmin] * Rz0; v[Vy_zmin] = o[Vy_zmin] * Rz0; }
+  else { v[Vx_zmin] = free_face<PLAIN>(o[Vx_zmin], za); v[Vy_zmin] = free_face<PLAIN>(o[Vy_zmin], zb); }
+  if (z == d.nz - 1) { v[Vx_zmax] = o[Vx_zmax] * Rz1; v[Vy_zmax] = o[Vy_zmax] * Rz1; }
+  else { v[Vx_zmax] = free_face<PLAIN>(o[Vx_zmax], zc_); v[Vy_zmax] = free_face<PLAIN>(o[Vy_zmax], zd); }
+}
 
-  T cur[12];
-  T pza = 0, pzb = 0;  // Vx_zmax, Vy_zmax of plane z-1
+// Bulk free-space cells, one cell per thread (zc = 1): block = 64 x by cells
+// of one plane.  Neighbour ports are loaded directly; they are the block's
+// own rows or the rows/planes of tiles that ran just before on the same XCD.
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_bulk_cell(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                  PullArgs a, BulkMap m) {
+  int bx, byi, z;
+  bulk_tile(m, bx, byi, z);
+  const int x = bx * 64 + threadIdx.x;
+  const int y = byi * m.by + threadIdx.y;
+  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
+  const long long P = d.P;
+  const long long sxy = (long long)d.nx * d.ny;
+  const long long c = x + (long long)d.nx * (y + (long long)d.ny * z);
+  if ((d.mask[c >> 5] >> (c & 31)) & 1u) return;
+  double v[12];
+  if (PULL) {
+    double o[12];
 #pragma unroll
-  for (int p = 0; p < 12; p++) cur[p] = valid ? Lin[p * P + c0 + z0 * sxy] : (T)0;
-  if (PULL && valid && z0 > 0) {
+    for (int p = 0; p < 12; p++) o[p] = (double)Lin[p * P + c];
+    double xa = 0, xb = 0, xc = 0, xd = 0, ya = 0, yb = 0, yc = 0, yd = 0, za = 0, zb = 0, zc_ = 0, zd = 0;
+    if (x > 0) { xa = Lin[Vy_xmax * P + c - 1]; xb = Lin[Vz_xmax * P + c - 1]; }
+    if (x + 1 < d.nx) { xc = Lin[Vy_xmin * P + c + 1]; xd = Lin[Vz_xmin * P + c + 1]; }
+    if (y > 0) { ya = Lin[Vx_ymax * P + c - d.nx]; yb = Lin[Vz_ymax * P + c - d.nx]; }
+    if (y + 1 < d.ny) { yc = Lin[Vx_ymin * P + c + d.nx]; yd = Lin[Vz_ymin * P + c + d.nx]; }
+    if (z > 0) { za = Lin[Vx_zmax * P + c - sxy]; zb = Lin[Vy_zmax * P + c - sxy]; }
+    if (z + 1 < d.nz) { zc_ = Lin[Vx_zmin * P + c + sxy]; zd = Lin[Vy_zmin * P + c + sxy]; }
+    bulk_incident<PLAIN>(d, a, x, y, z, o, xa, xb, xc, xd, ya, yb, yc, yd, za, zb, zc_, zd, v);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 12; p++) v[p] = (double)Lin[p * P + c];
+  }
+  if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
+  store_cell(Lout, P, c, v);
+}
+
+// Bulk free-space cells, marching zc planes per block.  The zmin-face ports
+// (Vx_zmax, Vy_zmax of plane z-1) are the previous iteration's own values
+// and the zmax-face ports (Vx_zmin, Vy_zmin of plane z+1) are kept for the
+// next iteration, so each plane's links are read once per block; x and y
+// neighbour ports are loaded (rows of the same tile or of tiles that run
+// alongside on the same XCD).  Special cells keep the register pipeline
+// but are not written.
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_bulk_z(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                               PullArgs a, BulkMap m) {
+  int bx, byi, chunk;
+  bulk_tile(m, bx, byi, chunk);
+  const int x = bx * 64 + threadIdx.x;
+  const int y = byi * m.by + threadIdx.y;
+  if (x >= d.nx || y >= d.ny) return;
+  const int z0 = chunk * m.zc;
+  const int z1 = min(z0 + m.zc, d.nz);
+  const long long P = d.P;
+  const long long sxy = (long long)d.nx * d.ny;
+  const long long c0 = x + (long long)d.nx * y;
+  T pza = 0, pzb = 0;  // Vx_zmax, Vy_zmax of plane z-1
+  T nza = 0, nzb = 0;  // Vx_zmin, Vy_zmin of plane z (loaded by the previous iteration)
+  if (PULL && z0 > 0) {
     pza = Lin[Vx_zmax * P + c0 + (z0 - 1) * sxy];
     pzb = Lin[Vy_zmax * P + c0 + (z0 - 1) * sxy];
   }
+  nza = Lin[Vx_zmin * P + c0 + z0 * sxy];
+  nzb = Lin[Vy_zmin * P + c0 + z0 * sxy];
   for (int z = z0; z < z1; z++) {
     const long long c = c0 + z * sxy;
-    T nxt[12];
-    const bool has_next = z + 1 < z1;
+    double o[12];
 #pragma unroll
-    for (int p = 0; p < 12; p++) nxt[p] = (T)0;
-    if (valid) {
-      if (has_next) {
-#pragma unroll
-        for (int p = 0; p < 12; p++) nxt[p] = Lin[p * P + c + sxy];
-      } else if (PULL && z + 1 < d.nz) {
-        nxt[Vx_zmin] = Lin[Vx_zmin * P + c + sxy];
-        nxt[Vy_zmin] = Lin[Vy_zmin * P + c + sxy];
-      }
+    for (int p = 0; p < 12; p++)
+      if (p != Vx_zmin && p != Vy_zmin) o[p] = (double)Lin[p * P + c];
+    o[Vx_zmin] = (double)nza;
+    o[Vy_zmin] = (double)nzb;
+    T qa = 0, qb = 0;
+    if (z + 1 < d.nz) {
+      qa = Lin[Vx_zmin * P + c + sxy];
+      qb = Lin[Vy_zmin * P + c + sxy];
     }
-    double v[12];
-    if (PULL) {
-      // x neighbours from the adjacent lanes (all lanes take part)
-      T xa = shfl_up1(cur[Vy_xmax]), xb = shfl_up1(cur[Vz_xmax]);    // cell x-1
-      T xc = shfl_down1(cur[Vy_xmin]), xd = shfl_down1(cur[Vz_xmin]);  // cell x+1
-      if (valid) {
-        if (lane == 0 && x > 0) {
-          xa = Lin[Vy_xmax * P + c - 1];
-          xb = Lin[Vz_xmax * P + c - 1];
-        }
-        if (lane == 63 && x + 1 < d.nx) {
-          xc = Lin[Vy_xmin * P + c + 1];
-          xd = Lin[Vz_xmin * P + c + 1];
-        }
-        T ya = 0, yb = 0, yc = 0, yd = 0;
-        if (y > 0) {
-          ya = Lin[Vx_ymax * P + c - d.nx];
-          yb = Lin[Vz_ymax * P + c - d.nx];
-        }
-        if (y + 1 < d.ny) {
-          yc = Lin[Vx_ymin * P + c + d.nx];
-          yd = Lin[Vz_ymin * P + c + d.nx];
-        }
-        double o[12];
+    const bool bulk = !((d.mask[c >> 5] >> (c & 31)) & 1u);
+    if (bulk) {
+      double v[12];
+      if (PULL) {
+        double xa = 0, xb = 0, xc = 0, xd = 0, ya = 0, yb = 0, yc = 0, yd = 0;
+        if (x > 0) { xa = Lin[Vy_xmax * P + c - 1]; xb = Lin[Vz_xmax * P + c - 1]; }
+        if (x + 1 < d.nx) { xc = Lin[Vy_xmin * P + c + 1]; xd = Lin[Vz_xmin * P + c + 1]; }
+        if (y > 0) { ya = Lin[Vx_ymax * P + c - d.nx]; yb = Lin[Vz_ymax * P + c - d.nx]; }
+        if (y + 1 < d.ny) { yc = Lin[Vx_ymin * P + c + d.nx]; yd = Lin[Vz_ymin * P + c + d.nx]; }
+        bulk_incident<PLAIN>(d, a, x, y, z, o, xa, xb, xc, xd, ya, yb, yc, yd, (double)pza, (double)pzb,
+                             (double)qa, (double)qb, v);
+      } else {
 #pragma unroll
-        for (int p = 0; p < 12; p++) o[p] = (double)cur[p];
-        if (x == 0) { v[Vy_xmin] = o[Vy_xmin] * Rx0; v[Vz_xmin] = o[Vz_xmin] * Rx0; }
-        else { v[Vy_xmin] = free_face<PLAIN>(o[Vy_xmin], (double)xa); v[Vz_xmin] = free_face<PLAIN>(o[Vz_xmin], (double)xb); }
-        if (x == d.nx - 1) { v[Vy_xmax] = o[Vy_xmax] * Rx1; v[Vz_xmax] = o[Vz_xmax] * Rx1; }
-        else { v[Vy_xmax] = free_face<PLAIN>(o[Vy_xmax], (double)xc); v[Vz_xmax] = free_face<PLAIN>(o[Vz_xmax], (double)xd); }
-        if (y == 0) { v[Vx_ymin] = o[Vx_ymin] * Ry0; v[Vz_ymin] = o[Vz_ymin] * Ry0; }
-        else { v[Vx_ymin] = free_face<PLAIN>(o[Vx_ymin], (double)ya); v[Vz_ymin] = free_face<PLAIN>(o[Vz_ymin], (double)yb); }
-        if (y == d.ny - 1) { v[Vx_ymax] = o[Vx_ymax] * Ry1; v[Vz_ymax] = o[Vz_ymax] * Ry1; }
-        else { v[Vx_ymax] = free_face<PLAIN>(o[Vx_ymax], (double)yc); v[Vz_ymax] = free_face<PLAIN>(o[Vz_ymax], (double)yd); }
-        if (z == 0) { v[Vx_zmin] = o[Vx_zmin] * Rz0; v[Vy_zmin] = o[Vy_zmin] * Rz0; }
-        else { v[Vx_zmin] = free_face<PLAIN>(o[Vx_zmin], (double)pza); v[Vy_zmin] = free_face<PLAIN>(o[Vy_zmin], (double)pzb); }
-        if (z == d.nz - 1) { v[Vx_zmax] = o[Vx_zmax] * Rz1; v[Vy_zmax] = o[Vy_zmax] * Rz1; }
-        else { v[Vx_zmax] = free_face<PLAIN>(o[Vx_zmax], (double)nxt[Vx_zmin]); v[Vy_zmax] = free_face<PLAIN>(o[Vy_zmax], (double)nxt[Vy_zmin]); }
+        for (int p = 0; p < 12; p++) v[p] = o[p];
       }
-    } else {
-#pragma unroll
-      for (int p = 0; p < 12; p++) v[p] = (double)cur[p];
-    }
-    if (valid && !((d.mask[c >> 5] >> (c & 31)) & 1u)) {
       if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
       store_cell(Lout, P, c, v);
     }
-    pza = cur[Vx_zmax];
-    pzb = cur[Vy_zmax];
-#pragma unroll
-    for (int p = 0; p < 12; p++) cur[p] = nxt[p];
+    pza = (T)o[Vx_zmax];
+    pzb = (T)o[Vy_zmax];
+    nza = qa;
+    nzb = qb;
   }
 }
 
@@ -509,14 +567,16 @@ __global__ __launch_bounds__(256) void k_pml(Dev d, const T* __restrict__ Lin, T
                                             PullArgs a, int slot) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.n_pml) return;
-  const long long c = d.pml_cell[i];
+  const int32_t cw = d.pml_cell[i];
+  const long long c = cw & kCellMask;
   const int nxy = d.nx * d.ny;
   const int z = (int)(c / nxy);
   const int rem = (int)(c - (long long)z * nxy);
   const int y = rem / d.nx, x = rem - y * d.nx;
   double v[12];
   if (PULL) {
-    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, d.pml_fd[i], a, v);
+    const int fdi = (cw & kFlagFd) ? sorted_lookup(d.pml_fd_tab, d.n_pml_fd, i) : -1;
+    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, fdi, a, v);
   } else {
 #pragma unroll
     for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
@@ -529,19 +589,21 @@ __global__ __launch_bounds__(256) void k_pml(Dev d, const T* __restrict__ Lin, T
   const double Csx2 = Pp.Csx2, Csy2 = Pp.Csy2, Csz2 = Pp.Csz2;
   const double Csy_sz = Pp.Csy_sz, Csz_sx = Pp.Csz_sx, Csx_sy = Pp.Csx_sy;
   const long long n = d.n_pml_state;
-  const long long si = d.pml_state_idx[i];
+  const long long si = d.pml_state_idx ? d.pml_state_idx[i] : i;
   double* S = d.pml_state;
-  // STAGE 2 (state reads)
-  const double last_Ix = S[PS_Ix * n + si], last_Iy = S[PS_Iy * n + si], last_Iz = S[PS_Iz * n + si];
-  double lv[12];
-#pragma unroll
-  for (int k = 0; k < 12; k++) lv[k] = S[(PS_Vi + k) * n + si];
-  const double last_Vxt = S[PS_Vxt * n + si], last_Vyt = S[PS_Vyt * n + si], last_Vzt = S[PS_Vzt * n + si];
-  const double last_Vyxt = S[PS_Vyxt * n + si], last_Vzxt = S[PS_Vzxt * n + si];
-  const double last_Vxyt = S[PS_Vxyt * n + si], last_Vzyt = S[PS_Vzyt * n + si];
-  const double last_Vyzt = S[PS_Vyzt * n + si], last_Vxzt = S[PS_Vxzt * n + si];
   // PML port aliases (PML_module.F90:60-71), 0-based
   enum { Vxny = 0, Vxnz, Vxpy, Vxpz, Vynx, Vynz, Vypx, Vypz, Vznx, Vzny, Vzpx, Vzpy };
+  // STAGE 2 (state reads)
+  constexpr int wIx = PLAIN ? PS_Ix : PC_Ix, wIy = PLAIN ? PS_Iy : PC_Iy, wIz = PLAIN ? PS_Iz : PC_Iz;
+  constexpr int wVxt = PLAIN ? PS_Vxt : PC_Vxt, wVyt = PLAIN ? PS_Vyt : PC_Vyt, wVzt = PLAIN ? PS_Vzt : PC_Vzt;
+  constexpr int wVyxt = PLAIN ? PS_Vyxt : PC_Vyxt, wVzxt = PLAIN ? PS_Vzxt : PC_Vzxt;
+  constexpr int wVxyt = PLAIN ? PS_Vxyt : PC_Vxyt, wVzyt = PLAIN ? PS_Vzyt : PC_Vzyt;
+  constexpr int wVyzt = PLAIN ? PS_Vyzt : PC_Vyzt, wVxzt = PLAIN ? PS_Vxzt : PC_Vxzt;
+  const double last_Ix = S[wIx * n + si], last_Iy = S[wIy * n + si], last_Iz = S[wIz * n + si];
+  const double last_Vxt = S[wVxt * n + si], last_Vyt = S[wVyt * n + si], last_Vzt = S[wVzt * n + si];
+  const double last_Vyxt = S[wVyxt * n + si], last_Vzxt = S[wVzxt * n + si];
+  const double last_Vxyt = S[wVxyt * n + si], last_Vzyt = S[wVzyt * n + si];
+  const double last_Vyzt = S[wVyzt * n + si], last_Vxzt = S[wVxzt * n + si];
   // STAGE 3
   v[Vynx] = v[Vynx] * exp_y; v[Vypx] = v[Vypx] * exp_y;
   v[Vznx] = v[Vznx] * exp_z; v[Vzpx] = v[Vzpx] * exp_z;
@@ -557,24 +619,40 @@ __global__ __launch_bounds__(256) void k_pml(Dev d, const T* __restrict__ Lin, T
   const double Iy = (v[Vzpx] - v[Vxpz] - v[Vznx] + v[Vxnz]) / (2.0 * Z0);
   const double Iz = (v[Vxpy] - v[Vypx] - v[Vxny] + v[Vynx]) / (2.0 * Z0);
   // STAGE 5
-  const double Vx_sh = ax * (-(lv[Vznx] + lv[Vzpx] + lv[Vynx] + lv[Vypx]) / 2.0
-                             + Csy * ((v[Vznx] + v[Vzpx]) + (lv[Vznx] + lv[Vzpx]))
-                             + Csz * ((v[Vynx] + v[Vypx]) + (lv[Vynx] + lv[Vypx]))
-                             + Csy_sz * last_Vxt);
-  const double Vy_sh = ay * (-(lv[Vxny] + lv[Vxpy] + lv[Vzny] + lv[Vzpy]) / 2.0
-                             + Csz * ((v[Vxny] + v[Vxpy]) + (lv[Vxny] + lv[Vxpy]))
-                             + Csx * ((v[Vzny] + v[Vzpy]) + (lv[Vzny] + lv[Vzpy]))
-                             + Csz_sx * last_Vyt);
-  const double Vz_sh = az * (-(lv[Vynz] + lv[Vypz] + lv[Vxnz] + lv[Vxpz]) / 2.0
-                             + Csx * ((v[Vynz] + v[Vypz]) + (lv[Vynz] + lv[Vypz]))
-                             + Csy * ((v[Vxnz] + v[Vxpz]) + (lv[Vxnz] + lv[Vxpz]))
-                             + Csx_sy * last_Vzt);
+  double Vx_sh, Vy_sh, Vz_sh;
+  if (PLAIN) {
+    double lv[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) lv[k] = S[(PS_Vi + k) * n + si];
+    Vx_sh = ax * (-(lv[Vznx] + lv[Vzpx] + lv[Vynx] + lv[Vypx]) / 2.0
+                  + Csy * ((v[Vznx] + v[Vzpx]) + (lv[Vznx] + lv[Vzpx]))
+                  + Csz * ((v[Vynx] + v[Vypx]) + (lv[Vynx] + lv[Vypx]))
+                  + Csy_sz * last_Vxt);
+    Vy_sh = ay * (-(lv[Vxny] + lv[Vxpy] + lv[Vzny] + lv[Vzpy]) / 2.0
+                  + Csz * ((v[Vxny] + v[Vxpy]) + (lv[Vxny] + lv[Vxpy]))
+                  + Csx * ((v[Vzny] + v[Vzpy]) + (lv[Vzny] + lv[Vzpy]))
+                  + Csz_sx * last_Vyt);
+    Vz_sh = az * (-(lv[Vynz] + lv[Vypz] + lv[Vxnz] + lv[Vxpz]) / 2.0
+                  + Csx * ((v[Vynz] + v[Vypz]) + (lv[Vynz] + lv[Vypz]))
+                  + Csy * ((v[Vxnz] + v[Vxpz]) + (lv[Vxnz] + lv[Vxpz]))
+                  + Csx_sy * last_Vzt);
+  } else {
+    const double Szx = S[PC_Szx * n + si], Syx = S[PC_Syx * n + si];
+    const double Sxy = S[PC_Sxy * n + si], Szy = S[PC_Szy * n + si];
+    const double Syz = S[PC_Syz * n + si], Sxz = S[PC_Sxz * n + si];
+    Vx_sh = ax * (-(Szx + Syx) * 0.5 + Csy * ((v[Vznx] + v[Vzpx]) + Szx) + Csz * ((v[Vynx] + v[Vypx]) + Syx)
+                  + Csy_sz * last_Vxt);
+    Vy_sh = ay * (-(Sxy + Szy) * 0.5 + Csz * ((v[Vxny] + v[Vxpy]) + Sxy) + Csx * ((v[Vzny] + v[Vzpy]) + Szy)
+                  + Csz_sx * last_Vyt);
+    Vz_sh = az * (-(Syz + Sxz) * 0.5 + Csx * ((v[Vynz] + v[Vypz]) + Syz) + Csy * ((v[Vxnz] + v[Vxpz]) + Sxz)
+                  + Csx_sy * last_Vzt);
+  }
   // STAGE 6
   const double Vxt = ax * Vx + Vx_sh;
   const double Vyt = ay * Vy + Vy_sh;
   const double Vzt = az * Vz + Vz_sh;
   // STAGE 7a
-  const int orow = d.pml_out[i];
+  const int orow = (cw & kFlagOut) ? sorted_lookup(d.pml_out_tab, d.n_pml_out, i) : -1;
   double Ixt = 0.0, Iyt = 0.0, Izt = 0.0;
   if (orow >= 0) {
     const double lIxt = d.pml_It[0 * n + si], lIyt = d.pml_It[1 * n + si], lIzt = d.pml_It[2 * n + si];
@@ -597,16 +675,22 @@ __global__ __launch_bounds__(256) void k_pml(Dev d, const T* __restrict__ Lin, T
   const double Vyzt = ax * Ix * Z0 + Vyz_se;
   const double Vzyt = ax * Ix * Z0 + Vzy_se;
   // STAGE 9
-  S[PS_Ix * n + si] = Ix; S[PS_Iy * n + si] = Iy; S[PS_Iz * n + si] = Iz;
+  S[wIx * n + si] = Ix; S[wIy * n + si] = Iy; S[wIz * n + si] = Iz;
+  if (PLAIN) {
 #pragma unroll
-  for (int k = 0; k < 12; k++) S[(PS_Vi + k) * n + si] = v[k];
-  S[PS_Vxt * n + si] = Vxt; S[PS_Vyt * n + si] = Vyt; S[PS_Vzt * n + si] = Vzt;
+    for (int k = 0; k < 12; k++) S[(PS_Vi + k) * n + si] = v[k];
+  } else {
+    S[PC_Szx * n + si] = v[Vznx] + v[Vzpx]; S[PC_Syx * n + si] = v[Vynx] + v[Vypx];
+    S[PC_Sxy * n + si] = v[Vxny] + v[Vxpy]; S[PC_Szy * n + si] = v[Vzny] + v[Vzpy];
+    S[PC_Syz * n + si] = v[Vynz] + v[Vypz]; S[PC_Sxz * n + si] = v[Vxnz] + v[Vxpz];
+  }
+  S[wVxt * n + si] = Vxt; S[wVyt * n + si] = Vyt; S[wVzt * n + si] = Vzt;
   if (orow >= 0) {
     d.pml_It[0 * n + si] = Ixt; d.pml_It[1 * n + si] = Iyt; d.pml_It[2 * n + si] = Izt;
   }
-  S[PS_Vxyt * n + si] = Vxyt; S[PS_Vyxt * n + si] = Vyxt;
-  S[PS_Vzxt * n + si] = Vzxt; S[PS_Vxzt * n + si] = Vxzt;
-  S[PS_Vzyt * n + si] = Vzyt; S[PS_Vyzt * n + si] = Vyzt;
+  S[wVxyt * n + si] = Vxyt; S[wVyxt * n + si] = Vyxt;
+  S[wVzxt * n + si] = Vzxt; S[wVxzt * n + si] = Vxzt;
+  S[wVzyt * n + si] = Vzyt; S[wVyzt * n + si] = Vyzt;
   // STAGE 10
   const double Vynx_P = Vx_sh - Vyx_se, Vypx_P = Vx_sh + Vyx_se;
   const double Vznx_P = Vx_sh + Vzx_se, Vzpx_P = Vx_sh - Vzx_se;
@@ -650,33 +734,14 @@ __global__ __launch_bounds__(256) void k_pml(Dev d, const T* __restrict__ Lin, T
   store_cell(Lout, d.P, c, v);
 }
 
-// Generic special cells: scatter.F90:274-547 (Thevenin node), or the
-// free-space branch for code 0 cells that only touch a special face.
-template <typename T, bool PLAIN, bool PULL>
-__global__ __launch_bounds__(256) void k_generic(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                PullArgs a, int timestep, int slot) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n_gen) return;
-  const long long c = d.gen_cell[i];
-  const int nxy = d.nx * d.ny;
-  const int z = (int)(c / nxy);
-  const int rem = (int)(c - (long long)z * nxy);
-  const int y = rem / d.nx, x = rem - y * d.nx;
-  double v[12];
-  if (PULL) {
-    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, d.gen_fd[i], a, v);
-  } else {
-#pragma unroll
-    for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
-  }
-  if (d.gen_code[i] == 0) {
-    if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
-    store_cell(Lout, d.P, c, v);
-    return;
-  }
+// Thevenin node of a special cell (scatter.F90:274-547): materials,
+// optional point source and output row, filter update, 12-port scatter.
+// IO = false compiles out the excitation and output branches.
+template <bool IO>
+__device__ inline void node_update(const Dev& d, double v[12], int mnum, long long f, int er, int orow,
+                                   int timestep, int slot) {
   const double Z0 = kZ0, dl = d.dl;
   const double Z04 = Z0 * 4.0, Y04 = kY0 * 4.0;
-  const int mnum = d.gen_mat[i];
   const tlm_volume_material* vm = mnum ? &d.vmat[mnum - 1] : nullptr;
   const int mtype = vm ? vm->type : 0;
   double VEx = (v[Vx_ymin] + v[Vx_ymax] + v[Vx_zmin] + v[Vx_zmax]) / 2.0;
@@ -689,7 +754,6 @@ __global__ __launch_bounds__(256) void k_generic(Dev d, const T* __restrict__ Li
   double Zx = Z04, Zy = Z04, Zz = Z04;
   double Vox = 0.0, Voy = 0.0, Voz = 0.0, Ys = 0.0;
   const long long nf = d.n_filt;
-  const long long f = d.gen_filt[i];
   if (mtype == TLM_VMAT_PEC) {
     VEx = 0.0; VEy = 0.0; VEz = 0.0;
   } else if (mtype == TLM_VMAT_PMC) {
@@ -734,8 +798,7 @@ __global__ __launch_bounds__(256) void k_generic(Dev d, const T* __restrict__ Li
   }
   double Vx = VEx, Vy = VEy, Vz = VEz;
   double Ix = VHx / Zx, Iy = VHy / Zy, Iz = VHz / Zz;
-  const int er = d.gen_exc[i];
-  if (er >= 0) {
+  if (IO && er >= 0) {
     // cell_excitation_field(row,1:9) at this timestep (excitation_points.F90:262-275)
     double fl[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int s = d.cex_start[er]; s < d.cex_start[er + 1]; s++) {
@@ -752,8 +815,7 @@ __global__ __launch_bounds__(256) void k_generic(Dev d, const T* __restrict__ Li
     Iy = (double)h[4] * Iy + fl[4] * dl / 2.0;
     Iz = (double)h[5] * Iz + fl[5] * dl / 2.0;
   }
-  const int orow = d.gen_out[i];
-  if (orow >= 0) {
+  if (IO && orow >= 0) {
     double* fo = d.ring + ((long long)slot * d.n_out_rows + orow) * 6;
     fo[0] = -Vx / dl; fo[1] = -Vy / dl; fo[2] = -Vz / dl;
     fo[3] = Ix / dl; fo[4] = Iy / dl; fo[5] = Iz / dl;
@@ -774,6 +836,60 @@ __global__ __launch_bounds__(256) void k_generic(Dev d, const T* __restrict__ Li
     }
   }
   scatter_ports(v, Vx, Vy, Vz, Ix, Iy, Iz);
+}
+
+__device__ inline void cell_coords(const Dev& d, long long c, int& x, int& y, int& z) {
+  const int nxy = d.nx * d.ny;
+  z = (int)(c / nxy);
+  const int rem = (int)(c - (long long)z * nxy);
+  y = rem / d.nx;
+  x = rem - y * d.nx;
+}
+
+// Generic special cells: point sources, probes, cells touching a special
+// face; the Thevenin node, or the free-space branch for code 0 cells that
+// only touch a special face.
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_generic(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                PullArgs a, int timestep, int slot) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n_gen) return;
+  const long long c = d.gen_cell[i];
+  int x, y, z;
+  cell_coords(d, c, x, y, z);
+  double v[12];
+  if (PULL) {
+    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, d.gen_fd[i], a, v);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
+  }
+  if (d.gen_code[i] == 0) {
+    if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
+  } else {
+    node_update<true>(d, v, d.gen_mat[i], d.gen_filt[i], d.gen_exc[i], d.gen_out[i], timestep, slot);
+  }
+  store_cell(Lout, d.P, c, v);
+}
+
+// Material cells (dispersive, PEC, PMC) without sources, probes or special
+// faces: the bulk of a dielectric body, with compact per-cell metadata.
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_material(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                 PullArgs a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.n_mat) return;
+  const long long c = d.mat_cell[i];
+  int x, y, z;
+  cell_coords(d, c, x, y, z);
+  double v[12];
+  if (PULL) {
+    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, -1, a, v);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
+  }
+  node_update<false>(d, v, d.mat_num[i], d.mat_filt[i], -1, -1, 0, 0);
   store_cell(Lout, d.P, c, v);
 }
 
@@ -804,7 +920,7 @@ __global__ __launch_bounds__(256) void k_connect_list(Dev d, const T* __restrict
   const int rem = (int)(c - (long long)z * nxy);
   const int y = rem / d.nx, x = rem - y * d.nx;
   double v[12];
-  pull_cell<T, PLAIN>(d, Lin, c, x, y, z, fds[i], a, v);
+  pull_cell<T, PLAIN>(d, Lin, c, x, y, z, fds ? fds[i] : -1, a, v);
   store_cell(Lout, d.P, c, v);
 }
 

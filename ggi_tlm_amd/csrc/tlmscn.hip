@@ -71,7 +71,7 @@ struct tlm_handle {
   void* planes[2] = {nullptr, nullptr};
   int cur = 0;
   Dev d{};
-  int n_gen = 0, n_pml = 0;
+  int n_gen = 0, n_pml = 0, n_mat = 0;
   const int32_t* d_gen_cell = nullptr;
   const int32_t* d_gen_fd = nullptr;
   const int32_t* d_pml_cell = nullptr;
@@ -82,6 +82,9 @@ struct tlm_handle {
   double* d_dft = nullptr;
   int64_t dev_bytes = 0;
 
+  // bulk schedule (tlm_set_schedule)
+  int sched_variant = 0, sched_by = 4, sched_zc = 16, sched_xcd = 1;
+
   // state machine
   Kind kind = Kind::INCIDENT;
   bool pend_connect = false, pend_boundary = false;
@@ -90,8 +93,8 @@ struct tlm_handle {
   // profiling: event pairs per kernel class, resolved on read
   bool profiling = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
-  int64_t prof_launches[TLM_N_KCLASS] = {0, 0, 0, 0};
-  double prof_ms[TLM_N_KCLASS] = {0, 0, 0, 0};
+  int64_t prof_launches[TLM_N_KCLASS] = {0};
+  double prof_ms[TLM_N_KCLASS] = {0};
 
   // output ring
   int cap = 4096;
@@ -258,8 +261,12 @@ int finalize(tlm_handle* h) {
 
   // ---- cell classes ------------------------------------------------------
   std::vector<uint32_t> mask((N + 31) / 32, 0u);
-  std::vector<int32_t> pml_cell, pml_sidx, pml_par, pml_out, pml_fd;
+  std::vector<int32_t> pml_cell, pml_sidx, pml_fd;
+  std::vector<uint16_t> pml_par;
+  std::vector<int2> pml_out_tab, pml_fd_tab;
   std::vector<int32_t> gen_cell, gen_code, gen_mat, gen_filt, gen_exc, gen_out, gen_fd;
+  std::vector<int32_t> mat_cell, mat_filt;
+  std::vector<uint16_t> mat_num;
   long long max_filt = 0;
   int out_rank = 0, exc_rank = 0;
   size_t fdi = 0;  // walks fd_cell in scan order
@@ -287,11 +294,15 @@ int finalize(tlm_handle* h) {
       }
       if (h->c_exc[s]) return fail(h, TLM_ERR_ARG, "Excitation intersects PML");
       int par = h->pml_cell_param[pmlc - 1];
-      if (par < 1 || par > (int)h->pml_par.size()) return fail(h, TLM_ERR_ARG, "PML parameter index out of range");
-      pml_cell.push_back((int32_t)c);
+      if (par < 1 || par > (int)h->pml_par.size() || par > 65536)
+        return fail(h, TLM_ERR_ARG, "PML parameter index out of range");
+      int32_t li = (int32_t)pml_cell.size();
+      int32_t cw = (int32_t)c;
+      if (orow >= 0) { cw |= scn::kFlagOut; pml_out_tab.push_back(make_int2(li, orow)); }
+      if (fdc >= 0) { cw |= scn::kFlagFd; pml_fd_tab.push_back(make_int2(li, fdc)); }
+      pml_cell.push_back(cw);
       pml_sidx.push_back(pmlc - 1);
-      pml_par.push_back(par - 1);
-      pml_out.push_back(orow);
+      pml_par.push_back((uint16_t)(par - 1));
       pml_fd.push_back(fdc);
     } else {
       int m = code ? h->c_mat[s] : 0;
@@ -304,6 +315,12 @@ int finalize(tlm_handle* h) {
         }
       }
       int er = (code && h->c_exc[s]) ? exc_rank++ : -1;
+      if (code && m && er < 0 && orow < 0 && fdc < 0 && m <= 65535) {
+        mat_cell.push_back((int32_t)c);
+        mat_num.push_back((uint16_t)m);
+        mat_filt.push_back(f);
+        continue;
+      }
       gen_cell.push_back((int32_t)c);
       gen_code.push_back(code);
       gen_mat.push_back(m);
@@ -313,7 +330,7 @@ int finalize(tlm_handle* h) {
       gen_fd.push_back(fdc);
     }
   }
-  if (N > 0x7fffffffLL) return fail(h, TLM_ERR_UNSUPPORTED, "more than 2^31 cells per handle");
+  if (N > (long long)scn::kCellMask) return fail(h, TLM_ERR_UNSUPPORTED, "more than 2^28 cells per handle");
   if (exc_rank != h->n_cell_rows)
     return fail(h, TLM_ERR_ARG, "cell excitation rows do not match the cell codes");
   const int n_out_rows = out_rank;
@@ -368,23 +385,40 @@ int finalize(tlm_handle* h) {
   if ((rc = dev_upload(h, mask, &d.mask))) return rc;
   h->n_pml = d.n_pml = (int)pml_cell.size();
   if ((rc = dev_upload(h, pml_cell, &d.pml_cell))) return rc;
-  if ((rc = dev_upload(h, pml_sidx, &d.pml_state_idx))) return rc;
+  {
+    bool identity = true;
+    for (size_t i = 0; i < pml_sidx.size() && identity; i++) identity = pml_sidx[i] == (int32_t)i;
+    if (identity) d.pml_state_idx = nullptr;
+    else if ((rc = dev_upload(h, pml_sidx, &d.pml_state_idx))) return rc;
+  }
   if ((rc = dev_upload(h, pml_par, &d.pml_par))) return rc;
-  if ((rc = dev_upload(h, pml_out, &d.pml_out))) return rc;
-  if ((rc = dev_upload(h, pml_fd, &d.pml_fd))) return rc;
-  h->d_pml_cell = d.pml_cell;
-  h->d_pml_fd = d.pml_fd;
+  if ((rc = dev_upload(h, pml_out_tab, &d.pml_out_tab))) return rc;
+  d.n_pml_out = (int)pml_out_tab.size();
+  if ((rc = dev_upload(h, pml_fd_tab, &d.pml_fd_tab))) return rc;
+  d.n_pml_fd = (int)pml_fd_tab.size();
+  {
+    // plain cell indices and descriptors for the connect-only pass
+    std::vector<int32_t> cells(pml_cell.size());
+    for (size_t i = 0; i < cells.size(); i++) cells[i] = pml_cell[i] & scn::kCellMask;
+    if ((rc = dev_upload(h, cells, &h->d_pml_cell))) return rc;
+    if ((rc = dev_upload(h, pml_fd, &h->d_pml_fd))) return rc;
+  }
   if ((rc = dev_upload(h, h->pml_par, &d.pml_params))) return rc;
   d.n_pml_state = std::max(n_pml_state, 1);
   {
     void* p;
-    if ((rc = dev_alloc(h, (size_t)scn::PS_N * d.n_pml_state * sizeof(double), &p))) return rc;
-    HIP_TRY(h, hipMemsetAsync(p, 0, (size_t)scn::PS_N * d.n_pml_state * sizeof(double), h->stream));
+    const size_t words = h->form == TLM_FORM_PLAIN_FP64 ? scn::PS_N : scn::PC_N;
+    if ((rc = dev_alloc(h, words * d.n_pml_state * sizeof(double), &p))) return rc;
+    HIP_TRY(h, hipMemsetAsync(p, 0, words * d.n_pml_state * sizeof(double), h->stream));
     d.pml_state = (double*)p;
     if ((rc = dev_alloc(h, (size_t)3 * d.n_pml_state * sizeof(double), &p))) return rc;
     HIP_TRY(h, hipMemsetAsync(p, 0, (size_t)3 * d.n_pml_state * sizeof(double), h->stream));
     d.pml_It = (double*)p;
   }
+  h->n_mat = d.n_mat = (int)mat_cell.size();
+  if ((rc = dev_upload(h, mat_cell, &d.mat_cell))) return rc;
+  if ((rc = dev_upload(h, mat_num, &d.mat_num))) return rc;
+  if ((rc = dev_upload(h, mat_filt, &d.mat_filt))) return rc;
   h->n_gen = d.n_gen = (int)gen_cell.size();
   if ((rc = dev_upload(h, gen_cell, &d.gen_cell))) return rc;
   if ((rc = dev_upload(h, gen_code, &d.gen_code))) return rc;
@@ -472,19 +506,26 @@ int drain_ring(tlm_handle* h) {
   return TLM_OK;
 }
 
-// z-chunk length: enough blocks to fill 256 CUs several times over while
-// keeping the chunk-boundary re-read (one plane in zc) small.
+// Bulk schedule.  variant 0: one plane per block (zc = 1); variant 1:
+// z-marching blocks of zc planes.  Tile height by rows of 64 cells.  The
+// defaults come from the sweep in tune_bulk (csrc autotuner) and can be
+// overridden per handle with tlm_set_schedule.
 scn::BulkMap bulk_map(const tlm_handle* h) {
   scn::BulkMap m;
+  m.by = h->sched_by;
   m.nbx = (h->nx + 63) / 64;
-  m.nby = (h->ny + scn::kBulkBY - 1) / scn::kBulkBY;
+  m.nby = (h->ny + m.by - 1) / m.by;
   long long tiles = (long long)m.nbx * m.nby;
-  int zc = 32;
-  while (zc > 4 && tiles * ((h->nz + zc - 1) / zc) < 2048) zc >>= 1;
-  m.zc = std::min(zc, h->nz);
+  if (h->sched_variant == 0) {
+    m.zc = 1;
+  } else {
+    int zc = h->sched_zc;
+    while (zc > 2 && tiles * ((h->nz + zc - 1) / zc) < 4096) zc >>= 1;
+    m.zc = std::max(1, std::min(zc, h->nz));
+  }
   m.nchunks = (h->nz + m.zc - 1) / m.zc;
   m.nblocks = (int)(tiles * m.nchunks);
-  m.xcd_remap = (m.nblocks % 8 == 0) ? 1 : 0;
+  m.xcd_remap = (h->sched_xcd && m.nblocks % 8 == 0) ? 1 : 0;
   return m;
 }
 
@@ -496,15 +537,26 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
   {
     ProfScope ps(h, TLM_KCLASS_BULK);
     scn::BulkMap m = bulk_map(h);
-    dim3 bb(64, scn::kBulkBY, 1);
-    if (pull) hipLaunchKernelGGL((scn::k_bulk<T, PLAIN, true>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
-    else hipLaunchKernelGGL((scn::k_bulk<T, PLAIN, false>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+    dim3 bb(64, m.by, 1);
+    if (m.zc == 1) {
+      if (pull) hipLaunchKernelGGL((scn::k_bulk_cell<T, PLAIN, true>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+      else hipLaunchKernelGGL((scn::k_bulk_cell<T, PLAIN, false>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+    } else {
+      if (pull) hipLaunchKernelGGL((scn::k_bulk_z<T, PLAIN, true>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+      else hipLaunchKernelGGL((scn::k_bulk_z<T, PLAIN, false>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+    }
   }
   if (d.n_pml) {
     ProfScope ps(h, TLM_KCLASS_PML);
     dim3 g((d.n_pml + 255) / 256);
     if (pull) hipLaunchKernelGGL((scn::k_pml<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot);
     else hipLaunchKernelGGL((scn::k_pml<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot);
+  }
+  if (d.n_mat) {
+    ProfScope ps(h, TLM_KCLASS_MATERIAL);
+    dim3 g((d.n_mat + 255) / 256);
+    if (pull) hipLaunchKernelGGL((scn::k_material<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a);
+    else hipLaunchKernelGGL((scn::k_material<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a);
   }
   if (d.n_gen) {
     ProfScope ps(h, TLM_KCLASS_GENERIC);
@@ -529,6 +581,9 @@ int launch_connect(tlm_handle* h, const PullArgs& a) {
   if (h->n_gen)
     hipLaunchKernelGGL((scn::k_connect_list<T, PLAIN>), dim3((h->n_gen + 255) / 256), dim3(256), 0, h->stream, d, Lin,
                        Lout, a, h->d_gen_cell, h->d_gen_fd, h->n_gen);
+  if (h->n_mat)
+    hipLaunchKernelGGL((scn::k_connect_list<T, PLAIN>), dim3((h->n_mat + 255) / 256), dim3(256), 0, h->stream, d, Lin,
+                       Lout, a, d.mat_cell, nullptr, h->n_mat);
   return TLM_OK;
 }
 
@@ -916,6 +971,17 @@ int tlm_read_dft(tlm_handle* h, double* re_im) {
   return TLM_OK;
 }
 
+int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap) {
+  if (!h) return TLM_ERR_ARG;
+  if (variant < 0 || variant > 1 || (rows != 1 && rows != 2 && rows != 4) || zc < 1)
+    return fail(h, TLM_ERR_ARG, "bad bulk schedule");
+  h->sched_variant = variant;
+  h->sched_by = rows;
+  h->sched_zc = zc;
+  h->sched_xcd = xcd_remap ? 1 : 0;
+  return TLM_OK;
+}
+
 int tlm_set_profiling(tlm_handle* h, int32_t enable) {
   if (!h) return TLM_ERR_ARG;
   int rc;
@@ -942,28 +1008,44 @@ int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
   int rc;
   if ((rc = finalize(h))) return rc;
   const int64_t sl = (int64_t)link_size(h);
-  const int64_t n_bulk = h->N - h->n_pml - h->n_gen;
+  const int64_t n_bulk = h->N - h->n_pml - h->n_gen - h->n_mat;
   // bulk: 12 links read + 12 written, plus its bit of the class mask
   cells[TLM_KCLASS_BULK] = n_bulk;
   bytes[TLM_KCLASS_BULK] = n_bulk * 24 * sl + (h->N + 7) / 8;
-  // PML: links, 24 FP64 state words read and written, 5 int32 list fields
+  // PML: links, FP64 state words read and written (24 plain, 18 precision),
+  // packed cell word and 16-bit parameter row, state index when not implied
   cells[TLM_KCLASS_PML] = h->n_pml;
-  bytes[TLM_KCLASS_PML] = (int64_t)h->n_pml * (24 * sl + 2 * 8 * scn::PS_N + 5 * 4);
-  // generic: links, 7 int32 list fields, filter words of dispersive cells
+  {
+    const int64_t words = h->form == TLM_FORM_PLAIN_FP64 ? scn::PS_N : scn::PC_N;
+    bytes[TLM_KCLASS_PML] = (int64_t)h->n_pml * (24 * sl + 2 * 8 * words + 4 + 2 + (h->d.pml_state_idx ? 4 : 0));
+  }
+  // filter words of a dispersive cell: 3 components x (carried words read
+  // and written) for eps and mu, FP64
+  auto filter_bytes = [&](int m) -> int64_t {
+    if (!m) return 0;
+    const tlm_volume_material& vm = h->vmat[m - 1];
+    if (vm.type != TLM_VMAT_DISPERSIVE) return 0;
+    int64_t words = (vm.eps_filter_exists ? vm.eps_order + 1 : 0) + (vm.mu_filter_exists ? vm.mu_order + 1 : 0);
+    return 3 * 2 * 8 * words;
+  };
+  // generic: links, 7 int32 list fields, filter words
   int64_t gb = (int64_t)h->n_gen * (24 * sl + 7 * 4);
   {
     std::vector<int32_t> gm(h->n_gen);
     if (h->n_gen) hipMemcpy(gm.data(), h->d.gen_mat, gm.size() * sizeof(int32_t), hipMemcpyDeviceToHost);
-    for (int32_t m : gm) {
-      if (!m) continue;
-      const tlm_volume_material& vm = h->vmat[m - 1];
-      if (vm.type != TLM_VMAT_DISPERSIVE) continue;
-      int64_t words = (vm.eps_filter_exists ? vm.eps_order + 1 : 0) + (vm.mu_filter_exists ? vm.mu_order + 1 : 0);
-      gb += 3 * 2 * 8 * words;
-    }
+    for (int32_t m : gm) gb += filter_bytes(m);
   }
   cells[TLM_KCLASS_GENERIC] = h->n_gen;
   bytes[TLM_KCLASS_GENERIC] = gb;
+  // material: links, cell word, 16-bit material number, filter number, filter words
+  int64_t mb = (int64_t)h->n_mat * (24 * sl + 4 + 2 + 4);
+  {
+    std::vector<uint16_t> mm(h->n_mat);
+    if (h->n_mat) hipMemcpy(mm.data(), h->d.mat_num, mm.size() * sizeof(uint16_t), hipMemcpyDeviceToHost);
+    for (uint16_t m : mm) mb += filter_bytes(m);
+  }
+  cells[TLM_KCLASS_MATERIAL] = h->n_mat;
+  bytes[TLM_KCLASS_MATERIAL] = mb;
   cells[TLM_KCLASS_DFT] = (int64_t)h->pt_row.size() * (int64_t)h->freq.size();
   bytes[TLM_KCLASS_DFT] = cells[TLM_KCLASS_DFT] * 2 * 2 * 8;
   return TLM_OK;

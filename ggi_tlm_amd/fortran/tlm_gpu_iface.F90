@@ -1,0 +1,253 @@
+!
+! tlm_gpu_iface.F90 -- iso_c_binding interface of the MI355X SCN engine
+! (include/tlmscn.h) for Fortran hosts.
+!
+! The engine replaces the argument-less hot-path subroutines of GGI_TLM
+! v2.1.7 (SRC/TLM_SOLVER/run_TLM.F90:242-283: excitation, scatter,
+! cell_output, connect, outer_boundary).  A host keeps its module tables in
+! the reference's layout and hands them over once; the thin wrappers below
+! turn a nonzero engine status into the reference's error convention
+! (write(*,*) message; STOP 1).
+!
+module tlm_gpu_iface
+
+  use iso_c_binding
+
+  implicit none
+
+  integer(c_int32_t), parameter :: TLM_FORM_PLAIN_FP64 = 0
+  integer(c_int32_t), parameter :: TLM_FORM_PRECISION_FP32 = 1
+  integer, parameter :: TLM_MAX_FILTER_ORDER = 8
+
+  ! TLM_general grid scalars (tlm_grid)
+  type, bind(c) :: tlm_grid
+    integer(c_int32_t) :: nx, ny, nz, n_timesteps
+    real(c_double) :: dl, dt, reduced_c_factor
+    real(c_double) :: r_xmin, r_xmax, r_ymin, r_ymax, r_zmin, r_zmax
+  end type tlm_grid
+
+  ! one volume_material_list row (tlm_volume_material)
+  type, bind(c) :: tlm_volume_material
+    integer(c_int32_t) :: mtype, eps_filter_exists, mu_filter_exists
+    integer(c_int32_t) :: eps_order, mu_order, pad_
+    real(c_double) :: Ys_eps_f, Ge, Zs_mu_f, Rm
+    real(c_double) :: eps_a(0:TLM_MAX_FILTER_ORDER), eps_b(0:TLM_MAX_FILTER_ORDER)
+    real(c_double) :: mu_a(0:TLM_MAX_FILTER_ORDER), mu_b(0:TLM_MAX_FILTER_ORDER)
+  end type tlm_volume_material
+
+  ! one PML_parameters row without the depth fields (tlm_pml_parameter)
+  type, bind(c) :: tlm_pml_parameter
+    real(c_double) :: sx, sy, sz, ax, ay, az, exp_x, exp_y, exp_z
+    real(c_double) :: Csx, Csy, Csz, Csx2, Csy2, Csz2, Csy_sz, Csz_sx, Csx_sy
+  end type tlm_pml_parameter
+
+  ! one point source (tlm_point_source)
+  type, bind(c) :: tlm_point_source
+    integer(c_int32_t) :: func, component, source_type, at_face, row
+  end type tlm_point_source
+
+  ! one Huygens face (tlm_huygens_face)
+  type, bind(c) :: tlm_huygens_face
+    integer(c_int32_t) :: row, nx, ny, nz
+    real(c_double) :: offset
+  end type tlm_huygens_face
+
+  interface
+
+    integer(c_int) function tlm_create(grid, form, h) bind(c, name='tlm_create')
+      import :: c_int, c_int32_t, c_ptr, tlm_grid
+      type(tlm_grid), intent(in) :: grid
+      integer(c_int32_t), value :: form
+      type(c_ptr), intent(out) :: h
+    end function tlm_create
+
+    subroutine tlm_destroy(h) bind(c, name='tlm_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: h
+    end subroutine tlm_destroy
+
+    type(c_ptr) function tlm_last_error(h) bind(c, name='tlm_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: h
+    end function tlm_last_error
+
+    integer(c_int) function tlm_set_cell_codes(h, code, n_special, to_material, to_pml, to_excitation, &
+                                               to_output, to_cable) bind(c, name='tlm_set_cell_codes')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(in) :: code(*)
+      integer(c_int32_t), value :: n_special
+      integer(c_int32_t), intent(in) :: to_material(*), to_pml(*), to_excitation(*), to_output(*), to_cable(*)
+    end function tlm_set_cell_codes
+
+    integer(c_int) function tlm_set_face_codes(h, code, n_special, to_material, to_excitation, to_output, &
+                                               to_cable) bind(c, name='tlm_set_face_codes')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(in) :: code(*)
+      integer(c_int32_t), value :: n_special
+      integer(c_int32_t), intent(in) :: to_material(*), to_excitation(*), to_output(*), to_cable(*)
+    end function tlm_set_face_codes
+
+    integer(c_int) function tlm_set_volume_materials(h, n, m) bind(c, name='tlm_set_volume_materials')
+      import :: c_int, c_int32_t, c_ptr, tlm_volume_material
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n
+      type(tlm_volume_material), intent(in) :: m(*)
+    end function tlm_set_volume_materials
+
+    integer(c_int) function tlm_set_surface_materials(h, n, mtype) bind(c, name='tlm_set_surface_materials')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n
+      integer(c_int32_t), intent(in) :: mtype(*)
+    end function tlm_set_surface_materials
+
+    integer(c_int) function tlm_set_pml(h, n_params, params, n_cells, cell_param) bind(c, name='tlm_set_pml')
+      import :: c_int, c_int32_t, c_ptr, tlm_pml_parameter
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n_params, n_cells
+      type(tlm_pml_parameter), intent(in) :: params(*)
+      integer(c_int32_t), intent(in) :: cell_param(*)
+    end function tlm_set_pml
+
+    integer(c_int) function tlm_set_excitation_functions(h, n_functions, value, value_face) &
+                                               bind(c, name='tlm_set_excitation_functions')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n_functions
+      real(c_double), intent(in) :: value(*), value_face(*)
+    end function tlm_set_excitation_functions
+
+    integer(c_int) function tlm_set_point_sources(h, n, sources, n_cell_rows, n_face_rows) &
+                                               bind(c, name='tlm_set_point_sources')
+      import :: c_int, c_int32_t, c_ptr, tlm_point_source
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n, n_cell_rows, n_face_rows
+      type(tlm_point_source), intent(in) :: sources(*)
+    end function tlm_set_point_sources
+
+    integer(c_int) function tlm_set_huygens_surface(h, func, offset_min, Ei, Hi, n_faces, faces) &
+                                               bind(c, name='tlm_set_huygens_surface')
+      import :: c_int, c_int32_t, c_ptr, c_double, tlm_huygens_face
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: func, n_faces
+      real(c_double), value :: offset_min
+      real(c_double), intent(in) :: Ei(3), Hi(3)
+      type(tlm_huygens_face), intent(in) :: faces(*)
+    end function tlm_set_huygens_surface
+
+    integer(c_int) function tlm_set_output_points(h, n, row, component) bind(c, name='tlm_set_output_points')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n
+      integer(c_int32_t), intent(in) :: row(*), component(*)
+    end function tlm_set_output_points
+
+    integer(c_int) function tlm_set_output_capacity(h, steps) bind(c, name='tlm_set_output_capacity')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: steps
+    end function tlm_set_output_capacity
+
+    integer(c_int) function tlm_upload_links(h, V) bind(c, name='tlm_upload_links')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: V(*)
+    end function tlm_upload_links
+
+    integer(c_int) function tlm_download_links(h, V) bind(c, name='tlm_download_links')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(inout) :: V(*)
+    end function tlm_download_links
+
+    integer(c_int) function tlm_excitation(h, timestep) bind(c, name='tlm_excitation')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: timestep
+    end function tlm_excitation
+
+    integer(c_int) function tlm_scatter(h, timestep) bind(c, name='tlm_scatter')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: timestep
+    end function tlm_scatter
+
+    integer(c_int) function tlm_connect(h, timestep) bind(c, name='tlm_connect')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: timestep
+    end function tlm_connect
+
+    integer(c_int) function tlm_outer_boundary(h) bind(c, name='tlm_outer_boundary')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function tlm_outer_boundary
+
+    integer(c_int) function tlm_step(h, first_timestep, n_steps) bind(c, name='tlm_step')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: first_timestep, n_steps
+    end function tlm_step
+
+    integer(c_int) function tlm_read_cell_outputs(h, out, max_steps, n_read, first_step) &
+                                               bind(c, name='tlm_read_cell_outputs')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(inout) :: out(*)
+      integer(c_int32_t), value :: max_steps
+      integer(c_int32_t), intent(out) :: n_read, first_step
+    end function tlm_read_cell_outputs
+
+  end interface
+
+contains
+
+  ! Reference error convention: write(*,*) message; STOP 1.
+  subroutine tlm_check(h, status, what)
+    type(c_ptr), intent(in) :: h
+    integer(c_int), intent(in) :: status
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    character(len=512) :: text
+    integer :: i
+    if (status .eq. 0) return
+    text = ''
+    if (c_associated(h)) then
+      call c_f_pointer(tlm_last_error(h), msg, (/ 512 /))
+      do i = 1, 512
+        if (msg(i) .eq. c_null_char) exit
+        text(i:i) = msg(i)
+      end do
+    end if
+    write(*,*)'ERROR in ', what, ': ', trim(text)
+    STOP 1
+  end subroutine tlm_check
+
+  ! Drop-in replacements for the reference's per-step calls
+  ! (run_TLM.F90:242, 244, 277, 283).
+  subroutine gpu_excitation(h, timestep)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: timestep
+    call tlm_check(h, tlm_excitation(h, int(timestep, c_int32_t)), 'excitation')
+  end subroutine gpu_excitation
+
+  subroutine gpu_scatter(h, timestep)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: timestep
+    call tlm_check(h, tlm_scatter(h, int(timestep, c_int32_t)), 'scatter')
+  end subroutine gpu_scatter
+
+  subroutine gpu_connect(h, timestep)
+    type(c_ptr), intent(in) :: h
+    integer, intent(in) :: timestep
+    call tlm_check(h, tlm_connect(h, int(timestep, c_int32_t)), 'connect')
+  end subroutine gpu_connect
+
+  subroutine gpu_outer_boundary(h)
+    type(c_ptr), intent(in) :: h
+    call tlm_check(h, tlm_outer_boundary(h), 'outer_boundary')
+  end subroutine gpu_outer_boundary
+
+end module tlm_gpu_iface

@@ -229,14 +229,23 @@ int tlm_read_cell_outputs(tlm_handle* h, double* out, int32_t max_steps,
 /* DFT accumulators: re_im[(p*n_freq + k)*2 + {0,1}]. */
 int tlm_read_dft(tlm_handle* h, double* re_im);
 
-/* ---- measurement (no counterpart in the reference) ----------------------- */
+/* ---- schedule and measurement (no counterpart in the reference) ---------- */
+
+/* Bulk-kernel schedule: variant 0 = one xy-plane tile per workgroup,
+ * variant 1 = workgroups marching zc planes; rows = tile height (1, 2 or 4
+ * rows of 64 cells); xcd_remap = give each XCD a contiguous tile range.
+ * Any schedule gives identical results; it only changes speed.  The
+ * autotuner (ggi_tlm_amd/autotune.py) picks one per grid shape. */
+int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc,
+                     int32_t xcd_remap);
 
 /* Kernel classes timed by the profiling hooks. */
 #define TLM_KCLASS_BULK 0
 #define TLM_KCLASS_PML 1
 #define TLM_KCLASS_GENERIC 2
 #define TLM_KCLASS_DFT 3
-#define TLM_N_KCLASS 4
+#define TLM_KCLASS_MATERIAL 4
+#define TLM_N_KCLASS 5
 
 /* When enabled, every kernel launch of a step is bracketed by HIP events
  * recorded on the handle's own stream.  tlm_read_kernel_times waits for the

@@ -1041,7 +1041,12 @@ int oracle_read_face_outputs(oracle_handle* h, double* out) {
   return TLM_OK;
 }
 
-/* Measurement hooks of the engine ABI; the oracle has no kernels to time. */
+/* Schedule and measurement hooks of the engine ABI; the oracle has no
+ * kernels to schedule or time. */
+int oracle_set_schedule(oracle_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap) {
+  (void)h; (void)variant; (void)rows; (void)zc; (void)xcd_remap;
+  return TLM_OK;
+}
 int oracle_set_profiling(oracle_handle* h, int32_t enable) { (void)h; (void)enable; return TLM_OK; }
 int oracle_read_kernel_times(oracle_handle* h, int64_t* launches, double* ms) {
   (void)h;

@@ -117,3 +117,19 @@ def test_precision_form(name, oracle_lib):
         rel = np.linalg.norm(sg[:, k] - so[:, k]) / den
         print(f"probe {k}: relative L2 {rel:.3e}")
         assert rel < 1e-5, (name, k, rel)
+
+
+@pytest.mark.parametrize("sched", [(0, 1, 1, 0), (0, 2, 1, 1), (1, 1, 4, 1), (1, 4, 8, 0), (1, 2, 32, 1)])
+def test_schedules_bitwise(sched, oracle_lib):
+    """Every bulk schedule gives the oracle's result bit for bit."""
+    from oracle import oracle
+
+    for name in ("free_cube_odd", "pml_sphere"):
+        t = build_tables(CASES[name]())
+        steps = min(t.n_timesteps, 60)
+        o = oracle.handle(t, output_capacity=steps)
+        g = engine(t, FORM_PLAIN_FP64)
+        g.set_schedule(*sched)
+        ro, rg = run(o, t, steps), run(g, t, steps)
+        assert np.array_equal(ro, rg)
+        assert np.array_equal(o.download_links()[:, 1:-1, 1:-1], g.download_links()[:, 1:-1, 1:-1])
