@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/tlmscn.h"
 
 namespace scn {
@@ -204,10 +206,10 @@ __device__ inline double ld(const T* __restrict__ L, long long P, int port, long
 // Ordinary face: S = V_min + V_max; incident = S - own
 // (connect_normal_to_x.F90:4-10).  The precision form takes the exact
 // swap, incident = neighbour, which has no rounding.
-template <bool PLAIN>
-__device__ inline double free_face(double own, double nb) {
+template <bool PLAIN, typename A>
+__device__ inline A free_face(A own, A nb) {
   if (PLAIN) {
-    double S = own + nb;
+    A S = own + nb;
     return S - own;
   }
   return nb;
@@ -385,6 +387,31 @@ __device__ inline void scatter_free_fast(double v[12]) {
   v[Vz_ymin] = Vz + Ix - M;  v[Vz_ymax] = Vz - Ix - m;
 }
 
+// FP32 arithmetic form of scatter_free_fast, used by the precision-form
+// vector bulk kernel (links are stored in FP32 anyway; each output carries
+// a few more FP32 roundings than the FP64 evaluation).
+__device__ inline void scatter_free_f32(float v[12]) {
+  float Vx = (v[Vx_ymin] + v[Vx_ymax] + v[Vx_zmin] + v[Vx_zmax]) * 0.5f;
+  float Vy = (v[Vy_xmin] + v[Vy_xmax] + v[Vy_zmin] + v[Vy_zmax]) * 0.5f;
+  float Vz = (v[Vz_xmin] + v[Vz_xmax] + v[Vz_ymin] + v[Vz_ymax]) * 0.5f;
+  float Ix = (v[Vz_ymax] - v[Vy_zmax] - v[Vz_ymin] + v[Vy_zmin]) * 0.5f;
+  float Iy = (v[Vx_zmax] - v[Vz_xmax] - v[Vx_zmin] + v[Vz_xmin]) * 0.5f;
+  float Iz = (v[Vy_xmax] - v[Vx_ymax] - v[Vy_xmin] + v[Vx_ymin]) * 0.5f;
+  float m, M;
+  m = v[Vx_ymin]; M = v[Vx_ymax];
+  v[Vx_ymin] = Vx - Iz - M;  v[Vx_ymax] = Vx + Iz - m;
+  m = v[Vx_zmin]; M = v[Vx_zmax];
+  v[Vx_zmin] = Vx + Iy - M;  v[Vx_zmax] = Vx - Iy - m;
+  m = v[Vy_zmin]; M = v[Vy_zmax];
+  v[Vy_zmin] = Vy - Ix - M;  v[Vy_zmax] = Vy + Ix - m;
+  m = v[Vy_xmin]; M = v[Vy_xmax];
+  v[Vy_xmin] = Vy + Iz - M;  v[Vy_xmax] = Vy - Iz - m;
+  m = v[Vz_xmin]; M = v[Vz_xmax];
+  v[Vz_xmin] = Vz - Iy - M;  v[Vz_xmax] = Vz + Iy - m;
+  m = v[Vz_ymin]; M = v[Vz_ymax];
+  v[Vz_ymin] = Vz + Ix - M;  v[Vz_ymax] = Vz - Ix - m;
+}
+
 template <typename T>
 __device__ inline void store_cell(T* __restrict__ L, long long P, long long c, const double v[12]) {
 #pragma unroll
@@ -493,6 +520,128 @@ __global__ __launch_bounds__(256) void k_bulk_cell(Dev d, const T* __restrict__ 
   }
   if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
   store_cell(Lout, P, c, v);
+}
+
+// Vector access of W consecutive link values (W*sizeof(T) = 16 bytes).
+template <typename T, int W> struct VecOf;
+template <> struct VecOf<float, 4> { typedef float4 type; };
+template <> struct VecOf<double, 2> { typedef double2 type; };
+
+template <typename T, int W>
+__device__ inline void vload(const T* __restrict__ p, T out[W]) {
+  typedef typename VecOf<T, W>::type V;
+  V v = *reinterpret_cast<const V*>(p);
+  const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+  for (int k = 0; k < W; k++) out[k] = e[k];
+}
+template <typename T, int W>
+__device__ inline void vstore(T* __restrict__ p, const T in[W]) {
+  typedef typename VecOf<T, W>::type V;
+  V v;
+  T* e = reinterpret_cast<T*>(&v);
+#pragma unroll
+  for (int k = 0; k < W; k++) e[k] = in[k];
+  *reinterpret_cast<V*>(p) = v;
+}
+
+// Bulk free-space cells, W consecutive x-cells per thread with 16-byte
+// plane accesses (float4 in the precision form, double2 in the plain form):
+// a 64-lane wavefront covers 64*W cells of a row.  Needs nx % W == 0 and a
+// plane stride that is a multiple of W.  Per group of W cells:
+//   own ports        12 vector loads
+//   y/z neighbours   8 vector loads (the facing ports of rows y-1, y+1 and
+//                    planes z-1, z+1)
+//   x neighbours     inside the group from the group itself, across groups
+//                    from the adjacent lanes, loads only at wavefront edges
+// Groups whose cells are all bulk cells store with 12 vector stores; groups
+// holding special cells store their bulk cells one by one.
+template <typename T, bool PLAIN, bool PULL, int W>
+__global__ __launch_bounds__(256) void k_bulk_vec(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                 PullArgs a, BulkMap m) {
+  int bx, byi, z;
+  bulk_tile(m, bx, byi, z);
+  const int lane = threadIdx.x;
+  const int x0 = (bx * 64 + lane) * W;
+  const int y = byi * m.by + threadIdx.y;
+  const bool valid = x0 < d.nx && y < d.ny && z < d.nz;
+  const long long P = d.P;
+  const long long sxy = (long long)d.nx * d.ny;
+  const long long c = valid ? x0 + (long long)d.nx * (y + (long long)d.ny * z) : 0;
+  T own[12][W];
+  if (valid) {
+#pragma unroll
+    for (int p = 0; p < 12; p++) vload<T, W>(Lin + p * P + c, own[p]);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 12; p++)
+#pragma unroll
+      for (int k = 0; k < W; k++) own[p][k] = (T)0;
+  }
+  // x neighbours across groups (all lanes take part in the moves)
+  T xa = __shfl_up(own[Vy_xmax][W - 1], 1, 64), xb = __shfl_up(own[Vz_xmax][W - 1], 1, 64);
+  T xc = __shfl_down(own[Vy_xmin][0], 1, 64), xd = __shfl_down(own[Vz_xmin][0], 1, 64);
+  if (!valid) return;
+  uint32_t bits = 0;
+  {
+    const uint32_t wbits = d.mask[c >> 5];
+    bits = (wbits >> (c & 31)) & ((1u << W) - 1u);  // W divides 32 and c % W == 0
+  }
+  if (bits == ((1u << W) - 1u)) return;  // all special
+  T ya[2][W], yb[2][W], za[2][W], zb[2][W];  // [0]: min-side neighbour, [1]: max-side
+  if (PULL) {
+    if (lane == 0 && x0 > 0) { xa = Lin[Vy_xmax * P + c - 1]; xb = Lin[Vz_xmax * P + c - 1]; }
+    if (lane == 63 && x0 + W < d.nx) { xc = Lin[Vy_xmin * P + c + W]; xd = Lin[Vz_xmin * P + c + W]; }
+    if (y > 0) { vload<T, W>(Lin + Vx_ymax * P + c - d.nx, ya[0]); vload<T, W>(Lin + Vz_ymax * P + c - d.nx, yb[0]); }
+    if (y + 1 < d.ny) { vload<T, W>(Lin + Vx_ymin * P + c + d.nx, ya[1]); vload<T, W>(Lin + Vz_ymin * P + c + d.nx, yb[1]); }
+    if (z > 0) { vload<T, W>(Lin + Vx_zmax * P + c - sxy, za[0]); vload<T, W>(Lin + Vy_zmax * P + c - sxy, zb[0]); }
+    if (z + 1 < d.nz) { vload<T, W>(Lin + Vx_zmin * P + c + sxy, za[1]); vload<T, W>(Lin + Vy_zmin * P + c + sxy, zb[1]); }
+  }
+  T out[12][W];
+  // arithmetic type: FP64 in the plain form (bit-faithful), FP32 in the
+  // precision form
+  typedef typename std::conditional<PLAIN, double, T>::type A;
+  const A Rx0 = a.apply_r ? (A)d.R[0] : (A)1, Rx1 = a.apply_r ? (A)d.R[1] : (A)1;
+  const A Ry0 = a.apply_r ? (A)d.R[2] : (A)1, Ry1 = a.apply_r ? (A)d.R[3] : (A)1;
+  const A Rz0 = a.apply_r ? (A)d.R[4] : (A)1, Rz1 = a.apply_r ? (A)d.R[5] : (A)1;
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    const int x = x0 + k;
+    A v[12];
+#pragma unroll
+    for (int p = 0; p < 12; p++) v[p] = (A)own[p][k];
+    if (PULL) {
+      const A nxa = k == 0 ? (A)xa : (A)own[Vy_xmax][k > 0 ? k - 1 : 0];
+      const A nxb = k == 0 ? (A)xb : (A)own[Vz_xmax][k > 0 ? k - 1 : 0];
+      const A nxc = k == W - 1 ? (A)xc : (A)own[Vy_xmin][k < W - 1 ? k + 1 : 0];
+      const A nxd = k == W - 1 ? (A)xd : (A)own[Vz_xmin][k < W - 1 ? k + 1 : 0];
+      if (x == 0) { v[Vy_xmin] = v[Vy_xmin] * Rx0; v[Vz_xmin] = v[Vz_xmin] * Rx0; }
+      else { v[Vy_xmin] = free_face<PLAIN>(v[Vy_xmin], nxa); v[Vz_xmin] = free_face<PLAIN>(v[Vz_xmin], nxb); }
+      if (x == d.nx - 1) { v[Vy_xmax] = v[Vy_xmax] * Rx1; v[Vz_xmax] = v[Vz_xmax] * Rx1; }
+      else { v[Vy_xmax] = free_face<PLAIN>(v[Vy_xmax], nxc); v[Vz_xmax] = free_face<PLAIN>(v[Vz_xmax], nxd); }
+      if (y == 0) { v[Vx_ymin] = v[Vx_ymin] * Ry0; v[Vz_ymin] = v[Vz_ymin] * Ry0; }
+      else { v[Vx_ymin] = free_face<PLAIN>(v[Vx_ymin], (A)ya[0][k]); v[Vz_ymin] = free_face<PLAIN>(v[Vz_ymin], (A)yb[0][k]); }
+      if (y == d.ny - 1) { v[Vx_ymax] = v[Vx_ymax] * Ry1; v[Vz_ymax] = v[Vz_ymax] * Ry1; }
+      else { v[Vx_ymax] = free_face<PLAIN>(v[Vx_ymax], (A)ya[1][k]); v[Vz_ymax] = free_face<PLAIN>(v[Vz_ymax], (A)yb[1][k]); }
+      if (z == 0) { v[Vx_zmin] = v[Vx_zmin] * Rz0; v[Vy_zmin] = v[Vy_zmin] * Rz0; }
+      else { v[Vx_zmin] = free_face<PLAIN>(v[Vx_zmin], (A)za[0][k]); v[Vy_zmin] = free_face<PLAIN>(v[Vy_zmin], (A)zb[0][k]); }
+      if (z == d.nz - 1) { v[Vx_zmax] = v[Vx_zmax] * Rz1; v[Vy_zmax] = v[Vy_zmax] * Rz1; }
+      else { v[Vx_zmax] = free_face<PLAIN>(v[Vx_zmax], (A)za[1][k]); v[Vy_zmax] = free_face<PLAIN>(v[Vy_zmax], (A)zb[1][k]); }
+    }
+    if (PLAIN) scatter_free_plain((double*)v); else scatter_free_f32((float*)v);
+#pragma unroll
+    for (int p = 0; p < 12; p++) out[p][k] = (T)v[p];
+  }
+  if (bits == 0) {
+#pragma unroll
+    for (int p = 0; p < 12; p++) vstore<T, W>(Lout + p * P + c, out[p]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < W; k++)
+      if (!((bits >> k) & 1u))
+#pragma unroll
+        for (int p = 0; p < 12; p++) Lout[p * P + c + k] = out[p][k];
+  }
 }
 
 // Bulk free-space cells, marching zc planes per block.  The zmin-face ports

@@ -83,7 +83,7 @@ struct tlm_handle {
   int64_t dev_bytes = 0;
 
   // bulk schedule (tlm_set_schedule)
-  int sched_variant = 0, sched_by = 4, sched_zc = 16, sched_xcd = 1;
+  int sched_variant = -1, sched_by = 4, sched_zc = 16, sched_xcd = 0;  // -1: per-form default
 
   // state machine
   Kind kind = Kind::INCIDENT;
@@ -510,13 +510,26 @@ int drain_ring(tlm_handle* h) {
 // z-marching blocks of zc planes.  Tile height by rows of 64 cells.  The
 // defaults come from the sweep in tune_bulk (csrc autotuner) and can be
 // overridden per handle with tlm_set_schedule.
+// Cells per thread of the vector bulk kernel (16-byte accesses).
+int sched_variant(const tlm_handle* h) {
+  if (h->sched_variant >= 0) return h->sched_variant;
+  return h->form == TLM_FORM_PLAIN_FP64 ? 0 : 2;
+}
+
+int bulk_vec_width(const tlm_handle* h) {
+  int w = h->form == TLM_FORM_PLAIN_FP64 ? 2 : 4;
+  if (sched_variant(h) != 2 || h->nx % w != 0 || h->P % w != 0) return 1;
+  return w;
+}
+
 scn::BulkMap bulk_map(const tlm_handle* h) {
   scn::BulkMap m;
   m.by = h->sched_by;
-  m.nbx = (h->nx + 63) / 64;
+  const int w = bulk_vec_width(h);
+  m.nbx = (h->nx + 64 * w - 1) / (64 * w);
   m.nby = (h->ny + m.by - 1) / m.by;
   long long tiles = (long long)m.nbx * m.nby;
-  if (h->sched_variant == 0) {
+  if (sched_variant(h) != 1) {
     m.zc = 1;
   } else {
     int zc = h->sched_zc;
@@ -538,7 +551,12 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
     ProfScope ps(h, TLM_KCLASS_BULK);
     scn::BulkMap m = bulk_map(h);
     dim3 bb(64, m.by, 1);
-    if (m.zc == 1) {
+    const int w = bulk_vec_width(h);
+    if (w > 1) {
+      constexpr int W = sizeof(T) == 4 ? 4 : 2;
+      if (pull) hipLaunchKernelGGL((scn::k_bulk_vec<T, PLAIN, true, W>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+      else hipLaunchKernelGGL((scn::k_bulk_vec<T, PLAIN, false, W>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+    } else if (m.zc == 1) {
       if (pull) hipLaunchKernelGGL((scn::k_bulk_cell<T, PLAIN, true>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
       else hipLaunchKernelGGL((scn::k_bulk_cell<T, PLAIN, false>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
     } else {
@@ -973,7 +991,7 @@ int tlm_read_dft(tlm_handle* h, double* re_im) {
 
 int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap) {
   if (!h) return TLM_ERR_ARG;
-  if (variant < 0 || variant > 1 || (rows != 1 && rows != 2 && rows != 4) || zc < 1)
+  if (variant < 0 || variant > 2 || (rows != 1 && rows != 2 && rows != 4) || zc < 1)
     return fail(h, TLM_ERR_ARG, "bad bulk schedule");
   h->sched_variant = variant;
   h->sched_by = rows;

@@ -231,9 +231,11 @@ int tlm_read_dft(tlm_handle* h, double* re_im);
 
 /* ---- schedule and measurement (no counterpart in the reference) ---------- */
 
-/* Bulk-kernel schedule: variant 0 = one xy-plane tile per workgroup,
- * variant 1 = workgroups marching zc planes; rows = tile height (1, 2 or 4
- * rows of 64 cells); xcd_remap = give each XCD a contiguous tile range.
+/* Bulk-kernel schedule: variant 0 = one cell per lane, one xy-plane tile
+ * per workgroup; variant 1 = workgroups marching zc planes; variant 2 = 16-byte
+ * vector accesses, several consecutive cells per lane (falls back to 0 when
+ * nx is not a multiple of the vector width); rows = tile height (1, 2 or 4
+ * rows); xcd_remap = give each XCD a contiguous tile range.
  * Any schedule gives identical results; it only changes speed.  The
  * autotuner (ggi_tlm_amd/autotune.py) picks one per grid shape. */
 int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc,

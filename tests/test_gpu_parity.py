@@ -119,12 +119,13 @@ def test_precision_form(name, oracle_lib):
         assert rel < 1e-5, (name, k, rel)
 
 
-@pytest.mark.parametrize("sched", [(0, 1, 1, 0), (0, 2, 1, 1), (1, 1, 4, 1), (1, 4, 8, 0), (1, 2, 32, 1)])
+@pytest.mark.parametrize("sched", [(0, 1, 1, 0), (0, 2, 1, 1), (1, 1, 4, 1), (1, 4, 8, 0), (1, 2, 32, 1),
+                                   (2, 4, 1, 0), (2, 1, 1, 1)])
 def test_schedules_bitwise(sched, oracle_lib):
     """Every bulk schedule gives the oracle's result bit for bit."""
     from oracle import oracle
 
-    for name in ("free_cube_odd", "pml_sphere"):
+    for name in ("free_cube_odd", "pml_sphere", "free_cube"):
         t = build_tables(CASES[name]())
         steps = min(t.n_timesteps, 60)
         o = oracle.handle(t, output_capacity=steps)
