@@ -25,7 +25,22 @@ __global__ __launch_bounds__(256) void k_copy1(const float4* __restrict__ a, flo
 // pattern of the link planes without neighbour reads
 __global__ __launch_bounds__(256) void k_planes12(const float* __restrict__ a, float* __restrict__ b, long long P) {
   long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= P) return;
+  if (i >= (long long)gridDim.x * blockDim.x || i >= P) return;
+  float v[12];
+#pragma unroll
+  for (int p = 0; p < 12; p++) v[p] = a[p * P + i];
+#pragma unroll
+  for (int p = 0; p < 12; p++) b[p * P + i] = v[(p + 1) % 12];
+}
+
+// as k_planes12, but rows of 256 cells skip their first and last `skip`
+// cells (written by another kernel): partial cache lines at both row ends
+__global__ __launch_bounds__(256) void k_planes12_skip(const float* __restrict__ a, float* __restrict__ b, long long P,
+                                                       long long n, int skip) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int x = (int)(i & 255);
+  if (x < skip || x >= 256 - skip) return;
   float v[12];
 #pragma unroll
   for (int p = 0; p < 12; p++) v[p] = a[p * P + i];
@@ -72,7 +87,46 @@ int main(int argc, char** argv) {
     per = ms / reps;
     printf("{\"kernel\": \"copy1\", \"avg_ms\": %.5f, \"GB_s\": %.1f}\n", per, 2.0 * bytes / (per * 1e-3) / 1e9);
   }
-  // twelve planes
+  // twelve planes of 2^24 cells with padded plane strides: power-of-two
+  // strides put every plane on the same memory channel at the same time
+  {
+    const long long N = 1LL << 24;
+    const long long pads[] = {0, 64, 256, 1024, 4096, 4288, 65536 + 4288};
+    for (long long pad : pads) {
+      long long P = N + pad;
+      if (12 * P * 4 > bytes) break;
+      int g1 = (int)((N + 255) / 256);
+      hipLaunchKernelGGL(k_planes12, dim3(g1), dim3(256), 0, 0, (const float*)a, (float*)b, P);
+      hipEventRecord(e0, 0);
+      for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_planes12, dim3(g1), dim3(256), 0, 0, (const float*)a, (float*)b, P);
+      hipEventRecord(e1, 0);
+      hipEventSynchronize(e1);
+      hipEventElapsedTime(&ms, e0, e1);
+      per = ms / reps;
+      printf("{\"kernel\": \"planes12\", \"cells\": %lld, \"pad\": %lld, \"avg_ms\": %.5f, \"GB_s\": %.1f}\n", N, pad, per,
+             2.0 * 48 * N / (per * 1e-3) / 1e9);
+    }
+  }
+  // partial rows
+  {
+    const long long N = 1LL << 24;
+    const int skips[] = {0, 8, 16, 32};
+    for (int sk : skips) {
+      int g1 = (int)((N + 255) / 256);
+      hipLaunchKernelGGL(k_planes12_skip, dim3(g1), dim3(256), 0, 0, (const float*)a, (float*)b, N, N, sk);
+      hipEventRecord(e0, 0);
+      for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_planes12_skip, dim3(g1), dim3(256), 0, 0, (const float*)a, (float*)b, N, N, sk);
+      hipEventRecord(e1, 0);
+      hipEventSynchronize(e1);
+      hipEventElapsedTime(&ms, e0, e1);
+      per = ms / reps;
+      double cells = (double)N * (256 - 2 * sk) / 256.0;
+      printf("{\"kernel\": \"planes12_skip\", \"skip\": %d, \"avg_ms\": %.5f, \"GB_s\": %.1f}\n", sk, per,
+             2.0 * 48 * cells / (per * 1e-3) / 1e9);
+    }
+  }
+  // twelve planes, original layout
   {
     long long P = bytes / 4 / 12;
     int g1 = (int)((P + 255) / 256);
