@@ -67,7 +67,7 @@ SIGNATURES = {
     "kernel_bytes": (C.c_int, [_h, _p, _p]),
 }
 
-KERNEL_CLASSES = ("bulk", "pml", "generic", "dft", "material")
+KERNEL_CLASSES = ("bulk", "pml", "generic", "dft", "material", "mixed")
 
 EXPORTED = tuple("tlm_" + n for n in SIGNATURES)
 

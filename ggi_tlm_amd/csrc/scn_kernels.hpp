@@ -92,11 +92,14 @@ struct Dev {
   double dl, dt, c_red;  // c_red = c0 / reduced_c_factor
   int n_timesteps;
   double R[6];        // xmin,xmax,ymin,ymax,zmin,zmax
-  const uint32_t* mask;  // 1 = special cell (not bulk)
+  const uint32_t* mask;  // 1 = cell not in a pure bulk line
+  int line_len;          // cells per 128-byte line of a plane
+  const uint32_t* line_info;  // per line: 0 bulk, 1 one special class, 2 | (record base << 2) mixed
+  const uint32_t* line_bits;  // mixed lines: bit k = cell k is special
+  const uint32_t* mixed_rec;  // class << 30 | list index, per special cell of the mixed lines
   // pml class
   int n_pml;
   const int32_t* pml_cell;       // cell | kFlagOut | kFlagFd
-  const int32_t* pml_state_idx;  // PML_cell - 1, or null when it equals the list position
   const uint16_t* pml_par;       // 0-based parameter row
   const int2* pml_out_tab;       // (list position, output row), sorted
   int n_pml_out;
@@ -486,21 +489,13 @@ __device__ inline void bulk_incident(const Dev& d, const PullArgs& a, int x, int
   else { v[Vx_zmax] = free_face<PLAIN>(o[Vx_zmax], zc_); v[Vy_zmax] = free_face<PLAIN>(o[Vy_zmax], zd); }
 }
 
-// Bulk free-space cells, one cell per thread (zc = 1): block = 64 x by cells
-// of one plane.  Neighbour ports are loaded directly; they are the block's
-// own rows or the rows/planes of tiles that ran just before on the same XCD.
+// Bulk free-space cell update, one cell per lane: pull the six faces
+// (connect and outer boundary) and scatter.
 template <typename T, bool PLAIN, bool PULL>
-__global__ __launch_bounds__(256) void k_bulk_cell(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                  PullArgs a, BulkMap m) {
-  int bx, byi, z;
-  bulk_tile(m, bx, byi, z);
-  const int x = bx * 64 + threadIdx.x;
-  const int y = byi * m.by + threadIdx.y;
-  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
+__device__ inline void bulk_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                   const PullArgs& a, long long c, int x, int y, int z) {
   const long long P = d.P;
   const long long sxy = (long long)d.nx * d.ny;
-  const long long c = x + (long long)d.nx * (y + (long long)d.ny * z);
-  if ((d.mask[c >> 5] >> (c & 31)) & 1u) return;
   double v[12];
   if (PULL) {
     double o[12];
@@ -520,6 +515,22 @@ __global__ __launch_bounds__(256) void k_bulk_cell(Dev d, const T* __restrict__ 
   }
   if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
   store_cell(Lout, P, c, v);
+}
+
+// Bulk free-space cells, one cell per thread (zc = 1): block = 64 x by cells
+// of one plane.  Neighbour ports are loaded directly; they are the block's
+// own rows or the rows/planes of tiles that ran just before on the same XCD.
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_bulk_cell(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                  PullArgs a, BulkMap m) {
+  int bx, byi, z;
+  bulk_tile(m, bx, byi, z);
+  const int x = bx * 64 + threadIdx.x;
+  const int y = byi * m.by + threadIdx.y;
+  if (x >= d.nx || y >= d.ny || z >= d.nz) return;
+  const long long c = x + (long long)d.nx * (y + (long long)d.ny * z);
+  if ((d.mask[c >> 5] >> (c & 31)) & 1u) return;
+  bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z);
 }
 
 // Vector access of W consecutive link values (W*sizeof(T) = 16 bytes).
@@ -712,10 +723,8 @@ __global__ __launch_bounds__(256) void k_bulk_z(Dev d, const T* __restrict__ Lin
 
 // PML cells: scatter_PML.F90 stages 1-13.
 template <typename T, bool PLAIN, bool PULL>
-__global__ __launch_bounds__(256) void k_pml(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                            PullArgs a, int slot) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n_pml) return;
+__device__ inline void pml_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                  const PullArgs& a, int slot, int i) {
   const int32_t cw = d.pml_cell[i];
   const long long c = cw & kCellMask;
   const int nxy = d.nx * d.ny;
@@ -738,7 +747,7 @@ __global__ __launch_bounds__(256) void k_pml(Dev d, const T* __restrict__ Lin, T
   const double Csx2 = Pp.Csx2, Csy2 = Pp.Csy2, Csz2 = Pp.Csz2;
   const double Csy_sz = Pp.Csy_sz, Csz_sx = Pp.Csz_sx, Csx_sy = Pp.Csx_sy;
   const long long n = d.n_pml_state;
-  const long long si = d.pml_state_idx ? d.pml_state_idx[i] : i;
+  const long long si = i;  // state slot = list position
   double* S = d.pml_state;
   // PML port aliases (PML_module.F90:60-71), 0-based
   enum { Vxny = 0, Vxnz, Vxpy, Vxpz, Vynx, Vynz, Vypx, Vypz, Vznx, Vzny, Vzpx, Vzpy };
@@ -883,6 +892,13 @@ __global__ __launch_bounds__(256) void k_pml(Dev d, const T* __restrict__ Lin, T
   store_cell(Lout, d.P, c, v);
 }
 
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_pml(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                            PullArgs a, int slot, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) pml_update<T, PLAIN, PULL>(d, Lin, Lout, a, slot, i);
+}
+
 // Thevenin node of a special cell (scatter.F90:274-547): materials,
 // optional point source and output row, filter update, 12-port scatter.
 // IO = false compiles out the excitation and output branches.
@@ -999,10 +1015,8 @@ __device__ inline void cell_coords(const Dev& d, long long c, int& x, int& y, in
 // face; the Thevenin node, or the free-space branch for code 0 cells that
 // only touch a special face.
 template <typename T, bool PLAIN, bool PULL>
-__global__ __launch_bounds__(256) void k_generic(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                PullArgs a, int timestep, int slot) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n_gen) return;
+__device__ inline void generic_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                      const PullArgs& a, int timestep, int slot, int i) {
   const long long c = d.gen_cell[i];
   int x, y, z;
   cell_coords(d, c, x, y, z);
@@ -1021,13 +1035,18 @@ __global__ __launch_bounds__(256) void k_generic(Dev d, const T* __restrict__ Li
   store_cell(Lout, d.P, c, v);
 }
 
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_generic(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                PullArgs a, int timestep, int slot, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) generic_update<T, PLAIN, PULL>(d, Lin, Lout, a, timestep, slot, i);
+}
+
 // Material cells (dispersive, PEC, PMC) without sources, probes or special
 // faces: the bulk of a dielectric body, with compact per-cell metadata.
 template <typename T, bool PLAIN, bool PULL>
-__global__ __launch_bounds__(256) void k_material(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                 PullArgs a) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.n_mat) return;
+__device__ inline void material_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                       const PullArgs& a, int i) {
   const long long c = d.mat_cell[i];
   int x, y, z;
   cell_coords(d, c, x, y, z);
@@ -1040,6 +1059,59 @@ __global__ __launch_bounds__(256) void k_material(Dev d, const T* __restrict__ L
   }
   node_update<false>(d, v, d.mat_num[i], d.mat_filt[i], -1, -1, 0, 0);
   store_cell(Lout, d.P, c, v);
+}
+
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_material(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                 PullArgs a, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) material_update<T, PLAIN, PULL>(d, Lin, Lout, a, i);
+}
+
+// Pure bulk lines: workgroup = 256 consecutive cells of a row segment;
+// cells of lines with any special cell are left to the mixed-line and class
+// kernels.
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_bulk_lines(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                   PullArgs a) {
+  const int nbx = (d.nx + 255) >> 8;
+  const int row = blockIdx.x / nbx;  // y + ny*z
+  const int x = (blockIdx.x - row * nbx) * 256 + threadIdx.x;
+  if (x >= d.nx) return;
+  const long long c = x + (long long)d.nx * row;
+  if (d.line_info[c / d.line_len] != 0u) return;
+  const int z = row / d.ny;
+  const int y = row - z * d.ny;
+  bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z);
+}
+
+// Mixed lines: each lane updates one cell of a line with its own class
+// code, so every line is written whole by one wavefront.
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_mixed_lines(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                    PullArgs a, const int32_t* __restrict__ lines, int n_lines,
+                                                    int timestep, int slot) {
+  const int per = 256 / d.line_len;
+  const int li = blockIdx.x * per + threadIdx.x / d.line_len;
+  if (li >= n_lines) return;
+  const int k = threadIdx.x % d.line_len;
+  const long long ln = lines[li];
+  const long long c = ln * d.line_len + k;
+  if (c >= d.N) return;
+  const uint32_t bits = d.line_bits[ln];
+  if (!((bits >> k) & 1u)) {
+    int x, y, z;
+    cell_coords(d, c, x, y, z);
+    bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z);
+    return;
+  }
+  const uint32_t base = d.line_info[ln] >> 2;
+  const uint32_t rec = d.mixed_rec[base + __popc(bits & ((1u << k) - 1u))];
+  const int cls = (int)(rec >> 30);
+  const int i = (int)(rec & 0x3fffffffu);
+  if (cls == 1) pml_update<T, PLAIN, PULL>(d, Lin, Lout, a, slot, i);
+  else if (cls == 2) material_update<T, PLAIN, PULL>(d, Lin, Lout, a, i);
+  else generic_update<T, PLAIN, PULL>(d, Lin, Lout, a, timestep, slot, i);
 }
 
 // Connect only (no scatter): incident voltages into Lout, used to hand the

@@ -72,6 +72,9 @@ struct tlm_handle {
   int cur = 0;
   Dev d{};
   int n_gen = 0, n_pml = 0, n_mat = 0;
+  int n_pure_pml = 0, n_pure_mat = 0, n_pure_gen = 0, n_mixed_lines = 0;
+  long long n_mixed_cells = 0;
+  const int32_t* d_mixed_lines = nullptr;
   const int32_t* d_gen_cell = nullptr;
   const int32_t* d_gen_fd = nullptr;
   const int32_t* d_pml_cell = nullptr;
@@ -260,13 +263,14 @@ int finalize(tlm_handle* h) {
   };
 
   // ---- cell classes ------------------------------------------------------
-  std::vector<uint32_t> mask((N + 31) / 32, 0u);
-  std::vector<int32_t> pml_cell, pml_sidx, pml_fd;
-  std::vector<uint16_t> pml_par;
-  std::vector<int2> pml_out_tab, pml_fd_tab;
-  std::vector<int32_t> gen_cell, gen_code, gen_mat, gen_filt, gen_exc, gen_out, gen_fd;
-  std::vector<int32_t> mat_cell, mat_filt;
-  std::vector<uint16_t> mat_num;
+  // class per cell: 0 bulk, 1 PML, 2 material, 3 generic
+  struct PmlRec { int32_t c; uint16_t par; int32_t orow, fdc; };
+  struct MatRec { int32_t c; uint16_t m; int32_t f; };
+  struct GenRec { int32_t c, code, m, f, er, orow, fdc; };
+  std::vector<uint8_t> cls(N, 0);
+  std::vector<PmlRec> pr;
+  std::vector<MatRec> mr;
+  std::vector<GenRec> gr;
   long long max_filt = 0;
   int out_rank = 0, exc_rank = 0;
   size_t fdi = 0;  // walks fd_cell in scan order
@@ -277,7 +281,6 @@ int finalize(tlm_handle* h) {
     int32_t fdc = has_fd ? (int32_t)fdi : -1;
     if (has_fd) fdi++;
     if (code == 0 && !has_fd) continue;
-    mask[c >> 5] |= 1u << (c & 31);
     if (code < 0 || code > h->n_special) return fail(h, TLM_ERR_ARG, "cell code out of range");
     int s = code - 1;
     int pmlc = code ? h->c_pml[s] : 0;
@@ -296,14 +299,8 @@ int finalize(tlm_handle* h) {
       int par = h->pml_cell_param[pmlc - 1];
       if (par < 1 || par > (int)h->pml_par.size() || par > 65536)
         return fail(h, TLM_ERR_ARG, "PML parameter index out of range");
-      int32_t li = (int32_t)pml_cell.size();
-      int32_t cw = (int32_t)c;
-      if (orow >= 0) { cw |= scn::kFlagOut; pml_out_tab.push_back(make_int2(li, orow)); }
-      if (fdc >= 0) { cw |= scn::kFlagFd; pml_fd_tab.push_back(make_int2(li, fdc)); }
-      pml_cell.push_back(cw);
-      pml_sidx.push_back(pmlc - 1);
-      pml_par.push_back((uint16_t)(par - 1));
-      pml_fd.push_back(fdc);
+      cls[c] = 1;
+      pr.push_back({(int32_t)c, (uint16_t)(par - 1), orow, fdc});
     } else {
       int m = code ? h->c_mat[s] : 0;
       int f = code ? h->c_filt[s] : 0;
@@ -316,20 +313,99 @@ int finalize(tlm_handle* h) {
       }
       int er = (code && h->c_exc[s]) ? exc_rank++ : -1;
       if (code && m && er < 0 && orow < 0 && fdc < 0 && m <= 65535) {
-        mat_cell.push_back((int32_t)c);
-        mat_num.push_back((uint16_t)m);
-        mat_filt.push_back(f);
+        cls[c] = 2;
+        mr.push_back({(int32_t)c, (uint16_t)m, f});
         continue;
       }
-      gen_cell.push_back((int32_t)c);
-      gen_code.push_back(code);
-      gen_mat.push_back(m);
-      gen_filt.push_back(f);
-      gen_exc.push_back(er);
-      gen_out.push_back(orow);
-      gen_fd.push_back(fdc);
+      cls[c] = 3;
+      gr.push_back({(int32_t)c, code, m, f, er, orow, fdc});
     }
   }
+  if (N > (long long)scn::kCellMask) return fail(h, TLM_ERR_UNSUPPORTED, "more than 2^28 cells per handle");
+
+  // ---- 128-byte lines ------------------------------------------------------
+  // A line is 128 bytes of one link plane (32 FP32 or 16 FP64 cells).  A
+  // line of one class only is written whole by that class's kernel; a mixed
+  // line is written whole by the mixed-line kernel, which updates each of
+  // its cells with that cell's class code.  No line is ever written in part.
+  const int L = (int)(128 / link_size(h));
+  const long long n_lines = (N + L - 1) / L;
+  std::vector<uint32_t> line_info(n_lines, 0), line_bits(n_lines, 0);
+  std::vector<uint8_t> line_pure(n_lines, 0);
+  std::vector<uint32_t> mask((N + 31) / 32, 0u);
+  long long n_mixed_cells = 0;
+  for (long long ln = 0; ln < n_lines; ln++) {
+    long long c0 = ln * L, c1 = std::min(N, c0 + L);
+    uint8_t k0 = cls[c0];
+    bool same = true;
+    uint32_t bits = 0;
+    for (long long c = c0; c < c1; c++) {
+      if (cls[c] != k0) same = false;
+      if (cls[c]) bits |= 1u << (c - c0);
+    }
+    if (same && k0 == 0) { line_info[ln] = 0; continue; }
+    for (long long c = c0; c < c1; c++) mask[c >> 5] |= 1u << (c & 31);
+    if (same) { line_info[ln] = 1; line_pure[ln] = 1; continue; }
+    line_info[ln] = 2;
+    line_bits[ln] = bits;
+  }
+  auto pure_of = [&](int32_t c) { return line_pure[c / L] != 0; };
+  std::stable_partition(pr.begin(), pr.end(), [&](const PmlRec& r) { return pure_of(r.c); });
+  std::stable_partition(mr.begin(), mr.end(), [&](const MatRec& r) { return pure_of(r.c); });
+  std::stable_partition(gr.begin(), gr.end(), [&](const GenRec& r) { return pure_of(r.c); });
+  int n_pure_pml = 0, n_pure_mat = 0, n_pure_gen = 0;
+  while (n_pure_pml < (int)pr.size() && pure_of(pr[n_pure_pml].c)) n_pure_pml++;
+  while (n_pure_mat < (int)mr.size() && pure_of(mr[n_pure_mat].c)) n_pure_mat++;
+  while (n_pure_gen < (int)gr.size() && pure_of(gr[n_pure_gen].c)) n_pure_gen++;
+  // mixed lines and their per-cell records (class << 30 | list index), in
+  // scan order: the tails of the three lists are exactly those cells
+  std::vector<int32_t> mixed_lines;
+  std::vector<uint32_t> mixed_rec;
+  {
+    int ip = n_pure_pml, im = n_pure_mat, ig = n_pure_gen;
+    for (long long ln = 0; ln < n_lines; ln++) {
+      if (line_info[ln] != 2) continue;
+      mixed_lines.push_back((int32_t)ln);
+      line_info[ln] = 2u | ((uint32_t)mixed_rec.size() << 2);
+      long long c0 = ln * L, c1 = std::min(N, c0 + L);
+      for (long long c = c0; c < c1; c++) {
+        n_mixed_cells++;
+        switch (cls[c]) {
+          case 1: mixed_rec.push_back((1u << 30) | (uint32_t)ip++); break;
+          case 2: mixed_rec.push_back((2u << 30) | (uint32_t)im++); break;
+          case 3: mixed_rec.push_back((3u << 30) | (uint32_t)ig++); break;
+          default: break;
+        }
+      }
+    }
+    if (ip != (int)pr.size() || im != (int)mr.size() || ig != (int)gr.size())
+      return fail(h, TLM_ERR_ARG, "internal: mixed-line records inconsistent");
+    if (mixed_rec.size() >= (1u << 28)) return fail(h, TLM_ERR_UNSUPPORTED, "too many mixed-line cells");
+  }
+  // flat per-class arrays in list order
+  std::vector<int32_t> pml_cell, pml_fd;
+  std::vector<uint16_t> pml_par;
+  std::vector<int2> pml_out_tab, pml_fd_tab;
+  for (size_t i = 0; i < pr.size(); i++) {
+    int32_t cw = pr[i].c;
+    if (pr[i].orow >= 0) { cw |= scn::kFlagOut; pml_out_tab.push_back(make_int2((int)i, pr[i].orow)); }
+    if (pr[i].fdc >= 0) { cw |= scn::kFlagFd; pml_fd_tab.push_back(make_int2((int)i, pr[i].fdc)); }
+    pml_cell.push_back(cw);
+    pml_par.push_back(pr[i].par);
+    pml_fd.push_back(pr[i].fdc);
+  }
+  std::vector<int32_t> mat_cell, mat_filt;
+  std::vector<uint16_t> mat_num;
+  for (auto& r : mr) { mat_cell.push_back(r.c); mat_num.push_back(r.m); mat_filt.push_back(r.f); }
+  std::vector<int32_t> gen_cell, gen_code, gen_mat, gen_filt, gen_exc, gen_out, gen_fd;
+  for (auto& r : gr) {
+    gen_cell.push_back(r.c); gen_code.push_back(r.code); gen_mat.push_back(r.m); gen_filt.push_back(r.f);
+    gen_exc.push_back(r.er); gen_out.push_back(r.orow); gen_fd.push_back(r.fdc);
+  }
+  h->n_pure_pml = n_pure_pml;
+  h->n_pure_mat = n_pure_mat;
+  h->n_pure_gen = n_pure_gen;
+  h->n_mixed_cells = n_mixed_cells;
   if (N > (long long)scn::kCellMask) return fail(h, TLM_ERR_UNSUPPORTED, "more than 2^28 cells per handle");
   if (exc_rank != h->n_cell_rows)
     return fail(h, TLM_ERR_ARG, "cell excitation rows do not match the cell codes");
@@ -385,12 +461,6 @@ int finalize(tlm_handle* h) {
   if ((rc = dev_upload(h, mask, &d.mask))) return rc;
   h->n_pml = d.n_pml = (int)pml_cell.size();
   if ((rc = dev_upload(h, pml_cell, &d.pml_cell))) return rc;
-  {
-    bool identity = true;
-    for (size_t i = 0; i < pml_sidx.size() && identity; i++) identity = pml_sidx[i] == (int32_t)i;
-    if (identity) d.pml_state_idx = nullptr;
-    else if ((rc = dev_upload(h, pml_sidx, &d.pml_state_idx))) return rc;
-  }
   if ((rc = dev_upload(h, pml_par, &d.pml_par))) return rc;
   if ((rc = dev_upload(h, pml_out_tab, &d.pml_out_tab))) return rc;
   d.n_pml_out = (int)pml_out_tab.size();
@@ -403,8 +473,14 @@ int finalize(tlm_handle* h) {
     if ((rc = dev_upload(h, cells, &h->d_pml_cell))) return rc;
     if ((rc = dev_upload(h, pml_fd, &h->d_pml_fd))) return rc;
   }
+  d.line_len = L;
+  if ((rc = dev_upload(h, line_info, &d.line_info))) return rc;
+  if ((rc = dev_upload(h, line_bits, &d.line_bits))) return rc;
+  if ((rc = dev_upload(h, mixed_rec, &d.mixed_rec))) return rc;
+  if ((rc = dev_upload(h, mixed_lines, &h->d_mixed_lines))) return rc;
+  h->n_mixed_lines = (int)mixed_lines.size();
   if ((rc = dev_upload(h, h->pml_par, &d.pml_params))) return rc;
-  d.n_pml_state = std::max(n_pml_state, 1);
+  d.n_pml_state = std::max(d.n_pml, 1);
   {
     void* p;
     const size_t words = h->form == TLM_FORM_PLAIN_FP64 ? scn::PS_N : scn::PC_N;
@@ -513,7 +589,7 @@ int drain_ring(tlm_handle* h) {
 // Cells per thread of the vector bulk kernel (16-byte accesses).
 int sched_variant(const tlm_handle* h) {
   if (h->sched_variant >= 0) return h->sched_variant;
-  return h->form == TLM_FORM_PLAIN_FP64 ? 0 : 2;
+  return 3;
 }
 
 int bulk_vec_width(const tlm_handle* h) {
@@ -549,38 +625,59 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
   T* Lout = (T*)h->planes[h->cur ^ 1];
   {
     ProfScope ps(h, TLM_KCLASS_BULK);
-    scn::BulkMap m = bulk_map(h);
-    dim3 bb(64, m.by, 1);
-    const int w = bulk_vec_width(h);
-    if (w > 1) {
-      constexpr int W = sizeof(T) == 4 ? 4 : 2;
-      if (pull) hipLaunchKernelGGL((scn::k_bulk_vec<T, PLAIN, true, W>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
-      else hipLaunchKernelGGL((scn::k_bulk_vec<T, PLAIN, false, W>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
-    } else if (m.zc == 1) {
-      if (pull) hipLaunchKernelGGL((scn::k_bulk_cell<T, PLAIN, true>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
-      else hipLaunchKernelGGL((scn::k_bulk_cell<T, PLAIN, false>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+    if (sched_variant(h) == 3) {
+      dim3 g((unsigned)(((d.nx + 255) / 256) * (long long)d.ny * d.nz));
+      if (pull) hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a);
+      else hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a);
     } else {
-      if (pull) hipLaunchKernelGGL((scn::k_bulk_z<T, PLAIN, true>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
-      else hipLaunchKernelGGL((scn::k_bulk_z<T, PLAIN, false>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+      scn::BulkMap m = bulk_map(h);
+      dim3 bb(64, m.by, 1);
+      const int w = bulk_vec_width(h);
+      if (w > 1) {
+        constexpr int W = sizeof(T) == 4 ? 4 : 2;
+        if (pull) hipLaunchKernelGGL((scn::k_bulk_vec<T, PLAIN, true, W>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+        else hipLaunchKernelGGL((scn::k_bulk_vec<T, PLAIN, false, W>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+      } else if (m.zc == 1) {
+        if (pull) hipLaunchKernelGGL((scn::k_bulk_cell<T, PLAIN, true>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+        else hipLaunchKernelGGL((scn::k_bulk_cell<T, PLAIN, false>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+      } else {
+        if (pull) hipLaunchKernelGGL((scn::k_bulk_z<T, PLAIN, true>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+        else hipLaunchKernelGGL((scn::k_bulk_z<T, PLAIN, false>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+      }
     }
   }
-  if (d.n_pml) {
+  if (h->n_mixed_lines) {
+    ProfScope ps(h, TLM_KCLASS_MIXED);
+    const int per = 256 / d.line_len;
+    dim3 g((h->n_mixed_lines + per - 1) / per);
+    if (pull)
+      hipLaunchKernelGGL((scn::k_mixed_lines<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a,
+                         h->d_mixed_lines, h->n_mixed_lines, timestep, slot);
+    else
+      hipLaunchKernelGGL((scn::k_mixed_lines<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a,
+                         h->d_mixed_lines, h->n_mixed_lines, timestep, slot);
+  }
+  if (h->n_pure_pml) {
     ProfScope ps(h, TLM_KCLASS_PML);
-    dim3 g((d.n_pml + 255) / 256);
-    if (pull) hipLaunchKernelGGL((scn::k_pml<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot);
-    else hipLaunchKernelGGL((scn::k_pml<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot);
+    dim3 g((h->n_pure_pml + 255) / 256);
+    if (pull) hipLaunchKernelGGL((scn::k_pml<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot, h->n_pure_pml);
+    else hipLaunchKernelGGL((scn::k_pml<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot, h->n_pure_pml);
   }
-  if (d.n_mat) {
+  if (h->n_pure_mat) {
     ProfScope ps(h, TLM_KCLASS_MATERIAL);
-    dim3 g((d.n_mat + 255) / 256);
-    if (pull) hipLaunchKernelGGL((scn::k_material<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a);
-    else hipLaunchKernelGGL((scn::k_material<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a);
+    dim3 g((h->n_pure_mat + 255) / 256);
+    if (pull) hipLaunchKernelGGL((scn::k_material<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, h->n_pure_mat);
+    else hipLaunchKernelGGL((scn::k_material<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, h->n_pure_mat);
   }
-  if (d.n_gen) {
+  if (h->n_pure_gen) {
     ProfScope ps(h, TLM_KCLASS_GENERIC);
-    dim3 g((d.n_gen + 255) / 256);
-    if (pull) hipLaunchKernelGGL((scn::k_generic<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, timestep, slot);
-    else hipLaunchKernelGGL((scn::k_generic<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, timestep, slot);
+    dim3 g((h->n_pure_gen + 255) / 256);
+    if (pull)
+      hipLaunchKernelGGL((scn::k_generic<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, timestep, slot,
+                         h->n_pure_gen);
+    else
+      hipLaunchKernelGGL((scn::k_generic<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, timestep, slot,
+                         h->n_pure_gen);
   }
   return TLM_OK;
 }
@@ -991,7 +1088,7 @@ int tlm_read_dft(tlm_handle* h, double* re_im) {
 
 int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap) {
   if (!h) return TLM_ERR_ARG;
-  if (variant < 0 || variant > 2 || (rows != 1 && rows != 2 && rows != 4) || zc < 1)
+  if (variant < 0 || variant > 3 || (rows != 1 && rows != 2 && rows != 4) || zc < 1)
     return fail(h, TLM_ERR_ARG, "bad bulk schedule");
   h->sched_variant = variant;
   h->sched_by = rows;
@@ -1026,17 +1123,8 @@ int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
   int rc;
   if ((rc = finalize(h))) return rc;
   const int64_t sl = (int64_t)link_size(h);
-  const int64_t n_bulk = h->N - h->n_pml - h->n_gen - h->n_mat;
-  // bulk: 12 links read + 12 written, plus its bit of the class mask
-  cells[TLM_KCLASS_BULK] = n_bulk;
-  bytes[TLM_KCLASS_BULK] = n_bulk * 24 * sl + (h->N + 7) / 8;
-  // PML: links, FP64 state words read and written (24 plain, 18 precision),
-  // packed cell word and 16-bit parameter row, state index when not implied
-  cells[TLM_KCLASS_PML] = h->n_pml;
-  {
-    const int64_t words = h->form == TLM_FORM_PLAIN_FP64 ? scn::PS_N : scn::PC_N;
-    bytes[TLM_KCLASS_PML] = (int64_t)h->n_pml * (24 * sl + 2 * 8 * words + 4 + 2 + (h->d.pml_state_idx ? 4 : 0));
-  }
+  const int64_t L = h->d.line_len;
+  const int64_t n_lines = (h->N + L - 1) / L;
   // filter words of a dispersive cell: 3 components x (carried words read
   // and written) for eps and mu, FP64
   auto filter_bytes = [&](int m) -> int64_t {
@@ -1046,24 +1134,45 @@ int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
     int64_t words = (vm.eps_filter_exists ? vm.eps_order + 1 : 0) + (vm.mu_filter_exists ? vm.mu_order + 1 : 0);
     return 3 * 2 * 8 * words;
   };
+  // PML: links, FP64 state words read and written (24 plain, 18 precision),
+  // packed cell word and 16-bit parameter row
+  const int64_t pml_words = h->form == TLM_FORM_PLAIN_FP64 ? scn::PS_N : scn::PC_N;
+  const int64_t pml_cell_bytes = 24 * sl + 2 * 8 * pml_words + 4 + 2;
   // generic: links, 7 int32 list fields, filter words
-  int64_t gb = (int64_t)h->n_gen * (24 * sl + 7 * 4);
-  {
-    std::vector<int32_t> gm(h->n_gen);
-    if (h->n_gen) hipMemcpy(gm.data(), h->d.gen_mat, gm.size() * sizeof(int32_t), hipMemcpyDeviceToHost);
-    for (int32_t m : gm) gb += filter_bytes(m);
-  }
-  cells[TLM_KCLASS_GENERIC] = h->n_gen;
-  bytes[TLM_KCLASS_GENERIC] = gb;
+  std::vector<int32_t> gm(h->n_gen);
+  if (h->n_gen) HIP_TRY(h, hipMemcpy(gm.data(), h->d.gen_mat, gm.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  auto gen_bytes = [&](int i0, int i1) {
+    int64_t b = 0;
+    for (int i = i0; i < i1; i++) b += 24 * sl + 7 * 4 + filter_bytes(gm[i]);
+    return b;
+  };
   // material: links, cell word, 16-bit material number, filter number, filter words
-  int64_t mb = (int64_t)h->n_mat * (24 * sl + 4 + 2 + 4);
-  {
-    std::vector<uint16_t> mm(h->n_mat);
-    if (h->n_mat) hipMemcpy(mm.data(), h->d.mat_num, mm.size() * sizeof(uint16_t), hipMemcpyDeviceToHost);
-    for (uint16_t m : mm) mb += filter_bytes(m);
-  }
-  cells[TLM_KCLASS_MATERIAL] = h->n_mat;
-  bytes[TLM_KCLASS_MATERIAL] = mb;
+  std::vector<uint16_t> mm(h->n_mat);
+  if (h->n_mat) HIP_TRY(h, hipMemcpy(mm.data(), h->d.mat_num, mm.size() * sizeof(uint16_t), hipMemcpyDeviceToHost));
+  auto mat_bytes = [&](int i0, int i1) {
+    int64_t b = 0;
+    for (int i = i0; i < i1; i++) b += 24 * sl + 4 + 2 + 4 + filter_bytes(mm[i]);
+    return b;
+  };
+  const int64_t n_special_mixed = (h->n_pml - h->n_pure_pml) + (h->n_mat - h->n_pure_mat) + (h->n_gen - h->n_pure_gen);
+  // bulk: cells of the pure bulk lines, 12 links read + 12 written, plus the
+  // line table (line kernel) or the class mask (tile kernels)
+  const int64_t n_bulk = h->N - h->n_mixed_cells - h->n_pure_pml - h->n_pure_mat - h->n_pure_gen;
+  cells[TLM_KCLASS_BULK] = n_bulk;
+  bytes[TLM_KCLASS_BULK] = n_bulk * 24 * sl + (sched_variant(h) == 3 ? 4 * n_lines : (h->N + 7) / 8);
+  // mixed lines: every cell with its class's bytes, plus the line word, the
+  // line bit mask, the line number and one record per special cell
+  cells[TLM_KCLASS_MIXED] = h->n_mixed_cells;
+  bytes[TLM_KCLASS_MIXED] = (h->n_mixed_cells - n_special_mixed) * 24 * sl +
+                            (int64_t)(h->n_pml - h->n_pure_pml) * pml_cell_bytes +
+                            mat_bytes(h->n_pure_mat, h->n_mat) + gen_bytes(h->n_pure_gen, h->n_gen) +
+                            (int64_t)h->n_mixed_lines * 12 + n_special_mixed * 4;
+  cells[TLM_KCLASS_PML] = h->n_pure_pml;
+  bytes[TLM_KCLASS_PML] = (int64_t)h->n_pure_pml * pml_cell_bytes;
+  cells[TLM_KCLASS_GENERIC] = h->n_pure_gen;
+  bytes[TLM_KCLASS_GENERIC] = gen_bytes(0, h->n_pure_gen);
+  cells[TLM_KCLASS_MATERIAL] = h->n_pure_mat;
+  bytes[TLM_KCLASS_MATERIAL] = mat_bytes(0, h->n_pure_mat);
   cells[TLM_KCLASS_DFT] = (int64_t)h->pt_row.size() * (int64_t)h->freq.size();
   bytes[TLM_KCLASS_DFT] = cells[TLM_KCLASS_DFT] * 2 * 2 * 8;
   return TLM_OK;

@@ -231,8 +231,9 @@ int tlm_read_dft(tlm_handle* h, double* re_im);
 
 /* ---- schedule and measurement (no counterpart in the reference) ---------- */
 
-/* Bulk-kernel schedule: variant 0 = one cell per lane, one xy-plane tile
- * per workgroup; variant 1 = workgroups marching zc planes; variant 2 = 16-byte
+/* Bulk-kernel schedule: variant 3 (default) = one 256-cell row segment per
+ * workgroup over the pure bulk lines; variant 0 = one cell per lane, 64 x rows
+ * tiles; variant 1 = workgroups marching zc planes; variant 2 = 16-byte
  * vector accesses, several consecutive cells per lane (falls back to 0 when
  * nx is not a multiple of the vector width); rows = tile height (1, 2 or 4
  * rows); xcd_remap = give each XCD a contiguous tile range.
@@ -247,7 +248,8 @@ int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc,
 #define TLM_KCLASS_GENERIC 2
 #define TLM_KCLASS_DFT 3
 #define TLM_KCLASS_MATERIAL 4
-#define TLM_N_KCLASS 5
+#define TLM_KCLASS_MIXED 5
+#define TLM_N_KCLASS 6
 
 /* When enabled, every kernel launch of a step is bracketed by HIP events
  * recorded on the handle's own stream.  tlm_read_kernel_times waits for the

@@ -99,7 +99,7 @@ def free_cube(n=16, steps=60, r=-1.0, dft=()):
                    dft_frequencies=dft)
 
 
-def pml_sphere(n=24, steps=80, npml=4, sphere=True, huygens=True, sigma=0.01):
+def pml_sphere(n=24, steps=80, npml=4, sphere=True, huygens=True, sigma=0.01, radius=None):
     """Shrunk 256^3 + PML + dielectric sphere + x-directed plane wave."""
     dl = 5e-3
     dt = dl / (2 * 2.99792458e8)
@@ -113,7 +113,7 @@ def pml_sphere(n=24, steps=80, npml=4, sphere=True, huygens=True, sigma=0.01):
                          OutputPoint((n // 2, n // 2, 1), "hx")])     # inside the PML
     if sphere:
         p.materials = [VolumeMaterial("dispersive", Sfilter.constant(4.0), Sfilter.constant(1.0), sigma_e=sigma)]
-        p.material_regions = [(Sphere(n / 2 + 1.3, n / 2 - 0.7, n / 2 + 0.4, n / 6), 1)]
+        p.material_regions = [(Sphere(n / 2 + 1.3, n / 2 - 0.7, n / 2 + 0.4, radius or n / 6), 1)]
         p.outputs.append(OutputPoint((n // 2 + 1, n // 2, n // 2), "ez"))
     if huygens:
         b = npml + 2

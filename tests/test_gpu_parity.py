@@ -21,6 +21,9 @@ CASES = {
     "cavity_free_space": lambda: problems.cavity(),
     "tem_pml": problems.tem_pml,
     "pml_sphere": problems.pml_sphere,
+    # wide enough for 128-byte lines of one class only (PML slabs, sphere
+    # chords, Huygens face rows) next to mixed lines
+    "pml_sphere_64": lambda: problems.pml_sphere(64, 50, 8, radius=20.0),
     "dispersive_blocks": problems.dispersive_blocks,
     "special_mix": problems.special_mix,
     "huygens_surface": problems.huygens_surface,
@@ -98,7 +101,8 @@ def test_upload_then_step(oracle_lib):
     assert np.array_equal(o.read_cell_outputs()[0], g.read_cell_outputs()[0])
 
 
-@pytest.mark.parametrize("name", ["free_cube", "pml_sphere", "dispersive_blocks", "special_mix", "huygens_surface"])
+@pytest.mark.parametrize("name", ["free_cube", "pml_sphere", "pml_sphere_64", "dispersive_blocks", "special_mix",
+                                  "huygens_surface"])
 def test_precision_form(name, oracle_lib):
     from oracle import oracle
 
@@ -120,12 +124,12 @@ def test_precision_form(name, oracle_lib):
 
 
 @pytest.mark.parametrize("sched", [(0, 1, 1, 0), (0, 2, 1, 1), (1, 1, 4, 1), (1, 4, 8, 0), (1, 2, 32, 1),
-                                   (2, 4, 1, 0), (2, 1, 1, 1)])
+                                   (2, 4, 1, 0), (2, 1, 1, 1), (3, 1, 1, 0)])
 def test_schedules_bitwise(sched, oracle_lib):
     """Every bulk schedule gives the oracle's result bit for bit."""
     from oracle import oracle
 
-    for name in ("free_cube_odd", "pml_sphere", "free_cube"):
+    for name in ("free_cube_odd", "pml_sphere", "free_cube", "pml_sphere_64"):
         t = build_tables(CASES[name]())
         steps = min(t.n_timesteps, 60)
         o = oracle.handle(t, output_capacity=steps)
