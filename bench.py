@@ -13,10 +13,16 @@ A step is one complete loop body (excitation, scatter, cell output,
 connect, outer boundary) over the whole grid with inputs resident in HBM.
 value = cells * timed steps / wall time of the timed region (Gcell-updates/s).
 
-The roofline object prices the dominant kernel (the bulk free-space
-kernel) at its algorithmic bytes per launch (24 link words of 4 B per
-cell plus the class bitmask) over its average launch duration, measured
-with HIP events on the engine's own stream in a separate profiled pass.
+The roofline object prices the dominant kernel (the kernel class with
+the longest average launch: the pure free-space line kernel or the
+mixed-line kernel on this workload) at its algorithmic bytes per launch
+(tlm_kernel_bytes: 24 link words per cell, state and table words) over its
+average launch duration, measured with HIP events on the engine's own
+stream in a separate profiled pass.  traffic is the HBM bytes per launch
+of the same kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this
+workload (profiles/traffic_<workload>.json, counters scaled by a copy of
+known size), or null when no such file matches the kernel's algorithmic
+bytes.
 
 cpu_baseline times the CPU oracle (a single-threaded FP64 restatement of
 the reference's loop, built with gcc -O2) on a bounded sample of the same
@@ -68,6 +74,20 @@ def workload(n=256, npml=8, radius=60, steps=4000, seed=2024):
         functions=[ExcitationFunction("gaussian", (1.0, 40 * dt, 160 * dt))],
         huygens=HuygensBox(Box(b, n - b + 1, b, n - b + 1, b, n - b + 1), 1, 90.0, 0.0, -1.0, 0.0, side=1),
         outputs=probes)
+
+
+def measured_traffic(a, kernel, alg_bytes):
+    """HBM bytes per launch of `kernel` from the committed counter summary of
+    this workload, if it was measured on the same kernel and tables."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{a.n}cube_{a.form}.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)["kernels"][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    if rec.get("algorithmic_bytes") != alg_bytes:
+        return None
+    return rec.get("traffic_bytes")
 
 
 def cpu_baseline(tables, steps):
@@ -141,8 +161,10 @@ def main():
         per_class[k] = {"cells": cells, "bytes_per_launch": b, "avg_ms": round(avg_ms, 5),
                         "GB_s": round(b / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None}
     print(f"[bench] kernels {json.dumps(per_class)}", file=sys.stderr, flush=True)
-    bulk = per_class["bulk"]
-    achieved = bulk["GB_s"]
+    dom = max((k for k in per_class if k != "dft" and per_class[k]["cells"]),
+              key=lambda k: per_class[k]["avg_ms"])
+    achieved = per_class[dom]["GB_s"]
+    traffic = measured_traffic(a, dom, per_class[dom]["bytes_per_launch"])
     h.read_cell_outputs()
     h.close()
 
@@ -177,8 +199,8 @@ def main():
             "whole_step_frac_of_hbm": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "kernels": per_class,
         },
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

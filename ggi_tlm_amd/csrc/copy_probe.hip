@@ -48,6 +48,23 @@ __global__ __launch_bounds__(256) void k_planes12_skip(const float* __restrict__
   for (int p = 0; p < 12; p++) b[p * P + i] = v[(p + 1) % 12];
 }
 
+// only the first and last 32 cells of each 256-cell row, one cell per
+// thread (threads packed onto those cells): whole 128-byte lines in
+// 256-byte islands 1 KiB apart, the pattern of lines at both ends of a row
+__global__ __launch_bounds__(256) void k_planes12_ends(const float* __restrict__ a, float* __restrict__ b, long long P,
+                                                       long long rows) {
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long row = t >> 6;
+  if (row >= rows) return;
+  int k = (int)(t & 63);
+  long long i = row * 256 + (k < 32 ? k : 192 + k);
+  float v[12];
+#pragma unroll
+  for (int p = 0; p < 12; p++) v[p] = a[p * P + i];
+#pragma unroll
+  for (int p = 0; p < 12; p++) b[p * P + i] = v[(p + 1) % 12];
+}
+
 int main(int argc, char** argv) {
   long long mib = argc > 1 ? atoll(argv[1]) : 1024;
   int reps = argc > 2 ? atoi(argv[2]) : 20;
@@ -108,7 +125,7 @@ int main(int argc, char** argv) {
     }
   }
   // partial rows
-  {
+  if (12 * (1LL << 24) * 4 <= bytes) {
     const long long N = 1LL << 24;
     const int skips[] = {0, 8, 16, 32};
     for (int sk : skips) {
@@ -125,6 +142,22 @@ int main(int argc, char** argv) {
       printf("{\"kernel\": \"planes12_skip\", \"skip\": %d, \"avg_ms\": %.5f, \"GB_s\": %.1f}\n", sk, per,
              2.0 * 48 * cells / (per * 1e-3) / 1e9);
     }
+  }
+  // row ends only
+  if (12 * (1LL << 24) * 4 <= bytes) {
+    const long long N = 1LL << 24;
+    const long long rows = N / 256;
+    int g1 = (int)((rows * 64 + 255) / 256);
+    hipLaunchKernelGGL(k_planes12_ends, dim3(g1), dim3(256), 0, 0, (const float*)a, (float*)b, N, rows);
+    hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; r++)
+      hipLaunchKernelGGL(k_planes12_ends, dim3(g1), dim3(256), 0, 0, (const float*)a, (float*)b, N, rows);
+    hipEventRecord(e1, 0);
+    hipEventSynchronize(e1);
+    hipEventElapsedTime(&ms, e0, e1);
+    per = ms / reps;
+    printf("{\"kernel\": \"planes12_ends\", \"avg_ms\": %.5f, \"GB_s\": %.1f}\n", per,
+           2.0 * 48 * rows * 64 / (per * 1e-3) / 1e9);
   }
   // twelve planes, original layout
   {

@@ -94,9 +94,7 @@ struct Dev {
   double R[6];        // xmin,xmax,ymin,ymax,zmin,zmax
   const uint32_t* mask;  // 1 = cell not in a pure bulk line
   int line_len;          // cells per 128-byte line of a plane
-  const uint32_t* line_info;  // per line: 0 bulk, 1 one special class, 2 | (record base << 2) mixed
-  const uint32_t* line_bits;  // mixed lines: bit k = cell k is special
-  const uint32_t* mixed_rec;  // class << 30 | list index, per special cell of the mixed lines
+  const uint32_t* line_info;  // per line: 0 bulk, 1 one special class, 2 mixed
   // pml class
   int n_pml;
   const int32_t* pml_cell;       // cell | kFlagOut | kFlagFd
@@ -128,7 +126,7 @@ struct Dev {
   double* mu_w;
   long long n_filt;
   // special faces
-  const int32_t* fd;         // 6 special-face codes per descriptor (0 = ordinary)
+  const int32_t* fd;         // 6 special-face codes per descriptor (0 = ordinary; kFdHuy form below)
   const int32_t* sf_mtype;   // per special face code (1-based index): surface material type
   const int32_t* sf_exc;     // excitation row (0-based) or -1
   // excitation
@@ -147,6 +145,8 @@ struct Dev {
   int huy_function;
   double huy_offset_min;
   double Ei[3], Hi[3];
+  double* huy_val;           // per Huygens face: its value at the pending connect's timestep
+  int n_huy;
   // outputs
   int n_out_rows;
   double* ring;              // slots of n_out_rows*6
@@ -169,6 +169,38 @@ __device__ inline double huygens_value(const Dev& d, int timestep, double offset
   double v1 = (t1 >= 1 && t1 <= d.n_timesteps) ? vf[t1 - 1] : 0.0;
   double v2 = (t2 >= 1 && t2 <= d.n_timesteps) ? vf[t2 - 1] : 0.0;
   return v1 + delta_t * (v2 - v1) / dt;
+}
+
+// A face whose only excitation is one Huygens face with an axis-aligned
+// unit normal, on a free-space (or no) surface material, is coded
+//   kFdHuy | face << 3 | normal,  normal = 2*axis + (1 if negative)
+// in the descriptor table, and its value is read from huy_val (filled once
+// per step by k_huygens_values) instead of walking the face row's lists.
+constexpr int32_t kFdHuy = 1 << 30;
+constexpr int kFdHuyFaceBits = 27;
+
+// Per-step Huygens face values: huy_val[f] = huygens_value(timestep, offset_f).
+__global__ void k_huygens_values(Dev d, int timestep) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < d.n_huy) d.huy_val[f] = huygens_value(d, timestep, d.hf[f].offset);
+}
+
+// face_excitation for a kFdHuy face: the Huygens branch of face_excitation
+// below with the face's value and normal taken from the code.
+__device__ inline void face_excitation_huy(const Dev& d, int32_t code, double f[12]) {
+  for (int k = 0; k < 12; k++) f[k] = 0.0;
+  const double value = d.huy_val[(code >> 3) & ((1 << kFdHuyFaceBits) - 1)];
+  const int axis = (code & 7) >> 1;
+  const double sg = (code & 1) ? -1.0 : 1.0;
+  const double nx = axis == 0 ? sg : 0.0, ny = axis == 1 ? sg : 0.0, nz = axis == 2 ? sg : 0.0;
+  double Js0 = (ny * value * (d.Hi[2]) - nz * value * (d.Hi[1]));
+  double Js1 = (nz * value * (d.Hi[0]) - nx * value * (d.Hi[2]));
+  double Js2 = (nx * value * (d.Hi[1]) - ny * value * (d.Hi[0]));
+  double Ms0 = -(ny * value * (d.Ei[2]) - nz * value * (d.Ei[1]));
+  double Ms1 = -(nz * value * (d.Ei[0]) - nx * value * (d.Ei[2]));
+  double Ms2 = -(nx * value * (d.Ei[1]) - ny * value * (d.Ei[0]));
+  f[6] = f[6] + Js0; f[7] = f[7] + Js1; f[8] = f[8] + Js2;
+  f[9] = f[9] + Ms0; f[10] = f[10] + Ms1; f[11] = f[11] + Ms2;
 }
 
 // face_excitation_field(row, side, 1:12) at `timestep`: zero, then the point
@@ -227,7 +259,8 @@ template <bool PLAIN>
 __device__ inline void special_face(const Dev& d, int code, int axis, bool is_min,
                                     double own_a, double own_b, double nb_a, double nb_b,
                                     int ts_connect, double& inc_a, double& inc_b) {
-  int mtype = d.sf_mtype[code];
+  const bool huy = (code & kFdHuy) != 0;
+  int mtype = huy ? 0 : d.sf_mtype[code];
   double min_a = is_min ? own_a : nb_a, max_a = is_min ? nb_a : own_a;
   double min_b = is_min ? own_b : nb_b, max_b = is_min ? nb_b : own_b;
   double a_min, a_max, b_min, b_max;
@@ -240,29 +273,31 @@ __device__ inline void special_face(const Dev& d, int code, int axis, bool is_mi
     a_min = min_a + max_a; a_max = a_min;
     b_min = min_b + max_b; b_max = b_min;
   }
-  int row = d.sf_exc[code];
+  int row = huy ? 0 : d.sf_exc[code];
   if (row >= 0) {
     double f[12];
-    face_excitation(d, row, ts_connect, f);
-    const int32_t* h = d.fex_type + 6 * (long long)row;  // side 1 types for both sides
+    if (huy) face_excitation_huy(d, code, f);
+    else face_excitation(d, row, ts_connect, f);
+    const int32_t* h = d.fex_type + 6 * (long long)row;  // side 1 types for both sides (soft for kFdHuy)
+    auto H = [&](int k) { return huy ? 1.0 : (double)h[k]; };
     const double dl = d.dl, Z0 = kZ0;
     const double* Js = f + 6;
     const double* Ms = f + 9;
     if (axis == 0) {
-      a_min = a_min * (double)h[1] - f[1] * dl - f[5] * Z0 * dl + Js[1] * dl * Z0 / 2.0 + Ms[2] * dl / 2.0;
-      a_max = a_max * (double)h[1] - f[1] * dl + f[5] * Z0 * dl + Js[1] * dl * Z0 / 2.0 - Ms[2] * dl / 2.0;
-      b_min = b_min * (double)h[2] - f[2] * dl + f[4] * Z0 * dl + Js[2] * dl * Z0 / 2.0 - Ms[1] * dl / 2.0;
-      b_max = b_max * (double)h[2] - f[2] * dl - f[4] * Z0 * dl + Js[2] * dl * Z0 / 2.0 + Ms[1] * dl / 2.0;
+      a_min = a_min * H(1) - f[1] * dl - f[5] * Z0 * dl + Js[1] * dl * Z0 / 2.0 + Ms[2] * dl / 2.0;
+      a_max = a_max * H(1) - f[1] * dl + f[5] * Z0 * dl + Js[1] * dl * Z0 / 2.0 - Ms[2] * dl / 2.0;
+      b_min = b_min * H(2) - f[2] * dl + f[4] * Z0 * dl + Js[2] * dl * Z0 / 2.0 - Ms[1] * dl / 2.0;
+      b_max = b_max * H(2) - f[2] * dl - f[4] * Z0 * dl + Js[2] * dl * Z0 / 2.0 + Ms[1] * dl / 2.0;
     } else if (axis == 1) {
-      a_min = a_min * (double)h[0] - f[0] * dl + f[5] * Z0 * dl + Js[0] * dl * Z0 / 2.0 - Ms[2] * dl / 2.0;
-      a_max = a_max * (double)h[0] - f[0] * dl - f[5] * Z0 * dl + Js[0] * dl * Z0 / 2.0 + Ms[2] * dl / 2.0;
-      b_min = b_min * (double)h[2] - f[2] * dl - f[3] * Z0 * dl + Js[2] * dl * Z0 / 2.0 + Ms[0] * dl / 2.0;
-      b_max = b_max * (double)h[2] - f[2] * dl + f[3] * Z0 * dl + Js[2] * dl * Z0 / 2.0 - Ms[0] * dl / 2.0;
+      a_min = a_min * H(0) - f[0] * dl + f[5] * Z0 * dl + Js[0] * dl * Z0 / 2.0 - Ms[2] * dl / 2.0;
+      a_max = a_max * H(0) - f[0] * dl - f[5] * Z0 * dl + Js[0] * dl * Z0 / 2.0 + Ms[2] * dl / 2.0;
+      b_min = b_min * H(2) - f[2] * dl - f[3] * Z0 * dl + Js[2] * dl * Z0 / 2.0 + Ms[0] * dl / 2.0;
+      b_max = b_max * H(2) - f[2] * dl + f[3] * Z0 * dl + Js[2] * dl * Z0 / 2.0 - Ms[0] * dl / 2.0;
     } else {
-      a_min = a_min * (double)h[0] - f[0] * dl - f[4] * Z0 * dl + Js[0] * dl * Z0 / 2.0 + Ms[1] * dl / 2.0;
-      a_max = a_max * (double)h[0] - f[0] * dl + f[4] * Z0 * dl + Js[0] * dl * Z0 / 2.0 - Ms[1] * dl / 2.0;
-      b_min = b_min * (double)h[1] - f[1] * dl + f[3] * Z0 * dl + Js[1] * dl * Z0 / 2.0 - Ms[0] * dl / 2.0;
-      b_max = b_max * (double)h[1] - f[1] * dl - f[3] * Z0 * dl + Js[1] * dl * Z0 / 2.0 + Ms[0] * dl / 2.0;
+      a_min = a_min * H(0) - f[0] * dl - f[4] * Z0 * dl + Js[0] * dl * Z0 / 2.0 + Ms[1] * dl / 2.0;
+      a_max = a_max * H(0) - f[0] * dl + f[4] * Z0 * dl + Js[0] * dl * Z0 / 2.0 - Ms[1] * dl / 2.0;
+      b_min = b_min * H(1) - f[1] * dl + f[3] * Z0 * dl + Js[1] * dl * Z0 / 2.0 - Ms[0] * dl / 2.0;
+      b_max = b_max * H(1) - f[1] * dl - f[3] * Z0 * dl + Js[1] * dl * Z0 / 2.0 + Ms[0] * dl / 2.0;
     }
   }
   if (is_min) {
@@ -487,6 +522,30 @@ __device__ inline void bulk_incident(const Dev& d, const PullArgs& a, int x, int
   else { v[Vx_zmin] = free_face<PLAIN>(o[Vx_zmin], za); v[Vy_zmin] = free_face<PLAIN>(o[Vy_zmin], zb); }
   if (z == d.nz - 1) { v[Vx_zmax] = o[Vx_zmax] * Rz1; v[Vy_zmax] = o[Vy_zmax] * Rz1; }
   else { v[Vx_zmax] = free_face<PLAIN>(o[Vx_zmax], zc_); v[Vy_zmax] = free_face<PLAIN>(o[Vy_zmax], zd); }
+}
+
+// Workgroup b runs on XCD b % 8.  With remap, the logical block of b is
+// chosen so that each XCD takes one contiguous range of logical blocks
+// (needs a block count divisible by 8; otherwise identity).
+__device__ inline int xcd_block(int b, int nblocks, int remap) {
+  if (!remap || (nblocks & 7)) return b;
+  return (b & 7) * (nblocks >> 3) + (b >> 3);
+}
+
+// Row-segment block of the line kernel (x-segment fastest, then y, z).
+// mode 2 (y-bands): XCD k takes rows with y in the k-th eighth of ny, in
+// z-major order, so a row's y-1 neighbour ran just before on the same XCD
+// and its z-1 neighbour ny/8 rows before (both still in that XCD's L2).
+__device__ inline int line_block(const Dev& d, int b, int nblocks, int nbx, int mode) {
+  if (mode == 2 && (d.ny & 7) == 0) {
+    const int per_y = d.ny >> 3;
+    const int j = b >> 3;
+    const int xb = j % nbx;
+    const int t = j / nbx;
+    const int yl = t % per_y, z = t / per_y;
+    return xb + nbx * ((b & 7) * per_y + yl + d.ny * z);
+  }
+  return xcd_block(b, nblocks, mode);
 }
 
 // Bulk free-space cell update, one cell per lane: pull the six faces
@@ -1073,10 +1132,11 @@ __global__ __launch_bounds__(256) void k_material(Dev d, const T* __restrict__ L
 // kernels.
 template <typename T, bool PLAIN, bool PULL>
 __global__ __launch_bounds__(256) void k_bulk_lines(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                   PullArgs a) {
+                                                   PullArgs a, int xcd_remap) {
   const int nbx = (d.nx + 255) >> 8;
-  const int row = blockIdx.x / nbx;  // y + ny*z
-  const int x = (blockIdx.x - row * nbx) * 256 + threadIdx.x;
+  const int b = line_block(d, blockIdx.x, gridDim.x, nbx, xcd_remap);
+  const int row = b / nbx;  // y + ny*z
+  const int x = (b - row * nbx) * 256 + threadIdx.x;
   if (x >= d.nx) return;
   const long long c = x + (long long)d.nx * row;
   if (d.line_info[c / d.line_len] != 0u) return;
@@ -1085,30 +1145,25 @@ __global__ __launch_bounds__(256) void k_bulk_lines(Dev d, const T* __restrict__
   bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z);
 }
 
-// Mixed lines: each lane updates one cell of a line with its own class
-// code, so every line is written whole by one wavefront.
+// Mixed lines: a workgroup owns 256/L whole lines; its 256 slots hold their
+// cells sorted by class, so each wavefront mostly runs one class and every
+// line is completed by one workgroup (one L2) within the launch.
 template <typename T, bool PLAIN, bool PULL>
 __global__ __launch_bounds__(256) void k_mixed_lines(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                    PullArgs a, const int32_t* __restrict__ lines, int n_lines,
-                                                    int timestep, int slot) {
-  const int per = 256 / d.line_len;
-  const int li = blockIdx.x * per + threadIdx.x / d.line_len;
-  if (li >= n_lines) return;
-  const int k = threadIdx.x % d.line_len;
-  const long long ln = lines[li];
-  const long long c = ln * d.line_len + k;
-  if (c >= d.N) return;
-  const uint32_t bits = d.line_bits[ln];
-  if (!((bits >> k) & 1u)) {
+                                                    PullArgs a, const int32_t* __restrict__ slot_cell,
+                                                    const int32_t* __restrict__ slot_idx, int timestep, int slot) {
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int32_t w = slot_cell[s];
+  if (w < 0) return;
+  const long long c = w & kCellMask;
+  const int cls = (w >> 28) & 3;
+  if (cls == 0) {
     int x, y, z;
     cell_coords(d, c, x, y, z);
     bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z);
     return;
   }
-  const uint32_t base = d.line_info[ln] >> 2;
-  const uint32_t rec = d.mixed_rec[base + __popc(bits & ((1u << k) - 1u))];
-  const int cls = (int)(rec >> 30);
-  const int i = (int)(rec & 0x3fffffffu);
+  const int i = slot_idx[s];
   if (cls == 1) pml_update<T, PLAIN, PULL>(d, Lin, Lout, a, slot, i);
   else if (cls == 2) material_update<T, PLAIN, PULL>(d, Lin, Lout, a, i);
   else generic_update<T, PLAIN, PULL>(d, Lin, Lout, a, timestep, slot, i);

@@ -72,9 +72,11 @@ struct tlm_handle {
   int cur = 0;
   Dev d{};
   int n_gen = 0, n_pml = 0, n_mat = 0;
-  int n_pure_pml = 0, n_pure_mat = 0, n_pure_gen = 0, n_mixed_lines = 0;
+  int n_pure_pml = 0, n_pure_mat = 0, n_pure_gen = 0, n_mixed_blocks = 0;
+  int n_huy_coded = 0;  // descriptor entries in the kFdHuy form
   long long n_mixed_cells = 0;
-  const int32_t* d_mixed_lines = nullptr;
+  const int32_t* d_slot_cell = nullptr;  // mixed-line workgroup slots
+  const int32_t* d_slot_idx = nullptr;
   const int32_t* d_gen_cell = nullptr;
   const int32_t* d_gen_fd = nullptr;
   const int32_t* d_pml_cell = nullptr;
@@ -330,7 +332,7 @@ int finalize(tlm_handle* h) {
   // its cells with that cell's class code.  No line is ever written in part.
   const int L = (int)(128 / link_size(h));
   const long long n_lines = (N + L - 1) / L;
-  std::vector<uint32_t> line_info(n_lines, 0), line_bits(n_lines, 0);
+  std::vector<uint32_t> line_info(n_lines, 0);
   std::vector<uint8_t> line_pure(n_lines, 0);
   std::vector<uint32_t> mask((N + 31) / 32, 0u);
   long long n_mixed_cells = 0;
@@ -338,16 +340,12 @@ int finalize(tlm_handle* h) {
     long long c0 = ln * L, c1 = std::min(N, c0 + L);
     uint8_t k0 = cls[c0];
     bool same = true;
-    uint32_t bits = 0;
-    for (long long c = c0; c < c1; c++) {
+    for (long long c = c0; c < c1; c++)
       if (cls[c] != k0) same = false;
-      if (cls[c]) bits |= 1u << (c - c0);
-    }
     if (same && k0 == 0) { line_info[ln] = 0; continue; }
     for (long long c = c0; c < c1; c++) mask[c >> 5] |= 1u << (c & 31);
     if (same) { line_info[ln] = 1; line_pure[ln] = 1; continue; }
     line_info[ln] = 2;
-    line_bits[ln] = bits;
   }
   auto pure_of = [&](int32_t c) { return line_pure[c / L] != 0; };
   std::stable_partition(pr.begin(), pr.end(), [&](const PmlRec& r) { return pure_of(r.c); });
@@ -357,30 +355,65 @@ int finalize(tlm_handle* h) {
   while (n_pure_pml < (int)pr.size() && pure_of(pr[n_pure_pml].c)) n_pure_pml++;
   while (n_pure_mat < (int)mr.size() && pure_of(mr[n_pure_mat].c)) n_pure_mat++;
   while (n_pure_gen < (int)gr.size() && pure_of(gr[n_pure_gen].c)) n_pure_gen++;
-  // mixed lines and their per-cell records (class << 30 | list index), in
-  // scan order: the tails of the three lists are exactly those cells
-  std::vector<int32_t> mixed_lines;
-  std::vector<uint32_t> mixed_rec;
+  // Mixed lines are grouped 256/L to a workgroup (256 cells).  Within a
+  // group the cells are sorted by class (PML, generic, material, bulk; scan
+  // order inside a class), so a wavefront mostly runs one class while the
+  // workgroup still writes all of its lines.  Slot word: cell | class << 28,
+  // or -1 for an empty slot; slot index: the cell's class-list position.
+  std::vector<int32_t> slot_cell, slot_idx;
   {
     int ip = n_pure_pml, im = n_pure_mat, ig = n_pure_gen;
+    const int G = 256 / L;
+    // groups of G consecutive mixed lines in scan order (an XCD-banded
+    // order was measured slower on the 256^3 workload)
+    std::vector<std::vector<long long>> groups;
+    {
+      std::vector<long long> gl;
+      for (long long ln = 0; ln < n_lines; ln++) {
+        if (line_info[ln] != 2) continue;
+        gl.push_back(ln);
+        if ((int)gl.size() == G) { groups.push_back(gl); gl.clear(); }
+      }
+      if (!gl.empty()) groups.push_back(gl);
+    }
+    // class-list positions of the mixed-line cells follow scan order (the
+    // tails of the three lists); pos_of_line = rank of a mixed line
+    std::vector<int32_t> pos_of_line(n_lines, -1);
+    std::vector<int32_t> cell_idx;  // per mixed-line cell, scan order
+    int32_t n_mixed_seen = 0;
     for (long long ln = 0; ln < n_lines; ln++) {
       if (line_info[ln] != 2) continue;
-      mixed_lines.push_back((int32_t)ln);
-      line_info[ln] = 2u | ((uint32_t)mixed_rec.size() << 2);
       long long c0 = ln * L, c1 = std::min(N, c0 + L);
       for (long long c = c0; c < c1; c++) {
-        n_mixed_cells++;
-        switch (cls[c]) {
-          case 1: mixed_rec.push_back((1u << 30) | (uint32_t)ip++); break;
-          case 2: mixed_rec.push_back((2u << 30) | (uint32_t)im++); break;
-          case 3: mixed_rec.push_back((3u << 30) | (uint32_t)ig++); break;
-          default: break;
+        int k2 = cls[c], idx = 0;
+        if (k2 == 1) idx = ip++;
+        else if (k2 == 2) idx = im++;
+        else if (k2 == 3) idx = ig++;
+        cell_idx.push_back(idx);
+      }
+      pos_of_line[ln] = n_mixed_seen++;
+    }
+    static const int rank_of[4] = {3, 0, 2, 1};  // bulk, PML, material, generic
+    for (auto& gl : groups) {
+      std::vector<std::pair<int, std::pair<int32_t, int32_t>>> cells;  // (rank, (word, idx))
+      for (long long ln : gl) {
+        long long c0 = ln * L, c1 = std::min(N, c0 + L);
+        const size_t base = (size_t)pos_of_line[ln] * L;  // lines before it are whole (only the last may be short)
+        for (long long c = c0; c < c1; c++) {
+          n_mixed_cells++;
+          const int k2 = cls[c];
+          cells.push_back({rank_of[k2], {(int32_t)(c | ((long long)k2 << 28)), cell_idx[base + (c - c0)]}});
         }
+      }
+      std::stable_sort(cells.begin(), cells.end(),
+                       [](const auto& a, const auto& b) { return a.first < b.first; });
+      for (int s = 0; s < 256; s++) {
+        slot_cell.push_back(s < (int)cells.size() ? cells[s].second.first : -1);
+        slot_idx.push_back(s < (int)cells.size() ? cells[s].second.second : 0);
       }
     }
     if (ip != (int)pr.size() || im != (int)mr.size() || ig != (int)gr.size())
-      return fail(h, TLM_ERR_ARG, "internal: mixed-line records inconsistent");
-    if (mixed_rec.size() >= (1u << 28)) return fail(h, TLM_ERR_UNSUPPORTED, "too many mixed-line cells");
+      return fail(h, TLM_ERR_ARG, "internal: mixed-line cells inconsistent");
   }
   // flat per-class arrays in list order
   std::vector<int32_t> pml_cell, pml_fd;
@@ -445,6 +478,25 @@ int finalize(tlm_handle* h) {
       fhy_face.insert(fhy_face.end(), hl[r].begin(), hl[r].end());
     }
   }
+  // Faces excited by exactly one Huygens face with an axis-aligned unit
+  // normal, no point source and a free-space (or no) surface material take
+  // the flattened descriptor code (value read from the per-step table).
+  int n_huy_coded = 0;
+  for (auto& code : fd) {
+    if (code <= 0) continue;
+    const int mt = sf_mtype[code], row = sf_exc[code];
+    if (mt != 0 && mt != TLM_SMAT_FREE_SPACE) continue;
+    if (row < 0 || fex_start[row + 1] != fex_start[row] || fhy_start[row + 1] - fhy_start[row] != 1) continue;
+    const int f = fhy_face[fhy_start[row]];
+    const tlm_huygens_face& hf = h->huy[f];
+    const int nz_count = (hf.nx != 0) + (hf.ny != 0) + (hf.nz != 0);
+    const int comp = hf.nx ? hf.nx : (hf.ny ? hf.ny : hf.nz);
+    if (nz_count != 1 || (comp != 1 && comp != -1) || f >= (1 << scn::kFdHuyFaceBits)) continue;
+    const int axis = hf.nx ? 0 : (hf.ny ? 1 : 2);
+    code = scn::kFdHuy | (f << 3) | (2 * axis + (comp < 0 ? 1 : 0));
+    n_huy_coded++;
+  }
+  h->n_huy_coded = n_huy_coded;
 
   // ---- device allocations -------------------------------------------------
   Dev& d = h->d;
@@ -475,10 +527,9 @@ int finalize(tlm_handle* h) {
   }
   d.line_len = L;
   if ((rc = dev_upload(h, line_info, &d.line_info))) return rc;
-  if ((rc = dev_upload(h, line_bits, &d.line_bits))) return rc;
-  if ((rc = dev_upload(h, mixed_rec, &d.mixed_rec))) return rc;
-  if ((rc = dev_upload(h, mixed_lines, &h->d_mixed_lines))) return rc;
-  h->n_mixed_lines = (int)mixed_lines.size();
+  if ((rc = dev_upload(h, slot_cell, &h->d_slot_cell))) return rc;
+  if ((rc = dev_upload(h, slot_idx, &h->d_slot_idx))) return rc;
+  h->n_mixed_blocks = (int)(slot_cell.size() / 256);
   if ((rc = dev_upload(h, h->pml_par, &d.pml_params))) return rc;
   d.n_pml_state = std::max(d.n_pml, 1);
   {
@@ -532,6 +583,12 @@ int finalize(tlm_handle* h) {
   if ((rc = dev_upload(h, fhy_face, &d.fhy_face))) return rc;
   if ((rc = dev_upload(h, h->ps, &d.ps))) return rc;
   if ((rc = dev_upload(h, h->huy, &d.hf))) return rc;
+  d.n_huy = (int)h->huy.size();
+  if (n_huy_coded) {
+    void* p;
+    if ((rc = dev_alloc(h, h->huy.size() * sizeof(double), &p))) return rc;
+    d.huy_val = (double*)p;
+  }
   d.huy_function = h->huy_function;
   d.huy_offset_min = h->huy_offset_min;
   for (int k = 0; k < 3; k++) { d.Ei[k] = h->Ei[k]; d.Hi[k] = h->Hi[k]; }
@@ -618,17 +675,28 @@ scn::BulkMap bulk_map(const tlm_handle* h) {
   return m;
 }
 
+// Per-step Huygens face values for the kFdHuy descriptor codes, at the
+// pending connect's timestep (before any kernel that pulls across faces).
+void launch_huygens_values(tlm_handle* h, const PullArgs& a) {
+  if (!h->n_huy_coded) return;
+  hipLaunchKernelGGL(scn::k_huygens_values, dim3((h->d.n_huy + 255) / 256), dim3(256), 0, h->stream, h->d,
+                     a.ts_connect);
+}
+
 template <typename T, bool PLAIN>
 int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, int slot) {
   const Dev& d = h->d;
+  if (pull) launch_huygens_values(h, a);
   const T* Lin = (const T*)h->planes[h->cur];
   T* Lout = (T*)h->planes[h->cur ^ 1];
   {
     ProfScope ps(h, TLM_KCLASS_BULK);
     if (sched_variant(h) == 3) {
       dim3 g((unsigned)(((d.nx + 255) / 256) * (long long)d.ny * d.nz));
-      if (pull) hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a);
-      else hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a);
+      if (pull)
+        hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, h->sched_xcd);
+      else
+        hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, h->sched_xcd);
     } else {
       scn::BulkMap m = bulk_map(h);
       dim3 bb(64, m.by, 1);
@@ -646,16 +714,15 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
       }
     }
   }
-  if (h->n_mixed_lines) {
+  if (h->n_mixed_blocks) {
     ProfScope ps(h, TLM_KCLASS_MIXED);
-    const int per = 256 / d.line_len;
-    dim3 g((h->n_mixed_lines + per - 1) / per);
+    dim3 g(h->n_mixed_blocks);
     if (pull)
       hipLaunchKernelGGL((scn::k_mixed_lines<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a,
-                         h->d_mixed_lines, h->n_mixed_lines, timestep, slot);
+                         h->d_slot_cell, h->d_slot_idx, timestep, slot);
     else
       hipLaunchKernelGGL((scn::k_mixed_lines<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a,
-                         h->d_mixed_lines, h->n_mixed_lines, timestep, slot);
+                         h->d_slot_cell, h->d_slot_idx, timestep, slot);
   }
   if (h->n_pure_pml) {
     ProfScope ps(h, TLM_KCLASS_PML);
@@ -685,6 +752,7 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
 template <typename T, bool PLAIN>
 int launch_connect(tlm_handle* h, const PullArgs& a) {
   const Dev& d = h->d;
+  launch_huygens_values(h, a);
   const T* Lin = (const T*)h->planes[h->cur];
   T* Lout = (T*)h->planes[h->cur ^ 1];
   dim3 bb(64, 4, 1);
@@ -1093,7 +1161,8 @@ int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, i
   h->sched_variant = variant;
   h->sched_by = rows;
   h->sched_zc = zc;
-  h->sched_xcd = xcd_remap ? 1 : 0;
+  if (xcd_remap < 0 || xcd_remap > 2) return fail(h, TLM_ERR_ARG, "bad XCD mapping");
+  h->sched_xcd = xcd_remap;
   return TLM_OK;
 }
 
@@ -1160,13 +1229,12 @@ int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
   const int64_t n_bulk = h->N - h->n_mixed_cells - h->n_pure_pml - h->n_pure_mat - h->n_pure_gen;
   cells[TLM_KCLASS_BULK] = n_bulk;
   bytes[TLM_KCLASS_BULK] = n_bulk * 24 * sl + (sched_variant(h) == 3 ? 4 * n_lines : (h->N + 7) / 8);
-  // mixed lines: every cell with its class's bytes, plus the line word, the
-  // line bit mask, the line number and one record per special cell
+  // mixed lines: every cell with its class's bytes, plus two slot words
   cells[TLM_KCLASS_MIXED] = h->n_mixed_cells;
   bytes[TLM_KCLASS_MIXED] = (h->n_mixed_cells - n_special_mixed) * 24 * sl +
                             (int64_t)(h->n_pml - h->n_pure_pml) * pml_cell_bytes +
                             mat_bytes(h->n_pure_mat, h->n_mat) + gen_bytes(h->n_pure_gen, h->n_gen) +
-                            (int64_t)h->n_mixed_lines * 12 + n_special_mixed * 4;
+                            (int64_t)h->n_mixed_blocks * 256 * 8;
   cells[TLM_KCLASS_PML] = h->n_pure_pml;
   bytes[TLM_KCLASS_PML] = (int64_t)h->n_pure_pml * pml_cell_bytes;
   cells[TLM_KCLASS_GENERIC] = h->n_pure_gen;

@@ -236,7 +236,9 @@ int tlm_read_dft(tlm_handle* h, double* re_im);
  * tiles; variant 1 = workgroups marching zc planes; variant 2 = 16-byte
  * vector accesses, several consecutive cells per lane (falls back to 0 when
  * nx is not a multiple of the vector width); rows = tile height (1, 2 or 4
- * rows); xcd_remap = give each XCD a contiguous tile range.
+ * rows); xcd_remap = 1: give each XCD a contiguous tile range; 2 (line
+ * kernel only, ny divisible by 8): XCD k takes the rows of the k-th y-band
+ * in z-major order, so the y-1 and z-1 neighbour rows are in its L2.
  * Any schedule gives identical results; it only changes speed.  The
  * autotuner (ggi_tlm_amd/autotune.py) picks one per grid shape. */
 int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc,
