@@ -63,11 +63,13 @@ SIGNATURES = {
     "read_dft": (C.c_int, [_h, _p]),
     "set_schedule": (C.c_int, [_h, _i32, _i32, _i32, _i32]),
     "set_profiling": (C.c_int, [_h, _i32]),
+    "set_small_to_zero": (C.c_int, [_h, _i32, C.c_double]),
+    "read_face_outputs": (C.c_int, [_h, _p, _i32, _p, _p]),
     "read_kernel_times": (C.c_int, [_h, _p, _p]),
     "kernel_bytes": (C.c_int, [_h, _p, _p]),
 }
 
-KERNEL_CLASSES = ("bulk", "pml", "generic", "dft", "material", "mixed")
+KERNEL_CLASSES = ("bulk", "pml", "generic", "dft", "material", "mixed", "pml_material")
 
 EXPORTED = tuple("tlm_" + n for n in SIGNATURES)
 
@@ -232,14 +234,29 @@ class Handle:
         self._check(self.lib.fn["read_cell_outputs"](self.h, _ptr(out), int(cap), C.byref(n), C.byref(first)))
         return out[: n.value, :rows], first.value
 
+    def read_face_outputs(self, max_steps=None):
+        """Recorded face_output_field rows: array (connects, n_output_faces,
+        2 sides, 6) and the timestep of the first connect."""
+        rows = self.t.n_output_faces
+        cap = max_steps if max_steps is not None else self.t.n_timesteps
+        out = np.zeros((max(cap, 1), max(rows, 1), 2, 6), dtype=np.float64)
+        n = C.c_int32(0)
+        first = C.c_int32(0)
+        self._check(self.lib.fn["read_face_outputs"](self.h, _ptr(out), int(cap), C.byref(n), C.byref(first)))
+        return out[: n.value, :rows], first.value
+
     def read_dft(self):
         npnt, nfreq = len(self.t.output_rows), len(self.t.dft_frequencies)
         out = np.zeros((max(npnt, 1), max(nfreq, 1), 2), dtype=np.float64)
         self._check(self.lib.fn["read_dft"](self.h, _ptr(out)))
         return out[:npnt, :nfreq, 0] + 1j * out[:npnt, :nfreq, 1]
 
-    def set_schedule(self, variant: int, rows: int, zc: int, xcd_remap: bool = True):
-        self._check(self.lib.fn["set_schedule"](self.h, int(variant), int(rows), int(zc), 1 if xcd_remap else 0))
+    def set_schedule(self, variant: int, rows: int, zc: int, xcd_remap: int = 0):
+        self._check(self.lib.fn["set_schedule"](self.h, int(variant), int(rows), int(zc), int(xcd_remap)))
+
+    def set_small_to_zero(self, n_small_to_zero: int, small_to_zero_value: float):
+        """TLM_set_small_to_zero every n steps (0: off)."""
+        self._check(self.lib.fn["set_small_to_zero"](self.h, int(n_small_to_zero), float(small_to_zero_value)))
 
     def set_profiling(self, on: bool):
         self._check(self.lib.fn["set_profiling"](self.h, 1 if on else 0))

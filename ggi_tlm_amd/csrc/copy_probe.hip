@@ -65,6 +65,36 @@ __global__ __launch_bounds__(256) void k_planes12_ends(const float* __restrict__
   for (int p = 0; p < 12; p++) b[p * P + i] = v[(p + 1) % 12];
 }
 
+// complement of k_planes12_skip: only the first and last `skip` cells of
+// each 256-cell row, over rows [r0, r1)
+__global__ __launch_bounds__(256) void k_planes12_edges(const float* __restrict__ a, float* __restrict__ b, long long P,
+                                                        long long r0, long long r1, int skip) {
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long row = r0 + t / (2 * skip);
+  if (row >= r1) return;
+  int k = (int)(t % (2 * skip));
+  long long i = row * 256 + (k < skip ? k : 256 - 2 * skip + k);
+  float v[12];
+#pragma unroll
+  for (int p = 0; p < 12; p++) v[p] = a[p * P + i];
+#pragma unroll
+  for (int p = 0; p < 12; p++) b[p * P + i] = v[(p + 1) % 12];
+}
+
+// k_planes12_skip over rows [r0, r1)
+__global__ __launch_bounds__(256) void k_planes12_skip_rows(const float* __restrict__ a, float* __restrict__ b,
+                                                            long long P, long long r0, long long r1, int skip) {
+  long long i = r0 * 256 + blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= r1 * 256) return;
+  int x = (int)(i & 255);
+  if (x < skip || x >= 256 - skip) return;
+  float v[12];
+#pragma unroll
+  for (int p = 0; p < 12; p++) v[p] = a[p * P + i];
+#pragma unroll
+  for (int p = 0; p < 12; p++) b[p * P + i] = v[(p + 1) % 12];
+}
+
 int main(int argc, char** argv) {
   long long mib = argc > 1 ? atoll(argv[1]) : 1024;
   int reps = argc > 2 ? atoi(argv[2]) : 20;
@@ -158,6 +188,37 @@ int main(int argc, char** argv) {
     per = ms / reps;
     printf("{\"kernel\": \"planes12_ends\", \"avg_ms\": %.5f, \"GB_s\": %.1f}\n", per,
            2.0 * 48 * rows * 64 / (per * 1e-3) / 1e9);
+  }
+  // complementary partial writes from two kernels (interior, then row
+  // edges), over the whole array or chunk by chunk: do the partial lines of
+  // the first kernel merge with the second's while still cached?
+  if (12 * (1LL << 24) * 4 <= bytes) {
+    const long long N = 1LL << 24;
+    const long long rows = N / 256;
+    const int skip = 8;
+    const long long chunks[] = {rows, rows / 4, rows / 16, rows / 64, rows / 256};
+    for (long long ch : chunks) {
+      auto pass = [&]() {
+        for (long long r0 = 0; r0 < rows; r0 += ch) {
+          long long r1 = r0 + ch < rows ? r0 + ch : rows;
+          int g1 = (int)(((r1 - r0) * 256 + 255) / 256);
+          hipLaunchKernelGGL(k_planes12_skip_rows, dim3(g1), dim3(256), 0, 0, (const float*)a, (float*)b, N, r0, r1,
+                             skip);
+          int g2 = (int)(((r1 - r0) * 2 * skip + 255) / 256);
+          hipLaunchKernelGGL(k_planes12_edges, dim3(g2), dim3(256), 0, 0, (const float*)a, (float*)b, N, r0, r1,
+                             skip);
+        }
+      };
+      pass();
+      hipEventRecord(e0, 0);
+      for (int r = 0; r < reps; r++) pass();
+      hipEventRecord(e1, 0);
+      hipEventSynchronize(e1);
+      hipEventElapsedTime(&ms, e0, e1);
+      per = ms / reps;
+      printf("{\"kernel\": \"planes12_split\", \"chunk_rows\": %lld, \"chunk_MB\": %.1f, \"avg_ms\": %.5f, "
+             "\"GB_s\": %.1f}\n", ch, 96.0 * ch * 256 / 1e6, per, 2.0 * 48 * N / (per * 1e-3) / 1e9);
+    }
   }
   // twelve planes, original layout
   {

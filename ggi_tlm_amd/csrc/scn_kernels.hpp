@@ -67,6 +67,23 @@ enum {
 };
 // Ixt,Iyt,Izt live in a separate array, touched only by output cells.
 
+// PML cells holding a non-dispersive material (scatter_PML_material.F90):
+// the PML_data_type words they use plus PML_material_data_type
+// (PML_module.F90:131-156), FP64 SoA in both forms.  PML_cell_data%Ix..Iz,
+// stored by the reference from stale scatter locals and never read, are
+// not kept.
+enum {
+  PM_Ixt, PM_Iyt, PM_Izt, PM_Vi,  // Vi: 12 words
+  PM_Vxt = PM_Vi + 12, PM_Vyt, PM_Vzt, PM_Vyxt, PM_Vzxt, PM_Vxyt, PM_Vzyt, PM_Vyzt, PM_Vxzt,
+  PM_Ixt_m1, PM_Iyt_m1, PM_Izt_m1, PM_Vcx_m1, PM_Vcy_m1, PM_Vcz_m1, PM_Vcx, PM_Vcy, PM_Vcz,
+  PM_Vi_m1,  // 12 words
+  PM_Vxt_m1 = PM_Vi_m1 + 12, PM_Vyt_m1, PM_Vzt_m1,
+  PM_Vyxt_m1, PM_Vzxt_m1, PM_Vxyt_m1, PM_Vzyt_m1, PM_Vyzt_m1, PM_Vxzt_m1,
+  PM_Vosxr, PM_Vosyr, PM_Voszr, PM_Vosx, PM_Vosy, PM_Vosz, PM_Vosx_m1, PM_Vosy_m1, PM_Vosz_m1,
+  PM_Vssxr, PM_Vssyr, PM_Vsszr, PM_Vsx, PM_Vsy, PM_Vsz, PM_Vsx_m1, PM_Vsy_m1, PM_Vsz_m1,
+  PM_N
+};
+
 // Packed per-cell word of the PML and material class lists.
 constexpr int kCellBits = 28;
 constexpr int32_t kCellMask = (1 << kCellBits) - 1;
@@ -106,6 +123,14 @@ struct Dev {
   const tlm_pml_parameter* pml_params;
   double* pml_state;             // PS_N (plain) or PC_N (precision) words * n_pml_state
   double* pml_It;                // 3 * n_pml_state (Ixt,Iyt,Izt)
+  // PML cells holding a material (scan order)
+  int n_pm;
+  const int32_t* pm_cell;        // cell index
+  const int32_t* pm_par;         // PML parameter row (0-based)
+  const int32_t* pm_mat;         // volume material number (1-based)
+  const int32_t* pm_out;         // output row (0-based) or -1
+  const int32_t* pm_fd;          // face descriptor or -1
+  double* pm_state;              // PM_N words * n_pm
   long long n_pml_state;
   // material class
   int n_mat;
@@ -129,6 +154,9 @@ struct Dev {
   const int32_t* fd;         // 6 special-face codes per descriptor (0 = ordinary; kFdHuy form below)
   const int32_t* sf_mtype;   // per special face code (1-based index): surface material type
   const int32_t* sf_exc;     // excitation row (0-based) or -1
+  const int32_t* sf_out;     // face output row (0-based) or -1
+  double* face_ring;         // slots of n_face_out_rows * 2 sides * 6 fields
+  int n_face_out_rows;
   // excitation
   const double* value;       // (n_timesteps, n_functions) column-major
   const double* value_face;
@@ -258,7 +286,7 @@ __device__ inline A free_face(A own, A nb) {
 template <bool PLAIN>
 __device__ inline void special_face(const Dev& d, int code, int axis, bool is_min,
                                     double own_a, double own_b, double nb_a, double nb_b,
-                                    int ts_connect, double& inc_a, double& inc_b) {
+                                    int ts_connect, int fslot, double& inc_a, double& inc_b) {
   const bool huy = (code & kFdHuy) != 0;
   int mtype = huy ? 0 : d.sf_mtype[code];
   double min_a = is_min ? own_a : nb_a, max_a = is_min ? nb_a : own_a;
@@ -307,13 +335,60 @@ __device__ inline void special_face(const Dev& d, int code, int axis, bool is_mi
     inc_a = a_max - own_a;
     inc_b = b_max - own_b;
   }
+  // Face output (connect_normal_to_x.F90:323-361, likewise y, z): this
+  // side's row; side 1 (the min cell) pairs the post-connect pulse as
+  // "positive" with its pre-connect one, side 2 the other way round.
+  if (fslot >= 0 && !huy) {
+    const int r = d.sf_out[code];
+    if (r >= 0) {
+      const double ap = is_min ? inc_a : own_a, an = is_min ? own_a : inc_a;
+      const double bp = is_min ? inc_b : own_b, bn = is_min ? own_b : inc_b;
+      const double dl = d.dl, Z0 = kZ0;
+      double* f = d.face_ring + (((long long)fslot * d.n_face_out_rows + r) * 2 + (is_min ? 0 : 1)) * 6;
+      if (axis == 0) {
+        f[0] = 0.0;
+        f[1] = -(ap + an) / dl;
+        f[2] = -(bp + bn) / dl;
+        f[3] = 0.0;
+        f[4] = (bp - bn) / (Z0 * dl);
+        f[5] = -(ap - an) / (Z0 * dl);
+      } else if (axis == 1) {
+        f[0] = -(ap + an) / dl;
+        f[1] = 0.0;
+        f[2] = -(bp + bn) / dl;
+        f[3] = -(bp - bn) / (Z0 * dl);
+        f[4] = 0.0;
+        f[5] = (ap - an) / (Z0 * dl);
+      } else {
+        f[0] = -(ap + an) / dl;
+        f[1] = -(bp + bn) / dl;
+        f[2] = 0.0;
+        f[3] = (bp - bn) / (Z0 * dl);
+        f[4] = -(ap - an) / (Z0 * dl);
+        f[5] = 0.0;
+      }
+    }
+  }
   (void)PLAIN;
 }
 
 struct PullArgs {
   bool apply_r;    // outer_boundary pending
   int ts_connect;  // timestep of the pending connect (face sources)
+  double small;    // > 0: incident voltages below it are set to zero first
+  int fslot;       // face output ring slot of the pending connect (-1: none)
 };
+
+// TLM_set_small_to_zero (TLM_SOLVER/TLM_set_small_to_zero.F90:34-68):
+// abs(V) < small_to_zero_value -> 0, on the incident voltages of the step
+// it is due (run_TLM.F90:236-238).
+__device__ inline void flush_small(double v[12], double small) {
+  if (small > 0.0) {
+#pragma unroll
+    for (int p = 0; p < 12; p++)
+      if (fabs(v[p]) < small) v[p] = 0.0;
+  }
+}
 
 // Incident voltages of cell c = (x,y,z) in v[12].  fdi: face descriptor
 // (-1 = six ordinary faces).
@@ -353,7 +428,7 @@ __device__ inline void pull_cell(const Dev& d, const T* __restrict__ L, long lon
       v[pb[s]] = free_face<PLAIN>(ob, nb);
     } else {
       double ia, ib;
-      special_face<PLAIN>(d, fc[s], s >> 1, (s & 1) == 0, oa, ob, na, nb, a.ts_connect, ia, ib);
+      special_face<PLAIN>(d, fc[s], s >> 1, (s & 1) == 0, oa, ob, na, nb, a.ts_connect, a.fslot, ia, ib);
       v[pa[s]] = ia;
       v[pb[s]] = ib;
     }
@@ -572,6 +647,7 @@ __device__ inline void bulk_update(const Dev& d, const T* __restrict__ Lin, T* _
 #pragma unroll
     for (int p = 0; p < 12; p++) v[p] = (double)Lin[p * P + c];
   }
+  flush_small(v, a.small);
   if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
   store_cell(Lout, P, c, v);
 }
@@ -798,6 +874,7 @@ __device__ inline void pml_update(const Dev& d, const T* __restrict__ Lin, T* __
 #pragma unroll
     for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
   }
+  flush_small(v, a.small);
   const double Z0 = kZ0, dl = d.dl;
   const tlm_pml_parameter& Pp = d.pml_params[d.pml_par[i]];
   const double ax = Pp.ax, ay = Pp.ay, az = Pp.az;
@@ -958,6 +1035,233 @@ __global__ __launch_bounds__(256) void k_pml(Dev d, const T* __restrict__ Lin, T
   if (i < n) pml_update<T, PLAIN, PULL>(d, Lin, Lout, a, slot, i);
 }
 
+// PML cell holding a non-dispersive material, scatter_PML_material.F90:22-534
+// (two-step recursions for the stretched node voltages and currents, a
+// capacitive and an inductive stub).  Same operation order as the
+// reference; the STAGE 1 coefficients are recomputed per cell as there.
+template <typename T, bool PLAIN, bool PULL>
+__device__ inline void pml_material_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                           const PullArgs& a, int slot, int i) {
+  const long long c = d.pm_cell[i];
+  int x, y, z;
+  {
+    const int nxy = d.nx * d.ny;
+    z = (int)(c / nxy);
+    const int rem = (int)(c - (long long)z * nxy);
+    y = rem / d.nx;
+    x = rem - y * d.nx;
+  }
+  double v[12];
+  if (PULL) {
+    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, d.pm_fd[i], a, v);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
+  }
+  flush_small(v, a.small);
+  enum { Vxny = 0, Vxnz, Vxpy, Vxpz, Vynx, Vynz, Vypx, Vypz, Vznx, Vzny, Vzpx, Vzpy };
+  const double Z0 = kZ0, dl = d.dl, dt = d.dt;
+  const tlm_volume_material& vm = d.vmat[d.pm_mat[i] - 1];
+  const tlm_pml_parameter& Pp = d.pml_params[d.pm_par[i]];
+  // STAGE 1
+  const double Ysx = vm.Ys_eps_f / kY0, Ysy = vm.Ys_eps_f / kY0, Ysz = vm.Ys_eps_f / kY0;
+  const double Gex = vm.Ge / kY0, Gey = vm.Ge / kY0, Gez = vm.Ge / kY0;
+  const double Zsx = vm.Zs_mu_f, Zsy = vm.Zs_mu_f, Zsz = vm.Zs_mu_f;
+  const double Rmx = vm.Rm, Rmy = vm.Rm, Rmz = vm.Rm;
+  const double sx = Pp.sx, sy = Pp.sy, sz = Pp.sz;
+  const double exp_x = Pp.exp_x, exp_y = Pp.exp_y, exp_z = Pp.exp_z;
+  double GpY2;
+  GpY2 = (Gex + Ysx) / 2.0;
+  const double Ax0 = 8.0 + 4.0 * GpY2 + 2.0 * dt * (sy + sz) * (1.0 + GpY2) + dt * dt * sy * sz * GpY2;
+  const double Ax1 = -2.0 * (8.0 + 4.0 * GpY2) + 2.0 * (dt * dt * sy * sz * GpY2);
+  const double Ax2 = 8.0 + 4.0 * GpY2 - 2.0 * dt * (sy + sz) * (1.0 + GpY2) + dt * dt * sy * sz * GpY2;
+  GpY2 = (Gey + Ysy) / 2.0;
+  const double Ay0 = 8.0 + 4.0 * GpY2 + 2.0 * dt * (sz + sx) * (1.0 + GpY2) + dt * dt * sz * sx * GpY2;
+  const double Ay1 = -2.0 * (8.0 + 4.0 * GpY2) + 2.0 * (dt * dt * sz * sx * GpY2);
+  const double Ay2 = 8.0 + 4.0 * GpY2 - 2.0 * dt * (sz + sx) * (1.0 + GpY2) + dt * dt * sz * sx * GpY2;
+  GpY2 = (Gez + Ysz) / 2.0;
+  const double Az0 = 8.0 + 4.0 * GpY2 + 2.0 * dt * (sx + sy) * (1.0 + GpY2) + dt * dt * sx * sy * GpY2;
+  const double Az1 = -2.0 * (8.0 + 4.0 * GpY2) + 2.0 * (dt * dt * sx * sy * GpY2);
+  const double Az2 = 8.0 + 4.0 * GpY2 - 2.0 * dt * (sx + sy) * (1.0 + GpY2) + dt * dt * sx * sy * GpY2;
+  const double Bx0 = 4.0 + 2.0 * dt * sy, Bx1 = -8.0, Bx2 = 4.0 - 2.0 * dt * sy;
+  const double By0 = 4.0 + 2.0 * dt * sz, By1 = -8.0, By2 = 4.0 - 2.0 * dt * sz;
+  const double Bz0 = 4.0 + 2.0 * dt * sx, Bz1 = -8.0, Bz2 = 4.0 - 2.0 * dt * sx;
+  const double Cx0 = 4.0 + 2.0 * dt * sz, Cx1 = -8.0, Cx2 = 4.0 - 2.0 * dt * sz;
+  const double Cy0 = 4.0 + 2.0 * dt * sx, Cy1 = -8.0, Cy2 = 4.0 - 2.0 * dt * sx;
+  const double Cz0 = 4.0 + 2.0 * dt * sy, Cz1 = -8.0, Cz2 = 4.0 - 2.0 * dt * sy;
+  const double Dx0 = 4.0 * Ysx + 2.0 * dt * (sy + sz) * Ysx + 1.0 * (dt * dt * sy * sz * Ysx);
+  const double Dx1 = -8.0 * Ysx + 2.0 * (dt * dt * sy * sz * Ysx);
+  const double Dx2 = 4.0 * Ysx - 2.0 * dt * (sy + sz) * Ysx + 1.0 * (dt * dt * sy * sz * Ysx);
+  const double Dy0 = 4.0 * Ysy + 2.0 * dt * (sz + sx) * Ysy + 1.0 * (dt * dt * sz * sx * Ysy);
+  const double Dy1 = -8.0 * Ysy + 2.0 * (dt * dt * sz * sx * Ysy);
+  const double Dy2 = 4.0 * Ysy - 2.0 * dt * (sz + sx) * Ysy + 1.0 * (dt * dt * sz * sx * Ysy);
+  const double Dz0 = 4.0 * Ysz + 2.0 * dt * (sx + sy) * Ysz + 1.0 * (dt * dt * sx * sy * Ysz);
+  const double Dz1 = -8.0 * Ysz + 2.0 * (dt * dt * sx * sy * Ysz);
+  const double Dz2 = 4.0 * Ysz - 2.0 * dt * (sx + sy) * Ysz + 1.0 * (dt * dt * sx * sy * Ysz);
+  double ZpR;
+  ZpR = Zsx + Rmx;
+  const double Sx0 = 16.0 * Z0 + 4.0 * ZpR + 2.0 * dt * (sy + sz) * (2.0 * Z0 + ZpR) + dt * dt * sy * sz * ZpR;
+  const double Sx1 = -32.0 * Z0 - 8.0 * ZpR + 2.0 * dt * dt * sy * sz * ZpR;
+  const double Sx2 = 16.0 * Z0 + 4.0 * ZpR - 2.0 * dt * (sy + sz) * (2.0 * Z0 + ZpR) + dt * dt * sy * sz * ZpR;
+  ZpR = Zsy + Rmy;
+  const double Sy0 = 16.0 * Z0 + 4.0 * ZpR + 2.0 * dt * (sz + sx) * (2.0 * Z0 + ZpR) + dt * dt * sz * sx * ZpR;
+  const double Sy1 = -32.0 * Z0 - 8.0 * ZpR + 2.0 * dt * dt * sz * sx * ZpR;
+  const double Sy2 = 16.0 * Z0 + 4.0 * ZpR - 2.0 * dt * (sz + sx) * (2.0 * Z0 + ZpR) + dt * dt * sz * sx * ZpR;
+  ZpR = Zsz + Rmz;
+  const double Sz0 = 16.0 * Z0 + 4.0 * ZpR + 2.0 * dt * (sx + sy) * (2.0 * Z0 + ZpR) + dt * dt * sx * sy * ZpR;
+  const double Sz1 = -32.0 * Z0 - 8.0 * ZpR + 2.0 * dt * dt * sx * sy * ZpR;
+  const double Sz2 = 16.0 * Z0 + 4.0 * ZpR - 2.0 * dt * (sx + sy) * (2.0 * Z0 + ZpR) + dt * dt * sx * sy * ZpR;
+  const double Tx0 = 8.0 + 4.0 * dt * sx, Tx1 = -16.0, Tx2 = 8.0 - 4.0 * dt * sx;
+  const double Ty0 = 8.0 + 4.0 * dt * sy, Ty1 = -16.0, Ty2 = 8.0 - 4.0 * dt * sy;
+  const double Tz0 = 8.0 + 4.0 * dt * sz, Tz1 = -16.0, Tz2 = 8.0 - 4.0 * dt * sz;
+  const double Uxy0 = 2.0 + dt * sz, Uxy1 = -2.0 + dt * sz, Wxy0 = Z0 * (2.0 + dt * sy), Wxy1 = Z0 * (-2.0 + dt * sy);
+  const double Uyx0 = 2.0 + dt * sz, Uyx1 = -2.0 + dt * sz, Wyx0 = Z0 * (2.0 + dt * sx), Wyx1 = Z0 * (-2.0 + dt * sx);
+  const double Uyz0 = 2.0 + dt * sx, Uyz1 = -2.0 + dt * sx, Wyz0 = Z0 * (2.0 + dt * sz), Wyz1 = Z0 * (-2.0 + dt * sz);
+  const double Uzy0 = 2.0 + dt * sx, Uzy1 = -2.0 + dt * sx, Wzy0 = Z0 * (2.0 + dt * sy), Wzy1 = Z0 * (-2.0 + dt * sy);
+  const double Uzx0 = 2.0 + dt * sy, Uzx1 = -2.0 + dt * sy, Wzx0 = Z0 * (2.0 + dt * sx), Wzx1 = Z0 * (-2.0 + dt * sx);
+  const double Uxz0 = 2.0 + dt * sy, Uxz1 = -2.0 + dt * sy, Wxz0 = Z0 * (2.0 + dt * sz), Wxz1 = Z0 * (-2.0 + dt * sz);
+  const double Px0 = 4.0 + 2.0 * dt * sx, Px1 = -8.0, Px2 = 4.0 - 2.0 * dt * sx;
+  const double Py0 = 4.0 + 2.0 * dt * sy, Py1 = -8.0, Py2 = 4.0 - 2.0 * dt * sy;
+  const double Pz0 = 4.0 + 2.0 * dt * sz, Pz1 = -8.0, Pz2 = 4.0 - 2.0 * dt * sz;
+  const double Qx0 = 4.0 * Zsx + 2.0 * dt * (sy + sz) * Zsx + 1.0 * (dt * dt * sy * sz * Zsx);
+  const double Qx1 = -8.0 * Zsx + 2.0 * (dt * dt * sy * sz * Zsx);
+  const double Qx2 = 4.0 * Zsx - 2.0 * dt * (sy + sz) * Zsx + 1.0 * (dt * dt * sy * sz * Zsx);
+  const double Qy0 = 4.0 * Zsy + 2.0 * dt * (sz + sx) * Zsy + 1.0 * (dt * dt * sz * sx * Zsy);
+  const double Qy1 = -8.0 * Zsy + 2.0 * (dt * dt * sz * sx * Zsy);
+  const double Qy2 = 4.0 * Zsy - 2.0 * dt * (sz + sx) * Zsy + 1.0 * (dt * dt * sz * sx * Zsy);
+  const double Qz0 = 4.0 * Zsz + 2.0 * dt * (sx + sy) * Zsz + 1.0 * (dt * dt * sx * sy * Zsz);
+  const double Qz1 = -8.0 * Zsz + 2.0 * (dt * dt * sx * sy * Zsz);
+  const double Qz2 = 4.0 * Zsz - 2.0 * dt * (sx + sy) * Zsz + 1.0 * (dt * dt * sx * sy * Zsz);
+  // STAGE 2
+  const long long n = d.n_pm;
+  double* M = d.pm_state;
+  auto W = [&](int w) -> double& { return M[(long long)w * n + i]; };
+  const double Ixt_m1 = W(PM_Ixt), Iyt_m1 = W(PM_Iyt), Izt_m1 = W(PM_Izt);
+  const double Vxt_m1 = W(PM_Vxt), Vyt_m1 = W(PM_Vyt), Vzt_m1 = W(PM_Vzt);
+  const double Vyxt_m1 = W(PM_Vyxt), Vzxt_m1 = W(PM_Vzxt), Vxyt_m1 = W(PM_Vxyt);
+  const double Vzyt_m1 = W(PM_Vzyt), Vyzt_m1 = W(PM_Vyzt), Vxzt_m1 = W(PM_Vxzt);
+  const double Ixt_m2 = W(PM_Ixt_m1), Iyt_m2 = W(PM_Iyt_m1), Izt_m2 = W(PM_Izt_m1);
+  const double Vcx_m2 = W(PM_Vcx_m1), Vcy_m2 = W(PM_Vcy_m1), Vcz_m2 = W(PM_Vcz_m1);
+  const double Vcx_m1 = W(PM_Vcx), Vcy_m1 = W(PM_Vcy), Vcz_m1 = W(PM_Vcz);
+  const double Vxt_m2 = W(PM_Vxt_m1), Vyt_m2 = W(PM_Vyt_m1), Vzt_m2 = W(PM_Vzt_m1);
+  const double Vosx = W(PM_Vosxr), Vosy = W(PM_Vosyr), Vosz = W(PM_Voszr);
+  const double Vosx_m1 = W(PM_Vosx), Vosx_m2 = W(PM_Vosx_m1);
+  const double Vosy_m1 = W(PM_Vosy), Vosy_m2 = W(PM_Vosy_m1);
+  const double Vosz_m1 = W(PM_Vosz), Vosz_m2 = W(PM_Vosz_m1);
+  const double Vssx = -W(PM_Vssxr), Vssy = -W(PM_Vssyr), Vssz = -W(PM_Vsszr);
+  const double Vsx_m1 = W(PM_Vsx), Vsx_m2 = W(PM_Vsx_m1);
+  const double Vsy_m1 = W(PM_Vsy), Vsy_m2 = W(PM_Vsy_m1);
+  const double Vsz_m1 = W(PM_Vsz), Vsz_m2 = W(PM_Vsz_m1);
+  auto Vi1 = [&](int p) { return W(PM_Vi + p); };
+  auto Vi2 = [&](int p) { return W(PM_Vi_m1 + p); };
+  // STAGE 3
+  v[Vynx] = v[Vynx] * exp_y; v[Vypx] = v[Vypx] * exp_y;
+  v[Vznx] = v[Vznx] * exp_z; v[Vzpx] = v[Vzpx] * exp_z;
+  v[Vxny] = v[Vxny] * exp_x; v[Vxpy] = v[Vxpy] * exp_x;
+  v[Vzny] = v[Vzny] * exp_z; v[Vzpy] = v[Vzpy] * exp_z;
+  v[Vxnz] = v[Vxnz] * exp_x; v[Vxpz] = v[Vxpz] * exp_x;
+  v[Vynz] = v[Vynz] * exp_y; v[Vypz] = v[Vypz] * exp_y;
+  // STAGE 4
+  const double Vxt = (1.0 / Ax0) * (Bx0 * (v[Vznx] + v[Vzpx]) + Cx0 * (v[Vynx] + v[Vypx]) + Dx0 * (Vosx)
+                                    + Bx1 * (Vi1(Vznx) + Vi1(Vzpx)) + Cx1 * (Vi1(Vynx) + Vi1(Vypx)) + Dx1 * (Vosx_m1)
+                                    + Bx2 * (Vi2(Vznx) + Vi2(Vzpx)) + Cx2 * (Vi2(Vynx) + Vi2(Vypx)) + Dx2 * (Vosx_m2)
+                                    - Ax1 * Vxt_m1 - Ax2 * Vxt_m2);
+  const double Vyt = (1.0 / Ay0) * (By0 * (v[Vxny] + v[Vxpy]) + Cy0 * (v[Vzny] + v[Vzpy]) + Dy0 * (Vosy)
+                                    + By1 * (Vi1(Vxny) + Vi1(Vxpy)) + Cy1 * (Vi1(Vzny) + Vi1(Vzpy)) + Dy1 * (Vosy_m1)
+                                    + By2 * (Vi2(Vxny) + Vi2(Vxpy)) + Cy2 * (Vi2(Vzny) + Vi2(Vzpy)) + Dy2 * (Vosy_m2)
+                                    - Ay1 * Vyt_m1 - Ay2 * Vyt_m2);
+  const double Vzt = (1.0 / Az0) * (Bz0 * (v[Vynz] + v[Vypz]) + Cz0 * (v[Vxnz] + v[Vxpz]) + Dz0 * (Vosz)
+                                    + Bz1 * (Vi1(Vynz) + Vi1(Vypz)) + Cz1 * (Vi1(Vxnz) + Vi1(Vxpz)) + Dz1 * (Vosz_m1)
+                                    + Bz2 * (Vi2(Vynz) + Vi2(Vypz)) + Cz2 * (Vi2(Vxnz) + Vi2(Vxpz)) + Dz2 * (Vosz_m2)
+                                    - Az1 * Vzt_m1 - Az2 * Vzt_m2);
+  // STAGE 5
+  const double Vcx = v[Vypz] - v[Vzpy] - v[Vynz] + v[Vzny] - Vssx;
+  const double Vcy = v[Vzpx] - v[Vxpz] - v[Vznx] + v[Vxnz] - Vssy;
+  const double Vcz = v[Vxpy] - v[Vypx] - v[Vxny] + v[Vynx] - Vssz;
+  const double Ixt = (1.0 / Sx0) * (Tx0 * Vcx + Tx1 * Vcx_m1 + Tx2 * Vcx_m2 - Sx1 * Ixt_m1 - Sx2 * Ixt_m2);
+  const double Iyt = (1.0 / Sy0) * (Ty0 * Vcy + Ty1 * Vcy_m1 + Ty2 * Vcy_m2 - Sy1 * Iyt_m1 - Sy2 * Iyt_m2);
+  const double Izt = (1.0 / Sz0) * (Tz0 * Vcz + Tz1 * Vcz_m1 + Tz2 * Vcz_m2 - Sz1 * Izt_m1 - Sz2 * Izt_m2);
+  const double Vxyt = (1.0 / Uxy0) * (Wxy0 * Izt + Wxy1 * Izt_m1 - Uxy1 * Vxyt_m1);
+  const double Vyxt = (1.0 / Uyx0) * (Wyx0 * Izt + Wyx1 * Izt_m1 - Uyx1 * Vyxt_m1);
+  const double Vyzt = (1.0 / Uyz0) * (Wyz0 * Ixt + Wyz1 * Ixt_m1 - Uyz1 * Vyzt_m1);
+  const double Vzyt = (1.0 / Uzy0) * (Wzy0 * Ixt + Wzy1 * Ixt_m1 - Uzy1 * Vzyt_m1);
+  const double Vxzt = (1.0 / Uxz0) * (Wxz0 * Iyt + Wxz1 * Iyt_m1 - Uxz1 * Vxzt_m1);
+  const double Vzxt = (1.0 / Uzx0) * (Wzx0 * Iyt + Wzx1 * Iyt_m1 - Uzx1 * Vzxt_m1);
+  // STAGE 9: this step's values in, the previous step's to the _m1 slots
+#pragma unroll
+  for (int p = 0; p < 12; p++) {
+    W(PM_Vi_m1 + p) = W(PM_Vi + p);
+    W(PM_Vi + p) = v[p];
+  }
+  W(PM_Vxt_m1) = Vxt_m1; W(PM_Vyt_m1) = Vyt_m1; W(PM_Vzt_m1) = Vzt_m1;
+  W(PM_Vxt) = Vxt; W(PM_Vyt) = Vyt; W(PM_Vzt) = Vzt;
+  W(PM_Ixt_m1) = Ixt_m1; W(PM_Iyt_m1) = Iyt_m1; W(PM_Izt_m1) = Izt_m1;
+  W(PM_Ixt) = Ixt; W(PM_Iyt) = Iyt; W(PM_Izt) = Izt;
+  W(PM_Vcx_m1) = Vcx_m1; W(PM_Vcy_m1) = Vcy_m1; W(PM_Vcz_m1) = Vcz_m1;
+  W(PM_Vcx) = Vcx; W(PM_Vcy) = Vcy; W(PM_Vcz) = Vcz;
+  W(PM_Vxyt_m1) = Vxyt_m1; W(PM_Vyxt_m1) = Vyxt_m1;
+  W(PM_Vxyt) = Vxyt; W(PM_Vyxt) = Vyxt;
+  W(PM_Vzxt_m1) = Vzxt_m1; W(PM_Vxzt_m1) = Vxzt_m1;
+  W(PM_Vzxt) = Vzxt; W(PM_Vxzt) = Vxzt;
+  W(PM_Vzyt_m1) = Vzyt_m1; W(PM_Vyzt_m1) = Vyzt_m1;
+  W(PM_Vzyt) = Vzyt; W(PM_Vyzt) = Vyzt;
+  // STAGE 11
+  double V_max, V_min;
+  V_max = v[Vypx]; V_min = v[Vynx];
+  v[Vynx] = Vxt - Vyxt - V_max;
+  v[Vypx] = Vxt + Vyxt - V_min;
+  V_max = v[Vzpx]; V_min = v[Vznx];
+  v[Vznx] = Vxt + Vzxt - V_max;
+  v[Vzpx] = Vxt - Vzxt - V_min;
+  V_max = v[Vzpy]; V_min = v[Vzny];
+  v[Vzny] = Vyt - Vzyt - V_max;
+  v[Vzpy] = Vyt + Vzyt - V_min;
+  V_max = v[Vxpy]; V_min = v[Vxny];
+  v[Vxny] = Vyt + Vxyt - V_max;
+  v[Vxpy] = Vyt - Vxyt - V_min;
+  V_max = v[Vxpz]; V_min = v[Vxnz];
+  v[Vxnz] = Vzt - Vxzt - V_max;
+  v[Vxpz] = Vzt + Vxzt - V_min;
+  V_max = v[Vypz]; V_min = v[Vynz];
+  v[Vynz] = Vzt + Vyzt - V_max;
+  v[Vypz] = Vzt - Vyzt - V_min;
+  // capacitive stub
+  W(PM_Vosxr) = Vxt - Vosx; W(PM_Vosyr) = Vyt - Vosy; W(PM_Voszr) = Vzt - Vosz;
+  W(PM_Vosx_m1) = Vosx_m1; W(PM_Vosx) = Vosx;
+  W(PM_Vosy_m1) = Vosy_m1; W(PM_Vosy) = Vosy;
+  W(PM_Vosz_m1) = Vosz_m1; W(PM_Vosz) = Vosz;
+  // inductive stub
+  const double Vsx = (1.0 / Px0) * (Qx0 * Ixt + Qx1 * Ixt_m1 + Qx2 * Ixt_m2 - Px1 * Vsx_m1 - Px2 * Vsx_m2);
+  const double Vsy = (1.0 / Py0) * (Qy0 * Iyt + Qy1 * Iyt_m1 + Qy2 * Iyt_m2 - Py1 * Vsy_m1 - Py2 * Vsy_m2);
+  const double Vsz = (1.0 / Pz0) * (Qz0 * Izt + Qz1 * Izt_m1 + Qz2 * Izt_m2 - Pz1 * Vsz_m1 - Pz2 * Vsz_m2);
+  W(PM_Vssxr) = (Vsx + Vssx); W(PM_Vssyr) = (Vsy + Vssy); W(PM_Vsszr) = (Vsz + Vssz);
+  W(PM_Vsx_m1) = Vsx_m1; W(PM_Vsx) = Vsx;
+  W(PM_Vsy_m1) = Vsy_m1; W(PM_Vsy) = Vsy;
+  W(PM_Vsz_m1) = Vsz_m1; W(PM_Vsz) = Vsz;
+  // STAGE 12
+  const int orow = d.pm_out[i];
+  if (orow >= 0) {
+    double* fo = d.ring + ((long long)slot * d.n_out_rows + orow) * 6;
+    fo[0] = -Vxt / dl; fo[1] = -Vyt / dl; fo[2] = -Vzt / dl;
+    fo[3] = Ixt / dl; fo[4] = Iyt / dl; fo[5] = Izt / dl;
+  }
+  // STAGE 13
+  v[Vynx] = v[Vynx] * exp_y; v[Vypx] = v[Vypx] * exp_y;
+  v[Vznx] = v[Vznx] * exp_z; v[Vzpx] = v[Vzpx] * exp_z;
+  v[Vxny] = v[Vxny] * exp_x; v[Vxpy] = v[Vxpy] * exp_x;
+  v[Vzny] = v[Vzny] * exp_z; v[Vzpy] = v[Vzpy] * exp_z;
+  v[Vxnz] = v[Vxnz] * exp_x; v[Vxpz] = v[Vxpz] * exp_x;
+  v[Vynz] = v[Vynz] * exp_y; v[Vypz] = v[Vypz] * exp_y;
+  store_cell(Lout, d.P, c, v);
+}
+
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_pml_material(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                     PullArgs a, int slot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.n_pm) pml_material_update<T, PLAIN, PULL>(d, Lin, Lout, a, slot, i);
+}
+
 // Thevenin node of a special cell (scatter.F90:274-547): materials,
 // optional point source and output row, filter update, 12-port scatter.
 // IO = false compiles out the excitation and output branches.
@@ -1086,6 +1390,7 @@ __device__ inline void generic_update(const Dev& d, const T* __restrict__ Lin, T
 #pragma unroll
     for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
   }
+  flush_small(v, a.small);
   if (d.gen_code[i] == 0) {
     if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
   } else {
@@ -1116,6 +1421,7 @@ __device__ inline void material_update(const Dev& d, const T* __restrict__ Lin, 
 #pragma unroll
     for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
   }
+  flush_small(v, a.small);
   node_update<false>(d, v, d.mat_num[i], d.mat_filt[i], -1, -1, 0, 0);
   store_cell(Lout, d.P, c, v);
 }
@@ -1145,25 +1451,88 @@ __global__ __launch_bounds__(256) void k_bulk_lines(Dev d, const T* __restrict__
   bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z);
 }
 
-// Mixed lines: a workgroup owns 256/L whole lines; its 256 slots hold their
-// cells sorted by class, so each wavefront mostly runs one class and every
-// line is completed by one workgroup (one L2) within the launch.
+// kMixedBulkPerLane bulk cells per lane of the mixed-line kernel: their
+// loads are issued together, so a wavefront of bulk lanes keeps as many
+// bytes in flight as the heavier PML lanes at the same occupancy.
+#ifndef TLM_MIXED_BULK_PER_LANE
+#define TLM_MIXED_BULK_PER_LANE 4
+#endif
+constexpr int kMixedBulkPerLane = TLM_MIXED_BULK_PER_LANE;
+
+template <typename T, bool PLAIN, bool PULL, int B>
+__device__ inline void bulk_update_n(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                     const PullArgs& a, const long long (&cs)[B]) {
+  const long long P = d.P;
+  const long long sxy = (long long)d.nx * d.ny;
+  T o[B][12], nb[B][12];
+  int x[B], y[B], z[B];
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    x[b] = y[b] = z[b] = 0;
+    const long long c = cs[b];
+    if (c < 0) continue;
+    cell_coords(d, c, x[b], y[b], z[b]);
+#pragma unroll
+    for (int p = 0; p < 12; p++) o[b][p] = Lin[p * P + c];
+    if (PULL) {
+#pragma unroll
+      for (int k = 0; k < 12; k++) nb[b][k] = (T)0;
+      if (x[b] > 0) { nb[b][0] = Lin[Vy_xmax * P + c - 1]; nb[b][1] = Lin[Vz_xmax * P + c - 1]; }
+      if (x[b] + 1 < d.nx) { nb[b][2] = Lin[Vy_xmin * P + c + 1]; nb[b][3] = Lin[Vz_xmin * P + c + 1]; }
+      if (y[b] > 0) { nb[b][4] = Lin[Vx_ymax * P + c - d.nx]; nb[b][5] = Lin[Vz_ymax * P + c - d.nx]; }
+      if (y[b] + 1 < d.ny) { nb[b][6] = Lin[Vx_ymin * P + c + d.nx]; nb[b][7] = Lin[Vz_ymin * P + c + d.nx]; }
+      if (z[b] > 0) { nb[b][8] = Lin[Vx_zmax * P + c - sxy]; nb[b][9] = Lin[Vy_zmax * P + c - sxy]; }
+      if (z[b] + 1 < d.nz) { nb[b][10] = Lin[Vx_zmin * P + c + sxy]; nb[b][11] = Lin[Vy_zmin * P + c + sxy]; }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    const long long c = cs[b];
+    if (c < 0) continue;
+    double v[12];
+    if (PULL) {
+      double od[12];
+#pragma unroll
+      for (int p = 0; p < 12; p++) od[p] = (double)o[b][p];
+      bulk_incident<PLAIN>(d, a, x[b], y[b], z[b], od, nb[b][0], nb[b][1], nb[b][2], nb[b][3], nb[b][4], nb[b][5],
+                           nb[b][6], nb[b][7], nb[b][8], nb[b][9], nb[b][10], nb[b][11], v);
+    } else {
+#pragma unroll
+      for (int p = 0; p < 12; p++) v[p] = (double)o[b][p];
+    }
+    flush_small(v, a.small);
+    if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
+    store_cell(Lout, P, c, v);
+  }
+}
+
+// Mixed lines: a workgroup owns whole 128-byte lines (as many as fit in
+// 256 lanes with kMixedBulkPerLane bulk cells per lane); its lanes hold the
+// special cells sorted by class (PML, generic, material) and then the bulk
+// cells, so each wavefront mostly runs one class and every line is
+// completed by one workgroup (one L2) within the launch.
+//   slot_a[lane] = cell | class << 28 (-1: idle lane)
+//   slot_b[k][lane] = class-list position (special), or the k-th further
+//                     bulk cell (-1: none)
 template <typename T, bool PLAIN, bool PULL>
 __global__ __launch_bounds__(256) void k_mixed_lines(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                    PullArgs a, const int32_t* __restrict__ slot_cell,
-                                                    const int32_t* __restrict__ slot_idx, int timestep, int slot) {
+                                                    PullArgs a, const int32_t* __restrict__ slot_a,
+                                                    const int32_t* __restrict__ slot_b, long long n_slots,
+                                                    int timestep, int slot) {
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
-  const int32_t w = slot_cell[s];
+  const int32_t w = slot_a[s];
   if (w < 0) return;
   const long long c = w & kCellMask;
   const int cls = (w >> 28) & 3;
   if (cls == 0) {
-    int x, y, z;
-    cell_coords(d, c, x, y, z);
-    bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z);
+    long long cs[kMixedBulkPerLane];
+    cs[0] = c;
+#pragma unroll
+    for (int k = 1; k < kMixedBulkPerLane; k++) cs[k] = slot_b[(k - 1) * n_slots + s];
+    bulk_update_n<T, PLAIN, PULL, kMixedBulkPerLane>(d, Lin, Lout, a, cs);
     return;
   }
-  const int i = slot_idx[s];
+  const int i = slot_b[s];
   if (cls == 1) pml_update<T, PLAIN, PULL>(d, Lin, Lout, a, slot, i);
   else if (cls == 2) material_update<T, PLAIN, PULL>(d, Lin, Lout, a, i);
   else generic_update<T, PLAIN, PULL>(d, Lin, Lout, a, timestep, slot, i);

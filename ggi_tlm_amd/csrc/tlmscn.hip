@@ -74,6 +74,9 @@ struct tlm_handle {
   int n_gen = 0, n_pml = 0, n_mat = 0;
   int n_pure_pml = 0, n_pure_mat = 0, n_pure_gen = 0, n_mixed_blocks = 0;
   int n_huy_coded = 0;  // descriptor entries in the kFdHuy form
+  int n_pm = 0;         // PML cells holding a material
+  int stz_n = 0;        // TLM_set_small_to_zero every stz_n steps (0: off)
+  double stz_value = 0.0;
   long long n_mixed_cells = 0;
   const int32_t* d_slot_cell = nullptr;  // mixed-line workgroup slots
   const int32_t* d_slot_idx = nullptr;
@@ -107,6 +110,10 @@ struct tlm_handle {
   int ring_first_step = 0;   // timestep of slot 0
   std::vector<double> host_out;  // drained rows awaiting read
   int host_first = 0, host_steps = 0;
+  // face outputs: one block of n_face_out_rows * 2 * 6 per connect
+  int fring_used = 0, fring_first_step = 0;
+  std::vector<double> host_fout;
+  int host_ffirst = 0, host_fsteps = 0;
 };
 
 namespace {
@@ -220,6 +227,8 @@ int finalize(tlm_handle* h) {
   }
   // surface material type, excitation and output rows per special face
   std::vector<int32_t> sf_mtype(h->n_special_faces + 1, 0), sf_exc(h->n_special_faces + 1, -1);
+  std::vector<int32_t> sf_out(h->n_special_faces + 1, -1);
+  int n_face_out_rows = 0;
   {
     int er = 0;
     // excitation rows follow the face scan order = code order
@@ -231,6 +240,8 @@ int finalize(tlm_handle* h) {
         sf_mtype[code] = h->smat[mn - 1];
       }
       if (h->f_exc[code - 1] != 0) sf_exc[code] = er++;
+      // output rows follow the face scan order too (output_face_number)
+      if (!h->f_out.empty() && h->f_out[code - 1] != 0) sf_out[code] = n_face_out_rows++;
     }
     if (er != h->n_face_rows) return fail(h, TLM_ERR_ARG, "face excitation rows do not match the face codes");
   }
@@ -269,7 +280,9 @@ int finalize(tlm_handle* h) {
   struct PmlRec { int32_t c; uint16_t par; int32_t orow, fdc; };
   struct MatRec { int32_t c; uint16_t m; int32_t f; };
   struct GenRec { int32_t c, code, m, f, er, orow, fdc; };
+  struct PmRec { int32_t c, par, m, orow, fdc; };  // PML cell holding a material (class 4)
   std::vector<uint8_t> cls(N, 0);
+  std::vector<PmRec> pmr;
   std::vector<PmlRec> pr;
   std::vector<MatRec> mr;
   std::vector<GenRec> gr;
@@ -295,12 +308,20 @@ int finalize(tlm_handle* h) {
         int t = h->vmat[m - 1].type;
         if (t == TLM_VMAT_PEC) return fail(h, TLM_ERR_ARG, "ERROR: PEC volume material in PML");
         if (t == TLM_VMAT_PMC) return fail(h, TLM_ERR_ARG, "ERROR: PMC volume material in PML");
-        return fail(h, TLM_ERR_UNSUPPORTED, "material inside the PML is not supported on the device path");
       }
       if (h->c_exc[s]) return fail(h, TLM_ERR_ARG, "Excitation intersects PML");
       int par = h->pml_cell_param[pmlc - 1];
       if (par < 1 || par > (int)h->pml_par.size() || par > 65536)
         return fail(h, TLM_ERR_ARG, "PML parameter index out of range");
+      if (m && h->vmat[m - 1].type == TLM_VMAT_DISPERSIVE) {
+        // set_update_codes.F90:300-333: only order-0 (non-dispersive)
+        // S-filters may sit in the PML; their stub Z-filters have order <= 1
+        if (h->vmat[m - 1].eps_order > 1 || h->vmat[m - 1].mu_order > 1)
+          return fail(h, TLM_ERR_ARG, "PML fatal error: dispersive material found");
+        cls[c] = 4;
+        pmr.push_back({(int32_t)c, par - 1, m, orow, fdc});
+        continue;
+      }
       cls[c] = 1;
       pr.push_back({(int32_t)c, (uint16_t)(par - 1), orow, fdc});
     } else {
@@ -355,34 +376,27 @@ int finalize(tlm_handle* h) {
   while (n_pure_pml < (int)pr.size() && pure_of(pr[n_pure_pml].c)) n_pure_pml++;
   while (n_pure_mat < (int)mr.size() && pure_of(mr[n_pure_mat].c)) n_pure_mat++;
   while (n_pure_gen < (int)gr.size() && pure_of(gr[n_pure_gen].c)) n_pure_gen++;
-  // Mixed lines are grouped 256/L to a workgroup (256 cells).  Within a
-  // group the cells are sorted by class (PML, generic, material, bulk; scan
-  // order inside a class), so a wavefront mostly runs one class while the
-  // workgroup still writes all of its lines.  Slot word: cell | class << 28,
-  // or -1 for an empty slot; slot index: the cell's class-list position.
-  std::vector<int32_t> slot_cell, slot_idx;
+  // Mixed lines: workgroups of 256 lanes, each owning whole lines.  A
+  // special cell takes one lane, bulk cells kMixedBulkPerLane per lane; a
+  // group takes consecutive mixed lines (scan order) while its lanes fit.
+  // Lanes: special cells sorted by class (PML, generic, material; scan order
+  // inside a class), then bulk lanes, bulk lane j holding bulk cells
+  // j, j + nb, j + 2 nb, ... of the group (nb = bulk lanes), so that
+  // neighbouring lanes touch neighbouring cells.
+  constexpr int KB = scn::kMixedBulkPerLane;
+  std::vector<int32_t> slot_a;
+  std::vector<std::vector<int32_t>> slot_b(KB > 1 ? KB - 1 : 1);
   {
     int ip = n_pure_pml, im = n_pure_mat, ig = n_pure_gen;
-    const int G = 256 / L;
-    // groups of G consecutive mixed lines in scan order (an XCD-banded
-    // order was measured slower on the 256^3 workload)
-    std::vector<std::vector<long long>> groups;
-    {
-      std::vector<long long> gl;
-      for (long long ln = 0; ln < n_lines; ln++) {
-        if (line_info[ln] != 2) continue;
-        gl.push_back(ln);
-        if ((int)gl.size() == G) { groups.push_back(gl); gl.clear(); }
-      }
-      if (!gl.empty()) groups.push_back(gl);
-    }
     // class-list positions of the mixed-line cells follow scan order (the
-    // tails of the three lists); pos_of_line = rank of a mixed line
-    std::vector<int32_t> pos_of_line(n_lines, -1);
-    std::vector<int32_t> cell_idx;  // per mixed-line cell, scan order
-    int32_t n_mixed_seen = 0;
+    // tails of the three lists)
+    std::vector<long long> mlines;
+    std::vector<int32_t> line_first_idx;  // per mixed line: offset into cell_idx
+    std::vector<int32_t> cell_idx;
     for (long long ln = 0; ln < n_lines; ln++) {
       if (line_info[ln] != 2) continue;
+      mlines.push_back(ln);
+      line_first_idx.push_back((int32_t)cell_idx.size());
       long long c0 = ln * L, c1 = std::min(N, c0 + L);
       for (long long c = c0; c < c1; c++) {
         int k2 = cls[c], idx = 0;
@@ -391,29 +405,65 @@ int finalize(tlm_handle* h) {
         else if (k2 == 3) idx = ig++;
         cell_idx.push_back(idx);
       }
-      pos_of_line[ln] = n_mixed_seen++;
-    }
-    static const int rank_of[4] = {3, 0, 2, 1};  // bulk, PML, material, generic
-    for (auto& gl : groups) {
-      std::vector<std::pair<int, std::pair<int32_t, int32_t>>> cells;  // (rank, (word, idx))
-      for (long long ln : gl) {
-        long long c0 = ln * L, c1 = std::min(N, c0 + L);
-        const size_t base = (size_t)pos_of_line[ln] * L;  // lines before it are whole (only the last may be short)
-        for (long long c = c0; c < c1; c++) {
-          n_mixed_cells++;
-          const int k2 = cls[c];
-          cells.push_back({rank_of[k2], {(int32_t)(c | ((long long)k2 << 28)), cell_idx[base + (c - c0)]}});
-        }
-      }
-      std::stable_sort(cells.begin(), cells.end(),
-                       [](const auto& a, const auto& b) { return a.first < b.first; });
-      for (int s = 0; s < 256; s++) {
-        slot_cell.push_back(s < (int)cells.size() ? cells[s].second.first : -1);
-        slot_idx.push_back(s < (int)cells.size() ? cells[s].second.second : 0);
-      }
     }
     if (ip != (int)pr.size() || im != (int)mr.size() || ig != (int)gr.size())
       return fail(h, TLM_ERR_ARG, "internal: mixed-line cells inconsistent");
+    static const int rank_of[4] = {3, 0, 2, 1};  // bulk, PML, material, generic
+    auto emit = [&](size_t l0, size_t l1) {
+      std::vector<std::pair<int, std::pair<int32_t, int32_t>>> spec;  // (rank, (word, idx))
+      std::vector<int32_t> bulk;
+      for (size_t q = l0; q < l1; q++) {
+        long long c0 = mlines[q] * L, c1 = std::min(N, c0 + L);
+        for (long long c = c0; c < c1; c++) {
+          const int k2 = cls[c];
+          if (k2 == 4) continue;  // own kernel
+          n_mixed_cells++;
+          if (k2 == 0) bulk.push_back((int32_t)c);
+          else
+            spec.push_back({rank_of[k2], {(int32_t)(c | ((long long)k2 << 28)),
+                                          cell_idx[line_first_idx[q] + (c - c0)]}});
+        }
+      }
+      std::stable_sort(spec.begin(), spec.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+      const int nb = (int)((bulk.size() + KB - 1) / KB);
+      int lane = 0;
+      for (auto& e : spec) {
+        slot_a.push_back(e.second.first);
+        slot_b[0].push_back(e.second.second);
+        for (int k = 1; k < KB - 1; k++) slot_b[k].push_back(-1);
+        lane++;
+      }
+      for (int j = 0; j < nb; j++) {
+        slot_a.push_back(bulk[j]);
+        for (int k = 1; k < KB; k++) {
+          const size_t q = (size_t)j + (size_t)k * nb;
+          slot_b[k - 1].push_back(q < bulk.size() ? bulk[q] : -1);
+        }
+        if (KB == 1) slot_b[0].push_back(-1);
+        lane++;
+      }
+      for (; lane < 256; lane++) {
+        slot_a.push_back(-1);
+        for (auto& sb : slot_b) sb.push_back(-1);
+      }
+    };
+    size_t l0 = 0;
+    int n_spec = 0, n_bulk = 0;
+    for (size_t q = 0; q < mlines.size(); q++) {
+      long long c0 = mlines[q] * L, c1 = std::min(N, c0 + L);
+      int s = 0, b = 0;
+      for (long long c = c0; c < c1; c++)
+        if (cls[c] != 4) (cls[c] ? s : b)++;
+      if (q > l0 && n_spec + s + (n_bulk + b + KB - 1) / KB > 256) {
+        emit(l0, q);
+        l0 = q;
+        n_spec = 0;
+        n_bulk = 0;
+      }
+      n_spec += s;
+      n_bulk += b;
+    }
+    if (l0 < mlines.size()) emit(l0, mlines.size());
   }
   // flat per-class arrays in list order
   std::vector<int32_t> pml_cell, pml_fd;
@@ -486,6 +536,7 @@ int finalize(tlm_handle* h) {
     if (code <= 0) continue;
     const int mt = sf_mtype[code], row = sf_exc[code];
     if (mt != 0 && mt != TLM_SMAT_FREE_SPACE) continue;
+    if (sf_out[code] >= 0) continue;  // face outputs read the code's tables
     if (row < 0 || fex_start[row + 1] != fex_start[row] || fhy_start[row + 1] - fhy_start[row] != 1) continue;
     const int f = fhy_face[fhy_start[row]];
     const tlm_huygens_face& hf = h->huy[f];
@@ -527,9 +578,13 @@ int finalize(tlm_handle* h) {
   }
   d.line_len = L;
   if ((rc = dev_upload(h, line_info, &d.line_info))) return rc;
-  if ((rc = dev_upload(h, slot_cell, &h->d_slot_cell))) return rc;
-  if ((rc = dev_upload(h, slot_idx, &h->d_slot_idx))) return rc;
-  h->n_mixed_blocks = (int)(slot_cell.size() / 256);
+  {
+    std::vector<int32_t> sb;
+    for (auto& v : slot_b) sb.insert(sb.end(), v.begin(), v.end());
+    if ((rc = dev_upload(h, slot_a, &h->d_slot_cell))) return rc;
+    if ((rc = dev_upload(h, sb, &h->d_slot_idx))) return rc;
+    h->n_mixed_blocks = (int)(slot_a.size() / 256);
+  }
   if ((rc = dev_upload(h, h->pml_par, &d.pml_params))) return rc;
   d.n_pml_state = std::max(d.n_pml, 1);
   {
@@ -546,6 +601,23 @@ int finalize(tlm_handle* h) {
   if ((rc = dev_upload(h, mat_cell, &d.mat_cell))) return rc;
   if ((rc = dev_upload(h, mat_num, &d.mat_num))) return rc;
   if ((rc = dev_upload(h, mat_filt, &d.mat_filt))) return rc;
+  {
+    std::vector<int32_t> pc, pp, pm, po, pf;
+    for (auto& r : pmr) { pc.push_back(r.c); pp.push_back(r.par); pm.push_back(r.m); po.push_back(r.orow); pf.push_back(r.fdc); }
+    h->n_pm = d.n_pm = (int)pmr.size();
+    if ((rc = dev_upload(h, pc, &d.pm_cell))) return rc;
+    if ((rc = dev_upload(h, pp, &d.pm_par))) return rc;
+    if ((rc = dev_upload(h, pm, &d.pm_mat))) return rc;
+    if ((rc = dev_upload(h, po, &d.pm_out))) return rc;
+    if ((rc = dev_upload(h, pf, &d.pm_fd))) return rc;
+    if (h->n_pm) {
+      void* p;
+      const size_t bytes = (size_t)scn::PM_N * h->n_pm * sizeof(double);
+      if ((rc = dev_alloc(h, bytes, &p))) return rc;
+      HIP_TRY(h, hipMemsetAsync(p, 0, bytes, h->stream));
+      d.pm_state = (double*)p;
+    }
+  }
   h->n_gen = d.n_gen = (int)gen_cell.size();
   if ((rc = dev_upload(h, gen_cell, &d.gen_cell))) return rc;
   if ((rc = dev_upload(h, gen_code, &d.gen_code))) return rc;
@@ -571,6 +643,13 @@ int finalize(tlm_handle* h) {
   if ((rc = dev_upload(h, fd, &d.fd))) return rc;
   if ((rc = dev_upload(h, sf_mtype, &d.sf_mtype))) return rc;
   if ((rc = dev_upload(h, sf_exc, &d.sf_exc))) return rc;
+  if ((rc = dev_upload(h, sf_out, &d.sf_out))) return rc;
+  d.n_face_out_rows = n_face_out_rows;
+  if (n_face_out_rows) {
+    void* p;
+    if ((rc = dev_alloc(h, (size_t)h->cap * n_face_out_rows * 12 * sizeof(double), &p))) return rc;
+    d.face_ring = (double*)p;
+  }
   if ((rc = dev_upload(h, h->value, &d.value))) return rc;
   if ((rc = dev_upload(h, h->value_face, &d.value_face))) return rc;
   if ((rc = dev_upload(h, cex_start, &d.cex_start))) return rc;
@@ -639,12 +718,39 @@ int drain_ring(tlm_handle* h) {
   return TLM_OK;
 }
 
-// Bulk schedule.  variant 0: one plane per block (zc = 1); variant 1:
-// z-marching blocks of zc planes.  Tile height by rows of 64 cells.  The
-// defaults come from the sweep in tune_bulk (csrc autotuner) and can be
-// overridden per handle with tlm_set_schedule.
-// Cells per thread of the vector bulk kernel (16-byte accesses).
+// Move filled face-output slots to the host staging vector.
+int drain_face_ring(tlm_handle* h) {
+  if (h->fring_used == 0) return TLM_OK;
+  const size_t w = (size_t)h->d.n_face_out_rows * 12;
+  const size_t n = (size_t)h->fring_used * w;
+  const size_t old = h->host_fout.size();
+  if (h->host_fsteps == 0) h->host_ffirst = h->fring_first_step;
+  h->host_fout.resize(old + n);
+  HIP_TRY(h, hipMemcpyAsync(h->host_fout.data() + old, h->d.face_ring, n * sizeof(double), hipMemcpyDeviceToHost,
+                            h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->host_fsteps += h->fring_used;
+  h->fring_used = 0;
+  return TLM_OK;
+}
+
+// Face-output slot for the connect of `timestep` (-1 when no face outputs).
+int reserve_face_slot(tlm_handle* h, int timestep, int* slot) {
+  *slot = -1;
+  if (h->d.n_face_out_rows == 0) return TLM_OK;
+  int rc;
+  if (h->fring_used == h->cap && (rc = drain_face_ring(h))) return rc;
+  if (h->fring_used == 0) h->fring_first_step = timestep;
+  *slot = h->fring_used++;
+  return TLM_OK;
+}
+
+// Bulk schedule (tlm_set_schedule): variant 3, the line kernel, unless
+// chosen otherwise per handle; variant 0: 64 x rows tiles of one plane;
+// variant 1: z-marching blocks of zc planes; variant 2: 16-byte vector
+// accesses.
 int sched_variant(const tlm_handle* h) {
+  if (h->stz_n > 0) return 3;  // the line kernel applies the small-value flush
   if (h->sched_variant >= 0) return h->sched_variant;
   return 3;
 }
@@ -719,10 +825,10 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
     dim3 g(h->n_mixed_blocks);
     if (pull)
       hipLaunchKernelGGL((scn::k_mixed_lines<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a,
-                         h->d_slot_cell, h->d_slot_idx, timestep, slot);
+                         h->d_slot_cell, h->d_slot_idx, (long long)h->n_mixed_blocks * 256, timestep, slot);
     else
       hipLaunchKernelGGL((scn::k_mixed_lines<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a,
-                         h->d_slot_cell, h->d_slot_idx, timestep, slot);
+                         h->d_slot_cell, h->d_slot_idx, (long long)h->n_mixed_blocks * 256, timestep, slot);
   }
   if (h->n_pure_pml) {
     ProfScope ps(h, TLM_KCLASS_PML);
@@ -746,6 +852,12 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
       hipLaunchKernelGGL((scn::k_generic<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, timestep, slot,
                          h->n_pure_gen);
   }
+  if (h->n_pm) {
+    ProfScope ps(h, TLM_KCLASS_PML_MATERIAL);
+    dim3 g((h->n_pm + 255) / 256);
+    if (pull) hipLaunchKernelGGL((scn::k_pml_material<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot);
+    else hipLaunchKernelGGL((scn::k_pml_material<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot);
+  }
   return TLM_OK;
 }
 
@@ -767,6 +879,9 @@ int launch_connect(tlm_handle* h, const PullArgs& a) {
   if (h->n_mat)
     hipLaunchKernelGGL((scn::k_connect_list<T, PLAIN>), dim3((h->n_mat + 255) / 256), dim3(256), 0, h->stream, d, Lin,
                        Lout, a, d.mat_cell, nullptr, h->n_mat);
+  if (h->n_pm)
+    hipLaunchKernelGGL((scn::k_connect_list<T, PLAIN>), dim3((h->n_pm + 255) / 256), dim3(256), 0, h->stream, d, Lin,
+                       Lout, a, d.pm_cell, d.pm_fd, h->n_pm);
   return TLM_OK;
 }
 
@@ -782,7 +897,10 @@ int do_scatter(tlm_handle* h, int timestep) {
     if (!h->pend_connect) return fail(h, TLM_ERR_STATE, "scatter called twice without connect");
     pull = true;
   }
-  PullArgs a{h->pend_boundary, h->pend_ts};
+  const double small = (h->stz_n > 0 && timestep % h->stz_n == 0) ? h->stz_value : 0.0;
+  int fslot = -1;
+  if (pull && (rc = reserve_face_slot(h, h->pend_ts, &fslot))) return rc;
+  PullArgs a{h->pend_boundary, h->pend_ts, small, fslot};
   if (h->d.n_out_rows > 0 && h->ring_used == h->cap)
     if ((rc = drain_ring(h))) return rc;
   int slot = h->ring_used;
@@ -813,7 +931,9 @@ int materialize(tlm_handle* h) {
   if ((rc = finalize(h))) return rc;
   if (h->kind == Kind::INCIDENT) return TLM_OK;
   if (!h->pend_connect) return TLM_OK;  // scattered state is what V holds
-  PullArgs a{h->pend_boundary, h->pend_ts};
+  int fslot = -1;
+  if ((rc = reserve_face_slot(h, h->pend_ts, &fslot))) return rc;
+  PullArgs a{h->pend_boundary, h->pend_ts, 0.0, fslot};
   if (h->form == TLM_FORM_PLAIN_FP64) rc = launch_connect<double, true>(h, a);
   else rc = launch_connect<float, false>(h, a);
   if (rc) return rc;
@@ -1142,6 +1262,25 @@ int tlm_read_cell_outputs(tlm_handle* h, double* out, int32_t max_steps, int32_t
   return TLM_OK;
 }
 
+int tlm_read_face_outputs(tlm_handle* h, double* out, int32_t max_steps, int32_t* n_read, int32_t* first_step) {
+  if (!h || max_steps < 0 || (max_steps && !out)) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  if ((rc = materialize(h))) return rc;  // a pending connect computes its face outputs
+  if ((rc = drain_face_ring(h))) return rc;
+  const size_t w = (size_t)h->d.n_face_out_rows * 12;
+  const int n = std::min(h->host_fsteps, (int)max_steps);
+  if (n > 0 && w > 0) std::memcpy(out, h->host_fout.data(), (size_t)n * w * sizeof(double));
+  if (n_read) *n_read = n;
+  if (first_step) *first_step = h->host_ffirst;
+  if (n > 0) {
+    h->host_fout.erase(h->host_fout.begin(), h->host_fout.begin() + (size_t)n * w);
+    h->host_fsteps -= n;
+    h->host_ffirst += n;
+  }
+  return TLM_OK;
+}
+
 int tlm_read_dft(tlm_handle* h, double* re_im) {
   if (!h || !re_im) return TLM_ERR_ARG;
   int rc;
@@ -1163,6 +1302,14 @@ int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, i
   h->sched_zc = zc;
   if (xcd_remap < 0 || xcd_remap > 2) return fail(h, TLM_ERR_ARG, "bad XCD mapping");
   h->sched_xcd = xcd_remap;
+  return TLM_OK;
+}
+
+int tlm_set_small_to_zero(tlm_handle* h, int32_t n_small_to_zero, double small_to_zero_value) {
+  if (!h) return TLM_ERR_ARG;
+  if (n_small_to_zero < 0 || !(small_to_zero_value >= 0.0)) return fail(h, TLM_ERR_ARG, "bad small_to_zero data");
+  h->stz_n = n_small_to_zero;
+  h->stz_value = small_to_zero_value;
   return TLM_OK;
 }
 
@@ -1226,7 +1373,7 @@ int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
   const int64_t n_special_mixed = (h->n_pml - h->n_pure_pml) + (h->n_mat - h->n_pure_mat) + (h->n_gen - h->n_pure_gen);
   // bulk: cells of the pure bulk lines, 12 links read + 12 written, plus the
   // line table (line kernel) or the class mask (tile kernels)
-  const int64_t n_bulk = h->N - h->n_mixed_cells - h->n_pure_pml - h->n_pure_mat - h->n_pure_gen;
+  const int64_t n_bulk = h->N - h->n_mixed_cells - h->n_pure_pml - h->n_pure_mat - h->n_pure_gen - h->n_pm;
   cells[TLM_KCLASS_BULK] = n_bulk;
   bytes[TLM_KCLASS_BULK] = n_bulk * 24 * sl + (sched_variant(h) == 3 ? 4 * n_lines : (h->N + 7) / 8);
   // mixed lines: every cell with its class's bytes, plus two slot words
@@ -1241,6 +1388,10 @@ int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
   bytes[TLM_KCLASS_GENERIC] = gen_bytes(0, h->n_pure_gen);
   cells[TLM_KCLASS_MATERIAL] = h->n_pure_mat;
   bytes[TLM_KCLASS_MATERIAL] = mat_bytes(0, h->n_pure_mat);
+  // PML cells holding a material: links, PM_N FP64 state words read and
+  // written, five int32 list fields
+  cells[TLM_KCLASS_PML_MATERIAL] = h->n_pm;
+  bytes[TLM_KCLASS_PML_MATERIAL] = (int64_t)h->n_pm * (24 * sl + 2 * 8 * scn::PM_N + 5 * 4);
   cells[TLM_KCLASS_DFT] = (int64_t)h->pt_row.size() * (int64_t)h->freq.size();
   bytes[TLM_KCLASS_DFT] = cells[TLM_KCLASS_DFT] * 2 * 2 * 8;
   return TLM_OK;

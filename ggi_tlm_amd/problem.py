@@ -459,9 +459,14 @@ def build_tables(p: Problem) -> Tables:
     if (pml_flag & (local_material != 0)).any():
         mats = np.unique(local_material[pml_flag & (local_material != 0)])
         for m in mats:
-            if p.materials[m - 1].kind in ("pec", "pmc"):
+            vm = p.materials[m - 1]
+            if vm.kind in ("pec", "pmc"):
                 raise ValueError("PEC/PMC volume material in PML")
-        raise NotImplementedError("materials intersecting the PML are not supported")
+            # set_update_codes.F90:300-333: PML cells may hold only order-0
+            # (non-dispersive) materials
+            for sf in (vm.eps, vm.mu):
+                if sf is not None and (len(sf.a) > 1 or len(sf.b) > 1):
+                    raise ValueError(f"PML fatal error: dispersive material found, material number {m}")
 
     local_exc = np.zeros(shape, dtype=bool)
     for s in p.sources:

@@ -226,10 +226,26 @@ int tlm_set_output_capacity(tlm_handle* h, int32_t steps);
 int tlm_read_cell_outputs(tlm_handle* h, double* out, int32_t max_steps,
                           int32_t* n_read, int32_t* first_step);
 
+/* Face output rows, face_output_field(row, side, 1:6)
+ * (connect_normal_to_x.F90:323-361 and the y, z includes; written each step
+ * by face_output_points, output.F90:232-272): one block of
+ * n_face_out_rows * 2 sides * 6 fields per connect, rows in face scan order,
+ * side 1 = the cell on the face's min side.  Reads the oldest max_steps
+ * blocks; *first_step = the timestep of the first connect returned.  A
+ * pending connect is applied first, so its block is included. */
+int tlm_read_face_outputs(tlm_handle* h, double* out, int32_t max_steps, int32_t* n_read, int32_t* first_step);
+
 /* DFT accumulators: re_im[(p*n_freq + k)*2 + {0,1}]. */
 int tlm_read_dft(tlm_handle* h, double* re_im);
 
 /* ---- schedule and measurement (no counterpart in the reference) ---------- */
+
+/* TLM_set_small_to_zero (TLM_SOLVER/TLM_set_small_to_zero.F90:34-68, gated
+ * at run_TLM.F90:236-238; packet read_small_to_zero.F90:55-59): at every
+ * step whose number is a multiple of n_small_to_zero, link voltages with
+ * abs(V) < small_to_zero_value are set to zero before that step's scatter.
+ * n_small_to_zero = 0 turns it off (the default). */
+int tlm_set_small_to_zero(tlm_handle* h, int32_t n_small_to_zero, double small_to_zero_value);
 
 /* Bulk-kernel schedule: variant 3 (default) = one 256-cell row segment per
  * workgroup over the pure bulk lines; variant 0 = one cell per lane, 64 x rows
@@ -251,7 +267,8 @@ int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc,
 #define TLM_KCLASS_DFT 3
 #define TLM_KCLASS_MATERIAL 4
 #define TLM_KCLASS_MIXED 5
-#define TLM_N_KCLASS 6
+#define TLM_KCLASS_PML_MATERIAL 6
+#define TLM_N_KCLASS 7
 
 /* When enabled, every kernel launch of a step is bracketed by HIP events
  * recorded on the handle's own stream.  tlm_read_kernel_times waits for the

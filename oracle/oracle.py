@@ -22,8 +22,7 @@ def library():
     global _lib
     if _lib is None:
         _lib = Library(ORACLE_SO, "oracle_",
-                       extra=[("read_face_outputs", (C.c_int, [C.c_void_p, C.c_void_p])),
-                              ("zfilter_sequence", (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32,
+                       extra=[("zfilter_sequence", (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32,
                                                               C.c_void_p, C.c_int32, C.c_void_p]))])
     return _lib
 
@@ -56,11 +55,7 @@ def zfilter_sequence(a, b, G):
 
 
 def face_outputs(h):
-    """face_output_field(row, side, 1:6) after the last connect."""
-    import numpy as np
-
-    n = h.t.n_output_faces
-    out = np.zeros((max(n, 1), 2, 6))
-    rc = library().fn["read_face_outputs"](h.h, out.ctypes.data_as(C.c_void_p))
-    assert rc == 0
-    return out[:n]
+    """face_output_field(row, side, 1:6) after the last connect (consumes
+    the recorded blocks)."""
+    blocks, _ = h.read_face_outputs()
+    return blocks[-1]

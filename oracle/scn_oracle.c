@@ -12,6 +12,8 @@
  *   TLM_SOLVER/scatter.F90:216-585        scatter, free space and Thevenin
  *   TLM_SOLVER/scatter.F90:606-661        add_shunt_admittance / series
  *   TLM_SOLVER/scatter_PML.F90:22-313     PML cell update (stages 1-13)
+ *   TLM_SOLVER/scatter_PML_material.F90:22-534  PML cell holding a material
+ *   TLM_SOLVER/TLM_set_small_to_zero.F90:34-68   small-value flush
  *   FILTER/filter_subroutines.F90:510-575 evaluate_Zfilter / timeshift
  *   TLM_SOLVER/connect.F90:180-214        face loop
  *   TLM_SOLVER/connect_normal_to_{x,y,z}.F90  face update, sources, outputs
@@ -74,6 +76,14 @@ struct oracle_handle {
   tlm_pml_parameter* pml_par;
   int32_t* pml_cell_param;
   double* pml_state; /* per PML cell: 27 words, see PS_* */
+  /* PML cells holding a (non-dispersive) material: PML_material_array_pos
+   * per PML cell (0 = none) and PML_material_data_type words, see PM_* */
+  int n_pml_mat;
+  int32_t* pml_mat_pos;
+  double* pml_mat_state;
+  /* set_small_to_zero packet: n_small_to_zero (0: off), small_to_zero_value */
+  int stz_n;
+  double stz_value;
   /* excitation */
   int n_functions;
   double *value, *value_face;
@@ -91,6 +101,8 @@ struct oracle_handle {
   /* outputs */
   double* cell_out; /* (row, 1:6) */
   double* face_out; /* (row, side, 1:6) */
+  int fring_count, fring_first;
+  double* fring; /* cap blocks of n_face_out_rows*12, one per connect */
   int n_points;
   int32_t *pt_row, *pt_comp;
   int n_freq;
@@ -98,6 +110,18 @@ struct oracle_handle {
   double* dft; /* (point, freq, 2) */
   int cap, ring_count, ring_first;
   double* ring; /* cap blocks of n_out_rows*6 */
+};
+
+/* PML_material_data_type words (PML_module.F90:131-156), without the
+ * never-read Ix_m1..Iz_m1. */
+enum {
+  PM_Ixt_m1, PM_Iyt_m1, PM_Izt_m1, PM_Vcx_m1, PM_Vcy_m1, PM_Vcz_m1, PM_Vcx, PM_Vcy, PM_Vcz,
+  PM_Vi_m1, /* 12 words */
+  PM_Vxt_m1 = PM_Vi_m1 + 12, PM_Vyt_m1, PM_Vzt_m1,
+  PM_Vyxt_m1, PM_Vzxt_m1, PM_Vxyt_m1, PM_Vzyt_m1, PM_Vyzt_m1, PM_Vxzt_m1,
+  PM_Vosxr, PM_Vosyr, PM_Voszr, PM_Vosx, PM_Vosy, PM_Vosz, PM_Vosx_m1, PM_Vosy_m1, PM_Vosz_m1,
+  PM_Vssxr, PM_Vssyr, PM_Vsszr, PM_Vsx, PM_Vsy, PM_Vsz, PM_Vsx_m1, PM_Vsy_m1, PM_Vsz_m1,
+  PM_N
 };
 
 /* PML_data_type words (PML_module.F90:117-129). */
@@ -140,8 +164,9 @@ void oracle_destroy(oracle_handle* h) {
   free(h->f_out); free(h->vmat); free(h->smat); free(h->eps_w); free(h->mu_w);
   free(h->eps_ord); free(h->mu_ord); free(h->pml_par); free(h->pml_cell_param);
   free(h->pml_state); free(h->value); free(h->value_face); free(h->ps);
+  free(h->pml_mat_pos); free(h->pml_mat_state);
   free(h->cell_exc_field); free(h->cell_exc_type); free(h->face_exc_field);
-  free(h->face_exc_type); free(h->huy); free(h->cell_out); free(h->face_out);
+  free(h->face_exc_type); free(h->huy); free(h->cell_out); free(h->face_out); free(h->fring);
   free(h->pt_row); free(h->pt_comp); free(h->freq); free(h->dft); free(h->ring);
   free(h);
 }
@@ -163,6 +188,8 @@ int oracle_set_cell_codes(oracle_handle* h, const int32_t* code, int32_t n_speci
                           const int32_t* to_excitation, const int32_t* to_output,
                           const int32_t* to_cable) {
   size_t N = (size_t)h->nx * h->ny * h->nz;
+  free(h->pml_mat_pos); free(h->pml_mat_state);
+  h->pml_mat_pos = NULL; h->pml_mat_state = NULL; h->n_pml_mat = 0;
   for (int32_t i = 0; i < n_special; i++)
     if (to_cable && to_cable[i]) return fail(h, TLM_ERR_UNSUPPORTED, "cable cells stay on the host path");
   h->cell_code = dup_i32(code, N);
@@ -198,6 +225,7 @@ int oracle_set_face_codes(oracle_handle* h, const int32_t* code, int32_t n_speci
   for (int32_t i = 0; i < n_special; i++) no += h->f_out[i] != 0;
   h->n_face_out_rows = no;
   free(h->face_out);
+  free(h->fring); h->fring = NULL; h->fring_count = 0;
   h->face_out = (double*)calloc((size_t)(no ? no : 1) * 12, sizeof(double));
   return TLM_OK;
 }
@@ -235,6 +263,8 @@ static int alloc_filters(oracle_handle* h) {
 
 int oracle_set_volume_materials(oracle_handle* h, int32_t n, const tlm_volume_material* m) {
   free(h->vmat);
+  free(h->pml_mat_pos); free(h->pml_mat_state);
+  h->pml_mat_pos = NULL; h->pml_mat_state = NULL; h->n_pml_mat = 0;
   h->n_vmat = n;
   h->vmat = (tlm_volume_material*)malloc((size_t)(n ? n : 1) * sizeof *m);
   if (n) memcpy(h->vmat, m, (size_t)n * sizeof *m);
@@ -259,6 +289,8 @@ int oracle_set_surface_materials(oracle_handle* h, int32_t n, const int32_t* typ
 int oracle_set_pml(oracle_handle* h, int32_t n_params, const tlm_pml_parameter* p,
                    int32_t n_cells, const int32_t* cell_param) {
   free(h->pml_par); free(h->pml_cell_param); free(h->pml_state);
+  free(h->pml_mat_pos); free(h->pml_mat_state);
+  h->pml_mat_pos = NULL; h->pml_mat_state = NULL; h->n_pml_mat = 0;
   h->n_pml_params = n_params;
   h->pml_par = (tlm_pml_parameter*)malloc((size_t)(n_params ? n_params : 1) * sizeof *p);
   if (n_params) memcpy(h->pml_par, p, (size_t)n_params * sizeof *p);
@@ -620,6 +652,234 @@ static void scatter_pml_cell(oracle_handle* h, double* v, int PML_cell, int outp
   v[Vynz] = v[Vynz] * exp_yr; v[Vypz] = v[Vypz] * exp_yr;
 }
 
+/* allocate_PML_material_data (TLM_PML_volume_materials.F90:597-746):
+ * number the PML cells holding a DISPERSIVE-type material in scan order,
+ * zero their state. */
+static int setup_pml_materials(oracle_handle* h) {
+  if (h->pml_mat_pos) return TLM_OK;
+  h->pml_mat_pos = (int32_t*)calloc((size_t)(h->n_pml_cells ? h->n_pml_cells : 1), sizeof(int32_t));
+  long long N = (long long)h->nx * h->ny * h->nz;
+  int n = 0;
+  for (long long c = 0; c < N; c++) {
+    int code = h->cell_code[c];
+    if (!code) continue;
+    int PML_cell = h->c_pml[code - 1];
+    if (!PML_cell) continue;
+    int m = h->c_mat[code - 1];
+    if (m && h->vmat[m - 1].type == TLM_VMAT_DISPERSIVE) {
+      /* set_update_codes.F90:300-333: only order-0 (non-dispersive) S-filters
+       * are allowed in the PML; their stub Z-filters are of order <= 1 */
+      if (h->vmat[m - 1].eps_order > 1 || h->vmat[m - 1].mu_order > 1)
+        return fail(h, TLM_ERR_ARG, "PML fatal error: dispersive material found");
+      h->pml_mat_pos[PML_cell - 1] = ++n;
+    }
+  }
+  h->n_pml_mat = n;
+  h->pml_mat_state = (double*)calloc((size_t)(n ? n : 1) * PM_N, sizeof(double));
+  return TLM_OK;
+}
+
+/* scatter_PML_material.F90:22-534, a PML cell holding a non-dispersive
+ * material.  The two-step recursion coefficients (STAGE 1) depend only on
+ * the PML parameter row and the material; they are recomputed here per cell
+ * as the reference does.  PML_cell_data%Ix..Iz, which this update stores
+ * from stale scatter locals and nothing reads, are not kept. */
+static void scatter_pml_material_cell(oracle_handle* h, double* v, int PML_cell, int material_number,
+                                      int output_number, int* output_cell_number) {
+  const double dl = h->g.dl, dt = h->g.dt;
+  const tlm_volume_material* vm = &h->vmat[material_number - 1];
+  /* STAGE 1 */
+  double Ysx = vm->Ys_eps_f / Y0, Ysy = vm->Ys_eps_f / Y0, Ysz = vm->Ys_eps_f / Y0;
+  double Gex = vm->Ge / Y0, Gey = vm->Ge / Y0, Gez = vm->Ge / Y0;
+  double Zsx = vm->Zs_mu_f, Zsy = vm->Zs_mu_f, Zsz = vm->Zs_mu_f;
+  double Rmx = vm->Rm, Rmy = vm->Rm, Rmz = vm->Rm;
+  const tlm_pml_parameter* P = &h->pml_par[h->pml_cell_param[PML_cell - 1] - 1];
+  double sx = P->sx, sy = P->sy, sz = P->sz;
+  double exp_xi = P->exp_x, exp_yi = P->exp_y, exp_zi = P->exp_z;
+  double exp_xr = P->exp_x, exp_yr = P->exp_y, exp_zr = P->exp_z;
+  double GpY2;
+  GpY2 = (Gex + Ysx) / 2.0;
+  double Ax0 = 8.0 + 4.0 * GpY2 + 2.0 * dt * (sy + sz) * (1.0 + GpY2) + dt * dt * sy * sz * GpY2;
+  double Ax1 = -2.0 * (8.0 + 4.0 * GpY2) + 2.0 * (dt * dt * sy * sz * GpY2);
+  double Ax2 = 8.0 + 4.0 * GpY2 - 2.0 * dt * (sy + sz) * (1.0 + GpY2) + dt * dt * sy * sz * GpY2;
+  GpY2 = (Gey + Ysy) / 2.0;
+  double Ay0 = 8.0 + 4.0 * GpY2 + 2.0 * dt * (sz + sx) * (1.0 + GpY2) + dt * dt * sz * sx * GpY2;
+  double Ay1 = -2.0 * (8.0 + 4.0 * GpY2) + 2.0 * (dt * dt * sz * sx * GpY2);
+  double Ay2 = 8.0 + 4.0 * GpY2 - 2.0 * dt * (sz + sx) * (1.0 + GpY2) + dt * dt * sz * sx * GpY2;
+  GpY2 = (Gez + Ysz) / 2.0;
+  double Az0 = 8.0 + 4.0 * GpY2 + 2.0 * dt * (sx + sy) * (1.0 + GpY2) + dt * dt * sx * sy * GpY2;
+  double Az1 = -2.0 * (8.0 + 4.0 * GpY2) + 2.0 * (dt * dt * sx * sy * GpY2);
+  double Az2 = 8.0 + 4.0 * GpY2 - 2.0 * dt * (sx + sy) * (1.0 + GpY2) + dt * dt * sx * sy * GpY2;
+  double Bx0 = 4.0 + 2.0 * dt * sy, Bx1 = -8.0, Bx2 = 4.0 - 2.0 * dt * sy;
+  double By0 = 4.0 + 2.0 * dt * sz, By1 = -8.0, By2 = 4.0 - 2.0 * dt * sz;
+  double Bz0 = 4.0 + 2.0 * dt * sx, Bz1 = -8.0, Bz2 = 4.0 - 2.0 * dt * sx;
+  double Cx0 = 4.0 + 2.0 * dt * sz, Cx1 = -8.0, Cx2 = 4.0 - 2.0 * dt * sz;
+  double Cy0 = 4.0 + 2.0 * dt * sx, Cy1 = -8.0, Cy2 = 4.0 - 2.0 * dt * sx;
+  double Cz0 = 4.0 + 2.0 * dt * sy, Cz1 = -8.0, Cz2 = 4.0 - 2.0 * dt * sy;
+  double Dx0 = 4.0 * Ysx + 2.0 * dt * (sy + sz) * Ysx + 1.0 * (dt * dt * sy * sz * Ysx);
+  double Dx1 = -8.0 * Ysx + 2.0 * (dt * dt * sy * sz * Ysx);
+  double Dx2 = 4.0 * Ysx - 2.0 * dt * (sy + sz) * Ysx + 1.0 * (dt * dt * sy * sz * Ysx);
+  double Dy0 = 4.0 * Ysy + 2.0 * dt * (sz + sx) * Ysy + 1.0 * (dt * dt * sz * sx * Ysy);
+  double Dy1 = -8.0 * Ysy + 2.0 * (dt * dt * sz * sx * Ysy);
+  double Dy2 = 4.0 * Ysy - 2.0 * dt * (sz + sx) * Ysy + 1.0 * (dt * dt * sz * sx * Ysy);
+  double Dz0 = 4.0 * Ysz + 2.0 * dt * (sx + sy) * Ysz + 1.0 * (dt * dt * sx * sy * Ysz);
+  double Dz1 = -8.0 * Ysz + 2.0 * (dt * dt * sx * sy * Ysz);
+  double Dz2 = 4.0 * Ysz - 2.0 * dt * (sx + sy) * Ysz + 1.0 * (dt * dt * sx * sy * Ysz);
+  double ZpR;
+  ZpR = Zsx + Rmx;
+  double Sx0 = 16.0 * Z0 + 4.0 * ZpR + 2.0 * dt * (sy + sz) * (2.0 * Z0 + ZpR) + dt * dt * sy * sz * ZpR;
+  double Sx1 = -32.0 * Z0 - 8.0 * ZpR + 2.0 * dt * dt * sy * sz * ZpR;
+  double Sx2 = 16.0 * Z0 + 4.0 * ZpR - 2.0 * dt * (sy + sz) * (2.0 * Z0 + ZpR) + dt * dt * sy * sz * ZpR;
+  ZpR = Zsy + Rmy;
+  double Sy0 = 16.0 * Z0 + 4.0 * ZpR + 2.0 * dt * (sz + sx) * (2.0 * Z0 + ZpR) + dt * dt * sz * sx * ZpR;
+  double Sy1 = -32.0 * Z0 - 8.0 * ZpR + 2.0 * dt * dt * sz * sx * ZpR;
+  double Sy2 = 16.0 * Z0 + 4.0 * ZpR - 2.0 * dt * (sz + sx) * (2.0 * Z0 + ZpR) + dt * dt * sz * sx * ZpR;
+  ZpR = Zsz + Rmz;
+  double Sz0 = 16.0 * Z0 + 4.0 * ZpR + 2.0 * dt * (sx + sy) * (2.0 * Z0 + ZpR) + dt * dt * sx * sy * ZpR;
+  double Sz1 = -32.0 * Z0 - 8.0 * ZpR + 2.0 * dt * dt * sx * sy * ZpR;
+  double Sz2 = 16.0 * Z0 + 4.0 * ZpR - 2.0 * dt * (sx + sy) * (2.0 * Z0 + ZpR) + dt * dt * sx * sy * ZpR;
+  double Tx0 = 8.0 + 4.0 * dt * sx, Tx1 = -16.0, Tx2 = 8.0 - 4.0 * dt * sx;
+  double Ty0 = 8.0 + 4.0 * dt * sy, Ty1 = -16.0, Ty2 = 8.0 - 4.0 * dt * sy;
+  double Tz0 = 8.0 + 4.0 * dt * sz, Tz1 = -16.0, Tz2 = 8.0 - 4.0 * dt * sz;
+  double Uxy0 = 2.0 + dt * sz, Uxy1 = -2.0 + dt * sz, Wxy0 = Z0 * (2.0 + dt * sy), Wxy1 = Z0 * (-2.0 + dt * sy);
+  double Uyx0 = 2.0 + dt * sz, Uyx1 = -2.0 + dt * sz, Wyx0 = Z0 * (2.0 + dt * sx), Wyx1 = Z0 * (-2.0 + dt * sx);
+  double Uyz0 = 2.0 + dt * sx, Uyz1 = -2.0 + dt * sx, Wyz0 = Z0 * (2.0 + dt * sz), Wyz1 = Z0 * (-2.0 + dt * sz);
+  double Uzy0 = 2.0 + dt * sx, Uzy1 = -2.0 + dt * sx, Wzy0 = Z0 * (2.0 + dt * sy), Wzy1 = Z0 * (-2.0 + dt * sy);
+  double Uzx0 = 2.0 + dt * sy, Uzx1 = -2.0 + dt * sy, Wzx0 = Z0 * (2.0 + dt * sx), Wzx1 = Z0 * (-2.0 + dt * sx);
+  double Uxz0 = 2.0 + dt * sy, Uxz1 = -2.0 + dt * sy, Wxz0 = Z0 * (2.0 + dt * sz), Wxz1 = Z0 * (-2.0 + dt * sz);
+  double Px0 = 4.0 + 2.0 * dt * sx, Px1 = -8.0, Px2 = 4.0 - 2.0 * dt * sx;
+  double Py0 = 4.0 + 2.0 * dt * sy, Py1 = -8.0, Py2 = 4.0 - 2.0 * dt * sy;
+  double Pz0 = 4.0 + 2.0 * dt * sz, Pz1 = -8.0, Pz2 = 4.0 - 2.0 * dt * sz;
+  double Qx0 = 4.0 * Zsx + 2.0 * dt * (sy + sz) * Zsx + 1.0 * (dt * dt * sy * sz * Zsx);
+  double Qx1 = -8.0 * Zsx + 2.0 * (dt * dt * sy * sz * Zsx);
+  double Qx2 = 4.0 * Zsx - 2.0 * dt * (sy + sz) * Zsx + 1.0 * (dt * dt * sy * sz * Zsx);
+  double Qy0 = 4.0 * Zsy + 2.0 * dt * (sz + sx) * Zsy + 1.0 * (dt * dt * sz * sx * Zsy);
+  double Qy1 = -8.0 * Zsy + 2.0 * (dt * dt * sz * sx * Zsy);
+  double Qy2 = 4.0 * Zsy - 2.0 * dt * (sz + sx) * Zsy + 1.0 * (dt * dt * sz * sx * Zsy);
+  double Qz0 = 4.0 * Zsz + 2.0 * dt * (sx + sy) * Zsz + 1.0 * (dt * dt * sx * sy * Zsz);
+  double Qz1 = -8.0 * Zsz + 2.0 * (dt * dt * sx * sy * Zsz);
+  double Qz2 = 4.0 * Zsz - 2.0 * dt * (sx + sy) * Zsz + 1.0 * (dt * dt * sx * sy * Zsz);
+  /* STAGE 2 */
+  double* S = &h->pml_state[(size_t)(PML_cell - 1) * PS_N];
+  double* M = &h->pml_mat_state[(size_t)(h->pml_mat_pos[PML_cell - 1] - 1) * PM_N];
+  double Ixt_m1 = S[PS_Ixt], Iyt_m1 = S[PS_Iyt], Izt_m1 = S[PS_Izt];
+  double Vi_m1[13], Vi_m2[13];
+  for (int i = 1; i <= 12; i++) Vi_m1[i] = S[PS_Vi + i - 1];
+  double Vxt_m1 = S[PS_Vxt], Vyt_m1 = S[PS_Vyt], Vzt_m1 = S[PS_Vzt];
+  double Vyxt_m1 = S[PS_Vyxt], Vzxt_m1 = S[PS_Vzxt], Vxyt_m1 = S[PS_Vxyt];
+  double Vzyt_m1 = S[PS_Vzyt], Vyzt_m1 = S[PS_Vyzt], Vxzt_m1 = S[PS_Vxzt];
+  double Ixt_m2 = M[PM_Ixt_m1], Iyt_m2 = M[PM_Iyt_m1], Izt_m2 = M[PM_Izt_m1];
+  double Vcx_m2 = M[PM_Vcx_m1], Vcy_m2 = M[PM_Vcy_m1], Vcz_m2 = M[PM_Vcz_m1];
+  double Vcx_m1 = M[PM_Vcx], Vcy_m1 = M[PM_Vcy], Vcz_m1 = M[PM_Vcz];
+  for (int i = 1; i <= 12; i++) Vi_m2[i] = M[PM_Vi_m1 + i - 1];
+  double Vxt_m2 = M[PM_Vxt_m1], Vyt_m2 = M[PM_Vyt_m1], Vzt_m2 = M[PM_Vzt_m1];
+  double Vyxt_m2 = M[PM_Vyxt_m1], Vzxt_m2 = M[PM_Vzxt_m1], Vxyt_m2 = M[PM_Vxyt_m1];
+  double Vzyt_m2 = M[PM_Vzyt_m1], Vyzt_m2 = M[PM_Vyzt_m1], Vxzt_m2 = M[PM_Vxzt_m1];
+  (void)Vyxt_m2; (void)Vzxt_m2; (void)Vxyt_m2; (void)Vzyt_m2; (void)Vyzt_m2; (void)Vxzt_m2;
+  double Vosx = M[PM_Vosxr], Vosy = M[PM_Vosyr], Vosz = M[PM_Voszr];
+  double Vosx_m1 = M[PM_Vosx], Vosx_m2 = M[PM_Vosx_m1];
+  double Vosy_m1 = M[PM_Vosy], Vosy_m2 = M[PM_Vosy_m1];
+  double Vosz_m1 = M[PM_Vosz], Vosz_m2 = M[PM_Vosz_m1];
+  double Vssx = -M[PM_Vssxr], Vssy = -M[PM_Vssyr], Vssz = -M[PM_Vsszr];
+  double Vsx_m1 = M[PM_Vsx], Vsx_m2 = M[PM_Vsx_m1];
+  double Vsy_m1 = M[PM_Vsy], Vsy_m2 = M[PM_Vsy_m1];
+  double Vsz_m1 = M[PM_Vsz], Vsz_m2 = M[PM_Vsz_m1];
+  /* STAGE 3 */
+  v[Vynx] = v[Vynx] * exp_yi; v[Vypx] = v[Vypx] * exp_yi;
+  v[Vznx] = v[Vznx] * exp_zi; v[Vzpx] = v[Vzpx] * exp_zi;
+  v[Vxny] = v[Vxny] * exp_xi; v[Vxpy] = v[Vxpy] * exp_xi;
+  v[Vzny] = v[Vzny] * exp_zi; v[Vzpy] = v[Vzpy] * exp_zi;
+  v[Vxnz] = v[Vxnz] * exp_xi; v[Vxpz] = v[Vxpz] * exp_xi;
+  v[Vynz] = v[Vynz] * exp_yi; v[Vypz] = v[Vypz] * exp_yi;
+  /* STAGE 4 */
+  double Vxt = (1.0 / Ax0) * (Bx0 * (v[Vznx] + v[Vzpx]) + Cx0 * (v[Vynx] + v[Vypx]) + Dx0 * (Vosx)
+                              + Bx1 * (Vi_m1[Vznx] + Vi_m1[Vzpx]) + Cx1 * (Vi_m1[Vynx] + Vi_m1[Vypx]) + Dx1 * (Vosx_m1)
+                              + Bx2 * (Vi_m2[Vznx] + Vi_m2[Vzpx]) + Cx2 * (Vi_m2[Vynx] + Vi_m2[Vypx]) + Dx2 * (Vosx_m2)
+                              - Ax1 * Vxt_m1 - Ax2 * Vxt_m2);
+  double Vyt = (1.0 / Ay0) * (By0 * (v[Vxny] + v[Vxpy]) + Cy0 * (v[Vzny] + v[Vzpy]) + Dy0 * (Vosy)
+                              + By1 * (Vi_m1[Vxny] + Vi_m1[Vxpy]) + Cy1 * (Vi_m1[Vzny] + Vi_m1[Vzpy]) + Dy1 * (Vosy_m1)
+                              + By2 * (Vi_m2[Vxny] + Vi_m2[Vxpy]) + Cy2 * (Vi_m2[Vzny] + Vi_m2[Vzpy]) + Dy2 * (Vosy_m2)
+                              - Ay1 * Vyt_m1 - Ay2 * Vyt_m2);
+  double Vzt = (1.0 / Az0) * (Bz0 * (v[Vynz] + v[Vypz]) + Cz0 * (v[Vxnz] + v[Vxpz]) + Dz0 * (Vosz)
+                              + Bz1 * (Vi_m1[Vynz] + Vi_m1[Vypz]) + Cz1 * (Vi_m1[Vxnz] + Vi_m1[Vxpz]) + Dz1 * (Vosz_m1)
+                              + Bz2 * (Vi_m2[Vynz] + Vi_m2[Vypz]) + Cz2 * (Vi_m2[Vxnz] + Vi_m2[Vxpz]) + Dz2 * (Vosz_m2)
+                              - Az1 * Vzt_m1 - Az2 * Vzt_m2);
+  /* STAGE 5 */
+  double Vcx = v[Vypz] - v[Vzpy] - v[Vynz] + v[Vzny] - Vssx;
+  double Vcy = v[Vzpx] - v[Vxpz] - v[Vznx] + v[Vxnz] - Vssy;
+  double Vcz = v[Vxpy] - v[Vypx] - v[Vxny] + v[Vynx] - Vssz;
+  double Ixt = (1.0 / Sx0) * (Tx0 * Vcx + Tx1 * Vcx_m1 + Tx2 * Vcx_m2 - Sx1 * Ixt_m1 - Sx2 * Ixt_m2);
+  double Iyt = (1.0 / Sy0) * (Ty0 * Vcy + Ty1 * Vcy_m1 + Ty2 * Vcy_m2 - Sy1 * Iyt_m1 - Sy2 * Iyt_m2);
+  double Izt = (1.0 / Sz0) * (Tz0 * Vcz + Tz1 * Vcz_m1 + Tz2 * Vcz_m2 - Sz1 * Izt_m1 - Sz2 * Izt_m2);
+  double Vxyt = (1.0 / Uxy0) * (Wxy0 * Izt + Wxy1 * Izt_m1 - Uxy1 * Vxyt_m1);
+  double Vyxt = (1.0 / Uyx0) * (Wyx0 * Izt + Wyx1 * Izt_m1 - Uyx1 * Vyxt_m1);
+  double Vyzt = (1.0 / Uyz0) * (Wyz0 * Ixt + Wyz1 * Ixt_m1 - Uyz1 * Vyzt_m1);
+  double Vzyt = (1.0 / Uzy0) * (Wzy0 * Ixt + Wzy1 * Ixt_m1 - Uzy1 * Vzyt_m1);
+  double Vxzt = (1.0 / Uxz0) * (Wxz0 * Iyt + Wxz1 * Iyt_m1 - Uxz1 * Vxzt_m1);
+  double Vzxt = (1.0 / Uzx0) * (Wzx0 * Iyt + Wzx1 * Iyt_m1 - Uzx1 * Vzxt_m1);
+  /* STAGE 9: the previous step's values move to the _m1 slots */
+  for (int i = 1; i <= 12; i++) { M[PM_Vi_m1 + i - 1] = S[PS_Vi + i - 1]; S[PS_Vi + i - 1] = v[i]; }
+  M[PM_Vxt_m1] = S[PS_Vxt]; M[PM_Vyt_m1] = S[PS_Vyt]; M[PM_Vzt_m1] = S[PS_Vzt];
+  S[PS_Vxt] = Vxt; S[PS_Vyt] = Vyt; S[PS_Vzt] = Vzt;
+  M[PM_Ixt_m1] = S[PS_Ixt]; M[PM_Iyt_m1] = S[PS_Iyt]; M[PM_Izt_m1] = S[PS_Izt];
+  S[PS_Ixt] = Ixt; S[PS_Iyt] = Iyt; S[PS_Izt] = Izt;
+  M[PM_Vcx_m1] = M[PM_Vcx]; M[PM_Vcy_m1] = M[PM_Vcy]; M[PM_Vcz_m1] = M[PM_Vcz];
+  M[PM_Vcx] = Vcx; M[PM_Vcy] = Vcy; M[PM_Vcz] = Vcz;
+  M[PM_Vxyt_m1] = S[PS_Vxyt]; M[PM_Vyxt_m1] = S[PS_Vyxt];
+  S[PS_Vxyt] = Vxyt; S[PS_Vyxt] = Vyxt;
+  M[PM_Vzxt_m1] = S[PS_Vzxt]; M[PM_Vxzt_m1] = S[PS_Vxzt];
+  S[PS_Vzxt] = Vzxt; S[PS_Vxzt] = Vxzt;
+  M[PM_Vzyt_m1] = S[PS_Vzyt]; M[PM_Vyzt_m1] = S[PS_Vyzt];
+  S[PS_Vzyt] = Vzyt; S[PS_Vyzt] = Vyzt;
+  /* STAGE 11 */
+  double V_max, V_min;
+  V_max = v[Vypx]; V_min = v[Vynx];
+  v[Vynx] = Vxt - Vyxt - V_max;
+  v[Vypx] = Vxt + Vyxt - V_min;
+  V_max = v[Vzpx]; V_min = v[Vznx];
+  v[Vznx] = Vxt + Vzxt - V_max;
+  v[Vzpx] = Vxt - Vzxt - V_min;
+  V_max = v[Vzpy]; V_min = v[Vzny];
+  v[Vzny] = Vyt - Vzyt - V_max;
+  v[Vzpy] = Vyt + Vzyt - V_min;
+  V_max = v[Vxpy]; V_min = v[Vxny];
+  v[Vxny] = Vyt + Vxyt - V_max;
+  v[Vxpy] = Vyt - Vxyt - V_min;
+  V_max = v[Vxpz]; V_min = v[Vxnz];
+  v[Vxnz] = Vzt - Vxzt - V_max;
+  v[Vxpz] = Vzt + Vxzt - V_min;
+  V_max = v[Vypz]; V_min = v[Vynz];
+  v[Vynz] = Vzt + Vyzt - V_max;
+  v[Vypz] = Vzt - Vyzt - V_min;
+  /* capacitive stub */
+  M[PM_Vosxr] = Vxt - Vosx; M[PM_Vosyr] = Vyt - Vosy; M[PM_Voszr] = Vzt - Vosz;
+  M[PM_Vosx_m1] = M[PM_Vosx]; M[PM_Vosx] = Vosx;
+  M[PM_Vosy_m1] = M[PM_Vosy]; M[PM_Vosy] = Vosy;
+  M[PM_Vosz_m1] = M[PM_Vosz]; M[PM_Vosz] = Vosz;
+  /* inductive stub */
+  double Vsx = (1.0 / Px0) * (Qx0 * Ixt + Qx1 * Ixt_m1 + Qx2 * Ixt_m2 - Px1 * Vsx_m1 - Px2 * Vsx_m2);
+  double Vsy = (1.0 / Py0) * (Qy0 * Iyt + Qy1 * Iyt_m1 + Qy2 * Iyt_m2 - Py1 * Vsy_m1 - Py2 * Vsy_m2);
+  double Vsz = (1.0 / Pz0) * (Qz0 * Izt + Qz1 * Izt_m1 + Qz2 * Izt_m2 - Pz1 * Vsz_m1 - Pz2 * Vsz_m2);
+  M[PM_Vssxr] = (Vsx + Vssx); M[PM_Vssyr] = (Vsy + Vssy); M[PM_Vsszr] = (Vsz + Vssz);
+  M[PM_Vsx_m1] = M[PM_Vsx]; M[PM_Vsx] = Vsx;
+  M[PM_Vsy_m1] = M[PM_Vsy]; M[PM_Vsy] = Vsy;
+  M[PM_Vsz_m1] = M[PM_Vsz]; M[PM_Vsz] = Vsz;
+  /* STAGE 12 */
+  if (output_number != 0) {
+    *output_cell_number += 1;
+    double* fo = &h->cell_out[(size_t)(*output_cell_number - 1) * 6];
+    fo[0] = -Vxt / dl; fo[1] = -Vyt / dl; fo[2] = -Vzt / dl;
+    fo[3] = Ixt / dl; fo[4] = Iyt / dl; fo[5] = Izt / dl;
+  }
+  /* STAGE 13 */
+  v[Vynx] = v[Vynx] * exp_yr; v[Vypx] = v[Vypx] * exp_yr;
+  v[Vznx] = v[Vznx] * exp_zr; v[Vzpx] = v[Vzpx] * exp_zr;
+  v[Vxny] = v[Vxny] * exp_xr; v[Vxpy] = v[Vxpy] * exp_xr;
+  v[Vzny] = v[Vzny] * exp_zr; v[Vzpy] = v[Vzpy] * exp_zr;
+  v[Vxnz] = v[Vxnz] * exp_xr; v[Vxpz] = v[Vxpz] * exp_xr;
+  v[Vynz] = v[Vynz] * exp_yr; v[Vypz] = v[Vypz] * exp_yr;
+}
+
 static void record_outputs(oracle_handle* h, int timestep) {
   /* cell_output(): copy this step's cell_output_field into the ring and
    * accumulate the probe DFT at time=(timestep-1)*dt (run_TLM.F90:240). */
@@ -653,6 +913,17 @@ int oracle_scatter(oracle_handle* h, int32_t timestep) {
   int nx = h->nx, ny = h->ny, nz = h->nz;
   int cell_number = 0, output_cell_number = 0, excitation_cell_number = 0;
   if (!h->cell_code) return fail(h, TLM_ERR_STATE, "cell codes not set");
+  /* TLM_set_small_to_zero.F90:34-68 at the top of a step whose number is a
+   * multiple of n_small_to_zero (run_TLM.F90:236-238); excitation() does
+   * not touch V, so doing it here, before the scatter, is the same */
+  if (h->stz_n > 0 && timestep % h->stz_n == 0)
+    for (int cz = 1; cz <= nz; cz++)
+      for (int cy = 1; cy <= ny; cy++)
+        for (int cx = 1; cx <= nx; cx++) {
+          double* v = cellV(h, cx, cy, cz);
+          for (int i = 1; i <= 12; i++)
+            if (fabs(v[i]) < h->stz_value) v[i] = 0.0;
+        }
   for (int cz = 1; cz <= nz; cz++)
     for (int cy = 1; cy <= ny; cy++)
       for (int cx = 1; cx <= nx; cx++) {
@@ -677,8 +948,12 @@ int oracle_scatter(oracle_handle* h, int32_t timestep) {
         if (PML_cell != 0) {
           if (material_type == TLM_VMAT_PEC || material_type == TLM_VMAT_PMC)
             return fail(h, TLM_ERR_ARG, "PEC/PMC volume material in PML");
-          if (material_type == TLM_VMAT_DISPERSIVE)
-            return fail(h, TLM_ERR_UNSUPPORTED, "material inside the PML is not supported");
+          if (material_type == TLM_VMAT_DISPERSIVE) {
+            int rc = setup_pml_materials(h);
+            if (rc) return rc;
+            scatter_pml_material_cell(h, v, PML_cell, material_number, output_number, &output_cell_number);
+            continue;
+          }
           scatter_pml_cell(h, v, PML_cell, output_number, &output_cell_number);
           continue;
         }
@@ -895,7 +1170,6 @@ static int connect_special_face(oracle_handle* h, int axis, double* vmin_cell, d
 }
 
 int oracle_connect(oracle_handle* h, int32_t timestep) {
-  (void)timestep;
   int nx = h->nx, ny = h->ny, nz = h->nz;
   int face_number = 0, excitation_face_number = 0, output_face_number = 0;
   if (!h->face_code) return fail(h, TLM_ERR_STATE, "face codes not set");
@@ -952,6 +1226,17 @@ int oracle_connect(oracle_handle* h, int32_t timestep) {
           }
         }
       }
+  /* face_output_points (output.F90:232-272): this connect's rows, one block
+   * per step, kept for reading */
+  if (h->n_face_out_rows) {
+    size_t w = (size_t)h->n_face_out_rows * 12;
+    if (!h->fring) h->fring = (double*)malloc((size_t)h->cap * w * sizeof(double));
+    if (h->fring_count == 0) h->fring_first = timestep;
+    if (h->fring_count < h->cap) {
+      memcpy(h->fring + (size_t)h->fring_count * w, h->face_out, w * sizeof(double));
+      h->fring_count++;
+    }
+  }
   return TLM_OK;
 }
 
@@ -1035,9 +1320,18 @@ int oracle_read_dft(oracle_handle* h, double* re_im) {
   return TLM_OK;
 }
 
-int oracle_read_face_outputs(oracle_handle* h, double* out) {
-  size_t n = (size_t)h->n_face_out_rows * 12;
-  if (n) memcpy(out, h->face_out, n * sizeof(double));
+int oracle_read_face_outputs(oracle_handle* h, double* out, int32_t max_steps, int32_t* n_read,
+                             int32_t* first_step) {
+  size_t w = (size_t)h->n_face_out_rows * 12;
+  int n = h->fring_count < max_steps ? h->fring_count : max_steps;
+  if (n > 0 && w) memcpy(out, h->fring, (size_t)n * w * sizeof(double));
+  if (n_read) *n_read = n;
+  if (first_step) *first_step = h->fring_first;
+  if (n < h->fring_count) {
+    memmove(h->fring, h->fring + (size_t)n * w, (size_t)(h->fring_count - n) * w * sizeof(double));
+    h->fring_first += n;
+  }
+  h->fring_count -= n;
   return TLM_OK;
 }
 
@@ -1048,6 +1342,13 @@ int oracle_set_schedule(oracle_handle* h, int32_t variant, int32_t rows, int32_t
   return TLM_OK;
 }
 int oracle_set_profiling(oracle_handle* h, int32_t enable) { (void)h; (void)enable; return TLM_OK; }
+
+int oracle_set_small_to_zero(oracle_handle* h, int32_t n_small_to_zero, double small_to_zero_value) {
+  if (n_small_to_zero < 0 || !(small_to_zero_value >= 0.0)) return fail(h, TLM_ERR_ARG, "bad small_to_zero data");
+  h->stz_n = n_small_to_zero;
+  h->stz_value = small_to_zero_value;
+  return TLM_OK;
+}
 int oracle_read_kernel_times(oracle_handle* h, int64_t* launches, double* ms) {
   (void)h;
   for (int k = 0; k < TLM_N_KCLASS; k++) { launches[k] = 0; ms[k] = 0.0; }

@@ -177,3 +177,50 @@ def special_mix(n=14, steps=50):
                    outputs=[OutputPoint((8, 8, 8), k) for k in ("ex", "ey", "ez", "hx", "hy", "hz")] +
                            [OutputPoint((9, 9, 3), "ex"), OutputPoint((2, 11, 10), "hx"),
                             OutputPoint((7, 3, 7), "ez")])
+
+
+def pml_material(n=16, steps=300, reflection=1e-3, mu_r=2.0, sigma_m=30.0, pml=True):
+    """A lossy dielectric/magnetic block (eps_r=3, sigma=0.05 S/m, mu_r,
+    sigma_m) running into a PML of uneven thickness: the PML cells of the
+    block take the PML-material update (scatter_PML_material.F90)."""
+    dl = 0.01
+    dt = dl / (2 * 2.99792458e8)
+    p = Problem(nx=n, ny=n, nz=n, dl=dl, n_timesteps=steps, reflection=(-1.0,) * 6,
+                functions=[ExcitationFunction("gaussian", (1.0, 5 * dt, 20 * dt))],
+                sources=[PointSource((8, 8, 8), "ez", 1)],
+                outputs=[OutputPoint((3, 8, 8), k) for k in ("ex", "ey", "ez", "hx", "hy", "hz")] +
+                        [OutputPoint((10, 9, 7), "ez"), OutputPoint((2, 2, 2), "hy")],
+                materials=[VolumeMaterial("dispersive", Sfilter.constant(3.0), Sfilter.constant(mu_r),
+                                          sigma_e=0.05, sigma_m=sigma_m)],
+                material_regions=[(Box(1, 5, 3, 13, 2, 14), 1)])
+    if pml:
+        p.pml = PML((4, 0, 3, 2, 0, 1), (reflection,) * 6, 2)
+    return p
+
+
+def face_outputs(n=14, steps=50):
+    """special_mix plus face output points: on a PEC face, a PMC face, a
+    free-space surface face, the face point source's face and ordinary
+    faces normal to x, y and z (face_output_field, connect_normal_to_*)."""
+    p = special_mix(n, steps)
+    p.outputs = p.outputs + [
+        OutputPoint((4, 5, 5), "ey", point="xmin"),    # PEC face
+        OutputPoint((6, 10, 7), "ex", point="ymin"),   # PMC face
+        OutputPoint((6, 6, 11), "ex", point="zmax"),   # free-space surface face
+        OutputPoint((5, 9, 6), "ez", point="xmax"),    # face source
+        OutputPoint((9, 4, 9), "hz", point="xmax"),
+        OutputPoint((3, 12, 5), "hx", point="ymin"),
+        OutputPoint((10, 10, 2), "ey", point="zmin"),
+        OutputPoint((7, 7, 9), "ez", point="zmax"),
+    ]
+    return p
+
+
+def huygens_face_outputs():
+    """pml_sphere with face outputs on Huygens faces and an ordinary face."""
+    p = pml_sphere(24, 80)
+    b = 4 + 2
+    p.outputs = p.outputs + [OutputPoint((b - 1, 12, 12), "ez", point="xmax"),
+                             OutputPoint((12, b, 11), "ez", point="ymin"),
+                             OutputPoint((12, 12, 12), "ez", point="xmin")]
+    return p

@@ -24,6 +24,7 @@ CASES = {
     # wide enough for 128-byte lines of one class only (PML slabs, sphere
     # chords, Huygens face rows) next to mixed lines
     "pml_sphere_64": lambda: problems.pml_sphere(64, 50, 8, radius=20.0),
+    "pml_material": lambda: problems.pml_material(steps=200),
     "dispersive_blocks": problems.dispersive_blocks,
     "special_mix": problems.special_mix,
     "huygens_surface": problems.huygens_surface,
@@ -63,7 +64,7 @@ def test_plain_bitwise(name, oracle_lib):
         assert np.max(np.abs(do - dg)) <= 1e-12 * scale
 
 
-@pytest.mark.parametrize("name", ["pml_sphere", "special_mix", "dispersive_blocks"])
+@pytest.mark.parametrize("name", ["pml_sphere", "special_mix", "dispersive_blocks", "pml_material"])
 def test_split_calls_match_fused_step(name, oracle_lib):
     """excitation/scatter/connect/outer_boundary one by one (the reference's
     call sites) == tlm_step, with downloads in between."""
@@ -102,7 +103,7 @@ def test_upload_then_step(oracle_lib):
 
 
 @pytest.mark.parametrize("name", ["free_cube", "pml_sphere", "pml_sphere_64", "dispersive_blocks", "special_mix",
-                                  "huygens_surface"])
+                                  "huygens_surface", "pml_material"])
 def test_precision_form(name, oracle_lib):
     from oracle import oracle
 
@@ -116,11 +117,16 @@ def test_precision_form(name, oracle_lib):
     # meaningful relative error; its norm is floored at 1e-3 of the largest
     # probe norm of the solve.
     floor = 1e-3 * max(np.linalg.norm(so[:, k]) for k in range(so.shape[1]))
+    # PML cells holding a material advance two-step recursions whose
+    # coefficients nearly cancel (4 V_n - 8 V_n-1 + 4 V_n-2 terms), so the FP32
+    # rounding of the link values is amplified there: 1e-4 for that deck
+    # (measured 2.4e-5); 1e-5 everywhere else.
+    tol = 1e-4 if name == "pml_material" else 1e-5
     for k in range(so.shape[1]):
         den = max(np.linalg.norm(so[:, k]), floor)
         rel = np.linalg.norm(sg[:, k] - so[:, k]) / den
         print(f"probe {k}: relative L2 {rel:.3e}")
-        assert rel < 1e-5, (name, k, rel)
+        assert rel < tol, (name, k, rel)
 
 
 @pytest.mark.parametrize("sched", [(0, 1, 1, 0), (0, 2, 1, 1), (1, 1, 4, 1), (1, 4, 8, 0), (1, 2, 32, 1),
@@ -138,3 +144,46 @@ def test_schedules_bitwise(sched, oracle_lib):
         ro, rg = run(o, t, steps), run(g, t, steps)
         assert np.array_equal(ro, rg)
         assert np.array_equal(o.download_links()[:, 1:-1, 1:-1], g.download_links()[:, 1:-1, 1:-1])
+
+
+@pytest.mark.parametrize("name", ["face_outputs", "huygens_face_outputs"])
+def test_face_outputs_bitwise(name, oracle_lib):
+    """face_output_field rows of every connect, plain form, bit for bit;
+    read once mid-run (which applies the pending connect) and at the end."""
+    from oracle import oracle
+
+    t = build_tables(getattr(problems, name)())
+    steps = min(t.n_timesteps, 60)
+    o = oracle.handle(t, output_capacity=steps)
+    g = engine(t, FORM_PLAIN_FP64)
+    half = steps // 2
+    o.step(1, half)
+    g.step(1, half)
+    fo1, f1 = o.read_face_outputs()
+    fg1, g1 = g.read_face_outputs()
+    o.step(half + 1, steps - half)
+    g.step(half + 1, steps - half)
+    fo2, f2 = o.read_face_outputs()
+    fg2, g2 = g.read_face_outputs()
+    assert (f1, f2) == (g1, g2) == (1, half + 1)
+    assert fo1.shape == fg1.shape and fo2.shape == fg2.shape and fo1.shape[0] == half
+    assert np.array_equal(fo1, fg1) and np.array_equal(fo2, fg2)
+    assert np.abs(fo2).max() > 0
+    ro, _ = o.read_cell_outputs()
+    rg, _ = g.read_cell_outputs()
+    assert np.array_equal(ro, rg)
+
+
+@pytest.mark.parametrize("name", ["pml_sphere", "special_mix", "pml_material"])
+def test_small_to_zero_bitwise(name, oracle_lib):
+    """TLM_set_small_to_zero every 7 steps with a threshold that bites."""
+    from oracle import oracle
+
+    t = build_tables(CASES[name]())
+    steps = min(t.n_timesteps, 60)
+    o = oracle.handle(t, output_capacity=steps)
+    g = engine(t, FORM_PLAIN_FP64)
+    o.set_small_to_zero(7, 1e-5)
+    g.set_small_to_zero(7, 1e-5)
+    assert np.array_equal(run(o, t, steps), run(g, t, steps))
+    assert np.array_equal(o.download_links()[:, 1:-1, 1:-1], g.download_links()[:, 1:-1, 1:-1])

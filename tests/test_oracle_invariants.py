@@ -158,6 +158,37 @@ def test_zero_conductivity_pml_equals_free_space(oracle_lib):
     assert np.linalg.norm(a - b) <= 1e-11 * np.linalg.norm(a)
 
 
+@pytest.mark.parametrize("mu_r,sigma_m", [(1.0, 0.0), (2.0, 30.0)])
+def test_zero_conductivity_pml_material_equals_material(oracle_lib, mu_r, sigma_m):
+    """R_pml = 1 gives sigma = 0: the PML-material update's two-step
+    recursions (scatter_PML_material.F90) reduce to the ordinary lossy
+    dielectric/magnetic cell (scatter.F90 Thevenin node with its stub
+    filters).  Their recursions have a double root at z = 1, so rounding
+    drifts slowly; 1e-9 relative L2 over 300 steps still fails for any
+    dropped or mis-signed coefficient term (those give O(1) errors)."""
+    base = problems.pml_material(mu_r=mu_r, sigma_m=sigma_m, pml=False)
+    pml = problems.pml_material(mu_r=mu_r, sigma_m=sigma_m, reflection=1.0)
+    a, b = _series(base, 300), _series(pml, 300)
+    for k in range(a.shape[1]):
+        assert np.linalg.norm(a[:, k]) > 0
+        assert np.linalg.norm(a[:, k] - b[:, k]) <= 1e-9 * np.linalg.norm(a[:, k]), k
+
+
+def test_pml_material_absorbs(oracle_lib):
+    """A zero-mean pulse in a box whose PML (R = 1e-3) holds part of a lossy
+    block: the fields stay bounded over 1200 steps and the link energy
+    decays far faster than with the sigma = 0 PML (material loss only)."""
+    energy = []
+    for r in (1e-3, 1.0):
+        p = problems.pml_material(steps=1200, reflection=r)
+        dt = p.dt
+        p.functions = [ExcitationFunction("gaussian_sinusoid", (1.0, 6 * dt, 24 * dt, 1.0 / (10 * dt), 0.0))]
+        h = oracle_handle(build_tables(p))
+        h.step(1, 1200)
+        energy.append(np.sum(links(h) ** 2))
+    assert np.isfinite(energy[0]) and energy[0] < 1e-4 * energy[1], energy
+
+
 def test_pec_and_pmc_cells(oracle_lib):
     t = build_tables(problems.special_mix())
     h = oracle_handle(t, 50)
@@ -185,6 +216,65 @@ def test_pec_face_zero_tangential_e(oracle_lib):
     assert fo.shape == (1, 2, 6)
     assert np.all(fo[0, :, 1:3] == 0.0)  # Ey, Ez both sides
     assert np.abs(fo[0, :, 4:6]).max() > 0  # surface current -> tangential H
+
+
+def test_pmc_face_zero_tangential_h(oracle_lib):
+    """A PMC surface face leaves zero tangential H on both of its sides and
+    a nonzero tangential E."""
+    from ggi_tlm_amd.constants import FACE_YMIN
+    p = problems.free_cube(10, 30)
+    p.surface_materials = ["pmc"]
+    p.surface_faces = [(np.array([[5, 7, 5, FACE_YMIN]]), 1)]
+    p.outputs.append(OutputPoint((5, 7, 5), "ex", point="ymin"))
+    t = build_tables(p)
+    h = oracle_handle(t, 30)
+    h.step(1, 30)
+    fo, first = h.read_face_outputs()
+    assert fo.shape == (30, 1, 2, 6) and first == 1
+    assert np.all(fo[:, 0, :, 3] == 0.0) and np.all(fo[:, 0, :, 5] == 0.0)  # Hx, Hz
+    assert np.abs(fo[:, 0, :, 2]).max() > 0  # Ez
+
+
+def _face_deck():
+    """Zero-mean pulse from an Ez source at the centre of a 32^3 box with
+    matched walls; cell probes either side of an x-face and a y-face six
+    cells out, and face outputs on those two faces."""
+    p = problems.free_cube(32, 160)
+    p.reflection = (0.0,) * 6
+    dt = p.dt
+    p.functions = [ExcitationFunction("gaussian_sinusoid", (1.0, 40 * dt, 200 * dt, 1.0 / (60 * dt), 0.0))]
+    p.sources = [PointSource((16, 16, 16), "ez", 1)]
+    p.outputs = [OutputPoint((22, 16, 16), k) for k in ("ez", "hy")] + \
+                [OutputPoint((23, 16, 16), k) for k in ("ez", "hy")] + \
+                [OutputPoint((16, 22, 16), k) for k in ("ez", "hx")] + \
+                [OutputPoint((16, 23, 16), k) for k in ("ez", "hx")] + \
+                [OutputPoint((23, 16, 16), "ez", point="xmin"), OutputPoint((16, 23, 16), "ez", point="ymin")]
+    return p
+
+
+def test_face_outputs_match_neighbouring_cell_fields(oracle_lib):
+    """Face fields of connect n (reference time (n-1)dt + dt/2) against the
+    mean of the two neighbouring cells' centre fields at steps n and n+1,
+    for a smooth zero-mean pulse: within 15% of the peak (TLM dispersion and
+    the half-cell offset), on both sides.  A wrong sign, side or field
+    component misses by O(1)."""
+    t = build_tables(_face_deck())
+    h = oracle_handle(t, 160)
+    h.step(1, 160)
+    cells, _ = h.read_cell_outputs()
+    faces, first = h.read_face_outputs()
+    assert first == 1 and faces.shape == (160, 2, 2, 6)
+    # cell rows in scan order: (22,16,16), (23,16,16), (16,22,16), (16,23,16)
+    for frow, crows, comps in ((0, (0, 1), (2, 4)), (1, (2, 3), (2, 3))):
+        for comp in comps:
+            a = 0.5 * (cells[:, crows[0], comp] + cells[:, crows[1], comp])
+            ref = 0.5 * (a[:-1] + a[1:])
+            for side in (0, 1):
+                err = np.abs(faces[:-1, frow, side, comp] - ref).max() / np.abs(ref).max()
+                assert err < 0.15, (frow, comp, side, err)
+        # an ordinary face: both sides see the same pulses
+        d = np.abs(faces[:, frow, 0] - faces[:, frow, 1]).max()
+        assert d <= 1e-12 * np.abs(faces[:, frow]).max(), d
 
 
 # ---- filters ------------------------------------------------------------------------
@@ -293,3 +383,41 @@ def test_table_ranks_scan_order():
     fc = t.face_code[t.face_code > 0]
     assert np.array_equal(fc, np.arange(1, t.n_special_faces + 1))
     assert t.n_face_excitation_rows == 1 and t.n_cell_excitation_rows == 3  # (7,7,7) twice, (11,6,5), (7,8,7)
+
+
+# ---- small-to-zero flush --------------------------------------------------------------
+
+def test_small_to_zero_matches_host_flush(oracle_lib):
+    """set_small_to_zero(n, v) at step k equals stopping before step k,
+    flushing |V| < v on the host (numpy) and continuing without it; a zero
+    threshold changes nothing, a huge one empties the mesh."""
+    p = problems.free_cube(12, 40)
+    t = build_tables(p)
+    thr, k = 1e-4, 20
+    a = oracle_handle(t)
+    a.set_small_to_zero(k, thr)
+    a.step(1, 30)
+    b = oracle_handle(t)
+    b.step(1, k - 1)
+    V = b.download_links()
+    inner = V[:, 1:-1, 1:-1, :]
+    assert np.count_nonzero(inner) and np.count_nonzero(np.abs(inner) < thr)
+    inner[np.abs(inner) < thr] = 0.0
+    b.upload_links(V)
+    b.step(k, 30 - k + 1)
+    assert np.array_equal(links(a), links(b))
+    c = oracle_handle(t)
+    c.set_small_to_zero(1, 0.0)
+    c.step(1, 30)
+    d = oracle_handle(t)
+    d.step(1, 30)
+    assert np.array_equal(links(c), links(d))
+    e = oracle_handle(t)
+    e.set_small_to_zero(30, 1e30)
+    e.step(1, 30)
+    # step 30 scatters an empty mesh: only the (still active) source cell
+    # radiates, so after the connect only its face neighbours hold values
+    nz = np.argwhere(links(e) != 0)
+    c = np.array(p.sources[0].cell) - 1
+    assert 0 < len(nz) <= 12
+    assert np.all(np.abs(nz[:, 1:] - c).sum(axis=1) == 1)
