@@ -17,7 +17,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SO = os.path.join(HERE, "libtlmscn.so")
-SRC = os.path.join(HERE, "csrc", "tlmscn.hip")
+SRC = [os.path.join(HERE, "csrc", "tlmscn.hip"), os.path.join(HERE, "csrc", "batch.hip")]
 ARCH = os.environ.get("TLM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -37,7 +37,7 @@ def hipcc() -> str:
 
 def build_engine(verbose: bool = False) -> str:
     tmp = SO + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), SRC, "-o", tmp]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), *SRC, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

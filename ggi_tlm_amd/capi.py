@@ -73,6 +73,22 @@ KERNEL_CLASSES = ("bulk", "pml", "generic", "dft", "material", "mixed", "pml_mat
 
 EXPORTED = tuple("tlm_" + n for n in SIGNATURES)
 
+# include/tlmbatch.h (engine only; the oracle has no batch)
+BATCH_SIGNATURES = {
+    "batch_create": (C.c_int, [_i32, C.c_int64, C.POINTER(C.c_void_p)]),
+    "batch_destroy": (None, [_h]),
+    "batch_last_error": (C.c_char_p, [_h]),
+    "batch_add": (C.c_int, [_h, _h, _i32, _i32]),
+    "batch_run": (C.c_int, [_h, _i32]),
+    "batch_collect": (C.c_int, [_h, _p, _i32, C.POINTER(_i32)]),
+    "batch_info": (C.c_int, [_h, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(C.c_int64)]),
+    "batch_set_graph_steps": (C.c_int, [_h, _i32]),
+    "batch_autotune": (C.c_int, [_h, _i32, C.POINTER(_i32), C.POINTER(_i32)]),
+}
+
+BATCH_EXPORTED = tuple("tlm_" + n for n in BATCH_SIGNATURES)
+ERR_BUDGET = 6
+
 
 class TlmError(RuntimeError):
     pass
@@ -103,7 +119,7 @@ def engine_library() -> Library:
     """The HIP engine (in-tree libtlmscn.so)."""
     global _engine
     if _engine is None:
-        _engine = Library(ENGINE_SO, "tlm_")
+        _engine = Library(ENGINE_SO, "tlm_", extra=list(BATCH_SIGNATURES.items()))
     return _engine
 
 
@@ -293,3 +309,67 @@ def probe_series(cell_outputs: np.ndarray, tables: Tables) -> np.ndarray:
     rows = tables.output_rows - 1
     comps = tables.output_components - 1
     return cell_outputs[:, rows, comps]
+
+
+class Batch:
+    """Resident solves stepped together (include/tlmbatch.h).
+
+    add() returns False when the solve does not fit the HBM budget left (the
+    caller keeps it queued); run(n) advances every unfinished solve by n
+    steps; collect() hands back the finished Handles, which then read like
+    after Handle.step()."""
+
+    def __init__(self, form: int, budget_bytes: int, lib: Optional[Library] = None):
+        self.lib = lib or engine_library()
+        self.b = C.c_void_p()
+        rc = self.lib.fn["batch_create"](int(form), int(budget_bytes), C.byref(self.b))
+        if rc != 0:
+            raise TlmError(f"tlm_batch_create failed (status {rc})")
+        self.members = {}
+
+    def _check(self, rc):
+        if rc != 0:
+            m = self.lib.fn["batch_last_error"](self.b)
+            raise TlmError(f"batch call failed (status {rc}): {m.decode() if m else ''}")
+
+    def add(self, handle: "Handle", first_timestep: int, n_steps: int) -> bool:
+        rc = self.lib.fn["batch_add"](self.b, handle.h, int(first_timestep), int(n_steps))
+        if rc == ERR_BUDGET:
+            return False
+        self._check(rc)
+        self.members[handle.h.value] = handle
+        return True
+
+    def run(self, n_steps: int):
+        self._check(self.lib.fn["batch_run"](self.b, int(n_steps)))
+
+    def collect(self):
+        n = max(len(self.members), 1)
+        arr = (C.c_void_p * n)()
+        got = _i32(0)
+        self._check(self.lib.fn["batch_collect"](self.b, arr, n, C.byref(got)))
+        return [self.members.pop(arr[i]) for i in range(got.value)]
+
+    def info(self):
+        n, u, r = _i32(0), _i32(0), C.c_int64(0)
+        self._check(self.lib.fn["batch_info"](self.b, C.byref(n), C.byref(u), C.byref(r)))
+        return n.value, u.value, r.value
+
+    def set_graph_steps(self, k: int):
+        self._check(self.lib.fn["batch_set_graph_steps"](self.b, int(k)))
+
+    def autotune(self, steps_per_candidate: int):
+        k, done = _i32(0), _i32(0)
+        self._check(self.lib.fn["batch_autotune"](self.b, int(steps_per_candidate), C.byref(k), C.byref(done)))
+        return k.value, done.value
+
+    def close(self):
+        if self.b:
+            self.lib.fn["batch_destroy"](self.b)
+            self.b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,547 @@
+// batch.hip -- many resident solves stepped together (continuous batching).
+//
+// A tlm_batch holds solves of any grid size, each a configured tlm_handle
+// whose device tables and link planes stay resident in HBM.  One step of
+// the batch is one launch per kernel class over the union of the solves'
+// work: a block table maps every workgroup to (solve, block of that solve),
+// and each solve's timestep, ping-pong parity and output slot live in a
+// device record, so the K-step launch sequence is captured once in a HIP
+// graph and replayed until the set of solves changes.  Solves that finish
+// inside a replay idle (their workgroups return at once) until collected.
+// The kernels call the same cell-update functions as the single-solve
+// path, so a solve gives the same bits alone and in any mix.
+//
+// Admission is by HBM budget: tlm_batch_add refuses a solve whose device
+// footprint would exceed the budget left, and the caller queues it.
+#include "engine.hpp"
+#include "../../include/tlmbatch.h"
+
+#include <chrono>
+
+using namespace engine;
+
+namespace engine {
+
+// Device record of one resident solve.
+struct BatchSolve {
+  Dev d;                     // the solve's tables (d.ring: its whole output series)
+  void* planes[2];
+  const int32_t* slot_a;     // mixed-line workgroup slots
+  const int32_t* slot_b;
+  long long n_slots;
+  int n_pure_pml, n_pure_mat, n_pure_gen, n_pm, n_huy_coded;
+  int cur;                   // plane set holding the last step's result
+  int ts, first, last;       // next timestep, first and last timestep
+  int stz_n;                 // small-to-zero period (0: off)
+  double stz_value;
+  int np, nf;                // probe DFT
+  const int32_t* prow;
+  const int32_t* pcomp;
+  const double* freq;
+  double* dft;
+  double dt;
+};
+
+}  // namespace engine
+
+struct tlm_batch {
+  struct Entry {
+    tlm_handle* h = nullptr;
+    int first = 0, last = 0, ts = 0;  // host copy of the device timestep
+    int cur0 = 0;
+    double* ring = nullptr;
+    int64_t bytes = 0;
+  };
+  int form = TLM_FORM_PRECISION_FP32;
+  int64_t budget = 0, resident = 0;
+  std::string err;
+  hipStream_t stream = nullptr;
+  std::vector<Entry> solves;
+  engine::BatchSolve* d_solves = nullptr;
+  int d_cap = 0;
+  // block tables: lines, mixed, pml, material, generic, pml-material, huygens, dft
+  enum { T_LINES, T_MIXED, T_PML, T_MAT, T_GEN, T_PM, T_HUY, T_DFT, T_N };
+  int2* d_tab[T_N] = {};
+  int n_tab[T_N] = {};
+  size_t cap_tab[T_N] = {};
+  bool dirty = true;
+  int graph_steps = 32;
+  hipGraphExec_t graph = nullptr;
+  int graph_k = 0;
+  double last_ms_per_step = 0.0;
+};
+
+namespace engine {
+
+int bfail(tlm_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  return code;
+}
+
+#define BHIP_TRY(b, expr)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) return bfail((b), TLM_ERR_HIP, std::string(#expr) + ": " + \
+                                                             hipGetErrorString(e_));   \
+  } while (0)
+
+// Upper bound of a solve's device bytes before its tables are built: two
+// link buffers, class tables, PML and filter states, output series.
+int64_t estimate_bytes(const tlm_handle* h, int n_steps) {
+  const int64_t ls = link_size(h);
+  int64_t b = 2 * 12 * (int64_t)h->P * ls;               // link planes
+  b += (int64_t)h->N / 8 + (int64_t)h->N / 8;            // mask, line table
+  const int64_t n_pml = (int64_t)h->pml_cell_param.size();
+  b += n_pml * (8 * (scn::PS_N + 3) + 12);               // PML state and lists
+  b += (int64_t)h->n_special * 64;                       // class lists, slots
+  b += (int64_t)h->n_special * 2 * 3 * 8 * scn::NW;      // filter states (bound)
+  b += (int64_t)h->n_special_faces * 48;
+  int64_t out_rows = 0;
+  for (int32_t o : h->c_out) out_rows += o != 0;
+  b += (int64_t)n_steps * out_rows * 6 * 8;              // output series
+  return b + (1 << 20);
+}
+
+template <typename T, bool PLAIN>
+__device__ inline void batch_pull_args(const BatchSolve& s, PullArgs& a, bool& pull, int& slot) {
+  pull = s.ts > s.first;
+  a.apply_r = true;
+  a.ts_connect = s.ts - 1;
+  a.small = (s.stz_n > 0 && s.ts % s.stz_n == 0) ? s.stz_value : 0.0;
+  a.fslot = -1;
+  slot = s.ts - s.first;
+}
+
+template <typename T, bool PLAIN>
+__global__ __launch_bounds__(256) void kb_lines(const BatchSolve* __restrict__ S, const int2* __restrict__ tab) {
+  const int2 e = tab[blockIdx.x];
+  const BatchSolve& s = S[e.x];
+  if (s.ts > s.last) return;
+  const Dev& d = s.d;
+  const int nbx = (d.nx + 255) >> 8;
+  const int row = e.y / nbx;
+  const int x = (e.y - row * nbx) * 256 + threadIdx.x;
+  if (x >= d.nx) return;
+  const long long c = x + (long long)d.nx * row;
+  if (d.line_info[c / d.line_len] != 0u) return;
+  const int z = row / d.ny;
+  const int y = row - z * d.ny;
+  PullArgs a;
+  bool pull;
+  int slot;
+  batch_pull_args<T, PLAIN>(s, a, pull, slot);
+  const T* Lin = (const T*)s.planes[s.cur];
+  T* Lout = (T*)s.planes[s.cur ^ 1];
+  if (pull) scn::bulk_update<T, PLAIN, true>(d, Lin, Lout, a, c, x, y, z);
+  else scn::bulk_update<T, PLAIN, false>(d, Lin, Lout, a, c, x, y, z);
+}
+
+template <typename T, bool PLAIN, bool PULL>
+__device__ inline void mixed_lane(const BatchSolve& s, const T* Lin, T* Lout, const PullArgs& a, int slot,
+                                  long long si) {
+  const Dev& d = s.d;
+  const int32_t w = s.slot_a[si];
+  if (w < 0) return;
+  const long long c = w & scn::kCellMask;
+  const int cls = (w >> 28) & 3;
+  if (cls == 0) {
+    long long cs[scn::kMixedBulkPerLane];
+    cs[0] = c;
+#pragma unroll
+    for (int k = 1; k < scn::kMixedBulkPerLane; k++) cs[k] = s.slot_b[(k - 1) * s.n_slots + si];
+    scn::bulk_update_n<T, PLAIN, PULL, scn::kMixedBulkPerLane>(d, Lin, Lout, a, cs);
+    return;
+  }
+  const int i = s.slot_b[si];
+  if (cls == 1) scn::pml_update<T, PLAIN, PULL>(d, Lin, Lout, a, slot, i);
+  else if (cls == 2) scn::material_update<T, PLAIN, PULL>(d, Lin, Lout, a, i);
+  else scn::generic_update<T, PLAIN, PULL>(d, Lin, Lout, a, s.ts, slot, i);
+}
+
+template <typename T, bool PLAIN>
+__global__ __launch_bounds__(256) void kb_mixed(const BatchSolve* __restrict__ S, const int2* __restrict__ tab) {
+  const int2 e = tab[blockIdx.x];
+  const BatchSolve& s = S[e.x];
+  if (s.ts > s.last) return;
+  PullArgs a;
+  bool pull;
+  int slot;
+  batch_pull_args<T, PLAIN>(s, a, pull, slot);
+  const T* Lin = (const T*)s.planes[s.cur];
+  T* Lout = (T*)s.planes[s.cur ^ 1];
+  const long long si = (long long)e.y * 256 + threadIdx.x;
+  if (pull) mixed_lane<T, PLAIN, true>(s, Lin, Lout, a, slot, si);
+  else mixed_lane<T, PLAIN, false>(s, Lin, Lout, a, slot, si);
+}
+
+// One class list of a solve: 0 PML, 1 material, 2 generic, 3 PML holding a material.
+template <typename T, bool PLAIN, int CLS>
+__global__ __launch_bounds__(256) void kb_list(const BatchSolve* __restrict__ S, const int2* __restrict__ tab) {
+  const int2 e = tab[blockIdx.x];
+  const BatchSolve& s = S[e.x];
+  if (s.ts > s.last) return;
+  const int i = e.y * 256 + threadIdx.x;
+  const int n = CLS == 0 ? s.n_pure_pml : CLS == 1 ? s.n_pure_mat : CLS == 2 ? s.n_pure_gen : s.n_pm;
+  if (i >= n) return;
+  PullArgs a;
+  bool pull;
+  int slot;
+  batch_pull_args<T, PLAIN>(s, a, pull, slot);
+  const Dev& d = s.d;
+  const T* Lin = (const T*)s.planes[s.cur];
+  T* Lout = (T*)s.planes[s.cur ^ 1];
+  if (pull) {
+    if (CLS == 0) scn::pml_update<T, PLAIN, true>(d, Lin, Lout, a, slot, i);
+    else if (CLS == 1) scn::material_update<T, PLAIN, true>(d, Lin, Lout, a, i);
+    else if (CLS == 2) scn::generic_update<T, PLAIN, true>(d, Lin, Lout, a, s.ts, slot, i);
+    else scn::pml_material_update<T, PLAIN, true>(d, Lin, Lout, a, slot, i);
+  } else {
+    if (CLS == 0) scn::pml_update<T, PLAIN, false>(d, Lin, Lout, a, slot, i);
+    else if (CLS == 1) scn::material_update<T, PLAIN, false>(d, Lin, Lout, a, i);
+    else if (CLS == 2) scn::generic_update<T, PLAIN, false>(d, Lin, Lout, a, s.ts, slot, i);
+    else scn::pml_material_update<T, PLAIN, false>(d, Lin, Lout, a, slot, i);
+  }
+}
+
+// Per-step Huygens values of the flattened face codes (pending connect = ts-1).
+__global__ __launch_bounds__(256) void kb_huygens(const BatchSolve* __restrict__ S, const int2* __restrict__ tab) {
+  const int2 e = tab[blockIdx.x];
+  const BatchSolve& s = S[e.x];
+  if (s.ts > s.last || s.ts <= s.first) return;
+  const int f = e.y * 256 + threadIdx.x;
+  if (f < s.d.n_huy) s.d.huy_val[f] = scn::huygens_value(s.d, s.ts - 1, s.d.hf[f].offset);
+}
+
+// Probe DFT of this step's output row (k_dft's accumulation).
+__global__ __launch_bounds__(256) void kb_dft(const BatchSolve* __restrict__ S, const int2* __restrict__ tab) {
+  const int2 e = tab[blockIdx.x];
+  const BatchSolve& s = S[e.x];
+  if (s.ts > s.last) return;
+  const int i = e.y * 256 + threadIdx.x;
+  if (i >= s.np * s.nf) return;
+  const int p = i / s.nf, k = i - p * s.nf;
+  const double v = s.d.ring[((long long)(s.ts - s.first) * s.d.n_out_rows + s.prow[p]) * 6 + s.pcomp[p]];
+  const double time = (double)(s.ts - 1) * s.dt;
+  const double ph = 2.0 * scn::kPI * s.freq[k] * time;
+  double sn, cs;
+  sincos(ph, &sn, &cs);
+  s.dft[2 * i] += v * cs * s.dt;
+  s.dft[2 * i + 1] -= v * sn * s.dt;
+}
+
+// End of a batch step: flip the ping-pong parity and advance the timestep.
+__global__ void kb_advance(BatchSolve* S, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  BatchSolve& s = S[i];
+  if (s.ts > s.last) return;
+  s.cur ^= 1;
+  s.ts += 1;
+}
+
+template <typename T, bool PLAIN>
+void launch_batch_step(tlm_batch* b) {
+  const hipStream_t st = b->stream;
+  BatchSolve* S = b->d_solves;
+  auto go = [&](int t, void (*k)(const BatchSolve*, const int2*)) {
+    if (b->n_tab[t]) hipLaunchKernelGGL(k, dim3(b->n_tab[t]), dim3(256), 0, st, S, b->d_tab[t]);
+  };
+  go(tlm_batch::T_HUY, kb_huygens);
+  go(tlm_batch::T_LINES, kb_lines<T, PLAIN>);
+  go(tlm_batch::T_MIXED, kb_mixed<T, PLAIN>);
+  go(tlm_batch::T_PML, kb_list<T, PLAIN, 0>);
+  go(tlm_batch::T_MAT, kb_list<T, PLAIN, 1>);
+  go(tlm_batch::T_GEN, kb_list<T, PLAIN, 2>);
+  go(tlm_batch::T_PM, kb_list<T, PLAIN, 3>);
+  go(tlm_batch::T_DFT, kb_dft);
+  const int n = (int)b->solves.size();
+  hipLaunchKernelGGL(kb_advance, dim3((n + 255) / 256), dim3(256), 0, st, S, n);
+}
+
+// Rebuild the device records and block tables after the set of solves changed.
+int rebuild(tlm_batch* b) {
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  const int n = (int)b->solves.size();
+  if (n > b->d_cap) {
+    if (b->d_solves) BHIP_TRY(b, hipFree(b->d_solves));
+    b->d_cap = std::max(n, 2 * b->d_cap);
+    BHIP_TRY(b, hipMalloc(&b->d_solves, (size_t)b->d_cap * sizeof(BatchSolve)));
+  }
+  std::vector<BatchSolve> rec(n);
+  std::vector<std::vector<int2>> tab(tlm_batch::T_N);
+  for (int i = 0; i < n; i++) {
+    const tlm_batch::Entry& e = b->solves[i];
+    const tlm_handle* h = e.h;
+    BatchSolve& s = rec[i];
+    std::memset(&s, 0, sizeof s);
+    s.d = h->d;
+    s.d.ring = e.ring;
+    s.planes[0] = h->planes[0];
+    s.planes[1] = h->planes[1];
+    s.slot_a = h->d_slot_cell;
+    s.slot_b = h->d_slot_idx;
+    s.n_slots = (long long)h->n_mixed_blocks * 256;
+    s.n_pure_pml = h->n_pure_pml;
+    s.n_pure_mat = h->n_pure_mat;
+    s.n_pure_gen = h->n_pure_gen;
+    s.n_pm = h->n_pm;
+    s.n_huy_coded = h->n_huy_coded;
+    // the host copy of ts / parity is exact: every step advances every
+    // unfinished solve by one
+    s.ts = e.ts;
+    s.cur = e.cur0 ^ ((e.ts - e.first) & 1);
+    s.first = e.first;
+    s.last = e.last;
+    s.stz_n = h->stz_n;
+    s.stz_value = h->stz_value;
+    s.np = (int)h->pt_row.size();
+    s.nf = (int)h->freq.size();
+    s.prow = h->d_prow;
+    s.pcomp = h->d_pcomp;
+    s.freq = h->d_freq;
+    s.dft = h->d_dft;
+    s.dt = h->g.dt;
+    if (e.ts > e.last) continue;  // finished, awaiting collection
+    const int nbx = (h->nx + 255) / 256;
+    const long long nl = (long long)nbx * h->ny * h->nz;
+    for (long long k = 0; k < nl; k++) tab[tlm_batch::T_LINES].push_back(make_int2(i, (int)k));
+    for (int k = 0; k < h->n_mixed_blocks; k++) tab[tlm_batch::T_MIXED].push_back(make_int2(i, k));
+    auto add = [&](int t, long long cnt) {
+      for (long long k = 0; k < (cnt + 255) / 256; k++) tab[t].push_back(make_int2(i, (int)k));
+    };
+    add(tlm_batch::T_PML, h->n_pure_pml);
+    add(tlm_batch::T_MAT, h->n_pure_mat);
+    add(tlm_batch::T_GEN, h->n_pure_gen);
+    add(tlm_batch::T_PM, h->n_pm);
+    if (h->n_huy_coded) add(tlm_batch::T_HUY, h->d.n_huy);
+    if (s.d.n_out_rows && s.np && s.nf) add(tlm_batch::T_DFT, (long long)s.np * s.nf);
+  }
+  if (n) BHIP_TRY(b, hipMemcpyAsync(b->d_solves, rec.data(), (size_t)n * sizeof(BatchSolve), hipMemcpyHostToDevice,
+                                    b->stream));
+  for (int t = 0; t < tlm_batch::T_N; t++) {
+    b->n_tab[t] = (int)tab[t].size();
+    if (tab[t].size() > b->cap_tab[t]) {
+      if (b->d_tab[t]) BHIP_TRY(b, hipFree(b->d_tab[t]));
+      b->cap_tab[t] = std::max(tab[t].size(), 2 * b->cap_tab[t]);
+      BHIP_TRY(b, hipMalloc(&b->d_tab[t], b->cap_tab[t] * sizeof(int2)));
+    }
+    if (!tab[t].empty())
+      BHIP_TRY(b, hipMemcpyAsync(b->d_tab[t], tab[t].data(), tab[t].size() * sizeof(int2), hipMemcpyHostToDevice,
+                                 b->stream));
+  }
+  BHIP_TRY(b, hipStreamSynchronize(b->stream));
+  b->dirty = false;
+  return TLM_OK;
+}
+
+int capture(tlm_batch* b, int k) {
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  BHIP_TRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < k; i++) {
+    if (b->form == TLM_FORM_PLAIN_FP64) launch_batch_step<double, true>(b);
+    else launch_batch_step<float, false>(b);
+  }
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess) return bfail(b, TLM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return bfail(b, TLM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+  b->graph_k = k;
+  return TLM_OK;
+}
+
+int unfinished(const tlm_batch* b) {
+  int n = 0;
+  for (auto& e : b->solves) n += e.ts <= e.last;
+  return n;
+}
+
+// Advance every unfinished solve by `steps` (those that end earlier idle).
+int run_steps(tlm_batch* b, int steps) {
+  int rc;
+  if (b->dirty && (rc = rebuild(b))) return rc;
+  int left = steps;
+  while (left > 0 && unfinished(b)) {
+    const int k = std::min(left, b->graph_steps);
+    if (k == b->graph_steps) {
+      if (!b->graph || b->graph_k != k)
+        if ((rc = capture(b, k))) return rc;
+      BHIP_TRY(b, hipGraphLaunch(b->graph, b->stream));
+    } else {
+      for (int i = 0; i < k; i++) {
+        if (b->form == TLM_FORM_PLAIN_FP64) launch_batch_step<double, true>(b);
+        else launch_batch_step<float, false>(b);
+      }
+      BHIP_TRY(b, hipGetLastError());
+    }
+    for (auto& e : b->solves) e.ts = std::min(e.ts + k, e.last + 1);
+    left -= k;
+  }
+  BHIP_TRY(b, hipStreamSynchronize(b->stream));
+  return TLM_OK;
+}
+
+}  // namespace engine
+
+extern "C" {
+
+int tlm_batch_create(int32_t form, int64_t hbm_budget_bytes, tlm_batch** out) {
+  if (!out) return TLM_ERR_ARG;
+  *out = nullptr;
+  if (form != TLM_FORM_PLAIN_FP64 && form != TLM_FORM_PRECISION_FP32) return TLM_ERR_ARG;
+  if (hbm_budget_bytes <= 0) return TLM_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return TLM_ERR_HIP;
+  tlm_batch* b = new (std::nothrow) tlm_batch();
+  if (!b) return TLM_ERR_NOMEM;
+  b->form = form;
+  b->budget = hbm_budget_bytes;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return TLM_ERR_HIP;
+  }
+  *out = b;
+  return TLM_OK;
+}
+
+void tlm_batch_destroy(tlm_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  for (auto& e : b->solves)
+    if (e.ring) (void)hipFree(e.ring);
+  if (b->d_solves) (void)hipFree(b->d_solves);
+  for (auto* t : b->d_tab)
+    if (t) (void)hipFree(t);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char* tlm_batch_last_error(const tlm_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int tlm_batch_add(tlm_batch* b, tlm_handle* h, int32_t first_timestep, int32_t n_steps) {
+  if (!b || !h) return TLM_ERR_ARG;
+  if (h->form != b->form) return bfail(b, TLM_ERR_ARG, "solve and batch numerical forms differ");
+  if (n_steps < 1 || first_timestep < 1 || first_timestep + n_steps - 1 > h->g.n_timesteps)
+    return bfail(b, TLM_ERR_ARG, "timestep range out of range");
+  if (h->kind != Kind::INCIDENT || h->ring_used || h->host_steps)
+    return bfail(b, TLM_ERR_STATE, "a batch takes solves that have not been stepped");
+  for (auto& e : b->solves)
+    if (e.h == h) return bfail(b, TLM_ERR_STATE, "solve is already in the batch");
+  const int64_t est = h->finalized ? h->dev_bytes : estimate_bytes(h, n_steps);
+  if (b->resident + est > b->budget)
+    return bfail(b, TLM_ERR_BUDGET, "HBM budget of the batch is exhausted");
+  int rc;
+  if ((rc = finalize(h))) return bfail(b, rc, std::string("finalize: ") + h->err);
+  if (h->d.n_face_out_rows) return bfail(b, TLM_ERR_UNSUPPORTED, "face outputs are read on the single-solve path");
+  tlm_batch::Entry e;
+  e.h = h;
+  e.first = first_timestep;
+  e.last = first_timestep + n_steps - 1;
+  e.ts = first_timestep;
+  e.cur0 = h->cur;
+  const size_t ring_bytes = (size_t)n_steps * h->d.n_out_rows * 6 * sizeof(double);
+  if (ring_bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, ring_bytes) != hipSuccess) return bfail(b, TLM_ERR_NOMEM, "output series allocation failed");
+    e.ring = (double*)p;
+  }
+  // the handle's own stream may still be uploading its tables
+  BHIP_TRY(b, hipStreamSynchronize(h->stream));
+  e.bytes = h->dev_bytes + (int64_t)ring_bytes;
+  b->resident += e.bytes;
+  b->solves.push_back(e);
+  b->dirty = true;
+  return TLM_OK;
+}
+
+int tlm_batch_run(tlm_batch* b, int32_t n_steps) {
+  if (!b || n_steps < 0) return TLM_ERR_ARG;
+  return run_steps(b, n_steps);
+}
+
+int tlm_batch_collect(tlm_batch* b, tlm_handle** done, int32_t max_done, int32_t* n_done) {
+  if (!b || (max_done && !done)) return TLM_ERR_ARG;
+  int n = 0;
+  std::vector<tlm_batch::Entry> keep;
+  for (auto& e : b->solves) {
+    if (e.ts <= e.last || n >= max_done) {
+      keep.push_back(e);
+      continue;
+    }
+    tlm_handle* h = e.h;
+    const int steps = e.last - e.first + 1;
+    const size_t w = (size_t)h->d.n_out_rows * 6;
+    if (w) {
+      h->host_out.resize((size_t)steps * w);
+      BHIP_TRY(b, hipMemcpy(h->host_out.data(), e.ring, (size_t)steps * w * sizeof(double), hipMemcpyDeviceToHost));
+      h->host_first = e.first;
+      h->host_steps = steps;
+    }
+    if (e.ring) BHIP_TRY(b, hipFree(e.ring));
+    // the handle continues as after tlm_step(first, n): last connect pending
+    h->cur = e.cur0 ^ (steps & 1);
+    h->kind = Kind::SCATTERED;
+    h->pend_connect = true;
+    h->pend_boundary = true;
+    h->pend_ts = e.last;
+    b->resident -= e.bytes;
+    done[n++] = h;
+  }
+  if (n) {
+    b->solves.swap(keep);
+    b->dirty = true;
+  }
+  if (n_done) *n_done = n;
+  return TLM_OK;
+}
+
+int tlm_batch_info(const tlm_batch* b, int32_t* n_solves, int32_t* n_unfinished, int64_t* resident_bytes) {
+  if (!b) return TLM_ERR_ARG;
+  if (n_solves) *n_solves = (int32_t)b->solves.size();
+  if (n_unfinished) *n_unfinished = unfinished(b);
+  if (resident_bytes) *resident_bytes = b->resident;
+  return TLM_OK;
+}
+
+int tlm_batch_set_graph_steps(tlm_batch* b, int32_t k) {
+  if (!b || k < 1 || k > 4096) return TLM_ERR_ARG;
+  b->graph_steps = k;
+  return TLM_OK;
+}
+
+int tlm_batch_autotune(tlm_batch* b, int32_t steps_per_candidate, int32_t* best_k, int32_t* steps_done) {
+  if (!b || steps_per_candidate < 1) return TLM_ERR_ARG;
+  static const int cand[] = {4, 16, 64};
+  int rc, done = 0, best = b->graph_steps;
+  double best_ms = 1e300;
+  for (int k : cand) {
+    if (!unfinished(b)) break;
+    b->graph_steps = k;
+    const int n = std::max(k, steps_per_candidate / k * k);
+    // first replay captures; time the second
+    if ((rc = run_steps(b, k))) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    if ((rc = run_steps(b, n))) return rc;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / n;
+    done += k + n;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = k;
+    }
+  }
+  b->graph_steps = best;
+  b->last_ms_per_step = best_ms;
+  if (best_k) *best_k = best;
+  if (steps_done) *steps_done = done;
+  return TLM_OK;
+}
+
+}  // extern "C"

@@ -208,7 +208,7 @@ constexpr int32_t kFdHuy = 1 << 30;
 constexpr int kFdHuyFaceBits = 27;
 
 // Per-step Huygens face values: huy_val[f] = huygens_value(timestep, offset_f).
-__global__ void k_huygens_values(Dev d, int timestep) {
+static __global__ void k_huygens_values(Dev d, int timestep) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f < d.n_huy) d.huy_val[f] = huygens_value(d, timestep, d.hf[f].offset);
 }
@@ -1570,7 +1570,7 @@ __global__ __launch_bounds__(256) void k_connect_list(Dev d, const T* __restrict
 }
 
 // Probe DFT: acc(p,k) += v_p(n) exp(-j 2 pi f_k t_n) dt, t_n = (n-1) dt.
-__global__ void k_dft(const double* __restrict__ ring, int n_out_rows, int slot, int timestep,
+static __global__ void k_dft(const double* __restrict__ ring, int n_out_rows, int slot, int timestep,
                       const int32_t* __restrict__ prow, const int32_t* __restrict__ pcomp, int n_points,
                       const double* __restrict__ freq, int n_freq, double dt, double* __restrict__ acc) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;

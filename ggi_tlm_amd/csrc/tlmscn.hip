@@ -12,152 +12,11 @@
 // and outer_boundary are recorded as pending and folded into the next
 // scatter's pull.  A download materialises the pending connect first, so
 // the caller always sees exactly the reference's V.
-#include <hip/hip_runtime.h>
+#include "engine.hpp"
 
-#include <algorithm>
-#include <cstdio>
-#include <cstring>
-#include <new>
-#include <string>
-#include <vector>
+using namespace engine;
 
-#include "scn_kernels.hpp"
-
-using scn::Dev;
-using scn::PullArgs;
-
-namespace {
-
-enum class Kind { INCIDENT, SCATTERED };
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-
-}  // namespace
-
-struct tlm_handle {
-  tlm_grid g{};
-  int form = TLM_FORM_PLAIN_FP64;
-  std::string err;
-  int nx = 0, ny = 0, nz = 0;
-  long long N = 0, P = 0;
-  hipStream_t stream = nullptr;
-
-  // host tables (reference layout)
-  bool cells_set = false, faces_set = false;
-  std::vector<int32_t> cell_code, c_mat, c_filt, c_pml, c_exc, c_out;
-  int n_special = 0;
-  std::vector<int32_t> face_code, f_mat, f_exc, f_out;
-  int n_special_faces = 0;
-  std::vector<tlm_volume_material> vmat;
-  std::vector<int32_t> smat;
-  std::vector<tlm_pml_parameter> pml_par;
-  std::vector<int32_t> pml_cell_param;
-  int n_functions = 0;
-  std::vector<double> value, value_face;
-  std::vector<tlm_point_source> ps;
-  int n_cell_rows = 0, n_face_rows = 0;
-  int huy_function = 0;
-  double huy_offset_min = 0.0, Ei[3] = {0, 0, 0}, Hi[3] = {0, 0, 0};
-  std::vector<tlm_huygens_face> huy;
-  std::vector<int32_t> pt_row, pt_comp;
-  std::vector<double> freq;
-
-  // device
-  bool finalized = false;
-  std::vector<DevBuf> bufs;
-  void* planes[2] = {nullptr, nullptr};
-  int cur = 0;
-  Dev d{};
-  int n_gen = 0, n_pml = 0, n_mat = 0;
-  int n_pure_pml = 0, n_pure_mat = 0, n_pure_gen = 0, n_mixed_blocks = 0;
-  int n_huy_coded = 0;  // descriptor entries in the kFdHuy form
-  int n_pm = 0;         // PML cells holding a material
-  int stz_n = 0;        // TLM_set_small_to_zero every stz_n steps (0: off)
-  double stz_value = 0.0;
-  long long n_mixed_cells = 0;
-  const int32_t* d_slot_cell = nullptr;  // mixed-line workgroup slots
-  const int32_t* d_slot_idx = nullptr;
-  const int32_t* d_gen_cell = nullptr;
-  const int32_t* d_gen_fd = nullptr;
-  const int32_t* d_pml_cell = nullptr;
-  const int32_t* d_pml_fd = nullptr;
-  int32_t* d_prow = nullptr;
-  int32_t* d_pcomp = nullptr;
-  double* d_freq = nullptr;
-  double* d_dft = nullptr;
-  int64_t dev_bytes = 0;
-
-  // bulk schedule (tlm_set_schedule)
-  int sched_variant = -1, sched_by = 4, sched_zc = 16, sched_xcd = 0;  // -1: per-form default
-
-  // state machine
-  Kind kind = Kind::INCIDENT;
-  bool pend_connect = false, pend_boundary = false;
-  int pend_ts = 0;
-
-  // profiling: event pairs per kernel class, resolved on read
-  bool profiling = false;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
-  int64_t prof_launches[TLM_N_KCLASS] = {0};
-  double prof_ms[TLM_N_KCLASS] = {0};
-
-  // output ring
-  int cap = 4096;
-  int ring_used = 0;         // slots holding unread steps on the device
-  int ring_first_step = 0;   // timestep of slot 0
-  std::vector<double> host_out;  // drained rows awaiting read
-  int host_first = 0, host_steps = 0;
-  // face outputs: one block of n_face_out_rows * 2 * 6 per connect
-  int fring_used = 0, fring_first_step = 0;
-  std::vector<double> host_fout;
-  int host_ffirst = 0, host_fsteps = 0;
-};
-
-namespace {
-
-int fail(tlm_handle* h, int code, const std::string& msg) {
-  if (h) h->err = msg;
-  return code;
-}
-
-int hip_fail(tlm_handle* h, hipError_t e, const char* what) {
-  return fail(h, TLM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define HIP_TRY(h, expr)                                   \
-  do {                                                     \
-    hipError_t e_ = (expr);                                \
-    if (e_ != hipSuccess) return hip_fail((h), e_, #expr); \
-  } while (0)
-
-template <typename V>
-int dev_upload(tlm_handle* h, const std::vector<V>& v, const V** out) {
-  size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(V);
-  void* p = nullptr;
-  hipError_t e = hipMalloc(&p, bytes);
-  if (e != hipSuccess) return fail(h, TLM_ERR_NOMEM, "hipMalloc failed for a table");
-  h->bufs.push_back({p, bytes});
-  h->dev_bytes += (int64_t)bytes;
-  if (!v.empty()) HIP_TRY(h, hipMemcpy(p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
-  *out = (const V*)p;
-  return TLM_OK;
-}
-
-int dev_alloc(tlm_handle* h, size_t bytes, void** out) {
-  void* p = nullptr;
-  bytes = std::max<size_t>(bytes, 8);
-  hipError_t e = hipMalloc(&p, bytes);
-  if (e != hipSuccess) return fail(h, TLM_ERR_NOMEM, "hipMalloc failed (" + std::to_string(bytes) + " bytes)");
-  h->bufs.push_back({p, bytes});
-  h->dev_bytes += (int64_t)bytes;
-  *out = p;
-  return TLM_OK;
-}
-
-size_t link_size(const tlm_handle* h) { return h->form == TLM_FORM_PLAIN_FP64 ? sizeof(double) : sizeof(float); }
+namespace engine {
 
 // Profiling brackets (no-ops unless enabled).
 struct ProfScope {
@@ -166,13 +25,13 @@ struct ProfScope {
   hipEvent_t a = nullptr, b = nullptr;
   ProfScope(tlm_handle* hh, int c) : h(hh), cls(c) {
     if (!h->profiling) return;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
-    hipEventRecord(a, h->stream);
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, h->stream);
   }
   ~ProfScope() {
     if (!h->profiling) return;
-    hipEventRecord(b, h->stream);
+    (void)hipEventRecord(b, h->stream);
     h->ev.push_back({cls, {a, b}});
   }
 };
@@ -182,11 +41,11 @@ int resolve_profile(tlm_handle* h) {
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   for (auto& e : h->ev) {
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e.second.first, e.second.second);
+    (void)hipEventElapsedTime(&ms, e.second.first, e.second.second);
     h->prof_ms[e.first] += ms;
     h->prof_launches[e.first] += 1;
-    hipEventDestroy(e.second.first);
-    hipEventDestroy(e.second.second);
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
   }
   h->ev.clear();
   return TLM_OK;
@@ -269,14 +128,9 @@ int finalize(tlm_handle* h) {
     for (int k = 0; k < 6; k++) fd.push_back(codes[k]);
     i = j;
   }
-  auto fd_of = [&](long long c) -> int32_t {
-    auto it = std::lower_bound(fd_cell.begin(), fd_cell.end(), c);
-    if (it != fd_cell.end() && *it == c) return (int32_t)(it - fd_cell.begin());
-    return -1;
-  };
 
   // ---- cell classes ------------------------------------------------------
-  // class per cell: 0 bulk, 1 PML, 2 material, 3 generic
+  // class per cell: 0 bulk, 1 PML, 2 material, 3 generic, 4 PML holding a material
   struct PmlRec { int32_t c; uint16_t par; int32_t orow, fdc; };
   struct MatRec { int32_t c; uint16_t m; int32_t f; };
   struct GenRec { int32_t c, code, m, f, er, orow, fdc; };
@@ -945,7 +799,7 @@ int materialize(tlm_handle* h) {
   return TLM_OK;
 }
 
-}  // namespace
+}  // namespace engine
 
 // ===========================================================================
 // C ABI
@@ -978,13 +832,13 @@ int tlm_create(const tlm_grid* g, int32_t form, tlm_handle** out) {
 
 void tlm_destroy(tlm_handle* h) {
   if (!h) return;
-  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (auto& e : h->ev) {
-    hipEventDestroy(e.second.first);
-    hipEventDestroy(e.second.second);
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
   }
-  for (auto& b : h->bufs) hipFree(b.p);
-  if (h->stream) hipStreamDestroy(h->stream);
+  for (auto& b : h->bufs) (void)hipFree(b.p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 
@@ -1159,7 +1013,7 @@ int tlm_upload_links(tlm_handle* h, const double* V) {
                          (float*)h->planes[h->cur], h->nx, h->ny, h->nz, h->P);
     e = hipStreamSynchronize(h->stream);
   }
-  hipFree(tmp);
+  (void)hipFree(tmp);
   if (e != hipSuccess) return hip_fail(h, e, "upload_links");
   h->kind = Kind::INCIDENT;
   h->pend_connect = h->pend_boundary = false;
@@ -1186,7 +1040,7 @@ int tlm_download_links(tlm_handle* h, double* V) {
     e = hipMemcpyAsync(V, tmp, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  hipFree(tmp);
+  (void)hipFree(tmp);
   if (e != hipSuccess) return hip_fail(h, e, "download_links");
   return TLM_OK;
 }

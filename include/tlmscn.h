@@ -38,6 +38,7 @@ extern "C" {
 #define TLM_ERR_HIP 3         /* HIP runtime failure */
 #define TLM_ERR_UNSUPPORTED 4 /* feature that stays on the host path */
 #define TLM_ERR_NOMEM 5       /* device allocation failed */
+#define TLM_ERR_BUDGET 6      /* batch: the solve does not fit the HBM budget left */
 
 /* Numerical forms.  PLAIN: FP64 link planes, bit-faithful to the reference
  * sequential build's operation order.  PRECISION: FP32 link planes; filter

@@ -15,9 +15,23 @@ def declared_symbols():
     return sorted(set(re.findall(r"\b(tlm_[a-z_]+)\s*\(", txt)))
 
 
+def batch_symbols():
+    txt = open(os.path.join(ROOT, "include", "tlmbatch.h")).read()
+    return sorted(set(re.findall(r"\b(tlm_[a-z_]+)\s*\(", txt)))
+
+
 def test_header_matches_binding():
     names = declared_symbols()
     assert set(names) == set(capi.EXPORTED), set(names) ^ set(capi.EXPORTED)
+    assert set(batch_symbols()) == set(capi.BATCH_EXPORTED), set(batch_symbols()) ^ set(capi.BATCH_EXPORTED)
+
+
+def test_engine_exports_batch_symbols():
+    if not os.path.exists(capi.ENGINE_SO):
+        pytest.fail("libtlmscn.so not built: run __graft_entry__.build()")
+    lib = ctypes.CDLL(capi.ENGINE_SO)
+    for n in batch_symbols():
+        assert hasattr(lib, n), n
 
 
 def test_engine_exports_every_symbol():
