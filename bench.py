@@ -44,36 +44,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from ggi_tlm_amd import capi  # noqa: E402
-from ggi_tlm_amd.filters import Sfilter  # noqa: E402
-from ggi_tlm_amd.problem import (PML, Box, ExcitationFunction, HuygensBox, OutputPoint,  # noqa: E402
-                                 Problem, Sphere, VolumeMaterial, build_tables)
+from ggi_tlm_amd import capi, workloads  # noqa: E402
+from ggi_tlm_amd.problem import build_tables  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
 
 def workload(n=256, npml=8, radius=60, steps=4000, seed=2024):
-    rng = np.random.default_rng(seed)
-    dl = 5e-3
-    dt = dl / (2 * 2.99792458e8)
-    off = rng.integers(-40, 41, size=3)
-    cx, cy, cz = (n / 2 + float(o) for o in off)
-    b = npml + 3  # Huygens box; source faces sit on the cells just outside it
-    probes = []
-    comps = ("ez", "ez", "ez", "ez", "ey", "hy", "hx", "ex")
-    while len(probes) < 16:
-        p = rng.integers(b + 1, n - b, size=3)
-        if (p[0] - 0.5 - cx) ** 2 + (p[1] - 0.5 - cy) ** 2 + (p[2] - 0.5 - cz) ** 2 <= (radius + 2) ** 2:
-            continue
-        probes.append(OutputPoint(tuple(int(v) for v in p), comps[len(probes) % len(comps)]))
-    return Problem(
-        nx=n, ny=n, nz=n, dl=dl, n_timesteps=steps, reflection=(-1.0,) * 6,
-        pml=PML((npml,) * 6, (1e-4,) * 6, 2),
-        materials=[VolumeMaterial("dispersive", Sfilter.constant(4.0), Sfilter.constant(1.0), sigma_e=0.01)],
-        material_regions=[(Sphere(cx, cy, cz, radius), 1)],
-        functions=[ExcitationFunction("gaussian", (1.0, 40 * dt, 160 * dt))],
-        huygens=HuygensBox(Box(b, n - b + 1, b, n - b + 1, b, n - b + 1), 1, 90.0, 0.0, -1.0, 0.0, side=1),
-        outputs=probes)
+    return workloads.single_large(n=n, npml=npml, radius=radius, steps=steps, seed=seed)
 
 
 def measured_traffic(a, kernel, alg_bytes):
@@ -108,6 +86,127 @@ def cpu_baseline(tables, steps):
                       f"({t1 - t0:.1f} s), single-threaded C oracle (gcc -O2, FP64)"}
 
 
+def _build(problem):
+    return build_tables(problem)
+
+
+def build_all(problems, workers=8):
+    """Tables of many decks, built in worker processes (CPU only, before any
+    GPU call of this process)."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork")) as ex:
+        return list(ex.map(_build, problems, chunksize=4))
+
+
+def main_batch(a):
+    """Batch of enclosure solves (shielding effectiveness), all resident and
+    stepped together; beside it, the same solves one at a time in the plain
+    FP64 single-solve path on a subset, as solves/s."""
+    from ggi_tlm_amd import serving
+
+    t_setup = time.perf_counter()
+    probs = [workloads.enclosure(s, steps=a.batch_steps) for s in range(a.batch_solves)]
+    tables = build_all(probs)
+    cells = [t.nx * t.ny * t.nz for t in tables]
+    print(f"[bench] batch: {len(tables)} enclosures, {sum(cells) / 1e6:.1f} M cells, tables in "
+          f"{time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+    lib = capi.engine_library()
+    b = capi.Batch(capi.FORM_PRECISION_FP32, 250 << 30, lib)
+    handles = []
+    for t in tables:
+        h = capi.Handle(lib, t, capi.FORM_PRECISION_FP32, output_capacity=a.batch_steps)
+        h.release_tables()
+        assert b.add(h, 1, a.batch_steps)
+        handles.append(h)
+    resident = b.info()[2]
+    b.run(1)
+    b.synchronize()
+    print(f"[bench] batch admitted, {resident / 2**30:.1f} GiB resident; set-up {time.perf_counter() - t_setup:.1f} s",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    k, tuned = b.autotune(64)
+    b.run(max(0, a.batch_steps - 1 - tuned))
+    done = b.collect()
+    elapsed = time.perf_counter() - t0
+    assert len(done) == len(handles)
+    work = sum(cells) * (a.batch_steps - 1)
+    value = len(handles) / elapsed * (a.batch_steps - 1) / a.batch_steps
+    # shielding effectiveness at the centre probe from one solve's DFT
+    se = None
+    h0 = handles[0]
+    dft = h0.read_dft()
+    if dft is not None and dft.size:
+        se = float(np.median(np.abs(dft[0])))
+    b.close()
+    for h in handles:
+        h.close()
+    base = None
+    if a.baseline_solves:
+        sub = [serving.Request(i, tables[i], a.batch_steps, 0.0) for i in range(a.baseline_solves)]
+        st = serving.run_sequential(sub, capi.FORM_PLAIN_FP64, lib)
+        base = {"solves_per_s": st["solves_per_s"], "gcell_updates_per_s": st["gcell_updates_per_s"],
+                "solves": len(sub), "form": "plain FP64, one solve at a time (tlm_step)"}
+    line = {
+        "metric": "solves/s", "value": round(value, 3), "unit": "solves/s", "n_gpus": 1,
+        "steps": a.batch_steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{len(tables)} shielding enclosures (PEC box, one aperture, 6-cell PML, "
+                               f"Huygens plane wave, centre probe DFT at 100 f), {a.batch_steps} steps each",
+                   "form": "precision: FP32 link planes", "graph_steps": k, "cells": int(sum(cells)),
+                   "gcell_updates_per_s": round(work / elapsed / 1e9, 3), "elapsed_s": round(elapsed, 3),
+                   "resident_GiB": round(resident / 2**30, 2), "median_abs_dft_solve0": se},
+        "baseline_one_at_a_time": base,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def main_serve(a):
+    """Poisson stream of mixed requests served with continuous batching
+    (precision form) against the same stream served one request at a time
+    in the plain FP64 path; p50/p99 latency and solves/s for both, and the
+    largest probe-series relative L2 difference between the two."""
+    from ggi_tlm_amd import serving
+
+    rng = np.random.default_rng(a.seed)
+    probs = [workloads.serving_request(rng) for _ in range(a.serve_requests)]
+    t_setup = time.perf_counter()
+    tables = build_all(probs)
+    print(f"[bench] serve: {len(tables)} requests, tables in {time.perf_counter() - t_setup:.1f} s",
+          file=sys.stderr, flush=True)
+    lib = capi.engine_library()
+    out = {}
+    for rate in a.rates:
+        arr = serving.poisson_arrivals(len(tables), rate, np.random.default_rng(a.seed + int(rate)))
+        reqs = [serving.Request(i, t, p.n_timesteps, float(arr[i])) for i, (t, p) in enumerate(zip(tables, probs))]
+        st_b = serving.run_batched(reqs, capi.FORM_PRECISION_FP32, chunk=a.chunk, lib=lib)
+        series_b = [r.outputs for r in reqs]
+        print(f"[bench] rate {rate}: batched {st_b}", file=sys.stderr, flush=True)
+        reqs_s = [serving.Request(i, t, p.n_timesteps, float(arr[i])) for i, (t, p) in enumerate(zip(tables, probs))]
+        st_s = serving.run_sequential(reqs_s, capi.FORM_PLAIN_FP64, lib)
+        print(f"[bench] rate {rate}: one at a time {st_s}", file=sys.stderr, flush=True)
+        err = 0.0
+        for r, sb in zip(reqs_s, series_b):
+            so = r.outputs
+            floor = 1e-3 * max(np.linalg.norm(so[:, k]) for k in range(so.shape[1]))
+            for k in range(so.shape[1]):
+                err = max(err, float(np.linalg.norm(sb[:, k] - so[:, k]) / max(np.linalg.norm(so[:, k]), floor)))
+        out[str(rate)] = {"batched_precision": st_b, "one_at_a_time_fp64": st_s,
+                          "speedup_solves_per_s": round(st_b["solves_per_s"] / st_s["solves_per_s"], 2),
+                          "max_probe_rel_l2_vs_fp64": err}
+    top = out[str(max(a.rates))]
+    line = {
+        "metric": "solves/s", "value": top["batched_precision"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+        "steps": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(tables)} Poisson-arrival requests, cubic n log-U[24,160], steps U[1000,8000], "
+                               "40% free / 35% sphere / 25% 2-pole blocks, 6-cell PML, 1-8 probes",
+                   "rates_req_per_s": a.rates, "chunk_steps": a.chunk, "by_rate": out},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,12 +217,24 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=("single", "batch", "serve"), default="single")
+    ap.add_argument("--batch-solves", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=4096)
+    ap.add_argument("--baseline-solves", type=int, default=8)
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--rates", type=float, nargs="+", default=[5.0, 200.0])
+    ap.add_argument("--chunk", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=11)
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs one solve per GPU on a single MI355X; "
                                    "multi-GPU z-slab sharding is designed but not built (--gpus 1 only)"}))
         sys.exit(2)
 
+    if a.mode == "batch":
+        return main_batch(a)
+    if a.mode == "serve":
+        return main_serve(a)
     total = a.warmup + a.steps + a.profile_steps
     t_setup = time.perf_counter()
     p = workload(n=a.n, steps=total)

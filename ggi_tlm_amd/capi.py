@@ -80,6 +80,7 @@ BATCH_SIGNATURES = {
     "batch_last_error": (C.c_char_p, [_h]),
     "batch_add": (C.c_int, [_h, _h, _i32, _i32]),
     "batch_run": (C.c_int, [_h, _i32]),
+    "batch_synchronize": (C.c_int, [_h]),
     "batch_collect": (C.c_int, [_h, _p, _i32, C.POINTER(_i32)]),
     "batch_info": (C.c_int, [_h, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(C.c_int64)]),
     "batch_set_graph_steps": (C.c_int, [_h, _i32]),
@@ -291,6 +292,17 @@ class Handle:
         self._check(self.lib.fn["kernel_bytes"](self.h, _ptr(c), _ptr(b)))
         return {k: (int(c[i]), int(b[i])) for i, k in enumerate(KERNEL_CLASSES)}
 
+    def release_tables(self):
+        """Drop the host copies of the tables (the library holds its own),
+        keeping the few fields the readers need."""
+        from types import SimpleNamespace
+        t = self.t
+        self.t = SimpleNamespace(n_output_cells=t.n_output_cells, n_output_faces=t.n_output_faces,
+                                 n_timesteps=t.n_timesteps, output_rows=t.output_rows,
+                                 output_components=t.output_components, dft_frequencies=t.dft_frequencies,
+                                 nx=t.nx, ny=t.ny, nz=t.nz)
+        self._keep = []
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.fn["destroy"](self.h)
@@ -341,7 +353,11 @@ class Batch:
         return True
 
     def run(self, n_steps: int):
+        """Enqueue n_steps steps (returns before the device is done)."""
         self._check(self.lib.fn["batch_run"](self.b, int(n_steps)))
+
+    def synchronize(self):
+        self._check(self.lib.fn["batch_synchronize"](self.b))
 
     def collect(self):
         n = max(len(self.members), 1)

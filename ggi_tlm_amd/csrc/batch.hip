@@ -363,7 +363,8 @@ int unfinished(const tlm_batch* b) {
   return n;
 }
 
-// Advance every unfinished solve by `steps` (those that end earlier idle).
+// Enqueue `steps` steps of every unfinished solve (those that end earlier
+// idle); the host copies of the timesteps advance at once.
 int run_steps(tlm_batch* b, int steps) {
   int rc;
   if (b->dirty && (rc = rebuild(b))) return rc;
@@ -384,7 +385,6 @@ int run_steps(tlm_batch* b, int steps) {
     for (auto& e : b->solves) e.ts = std::min(e.ts + k, e.last + 1);
     left -= k;
   }
-  BHIP_TRY(b, hipStreamSynchronize(b->stream));
   return TLM_OK;
 }
 
@@ -467,8 +467,15 @@ int tlm_batch_run(tlm_batch* b, int32_t n_steps) {
   return run_steps(b, n_steps);
 }
 
+int tlm_batch_synchronize(tlm_batch* b) {
+  if (!b) return TLM_ERR_ARG;
+  BHIP_TRY(b, hipStreamSynchronize(b->stream));
+  return TLM_OK;
+}
+
 int tlm_batch_collect(tlm_batch* b, tlm_handle** done, int32_t max_done, int32_t* n_done) {
   if (!b || (max_done && !done)) return TLM_ERR_ARG;
+  BHIP_TRY(b, hipStreamSynchronize(b->stream));
   int n = 0;
   std::vector<tlm_batch::Entry> keep;
   for (auto& e : b->solves) {
@@ -528,8 +535,10 @@ int tlm_batch_autotune(tlm_batch* b, int32_t steps_per_candidate, int32_t* best_
     const int n = std::max(k, steps_per_candidate / k * k);
     // first replay captures; time the second
     if ((rc = run_steps(b, k))) return rc;
+    BHIP_TRY(b, hipStreamSynchronize(b->stream));
     const auto t0 = std::chrono::steady_clock::now();
     if ((rc = run_steps(b, n))) return rc;
+    BHIP_TRY(b, hipStreamSynchronize(b->stream));
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / n;
     done += k + n;
     if (ms < best_ms) {

@@ -497,8 +497,13 @@ def build_tables(p: Problem) -> Tables:
     pml_cell_param = np.zeros(0, dtype=np.int32)
     if n_pml_cells:
         d = np.stack([dist[a].ravel()[flat_pml] for a in range(3)], axis=1)
-        # first appearance in scan order defines the parameter number
-        uniq, first, inverse = np.unique(d, axis=0, return_index=True, return_inverse=True)
+        # first appearance in scan order defines the parameter number; the
+        # depth triples are packed into one integer key for the uniquing
+        dmin = d.min(axis=0).astype(np.int64)
+        span = d.max(axis=0).astype(np.int64) - dmin + 1
+        key = ((d[:, 0] - dmin[0]) * span[1] + (d[:, 1] - dmin[1])) * span[2] + (d[:, 2] - dmin[2])
+        ukey, first, inverse = np.unique(key, return_index=True, return_inverse=True)
+        uniq = d[first]
         order = np.argsort(first)
         rank = np.empty(len(order), dtype=np.int32)
         rank[order] = np.arange(1, len(order) + 1, dtype=np.int32)

@@ -30,10 +30,15 @@ const char* tlm_batch_last_error(const tlm_batch* b);
  * queued and retry after tlm_batch_collect).  The handle must not be used
  * until collected.  Face outputs are not recorded in a batch. */
 int tlm_batch_add(tlm_batch* b, tlm_handle* h, int32_t first_timestep, int32_t n_steps);
-/* Advance every unfinished solve by n_steps (solves that reach their last
- * step earlier stop there); returns when the device is done. */
+/* Enqueue n_steps steps of every unfinished solve (solves that reach their
+ * last step earlier stop there) and return at once: the host can build the
+ * next arrivals meanwhile.  A change of the set of solves (add, collect)
+ * takes effect at the next run. */
 int tlm_batch_run(tlm_batch* b, int32_t n_steps);
-/* Hand back up to max_done finished solves.  Each is then as after
+/* Wait until the enqueued steps are done. */
+int tlm_batch_synchronize(tlm_batch* b);
+/* Wait for the enqueued steps, then hand back up to max_done finished
+ * solves.  Each is then as after
  * tlm_step over its range: tlm_read_cell_outputs, tlm_read_dft and
  * tlm_download_links work on it. */
 int tlm_batch_collect(tlm_batch* b, tlm_handle** done, int32_t max_done, int32_t* n_done);
