@@ -57,7 +57,8 @@ def workload(n=256, npml=8, radius=60, steps=4000, seed=2024):
 def measured_traffic(a, kernel, alg_bytes):
     """HBM bytes per launch of `kernel` from the committed counter summary of
     this workload, if it was measured on the same kernel and tables."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{a.n}cube_{a.form}.json")
+    name = f"{a.n}cube" if a.workload == "large" else a.workload
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}_{a.form}.json")
     try:
         with open(path) as f:
             rec = json.load(f)["kernels"][kernel]
@@ -218,6 +219,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=("single", "batch", "serve"), default="single")
+    ap.add_argument("--workload", choices=("large", "dispersive"), default="large")
+    ap.add_argument("--tune-steps", type=int, default=4)
     ap.add_argument("--batch-solves", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=4096)
     ap.add_argument("--baseline-solves", type=int, default=8)
@@ -235,9 +238,17 @@ def main():
         return main_batch(a)
     if a.mode == "serve":
         return main_serve(a)
-    total = a.warmup + a.steps + a.profile_steps
+    tune_steps = 9 * (a.tune_steps + 1) if a.tune_steps else 0
+    total = tune_steps + a.warmup + a.steps + a.profile_steps
     t_setup = time.perf_counter()
-    p = workload(n=a.n, steps=total)
+    if a.workload == "dispersive":
+        p = workloads.dispersive_load(steps=total)
+        desc = (f"{p.nx}x{p.ny}x{p.nz} cells, 10-cell PML, 40 two-pole Debye/Lorentz blocks, probe DFT at "
+                f"200 frequencies, 16 probes, {a.steps} timed steps")
+    else:
+        p = workload(n=a.n, steps=total)
+        desc = (f"{p.nx}^3 cells, 8-cell PML, eps_r=4 sigma=0.01 sphere r=60, Huygens plane wave, "
+                f"16 probes, {a.steps} timed steps")
     t = build_tables(p)
     form = capi.FORM_PRECISION_FP32 if a.form == "precision" else capi.FORM_PLAIN_FP64
     h = capi.Handle(capi.engine_library(), t, form, output_capacity=1024)
@@ -245,11 +256,16 @@ def main():
     print(f"[bench] set-up {setup_s:.1f} s; cells {t.nx * t.ny * t.nz}, PML {t.n_pml_cells}, "
           f"special {t.n_special_cells}, special faces {t.n_special_faces}", file=sys.stderr, flush=True)
 
-    # warm-up, then the timed region
-    h.step(1, a.warmup)
+    # schedule autotuning and warm-up (untimed), then the timed region
+    first = 1
+    if tune_steps:
+        first += h.autotune(1, a.tune_steps)
+    h.step(first, a.warmup)
+    first += a.warmup
     h.synchronize()
     t0 = time.perf_counter()
-    h.step(a.warmup + 1, a.steps)
+    h.step(first, a.steps)
+    first += a.steps
     h.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
@@ -259,7 +275,7 @@ def main():
 
     # profiled pass: per-class kernel durations with HIP events on the engine stream
     h.set_profiling(True)
-    h.step(a.warmup + a.steps + 1, a.profile_steps)
+    h.step(first, a.profile_steps)
     kt = h.read_kernel_times()
     h.set_profiling(False)
     kb = h.kernel_bytes()
@@ -299,8 +315,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{t.nx}^3 cells, 8-cell PML, eps_r=4 sigma=0.01 sphere r=60, Huygens plane wave, "
-                        f"16 probes, {a.steps} timed steps",
+            "workload": desc,
             "form": "precision: FP32 link planes, FP64 arithmetic and states" if form else "plain FP64",
             "link_bytes": 4 if form else 8,
             "cells": N,

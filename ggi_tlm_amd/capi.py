@@ -64,6 +64,7 @@ SIGNATURES = {
     "set_schedule": (C.c_int, [_h, _i32, _i32, _i32, _i32]),
     "set_profiling": (C.c_int, [_h, _i32]),
     "set_small_to_zero": (C.c_int, [_h, _i32, C.c_double]),
+    "autotune": (C.c_int, [_h, _i32, _i32, C.POINTER(_i32)]),
     "read_face_outputs": (C.c_int, [_h, _p, _i32, _p, _p]),
     "read_kernel_times": (C.c_int, [_h, _p, _p]),
     "kernel_bytes": (C.c_int, [_h, _p, _p]),
@@ -270,6 +271,13 @@ class Handle:
 
     def set_schedule(self, variant: int, rows: int, zc: int, xcd_remap: int = 0):
         self._check(self.lib.fn["set_schedule"](self.h, int(variant), int(rows), int(zc), int(xcd_remap)))
+
+    def autotune(self, first_timestep: int, steps_per_candidate: int) -> int:
+        """Run candidate bulk schedules on the next steps and keep the
+        fastest; returns the number of steps run."""
+        done = _i32(0)
+        self._check(self.lib.fn["autotune"](self.h, int(first_timestep), int(steps_per_candidate), C.byref(done)))
+        return done.value
 
     def set_small_to_zero(self, n_small_to_zero: int, small_to_zero_value: float):
         """TLM_set_small_to_zero every n steps (0: off)."""

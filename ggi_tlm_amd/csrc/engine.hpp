@@ -121,6 +121,8 @@ inline int hip_fail(tlm_handle* h, hipError_t e, const char* what) {
   return fail(h, TLM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+inline int hip_ok(tlm_handle* h, hipError_t e) { return e == hipSuccess ? TLM_OK : hip_fail(h, e, "HIP call"); }
+
 #define HIP_TRY(h, expr)                                   \
   do {                                                     \
     hipError_t e_ = (expr);                                \

@@ -1159,6 +1159,52 @@ int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, i
   return TLM_OK;
 }
 
+int tlm_autotune(tlm_handle* h, int32_t first_timestep, int32_t steps_per_candidate, int32_t* steps_done) {
+  if (!h || steps_per_candidate < 1 || first_timestep < 1) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  struct Cand { int v, rows, zc, xcd; };
+  std::vector<Cand> cand = {{3, 1, 1, 0}, {3, 1, 1, 1}, {0, 1, 1, 0}, {0, 2, 1, 0}, {0, 4, 1, 0},
+                            {2, 1, 1, 0}, {2, 4, 1, 0}, {1, 4, 8, 0}};
+  if (h->ny % 8 == 0) cand.push_back({3, 1, 1, 2});
+  const int total = (int)cand.size() * (steps_per_candidate + 1);
+  if (first_timestep + total - 1 > h->g.n_timesteps)
+    return fail(h, TLM_ERR_ARG, "not enough timesteps left for the autotuner");
+  hipEvent_t e0, e1;
+  HIP_TRY(h, hipEventCreate(&e0));
+  HIP_TRY(h, hipEventCreate(&e1));
+  int ts = first_timestep, best = 0;
+  float best_ms = 1e30f;
+  for (size_t k = 0; k < cand.size(); k++) {
+    h->sched_variant = cand[k].v;
+    h->sched_by = cand[k].rows;
+    h->sched_zc = cand[k].zc;
+    h->sched_xcd = cand[k].xcd;
+    if ((rc = tlm_step(h, ts, 1))) break;  // first launch of this schedule, untimed
+    ts += 1;
+    if ((rc = hip_ok(h, hipEventRecord(e0, h->stream)))) break;
+    if ((rc = tlm_step(h, ts, steps_per_candidate))) break;
+    ts += steps_per_candidate;
+    if ((rc = hip_ok(h, hipEventRecord(e1, h->stream)))) break;
+    if ((rc = hip_ok(h, hipEventSynchronize(e1)))) break;
+    float ms = 0.f;
+    if ((rc = hip_ok(h, hipEventElapsedTime(&ms, e0, e1)))) break;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = (int)k;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  h->sched_variant = cand[best].v;
+  h->sched_by = cand[best].rows;
+  h->sched_zc = cand[best].zc;
+  h->sched_xcd = cand[best].xcd;
+  if (steps_done) *steps_done = ts - first_timestep;
+  return TLM_OK;
+}
+
 int tlm_set_small_to_zero(tlm_handle* h, int32_t n_small_to_zero, double small_to_zero_value) {
   if (!h) return TLM_ERR_ARG;
   if (n_small_to_zero < 0 || !(small_to_zero_value >= 0.0)) return fail(h, TLM_ERR_ARG, "bad small_to_zero data");

@@ -241,6 +241,15 @@ int tlm_read_dft(tlm_handle* h, double* re_im);
 
 /* ---- schedule and measurement (no counterpart in the reference) ---------- */
 
+/* Autotuner of the bulk schedule (no counterpart in the reference): runs
+ * steps first_timestep .. first_timestep + *steps_done - 1 of the solve,
+ * each candidate schedule (line kernel with each XCD mapping, tile kernels
+ * of 1/2/4 rows, vector kernel, z-marching kernel) for steps_per_candidate
+ * timed steps after one untimed step, and keeps the fastest.  Every
+ * schedule gives the same results, so these are ordinary steps of the
+ * solve. */
+int tlm_autotune(tlm_handle* h, int32_t first_timestep, int32_t steps_per_candidate, int32_t* steps_done);
+
 /* TLM_set_small_to_zero (TLM_SOLVER/TLM_set_small_to_zero.F90:34-68, gated
  * at run_TLM.F90:236-238; packet read_small_to_zero.F90:55-59): at every
  * step whose number is a multiple of n_small_to_zero, link voltages with

@@ -1343,6 +1343,14 @@ int oracle_set_schedule(oracle_handle* h, int32_t variant, int32_t rows, int32_t
 }
 int oracle_set_profiling(oracle_handle* h, int32_t enable) { (void)h; (void)enable; return TLM_OK; }
 
+/* The oracle has one schedule: the autotuner just steps. */
+int oracle_autotune(oracle_handle* h, int32_t first_timestep, int32_t steps_per_candidate, int32_t* steps_done) {
+  int n = (8 + (h->ny % 8 == 0)) * (steps_per_candidate + 1);  /* as many as the engine's candidates */
+  int rc = oracle_step(h, first_timestep, n);
+  if (!rc && steps_done) *steps_done = n;
+  return rc;
+}
+
 int oracle_set_small_to_zero(oracle_handle* h, int32_t n_small_to_zero, double small_to_zero_value) {
   if (n_small_to_zero < 0 || !(small_to_zero_value >= 0.0)) return fail(h, TLM_ERR_ARG, "bad small_to_zero data");
   h->stz_n = n_small_to_zero;

@@ -187,3 +187,21 @@ def test_small_to_zero_bitwise(name, oracle_lib):
     g.set_small_to_zero(7, 1e-5)
     assert np.array_equal(run(o, t, steps), run(g, t, steps))
     assert np.array_equal(o.download_links()[:, 1:-1, 1:-1], g.download_links()[:, 1:-1, 1:-1])
+
+
+@pytest.mark.parametrize("name", ["pml_sphere_64", "special_mix"])
+def test_autotune_keeps_results(name, oracle_lib):
+    """The autotuner's candidate schedules are ordinary steps: the series
+    and links match the oracle bit for bit (plain form)."""
+    from oracle import oracle
+
+    t = build_tables(CASES[name]())
+    steps = min(t.n_timesteps, 50)
+    o = oracle.handle(t, output_capacity=steps)
+    g = engine(t, FORM_PLAIN_FP64)
+    done = g.autotune(1, 2)
+    assert done == o.autotune(1, 2) and 0 < done < steps
+    g.step(done + 1, steps - done)
+    o.step(done + 1, steps - done)
+    assert np.array_equal(o.read_cell_outputs()[0], g.read_cell_outputs()[0])
+    assert np.array_equal(o.download_links()[:, 1:-1, 1:-1], g.download_links()[:, 1:-1, 1:-1])
