@@ -200,6 +200,44 @@ module tlm_gpu_iface
       integer(c_int32_t), intent(out) :: n_read, first_step
     end function tlm_read_cell_outputs
 
+    integer(c_int) function tlm_read_face_outputs(h, out, max_steps, n_read, first_step) &
+                                               bind(c, name='tlm_read_face_outputs')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(inout) :: out(*)
+      integer(c_int32_t), value :: max_steps
+      integer(c_int32_t), intent(out) :: n_read, first_step
+    end function tlm_read_face_outputs
+
+    integer(c_int) function tlm_set_small_to_zero(h, n_small_to_zero, small_to_zero_value) &
+                                               bind(c, name='tlm_set_small_to_zero')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n_small_to_zero
+      real(c_double), value :: small_to_zero_value
+    end function tlm_set_small_to_zero
+
+    integer(c_int) function tlm_set_dft_frequencies(h, n, f) bind(c, name='tlm_set_dft_frequencies')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n
+      real(c_double), intent(in) :: f(*)
+    end function tlm_set_dft_frequencies
+
+    integer(c_int) function tlm_read_dft(h, re_im) bind(c, name='tlm_read_dft')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(inout) :: re_im(*)
+    end function tlm_read_dft
+
+    integer(c_int) function tlm_autotune(h, first_timestep, steps_per_candidate, steps_done) &
+                                               bind(c, name='tlm_autotune')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: first_timestep, steps_per_candidate
+      integer(c_int32_t), intent(out) :: steps_done
+    end function tlm_autotune
+
   end interface
 
 contains

@@ -23,13 +23,25 @@ for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
     FILES[f"huygens_{c.lower()}_centre_tout"] = f"HUYGENS_SURFACE/PROBLEM_SPECIFICATION_FILES/{c}_centre.tout_ref"
 
 
+# The first lines of one probe-output data file, kept as text to pin the
+# .tout format byte for byte (header and (E16.7,I5,E16.7) records).
+TOUT_TEXT = ("TEM_PML_TEST/PROBLEM_SPECIFICATION_FILES/Ey.tout_ref", "tem_pml_ey_head.tout", 400)
+
+
 def main(root):
     out = {}
+    here = os.path.dirname(os.path.abspath(__file__))
     for key, rel in FILES.items():
         out[key] = np.loadtxt(os.path.join(root, rel), comments="#")
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_goldens.npz")
+    path = os.path.join(here, "reference_goldens.npz")
     np.savez_compressed(path, **out)
     print(path, {k: v.shape for k, v in out.items()})
+    rel, name, n = TOUT_TEXT
+    with open(os.path.join(root, rel)) as f:
+        lines = [next(f) for _ in range(n)]
+    with open(os.path.join(here, name), "w") as f:
+        f.writelines(lines)
+    print(os.path.join(here, name), len(lines), "lines")
 
 
 if __name__ == "__main__":
