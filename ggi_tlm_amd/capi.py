@@ -61,7 +61,7 @@ SIGNATURES = {
     "set_output_capacity": (C.c_int, [_h, _i32]),
     "read_cell_outputs": (C.c_int, [_h, _p, _i32, C.POINTER(_i32), C.POINTER(_i32)]),
     "read_dft": (C.c_int, [_h, _p]),
-    "set_schedule": (C.c_int, [_h, _i32, _i32, _i32, _i32]),
+    "set_schedule": (C.c_int, [_h, _i32, _i32, _i32, _i32, _i32]),
     "set_profiling": (C.c_int, [_h, _i32]),
     "set_small_to_zero": (C.c_int, [_h, _i32, C.c_double]),
     "autotune": (C.c_int, [_h, _i32, _i32, C.POINTER(_i32)]),
@@ -269,8 +269,9 @@ class Handle:
         self._check(self.lib.fn["read_dft"](self.h, _ptr(out)))
         return out[:npnt, :nfreq, 0] + 1j * out[:npnt, :nfreq, 1]
 
-    def set_schedule(self, variant: int, rows: int, zc: int, xcd_remap: int = 0):
-        self._check(self.lib.fn["set_schedule"](self.h, int(variant), int(rows), int(zc), int(xcd_remap)))
+    def set_schedule(self, variant: int, rows: int, zc: int, xcd_remap: int = 0, mixed: int = -1):
+        self._check(self.lib.fn["set_schedule"](self.h, int(variant), int(rows), int(zc), int(xcd_remap),
+                                                int(mixed)))
 
     def autotune(self, first_timestep: int, steps_per_candidate: int) -> int:
         """Run candidate bulk schedules on the next steps and keep the

@@ -72,6 +72,12 @@ struct tlm_handle {
   int stz_n = 0;        // TLM_set_small_to_zero every stz_n steps (0: off)
   double stz_value = 0.0;
   long long n_mixed_cells = 0;
+  int mixed_variant = -1;                 // 0: lane slots, 1: LDS-staged, -1: per form
+  const int32_t* d_lds_lines = nullptr;   // LDS-staged mixed-line workgroups
+  const int32_t* d_lds_fd = nullptr;
+  const int32_t* d_lds_slot_p = nullptr;
+  const int32_t* d_lds_slot_i = nullptr;
+  int n_lds_wg = 0;
   const int32_t* d_slot_cell = nullptr;  // mixed-line workgroup slots
   const int32_t* d_slot_idx = nullptr;
   const int32_t* d_gen_cell = nullptr;

@@ -857,6 +857,9 @@ __global__ __launch_bounds__(256) void k_bulk_z(Dev d, const T* __restrict__ Lin
 }
 
 // PML cells: scatter_PML.F90 stages 1-13.
+template <bool PLAIN>
+__device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int32_t cw);
+
 template <typename T, bool PLAIN, bool PULL>
 __device__ inline void pml_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                   const PullArgs& a, int slot, int i) {
@@ -875,6 +878,14 @@ __device__ inline void pml_update(const Dev& d, const T* __restrict__ Lin, T* __
     for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
   }
   flush_small(v, a.small);
+  pml_node<PLAIN>(d, v, slot, i, cw);
+  store_cell(Lout, d.P, c, v);
+}
+
+// scatter_PML.F90:22-313 stages 1-13 on the incident voltages v of PML list
+// entry i (cw: its packed cell word), in place.
+template <bool PLAIN>
+__device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int32_t cw) {
   const double Z0 = kZ0, dl = d.dl;
   const tlm_pml_parameter& Pp = d.pml_params[d.pml_par[i]];
   const double ax = Pp.ax, ay = Pp.ay, az = Pp.az;
@@ -1025,7 +1036,6 @@ __device__ inline void pml_update(const Dev& d, const T* __restrict__ Lin, T* __
   v[Vzny] = v[Vzny] * exp_z; v[Vzpy] = v[Vzpy] * exp_z;
   v[Vxnz] = v[Vxnz] * exp_x; v[Vxpz] = v[Vxpz] * exp_x;
   v[Vynz] = v[Vynz] * exp_y; v[Vypz] = v[Vypz] * exp_y;
-  store_cell(Lout, d.P, c, v);
 }
 
 template <typename T, bool PLAIN, bool PULL>
@@ -1536,6 +1546,87 @@ __global__ __launch_bounds__(256) void k_mixed_lines(Dev d, const T* __restrict_
   if (cls == 1) pml_update<T, PLAIN, PULL>(d, Lin, Lout, a, slot, i);
   else if (cls == 2) material_update<T, PLAIN, PULL>(d, Lin, Lout, a, i);
   else generic_update<T, PLAIN, PULL>(d, Lin, Lout, a, timestep, slot, i);
+}
+
+// Node updates on incident voltages already in v (no loads or stores of
+// links): the material and generic cells' part of material_update and
+// generic_update.
+__device__ inline void material_node(const Dev& d, double v[12], int i) {
+  node_update<false>(d, v, d.mat_num[i], d.mat_filt[i], -1, -1, 0, 0);
+}
+
+template <bool PLAIN>
+__device__ inline void generic_node(const Dev& d, double v[12], int timestep, int slot, int i) {
+  if (d.gen_code[i] == 0) {
+    if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
+  } else {
+    node_update<true>(d, v, d.gen_mat[i], d.gen_filt[i], d.gen_exc[i], d.gen_out[i], timestep, slot);
+  }
+}
+
+// Mixed lines, LDS-staged.  A workgroup owns 256/L whole lines (256 cells).
+//   phase 1, cell order: lane t pulls cell t's incident voltages (plane
+//            loads coalesced along the lines) into LDS;
+//   phase 2, class order: lane t runs the node update of the t-th entry of
+//            the workgroup's class-sorted slot list (PML, generic, material,
+//            bulk) on its LDS values;
+//   phase 3, cell order: lane t stores cell t's scattered voltages, so every
+//            line is written whole and coalesced.
+// cell_fd: face descriptor of each cell position (-1 none; -2 no cell here:
+// padding, or a cell of another kernel).  slot_p: class << 16 | position,
+// or -1; slot_i: the class-list position of special cells.
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_mixed_lds(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                  PullArgs a, const int32_t* __restrict__ wg_lines,
+                                                  const int32_t* __restrict__ cell_fd,
+                                                  const int32_t* __restrict__ slot_p,
+                                                  const int32_t* __restrict__ slot_i, int timestep, int slot) {
+  __shared__ double sv[12 * 256];
+  const int t = threadIdx.x;
+  const long long base = (long long)blockIdx.x * 256;
+  const int L = d.line_len;
+  const int line = wg_lines[(long long)blockIdx.x * (256 / L) + t / L];
+  const int fd = cell_fd[base + t];
+  const long long c = (long long)line * L + (t % L);
+  const bool mine = fd != -2;
+  if (mine) {
+    int x, y, z;
+    cell_coords(d, c, x, y, z);
+    double v[12];
+    if (PULL) {
+      pull_cell<T, PLAIN>(d, Lin, c, x, y, z, fd, a, v);
+    } else {
+#pragma unroll
+      for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
+    }
+    flush_small(v, a.small);
+#pragma unroll
+    for (int k = 0; k < 12; k++) sv[k * 256 + t] = v[k];
+  }
+  __syncthreads();
+  const int sp = slot_p[base + t];
+  if (sp >= 0) {
+    const int p = sp & 0xffff;
+    const int cls = sp >> 16;
+    double v[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = sv[k * 256 + p];
+    if (cls == 0) {
+      if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
+    } else {
+      const int i = slot_i[base + t];
+      if (cls == 1) pml_node<PLAIN>(d, v, slot, i, d.pml_cell[i]);
+      else if (cls == 2) material_node(d, v, i);
+      else generic_node<PLAIN>(d, v, timestep, slot, i);
+    }
+#pragma unroll
+    for (int k = 0; k < 12; k++) sv[k * 256 + p] = v[k];
+  }
+  __syncthreads();
+  if (mine) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) Lout[(long long)k * d.P + c] = (T)sv[k * 256 + t];
+  }
 }
 
 // Connect only (no scatter): incident voltages into Lout, used to hand the

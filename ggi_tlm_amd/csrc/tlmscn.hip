@@ -238,6 +238,7 @@ int finalize(tlm_handle* h) {
   // j, j + nb, j + 2 nb, ... of the group (nb = bulk lanes), so that
   // neighbouring lanes touch neighbouring cells.
   constexpr int KB = scn::kMixedBulkPerLane;
+  std::vector<int32_t> lds_lines, lds_fd, lds_slot_p, lds_slot_i;
   std::vector<int32_t> slot_a;
   std::vector<std::vector<int32_t>> slot_b(KB > 1 ? KB - 1 : 1);
   {
@@ -318,6 +319,35 @@ int finalize(tlm_handle* h) {
       n_bulk += b;
     }
     if (l0 < mlines.size()) emit(l0, mlines.size());
+    // LDS-staged workgroups (k_mixed_lds): exactly 256/L consecutive mixed
+    // lines each; per cell position its face descriptor (-2: not this
+    // kernel's), and a class-sorted slot list of (class << 16 | position).
+    const int GL = 256 / L;
+    for (size_t g0 = 0; g0 < mlines.size(); g0 += GL) {
+      std::vector<std::pair<int, std::pair<int32_t, int32_t>>> sl;  // (rank, (cls<<16|p, idx))
+      for (int q = 0; q < GL; q++) {
+        const size_t qi = g0 + q;
+        const long long ln = qi < mlines.size() ? mlines[qi] : -1;
+        lds_lines.push_back((int32_t)ln);
+        for (int k = 0; k < L; k++) {
+          const int p = q * L + k;
+          const long long c = ln * L + k;
+          if (ln < 0 || c >= N || cls[c] == 4) {
+            lds_fd.push_back(-2);
+            continue;
+          }
+          const int k2 = cls[c];
+          const int32_t idx = cell_idx[line_first_idx[qi] + k];
+          lds_fd.push_back(k2 == 1 ? pr[idx].fdc : (k2 == 3 ? gr[idx].fdc : -1));
+          sl.push_back({rank_of[k2], {(int32_t)((k2 << 16) | p), k2 ? idx : 0}});
+        }
+      }
+      std::stable_sort(sl.begin(), sl.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+      for (int s2 = 0; s2 < 256; s2++) {
+        lds_slot_p.push_back(s2 < (int)sl.size() ? sl[s2].second.first : -1);
+        lds_slot_i.push_back(s2 < (int)sl.size() ? sl[s2].second.second : 0);
+      }
+    }
   }
   // flat per-class arrays in list order
   std::vector<int32_t> pml_cell, pml_fd;
@@ -432,6 +462,11 @@ int finalize(tlm_handle* h) {
   }
   d.line_len = L;
   if ((rc = dev_upload(h, line_info, &d.line_info))) return rc;
+  if ((rc = dev_upload(h, lds_lines, &h->d_lds_lines))) return rc;
+  if ((rc = dev_upload(h, lds_fd, &h->d_lds_fd))) return rc;
+  if ((rc = dev_upload(h, lds_slot_p, &h->d_lds_slot_p))) return rc;
+  if ((rc = dev_upload(h, lds_slot_i, &h->d_lds_slot_i))) return rc;
+  h->n_lds_wg = (int)(lds_slot_p.size() / 256);
   {
     std::vector<int32_t> sb;
     for (auto& v : slot_b) sb.insert(sb.end(), v.begin(), v.end());
@@ -674,7 +709,16 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
       }
     }
   }
-  if (h->n_mixed_blocks) {
+  const int mixed = h->mixed_variant >= 0 ? h->mixed_variant : (PLAIN ? 1 : 0);  // measured faster per form
+  if (mixed == 1 && h->n_lds_wg) {
+    ProfScope ps(h, TLM_KCLASS_MIXED);
+    if (pull)
+      hipLaunchKernelGGL((scn::k_mixed_lds<T, PLAIN, true>), dim3(h->n_lds_wg), dim3(256), 0, h->stream, d, Lin, Lout,
+                         a, h->d_lds_lines, h->d_lds_fd, h->d_lds_slot_p, h->d_lds_slot_i, timestep, slot);
+    else
+      hipLaunchKernelGGL((scn::k_mixed_lds<T, PLAIN, false>), dim3(h->n_lds_wg), dim3(256), 0, h->stream, d, Lin, Lout,
+                         a, h->d_lds_lines, h->d_lds_fd, h->d_lds_slot_p, h->d_lds_slot_i, timestep, slot);
+  } else if (h->n_mixed_blocks) {
     ProfScope ps(h, TLM_KCLASS_MIXED);
     dim3 g(h->n_mixed_blocks);
     if (pull)
@@ -1147,8 +1191,10 @@ int tlm_read_dft(tlm_handle* h, double* re_im) {
   return TLM_OK;
 }
 
-int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap) {
+int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap, int32_t mixed) {
   if (!h) return TLM_ERR_ARG;
+  if (mixed < -1 || mixed > 1) return fail(h, TLM_ERR_ARG, "bad mixed-line kernel");
+  h->mixed_variant = mixed;
   if (variant < 0 || variant > 3 || (rows != 1 && rows != 2 && rows != 4) || zc < 1)
     return fail(h, TLM_ERR_ARG, "bad bulk schedule");
   h->sched_variant = variant;
@@ -1163,10 +1209,10 @@ int tlm_autotune(tlm_handle* h, int32_t first_timestep, int32_t steps_per_candid
   if (!h || steps_per_candidate < 1 || first_timestep < 1) return TLM_ERR_ARG;
   int rc;
   if ((rc = finalize(h))) return rc;
-  struct Cand { int v, rows, zc, xcd; };
-  std::vector<Cand> cand = {{3, 1, 1, 0}, {3, 1, 1, 1}, {0, 1, 1, 0}, {0, 2, 1, 0}, {0, 4, 1, 0},
-                            {2, 1, 1, 0}, {2, 4, 1, 0}, {1, 4, 8, 0}};
-  if (h->ny % 8 == 0) cand.push_back({3, 1, 1, 2});
+  struct Cand { int v, rows, zc, xcd, mixed; };
+  std::vector<Cand> cand = {{3, 1, 1, 0, 1}, {3, 1, 1, 0, 0}, {3, 1, 1, 1, 1}, {0, 1, 1, 0, 1}, {0, 2, 1, 0, 1},
+                            {0, 4, 1, 0, 1}, {2, 1, 1, 0, 1}, {2, 4, 1, 0, 1}, {1, 4, 8, 0, 1}};
+  if (h->ny % 8 == 0) cand.push_back({3, 1, 1, 2, 1});
   const int total = (int)cand.size() * (steps_per_candidate + 1);
   if (first_timestep + total - 1 > h->g.n_timesteps)
     return fail(h, TLM_ERR_ARG, "not enough timesteps left for the autotuner");
@@ -1180,6 +1226,7 @@ int tlm_autotune(tlm_handle* h, int32_t first_timestep, int32_t steps_per_candid
     h->sched_by = cand[k].rows;
     h->sched_zc = cand[k].zc;
     h->sched_xcd = cand[k].xcd;
+    h->mixed_variant = cand[k].mixed;
     if ((rc = tlm_step(h, ts, 1))) break;  // first launch of this schedule, untimed
     ts += 1;
     if ((rc = hip_ok(h, hipEventRecord(e0, h->stream)))) break;
@@ -1201,6 +1248,7 @@ int tlm_autotune(tlm_handle* h, int32_t first_timestep, int32_t steps_per_candid
   h->sched_by = cand[best].rows;
   h->sched_zc = cand[best].zc;
   h->sched_xcd = cand[best].xcd;
+  h->mixed_variant = cand[best].mixed;
   if (steps_done) *steps_done = ts - first_timestep;
   return TLM_OK;
 }

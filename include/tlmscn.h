@@ -265,10 +265,14 @@ int tlm_set_small_to_zero(tlm_handle* h, int32_t n_small_to_zero, double small_t
  * rows); xcd_remap = 1: give each XCD a contiguous tile range; 2 (line
  * kernel only, ny divisible by 8): XCD k takes the rows of the k-th y-band
  * in z-major order, so the y-1 and z-1 neighbour rows are in its L2.
- * Any schedule gives identical results; it only changes speed.  The
- * autotuner (ggi_tlm_amd/autotune.py) picks one per grid shape. */
-int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc,
-                     int32_t xcd_remap);
+ * mixed: mixed-line kernel, 1 = LDS-staged cell-order loads and stores
+ * around class-sorted node updates (default in the plain form), 0 =
+ * class-sorted lanes with four bulk cells per lane (default in the
+ * precision form), -1 = the default of the form.
+ * Any schedule gives identical results; it only changes speed
+ * (tlm_autotune picks one by timing). */
+int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap,
+                     int32_t mixed);
 
 /* Kernel classes timed by the profiling hooks. */
 #define TLM_KCLASS_BULK 0

@@ -1337,15 +1337,16 @@ int oracle_read_face_outputs(oracle_handle* h, double* out, int32_t max_steps, i
 
 /* Schedule and measurement hooks of the engine ABI; the oracle has no
  * kernels to schedule or time. */
-int oracle_set_schedule(oracle_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap) {
-  (void)h; (void)variant; (void)rows; (void)zc; (void)xcd_remap;
+int oracle_set_schedule(oracle_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap,
+                        int32_t mixed) {
+  (void)h; (void)variant; (void)rows; (void)zc; (void)xcd_remap; (void)mixed;
   return TLM_OK;
 }
 int oracle_set_profiling(oracle_handle* h, int32_t enable) { (void)h; (void)enable; return TLM_OK; }
 
 /* The oracle has one schedule: the autotuner just steps. */
 int oracle_autotune(oracle_handle* h, int32_t first_timestep, int32_t steps_per_candidate, int32_t* steps_done) {
-  int n = (8 + (h->ny % 8 == 0)) * (steps_per_candidate + 1);  /* as many as the engine's candidates */
+  int n = (9 + (h->ny % 8 == 0)) * (steps_per_candidate + 1);  /* as many as the engine's candidates */
   int rc = oracle_step(h, first_timestep, n);
   if (!rc && steps_done) *steps_done = n;
   return rc;

@@ -59,7 +59,8 @@ int oracle_set_output_capacity(oracle_handle* h, int32_t steps);
 int oracle_read_cell_outputs(oracle_handle* h, double* out, int32_t max_steps,
                              int32_t* n_read, int32_t* first_step);
 int oracle_read_dft(oracle_handle* h, double* re_im);
-int oracle_set_schedule(oracle_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap);
+int oracle_set_schedule(oracle_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap,
+                        int32_t mixed);
 int oracle_set_profiling(oracle_handle* h, int32_t enable);
 int oracle_autotune(oracle_handle* h, int32_t first_timestep, int32_t steps_per_candidate, int32_t* steps_done);
 int oracle_set_small_to_zero(oracle_handle* h, int32_t n_small_to_zero, double small_to_zero_value);

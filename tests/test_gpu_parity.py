@@ -130,7 +130,8 @@ def test_precision_form(name, oracle_lib):
 
 
 @pytest.mark.parametrize("sched", [(0, 1, 1, 0), (0, 2, 1, 1), (1, 1, 4, 1), (1, 4, 8, 0), (1, 2, 32, 1),
-                                   (2, 4, 1, 0), (2, 1, 1, 1), (3, 1, 1, 0), (3, 1, 1, 1), (3, 1, 1, 2)])
+                                   (2, 4, 1, 0), (2, 1, 1, 1), (3, 1, 1, 0), (3, 1, 1, 1), (3, 1, 1, 2),
+                                   (3, 1, 1, 0, 0), (0, 4, 1, 0, 1)])
 def test_schedules_bitwise(sched, oracle_lib):
     """Every bulk schedule gives the oracle's result bit for bit."""
     from oracle import oracle
