@@ -29,6 +29,10 @@ struct BatchSolve {
   const int32_t* slot_a;     // mixed-line workgroup slots
   const int32_t* slot_b;
   long long n_slots;
+  const int32_t* lds_lines;  // LDS-staged mixed-line workgroups
+  const int32_t* lds_fd;
+  const int32_t* lds_slot_p;
+  const int32_t* lds_slot_i;
   int n_pure_pml, n_pure_mat, n_pure_gen, n_pm, n_huy_coded;
   int cur;                   // plane set holding the last step's result
   int ts, first, last;       // next timestep, first and last timestep
@@ -174,6 +178,26 @@ __global__ __launch_bounds__(256) void kb_mixed(const BatchSolve* __restrict__ S
   else mixed_lane<T, PLAIN, false>(s, Lin, Lout, a, slot, si);
 }
 
+template <typename T, bool PLAIN>
+__global__ __launch_bounds__(256) void kb_mixed_lds(const BatchSolve* __restrict__ S, const int2* __restrict__ tab) {
+  __shared__ double sv[12 * 256];
+  const int2 e = tab[blockIdx.x];
+  const BatchSolve& s = S[e.x];
+  if (s.ts > s.last) return;  // the whole workgroup leaves together
+  PullArgs a;
+  bool pull;
+  int slot;
+  batch_pull_args<T, PLAIN>(s, a, pull, slot);
+  const T* Lin = (const T*)s.planes[s.cur];
+  T* Lout = (T*)s.planes[s.cur ^ 1];
+  if (pull)
+    scn::mixed_lds_body<T, PLAIN, true>(s.d, Lin, Lout, a, e.y, s.lds_lines, s.lds_fd, s.lds_slot_p, s.lds_slot_i,
+                                        s.ts, slot, sv);
+  else
+    scn::mixed_lds_body<T, PLAIN, false>(s.d, Lin, Lout, a, e.y, s.lds_lines, s.lds_fd, s.lds_slot_p, s.lds_slot_i,
+                                         s.ts, slot, sv);
+}
+
 // One class list of a solve: 0 PML, 1 material, 2 generic, 3 PML holding a material.
 template <typename T, bool PLAIN, int CLS>
 __global__ __launch_bounds__(256) void kb_list(const BatchSolve* __restrict__ S, const int2* __restrict__ tab) {
@@ -248,7 +272,7 @@ void launch_batch_step(tlm_batch* b) {
   };
   go(tlm_batch::T_HUY, kb_huygens);
   go(tlm_batch::T_LINES, kb_lines<T, PLAIN>);
-  go(tlm_batch::T_MIXED, kb_mixed<T, PLAIN>);
+  go(tlm_batch::T_MIXED, PLAIN ? kb_mixed_lds<T, PLAIN> : kb_mixed<T, PLAIN>);  // as the single-solve defaults
   go(tlm_batch::T_PML, kb_list<T, PLAIN, 0>);
   go(tlm_batch::T_MAT, kb_list<T, PLAIN, 1>);
   go(tlm_batch::T_GEN, kb_list<T, PLAIN, 2>);
@@ -284,6 +308,10 @@ int rebuild(tlm_batch* b) {
     s.slot_a = h->d_slot_cell;
     s.slot_b = h->d_slot_idx;
     s.n_slots = (long long)h->n_mixed_blocks * 256;
+    s.lds_lines = h->d_lds_lines;
+    s.lds_fd = h->d_lds_fd;
+    s.lds_slot_p = h->d_lds_slot_p;
+    s.lds_slot_i = h->d_lds_slot_i;
     s.n_pure_pml = h->n_pure_pml;
     s.n_pure_mat = h->n_pure_mat;
     s.n_pure_gen = h->n_pure_gen;
@@ -308,7 +336,8 @@ int rebuild(tlm_batch* b) {
     const int nbx = (h->nx + 255) / 256;
     const long long nl = (long long)nbx * h->ny * h->nz;
     for (long long k = 0; k < nl; k++) tab[tlm_batch::T_LINES].push_back(make_int2(i, (int)k));
-    for (int k = 0; k < h->n_mixed_blocks; k++) tab[tlm_batch::T_MIXED].push_back(make_int2(i, k));
+    const int n_mixed_wg = b->form == TLM_FORM_PLAIN_FP64 ? h->n_lds_wg : h->n_mixed_blocks;
+    for (int k = 0; k < n_mixed_wg; k++) tab[tlm_batch::T_MIXED].push_back(make_int2(i, k));
     auto add = [&](int t, long long cnt) {
       for (long long k = 0; k < (cnt + 255) / 256; k++) tab[t].push_back(make_int2(i, (int)k));
     };

@@ -1576,16 +1576,14 @@ __device__ inline void generic_node(const Dev& d, double v[12], int timestep, in
 // padding, or a cell of another kernel).  slot_p: class << 16 | position,
 // or -1; slot_i: the class-list position of special cells.
 template <typename T, bool PLAIN, bool PULL>
-__global__ __launch_bounds__(256) void k_mixed_lds(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                  PullArgs a, const int32_t* __restrict__ wg_lines,
-                                                  const int32_t* __restrict__ cell_fd,
-                                                  const int32_t* __restrict__ slot_p,
-                                                  const int32_t* __restrict__ slot_i, int timestep, int slot) {
-  __shared__ double sv[12 * 256];
+__device__ inline void mixed_lds_body(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                      const PullArgs& a, long long wg, const int32_t* __restrict__ wg_lines,
+                                      const int32_t* __restrict__ cell_fd, const int32_t* __restrict__ slot_p,
+                                      const int32_t* __restrict__ slot_i, int timestep, int slot, double* sv) {
   const int t = threadIdx.x;
-  const long long base = (long long)blockIdx.x * 256;
+  const long long base = wg * 256;
   const int L = d.line_len;
-  const int line = wg_lines[(long long)blockIdx.x * (256 / L) + t / L];
+  const int line = wg_lines[wg * (256 / L) + t / L];
   const int fd = cell_fd[base + t];
   const long long c = (long long)line * L + (t % L);
   const bool mine = fd != -2;
@@ -1627,6 +1625,16 @@ __global__ __launch_bounds__(256) void k_mixed_lds(Dev d, const T* __restrict__ 
 #pragma unroll
     for (int k = 0; k < 12; k++) Lout[(long long)k * d.P + c] = (T)sv[k * 256 + t];
   }
+}
+
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_mixed_lds(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                  PullArgs a, const int32_t* __restrict__ wg_lines,
+                                                  const int32_t* __restrict__ cell_fd,
+                                                  const int32_t* __restrict__ slot_p,
+                                                  const int32_t* __restrict__ slot_i, int timestep, int slot) {
+  __shared__ double sv[12 * 256];
+  mixed_lds_body<T, PLAIN, PULL>(d, Lin, Lout, a, blockIdx.x, wg_lines, cell_fd, slot_p, slot_i, timestep, slot, sv);
 }
 
 // Connect only (no scatter): incident voltages into Lout, used to hand the
