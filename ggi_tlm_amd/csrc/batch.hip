@@ -179,8 +179,9 @@ __global__ __launch_bounds__(256) void kb_mixed(const BatchSolve* __restrict__ S
 }
 
 template <typename T, bool PLAIN>
-__global__ __launch_bounds__(256) void kb_mixed_lds(const BatchSolve* __restrict__ S, const int2* __restrict__ tab) {
-  __shared__ double sv[12 * 256];
+__global__ __launch_bounds__(scn::kLdsCells) void kb_mixed_lds(const BatchSolve* __restrict__ S,
+                                                              const int2* __restrict__ tab) {
+  __shared__ double sv[12 * scn::kLdsCells];
   const int2 e = tab[blockIdx.x];
   const BatchSolve& s = S[e.x];
   if (s.ts > s.last) return;  // the whole workgroup leaves together
@@ -267,12 +268,13 @@ template <typename T, bool PLAIN>
 void launch_batch_step(tlm_batch* b) {
   const hipStream_t st = b->stream;
   BatchSolve* S = b->d_solves;
-  auto go = [&](int t, void (*k)(const BatchSolve*, const int2*)) {
-    if (b->n_tab[t]) hipLaunchKernelGGL(k, dim3(b->n_tab[t]), dim3(256), 0, st, S, b->d_tab[t]);
+  auto go = [&](int t, void (*k)(const BatchSolve*, const int2*), int threads = 256) {
+    if (b->n_tab[t]) hipLaunchKernelGGL(k, dim3(b->n_tab[t]), dim3(threads), 0, st, S, b->d_tab[t]);
   };
   go(tlm_batch::T_HUY, kb_huygens);
   go(tlm_batch::T_LINES, kb_lines<T, PLAIN>);
-  go(tlm_batch::T_MIXED, PLAIN ? kb_mixed_lds<T, PLAIN> : kb_mixed<T, PLAIN>);  // as the single-solve defaults
+  if (PLAIN) go(tlm_batch::T_MIXED, kb_mixed_lds<T, PLAIN>, scn::kLdsCells);  // as the single-solve defaults
+  else go(tlm_batch::T_MIXED, kb_mixed<T, PLAIN>);
   go(tlm_batch::T_PML, kb_list<T, PLAIN, 0>);
   go(tlm_batch::T_MAT, kb_list<T, PLAIN, 1>);
   go(tlm_batch::T_GEN, kb_list<T, PLAIN, 2>);

@@ -1564,7 +1564,13 @@ __device__ inline void generic_node(const Dev& d, double v[12], int timestep, in
   }
 }
 
-// Mixed lines, LDS-staged.  A workgroup owns 256/L whole lines (256 cells).
+// Cells (lanes) per workgroup of the LDS-staged mixed-line kernel.
+#ifndef TLM_LDS_CELLS
+#define TLM_LDS_CELLS 256
+#endif
+constexpr int kLdsCells = TLM_LDS_CELLS;
+
+// Mixed lines, LDS-staged.  A workgroup owns kLdsCells/L whole lines.
 //   phase 1, cell order: lane t pulls cell t's incident voltages (plane
 //            loads coalesced along the lines) into LDS;
 //   phase 2, class order: lane t runs the node update of the t-th entry of
@@ -1580,10 +1586,11 @@ __device__ inline void mixed_lds_body(const Dev& d, const T* __restrict__ Lin, T
                                       const PullArgs& a, long long wg, const int32_t* __restrict__ wg_lines,
                                       const int32_t* __restrict__ cell_fd, const int32_t* __restrict__ slot_p,
                                       const int32_t* __restrict__ slot_i, int timestep, int slot, double* sv) {
+  constexpr int C = kLdsCells;
   const int t = threadIdx.x;
-  const long long base = wg * 256;
+  const long long base = wg * C;
   const int L = d.line_len;
-  const int line = wg_lines[wg * (256 / L) + t / L];
+  const int line = wg_lines[wg * (C / L) + t / L];
   const int fd = cell_fd[base + t];
   const long long c = (long long)line * L + (t % L);
   const bool mine = fd != -2;
@@ -1599,7 +1606,7 @@ __device__ inline void mixed_lds_body(const Dev& d, const T* __restrict__ Lin, T
     }
     flush_small(v, a.small);
 #pragma unroll
-    for (int k = 0; k < 12; k++) sv[k * 256 + t] = v[k];
+    for (int k = 0; k < 12; k++) sv[k * C + t] = v[k];
   }
   __syncthreads();
   const int sp = slot_p[base + t];
@@ -1608,7 +1615,7 @@ __device__ inline void mixed_lds_body(const Dev& d, const T* __restrict__ Lin, T
     const int cls = sp >> 16;
     double v[12];
 #pragma unroll
-    for (int k = 0; k < 12; k++) v[k] = sv[k * 256 + p];
+    for (int k = 0; k < 12; k++) v[k] = sv[k * C + p];
     if (cls == 0) {
       if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
     } else {
@@ -1618,22 +1625,22 @@ __device__ inline void mixed_lds_body(const Dev& d, const T* __restrict__ Lin, T
       else generic_node<PLAIN>(d, v, timestep, slot, i);
     }
 #pragma unroll
-    for (int k = 0; k < 12; k++) sv[k * 256 + p] = v[k];
+    for (int k = 0; k < 12; k++) sv[k * C + p] = v[k];
   }
   __syncthreads();
   if (mine) {
 #pragma unroll
-    for (int k = 0; k < 12; k++) Lout[(long long)k * d.P + c] = (T)sv[k * 256 + t];
+    for (int k = 0; k < 12; k++) Lout[(long long)k * d.P + c] = (T)sv[k * C + t];
   }
 }
 
 template <typename T, bool PLAIN, bool PULL>
-__global__ __launch_bounds__(256) void k_mixed_lds(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+__global__ __launch_bounds__(kLdsCells) void k_mixed_lds(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                                   PullArgs a, const int32_t* __restrict__ wg_lines,
                                                   const int32_t* __restrict__ cell_fd,
                                                   const int32_t* __restrict__ slot_p,
                                                   const int32_t* __restrict__ slot_i, int timestep, int slot) {
-  __shared__ double sv[12 * 256];
+  __shared__ double sv[12 * kLdsCells];
   mixed_lds_body<T, PLAIN, PULL>(d, Lin, Lout, a, blockIdx.x, wg_lines, cell_fd, slot_p, slot_i, timestep, slot, sv);
 }
 

@@ -322,7 +322,8 @@ int finalize(tlm_handle* h) {
     // LDS-staged workgroups (k_mixed_lds): exactly 256/L consecutive mixed
     // lines each; per cell position its face descriptor (-2: not this
     // kernel's), and a class-sorted slot list of (class << 16 | position).
-    const int GL = 256 / L;
+    constexpr int LC = scn::kLdsCells;
+    const int GL = LC / L;
     for (size_t g0 = 0; g0 < mlines.size(); g0 += GL) {
       std::vector<std::pair<int, std::pair<int32_t, int32_t>>> sl;  // (rank, (cls<<16|p, idx))
       for (int q = 0; q < GL; q++) {
@@ -343,7 +344,7 @@ int finalize(tlm_handle* h) {
         }
       }
       std::stable_sort(sl.begin(), sl.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
-      for (int s2 = 0; s2 < 256; s2++) {
+      for (int s2 = 0; s2 < LC; s2++) {
         lds_slot_p.push_back(s2 < (int)sl.size() ? sl[s2].second.first : -1);
         lds_slot_i.push_back(s2 < (int)sl.size() ? sl[s2].second.second : 0);
       }
@@ -466,7 +467,7 @@ int finalize(tlm_handle* h) {
   if ((rc = dev_upload(h, lds_fd, &h->d_lds_fd))) return rc;
   if ((rc = dev_upload(h, lds_slot_p, &h->d_lds_slot_p))) return rc;
   if ((rc = dev_upload(h, lds_slot_i, &h->d_lds_slot_i))) return rc;
-  h->n_lds_wg = (int)(lds_slot_p.size() / 256);
+  h->n_lds_wg = (int)(lds_slot_p.size() / scn::kLdsCells);
   {
     std::vector<int32_t> sb;
     for (auto& v : slot_b) sb.insert(sb.end(), v.begin(), v.end());
@@ -713,10 +714,10 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
   if (mixed == 1 && h->n_lds_wg) {
     ProfScope ps(h, TLM_KCLASS_MIXED);
     if (pull)
-      hipLaunchKernelGGL((scn::k_mixed_lds<T, PLAIN, true>), dim3(h->n_lds_wg), dim3(256), 0, h->stream, d, Lin, Lout,
+      hipLaunchKernelGGL((scn::k_mixed_lds<T, PLAIN, true>), dim3(h->n_lds_wg), dim3(scn::kLdsCells), 0, h->stream, d, Lin, Lout,
                          a, h->d_lds_lines, h->d_lds_fd, h->d_lds_slot_p, h->d_lds_slot_i, timestep, slot);
     else
-      hipLaunchKernelGGL((scn::k_mixed_lds<T, PLAIN, false>), dim3(h->n_lds_wg), dim3(256), 0, h->stream, d, Lin, Lout,
+      hipLaunchKernelGGL((scn::k_mixed_lds<T, PLAIN, false>), dim3(h->n_lds_wg), dim3(scn::kLdsCells), 0, h->stream, d, Lin, Lout,
                          a, h->d_lds_lines, h->d_lds_fd, h->d_lds_slot_p, h->d_lds_slot_i, timestep, slot);
   } else if (h->n_mixed_blocks) {
     ProfScope ps(h, TLM_KCLASS_MIXED);
