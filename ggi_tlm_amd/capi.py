@@ -68,7 +68,13 @@ SIGNATURES = {
     "read_face_outputs": (C.c_int, [_h, _p, _i32, _p, _p]),
     "read_kernel_times": (C.c_int, [_h, _p, _p]),
     "kernel_bytes": (C.c_int, [_h, _p, _p]),
+    "set_layout": (C.c_int, [_h, _i32]),
+    "layout_segments": (C.c_int, [_h, C.POINTER(_i32), _p]),
 }
+
+LAYOUT_X_SEGMENTS = 0
+LAYOUT_ROWS = 1
+MAX_SEGMENTS = 16
 
 KERNEL_CLASSES = ("bulk", "pml", "generic", "dft", "material", "mixed", "pml_material")
 
@@ -268,6 +274,17 @@ class Handle:
         out = np.zeros((max(npnt, 1), max(nfreq, 1), 2), dtype=np.float64)
         self._check(self.lib.fn["read_dft"](self.h, _ptr(out)))
         return out[:npnt, :nfreq, 0] + 1j * out[:npnt, :nfreq, 1]
+
+    def set_layout(self, layout: int):
+        """Storage order of the link planes (before the first step)."""
+        self._check(self.lib.fn["set_layout"](self.h, int(layout)))
+
+    def layout_segments(self) -> list:
+        """x-segment boundaries x0[0..n] of the layout in use."""
+        n = C.c_int32(0)
+        x0 = np.zeros(MAX_SEGMENTS + 1, dtype=np.int32)
+        self._check(self.lib.fn["layout_segments"](self.h, C.byref(n), _ptr(x0)))
+        return [int(v) for v in x0[: n.value + 1]]
 
     def set_schedule(self, variant: int, rows: int, zc: int, xcd_remap: int = 0, mixed: int = -1):
         self._check(self.lib.fn["set_schedule"](self.h, int(variant), int(rows), int(zc), int(xcd_remap),

@@ -93,7 +93,7 @@ int bfail(tlm_batch* b, int code, const std::string& msg) {
 // link buffers, class tables, PML and filter states, output series.
 int64_t estimate_bytes(const tlm_handle* h, int n_steps) {
   const int64_t ls = link_size(h);
-  int64_t b = 2 * 12 * (int64_t)h->P * ls;               // link planes
+  int64_t b = 2 * 12 * ((int64_t)h->N + 64 * TLM_MAX_SEGMENTS) * ls;  // link planes, segment padding
   b += (int64_t)h->N / 8 + (int64_t)h->N / 8;            // mask, line table
   const int64_t n_pml = (int64_t)h->pml_cell_param.size();
   b += n_pml * (8 * (scn::PS_N + 3) + 12);               // PML state and lists
@@ -121,23 +121,15 @@ __global__ __launch_bounds__(256) void kb_lines(const BatchSolve* __restrict__ S
   const int2 e = tab[blockIdx.x];
   const BatchSolve& s = S[e.x];
   if (s.ts > s.last) return;
-  const Dev& d = s.d;
-  const int nbx = (d.nx + 255) >> 8;
-  const int row = e.y / nbx;
-  const int x = (e.y - row * nbx) * 256 + threadIdx.x;
-  if (x >= d.nx) return;
-  const long long c = x + (long long)d.nx * row;
-  if (d.line_info[c / d.line_len] != 0u) return;
-  const int z = row / d.ny;
-  const int y = row - z * d.ny;
   PullArgs a;
   bool pull;
   int slot;
   batch_pull_args<T, PLAIN>(s, a, pull, slot);
   const T* Lin = (const T*)s.planes[s.cur];
   T* Lout = (T*)s.planes[s.cur ^ 1];
-  if (pull) scn::bulk_update<T, PLAIN, true>(d, Lin, Lout, a, c, x, y, z);
-  else scn::bulk_update<T, PLAIN, false>(d, Lin, Lout, a, c, x, y, z);
+  const long long c = (long long)e.y * 256 + threadIdx.x;
+  if (pull) scn::bulk_line_cell<T, PLAIN, true>(s.d, Lin, Lout, a, c);
+  else scn::bulk_line_cell<T, PLAIN, false>(s.d, Lin, Lout, a, c);
 }
 
 template <typename T, bool PLAIN, bool PULL>
@@ -335,8 +327,7 @@ int rebuild(tlm_batch* b) {
     s.dft = h->d_dft;
     s.dt = h->g.dt;
     if (e.ts > e.last) continue;  // finished, awaiting collection
-    const int nbx = (h->nx + 255) / 256;
-    const long long nl = (long long)nbx * h->ny * h->nz;
+    const long long nl = (h->P + 255) / 256;
     for (long long k = 0; k < nl; k++) tab[tlm_batch::T_LINES].push_back(make_int2(i, (int)k));
     const int n_mixed_wg = b->form == TLM_FORM_PLAIN_FP64 ? h->n_lds_wg : h->n_mixed_blocks;
     for (int k = 0; k < n_mixed_wg; k++) tab[tlm_batch::T_MIXED].push_back(make_int2(i, k));

@@ -92,6 +92,7 @@ struct tlm_handle {
 
   // bulk schedule (tlm_set_schedule)
   int sched_variant = -1, sched_by = 4, sched_zc = 16, sched_xcd = 0;  // -1: per-form default
+  int layout = TLM_LAYOUT_X_SEGMENTS;     // tlm_set_layout
 
   // state machine
   Kind kind = Kind::INCIDENT;

@@ -1,8 +1,13 @@
 // scn_kernels.hpp -- gfx950 kernels of the SCN scatter/connect hot path.
 //
 // Layout in HBM: twelve structure-of-arrays link planes per buffer,
-// plane p holding port p+1 of every cell, cell index c = x + nx*(y + ny*z)
-// (x fastest, as the reference's scan order).  Two buffers ping-pong: a
+// plane p holding port p+1 of every cell.  Cells are stored by x-segment:
+// the x range is cut into segments [x0_s, x1_s) where the cell classes of
+// whole x-planes change (PML slabs, source planes, walls), and segment s
+// holds its cells x-fastest with row width w_s = x1_s - x0_s from storage
+// index base_s: c = base_s + (x - x0_s) + w_s*(y + ny*z).  A PML slab or a
+// source plane thus fills 128-byte lines by itself.  One segment is the
+// reference's scan order c = x + nx*(y + ny*z).  Two buffers ping-pong: a
 // step reads the previous step's scattered voltages from one buffer, pulls
 // each cell's incident voltages across its six faces (the reference's
 // connect and outer_boundary, SRC/TLM_SOLVER/connect.F90:180-214 and
@@ -102,10 +107,19 @@ __device__ inline int32_t sorted_lookup(const int2* tab, int n, int32_t key) {
   return -1;
 }
 
+constexpr int kMaxSeg = TLM_MAX_SEGMENTS;
+
 struct Dev {
   int nx, ny, nz;
   long long N;        // cells
-  long long P;        // plane stride (elements)
+  long long P;        // plane stride (elements), >= seg_base[nseg]
+  // x-segments: segment s holds x in [seg_x0[s], seg_x0[s+1]) from storage
+  // index seg_base[s]; seg_base[nseg] ends the storage (bases are multiples
+  // of 64 cells; the cells between a segment's end and the next base are
+  // padding, never read)
+  int nseg;
+  int seg_x0[kMaxSeg + 1];
+  long long seg_base[kMaxSeg + 1];
   double dl, dt, c_red;  // c_red = c0 / reduced_c_factor
   int n_timesteps;
   double R[6];        // xmin,xmax,ymin,ymax,zmin,zmax
@@ -179,6 +193,49 @@ struct Dev {
   int n_out_rows;
   double* ring;              // slots of n_out_rows*6
 };
+
+// ---------------------------------------------------------------------------
+// storage layout (x-segments)
+// ---------------------------------------------------------------------------
+
+__device__ inline int seg_width(const Dev& d, int s) { return d.seg_x0[s + 1] - d.seg_x0[s]; }
+
+// Coordinates and segment of storage index c (0 <= c < seg_base[nseg]);
+// false for the padding after a segment's cells.
+__device__ inline bool cell_coords(const Dev& d, long long c, int& x, int& y, int& z, int& s) {
+  s = 0;
+  while (s + 1 < d.nseg && c >= d.seg_base[s + 1]) s++;
+  const unsigned w = (unsigned)seg_width(d, s);
+  const unsigned r = (unsigned)(c - d.seg_base[s]);
+  const unsigned row = r / w;
+  x = d.seg_x0[s] + (int)(r - row * w);
+  z = (int)(row / (unsigned)d.ny);
+  y = (int)row - z * d.ny;
+  return z < d.nz;
+}
+
+// Storage index of cell (x, y, z); s receives its segment.
+__device__ inline long long storage_index(const Dev& d, int x, int y, int z, int& s) {
+  s = 0;
+  while (s + 1 < d.nseg && x >= d.seg_x0[s + 1]) s++;
+  return d.seg_base[s] + (x - d.seg_x0[s]) + (long long)seg_width(d, s) * (y + (long long)d.ny * z);
+}
+
+// Storage index of the x-1 neighbour of cell c = (x, y, z) in segment s
+// (x > 0): the last cell of the same row of segment s-1 at a segment edge.
+__device__ inline long long nb_xm(const Dev& d, long long c, int x, int y, int z, int s) {
+  if (x > d.seg_x0[s]) return c - 1;
+  const long long w = seg_width(d, s - 1);
+  return d.seg_base[s - 1] + w * (1 + y + (long long)d.ny * z) - 1;
+}
+
+// Storage index of the x+1 neighbour (x < nx-1): the first cell of the same
+// row of segment s+1 at a segment edge.
+__device__ inline long long nb_xp(const Dev& d, long long c, int x, int y, int z, int s) {
+  if (x + 1 < d.seg_x0[s + 1]) return c + 1;
+  const long long w = seg_width(d, s + 1);
+  return d.seg_base[s + 1] + w * (y + (long long)d.ny * z);
+}
 
 // ---------------------------------------------------------------------------
 // excitation values on the fly
@@ -390,13 +447,13 @@ __device__ inline void flush_small(double v[12], double small) {
   }
 }
 
-// Incident voltages of cell c = (x,y,z) in v[12].  fdi: face descriptor
-// (-1 = six ordinary faces).
+// Incident voltages of cell c = (x,y,z) of segment sg in v[12].  fdi: face
+// descriptor (-1 = six ordinary faces).
 template <typename T, bool PLAIN>
-__device__ inline void pull_cell(const Dev& d, const T* __restrict__ L, long long c, int x, int y, int z,
+__device__ inline void pull_cell(const Dev& d, const T* __restrict__ L, long long c, int x, int y, int z, int sg,
                                  int fdi, const PullArgs& a, double v[12]) {
   const long long P = d.P;
-  const long long sx = 1, sy = d.nx, sz = (long long)d.nx * d.ny;
+  const long long sy = seg_width(d, sg), sz = sy * d.ny;
   double o[12];
 #pragma unroll
   for (int p = 0; p < 12; p++) o[p] = ld(L, P, p, c);
@@ -410,7 +467,6 @@ __device__ inline void pull_cell(const Dev& d, const T* __restrict__ L, long lon
   const int pb[6] = {Vz_xmin, Vz_xmax, Vz_ymin, Vz_ymax, Vy_zmin, Vy_zmax};
   const int qa[6] = {Vy_xmax, Vy_xmin, Vx_ymax, Vx_ymin, Vx_zmax, Vx_zmin};
   const int qb[6] = {Vz_xmax, Vz_xmin, Vz_ymax, Vz_ymin, Vy_zmax, Vy_zmin};
-  const long long off[6] = {-sx, sx, -sy, sy, -sz, sz};
   const bool bnd[6] = {x == 0, x == d.nx - 1, y == 0, y == d.ny - 1, z == 0, z == d.nz - 1};
 #pragma unroll
   for (int s = 0; s < 6; s++) {
@@ -421,8 +477,11 @@ __device__ inline void pull_cell(const Dev& d, const T* __restrict__ L, long lon
       v[pb[s]] = ob * R;
       continue;
     }
-    double na = ld(L, P, qa[s], c + off[s]);
-    double nb = ld(L, P, qb[s], c + off[s]);
+    const long long cn = s == 0 ? nb_xm(d, c, x, y, z, sg)
+                         : s == 1 ? nb_xp(d, c, x, y, z, sg)
+                         : s == 2 ? c - sy : s == 3 ? c + sy : s == 4 ? c - sz : c + sz;
+    double na = ld(L, P, qa[s], cn);
+    double nb = ld(L, P, qb[s], cn);
     if (fc[s] == 0) {
       v[pa[s]] = free_face<PLAIN>(oa, na);
       v[pb[s]] = free_face<PLAIN>(ob, nb);
@@ -607,41 +666,31 @@ __device__ inline int xcd_block(int b, int nblocks, int remap) {
   return (b & 7) * (nblocks >> 3) + (b >> 3);
 }
 
-// Row-segment block of the line kernel (x-segment fastest, then y, z).
-// mode 2 (y-bands): XCD k takes rows with y in the k-th eighth of ny, in
-// z-major order, so a row's y-1 neighbour ran just before on the same XCD
-// and its z-1 neighbour ny/8 rows before (both still in that XCD's L2).
-__device__ inline int line_block(const Dev& d, int b, int nblocks, int nbx, int mode) {
-  if (mode == 2 && (d.ny & 7) == 0) {
-    const int per_y = d.ny >> 3;
-    const int j = b >> 3;
-    const int xb = j % nbx;
-    const int t = j / nbx;
-    const int yl = t % per_y, z = t / per_y;
-    return xb + nbx * ((b & 7) * per_y + yl + d.ny * z);
-  }
-  return xcd_block(b, nblocks, mode);
-}
-
 // Bulk free-space cell update, one cell per lane: pull the six faces
 // (connect and outer boundary) and scatter.
 template <typename T, bool PLAIN, bool PULL>
 __device__ inline void bulk_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                   const PullArgs& a, long long c, int x, int y, int z) {
+                                   const PullArgs& a, long long c, int x, int y, int z, int sg) {
   const long long P = d.P;
-  const long long sxy = (long long)d.nx * d.ny;
+  const long long sy = seg_width(d, sg), sz = sy * d.ny;
   double v[12];
   if (PULL) {
     double o[12];
 #pragma unroll
     for (int p = 0; p < 12; p++) o[p] = (double)Lin[p * P + c];
     double xa = 0, xb = 0, xc = 0, xd = 0, ya = 0, yb = 0, yc = 0, yd = 0, za = 0, zb = 0, zc_ = 0, zd = 0;
-    if (x > 0) { xa = Lin[Vy_xmax * P + c - 1]; xb = Lin[Vz_xmax * P + c - 1]; }
-    if (x + 1 < d.nx) { xc = Lin[Vy_xmin * P + c + 1]; xd = Lin[Vz_xmin * P + c + 1]; }
-    if (y > 0) { ya = Lin[Vx_ymax * P + c - d.nx]; yb = Lin[Vz_ymax * P + c - d.nx]; }
-    if (y + 1 < d.ny) { yc = Lin[Vx_ymin * P + c + d.nx]; yd = Lin[Vz_ymin * P + c + d.nx]; }
-    if (z > 0) { za = Lin[Vx_zmax * P + c - sxy]; zb = Lin[Vy_zmax * P + c - sxy]; }
-    if (z + 1 < d.nz) { zc_ = Lin[Vx_zmin * P + c + sxy]; zd = Lin[Vy_zmin * P + c + sxy]; }
+    if (x > 0) {
+      const long long n = nb_xm(d, c, x, y, z, sg);
+      xa = Lin[Vy_xmax * P + n]; xb = Lin[Vz_xmax * P + n];
+    }
+    if (x + 1 < d.nx) {
+      const long long n = nb_xp(d, c, x, y, z, sg);
+      xc = Lin[Vy_xmin * P + n]; xd = Lin[Vz_xmin * P + n];
+    }
+    if (y > 0) { ya = Lin[Vx_ymax * P + c - sy]; yb = Lin[Vz_ymax * P + c - sy]; }
+    if (y + 1 < d.ny) { yc = Lin[Vx_ymin * P + c + sy]; yd = Lin[Vz_ymin * P + c + sy]; }
+    if (z > 0) { za = Lin[Vx_zmax * P + c - sz]; zb = Lin[Vy_zmax * P + c - sz]; }
+    if (z + 1 < d.nz) { zc_ = Lin[Vx_zmin * P + c + sz]; zd = Lin[Vy_zmin * P + c + sz]; }
     bulk_incident<PLAIN>(d, a, x, y, z, o, xa, xb, xc, xd, ya, yb, yc, yd, za, zb, zc_, zd, v);
   } else {
 #pragma unroll
@@ -651,6 +700,10 @@ __device__ inline void bulk_update(const Dev& d, const T* __restrict__ Lin, T* _
   if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
   store_cell(Lout, P, c, v);
 }
+
+// The tile kernels below (k_bulk_cell, k_bulk_vec, k_bulk_z) address cells
+// as c = x + nx*(y + ny*z): they run on the one-segment (row-major) layout
+// only, which the host enforces.
 
 // Bulk free-space cells, one cell per thread (zc = 1): block = 64 x by cells
 // of one plane.  Neighbour ports are loaded directly; they are the block's
@@ -665,7 +718,7 @@ __global__ __launch_bounds__(256) void k_bulk_cell(Dev d, const T* __restrict__ 
   if (x >= d.nx || y >= d.ny || z >= d.nz) return;
   const long long c = x + (long long)d.nx * (y + (long long)d.ny * z);
   if ((d.mask[c >> 5] >> (c & 31)) & 1u) return;
-  bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z);
+  bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z, 0);
 }
 
 // Vector access of W consecutive link values (W*sizeof(T) = 16 bytes).
@@ -865,14 +918,12 @@ __device__ inline void pml_update(const Dev& d, const T* __restrict__ Lin, T* __
                                   const PullArgs& a, int slot, int i) {
   const int32_t cw = d.pml_cell[i];
   const long long c = cw & kCellMask;
-  const int nxy = d.nx * d.ny;
-  const int z = (int)(c / nxy);
-  const int rem = (int)(c - (long long)z * nxy);
-  const int y = rem / d.nx, x = rem - y * d.nx;
   double v[12];
   if (PULL) {
+    int x, y, z, sg;
+    cell_coords(d, c, x, y, z, sg);
     const int fdi = (cw & kFlagFd) ? sorted_lookup(d.pml_fd_tab, d.n_pml_fd, i) : -1;
-    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, fdi, a, v);
+    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, sg, fdi, a, v);
   } else {
 #pragma unroll
     for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
@@ -1053,17 +1104,11 @@ template <typename T, bool PLAIN, bool PULL>
 __device__ inline void pml_material_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                            const PullArgs& a, int slot, int i) {
   const long long c = d.pm_cell[i];
-  int x, y, z;
-  {
-    const int nxy = d.nx * d.ny;
-    z = (int)(c / nxy);
-    const int rem = (int)(c - (long long)z * nxy);
-    y = rem / d.nx;
-    x = rem - y * d.nx;
-  }
   double v[12];
   if (PULL) {
-    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, d.pm_fd[i], a, v);
+    int x, y, z, sg;
+    cell_coords(d, c, x, y, z, sg);
+    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, sg, d.pm_fd[i], a, v);
   } else {
 #pragma unroll
     for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
@@ -1376,14 +1421,6 @@ __device__ inline void node_update(const Dev& d, double v[12], int mnum, long lo
   scatter_ports(v, Vx, Vy, Vz, Ix, Iy, Iz);
 }
 
-__device__ inline void cell_coords(const Dev& d, long long c, int& x, int& y, int& z) {
-  const int nxy = d.nx * d.ny;
-  z = (int)(c / nxy);
-  const int rem = (int)(c - (long long)z * nxy);
-  y = rem / d.nx;
-  x = rem - y * d.nx;
-}
-
 // Generic special cells: point sources, probes, cells touching a special
 // face; the Thevenin node, or the free-space branch for code 0 cells that
 // only touch a special face.
@@ -1391,11 +1428,11 @@ template <typename T, bool PLAIN, bool PULL>
 __device__ inline void generic_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                       const PullArgs& a, int timestep, int slot, int i) {
   const long long c = d.gen_cell[i];
-  int x, y, z;
-  cell_coords(d, c, x, y, z);
   double v[12];
   if (PULL) {
-    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, d.gen_fd[i], a, v);
+    int x, y, z, sg;
+    cell_coords(d, c, x, y, z, sg);
+    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, sg, d.gen_fd[i], a, v);
   } else {
 #pragma unroll
     for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
@@ -1422,11 +1459,11 @@ template <typename T, bool PLAIN, bool PULL>
 __device__ inline void material_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                        const PullArgs& a, int i) {
   const long long c = d.mat_cell[i];
-  int x, y, z;
-  cell_coords(d, c, x, y, z);
   double v[12];
   if (PULL) {
-    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, -1, a, v);
+    int x, y, z, sg;
+    cell_coords(d, c, x, y, z, sg);
+    pull_cell<T, PLAIN>(d, Lin, c, x, y, z, sg, -1, a, v);
   } else {
 #pragma unroll
     for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
@@ -1443,22 +1480,24 @@ __global__ __launch_bounds__(256) void k_material(Dev d, const T* __restrict__ L
   if (i < n) material_update<T, PLAIN, PULL>(d, Lin, Lout, a, i);
 }
 
-// Pure bulk lines: workgroup = 256 consecutive cells of a row segment;
-// cells of lines with any special cell are left to the mixed-line and class
-// kernels.
+// Pure bulk lines: workgroup b = storage cells [256 b, 256 b + 256); cells
+// of lines with any special cell are left to the mixed-line and class
+// kernels, padding cells to no one.
+template <typename T, bool PLAIN, bool PULL>
+__device__ inline void bulk_line_cell(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                      const PullArgs& a, long long c) {
+  if (c >= d.seg_base[d.nseg]) return;
+  if (d.line_info[c / d.line_len] != 0u) return;
+  int x, y, z, sg;
+  if (!cell_coords(d, c, x, y, z, sg)) return;
+  bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z, sg);
+}
+
 template <typename T, bool PLAIN, bool PULL>
 __global__ __launch_bounds__(256) void k_bulk_lines(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                                    PullArgs a, int xcd_remap) {
-  const int nbx = (d.nx + 255) >> 8;
-  const int b = line_block(d, blockIdx.x, gridDim.x, nbx, xcd_remap);
-  const int row = b / nbx;  // y + ny*z
-  const int x = (b - row * nbx) * 256 + threadIdx.x;
-  if (x >= d.nx) return;
-  const long long c = x + (long long)d.nx * row;
-  if (d.line_info[c / d.line_len] != 0u) return;
-  const int z = row / d.ny;
-  const int y = row - z * d.ny;
-  bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z);
+  const int b = xcd_block(blockIdx.x, gridDim.x, xcd_remap);
+  bulk_line_cell<T, PLAIN, PULL>(d, Lin, Lout, a, (long long)b * 256 + threadIdx.x);
 }
 
 // kMixedBulkPerLane bulk cells per lane of the mixed-line kernel: their
@@ -1473,7 +1512,6 @@ template <typename T, bool PLAIN, bool PULL, int B>
 __device__ inline void bulk_update_n(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                      const PullArgs& a, const long long (&cs)[B]) {
   const long long P = d.P;
-  const long long sxy = (long long)d.nx * d.ny;
   T o[B][12], nb[B][12];
   int x[B], y[B], z[B];
 #pragma unroll
@@ -1481,18 +1519,26 @@ __device__ inline void bulk_update_n(const Dev& d, const T* __restrict__ Lin, T*
     x[b] = y[b] = z[b] = 0;
     const long long c = cs[b];
     if (c < 0) continue;
-    cell_coords(d, c, x[b], y[b], z[b]);
+    int sg;
+    cell_coords(d, c, x[b], y[b], z[b], sg);
 #pragma unroll
     for (int p = 0; p < 12; p++) o[b][p] = Lin[p * P + c];
     if (PULL) {
+      const long long sy = seg_width(d, sg), sz = sy * d.ny;
 #pragma unroll
       for (int k = 0; k < 12; k++) nb[b][k] = (T)0;
-      if (x[b] > 0) { nb[b][0] = Lin[Vy_xmax * P + c - 1]; nb[b][1] = Lin[Vz_xmax * P + c - 1]; }
-      if (x[b] + 1 < d.nx) { nb[b][2] = Lin[Vy_xmin * P + c + 1]; nb[b][3] = Lin[Vz_xmin * P + c + 1]; }
-      if (y[b] > 0) { nb[b][4] = Lin[Vx_ymax * P + c - d.nx]; nb[b][5] = Lin[Vz_ymax * P + c - d.nx]; }
-      if (y[b] + 1 < d.ny) { nb[b][6] = Lin[Vx_ymin * P + c + d.nx]; nb[b][7] = Lin[Vz_ymin * P + c + d.nx]; }
-      if (z[b] > 0) { nb[b][8] = Lin[Vx_zmax * P + c - sxy]; nb[b][9] = Lin[Vy_zmax * P + c - sxy]; }
-      if (z[b] + 1 < d.nz) { nb[b][10] = Lin[Vx_zmin * P + c + sxy]; nb[b][11] = Lin[Vy_zmin * P + c + sxy]; }
+      if (x[b] > 0) {
+        const long long n = nb_xm(d, c, x[b], y[b], z[b], sg);
+        nb[b][0] = Lin[Vy_xmax * P + n]; nb[b][1] = Lin[Vz_xmax * P + n];
+      }
+      if (x[b] + 1 < d.nx) {
+        const long long n = nb_xp(d, c, x[b], y[b], z[b], sg);
+        nb[b][2] = Lin[Vy_xmin * P + n]; nb[b][3] = Lin[Vz_xmin * P + n];
+      }
+      if (y[b] > 0) { nb[b][4] = Lin[Vx_ymax * P + c - sy]; nb[b][5] = Lin[Vz_ymax * P + c - sy]; }
+      if (y[b] + 1 < d.ny) { nb[b][6] = Lin[Vx_ymin * P + c + sy]; nb[b][7] = Lin[Vz_ymin * P + c + sy]; }
+      if (z[b] > 0) { nb[b][8] = Lin[Vx_zmax * P + c - sz]; nb[b][9] = Lin[Vy_zmax * P + c - sz]; }
+      if (z[b] + 1 < d.nz) { nb[b][10] = Lin[Vx_zmin * P + c + sz]; nb[b][11] = Lin[Vy_zmin * P + c + sz]; }
     }
   }
 #pragma unroll
@@ -1595,11 +1641,11 @@ __device__ inline void mixed_lds_body(const Dev& d, const T* __restrict__ Lin, T
   const long long c = (long long)line * L + (t % L);
   const bool mine = fd != -2;
   if (mine) {
-    int x, y, z;
-    cell_coords(d, c, x, y, z);
     double v[12];
     if (PULL) {
-      pull_cell<T, PLAIN>(d, Lin, c, x, y, z, fd, a, v);
+      int x, y, z, sg;
+      cell_coords(d, c, x, y, z, sg);
+      pull_cell<T, PLAIN>(d, Lin, c, x, y, z, sg, fd, a, v);
     } else {
 #pragma unroll
       for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
@@ -1654,9 +1700,10 @@ __global__ __launch_bounds__(256) void k_connect_bulk(Dev d, const T* __restrict
   int y = blockIdx.y * blockDim.y + threadIdx.y;
   int z = blockIdx.z;
   if (x >= d.nx || y >= d.ny) return;
-  long long c = x + (long long)d.nx * (y + (long long)d.ny * z);
+  int sg;
+  const long long c = storage_index(d, x, y, z, sg);
   double v[12];
-  pull_cell<T, PLAIN>(d, Lin, c, x, y, z, -1, a, v);
+  pull_cell<T, PLAIN>(d, Lin, c, x, y, z, sg, -1, a, v);
   store_cell(Lout, d.P, c, v);
 }
 
@@ -1666,12 +1713,10 @@ __global__ __launch_bounds__(256) void k_connect_list(Dev d, const T* __restrict
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const long long c = cells[i];
-  const int nxy = d.nx * d.ny;
-  const int z = (int)(c / nxy);
-  const int rem = (int)(c - (long long)z * nxy);
-  const int y = rem / d.nx, x = rem - y * d.nx;
+  int x, y, z, sg;
+  cell_coords(d, c, x, y, z, sg);
   double v[12];
-  pull_cell<T, PLAIN>(d, Lin, c, x, y, z, fds ? fds[i] : -1, a, v);
+  pull_cell<T, PLAIN>(d, Lin, c, x, y, z, sg, fds ? fds[i] : -1, a, v);
   store_cell(Lout, d.P, c, v);
 }
 
@@ -1691,31 +1736,27 @@ static __global__ void k_dft(const double* __restrict__ ring, int n_out_rows, in
   acc[2 * i + 1] -= v * s * dt;
 }
 
-// Fortran V(1:12,0:nx+1,0:ny+1,1:nz) <-> planes.
+// Fortran V(1:12,0:nx+1,0:ny+1,1:nz) <-> planes; thread = storage index.
 template <typename T>
-__global__ void k_aos_to_planes(const double* __restrict__ V, T* __restrict__ L, int nx, int ny, int nz, long long P) {
-  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  long long N = (long long)nx * ny * nz;
-  if (c >= N) return;
-  int x = (int)(c % nx);
-  long long r = c / nx;
-  int y = (int)(r % ny);
-  int z = (int)(r / ny);
-  long long fi = 12 * ((long long)(x + 1) + (long long)(nx + 2) * ((long long)(y + 1) + (long long)(ny + 2) * z));
-  for (int p = 0; p < 12; p++) L[p * P + c] = (T)V[fi + p];
+__global__ void k_aos_to_planes(Dev d, const double* __restrict__ V, T* __restrict__ L) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= d.seg_base[d.nseg]) return;
+  int x, y, z, s;
+  if (!cell_coords(d, c, x, y, z, s)) return;
+  const long long fi =
+      12 * ((long long)(x + 1) + (long long)(d.nx + 2) * ((long long)(y + 1) + (long long)(d.ny + 2) * z));
+  for (int p = 0; p < 12; p++) L[p * d.P + c] = (T)V[fi + p];
 }
 
 template <typename T>
-__global__ void k_planes_to_aos(const T* __restrict__ L, double* __restrict__ V, int nx, int ny, int nz, long long P) {
-  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  long long N = (long long)nx * ny * nz;
-  if (c >= N) return;
-  int x = (int)(c % nx);
-  long long r = c / nx;
-  int y = (int)(r % ny);
-  int z = (int)(r / ny);
-  long long fi = 12 * ((long long)(x + 1) + (long long)(nx + 2) * ((long long)(y + 1) + (long long)(ny + 2) * z));
-  for (int p = 0; p < 12; p++) V[fi + p] = (double)L[p * P + c];
+__global__ void k_planes_to_aos(Dev d, const T* __restrict__ L, double* __restrict__ V) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= d.seg_base[d.nseg]) return;
+  int x, y, z, s;
+  if (!cell_coords(d, c, x, y, z, s)) return;
+  const long long fi =
+      12 * ((long long)(x + 1) + (long long)(d.nx + 2) * ((long long)(y + 1) + (long long)(d.ny + 2) * z));
+  for (int p = 0; p < 12; p++) V[fi + p] = (double)L[p * d.P + c];
 }
 
 }  // namespace scn

@@ -135,7 +135,7 @@ int finalize(tlm_handle* h) {
   struct MatRec { int32_t c; uint16_t m; int32_t f; };
   struct GenRec { int32_t c, code, m, f, er, orow, fdc; };
   struct PmRec { int32_t c, par, m, orow, fdc; };  // PML cell holding a material (class 4)
-  std::vector<uint8_t> cls(N, 0);
+  std::vector<uint8_t> cls(N, 0);  // by reference cell index
   std::vector<PmRec> pmr;
   std::vector<PmlRec> pr;
   std::vector<MatRec> mr;
@@ -198,25 +198,81 @@ int finalize(tlm_handle* h) {
       gr.push_back({(int32_t)c, code, m, f, er, orow, fdc});
     }
   }
-  if (N > (long long)scn::kCellMask) return fail(h, TLM_ERR_UNSUPPORTED, "more than 2^28 cells per handle");
+
+  // ---- storage layout: x-segments ---------------------------------------------
+  // Cut between x and x+1 where at least an eighth of the (y, z) positions
+  // change class, keeping the TLM_MAX_SEGMENTS - 1 strongest cuts; segment
+  // bases are multiples of 64 cells.
+  std::vector<int> seg_x0 = {0};
+  if (h->layout == TLM_LAYOUT_X_SEGMENTS && nx > 1) {
+    std::vector<long long> T(nx - 1, 0);
+    for (long long r = 0; r < (long long)ny * nz; r++) {
+      const uint8_t* row = &cls[r * nx];
+      for (int x = 0; x + 1 < nx; x++) T[x] += row[x] != row[x + 1];
+    }
+    std::vector<int> cut;
+    for (int x = 0; x + 1 < nx; x++)
+      if (8 * T[x] >= (long long)ny * nz) cut.push_back(x + 1);
+    if ((int)cut.size() > scn::kMaxSeg - 1) {
+      std::stable_sort(cut.begin(), cut.end(), [&](int a, int b) { return T[a - 1] > T[b - 1]; });
+      cut.resize(scn::kMaxSeg - 1);
+      std::sort(cut.begin(), cut.end());
+    }
+    seg_x0.insert(seg_x0.end(), cut.begin(), cut.end());
+  }
+  seg_x0.push_back(nx);
+  const int nseg = (int)seg_x0.size() - 1;
+  std::vector<long long> seg_base(nseg + 1, 0);
+  for (int sg = 0; sg < nseg; sg++)
+    seg_base[sg + 1] = (seg_base[sg] + (long long)(seg_x0[sg + 1] - seg_x0[sg]) * ny * nz + 63) / 64 * 64;
+  const long long S = seg_base[nseg];  // storage cells, padding included
+  if (S > (long long)scn::kCellMask) return fail(h, TLM_ERR_UNSUPPORTED, "more than 2^28 cells per handle");
+  std::vector<int> seg_of_x(nx);
+  for (int sg = 0; sg < nseg; sg++)
+    for (int x = seg_x0[sg]; x < seg_x0[sg + 1]; x++) seg_of_x[x] = sg;
+  auto sidx = [&](long long c) -> int32_t {  // reference cell index -> storage index
+    const long long r = c / nx;
+    const int x = (int)(c - r * nx), sg = seg_of_x[x];
+    return (int32_t)(seg_base[sg] + (x - seg_x0[sg]) + (long long)(seg_x0[sg + 1] - seg_x0[sg]) * r);
+  };
+  // storage class per storage cell (5: padding), class lists in storage order
+  std::vector<uint8_t> scls(S, 5);
+  for (long long c = 0; c < N; c++) scls[sidx(c)] = cls[c];
+  {
+    std::vector<uint8_t>().swap(cls);
+    auto by_cell = [](const auto& a, const auto& b) { return a.c < b.c; };
+    for (auto& r : pr) r.c = sidx(r.c);
+    for (auto& r : mr) r.c = sidx(r.c);
+    for (auto& r : gr) r.c = sidx(r.c);
+    for (auto& r : pmr) r.c = sidx(r.c);
+    std::sort(pr.begin(), pr.end(), by_cell);
+    std::sort(mr.begin(), mr.end(), by_cell);
+    std::sort(gr.begin(), gr.end(), by_cell);
+    std::sort(pmr.begin(), pmr.end(), by_cell);
+  }
 
   // ---- 128-byte lines ------------------------------------------------------
   // A line is 128 bytes of one link plane (32 FP32 or 16 FP64 cells).  A
   // line of one class only is written whole by that class's kernel; a mixed
   // line is written whole by the mixed-line kernel, which updates each of
   // its cells with that cell's class code.  No line is ever written in part.
+  // Padding cells belong to no line class; a line of padding only is 3.
   const int L = (int)(128 / link_size(h));
-  const long long n_lines = (N + L - 1) / L;
+  const long long n_lines = S / L;
   std::vector<uint32_t> line_info(n_lines, 0);
   std::vector<uint8_t> line_pure(n_lines, 0);
-  std::vector<uint32_t> mask((N + 31) / 32, 0u);
+  std::vector<uint32_t> mask((S + 31) / 32, 0u);
   long long n_mixed_cells = 0;
   for (long long ln = 0; ln < n_lines; ln++) {
-    long long c0 = ln * L, c1 = std::min(N, c0 + L);
-    uint8_t k0 = cls[c0];
+    const long long c0 = ln * L, c1 = c0 + L;
+    int k0 = -1;
     bool same = true;
-    for (long long c = c0; c < c1; c++)
-      if (cls[c] != k0) same = false;
+    for (long long c = c0; c < c1; c++) {
+      if (scls[c] == 5) continue;
+      if (k0 < 0) k0 = scls[c];
+      else if (scls[c] != k0) same = false;
+    }
+    if (k0 < 0) { line_info[ln] = 3; continue; }
     if (same && k0 == 0) { line_info[ln] = 0; continue; }
     for (long long c = c0; c < c1; c++) mask[c >> 5] |= 1u << (c & 31);
     if (same) { line_info[ln] = 1; line_pure[ln] = 1; continue; }
@@ -243,7 +299,7 @@ int finalize(tlm_handle* h) {
   std::vector<std::vector<int32_t>> slot_b(KB > 1 ? KB - 1 : 1);
   {
     int ip = n_pure_pml, im = n_pure_mat, ig = n_pure_gen;
-    // class-list positions of the mixed-line cells follow scan order (the
+    // class-list positions of the mixed-line cells follow storage order (the
     // tails of the three lists)
     std::vector<long long> mlines;
     std::vector<int32_t> line_first_idx;  // per mixed line: offset into cell_idx
@@ -252,9 +308,9 @@ int finalize(tlm_handle* h) {
       if (line_info[ln] != 2) continue;
       mlines.push_back(ln);
       line_first_idx.push_back((int32_t)cell_idx.size());
-      long long c0 = ln * L, c1 = std::min(N, c0 + L);
+      const long long c0 = ln * L, c1 = c0 + L;
       for (long long c = c0; c < c1; c++) {
-        int k2 = cls[c], idx = 0;
+        int k2 = scls[c], idx = 0;
         if (k2 == 1) idx = ip++;
         else if (k2 == 2) idx = im++;
         else if (k2 == 3) idx = ig++;
@@ -268,10 +324,10 @@ int finalize(tlm_handle* h) {
       std::vector<std::pair<int, std::pair<int32_t, int32_t>>> spec;  // (rank, (word, idx))
       std::vector<int32_t> bulk;
       for (size_t q = l0; q < l1; q++) {
-        long long c0 = mlines[q] * L, c1 = std::min(N, c0 + L);
+        const long long c0 = mlines[q] * L, c1 = c0 + L;
         for (long long c = c0; c < c1; c++) {
-          const int k2 = cls[c];
-          if (k2 == 4) continue;  // own kernel
+          const int k2 = scls[c];
+          if (k2 >= 4) continue;  // own kernel, or padding
           n_mixed_cells++;
           if (k2 == 0) bulk.push_back((int32_t)c);
           else
@@ -305,10 +361,10 @@ int finalize(tlm_handle* h) {
     size_t l0 = 0;
     int n_spec = 0, n_bulk = 0;
     for (size_t q = 0; q < mlines.size(); q++) {
-      long long c0 = mlines[q] * L, c1 = std::min(N, c0 + L);
+      const long long c0 = mlines[q] * L, c1 = c0 + L;
       int s = 0, b = 0;
       for (long long c = c0; c < c1; c++)
-        if (cls[c] != 4) (cls[c] ? s : b)++;
+        if (scls[c] < 4) (scls[c] ? s : b)++;
       if (q > l0 && n_spec + s + (n_bulk + b + KB - 1) / KB > 256) {
         emit(l0, q);
         l0 = q;
@@ -333,11 +389,11 @@ int finalize(tlm_handle* h) {
         for (int k = 0; k < L; k++) {
           const int p = q * L + k;
           const long long c = ln * L + k;
-          if (ln < 0 || c >= N || cls[c] == 4) {
+          if (ln < 0 || scls[c] >= 4) {
             lds_fd.push_back(-2);
             continue;
           }
-          const int k2 = cls[c];
+          const int k2 = scls[c];
           const int32_t idx = cell_idx[line_first_idx[qi] + k];
           lds_fd.push_back(k2 == 1 ? pr[idx].fdc : (k2 == 3 ? gr[idx].fdc : -1));
           sl.push_back({rank_of[k2], {(int32_t)((k2 << 16) | p), k2 ? idx : 0}});
@@ -374,7 +430,6 @@ int finalize(tlm_handle* h) {
   h->n_pure_mat = n_pure_mat;
   h->n_pure_gen = n_pure_gen;
   h->n_mixed_cells = n_mixed_cells;
-  if (N > (long long)scn::kCellMask) return fail(h, TLM_ERR_UNSUPPORTED, "more than 2^28 cells per handle");
   if (exc_rank != h->n_cell_rows)
     return fail(h, TLM_ERR_ARG, "cell excitation rows do not match the cell codes");
   const int n_out_rows = out_rank;
@@ -436,7 +491,13 @@ int finalize(tlm_handle* h) {
 
   // ---- device allocations -------------------------------------------------
   Dev& d = h->d;
+  h->P = S;
   d.nx = nx; d.ny = ny; d.nz = nz; d.N = N; d.P = h->P;
+  d.nseg = nseg;
+  for (int sg = 0; sg <= nseg; sg++) {
+    d.seg_x0[sg] = seg_x0[sg];
+    d.seg_base[sg] = seg_base[sg];
+  }
   d.dl = h->g.dl; d.dt = h->g.dt; d.c_red = scn::kC0 / h->g.reduced_c_factor;
   d.n_timesteps = h->g.n_timesteps;
   d.R[0] = h->g.r_xmin; d.R[1] = h->g.r_xmax; d.R[2] = h->g.r_ymin;
@@ -688,7 +749,7 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
   {
     ProfScope ps(h, TLM_KCLASS_BULK);
     if (sched_variant(h) == 3) {
-      dim3 g((unsigned)(((d.nx + 255) / 256) * (long long)d.ny * d.nz));
+      dim3 g((unsigned)((d.seg_base[d.nseg] + 255) / 256));
       if (pull)
         hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, h->sched_xcd);
       else
@@ -1049,13 +1110,13 @@ int tlm_upload_links(tlm_handle* h, const double* V) {
   HIP_TRY(h, hipMalloc(&tmp, n * sizeof(double)));
   hipError_t e = hipMemcpyAsync(tmp, V, n * sizeof(double), hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess) {
-    unsigned g = (unsigned)((h->N + 255) / 256);
+    unsigned g = (unsigned)((h->P + 255) / 256);
     if (h->form == TLM_FORM_PLAIN_FP64)
-      hipLaunchKernelGGL(scn::k_aos_to_planes<double>, dim3(g), dim3(256), 0, h->stream, (const double*)tmp,
-                         (double*)h->planes[h->cur], h->nx, h->ny, h->nz, h->P);
+      hipLaunchKernelGGL(scn::k_aos_to_planes<double>, dim3(g), dim3(256), 0, h->stream, h->d, (const double*)tmp,
+                         (double*)h->planes[h->cur]);
     else
-      hipLaunchKernelGGL(scn::k_aos_to_planes<float>, dim3(g), dim3(256), 0, h->stream, (const double*)tmp,
-                         (float*)h->planes[h->cur], h->nx, h->ny, h->nz, h->P);
+      hipLaunchKernelGGL(scn::k_aos_to_planes<float>, dim3(g), dim3(256), 0, h->stream, h->d, (const double*)tmp,
+                         (float*)h->planes[h->cur]);
     e = hipStreamSynchronize(h->stream);
   }
   (void)hipFree(tmp);
@@ -1075,13 +1136,13 @@ int tlm_download_links(tlm_handle* h, double* V) {
   // keep the caller's halo: copy it in first, overwrite the interior
   hipError_t e = hipMemcpyAsync(tmp, V, n * sizeof(double), hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess) {
-    unsigned g = (unsigned)((h->N + 255) / 256);
+    unsigned g = (unsigned)((h->P + 255) / 256);
     if (h->form == TLM_FORM_PLAIN_FP64)
-      hipLaunchKernelGGL(scn::k_planes_to_aos<double>, dim3(g), dim3(256), 0, h->stream,
-                         (const double*)h->planes[h->cur], (double*)tmp, h->nx, h->ny, h->nz, h->P);
+      hipLaunchKernelGGL(scn::k_planes_to_aos<double>, dim3(g), dim3(256), 0, h->stream, h->d,
+                         (const double*)h->planes[h->cur], (double*)tmp);
     else
-      hipLaunchKernelGGL(scn::k_planes_to_aos<float>, dim3(g), dim3(256), 0, h->stream,
-                         (const float*)h->planes[h->cur], (double*)tmp, h->nx, h->ny, h->nz, h->P);
+      hipLaunchKernelGGL(scn::k_planes_to_aos<float>, dim3(g), dim3(256), 0, h->stream, h->d,
+                         (const float*)h->planes[h->cur], (double*)tmp);
     e = hipMemcpyAsync(V, tmp, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -1192,16 +1253,38 @@ int tlm_read_dft(tlm_handle* h, double* re_im) {
   return TLM_OK;
 }
 
+int tlm_set_layout(tlm_handle* h, int32_t layout) {
+  if (!h) return TLM_ERR_ARG;
+  if (layout != TLM_LAYOUT_X_SEGMENTS && layout != TLM_LAYOUT_ROWS) return fail(h, TLM_ERR_ARG, "bad layout");
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "the layout is fixed once the device tables exist");
+  if (layout != TLM_LAYOUT_ROWS && h->sched_variant >= 0 && h->sched_variant != 3)
+    return fail(h, TLM_ERR_STATE, "the tile schedules need TLM_LAYOUT_ROWS");
+  h->layout = layout;
+  return TLM_OK;
+}
+
+int tlm_layout_segments(tlm_handle* h, int32_t* n_segments, int32_t* x0) {
+  if (!h || !n_segments) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  *n_segments = h->d.nseg;
+  if (x0)
+    for (int s = 0; s <= h->d.nseg; s++) x0[s] = h->d.seg_x0[s];
+  return TLM_OK;
+}
+
 int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap, int32_t mixed) {
   if (!h) return TLM_ERR_ARG;
   if (mixed < -1 || mixed > 1) return fail(h, TLM_ERR_ARG, "bad mixed-line kernel");
-  h->mixed_variant = mixed;
   if (variant < 0 || variant > 3 || (rows != 1 && rows != 2 && rows != 4) || zc < 1)
     return fail(h, TLM_ERR_ARG, "bad bulk schedule");
+  if (variant != 3 && h->layout != TLM_LAYOUT_ROWS)
+    return fail(h, TLM_ERR_ARG, "the tile schedules (variants 0-2) need TLM_LAYOUT_ROWS");
+  if (xcd_remap < 0 || xcd_remap > 1) return fail(h, TLM_ERR_ARG, "bad XCD mapping");
+  h->mixed_variant = mixed;
   h->sched_variant = variant;
   h->sched_by = rows;
   h->sched_zc = zc;
-  if (xcd_remap < 0 || xcd_remap > 2) return fail(h, TLM_ERR_ARG, "bad XCD mapping");
   h->sched_xcd = xcd_remap;
   return TLM_OK;
 }
@@ -1211,9 +1294,12 @@ int tlm_autotune(tlm_handle* h, int32_t first_timestep, int32_t steps_per_candid
   int rc;
   if ((rc = finalize(h))) return rc;
   struct Cand { int v, rows, zc, xcd, mixed; };
-  std::vector<Cand> cand = {{3, 1, 1, 0, 1}, {3, 1, 1, 0, 0}, {3, 1, 1, 1, 1}, {0, 1, 1, 0, 1}, {0, 2, 1, 0, 1},
-                            {0, 4, 1, 0, 1}, {2, 1, 1, 0, 1}, {2, 4, 1, 0, 1}, {1, 4, 8, 0, 1}};
-  if (h->ny % 8 == 0) cand.push_back({3, 1, 1, 2, 1});
+  std::vector<Cand> cand = {{3, 1, 1, 0, 1}, {3, 1, 1, 0, 0}, {3, 1, 1, 1, 1}, {3, 1, 1, 1, 0}};
+  if (h->layout == TLM_LAYOUT_ROWS) {
+    const Cand tiles[] = {{0, 1, 1, 0, 1}, {0, 2, 1, 0, 1}, {0, 4, 1, 0, 1}, {2, 1, 1, 0, 1}, {2, 4, 1, 0, 1},
+                          {1, 4, 8, 0, 1}};
+    cand.insert(cand.end(), std::begin(tiles), std::end(tiles));
+  }
   const int total = (int)cand.size() * (steps_per_candidate + 1);
   if (first_timestep + total - 1 > h->g.n_timesteps)
     return fail(h, TLM_ERR_ARG, "not enough timesteps left for the autotuner");
@@ -1289,7 +1375,7 @@ int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
   if ((rc = finalize(h))) return rc;
   const int64_t sl = (int64_t)link_size(h);
   const int64_t L = h->d.line_len;
-  const int64_t n_lines = (h->N + L - 1) / L;
+  const int64_t n_lines = h->P / L;
   // filter words of a dispersive cell: 3 components x (carried words read
   // and written) for eps and mu, FP64
   auto filter_bytes = [&](int m) -> int64_t {

@@ -217,6 +217,19 @@ module tlm_gpu_iface
       real(c_double), value :: small_to_zero_value
     end function tlm_set_small_to_zero
 
+    integer(c_int) function tlm_set_layout(h, layout) bind(c, name='tlm_set_layout')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: layout
+    end function tlm_set_layout
+
+    integer(c_int) function tlm_layout_segments(h, n_segments, x0) bind(c, name='tlm_layout_segments')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: n_segments
+      integer(c_int32_t), intent(out) :: x0(*)
+    end function tlm_layout_segments
+
     integer(c_int) function tlm_set_dft_frequencies(h, n, f) bind(c, name='tlm_set_dft_frequencies')
       import :: c_int, c_int32_t, c_ptr, c_double
       type(c_ptr), value :: h

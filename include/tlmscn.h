@@ -243,11 +243,12 @@ int tlm_read_dft(tlm_handle* h, double* re_im);
 
 /* Autotuner of the bulk schedule (no counterpart in the reference): runs
  * steps first_timestep .. first_timestep + *steps_done - 1 of the solve,
- * each candidate schedule (line kernel with each XCD mapping, tile kernels
- * of 1/2/4 rows, vector kernel, z-marching kernel) for steps_per_candidate
- * timed steps after one untimed step, and keeps the fastest.  Every
- * schedule gives the same results, so these are ordinary steps of the
- * solve. */
+ * each candidate schedule (line kernel with and without the XCD mapping,
+ * each mixed-line kernel; with TLM_LAYOUT_ROWS also tile kernels of 1/2/4
+ * rows, vector kernel, z-marching kernel: 4 candidates, or 10) for
+ * steps_per_candidate timed steps after one untimed step, and keeps the
+ * fastest.  Every schedule gives the same results, so these are ordinary
+ * steps of the solve. */
 int tlm_autotune(tlm_handle* h, int32_t first_timestep, int32_t steps_per_candidate, int32_t* steps_done);
 
 /* TLM_set_small_to_zero (TLM_SOLVER/TLM_set_small_to_zero.F90:34-68, gated
@@ -257,14 +258,33 @@ int tlm_autotune(tlm_handle* h, int32_t first_timestep, int32_t steps_per_candid
  * n_small_to_zero = 0 turns it off (the default). */
 int tlm_set_small_to_zero(tlm_handle* h, int32_t n_small_to_zero, double small_to_zero_value);
 
-/* Bulk-kernel schedule: variant 3 (default) = one 256-cell row segment per
- * workgroup over the pure bulk lines; variant 0 = one cell per lane, 64 x rows
- * tiles; variant 1 = workgroups marching zc planes; variant 2 = 16-byte
- * vector accesses, several consecutive cells per lane (falls back to 0 when
- * nx is not a multiple of the vector width); rows = tile height (1, 2 or 4
- * rows); xcd_remap = 1: give each XCD a contiguous tile range; 2 (line
- * kernel only, ny divisible by 8): XCD k takes the rows of the k-th y-band
- * in z-major order, so the y-1 and z-1 neighbour rows are in its L2.
+/* Storage order of the cells in the link planes (no counterpart in the
+ * reference, whose V array is scan-ordered; results do not depend on it).
+ * TLM_LAYOUT_X_SEGMENTS (default): the x range is cut where at least an
+ * eighth of an x-plane's cells change class between neighbouring planes
+ * (PML slabs, Huygens planes, walls), at most TLM_MAX_SEGMENTS segments;
+ * segment s stores its cells x-fastest with its own row width, so a PML
+ * slab or a source plane fills whole 128-byte lines by itself.
+ * TLM_LAYOUT_ROWS: one segment, the reference's scan order.  Set before the
+ * first call that builds the device tables (step, scatter, upload, kernel
+ * byte query); TLM_ERR_STATE after. */
+#define TLM_LAYOUT_X_SEGMENTS 0
+#define TLM_LAYOUT_ROWS 1
+#define TLM_MAX_SEGMENTS 16
+int tlm_set_layout(tlm_handle* h, int32_t layout);
+
+/* The layout in use (builds the device tables): *n_segments segments,
+ * segment s holding x in [x0[s], x0[s+1]) (0-based); x0 has room for
+ * TLM_MAX_SEGMENTS + 1 entries, or is NULL. */
+int tlm_layout_segments(tlm_handle* h, int32_t* n_segments, int32_t* x0);
+
+/* Bulk-kernel schedule: variant 3 (default) = 256 consecutive storage cells
+ * per workgroup over the pure bulk lines; variants 0-2 need TLM_LAYOUT_ROWS:
+ * 0 = one cell per lane, 64 x rows tiles; 1 = workgroups marching zc planes;
+ * 2 = 16-byte vector accesses, several consecutive cells per lane (falls
+ * back to 0 when nx is not a multiple of the vector width); rows = tile
+ * height (1, 2 or 4 rows); xcd_remap = 1: give each XCD a contiguous range
+ * of workgroups.
  * mixed: mixed-line kernel, 1 = LDS-staged cell-order loads and stores
  * around class-sorted node updates (default in the plain form), 0 =
  * class-sorted lanes with four bulk cells per lane (default in the

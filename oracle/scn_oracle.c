@@ -83,6 +83,7 @@ struct oracle_handle {
   double* pml_mat_state;
   /* set_small_to_zero packet: n_small_to_zero (0: off), small_to_zero_value */
   int stz_n;
+  int layout; /* tlm_set_layout's value: decides the autotuner's candidate count only */
   double stz_value;
   /* excitation */
   int n_functions;
@@ -1344,9 +1345,24 @@ int oracle_set_schedule(oracle_handle* h, int32_t variant, int32_t rows, int32_t
 }
 int oracle_set_profiling(oracle_handle* h, int32_t enable) { (void)h; (void)enable; return TLM_OK; }
 
-/* The oracle has one schedule: the autotuner just steps. */
+/* The oracle stores V in the reference's scan order whatever the layout:
+ * one segment. */
+int oracle_set_layout(oracle_handle* h, int32_t layout) {
+  if (layout != TLM_LAYOUT_X_SEGMENTS && layout != TLM_LAYOUT_ROWS) return fail(h, TLM_ERR_ARG, "bad layout");
+  h->layout = layout;
+  return TLM_OK;
+}
+int oracle_layout_segments(oracle_handle* h, int32_t* n_segments, int32_t* x0) {
+  if (!n_segments) return TLM_ERR_ARG;
+  *n_segments = 1;
+  if (x0) { x0[0] = 0; x0[1] = h->nx; }
+  return TLM_OK;
+}
+
+/* The oracle has one schedule: the autotuner just steps, as many steps as
+ * the engine's candidates take (4, or 10 with TLM_LAYOUT_ROWS). */
 int oracle_autotune(oracle_handle* h, int32_t first_timestep, int32_t steps_per_candidate, int32_t* steps_done) {
-  int n = (9 + (h->ny % 8 == 0)) * (steps_per_candidate + 1);  /* as many as the engine's candidates */
+  int n = (h->layout == TLM_LAYOUT_ROWS ? 10 : 4) * (steps_per_candidate + 1);
   int rc = oracle_step(h, first_timestep, n);
   if (!rc && steps_done) *steps_done = n;
   return rc;
