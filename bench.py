@@ -238,7 +238,7 @@ def main():
         return main_batch(a)
     if a.mode == "serve":
         return main_serve(a)
-    tune_steps = 10 * (a.tune_steps + 1) if a.tune_steps else 0  # at most 10 candidates
+    tune_steps = 14 * (a.tune_steps + 1) if a.tune_steps else 0  # at most 14 candidates
     total = tune_steps + a.warmup + a.steps + a.profile_steps
     t_setup = time.perf_counter()
     if a.workload == "dispersive":

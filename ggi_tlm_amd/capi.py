@@ -70,6 +70,7 @@ SIGNATURES = {
     "kernel_bytes": (C.c_int, [_h, _p, _p]),
     "set_layout": (C.c_int, [_h, _i32]),
     "layout_segments": (C.c_int, [_h, C.POINTER(_i32), _p]),
+    "set_streams": (C.c_int, [_h, _i32]),
 }
 
 LAYOUT_X_SEGMENTS = 0
@@ -285,6 +286,10 @@ class Handle:
         x0 = np.zeros(MAX_SEGMENTS + 1, dtype=np.int32)
         self._check(self.lib.fn["layout_segments"](self.h, C.byref(n), _ptr(x0)))
         return [int(v) for v in x0[: n.value + 1]]
+
+    def set_streams(self, streams: int):
+        """1: a step's kernels in turn on one stream; 2: class kernels concurrent."""
+        self._check(self.lib.fn["set_streams"](self.h, int(streams)))
 
     def set_schedule(self, variant: int, rows: int, zc: int, xcd_remap: int = 0, mixed: int = -1):
         self._check(self.lib.fn["set_schedule"](self.h, int(variant), int(rows), int(zc), int(xcd_remap),

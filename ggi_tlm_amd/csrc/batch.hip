@@ -327,8 +327,7 @@ int rebuild(tlm_batch* b) {
     s.dft = h->d_dft;
     s.dt = h->g.dt;
     if (e.ts > e.last) continue;  // finished, awaiting collection
-    const long long nl = (h->P + 255) / 256;
-    for (long long k = 0; k < nl; k++) tab[tlm_batch::T_LINES].push_back(make_int2(i, (int)k));
+    for (int32_t k : h->bulk_blocks) tab[tlm_batch::T_LINES].push_back(make_int2(i, k));
     const int n_mixed_wg = b->form == TLM_FORM_PLAIN_FP64 ? h->n_lds_wg : h->n_mixed_blocks;
     for (int k = 0; k < n_mixed_wg; k++) tab[tlm_batch::T_MIXED].push_back(make_int2(i, k));
     auto add = [&](int t, long long cnt) {

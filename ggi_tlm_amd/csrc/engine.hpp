@@ -73,6 +73,13 @@ struct tlm_handle {
   double stz_value = 0.0;
   long long n_mixed_cells = 0;
   int mixed_variant = -1;                 // 0: lane slots, 1: LDS-staged, -1: per form
+  // workgroups of the bulk line kernel: 256-cell storage blocks holding a
+  // pure bulk line, in storage order (XCD mappings 0, 1) and interleaved by
+  // y-band (mapping 2; -1 entries idle)
+  std::vector<int32_t> bulk_blocks;
+  const int32_t* d_bulk_blocks = nullptr;
+  const int32_t* d_bulk_blocks_yband = nullptr;
+  int n_bulk_blocks_yband = 0;
   const int32_t* d_lds_lines = nullptr;   // LDS-staged mixed-line workgroups
   const int32_t* d_lds_fd = nullptr;
   const int32_t* d_lds_slot_p = nullptr;
@@ -91,8 +98,11 @@ struct tlm_handle {
   int64_t dev_bytes = 0;
 
   // bulk schedule (tlm_set_schedule)
-  int sched_variant = -1, sched_by = 4, sched_zc = 16, sched_xcd = 0;  // -1: per-form default
+  int sched_variant = -1, sched_by = 1, sched_zc = 16, sched_xcd = 0;  // -1: per-form default
   int layout = TLM_LAYOUT_X_SEGMENTS;     // tlm_set_layout
+  int streams = 2;                        // tlm_set_streams
+  hipStream_t side = nullptr;             // second stream of a step (created on first use)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   // state machine
   Kind kind = Kind::INCIDENT;

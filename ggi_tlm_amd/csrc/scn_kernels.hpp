@@ -675,6 +675,10 @@ __device__ inline void bulk_update(const Dev& d, const T* __restrict__ Lin, T* _
   const long long sy = seg_width(d, sg), sz = sy * d.ny;
   double v[12];
   if (PULL) {
+    // Own values stream every line of the cell's planes in once; the
+    // neighbour reads of the same lines then hit the caches.  (Reading only
+    // the facing ports, which the precision form's ordinary faces need, was
+    // measured slower.)
     double o[12];
 #pragma unroll
     for (int p = 0; p < 12; p++) o[p] = (double)Lin[p * P + c];
@@ -1493,11 +1497,29 @@ __device__ inline void bulk_line_cell(const Dev& d, const T* __restrict__ Lin, T
   bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z, sg);
 }
 
+// blocks[0..n_blocks): the storage blocks to update (-1: none).  A
+// workgroup takes per_wg of them in turn: consecutive entries (mode 0, 1;
+// mode 1 gives each XCD a contiguous range of workgroups), or entries 8
+// apart starting at its XCD's column (mode 2, the y-band table).
 template <typename T, bool PLAIN, bool PULL>
 __global__ __launch_bounds__(256) void k_bulk_lines(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                   PullArgs a, int xcd_remap) {
-  const int b = xcd_block(blockIdx.x, gridDim.x, xcd_remap);
-  bulk_line_cell<T, PLAIN, PULL>(d, Lin, Lout, a, (long long)b * 256 + threadIdx.x);
+                                                   PullArgs a, const int32_t* __restrict__ blocks, int n_blocks,
+                                                   int per_wg, int mode) {
+  const int g = blockIdx.x;
+  int base, stride;
+  if (mode == 2) {
+    base = 8 * (g >> 3) * per_wg + (g & 7);
+    stride = 8;
+  } else {
+    base = xcd_block(g, gridDim.x, mode) * per_wg;
+    stride = 1;
+  }
+  for (int k = 0; k < per_wg; k++) {
+    const int idx = base + k * stride;
+    if (idx >= n_blocks) break;
+    const int b = blocks[idx];
+    if (b >= 0) bulk_line_cell<T, PLAIN, PULL>(d, Lin, Lout, a, (long long)b * 256 + threadIdx.x);
+  }
 }
 
 // kMixedBulkPerLane bulk cells per lane of the mixed-line kernel: their

@@ -524,6 +524,44 @@ int finalize(tlm_handle* h) {
   }
   d.line_len = L;
   if ((rc = dev_upload(h, line_info, &d.line_info))) return rc;
+  {
+    // Workgroups of the bulk line kernel: the 256-cell storage blocks that
+    // hold a pure bulk line.  The y-band table splits the blocks of every
+    // z-plane of a segment into eight equal runs in storage order and gives
+    // run k to XCD k (workgroup b % 8 == k), so a block's y-1 and z-1
+    // neighbour rows were read just before by the same XCD (its own L2) and
+    // every XCD gets the same share of each plane.
+    const int lpb = 256 / L;
+    h->bulk_blocks.clear();
+    std::vector<long long> plane_of;  // (segment, z) key per listed block
+    for (long long b = 0; b < (S + 255) / 256; b++) {
+      long long first = -1;
+      for (int q = 0; q < lpb && first < 0 && b * lpb + q < n_lines; q++)
+        if (line_info[b * lpb + q] == 0) first = (b * lpb + q) * L;
+      if (first < 0) continue;
+      h->bulk_blocks.push_back((int32_t)b);
+      int sg = 0;
+      while (sg + 1 < nseg && first >= seg_base[sg + 1]) sg++;
+      const long long row = (first - seg_base[sg]) / (seg_x0[sg + 1] - seg_x0[sg]);
+      plane_of.push_back((long long)sg * nz + row / ny);
+    }
+    std::vector<std::vector<int32_t>> band(8);
+    for (size_t i0 = 0; i0 < plane_of.size();) {
+      size_t i1 = i0;
+      while (i1 < plane_of.size() && plane_of[i1] == plane_of[i0]) i1++;
+      const size_t m = i1 - i0;
+      for (size_t j = 0; j < m; j++) band[j * 8 / m].push_back(h->bulk_blocks[i0 + j]);
+      i0 = i1;
+    }
+    size_t per = 0;
+    for (auto& v : band) per = std::max(per, v.size());
+    std::vector<int32_t> yb(8 * per, -1);
+    for (int k = 0; k < 8; k++)
+      for (size_t j = 0; j < band[k].size(); j++) yb[8 * j + k] = band[k][j];
+    if ((rc = dev_upload(h, h->bulk_blocks, &h->d_bulk_blocks))) return rc;
+    if ((rc = dev_upload(h, yb, &h->d_bulk_blocks_yband))) return rc;
+    h->n_bulk_blocks_yband = (int)yb.size();
+  }
   if ((rc = dev_upload(h, lds_lines, &h->d_lds_lines))) return rc;
   if ((rc = dev_upload(h, lds_fd, &h->d_lds_fd))) return rc;
   if ((rc = dev_upload(h, lds_slot_p, &h->d_lds_slot_p))) return rc;
@@ -746,14 +784,79 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
   if (pull) launch_huygens_values(h, a);
   const T* Lin = (const T*)h->planes[h->cur];
   T* Lout = (T*)h->planes[h->cur ^ 1];
+  // The class kernels write cells disjoint from the bulk and PML kernels'
+  // and read only the input buffer, so with two streams they run on the
+  // second stream concurrently with those (joined before the next kernel
+  // of the handle's stream).  Profiling serialises them.
+  hipStream_t cs = h->stream;
+  const bool fork = h->streams > 1 && !h->profiling;
+  if (fork) {
+    if (!h->side) {
+      HIP_TRY(h, hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+      HIP_TRY(h, hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+      HIP_TRY(h, hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    }
+    HIP_TRY(h, hipEventRecord(h->ev_fork, h->stream));
+    HIP_TRY(h, hipStreamWaitEvent(h->side, h->ev_fork, 0));
+    cs = h->side;
+  }
+  const int mixed = h->mixed_variant >= 0 ? h->mixed_variant : (PLAIN ? 1 : 0);  // measured faster per form
+  if (mixed == 1 && h->n_lds_wg) {
+    ProfScope ps(h, TLM_KCLASS_MIXED);
+    if (pull)
+      hipLaunchKernelGGL((scn::k_mixed_lds<T, PLAIN, true>), dim3(h->n_lds_wg), dim3(scn::kLdsCells), 0, cs, d, Lin, Lout,
+                         a, h->d_lds_lines, h->d_lds_fd, h->d_lds_slot_p, h->d_lds_slot_i, timestep, slot);
+    else
+      hipLaunchKernelGGL((scn::k_mixed_lds<T, PLAIN, false>), dim3(h->n_lds_wg), dim3(scn::kLdsCells), 0, cs, d, Lin, Lout,
+                         a, h->d_lds_lines, h->d_lds_fd, h->d_lds_slot_p, h->d_lds_slot_i, timestep, slot);
+  } else if (h->n_mixed_blocks) {
+    ProfScope ps(h, TLM_KCLASS_MIXED);
+    dim3 g(h->n_mixed_blocks);
+    if (pull)
+      hipLaunchKernelGGL((scn::k_mixed_lines<T, PLAIN, true>), g, dim3(256), 0, cs, d, Lin, Lout, a,
+                         h->d_slot_cell, h->d_slot_idx, (long long)h->n_mixed_blocks * 256, timestep, slot);
+    else
+      hipLaunchKernelGGL((scn::k_mixed_lines<T, PLAIN, false>), g, dim3(256), 0, cs, d, Lin, Lout, a,
+                         h->d_slot_cell, h->d_slot_idx, (long long)h->n_mixed_blocks * 256, timestep, slot);
+  }
+  if (h->n_pure_mat) {
+    ProfScope ps(h, TLM_KCLASS_MATERIAL);
+    dim3 g((h->n_pure_mat + 255) / 256);
+    if (pull) hipLaunchKernelGGL((scn::k_material<T, PLAIN, true>), g, dim3(256), 0, cs, d, Lin, Lout, a, h->n_pure_mat);
+    else hipLaunchKernelGGL((scn::k_material<T, PLAIN, false>), g, dim3(256), 0, cs, d, Lin, Lout, a, h->n_pure_mat);
+  }
+  if (h->n_pure_gen) {
+    ProfScope ps(h, TLM_KCLASS_GENERIC);
+    dim3 g((h->n_pure_gen + 255) / 256);
+    if (pull)
+      hipLaunchKernelGGL((scn::k_generic<T, PLAIN, true>), g, dim3(256), 0, cs, d, Lin, Lout, a, timestep, slot,
+                         h->n_pure_gen);
+    else
+      hipLaunchKernelGGL((scn::k_generic<T, PLAIN, false>), g, dim3(256), 0, cs, d, Lin, Lout, a, timestep, slot,
+                         h->n_pure_gen);
+  }
+  if (h->n_pm) {
+    ProfScope ps(h, TLM_KCLASS_PML_MATERIAL);
+    dim3 g((h->n_pm + 255) / 256);
+    if (pull) hipLaunchKernelGGL((scn::k_pml_material<T, PLAIN, true>), g, dim3(256), 0, cs, d, Lin, Lout, a, slot);
+    else hipLaunchKernelGGL((scn::k_pml_material<T, PLAIN, false>), g, dim3(256), 0, cs, d, Lin, Lout, a, slot);
+  }
   {
     ProfScope ps(h, TLM_KCLASS_BULK);
     if (sched_variant(h) == 3) {
-      dim3 g((unsigned)((d.seg_base[d.nseg] + 255) / 256));
-      if (pull)
-        hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, h->sched_xcd);
-      else
-        hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, h->sched_xcd);
+      const bool yband = h->sched_xcd == 2;
+      const int32_t* blocks = yband ? h->d_bulk_blocks_yband : h->d_bulk_blocks;
+      const int nb = yband ? h->n_bulk_blocks_yband : (int)h->bulk_blocks.size();
+      const int per = h->sched_by;  // storage blocks per workgroup
+      const int nwg = yband ? 8 * ((nb / 8 + per - 1) / per) : (nb + per - 1) / per;
+      if (nb) {
+        if (pull)
+          hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, true>), dim3(nwg), dim3(256), 0, h->stream, d, Lin, Lout, a,
+                             blocks, nb, per, h->sched_xcd);
+        else
+          hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, false>), dim3(nwg), dim3(256), 0, h->stream, d, Lin, Lout, a,
+                             blocks, nb, per, h->sched_xcd);
+      }
     } else {
       scn::BulkMap m = bulk_map(h);
       dim3 bb(64, m.by, 1);
@@ -771,52 +874,15 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
       }
     }
   }
-  const int mixed = h->mixed_variant >= 0 ? h->mixed_variant : (PLAIN ? 1 : 0);  // measured faster per form
-  if (mixed == 1 && h->n_lds_wg) {
-    ProfScope ps(h, TLM_KCLASS_MIXED);
-    if (pull)
-      hipLaunchKernelGGL((scn::k_mixed_lds<T, PLAIN, true>), dim3(h->n_lds_wg), dim3(scn::kLdsCells), 0, h->stream, d, Lin, Lout,
-                         a, h->d_lds_lines, h->d_lds_fd, h->d_lds_slot_p, h->d_lds_slot_i, timestep, slot);
-    else
-      hipLaunchKernelGGL((scn::k_mixed_lds<T, PLAIN, false>), dim3(h->n_lds_wg), dim3(scn::kLdsCells), 0, h->stream, d, Lin, Lout,
-                         a, h->d_lds_lines, h->d_lds_fd, h->d_lds_slot_p, h->d_lds_slot_i, timestep, slot);
-  } else if (h->n_mixed_blocks) {
-    ProfScope ps(h, TLM_KCLASS_MIXED);
-    dim3 g(h->n_mixed_blocks);
-    if (pull)
-      hipLaunchKernelGGL((scn::k_mixed_lines<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a,
-                         h->d_slot_cell, h->d_slot_idx, (long long)h->n_mixed_blocks * 256, timestep, slot);
-    else
-      hipLaunchKernelGGL((scn::k_mixed_lines<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a,
-                         h->d_slot_cell, h->d_slot_idx, (long long)h->n_mixed_blocks * 256, timestep, slot);
-  }
   if (h->n_pure_pml) {
     ProfScope ps(h, TLM_KCLASS_PML);
     dim3 g((h->n_pure_pml + 255) / 256);
     if (pull) hipLaunchKernelGGL((scn::k_pml<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot, h->n_pure_pml);
     else hipLaunchKernelGGL((scn::k_pml<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot, h->n_pure_pml);
   }
-  if (h->n_pure_mat) {
-    ProfScope ps(h, TLM_KCLASS_MATERIAL);
-    dim3 g((h->n_pure_mat + 255) / 256);
-    if (pull) hipLaunchKernelGGL((scn::k_material<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, h->n_pure_mat);
-    else hipLaunchKernelGGL((scn::k_material<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, h->n_pure_mat);
-  }
-  if (h->n_pure_gen) {
-    ProfScope ps(h, TLM_KCLASS_GENERIC);
-    dim3 g((h->n_pure_gen + 255) / 256);
-    if (pull)
-      hipLaunchKernelGGL((scn::k_generic<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, timestep, slot,
-                         h->n_pure_gen);
-    else
-      hipLaunchKernelGGL((scn::k_generic<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, timestep, slot,
-                         h->n_pure_gen);
-  }
-  if (h->n_pm) {
-    ProfScope ps(h, TLM_KCLASS_PML_MATERIAL);
-    dim3 g((h->n_pm + 255) / 256);
-    if (pull) hipLaunchKernelGGL((scn::k_pml_material<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot);
-    else hipLaunchKernelGGL((scn::k_pml_material<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot);
+  if (fork) {
+    HIP_TRY(h, hipEventRecord(h->ev_join, h->side));
+    HIP_TRY(h, hipStreamWaitEvent(h->stream, h->ev_join, 0));
   }
   return TLM_OK;
 }
@@ -939,12 +1005,16 @@ int tlm_create(const tlm_grid* g, int32_t form, tlm_handle** out) {
 void tlm_destroy(tlm_handle* h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->side) (void)hipStreamSynchronize(h->side);
   for (auto& e : h->ev) {
     (void)hipEventDestroy(e.second.first);
     (void)hipEventDestroy(e.second.second);
   }
   for (auto& b : h->bufs) (void)hipFree(b.p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->side) (void)hipStreamDestroy(h->side);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
   delete h;
 }
 
@@ -1273,14 +1343,22 @@ int tlm_layout_segments(tlm_handle* h, int32_t* n_segments, int32_t* x0) {
   return TLM_OK;
 }
 
+int tlm_set_streams(tlm_handle* h, int32_t streams) {
+  if (!h) return TLM_ERR_ARG;
+  if (streams != 1 && streams != 2) return fail(h, TLM_ERR_ARG, "streams must be 1 or 2");
+  h->streams = streams;
+  return TLM_OK;
+}
+
 int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap, int32_t mixed) {
   if (!h) return TLM_ERR_ARG;
   if (mixed < -1 || mixed > 1) return fail(h, TLM_ERR_ARG, "bad mixed-line kernel");
-  if (variant < 0 || variant > 3 || (rows != 1 && rows != 2 && rows != 4) || zc < 1)
+  if (variant < 0 || variant > 3 || (rows != 1 && rows != 2 && rows != 4 && (variant != 3 || (rows != 8 && rows != 16))) ||
+      zc < 1)
     return fail(h, TLM_ERR_ARG, "bad bulk schedule");
   if (variant != 3 && h->layout != TLM_LAYOUT_ROWS)
     return fail(h, TLM_ERR_ARG, "the tile schedules (variants 0-2) need TLM_LAYOUT_ROWS");
-  if (xcd_remap < 0 || xcd_remap > 1) return fail(h, TLM_ERR_ARG, "bad XCD mapping");
+  if (xcd_remap < 0 || xcd_remap > 2 || (xcd_remap == 2 && variant != 3)) return fail(h, TLM_ERR_ARG, "bad XCD mapping");
   h->mixed_variant = mixed;
   h->sched_variant = variant;
   h->sched_by = rows;
@@ -1293,11 +1371,14 @@ int tlm_autotune(tlm_handle* h, int32_t first_timestep, int32_t steps_per_candid
   if (!h || steps_per_candidate < 1 || first_timestep < 1) return TLM_ERR_ARG;
   int rc;
   if ((rc = finalize(h))) return rc;
-  struct Cand { int v, rows, zc, xcd, mixed; };
-  std::vector<Cand> cand = {{3, 1, 1, 0, 1}, {3, 1, 1, 0, 0}, {3, 1, 1, 1, 1}, {3, 1, 1, 1, 0}};
+  struct Cand { int v, rows, zc, xcd, mixed, streams; };
+  std::vector<Cand> cand;
+  for (int st = 2; st >= 1; st--)
+    for (int xcd = 0; xcd < 3; xcd += 2)
+      for (int mixed = 1; mixed >= 0; mixed--) cand.push_back({3, 1, 1, xcd, mixed, st});
   if (h->layout == TLM_LAYOUT_ROWS) {
-    const Cand tiles[] = {{0, 1, 1, 0, 1}, {0, 2, 1, 0, 1}, {0, 4, 1, 0, 1}, {2, 1, 1, 0, 1}, {2, 4, 1, 0, 1},
-                          {1, 4, 8, 0, 1}};
+    const Cand tiles[] = {{0, 1, 1, 0, 1, 2}, {0, 2, 1, 0, 1, 2}, {0, 4, 1, 0, 1, 2}, {2, 1, 1, 0, 1, 2},
+                          {2, 4, 1, 0, 1, 2}, {1, 4, 8, 0, 1, 2}};
     cand.insert(cand.end(), std::begin(tiles), std::end(tiles));
   }
   const int total = (int)cand.size() * (steps_per_candidate + 1);
@@ -1314,6 +1395,7 @@ int tlm_autotune(tlm_handle* h, int32_t first_timestep, int32_t steps_per_candid
     h->sched_zc = cand[k].zc;
     h->sched_xcd = cand[k].xcd;
     h->mixed_variant = cand[k].mixed;
+    h->streams = cand[k].streams;
     if ((rc = tlm_step(h, ts, 1))) break;  // first launch of this schedule, untimed
     ts += 1;
     if ((rc = hip_ok(h, hipEventRecord(e0, h->stream)))) break;
@@ -1336,6 +1418,7 @@ int tlm_autotune(tlm_handle* h, int32_t first_timestep, int32_t steps_per_candid
   h->sched_zc = cand[best].zc;
   h->sched_xcd = cand[best].xcd;
   h->mixed_variant = cand[best].mixed;
+  h->streams = cand[best].streams;
   if (steps_done) *steps_done = ts - first_timestep;
   return TLM_OK;
 }

@@ -223,6 +223,12 @@ module tlm_gpu_iface
       integer(c_int32_t), value :: layout
     end function tlm_set_layout
 
+    integer(c_int) function tlm_set_streams(h, streams) bind(c, name='tlm_set_streams')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: streams
+    end function tlm_set_streams
+
     integer(c_int) function tlm_layout_segments(h, n_segments, x0) bind(c, name='tlm_layout_segments')
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h

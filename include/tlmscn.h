@@ -21,7 +21,8 @@
  * Fortran wrapper module on a nonzero status.
  *
  * Threading: one handle per solve, driven from one host thread; each handle
- * owns one HIP stream.  Calls on one handle are not re-entrant.
+ * owns its HIP streams (see tlm_set_streams).  Calls on one handle are not
+ * re-entrant.
  */
 #ifndef TLMSCN_H
 #define TLMSCN_H
@@ -241,14 +242,14 @@ int tlm_read_dft(tlm_handle* h, double* re_im);
 
 /* ---- schedule and measurement (no counterpart in the reference) ---------- */
 
-/* Autotuner of the bulk schedule (no counterpart in the reference): runs
+/* Autotuner of the step schedule (no counterpart in the reference): runs
  * steps first_timestep .. first_timestep + *steps_done - 1 of the solve,
  * each candidate schedule (line kernel with and without the XCD mapping,
- * each mixed-line kernel; with TLM_LAYOUT_ROWS also tile kernels of 1/2/4
- * rows, vector kernel, z-marching kernel: 4 candidates, or 10) for
- * steps_per_candidate timed steps after one untimed step, and keeps the
- * fastest.  Every schedule gives the same results, so these are ordinary
- * steps of the solve. */
+ * each mixed-line kernel, one or two streams; with TLM_LAYOUT_ROWS also
+ * tile kernels of 1/2/4 rows, vector kernel, z-marching kernel: 8
+ * candidates, or 14) for steps_per_candidate timed steps after one untimed
+ * step, and keeps the fastest.  Every schedule gives the same results, so
+ * these are ordinary steps of the solve. */
 int tlm_autotune(tlm_handle* h, int32_t first_timestep, int32_t steps_per_candidate, int32_t* steps_done);
 
 /* TLM_set_small_to_zero (TLM_SOLVER/TLM_set_small_to_zero.F90:34-68, gated
@@ -284,7 +285,9 @@ int tlm_layout_segments(tlm_handle* h, int32_t* n_segments, int32_t* x0);
  * 2 = 16-byte vector accesses, several consecutive cells per lane (falls
  * back to 0 when nx is not a multiple of the vector width); rows = tile
  * height (1, 2 or 4 rows); xcd_remap = 1: give each XCD a contiguous range
- * of workgroups.
+ * of workgroups; 2 (line kernel only): XCD k takes the blocks of the k-th
+ * y-band of every z-plane in storage order, so the y-1 and z-1 neighbour
+ * rows of a block were read by the same XCD just before (its own L2).
  * mixed: mixed-line kernel, 1 = LDS-staged cell-order loads and stores
  * around class-sorted node updates (default in the plain form), 0 =
  * class-sorted lanes with four bulk cells per lane (default in the
@@ -293,6 +296,15 @@ int tlm_layout_segments(tlm_handle* h, int32_t* n_segments, int32_t* x0);
  * (tlm_autotune picks one by timing). */
 int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap,
                      int32_t mixed);
+
+/* Streams of a step (no counterpart in the reference): 1 = every kernel of
+ * a step on the handle's stream in turn; 2 (default) = the special-cell
+ * class kernels (mixed lines, material, generic, PML cells holding a
+ * material) on a second stream, concurrent with the bulk and PML kernels
+ * (they write disjoint cells and read only the previous step's buffer),
+ * joined before the step ends.  Profiling serialises them.  Results do not
+ * depend on it. */
+int tlm_set_streams(tlm_handle* h, int32_t streams);
 
 /* Kernel classes timed by the profiling hooks. */
 #define TLM_KCLASS_BULK 0

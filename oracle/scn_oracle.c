@@ -1359,10 +1359,15 @@ int oracle_layout_segments(oracle_handle* h, int32_t* n_segments, int32_t* x0) {
   return TLM_OK;
 }
 
+int oracle_set_streams(oracle_handle* h, int32_t streams) {
+  if (streams != 1 && streams != 2) return fail(h, TLM_ERR_ARG, "streams must be 1 or 2");
+  return TLM_OK;
+}
+
 /* The oracle has one schedule: the autotuner just steps, as many steps as
- * the engine's candidates take (4, or 10 with TLM_LAYOUT_ROWS). */
+ * the engine's candidates take (8, or 14 with TLM_LAYOUT_ROWS). */
 int oracle_autotune(oracle_handle* h, int32_t first_timestep, int32_t steps_per_candidate, int32_t* steps_done) {
-  int n = (h->layout == TLM_LAYOUT_ROWS ? 10 : 4) * (steps_per_candidate + 1);
+  int n = (h->layout == TLM_LAYOUT_ROWS ? 14 : 8) * (steps_per_candidate + 1);
   int rc = oracle_step(h, first_timestep, n);
   if (!rc && steps_done) *steps_done = n;
   return rc;

@@ -130,7 +130,8 @@ def test_precision_form(name, oracle_lib):
 
 
 @pytest.mark.parametrize("sched", [(0, 1, 1, 0), (0, 2, 1, 1), (1, 1, 4, 1), (1, 4, 8, 0), (1, 2, 32, 1),
-                                   (2, 4, 1, 0), (2, 1, 1, 1), (3, 1, 1, 0), (3, 1, 1, 1),
+                                   (2, 4, 1, 0), (2, 1, 1, 1), (3, 1, 1, 0), (3, 1, 1, 1), (3, 1, 1, 2),
+                                   (3, 4, 1, 2), (3, 2, 1, 0), (3, 8, 1, 1),
                                    (3, 1, 1, 0, 0), (3, 1, 1, 1, 0), (0, 4, 1, 0, 1)])
 def test_schedules_bitwise(sched, oracle_lib):
     """Every bulk schedule gives the oracle's result bit for bit (the tile
@@ -236,6 +237,23 @@ def test_layouts_bitwise(name, form):
         g.close()
     for a, b in zip(*res):
         assert np.array_equal(a, b), name
+
+
+@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
+def test_streams_bitwise(form):
+    """The class kernels on a second stream (default) or in turn on one give
+    the same bits."""
+    t = build_tables(CASES["pml_sphere_64"]())
+    steps = 40
+    res = []
+    for streams in (1, 2):
+        g = engine(t, form)
+        g.set_streams(streams)
+        g.step(1, steps)
+        res.append((g.read_cell_outputs()[0], g.download_links()[:, 1:-1, 1:-1]))
+        g.close()
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
 
 
 def test_layout_cuts_pml_slabs():
