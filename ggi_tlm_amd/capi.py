@@ -92,6 +92,7 @@ BATCH_SIGNATURES = {
     "batch_collect": (C.c_int, [_h, _p, _i32, C.POINTER(_i32)]),
     "batch_info": (C.c_int, [_h, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(C.c_int64)]),
     "batch_set_graph_steps": (C.c_int, [_h, _i32]),
+    "batch_set_kernels": (C.c_int, [_h, _i32, _i32]),
     "batch_autotune": (C.c_int, [_h, _i32, C.POINTER(_i32), C.POINTER(_i32)]),
 }
 
@@ -404,6 +405,9 @@ class Batch:
 
     def set_graph_steps(self, k: int):
         self._check(self.lib.fn["batch_set_graph_steps"](self.b, int(k)))
+
+    def set_kernels(self, mixed: int = -1, streams: int = 2):
+        self._check(self.lib.fn["batch_set_kernels"](self.b, int(mixed), int(streams)))
 
     def autotune(self, steps_per_candidate: int):
         k, done = _i32(0), _i32(0)

@@ -70,6 +70,10 @@ struct tlm_batch {
   size_t cap_tab[T_N] = {};
   bool dirty = true;
   int graph_steps = 32;
+  int mixed = -1;    // mixed-line kernel: 1 LDS-staged, 0 lane slots, -1 per form
+  int streams = 2;   // 2: class kernels on a second stream inside each step
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraphExec_t graph = nullptr;
   int graph_k = 0;
   double last_ms_per_step = 0.0;
@@ -256,22 +260,39 @@ __global__ void kb_advance(BatchSolve* S, int n) {
   s.ts += 1;
 }
 
+bool batch_lds(const tlm_batch* b) { return b->mixed != 0; }  // LDS-staged unless lane slots are asked for
+
+// One batch step.  With two streams the class kernels (mixed lines,
+// materials, generic cells, PML cells holding a material) run on the side
+// stream concurrently with the line and PML kernels: they write disjoint
+// cells and read only the previous step's buffers; the probe DFT and the
+// advance follow the join.
 template <typename T, bool PLAIN>
 void launch_batch_step(tlm_batch* b) {
   const hipStream_t st = b->stream;
+  const bool fork = b->streams > 1;
+  const hipStream_t cs = fork ? b->side : st;
   BatchSolve* S = b->d_solves;
-  auto go = [&](int t, void (*k)(const BatchSolve*, const int2*), int threads = 256) {
-    if (b->n_tab[t]) hipLaunchKernelGGL(k, dim3(b->n_tab[t]), dim3(threads), 0, st, S, b->d_tab[t]);
+  auto go = [&](hipStream_t s, int t, void (*k)(const BatchSolve*, const int2*), int threads = 256) {
+    if (b->n_tab[t]) hipLaunchKernelGGL(k, dim3(b->n_tab[t]), dim3(threads), 0, s, S, b->d_tab[t]);
   };
-  go(tlm_batch::T_HUY, kb_huygens);
-  go(tlm_batch::T_LINES, kb_lines<T, PLAIN>);
-  if (PLAIN) go(tlm_batch::T_MIXED, kb_mixed_lds<T, PLAIN>, scn::kLdsCells);  // as the single-solve defaults
-  else go(tlm_batch::T_MIXED, kb_mixed<T, PLAIN>);
-  go(tlm_batch::T_PML, kb_list<T, PLAIN, 0>);
-  go(tlm_batch::T_MAT, kb_list<T, PLAIN, 1>);
-  go(tlm_batch::T_GEN, kb_list<T, PLAIN, 2>);
-  go(tlm_batch::T_PM, kb_list<T, PLAIN, 3>);
-  go(tlm_batch::T_DFT, kb_dft);
+  go(st, tlm_batch::T_HUY, kb_huygens);
+  if (fork) {
+    (void)hipEventRecord(b->ev_fork, st);
+    (void)hipStreamWaitEvent(b->side, b->ev_fork, 0);
+  }
+  go(st, tlm_batch::T_LINES, kb_lines<T, PLAIN>);
+  if (batch_lds(b)) go(cs, tlm_batch::T_MIXED, kb_mixed_lds<T, PLAIN>, scn::kLdsCells);
+  else go(cs, tlm_batch::T_MIXED, kb_mixed<T, PLAIN>);
+  go(cs, tlm_batch::T_MAT, kb_list<T, PLAIN, 1>);
+  go(cs, tlm_batch::T_GEN, kb_list<T, PLAIN, 2>);
+  go(cs, tlm_batch::T_PM, kb_list<T, PLAIN, 3>);
+  go(st, tlm_batch::T_PML, kb_list<T, PLAIN, 0>);
+  if (fork) {
+    (void)hipEventRecord(b->ev_join, b->side);
+    (void)hipStreamWaitEvent(st, b->ev_join, 0);
+  }
+  go(st, tlm_batch::T_DFT, kb_dft);
   const int n = (int)b->solves.size();
   hipLaunchKernelGGL(kb_advance, dim3((n + 255) / 256), dim3(256), 0, st, S, n);
 }
@@ -328,7 +349,7 @@ int rebuild(tlm_batch* b) {
     s.dt = h->g.dt;
     if (e.ts > e.last) continue;  // finished, awaiting collection
     for (int32_t k : h->bulk_blocks) tab[tlm_batch::T_LINES].push_back(make_int2(i, k));
-    const int n_mixed_wg = b->form == TLM_FORM_PLAIN_FP64 ? h->n_lds_wg : h->n_mixed_blocks;
+    const int n_mixed_wg = batch_lds(b) ? h->n_lds_wg : h->n_mixed_blocks;
     for (int k = 0; k < n_mixed_wg; k++) tab[tlm_batch::T_MIXED].push_back(make_int2(i, k));
     auto add = [&](int t, long long cnt) {
       for (long long k = 0; k < (cnt + 255) / 256; k++) tab[t].push_back(make_int2(i, (int)k));
@@ -424,8 +445,11 @@ int tlm_batch_create(int32_t form, int64_t hbm_budget_bytes, tlm_batch** out) {
   if (!b) return TLM_ERR_NOMEM;
   b->form = form;
   b->budget = hbm_budget_bytes;
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete b;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess) {
+    tlm_batch_destroy(b);
     return TLM_ERR_HIP;
   }
   *out = b;
@@ -442,6 +466,9 @@ void tlm_batch_destroy(tlm_batch* b) {
   for (auto* t : b->d_tab)
     if (t) (void)hipFree(t);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->side) (void)hipStreamDestroy(b->side);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
   delete b;
 }
 
@@ -545,13 +572,30 @@ int tlm_batch_set_graph_steps(tlm_batch* b, int32_t k) {
   return TLM_OK;
 }
 
+int tlm_batch_set_kernels(tlm_batch* b, int32_t mixed, int32_t streams) {
+  if (!b) return TLM_ERR_ARG;
+  if (mixed < -1 || mixed > 1 || (streams != 1 && streams != 2))
+    return bfail(b, TLM_ERR_ARG, "mixed must be -1, 0 or 1 and streams 1 or 2");
+  if (mixed != b->mixed) b->dirty = true;  // the mixed-line block table changes
+  b->mixed = mixed;
+  if (streams != b->streams && b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  b->streams = streams;
+  return TLM_OK;
+}
+
 int tlm_batch_autotune(tlm_batch* b, int32_t steps_per_candidate, int32_t* best_k, int32_t* steps_done) {
   if (!b || steps_per_candidate < 1) return TLM_ERR_ARG;
-  static const int cand[] = {4, 16, 64};
-  int rc, done = 0, best = b->graph_steps;
+  struct Cand { int k, mixed; };
+  static const Cand cand[] = {{4, 1}, {16, 1}, {64, 1}, {4, 0}, {16, 0}, {64, 0}};
+  int rc, done = 0, best = 0;
   double best_ms = 1e300;
-  for (int k : cand) {
+  for (int c = 0; c < (int)(sizeof cand / sizeof cand[0]); c++) {
     if (!unfinished(b)) break;
+    const int k = cand[c].k;
+    if ((rc = tlm_batch_set_kernels(b, cand[c].mixed, b->streams))) return rc;
     b->graph_steps = k;
     const int n = std::max(k, steps_per_candidate / k * k);
     // first replay captures; time the second
@@ -564,12 +608,13 @@ int tlm_batch_autotune(tlm_batch* b, int32_t steps_per_candidate, int32_t* best_
     done += k + n;
     if (ms < best_ms) {
       best_ms = ms;
-      best = k;
+      best = c;
     }
   }
-  b->graph_steps = best;
+  b->graph_steps = cand[best].k;
+  if ((rc = tlm_batch_set_kernels(b, cand[best].mixed, b->streams))) return rc;
   b->last_ms_per_step = best_ms;
-  if (best_k) *best_k = best;
+  if (best_k) *best_k = cand[best].k;
   if (steps_done) *steps_done = done;
   return TLM_OK;
 }

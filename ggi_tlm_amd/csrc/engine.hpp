@@ -72,7 +72,7 @@ struct tlm_handle {
   int stz_n = 0;        // TLM_set_small_to_zero every stz_n steps (0: off)
   double stz_value = 0.0;
   long long n_mixed_cells = 0;
-  int mixed_variant = -1;                 // 0: lane slots, 1: LDS-staged, -1: per form
+  int mixed_variant = -1;                 // 0: lane slots, 1: LDS-staged, -1: default (LDS-staged)
   // workgroups of the bulk line kernel: 256-cell storage blocks holding a
   // pure bulk line, in storage order (XCD mappings 0, 1) and interleaved by
   // y-band (mapping 2; -1 entries idle)

@@ -1497,14 +1497,22 @@ __device__ inline void bulk_line_cell(const Dev& d, const T* __restrict__ Lin, T
   bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z, sg);
 }
 
-// blocks[0..n_blocks): the storage blocks to update (-1: none).  A
-// workgroup takes per_wg of them in turn: consecutive entries (mode 0, 1;
-// mode 1 gives each XCD a contiguous range of workgroups), or entries 8
-// apart starting at its XCD's column (mode 2, the y-band table).
+// Workgroup g updates storage block g (mode 1: each XCD takes a contiguous
+// range of blocks).
 template <typename T, bool PLAIN, bool PULL>
 __global__ __launch_bounds__(256) void k_bulk_lines(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                   PullArgs a, const int32_t* __restrict__ blocks, int n_blocks,
-                                                   int per_wg, int mode) {
+                                                   PullArgs a, int mode) {
+  const int b = xcd_block(blockIdx.x, gridDim.x, mode);
+  bulk_line_cell<T, PLAIN, PULL>(d, Lin, Lout, a, (long long)b * 256 + threadIdx.x);
+}
+
+// blocks[0..n_blocks) lists the storage blocks to update (-1: none); a
+// workgroup takes per_wg of them in turn: consecutive entries (mode 0, 1),
+// or entries 8 apart from its XCD's column (mode 2, the y-band table).
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(256) void k_bulk_blocks(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                    PullArgs a, const int32_t* __restrict__ blocks, int n_blocks,
+                                                    int per_wg, int mode) {
   const int g = blockIdx.x;
   int base, stride;
   if (mode == 2) {

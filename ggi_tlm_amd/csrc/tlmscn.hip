@@ -787,7 +787,8 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
   // The class kernels write cells disjoint from the bulk and PML kernels'
   // and read only the input buffer, so with two streams they run on the
   // second stream concurrently with those (joined before the next kernel
-  // of the handle's stream).  Profiling serialises them.
+  // of the handle's stream).  Profiling serialises them.  The bulk kernel
+  // goes first (measured faster).
   hipStream_t cs = h->stream;
   const bool fork = h->streams > 1 && !h->profiling;
   if (fork) {
@@ -800,7 +801,50 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
     HIP_TRY(h, hipStreamWaitEvent(h->side, h->ev_fork, 0));
     cs = h->side;
   }
-  const int mixed = h->mixed_variant >= 0 ? h->mixed_variant : (PLAIN ? 1 : 0);  // measured faster per form
+  {
+    ProfScope ps(h, TLM_KCLASS_BULK);
+    if (sched_variant(h) == 3) {
+      // one storage block per workgroup over the whole storage (measured
+      // faster than a block table), or the block tables
+      const bool yband = h->sched_xcd == 2, direct = !yband && h->sched_by == 1;
+      const int32_t* blocks = direct ? nullptr : (yband ? h->d_bulk_blocks_yband : h->d_bulk_blocks);
+      const int nb = yband ? h->n_bulk_blocks_yband : (int)h->bulk_blocks.size();
+      const int per = h->sched_by;  // storage blocks per workgroup
+      const int nwg = direct ? (int)((d.seg_base[d.nseg] + 255) / 256)
+                             : (yband ? 8 * ((nb / 8 + per - 1) / per) : (nb + per - 1) / per);
+      if (direct) {
+        if (pull)
+          hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, true>), dim3(nwg), dim3(256), 0, h->stream, d, Lin, Lout, a,
+                             h->sched_xcd);
+        else
+          hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, false>), dim3(nwg), dim3(256), 0, h->stream, d, Lin, Lout, a,
+                             h->sched_xcd);
+      } else if (nb) {
+        if (pull)
+          hipLaunchKernelGGL((scn::k_bulk_blocks<T, PLAIN, true>), dim3(nwg), dim3(256), 0, h->stream, d, Lin, Lout, a,
+                             blocks, nb, per, h->sched_xcd);
+        else
+          hipLaunchKernelGGL((scn::k_bulk_blocks<T, PLAIN, false>), dim3(nwg), dim3(256), 0, h->stream, d, Lin, Lout, a,
+                             blocks, nb, per, h->sched_xcd);
+      }
+    } else {
+      scn::BulkMap m = bulk_map(h);
+      dim3 bb(64, m.by, 1);
+      const int w = bulk_vec_width(h);
+      if (w > 1) {
+        constexpr int W = sizeof(T) == 4 ? 4 : 2;
+        if (pull) hipLaunchKernelGGL((scn::k_bulk_vec<T, PLAIN, true, W>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+        else hipLaunchKernelGGL((scn::k_bulk_vec<T, PLAIN, false, W>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+      } else if (m.zc == 1) {
+        if (pull) hipLaunchKernelGGL((scn::k_bulk_cell<T, PLAIN, true>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+        else hipLaunchKernelGGL((scn::k_bulk_cell<T, PLAIN, false>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+      } else {
+        if (pull) hipLaunchKernelGGL((scn::k_bulk_z<T, PLAIN, true>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+        else hipLaunchKernelGGL((scn::k_bulk_z<T, PLAIN, false>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
+      }
+    }
+  }
+  const int mixed = h->mixed_variant >= 0 ? h->mixed_variant : 1;  // LDS-staged: measured faster in both forms
   if (mixed == 1 && h->n_lds_wg) {
     ProfScope ps(h, TLM_KCLASS_MIXED);
     if (pull)
@@ -840,39 +884,6 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
     dim3 g((h->n_pm + 255) / 256);
     if (pull) hipLaunchKernelGGL((scn::k_pml_material<T, PLAIN, true>), g, dim3(256), 0, cs, d, Lin, Lout, a, slot);
     else hipLaunchKernelGGL((scn::k_pml_material<T, PLAIN, false>), g, dim3(256), 0, cs, d, Lin, Lout, a, slot);
-  }
-  {
-    ProfScope ps(h, TLM_KCLASS_BULK);
-    if (sched_variant(h) == 3) {
-      const bool yband = h->sched_xcd == 2;
-      const int32_t* blocks = yband ? h->d_bulk_blocks_yband : h->d_bulk_blocks;
-      const int nb = yband ? h->n_bulk_blocks_yband : (int)h->bulk_blocks.size();
-      const int per = h->sched_by;  // storage blocks per workgroup
-      const int nwg = yband ? 8 * ((nb / 8 + per - 1) / per) : (nb + per - 1) / per;
-      if (nb) {
-        if (pull)
-          hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, true>), dim3(nwg), dim3(256), 0, h->stream, d, Lin, Lout, a,
-                             blocks, nb, per, h->sched_xcd);
-        else
-          hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, false>), dim3(nwg), dim3(256), 0, h->stream, d, Lin, Lout, a,
-                             blocks, nb, per, h->sched_xcd);
-      }
-    } else {
-      scn::BulkMap m = bulk_map(h);
-      dim3 bb(64, m.by, 1);
-      const int w = bulk_vec_width(h);
-      if (w > 1) {
-        constexpr int W = sizeof(T) == 4 ? 4 : 2;
-        if (pull) hipLaunchKernelGGL((scn::k_bulk_vec<T, PLAIN, true, W>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
-        else hipLaunchKernelGGL((scn::k_bulk_vec<T, PLAIN, false, W>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
-      } else if (m.zc == 1) {
-        if (pull) hipLaunchKernelGGL((scn::k_bulk_cell<T, PLAIN, true>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
-        else hipLaunchKernelGGL((scn::k_bulk_cell<T, PLAIN, false>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
-      } else {
-        if (pull) hipLaunchKernelGGL((scn::k_bulk_z<T, PLAIN, true>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
-        else hipLaunchKernelGGL((scn::k_bulk_z<T, PLAIN, false>), dim3(m.nblocks), bb, 0, h->stream, d, Lin, Lout, a, m);
-      }
-    }
   }
   if (h->n_pure_pml) {
     ProfScope ps(h, TLM_KCLASS_PML);

@@ -45,8 +45,15 @@ int tlm_batch_collect(tlm_batch* b, tlm_handle** done, int32_t max_done, int32_t
 int tlm_batch_info(const tlm_batch* b, int32_t* n_solves, int32_t* n_unfinished, int64_t* resident_bytes);
 /* Steps per graph replay (default 32). */
 int tlm_batch_set_graph_steps(tlm_batch* b, int32_t k);
-/* Time candidate graph lengths on the current solves (these are real steps:
- * *steps_done of them were run) and keep the fastest. */
+/* Kernels of a batch step: mixed = 1 LDS-staged mixed-line kernel, 0 lane
+ * slots, -1 the default (LDS-staged); streams = 2
+ * (default) runs the class kernels on a second stream concurrently with
+ * the line and PML kernels, 1 runs every kernel in turn.  Results do not
+ * depend on either. */
+int tlm_batch_set_kernels(tlm_batch* b, int32_t mixed, int32_t streams);
+/* Time candidate graph lengths (4, 16, 64 steps) with each mixed-line
+ * kernel on the current solves (these are real steps: *steps_done of them
+ * were run) and keep the fastest. */
 int tlm_batch_autotune(tlm_batch* b, int32_t steps_per_candidate, int32_t* best_k, int32_t* steps_done);
 
 #ifdef __cplusplus

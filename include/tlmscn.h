@@ -289,9 +289,8 @@ int tlm_layout_segments(tlm_handle* h, int32_t* n_segments, int32_t* x0);
  * y-band of every z-plane in storage order, so the y-1 and z-1 neighbour
  * rows of a block were read by the same XCD just before (its own L2).
  * mixed: mixed-line kernel, 1 = LDS-staged cell-order loads and stores
- * around class-sorted node updates (default in the plain form), 0 =
- * class-sorted lanes with four bulk cells per lane (default in the
- * precision form), -1 = the default of the form.
+ * around class-sorted node updates, 0 = class-sorted lanes with four bulk
+ * cells per lane, -1 = the default (LDS-staged).
  * Any schedule gives identical results; it only changes speed
  * (tlm_autotune picks one by timing). */
 int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap,

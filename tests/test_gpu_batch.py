@@ -27,8 +27,9 @@ def results(h):
     return out, first, h.read_dft(), h.download_links()[:, 1:-1, 1:-1]
 
 
+@pytest.mark.parametrize("kernels", [(-1, 2), (1, 1), (0, 2)])
 @pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
-def test_batch_matches_alone(form):
+def test_batch_matches_alone(form, kernels):
     lib = capi.engine_library()
     tables = [(name, build_tables(f()), n) for name, f, n in DECKS]
     alone = {}
@@ -39,6 +40,7 @@ def test_batch_matches_alone(form):
         h.close()
     b = capi.Batch(form, 16 << 30)
     b.set_graph_steps(8)
+    b.set_kernels(*kernels)
     handles = {name: capi.Handle(lib, t, form, output_capacity=n) for name, t, n in tables}
     steps = {name: n for name, _, n in tables}
     done = {}
@@ -48,7 +50,7 @@ def test_batch_matches_alone(form):
     b.run(7)
     for name in list(handles)[3:]:
         assert b.add(handles[name], 1, steps[name])
-    print(f"[batch] form={form} solves={b.info()}", flush=True)
+    print(f"[batch] form={form} kernels={kernels} solves={b.info()}", flush=True)
     for _ in range(20):
         b.run(13)
         for h in b.collect():
