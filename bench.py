@@ -292,6 +292,7 @@ def main():
               key=lambda k: per_class[k]["avg_ms"])
     achieved = per_class[dom]["GB_s"]
     traffic = measured_traffic(a, dom, per_class[dom]["bytes_per_launch"])
+    sched, segs = h.get_schedule(), h.layout_segments()
     h.read_cell_outputs()
     h.close()
 
@@ -324,6 +325,8 @@ def main():
             "whole_step_GB_s": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
             "whole_step_frac_of_hbm": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "kernels": per_class,
+            "schedule": sched,
+            "x_segments": segs,
         },
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic},

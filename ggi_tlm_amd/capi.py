@@ -71,6 +71,7 @@ SIGNATURES = {
     "set_layout": (C.c_int, [_h, _i32]),
     "layout_segments": (C.c_int, [_h, C.POINTER(_i32), _p]),
     "set_streams": (C.c_int, [_h, _i32]),
+    "get_schedule": (C.c_int, [_h, _p, _p, _p, _p, _p, _p]),
 }
 
 LAYOUT_X_SEGMENTS = 0
@@ -287,6 +288,13 @@ class Handle:
         x0 = np.zeros(MAX_SEGMENTS + 1, dtype=np.int32)
         self._check(self.lib.fn["layout_segments"](self.h, C.byref(n), _ptr(x0)))
         return [int(v) for v in x0[: n.value + 1]]
+
+    def get_schedule(self) -> dict:
+        """The schedule in use (after autotune: the one it kept)."""
+        v = np.zeros(6, dtype=np.int32)
+        ptrs = [C.c_void_p(v.ctypes.data + 4 * k) for k in range(6)]
+        self._check(self.lib.fn["get_schedule"](self.h, *ptrs))
+        return dict(zip(("variant", "rows", "zc", "xcd_remap", "mixed", "streams"), (int(x) for x in v)))
 
     def set_streams(self, streams: int):
         """1: a step's kernels in turn on one stream; 2: class kernels concurrent."""

@@ -1354,6 +1354,18 @@ int tlm_layout_segments(tlm_handle* h, int32_t* n_segments, int32_t* x0) {
   return TLM_OK;
 }
 
+int tlm_get_schedule(const tlm_handle* h, int32_t* variant, int32_t* rows, int32_t* zc, int32_t* xcd_remap,
+                     int32_t* mixed, int32_t* streams) {
+  if (!h) return TLM_ERR_ARG;
+  if (variant) *variant = sched_variant(h);
+  if (rows) *rows = h->sched_by;
+  if (zc) *zc = h->sched_zc;
+  if (xcd_remap) *xcd_remap = h->sched_xcd;
+  if (mixed) *mixed = h->mixed_variant >= 0 ? h->mixed_variant : 1;
+  if (streams) *streams = h->streams;
+  return TLM_OK;
+}
+
 int tlm_set_streams(tlm_handle* h, int32_t streams) {
   if (!h) return TLM_ERR_ARG;
   if (streams != 1 && streams != 2) return fail(h, TLM_ERR_ARG, "streams must be 1 or 2");

@@ -223,6 +223,13 @@ module tlm_gpu_iface
       integer(c_int32_t), value :: layout
     end function tlm_set_layout
 
+    integer(c_int) function tlm_get_schedule(h, variant, rows, zc, xcd_remap, mixed, streams) &
+                                               bind(c, name='tlm_get_schedule')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: variant, rows, zc, xcd_remap, mixed, streams
+    end function tlm_get_schedule
+
     integer(c_int) function tlm_set_streams(h, streams) bind(c, name='tlm_set_streams')
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h

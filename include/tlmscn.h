@@ -296,6 +296,11 @@ int tlm_layout_segments(tlm_handle* h, int32_t* n_segments, int32_t* x0);
 int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap,
                      int32_t mixed);
 
+/* The schedule in use (after tlm_autotune: the one it kept); any output
+ * pointer may be NULL. */
+int tlm_get_schedule(const tlm_handle* h, int32_t* variant, int32_t* rows, int32_t* zc, int32_t* xcd_remap,
+                     int32_t* mixed, int32_t* streams);
+
 /* Streams of a step (no counterpart in the reference): 1 = every kernel of
  * a step on the handle's stream in turn; 2 (default) = the special-cell
  * class kernels (mixed lines, material, generic, PML cells holding a

@@ -1359,6 +1359,18 @@ int oracle_layout_segments(oracle_handle* h, int32_t* n_segments, int32_t* x0) {
   return TLM_OK;
 }
 
+int oracle_get_schedule(const oracle_handle* h, int32_t* variant, int32_t* rows, int32_t* zc, int32_t* xcd_remap,
+                        int32_t* mixed, int32_t* streams) {
+  (void)h;
+  if (variant) *variant = 3;
+  if (rows) *rows = 1;
+  if (zc) *zc = 1;
+  if (xcd_remap) *xcd_remap = 0;
+  if (mixed) *mixed = 1;
+  if (streams) *streams = 1;
+  return TLM_OK;
+}
+
 int oracle_set_streams(oracle_handle* h, int32_t streams) {
   if (streams != 1 && streams != 2) return fail(h, TLM_ERR_ARG, "streams must be 1 or 2");
   return TLM_OK;

@@ -64,6 +64,8 @@ int oracle_set_schedule(oracle_handle* h, int32_t variant, int32_t rows, int32_t
 int oracle_set_profiling(oracle_handle* h, int32_t enable);
 int oracle_set_layout(oracle_handle* h, int32_t layout);
 int oracle_set_streams(oracle_handle* h, int32_t streams);
+int oracle_get_schedule(const oracle_handle* h, int32_t* variant, int32_t* rows, int32_t* zc, int32_t* xcd_remap,
+                        int32_t* mixed, int32_t* streams);
 int oracle_layout_segments(oracle_handle* h, int32_t* n_segments, int32_t* x0);
 int oracle_autotune(oracle_handle* h, int32_t first_timestep, int32_t steps_per_candidate, int32_t* steps_done);
 int oracle_set_small_to_zero(oracle_handle* h, int32_t n_small_to_zero, double small_to_zero_value);
