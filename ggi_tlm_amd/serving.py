@@ -7,7 +7,7 @@ step count and an arrival time.  The loop
   * enqueues K steps of every resident solve (asynchronous, one HIP-graph
     replay over all of them),
   * meanwhile admits arrived requests in arrival order while they fit the
-    HBM budget (their tables are built on the device while the GPU runs),
+    HBM budget (their device tables are built while the GPU runs),
   * collects the solves that finished, reads their probe series and DFT and
     frees them, recording latency = results ready - arrival.
 
@@ -55,7 +55,11 @@ def _results(req: Request, h):
 
 def run_batched(requests: List[Request], form=capi.FORM_PRECISION_FP32, budget_bytes: int = 200 << 30,
                 chunk: int = 64, lib=None, log=None) -> dict:
-    """Serve the stream with continuous batching; returns statistics."""
+    """Serve the stream with continuous batching; returns statistics.
+
+    Each round enqueues `chunk` steps of the resident solves (asynchronous),
+    admits the requests that arrived meanwhile (their set-up runs on the host
+    while the GPU steps), then collects the solves that finished."""
     lib = lib or capi.engine_library()
     b = capi.Batch(form, budget_bytes, lib)
     b.set_graph_steps(chunk)
@@ -64,10 +68,9 @@ def run_batched(requests: List[Request], form=capi.FORM_PRECISION_FP32, budget_b
     inflight = {}
     t0 = time.perf_counter()
     peak_resident = 0
-    while pending or waiting or inflight:
-        now = time.perf_counter() - t0
-        while pending and pending[0].arrival <= now:
-            waiting.append(pending.popleft())
+
+    def admit() -> int:
+        n = 0
         while waiting:
             r = waiting[0]
             if r.handle is None:
@@ -79,12 +82,22 @@ def run_batched(requests: List[Request], form=capi.FORM_PRECISION_FP32, budget_b
                 break
             inflight[r.handle.h.value] = r
             waiting.popleft()
-        if not inflight:
-            if pending:
+            n += 1
+        return n
+
+    while pending or waiting or inflight:
+        now = time.perf_counter() - t0
+        while pending and pending[0].arrival <= now:
+            waiting.append(pending.popleft())
+        running = bool(inflight)
+        if running:
+            b.run(chunk)
+            peak_resident = max(peak_resident, b.info()[2])
+        admitted = admit()
+        if not running:
+            if not admitted and pending:
                 time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
             continue
-        b.run(chunk)
-        peak_resident = max(peak_resident, b.info()[2])
         for h in b.collect():
             r = inflight.pop(h.h.value)
             _results(r, h)
