@@ -63,8 +63,10 @@ struct tlm_batch {
   std::vector<Entry> solves;
   engine::BatchSolve* d_solves = nullptr;
   int d_cap = 0;
-  // block tables: lines, mixed, pml, material, generic, pml-material, huygens, dft
-  enum { T_LINES, T_MIXED, T_PML, T_MAT, T_GEN, T_PM, T_HUY, T_DFT, T_N };
+  // block tables: lines, mixed (lane slots, or LDS-staged without PML
+  // cells), LDS-staged with PML cells, pml, material, generic,
+  // pml-material, huygens, dft
+  enum { T_LINES, T_MIXED, T_MIXED_P, T_PML, T_MAT, T_GEN, T_PM, T_HUY, T_DFT, T_N };
   int2* d_tab[T_N] = {};
   int n_tab[T_N] = {};
   size_t cap_tab[T_N] = {};
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(256) void kb_mixed(const BatchSolve* __restrict__ S
   else mixed_lane<T, PLAIN, false>(s, Lin, Lout, a, slot, si);
 }
 
-template <typename T, bool PLAIN>
+template <typename T, bool PLAIN, int CM>
 __global__ __launch_bounds__(scn::kLdsCells) void kb_mixed_lds(const BatchSolve* __restrict__ S,
                                                               const int2* __restrict__ tab) {
   __shared__ double sv[12 * scn::kLdsCells];
@@ -188,11 +190,11 @@ __global__ __launch_bounds__(scn::kLdsCells) void kb_mixed_lds(const BatchSolve*
   const T* Lin = (const T*)s.planes[s.cur];
   T* Lout = (T*)s.planes[s.cur ^ 1];
   if (pull)
-    scn::mixed_lds_body<T, PLAIN, true>(s.d, Lin, Lout, a, e.y, s.lds_lines, s.lds_fd, s.lds_slot_p, s.lds_slot_i,
-                                        s.ts, slot, sv);
+    scn::mixed_lds_body<T, PLAIN, true, CM>(s.d, Lin, Lout, a, e.y, s.lds_lines, s.lds_fd, s.lds_slot_p,
+                                            s.lds_slot_i, s.ts, slot, sv);
   else
-    scn::mixed_lds_body<T, PLAIN, false>(s.d, Lin, Lout, a, e.y, s.lds_lines, s.lds_fd, s.lds_slot_p, s.lds_slot_i,
-                                         s.ts, slot, sv);
+    scn::mixed_lds_body<T, PLAIN, false, CM>(s.d, Lin, Lout, a, e.y, s.lds_lines, s.lds_fd, s.lds_slot_p,
+                                             s.lds_slot_i, s.ts, slot, sv);
 }
 
 // One class list of a solve: 0 PML, 1 material, 2 generic, 3 PML holding a material.
@@ -282,8 +284,12 @@ void launch_batch_step(tlm_batch* b) {
     (void)hipStreamWaitEvent(b->side, b->ev_fork, 0);
   }
   go(st, tlm_batch::T_LINES, kb_lines<T, PLAIN>);
-  if (batch_lds(b)) go(cs, tlm_batch::T_MIXED, kb_mixed_lds<T, PLAIN>, scn::kLdsCells);
-  else go(cs, tlm_batch::T_MIXED, kb_mixed<T, PLAIN>);
+  if (batch_lds(b)) {
+    go(cs, tlm_batch::T_MIXED, kb_mixed_lds<T, PLAIN, scn::kMixedMasks[0]>, scn::kLdsCells);
+    go(cs, tlm_batch::T_MIXED_P, kb_mixed_lds<T, PLAIN, scn::kMixedMasks[1]>, scn::kLdsCells);
+  } else {
+    go(cs, tlm_batch::T_MIXED, kb_mixed<T, PLAIN>);
+  }
   go(cs, tlm_batch::T_MAT, kb_list<T, PLAIN, 1>);
   go(cs, tlm_batch::T_GEN, kb_list<T, PLAIN, 2>);
   go(cs, tlm_batch::T_PM, kb_list<T, PLAIN, 3>);
@@ -349,8 +355,14 @@ int rebuild(tlm_batch* b) {
     s.dt = h->g.dt;
     if (e.ts > e.last) continue;  // finished, awaiting collection
     for (int32_t k : h->bulk_blocks) tab[tlm_batch::T_LINES].push_back(make_int2(i, k));
-    const int n_mixed_wg = batch_lds(b) ? h->n_lds_wg : h->n_mixed_blocks;
-    for (int k = 0; k < n_mixed_wg; k++) tab[tlm_batch::T_MIXED].push_back(make_int2(i, k));
+    if (batch_lds(b)) {
+      static const int lt[2] = {tlm_batch::T_MIXED, tlm_batch::T_MIXED_P};
+      int wg = 0;
+      for (int v = 0; v < 2; v++)
+        for (int k = 0; k < h->n_lds_var[v]; k++) tab[lt[v]].push_back(make_int2(i, wg++));
+    } else {
+      for (int k = 0; k < h->n_mixed_blocks; k++) tab[tlm_batch::T_MIXED].push_back(make_int2(i, k));
+    }
     auto add = [&](int t, long long cnt) {
       for (long long k = 0; k < (cnt + 255) / 256; k++) tab[t].push_back(make_int2(i, (int)k));
     };

@@ -85,6 +85,7 @@ struct tlm_handle {
   const int32_t* d_lds_slot_p = nullptr;
   const int32_t* d_lds_slot_i = nullptr;
   int n_lds_wg = 0;
+  int n_lds_var[2] = {0, 0};              // LDS workgroups per class-mask variant, in table order
   const int32_t* d_slot_cell = nullptr;  // mixed-line workgroup slots
   const int32_t* d_slot_idx = nullptr;
   const int32_t* d_gen_cell = nullptr;

@@ -448,8 +448,9 @@ __device__ inline void flush_small(double v[12], double small) {
 }
 
 // Incident voltages of cell c = (x,y,z) of segment sg in v[12].  fdi: face
-// descriptor (-1 = six ordinary faces).
-template <typename T, bool PLAIN>
+// descriptor (-1 = six ordinary faces); FD = false compiles the special-face
+// path out for callers whose cells have none.
+template <typename T, bool PLAIN, bool FD = true>
 __device__ inline void pull_cell(const Dev& d, const T* __restrict__ L, long long c, int x, int y, int z, int sg,
                                  int fdi, const PullArgs& a, double v[12]) {
   const long long P = d.P;
@@ -458,7 +459,7 @@ __device__ inline void pull_cell(const Dev& d, const T* __restrict__ L, long lon
 #pragma unroll
   for (int p = 0; p < 12; p++) o[p] = ld(L, P, p, c);
   int fc[6] = {0, 0, 0, 0, 0, 0};
-  if (fdi >= 0) {
+  if (FD && fdi >= 0) {
 #pragma unroll
     for (int k = 0; k < 6; k++) fc[k] = d.fd[6 * fdi + k];
   }
@@ -482,7 +483,7 @@ __device__ inline void pull_cell(const Dev& d, const T* __restrict__ L, long lon
                          : s == 2 ? c - sy : s == 3 ? c + sy : s == 4 ? c - sz : c + sz;
     double na = ld(L, P, qa[s], cn);
     double nb = ld(L, P, qb[s], cn);
-    if (fc[s] == 0) {
+    if (!FD || fc[s] == 0) {
       v[pa[s]] = free_face<PLAIN>(oa, na);
       v[pb[s]] = free_face<PLAIN>(ob, nb);
     } else {
@@ -1656,8 +1657,11 @@ constexpr int kLdsCells = TLM_LDS_CELLS;
 //            line is written whole and coalesced.
 // cell_fd: face descriptor of each cell position (-1 none; -2 no cell here:
 // padding, or a cell of another kernel).  slot_p: class << 16 | position,
-// or -1; slot_i: the class-list position of special cells.
-template <typename T, bool PLAIN, bool PULL>
+// or -1; slot_i: the class-list position of special cells.  CM: the special
+// classes the workgroups of this launch hold (bit 0 PML, 1 material, 2
+// generic); the paths of absent classes are compiled out, which lowers the
+// register count (the PML path sets it).
+template <typename T, bool PLAIN, bool PULL, int CM>
 __device__ inline void mixed_lds_body(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                       const PullArgs& a, long long wg, const int32_t* __restrict__ wg_lines,
                                       const int32_t* __restrict__ cell_fd, const int32_t* __restrict__ slot_p,
@@ -1675,7 +1679,8 @@ __device__ inline void mixed_lds_body(const Dev& d, const T* __restrict__ Lin, T
     if (PULL) {
       int x, y, z, sg;
       cell_coords(d, c, x, y, z, sg);
-      pull_cell<T, PLAIN>(d, Lin, c, x, y, z, sg, fd, a, v);
+      // only PML and generic cells carry face descriptors
+      pull_cell<T, PLAIN, (CM & 5) != 0>(d, Lin, c, x, y, z, sg, fd, a, v);
     } else {
 #pragma unroll
       for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
@@ -1696,9 +1701,9 @@ __device__ inline void mixed_lds_body(const Dev& d, const T* __restrict__ Lin, T
       if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
     } else {
       const int i = slot_i[base + t];
-      if (cls == 1) pml_node<PLAIN>(d, v, slot, i, d.pml_cell[i]);
-      else if (cls == 2) material_node(d, v, i);
-      else generic_node<PLAIN>(d, v, timestep, slot, i);
+      if ((CM & 1) && cls == 1) pml_node<PLAIN>(d, v, slot, i, d.pml_cell[i]);
+      else if ((CM & 2) && cls == 2) material_node(d, v, i);
+      else if ((CM & 4) && cls == 3) generic_node<PLAIN>(d, v, timestep, slot, i);
     }
 #pragma unroll
     for (int k = 0; k < 12; k++) sv[k * C + p] = v[k];
@@ -1710,15 +1715,23 @@ __device__ inline void mixed_lds_body(const Dev& d, const T* __restrict__ Lin, T
   }
 }
 
-template <typename T, bool PLAIN, bool PULL>
+// Workgroup wg0 + blockIdx.x of the LDS-staged tables.
+template <typename T, bool PLAIN, bool PULL, int CM>
 __global__ __launch_bounds__(kLdsCells) void k_mixed_lds(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                                   PullArgs a, const int32_t* __restrict__ wg_lines,
                                                   const int32_t* __restrict__ cell_fd,
                                                   const int32_t* __restrict__ slot_p,
-                                                  const int32_t* __restrict__ slot_i, int timestep, int slot) {
+                                                  const int32_t* __restrict__ slot_i, int timestep, int slot,
+                                                  int wg0) {
   __shared__ double sv[12 * kLdsCells];
-  mixed_lds_body<T, PLAIN, PULL>(d, Lin, Lout, a, blockIdx.x, wg_lines, cell_fd, slot_p, slot_i, timestep, slot, sv);
+  mixed_lds_body<T, PLAIN, PULL, CM>(d, Lin, Lout, a, wg0 + (long long)blockIdx.x, wg_lines, cell_fd, slot_p, slot_i,
+                                     timestep, slot, sv);
 }
+
+// Class masks of the LDS-staged mixed-line launches: workgroups without PML
+// cells (the PML path sets the register count: 106 against 168 VGPRs in the
+// precision form), and the rest.
+constexpr int kMixedMasks[2] = {6, 7};
 
 // Connect only (no scatter): incident voltages into Lout, used to hand the
 // state back after a connect.  Bulk variant over all cells; the class

@@ -405,6 +405,36 @@ int finalize(tlm_handle* h) {
         lds_slot_i.push_back(s2 < (int)sl.size() ? sl[s2].second.second : 0);
       }
     }
+    // Order the workgroups by the kernel variant their classes need
+    // (scn::kMixedMasks: without PML cells; with PML cells).
+    {
+      const size_t ng = lds_lines.size() / GL;
+      std::vector<int> var(ng);
+      for (size_t g = 0; g < ng; g++) {
+        int m = 0;
+        for (int s2 = 0; s2 < LC; s2++) {
+          const int32_t sp = lds_slot_p[g * LC + s2];
+          if (sp >= 0 && (sp >> 16) > 0) m |= 1 << ((sp >> 16) - 1);
+        }
+        var[g] = (m & 1) ? 1 : 0;
+      }
+      std::vector<int32_t> l2, f2, p2, i2;
+      for (int v = 0; v < 2; v++) {
+        h->n_lds_var[v] = 0;
+        for (size_t g = 0; g < ng; g++) {
+          if (var[g] != v) continue;
+          h->n_lds_var[v]++;
+          l2.insert(l2.end(), lds_lines.begin() + g * GL, lds_lines.begin() + (g + 1) * GL);
+          f2.insert(f2.end(), lds_fd.begin() + g * LC, lds_fd.begin() + (g + 1) * LC);
+          p2.insert(p2.end(), lds_slot_p.begin() + g * LC, lds_slot_p.begin() + (g + 1) * LC);
+          i2.insert(i2.end(), lds_slot_i.begin() + g * LC, lds_slot_i.begin() + (g + 1) * LC);
+        }
+      }
+      lds_lines.swap(l2);
+      lds_fd.swap(f2);
+      lds_slot_p.swap(p2);
+      lds_slot_i.swap(i2);
+    }
   }
   // flat per-class arrays in list order
   std::vector<int32_t> pml_cell, pml_fd;
@@ -847,12 +877,23 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
   const int mixed = h->mixed_variant >= 0 ? h->mixed_variant : 1;  // LDS-staged: measured faster in both forms
   if (mixed == 1 && h->n_lds_wg) {
     ProfScope ps(h, TLM_KCLASS_MIXED);
-    if (pull)
-      hipLaunchKernelGGL((scn::k_mixed_lds<T, PLAIN, true>), dim3(h->n_lds_wg), dim3(scn::kLdsCells), 0, cs, d, Lin, Lout,
-                         a, h->d_lds_lines, h->d_lds_fd, h->d_lds_slot_p, h->d_lds_slot_i, timestep, slot);
-    else
-      hipLaunchKernelGGL((scn::k_mixed_lds<T, PLAIN, false>), dim3(h->n_lds_wg), dim3(scn::kLdsCells), 0, cs, d, Lin, Lout,
-                         a, h->d_lds_lines, h->d_lds_fd, h->d_lds_slot_p, h->d_lds_slot_i, timestep, slot);
+    int wg0 = 0;
+    for (int v = 0; v < 2; v++) {
+      const int n = h->n_lds_var[v];
+      if (!n) continue;
+      auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(n), dim3(scn::kLdsCells), 0, cs, d, Lin, Lout, a, h->d_lds_lines, h->d_lds_fd,
+                           h->d_lds_slot_p, h->d_lds_slot_i, timestep, slot, wg0);
+      };
+      if (pull) {
+        if (v == 0) go(scn::k_mixed_lds<T, PLAIN, true, scn::kMixedMasks[0]>);
+        else go(scn::k_mixed_lds<T, PLAIN, true, scn::kMixedMasks[1]>);
+      } else {
+        if (v == 0) go(scn::k_mixed_lds<T, PLAIN, false, scn::kMixedMasks[0]>);
+        else go(scn::k_mixed_lds<T, PLAIN, false, scn::kMixedMasks[1]>);
+      }
+      wg0 += n;
+    }
   } else if (h->n_mixed_blocks) {
     ProfScope ps(h, TLM_KCLASS_MIXED);
     dim3 g(h->n_mixed_blocks);
