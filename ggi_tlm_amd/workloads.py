@@ -1,5 +1,6 @@
 """Synthetic problem decks of the measured workloads (seeded).
 
+free_space_cube   64^3 PEC-bounded free space, soft Ez point source, 3 probes
 single_large      256^3 + 8-cell PML + eps_r=4 sphere + Huygens plane wave
 dispersive_load   384x384x192 + 10-cell PML + 40 two-pole Debye/Lorentz blocks
 enclosure         PEC-walled box with one aperture, 6-cell PML, plane wave,
@@ -20,6 +21,26 @@ from .constants import C0
 from .filters import Sfilter, debye_lorentz_eps
 from .problem import (PML, Box, ExcitationFunction, HuygensBox, OutputPoint, PointSource, Problem, Sphere,
                       VolumeMaterial)
+
+
+def free_space_cube(n=64, steps=2000, seed=64):
+    """64^3 cells, dl = 0.01 m, PEC outer boundary (R = -1), soft Ez point
+    source at cell (32, 32, 32) (1-based) driven by a Gaussian (1 V/m, width
+    20 dt, delay 80 dt), Ex, Ey and Ez recorded every step at 3 seeded probe
+    cells (9 output points), 2000 steps: the plain FP64 correctness case."""
+    rng = np.random.default_rng(seed)
+    dl = 0.01
+    dt = dl / (2 * C0)
+    src = (n // 2, n // 2, n // 2)
+    cells = []
+    while len(cells) < 3:
+        p = tuple(int(v) for v in rng.integers(1, n + 1, size=3))
+        if p != src and p not in cells:
+            cells.append(p)
+    return Problem(nx=n, ny=n, nz=n, dl=dl, n_timesteps=steps, reflection=(-1.0,) * 6,
+                   functions=[ExcitationFunction("gaussian", (1.0, 20 * dt, 80 * dt))],
+                   sources=[PointSource(src, "ez", 1)],
+                   outputs=[OutputPoint(c, k) for c in cells for k in ("ex", "ey", "ez")])
 
 
 def single_large(n=256, npml=8, radius=60, steps=4000, seed=2024):

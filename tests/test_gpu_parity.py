@@ -267,3 +267,20 @@ def test_layout_cuts_pml_slabs():
     with pytest.raises(capi.TlmError):
         g.set_schedule(0, 1, 1, 0)          # tile schedules need the row-major layout
     g.close()
+
+
+def test_free_space_cube_64_full_run(oracle_lib):
+    """The 64^3 plain FP64 correctness case over its full 2000 steps: the
+    9 probe series and the final links equal the oracle's bit for bit."""
+    from oracle import oracle
+    from ggi_tlm_amd import workloads
+
+    t = build_tables(workloads.free_space_cube())
+    steps = t.n_timesteps
+    o = oracle.handle(t, output_capacity=steps)
+    g = capi.Handle(capi.engine_library(), t, FORM_PLAIN_FP64, output_capacity=steps)
+    ro, rg = run(o, t, steps), run(g, t, steps)
+    assert ro.shape == rg.shape == (steps, t.n_output_cells, 6)
+    assert np.array_equal(ro, rg)
+    assert np.array_equal(o.download_links()[:, 1:-1, 1:-1], g.download_links()[:, 1:-1, 1:-1])
+    assert np.abs(probe_series(rg, t)).max() > 0
