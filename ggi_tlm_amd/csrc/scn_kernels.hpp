@@ -15,12 +15,18 @@
 // new scattered voltages to the other buffer.  Each link therefore crosses
 // HBM once per step in each direction; the neighbour reads hit the caches.
 //
-// Cell classes run as separate, wavefront-uniform kernels:
-//   bulk    free-space cells with six ordinary faces (bitmask-selected)
-//   pml     PML cells (scatter_PML.F90), FP64 state in SoA arrays
-//   generic every other special cell: materials, point sources, probes,
-//           and cells touching a special face (PEC/PMC/free-space surface,
-//           face sources), following the Thevenin path of scatter.F90.
+// Every 128-byte line of a link plane is written whole by one kernel.  Lines
+// holding one class only run in that class's wavefront-uniform kernel:
+//   bulk     free-space cells with six ordinary faces (line kernel)
+//   pml      PML cells (scatter_PML.F90), FP64 state in SoA arrays
+//   material dispersive/PEC/PMC cells without sources, probes or special faces
+//   generic  point sources, probes and cells touching a special face
+//            (PEC/PMC/free-space surface, face sources), the Thevenin path of
+//            scatter.F90
+// Lines holding several classes run in the mixed-line kernels, which stage
+// a workgroup's lines through LDS (or give each lane one special cell or
+// several bulk cells); PML cells holding a material have a kernel of their
+// own.
 //
 // Template parameters: T is the link storage type (double: plain form,
 // float: precision form); PULL selects between reading incident voltages
@@ -137,7 +143,7 @@ struct Dev {
   const tlm_pml_parameter* pml_params;
   double* pml_state;             // PS_N (plain) or PC_N (precision) words * n_pml_state
   double* pml_It;                // 3 * n_pml_state (Ixt,Iyt,Izt)
-  // PML cells holding a material (scan order)
+  // PML cells holding a material (storage order)
   int n_pm;
   const int32_t* pm_cell;        // cell index
   const int32_t* pm_par;         // PML parameter row (0-based)

@@ -2,9 +2,9 @@
 // (declarations and reference mapping in include/tlmscn.h).
 //
 // Host side: validates the reference tables, derives the device layout
-// (link planes, bulk bitmask, PML and generic class lists, special-face
-// descriptors, excitation CSR tables), owns one HIP stream per handle and
-// drives the step kernels of scn_kernels.hpp.
+// (x-segment storage order, 128-byte line classes, class lists, mixed-line
+// workgroup tables, special-face descriptors, excitation CSR tables), owns
+// the handle's HIP streams and drives the step kernels of scn_kernels.hpp.
 //
 // Link-state machine.  The reference keeps one array V that scatter and
 // connect update in place.  Here a buffer holds either incident voltages
@@ -288,8 +288,8 @@ int finalize(tlm_handle* h) {
   while (n_pure_gen < (int)gr.size() && pure_of(gr[n_pure_gen].c)) n_pure_gen++;
   // Mixed lines: workgroups of 256 lanes, each owning whole lines.  A
   // special cell takes one lane, bulk cells kMixedBulkPerLane per lane; a
-  // group takes consecutive mixed lines (scan order) while its lanes fit.
-  // Lanes: special cells sorted by class (PML, generic, material; scan order
+  // group takes consecutive mixed lines (storage order) while its lanes fit.
+  // Lanes: special cells sorted by class (PML, generic, material; storage order
   // inside a class), then bulk lanes, bulk lane j holding bulk cells
   // j, j + nb, j + 2 nb, ... of the group (nb = bulk lanes), so that
   // neighbouring lanes touch neighbouring cells.
