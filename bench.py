@@ -163,6 +163,52 @@ def main_batch(a):
     print(json.dumps(line), flush=True)
 
 
+def main_accuracy(a):
+    """Precision form against FP64 on the full-size workloads: relative L2 of
+    every probe series and of every probe's DFT (transfer function).  The
+    FP64 side is the plain form of the engine, which equals the CPU oracle
+    bit for bit on every parity deck (the oracle would take hours here)."""
+    out = []
+    for name in a.acc_workloads:
+        if name == "large":
+            p = workload(n=a.n, steps=a.acc_steps_large)
+        else:
+            p = workloads.dispersive_load(steps=a.acc_steps_dispersive)
+        t = build_tables(p)
+        steps = t.n_timesteps
+        res = {}
+        for form in (capi.FORM_PLAIN_FP64, capi.FORM_PRECISION_FP32):
+            h = capi.Handle(capi.engine_library(), t, form, output_capacity=steps)
+            t0 = time.perf_counter()
+            h.step(1, steps)
+            h.synchronize()
+            el = time.perf_counter() - t0
+            series = capi.probe_series(h.read_cell_outputs()[0], t)
+            dft = h.read_dft() if len(t.dft_frequencies) else None
+            h.close()
+            res[form] = (series, dft)
+            print(f"[bench] accuracy {name}: form {form}, {steps} steps in {el:.1f} s", file=sys.stderr, flush=True)
+        (s64, d64), (s32, d32) = res[capi.FORM_PLAIN_FP64], res[capi.FORM_PRECISION_FP32]
+        norms = np.array([np.linalg.norm(s64[:, k]) for k in range(s64.shape[1])])
+        rel = [float(np.linalg.norm(s32[:, k] - s64[:, k]) / norms[k]) for k in range(s64.shape[1])]
+        rec = {"workload": name, "steps": steps, "probe_rel_l2": [float(f"{e:.3e}") for e in rel],
+               "probe_norm_rel_to_peak": [round(float(v), 4) for v in norms / norms.max()],
+               "max_rel_l2_probes_above_5pct_of_peak": float(f"{max(e for e, n in zip(rel, norms) if n >= 0.05 * norms.max()):.3e}")}
+        if d64 is not None:
+            dn = np.array([np.linalg.norm(d64[k]) for k in range(d64.shape[0])])
+            drel = [float(np.linalg.norm(d32[k] - d64[k]) / dn[k]) for k in range(d64.shape[0])]
+            rec["dft_frequencies"] = len(t.dft_frequencies)
+            rec["dft_rel_l2"] = [float(f"{e:.3e}") for e in drel]
+            rec["dft_norm_rel_to_peak"] = [round(float(v), 4) for v in dn / dn.max()]
+        out.append(rec)
+        print(f"[bench] accuracy {json.dumps(rec)}", file=sys.stderr, flush=True)
+    line = {"metric": "relative L2, precision form vs FP64", "value": max(r["max_rel_l2_probes_above_5pct_of_peak"]
+                                                                          for r in out),
+            "unit": "relative L2", "n_gpus": 1, "higher_is_better": False, "dtype": "f32 links / f64",
+            "data": "synthetic", "config": {"workloads": out}}
+    print(json.dumps(line), flush=True)
+
+
 def main_serve(a):
     """Poisson stream of mixed requests served with continuous batching
     (precision form) against the same stream served one request at a time
@@ -218,7 +264,10 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=("single", "batch", "serve"), default="single")
+    ap.add_argument("--mode", choices=("single", "batch", "serve", "accuracy"), default="single")
+    ap.add_argument("--acc-workloads", nargs="+", default=["large", "dispersive"])
+    ap.add_argument("--acc-steps-large", type=int, default=4000)
+    ap.add_argument("--acc-steps-dispersive", type=int, default=6000)
     ap.add_argument("--workload", choices=("large", "dispersive"), default="large")
     ap.add_argument("--tune-steps", type=int, default=4)
     ap.add_argument("--batch-solves", type=int, default=512)
@@ -238,6 +287,8 @@ def main():
         return main_batch(a)
     if a.mode == "serve":
         return main_serve(a)
+    if a.mode == "accuracy":
+        return main_accuracy(a)
     tune_steps = 14 * (a.tune_steps + 1) if a.tune_steps else 0  # at most 14 candidates
     total = tune_steps + a.warmup + a.steps + a.profile_steps
     t_setup = time.perf_counter()

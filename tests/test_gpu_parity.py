@@ -127,6 +127,15 @@ def test_precision_form(name, oracle_lib):
         rel = np.linalg.norm(sg[:, k] - so[:, k]) / den
         print(f"probe {k}: relative L2 {rel:.3e}")
         assert rel < tol, (name, k, rel)
+    # probe DFT (transfer functions): relative L2 over the frequencies, FP64
+    # accumulators in both forms
+    if len(t.dft_frequencies) and len(t.output_rows):
+        do, dg = o.read_dft(), g.read_dft()
+        dfloor = 1e-3 * max(np.linalg.norm(do[p]) for p in range(do.shape[0]))
+        for p in range(do.shape[0]):
+            rel = np.linalg.norm(dg[p] - do[p]) / max(np.linalg.norm(do[p]), dfloor)
+            print(f"DFT of probe {p}: relative L2 {rel:.3e}")
+            assert rel < tol, (name, "dft", p, rel)
 
 
 @pytest.mark.parametrize("sched", [(0, 1, 1, 0), (0, 2, 1, 1), (1, 1, 4, 1), (1, 4, 8, 0), (1, 2, 32, 1),
