@@ -591,10 +591,14 @@ __device__ inline void scatter_free_f32(float v[12]) {
   v[Vz_ymin] = Vz + Ix - M;  v[Vz_ymax] = Vz - Ix - m;
 }
 
+// The scattered values are read again only in the next step, after the
+// whole grid has streamed through the caches: non-temporal stores keep them
+// from displacing the lines this step's neighbours still read (measured 3%
+// faster steps in both forms).
 template <typename T>
 __device__ inline void store_cell(T* __restrict__ L, long long P, long long c, const double v[12]) {
 #pragma unroll
-  for (int p = 0; p < 12; p++) L[(long long)p * P + c] = (T)v[p];
+  for (int p = 0; p < 12; p++) __builtin_nontemporal_store((T)v[p], &L[(long long)p * P + c]);
 }
 
 // evaluate_Zfilter / timeshift_Zfilter (FILTER/filter_subroutines.F90:510-575)
@@ -1717,7 +1721,7 @@ __device__ inline void mixed_lds_body(const Dev& d, const T* __restrict__ Lin, T
   __syncthreads();
   if (mine) {
 #pragma unroll
-    for (int k = 0; k < 12; k++) Lout[(long long)k * d.P + c] = (T)sv[k * C + t];
+    for (int k = 0; k < 12; k++) __builtin_nontemporal_store((T)sv[k * C + t], &Lout[(long long)k * d.P + c]);
   }
 }
 
