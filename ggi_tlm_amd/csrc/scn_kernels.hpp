@@ -948,6 +948,11 @@ __device__ inline void pml_update(const Dev& d, const T* __restrict__ Lin, T* __
   store_cell(Lout, d.P, c, v);
 }
 
+// A state word is read once per step and not again until the next one:
+// non-temporal loads keep the state streams out of the caches (measured 3%
+// off the PML kernel).
+__device__ inline double state_ld(const double* p) { return __builtin_nontemporal_load(p); }
+
 // scatter_PML.F90:22-313 stages 1-13 on the incident voltages v of PML list
 // entry i (cw: its packed cell word), in place.
 template <bool PLAIN>
@@ -970,11 +975,11 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
   constexpr int wVyxt = PLAIN ? PS_Vyxt : PC_Vyxt, wVzxt = PLAIN ? PS_Vzxt : PC_Vzxt;
   constexpr int wVxyt = PLAIN ? PS_Vxyt : PC_Vxyt, wVzyt = PLAIN ? PS_Vzyt : PC_Vzyt;
   constexpr int wVyzt = PLAIN ? PS_Vyzt : PC_Vyzt, wVxzt = PLAIN ? PS_Vxzt : PC_Vxzt;
-  const double last_Ix = S[wIx * n + si], last_Iy = S[wIy * n + si], last_Iz = S[wIz * n + si];
-  const double last_Vxt = S[wVxt * n + si], last_Vyt = S[wVyt * n + si], last_Vzt = S[wVzt * n + si];
-  const double last_Vyxt = S[wVyxt * n + si], last_Vzxt = S[wVzxt * n + si];
-  const double last_Vxyt = S[wVxyt * n + si], last_Vzyt = S[wVzyt * n + si];
-  const double last_Vyzt = S[wVyzt * n + si], last_Vxzt = S[wVxzt * n + si];
+  const double last_Ix = state_ld(&S[wIx * n + si]), last_Iy = state_ld(&S[wIy * n + si]), last_Iz = state_ld(&S[wIz * n + si]);
+  const double last_Vxt = state_ld(&S[wVxt * n + si]), last_Vyt = state_ld(&S[wVyt * n + si]), last_Vzt = state_ld(&S[wVzt * n + si]);
+  const double last_Vyxt = state_ld(&S[wVyxt * n + si]), last_Vzxt = state_ld(&S[wVzxt * n + si]);
+  const double last_Vxyt = state_ld(&S[wVxyt * n + si]), last_Vzyt = state_ld(&S[wVzyt * n + si]);
+  const double last_Vyzt = state_ld(&S[wVyzt * n + si]), last_Vxzt = state_ld(&S[wVxzt * n + si]);
   // STAGE 3
   v[Vynx] = v[Vynx] * exp_y; v[Vypx] = v[Vypx] * exp_y;
   v[Vznx] = v[Vznx] * exp_z; v[Vzpx] = v[Vzpx] * exp_z;
@@ -994,7 +999,7 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
   if (PLAIN) {
     double lv[12];
 #pragma unroll
-    for (int k = 0; k < 12; k++) lv[k] = S[(PS_Vi + k) * n + si];
+    for (int k = 0; k < 12; k++) lv[k] = state_ld(&S[(PS_Vi + k) * n + si]);
     Vx_sh = ax * (-(lv[Vznx] + lv[Vzpx] + lv[Vynx] + lv[Vypx]) / 2.0
                   + Csy * ((v[Vznx] + v[Vzpx]) + (lv[Vznx] + lv[Vzpx]))
                   + Csz * ((v[Vynx] + v[Vypx]) + (lv[Vynx] + lv[Vypx]))
@@ -1008,9 +1013,9 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
                   + Csy * ((v[Vxnz] + v[Vxpz]) + (lv[Vxnz] + lv[Vxpz]))
                   + Csx_sy * last_Vzt);
   } else {
-    const double Szx = S[PC_Szx * n + si], Syx = S[PC_Syx * n + si];
-    const double Sxy = S[PC_Sxy * n + si], Szy = S[PC_Szy * n + si];
-    const double Syz = S[PC_Syz * n + si], Sxz = S[PC_Sxz * n + si];
+    const double Szx = state_ld(&S[PC_Szx * n + si]), Syx = state_ld(&S[PC_Syx * n + si]);
+    const double Sxy = state_ld(&S[PC_Sxy * n + si]), Szy = state_ld(&S[PC_Szy * n + si]);
+    const double Syz = state_ld(&S[PC_Syz * n + si]), Sxz = state_ld(&S[PC_Sxz * n + si]);
     Vx_sh = ax * (-(Szx + Syx) * 0.5 + Csy * ((v[Vznx] + v[Vzpx]) + Szx) + Csz * ((v[Vynx] + v[Vypx]) + Syx)
                   + Csy_sz * last_Vxt);
     Vy_sh = ay * (-(Sxy + Szy) * 0.5 + Csz * ((v[Vxny] + v[Vxpy]) + Sxy) + Csx * ((v[Vzny] + v[Vzpy]) + Szy)
