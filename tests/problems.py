@@ -179,6 +179,36 @@ def special_mix(n=14, steps=50):
                             OutputPoint((7, 3, 7), "ez")])
 
 
+def layered_slabs(n=48, steps=60, npml=4):
+    """Many x-planes whose cells change class: dielectric and PEC-with-holes
+    slabs every other plane (more candidate x-cuts than the x-segment cap),
+    a PML, a surface PEC patch on a slab face, a point source and probes."""
+    dl = 0.01
+    dt = dl / (2 * 2.99792458e8)
+    ny, nz = n - 6, n - 10
+    regions = []
+    for k, x in enumerate(range(npml + 3, n - npml - 3, 2)):
+        slab = Box(x, x, npml + 1, ny - npml, npml + 1, nz - npml)
+        if k % 2 == 0:
+            regions.append((slab, 1))
+        else:
+            regions.append((slab, 2))
+            regions.append((Box(x, x, ny // 2 - 2, ny // 2 + 2, nz // 2 - 2, nz // 2 + 2), 0))
+    faces = np.array([[npml + 2, j, k, FACE_XMIN] for j in range(10, 14) for k in range(10, 14)])
+    return Problem(nx=n, ny=ny, nz=nz, dl=dl, n_timesteps=steps, reflection=(-1.0,) * 6,
+                   pml=PML((npml,) * 6, (1e-4,) * 6, 2),
+                   materials=[VolumeMaterial("dispersive", Sfilter.constant(3.0), Sfilter.constant(1.0),
+                                             sigma_e=0.01),
+                              VolumeMaterial("pec")],
+                   material_regions=regions,
+                   surface_materials=["pec"], surface_faces=[(faces, 1)],
+                   functions=[ExcitationFunction("gaussian", (1.0, 5 * dt, 20 * dt))],
+                   sources=[PointSource((n // 2 + 2, ny // 2 - 3, nz // 2 - 5), "ez", 1)],
+                   outputs=[OutputPoint((n // 2 + 4, ny // 2 - 4, nz // 2 - 6), "ez"),
+                            OutputPoint((npml + 2, 12, 12), "ey"),
+                            OutputPoint((n - npml - 2, 20, 15), "hx")])
+
+
 def pml_material(n=16, steps=300, reflection=1e-3, mu_r=2.0, sigma_m=30.0, pml=True):
     """A lossy dielectric/magnetic block (eps_r=3, sigma=0.05 S/m, mu_r,
     sigma_m) running into a PML of uneven thickness: the PML cells of the

@@ -28,6 +28,7 @@ CASES = {
     "dispersive_blocks": problems.dispersive_blocks,
     "special_mix": problems.special_mix,
     "huygens_surface": problems.huygens_surface,
+    "layered_slabs": problems.layered_slabs,
 }
 
 
@@ -222,7 +223,7 @@ def test_autotune_keeps_results(name, oracle_lib):
 
 
 LAYOUT_DECKS = ["pml_sphere", "pml_sphere_64", "pml_material", "dispersive_blocks", "special_mix",
-                "huygens_surface", "tem_pml", "free_cube_odd"]
+                "huygens_surface", "tem_pml", "free_cube_odd", "layered_slabs"]
 
 
 @pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
@@ -293,3 +294,20 @@ def test_free_space_cube_64_full_run(oracle_lib):
     assert np.array_equal(ro, rg)
     assert np.array_equal(o.download_links()[:, 1:-1, 1:-1], g.download_links()[:, 1:-1, 1:-1])
     assert np.abs(probe_series(rg, t)).max() > 0
+
+
+def test_layout_segment_cap():
+    """More x-planes change class than there are segments: the layout keeps
+    the TLM_MAX_SEGMENTS - 1 strongest cuts and stays bit-identical to the
+    oracle (plain form)."""
+    from oracle import oracle
+
+    t = build_tables(problems.layered_slabs())
+    g = engine(t, FORM_PLAIN_FP64)
+    segs = g.layout_segments()
+    print(f"x-segments {segs}", flush=True)
+    assert len(segs) == capi.MAX_SEGMENTS + 1
+    o = oracle.handle(t, output_capacity=t.n_timesteps)
+    steps = t.n_timesteps
+    assert np.array_equal(run(o, t, steps), run(g, t, steps))
+    assert np.array_equal(o.download_links()[:, 1:-1, 1:-1], g.download_links()[:, 1:-1, 1:-1])
