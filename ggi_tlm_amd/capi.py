@@ -72,6 +72,9 @@ SIGNATURES = {
     "layout_segments": (C.c_int, [_h, C.POINTER(_i32), _p]),
     "set_streams": (C.c_int, [_h, _i32]),
     "get_schedule": (C.c_int, [_h, _p, _p, _p, _p, _p, _p]),
+    "host_cells": (C.c_int, [_h, C.POINTER(_i32), _p]),
+    "gather_host_cells": (C.c_int, [_h, _p]),
+    "scatter_host_cells": (C.c_int, [_h, _p]),
 }
 
 LAYOUT_X_SEGMENTS = 0
@@ -295,6 +298,26 @@ class Handle:
         ptrs = [C.c_void_p(v.ctypes.data + 4 * k) for k in range(6)]
         self._check(self.lib.fn["get_schedule"](self.h, *ptrs))
         return dict(zip(("variant", "rows", "zc", "xcd_remap", "mixed", "streams"), (int(x) for x in v)))
+
+    def host_cells(self) -> np.ndarray:
+        """Host-path (cable) cells, (n, 3) 1-based (x, y, z) in scan order."""
+        n = C.c_int32(0)
+        self._check(self.lib.fn["host_cells"](self.h, C.byref(n), None))
+        cells = np.zeros((n.value, 3), dtype=np.int32)
+        if n.value:
+            self._check(self.lib.fn["host_cells"](self.h, C.byref(n), _ptr(cells)))
+        return cells
+
+    def gather_host_cells(self, n_host: int) -> np.ndarray:
+        """Incident voltages (n_host, 12) of the host-path cells; between scatter and connect."""
+        V = np.zeros((n_host, 12), dtype=np.float64)
+        self._check(self.lib.fn["gather_host_cells"](self.h, _ptr(V)))
+        return V
+
+    def scatter_host_cells(self, V: np.ndarray):
+        """Write the host's scattered voltages (n_host, 12) of the host-path cells back."""
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        self._check(self.lib.fn["scatter_host_cells"](self.h, _ptr(V)))
 
     def set_streams(self, streams: int):
         """1: a step's kernels in turn on one stream; 2: class kernels concurrent."""

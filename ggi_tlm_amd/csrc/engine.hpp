@@ -41,7 +41,7 @@ struct tlm_handle {
 
   // host tables (reference layout)
   bool cells_set = false, faces_set = false;
-  std::vector<int32_t> cell_code, c_mat, c_filt, c_pml, c_exc, c_out;
+  std::vector<int32_t> cell_code, c_mat, c_filt, c_pml, c_exc, c_out, c_cable;
   int n_special = 0;
   std::vector<int32_t> face_code, f_mat, f_exc, f_out;
   int n_special_faces = 0;
@@ -85,6 +85,11 @@ struct tlm_handle {
   const int32_t* d_lds_slot_p = nullptr;
   const int32_t* d_lds_slot_i = nullptr;
   int n_lds_wg = 0;
+  // host-path (cable junction) cells: storage index and 1-based (x,y,z), scan order
+  int n_host = 0;
+  const int32_t* d_host_cells = nullptr;
+  std::vector<int32_t> host_xyz;
+  double* d_host_buf = nullptr;   // 12 * n_host staging words
   int n_lds_var[2] = {0, 0};              // LDS workgroups per class-mask variant, in table order
   const int32_t* d_slot_cell = nullptr;  // mixed-line workgroup slots
   const int32_t* d_slot_idx = nullptr;

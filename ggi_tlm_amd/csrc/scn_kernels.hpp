@@ -1441,6 +1441,19 @@ __device__ inline void node_update(const Dev& d, double v[12], int mnum, long lo
   scatter_ports(v, Vx, Vy, Vz, Ix, Iy, Iz);
 }
 
+// gen_code: the reference's update code, 0 for a free-space cell beside a
+// special face, negative for a host-path cell (left incident: the host
+// scatters it between tlm_scatter and tlm_connect).
+template <bool PLAIN>
+__device__ inline void generic_node(const Dev& d, double v[12], int timestep, int slot, int i) {
+  const int code = d.gen_code[i];
+  if (code == 0) {
+    if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
+  } else if (code > 0) {
+    node_update<true>(d, v, d.gen_mat[i], d.gen_filt[i], d.gen_exc[i], d.gen_out[i], timestep, slot);
+  }
+}
+
 // Generic special cells: point sources, probes, cells touching a special
 // face; the Thevenin node, or the free-space branch for code 0 cells that
 // only touch a special face.
@@ -1458,11 +1471,7 @@ __device__ inline void generic_update(const Dev& d, const T* __restrict__ Lin, T
     for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
   }
   flush_small(v, a.small);
-  if (d.gen_code[i] == 0) {
-    if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
-  } else {
-    node_update<true>(d, v, d.gen_mat[i], d.gen_filt[i], d.gen_exc[i], d.gen_out[i], timestep, slot);
-  }
+  generic_node<PLAIN>(d, v, timestep, slot, i);
   store_cell(Lout, d.P, c, v);
 }
 
@@ -1647,15 +1656,6 @@ __device__ inline void material_node(const Dev& d, double v[12], int i) {
   node_update<false>(d, v, d.mat_num[i], d.mat_filt[i], -1, -1, 0, 0);
 }
 
-template <bool PLAIN>
-__device__ inline void generic_node(const Dev& d, double v[12], int timestep, int slot, int i) {
-  if (d.gen_code[i] == 0) {
-    if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
-  } else {
-    node_update<true>(d, v, d.gen_mat[i], d.gen_filt[i], d.gen_exc[i], d.gen_out[i], timestep, slot);
-  }
-}
-
 // Cells (lanes) per workgroup of the LDS-staged mixed-line kernel.
 #ifndef TLM_LDS_CELLS
 #define TLM_LDS_CELLS 256
@@ -1776,6 +1776,25 @@ __global__ __launch_bounds__(256) void k_connect_list(Dev d, const T* __restrict
   double v[12];
   pull_cell<T, PLAIN>(d, Lin, c, x, y, z, sg, fds ? fds[i] : -1, a, v);
   store_cell(Lout, d.P, c, v);
+}
+
+// Host-path cells: the 12 link values of cell cells[k] <-> buf[12 k + p].
+template <typename T>
+__global__ void k_gather_host(const T* __restrict__ L, long long P, const int32_t* __restrict__ cells, int n,
+                              double* __restrict__ buf) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const long long c = cells[k];
+  for (int p = 0; p < 12; p++) buf[12 * (long long)k + p] = (double)L[p * P + c];
+}
+
+template <typename T>
+__global__ void k_scatter_host(T* __restrict__ L, long long P, const int32_t* __restrict__ cells, int n,
+                               const double* __restrict__ buf) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const long long c = cells[k];
+  for (int p = 0; p < 12; p++) L[p * P + c] = (T)buf[12 * (long long)k + p];
 }
 
 // Probe DFT: acc(p,k) += v_p(n) exp(-j 2 pi f_k t_n) dt, t_n = (n-1) dt.

@@ -144,6 +144,7 @@ int finalize(tlm_handle* h) {
   int out_rank = 0, exc_rank = 0;
   size_t fdi = 0;  // walks fd_cell in scan order
   int n_pml_state = (int)h->pml_cell_param.size();
+  std::vector<int32_t> host_ref;  // host-path cells, reference index, scan order
   for (long long c = 0; c < N; c++) {
     int code = h->cell_code[c];
     bool has_fd = fdi < fd_cell.size() && fd_cell[fdi] == c;
@@ -154,6 +155,16 @@ int finalize(tlm_handle* h) {
     int s = code - 1;
     int pmlc = code ? h->c_pml[s] : 0;
     int orow = (code && h->c_out[s]) ? out_rank++ : -1;
+    if (code && h->c_cable[s]) {
+      // host-path (cable junction) cell: pulled on the device, scattered by
+      // the host between tlm_scatter and tlm_connect
+      if (pmlc || orow >= 0 || h->c_exc[s])
+        return fail(h, TLM_ERR_UNSUPPORTED, "host-path (cable) cells carry no PML, excitation or output row");
+      cls[c] = 3;
+      gr.push_back({(int32_t)c, -code, 0, 0, -1, -1, fdc});
+      host_ref.push_back((int32_t)c);
+      continue;
+    }
     if (pmlc != 0) {
       if (pmlc < 1 || pmlc > n_pml_state) return fail(h, TLM_ERR_ARG, "PML cell number out of range");
       int m = h->c_mat[s];
@@ -646,6 +657,25 @@ int finalize(tlm_handle* h) {
   if ((rc = dev_upload(h, gen_out, &d.gen_out))) return rc;
   if ((rc = dev_upload(h, gen_fd, &d.gen_fd))) return rc;
   h->d_gen_cell = d.gen_cell;
+  {
+    // host-path cells: storage indices and 1-based coordinates, scan order
+    std::vector<int32_t> hs;
+    h->host_xyz.clear();
+    for (int32_t c : host_ref) {
+      hs.push_back(sidx(c));
+      const long long r = c / nx;
+      h->host_xyz.push_back((int32_t)(c - r * nx) + 1);
+      h->host_xyz.push_back((int32_t)(r % ny) + 1);
+      h->host_xyz.push_back((int32_t)(r / ny) + 1);
+    }
+    h->n_host = (int)hs.size();
+    if ((rc = dev_upload(h, hs, &h->d_host_cells))) return rc;
+    if (h->n_host) {
+      void* p;
+      if ((rc = dev_alloc(h, (size_t)12 * h->n_host * sizeof(double), &p))) return rc;
+      h->d_host_buf = (double*)p;
+    }
+  }
   h->d_gen_fd = d.gen_fd;
   if ((rc = dev_upload(h, h->vmat, &d.vmat))) return rc;
   d.n_filt = std::max<long long>(max_filt, 1);
@@ -1091,9 +1121,7 @@ int tlm_set_cell_codes(tlm_handle* h, const int32_t* code, int32_t n_special, co
   cp(to_pml, h->c_pml);
   cp(to_excitation, h->c_exc);
   cp(to_output, h->c_out);
-  if (to_cable)
-    for (int i = 0; i < n_special; i++)
-      if (to_cable[i]) return fail(h, TLM_ERR_UNSUPPORTED, "cable cells stay on the host path");
+  cp(to_cable, h->c_cable);
   h->cells_set = true;
   return TLM_OK;
 }
@@ -1306,11 +1334,71 @@ int tlm_outer_boundary(tlm_handle* h) {
   return TLM_OK;
 }
 
+int tlm_host_cells(tlm_handle* h, int32_t* n_host, int32_t* cells) {
+  if (!h || !n_host) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  *n_host = h->n_host;
+  if (cells && h->n_host) std::copy(h->host_xyz.begin(), h->host_xyz.end(), cells);
+  return TLM_OK;
+}
+
+// Host-path cells are handed over between a step's scatter and its connect:
+// the buffer then holds their incident voltages (the scatter left them).
+static int host_window(tlm_handle* h) {
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  if (h->kind != Kind::SCATTERED || h->pend_connect)
+    return fail(h, TLM_ERR_STATE, "host-path cells are exchanged between tlm_scatter and tlm_connect");
+  return TLM_OK;
+}
+
+int tlm_gather_host_cells(tlm_handle* h, double* V) {
+  if (!h || !V) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = host_window(h))) return rc;
+  if (!h->n_host) return TLM_OK;
+  const dim3 g((h->n_host + 255) / 256);
+  if (h->form == TLM_FORM_PLAIN_FP64)
+    hipLaunchKernelGGL(scn::k_gather_host<double>, g, dim3(256), 0, h->stream, (const double*)h->planes[h->cur], h->P,
+                       h->d_host_cells, h->n_host, h->d_host_buf);
+  else
+    hipLaunchKernelGGL(scn::k_gather_host<float>, g, dim3(256), 0, h->stream, (const float*)h->planes[h->cur], h->P,
+                       h->d_host_cells, h->n_host, h->d_host_buf);
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipMemcpyAsync(V, h->d_host_buf, (size_t)12 * h->n_host * sizeof(double), hipMemcpyDeviceToHost,
+                            h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return TLM_OK;
+}
+
+int tlm_scatter_host_cells(tlm_handle* h, const double* V) {
+  if (!h || !V) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = host_window(h))) return rc;
+  if (!h->n_host) return TLM_OK;
+  HIP_TRY(h, hipMemcpyAsync(h->d_host_buf, V, (size_t)12 * h->n_host * sizeof(double), hipMemcpyHostToDevice,
+                            h->stream));
+  const dim3 g((h->n_host + 255) / 256);
+  if (h->form == TLM_FORM_PLAIN_FP64)
+    hipLaunchKernelGGL(scn::k_scatter_host<double>, g, dim3(256), 0, h->stream, (double*)h->planes[h->cur], h->P,
+                       h->d_host_cells, h->n_host, h->d_host_buf);
+  else
+    hipLaunchKernelGGL(scn::k_scatter_host<float>, g, dim3(256), 0, h->stream, (float*)h->planes[h->cur], h->P,
+                       h->d_host_cells, h->n_host, h->d_host_buf);
+  HIP_TRY(h, hipGetLastError());
+  // the caller's array may be reused at once: wait for the copy
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return TLM_OK;
+}
+
 int tlm_step(tlm_handle* h, int32_t first_timestep, int32_t n_steps) {
   if (!h) return TLM_ERR_ARG;
   if (n_steps < 0 || first_timestep < 1 || first_timestep + n_steps - 1 > h->g.n_timesteps)
     return fail(h, TLM_ERR_ARG, "timestep range out of range");
   int rc;
+  if ((rc = finalize(h))) return rc;
+  if (h->n_host) return fail(h, TLM_ERR_STATE, "host-path cells need the split calls (tlm_scatter, tlm_connect)");
   for (int t = first_timestep; t < first_timestep + n_steps; t++) {
     if ((rc = do_scatter(h, t))) return rc;
     h->pend_connect = true;

@@ -243,6 +243,28 @@ module tlm_gpu_iface
       integer(c_int32_t), intent(out) :: x0(*)
     end function tlm_layout_segments
 
+    ! Host-path (cable) cells: cells = c_null_ptr asks for the count only,
+    ! else c_loc of an integer(c_int32_t) array of 3*n_host entries.
+    integer(c_int) function tlm_host_cells(h, n_host, cells) bind(c, name='tlm_host_cells')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: n_host
+      type(c_ptr), value :: cells
+    end function tlm_host_cells
+
+    ! V(1:12, k) = incident voltages of host-path cell k (between scatter and connect)
+    integer(c_int) function tlm_gather_host_cells(h, V) bind(c, name='tlm_gather_host_cells')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: V(12, *)
+    end function tlm_gather_host_cells
+
+    integer(c_int) function tlm_scatter_host_cells(h, V) bind(c, name='tlm_scatter_host_cells')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: V(12, *)
+    end function tlm_scatter_host_cells
+
     integer(c_int) function tlm_set_dft_frequencies(h, n, f) bind(c, name='tlm_set_dft_frequencies')
       import :: c_int, c_int32_t, c_ptr, c_double
       type(c_ptr), value :: h

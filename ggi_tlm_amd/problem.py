@@ -169,6 +169,9 @@ class Problem:
     huygens: Optional[HuygensBox] = None
     outputs: List[OutputPoint] = field(default_factory=list)
     dft_frequencies: Sequence[float] = ()
+    # cells whose scatter stays on the host path (cable junction cells,
+    # cell_update_code_to_cable_cell_number != 0), 1-based (cx, cy, cz)
+    host_cells: List[Tuple[int, int, int]] = field(default_factory=list)
 
     @property
     def dt(self) -> float:
@@ -478,12 +481,18 @@ def build_tables(p: Problem) -> Tables:
             local_out[o.cell[2] - 1, o.cell[1] - 1, o.cell[0] - 1] = True
     if (pml_flag & local_exc).any():
         raise ValueError("Excitation intersects PML")
+    local_host = np.zeros(shape, dtype=bool)
+    for c in p.host_cells:
+        local_host[c[2] - 1, c[1] - 1, c[0] - 1] = True
+    if (local_host & (pml_flag | local_exc | local_out)).any():
+        raise ValueError("host-path cells carry no PML, excitation or output on the device path")
 
     flat_mat = local_material.ravel()
     flat_pml = pml_flag.ravel()
     flat_exc = local_exc.ravel()
     flat_out = local_out.ravel()
-    special = (flat_mat != 0) | flat_pml | flat_exc | flat_out
+    flat_host = local_host.ravel()
+    special = (flat_mat != 0) | flat_pml | flat_exc | flat_out | flat_host
     n_special = int(special.sum())
     cell_code = np.zeros(N, dtype=np.int32)
     cell_code[special] = np.arange(1, n_special + 1, dtype=np.int32)
@@ -577,7 +586,10 @@ def build_tables(p: Problem) -> Tables:
     to_pml = pml_cell_number[sp_idx].astype(np.int32)
     to_exc = flat_exc[sp_idx].astype(np.int32)
     to_out = flat_out[sp_idx].astype(np.int32)
-    to_cable = np.zeros(n_special, dtype=np.int32)
+    # host-path (cable junction) cells: numbered 1.. in scan order
+    host_rank = np.zeros(N, dtype=np.int32)
+    host_rank[flat_host] = np.arange(1, int(flat_host.sum()) + 1, dtype=np.int32)
+    to_cable = host_rank[sp_idx].astype(np.int32)
 
     # excitation / output rows (scan-order ranks)
     exc_rank = np.zeros(N, dtype=np.int32)

@@ -137,7 +137,10 @@ int64_t tlm_device_bytes(const tlm_handle* h);
  * set_cell_update_codes, SRC/TLM_SOLVER/set_update_codes.F90:38-363, and
  * allocate_volume_material_filter_data, SRC/MATERIALS/
  * TLM_volume_materials.F90:401-518).  to_material is the Fortran (n,2)
- * array, column-major.  to_cable must be all zero on the device path. */
+ * array, column-major.  A nonzero to_cable entry (the cable junction number,
+ * cell_update_code_to_cable_cell_number) makes the cell a host-path cell:
+ * see tlm_gather_host_cells.  Host-path cells carry no PML, excitation or
+ * output row (TLM_ERR_UNSUPPORTED). */
 int tlm_set_cell_codes(tlm_handle* h, const int32_t* cell_centre_update_code,
                        int32_t n_special_cells, const int32_t* to_material,
                        const int32_t* to_pml, const int32_t* to_excitation,
@@ -239,6 +242,21 @@ int tlm_read_face_outputs(tlm_handle* h, double* out, int32_t max_steps, int32_t
 
 /* DFT accumulators: re_im[(p*n_freq + k)*2 + {0,1}]. */
 int tlm_read_dft(tlm_handle* h, double* re_im);
+
+/* ---- host-path cells ------------------------------------------------------
+ * Cable junction cells (scatter.F90:452-462, cell_centre_cable_junction)
+ * stay on the host: the device pulls their incident voltages (connect and
+ * outer boundary) but does not scatter them.  Between tlm_scatter and
+ * tlm_connect of a step the host reads their 12 incident voltages with
+ * tlm_gather_host_cells, runs its own scatter for those cells (the
+ * reference's Thevenin path with the cable junction), and writes the 12
+ * scattered voltages back with tlm_scatter_host_cells.  Rows follow the
+ * reference's scan order (x fastest); V[12*k + p] is port p+1 of host cell
+ * k.  TLM_ERR_STATE outside that window; tlm_step refuses handles with
+ * host-path cells (use the split calls). */
+int tlm_host_cells(tlm_handle* h, int32_t* n_host, int32_t* cells /* 3*n_host, 1-based, or NULL */);
+int tlm_gather_host_cells(tlm_handle* h, double* V);
+int tlm_scatter_host_cells(tlm_handle* h, const double* V);
 
 /* ---- schedule and measurement (no counterpart in the reference) ---------- */
 

@@ -55,7 +55,7 @@ struct oracle_handle {
   /* cell codes */
   int32_t* cell_code;
   int n_special;
-  int32_t *c_mat, *c_filt, *c_pml, *c_exc, *c_out;
+  int32_t *c_mat, *c_filt, *c_pml, *c_exc, *c_out, *c_cable;
   int n_out_rows, n_exc_rows;
   /* face codes */
   int32_t* face_code;
@@ -161,7 +161,7 @@ int oracle_create(const tlm_grid* g, int32_t form, oracle_handle** out) {
 void oracle_destroy(oracle_handle* h) {
   if (!h) return;
   free(h->V); free(h->cell_code); free(h->c_mat); free(h->c_filt); free(h->c_pml);
-  free(h->c_exc); free(h->c_out); free(h->face_code); free(h->f_mat); free(h->f_exc);
+  free(h->c_exc); free(h->c_out); free(h->c_cable); free(h->face_code); free(h->f_mat); free(h->f_exc);
   free(h->f_out); free(h->vmat); free(h->smat); free(h->eps_w); free(h->mu_w);
   free(h->eps_ord); free(h->mu_ord); free(h->pml_par); free(h->pml_cell_param);
   free(h->pml_state); free(h->value); free(h->value_face); free(h->ps);
@@ -192,7 +192,9 @@ int oracle_set_cell_codes(oracle_handle* h, const int32_t* code, int32_t n_speci
   free(h->pml_mat_pos); free(h->pml_mat_state);
   h->pml_mat_pos = NULL; h->pml_mat_state = NULL; h->n_pml_mat = 0;
   for (int32_t i = 0; i < n_special; i++)
-    if (to_cable && to_cable[i]) return fail(h, TLM_ERR_UNSUPPORTED, "cable cells stay on the host path");
+    if (to_cable && to_cable[i] &&
+        ((to_pml && to_pml[i]) || (to_excitation && to_excitation[i]) || (to_output && to_output[i])))
+      return fail(h, TLM_ERR_UNSUPPORTED, "host-path cells carry no PML, excitation or output");
   h->cell_code = dup_i32(code, N);
   h->n_special = n_special;
   h->c_mat = dup_i32(to_material, n_special);
@@ -200,6 +202,8 @@ int oracle_set_cell_codes(oracle_handle* h, const int32_t* code, int32_t n_speci
   h->c_pml = dup_i32(to_pml, n_special);
   h->c_exc = dup_i32(to_excitation, n_special);
   h->c_out = dup_i32(to_output, n_special);
+  free(h->c_cable);
+  h->c_cable = dup_i32(to_cable, n_special);
   int no = 0, ne = 0;
   for (int32_t i = 0; i < n_special; i++) { no += h->c_out[i] != 0; ne += h->c_exc[i] != 0; }
   h->n_out_rows = no;
@@ -942,6 +946,9 @@ int oracle_scatter(oracle_handle* h, int32_t timestep) {
           continue;
         }
         int special = code;
+        /* host-path (cable) cell: left incident for the host's own scatter
+           (scatter.F90:290,453-462 run the cable junction inside this update) */
+        if (h->c_cable[special - 1]) continue;
         int PML_cell = h->c_pml[special - 1];
         int material_number = h->c_mat[special - 1];
         int material_type = material_number ? h->vmat[material_number - 1].type : 0;
@@ -1352,6 +1359,42 @@ int oracle_set_layout(oracle_handle* h, int32_t layout) {
   h->layout = layout;
   return TLM_OK;
 }
+/* Host-path (cable) cells in scan order (x fastest), 1-based coordinates. */
+static int host_cell_scan(oracle_handle* h, int32_t* n, int32_t* cells, double* out, const double* in) {
+  if (!h->cell_code) return fail(h, TLM_ERR_STATE, "cell codes not set");
+  int32_t k = 0, cell_number = 0;
+  for (int cz = 1; cz <= h->nz; cz++)
+    for (int cy = 1; cy <= h->ny; cy++)
+      for (int cx = 1; cx <= h->nx; cx++) {
+        int code = h->cell_code[cell_number++];
+        if (code == 0 || !h->c_cable[code - 1]) continue;
+        if (cells) { cells[3 * k] = cx; cells[3 * k + 1] = cy; cells[3 * k + 2] = cz; }
+        double* v = cellV(h, cx, cy, cz);
+        for (int p = 0; p < 12; p++) {
+          if (out) out[12 * k + p] = v[p + 1];
+          if (in) v[p + 1] = in[12 * k + p];
+        }
+        k++;
+      }
+  if (n) *n = k;
+  return TLM_OK;
+}
+
+int oracle_host_cells(oracle_handle* h, int32_t* n_host, int32_t* cells) {
+  if (!h || !n_host) return TLM_ERR_ARG;
+  return host_cell_scan(h, n_host, cells, NULL, NULL);
+}
+
+int oracle_gather_host_cells(oracle_handle* h, double* V) {
+  if (!h || !V) return TLM_ERR_ARG;
+  return host_cell_scan(h, NULL, NULL, V, NULL);
+}
+
+int oracle_scatter_host_cells(oracle_handle* h, const double* V) {
+  if (!h || !V) return TLM_ERR_ARG;
+  return host_cell_scan(h, NULL, NULL, NULL, V);
+}
+
 int oracle_layout_segments(oracle_handle* h, int32_t* n_segments, int32_t* x0) {
   if (!n_segments) return TLM_ERR_ARG;
   *n_segments = 1;

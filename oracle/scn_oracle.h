@@ -67,6 +67,10 @@ int oracle_set_streams(oracle_handle* h, int32_t streams);
 int oracle_get_schedule(const oracle_handle* h, int32_t* variant, int32_t* rows, int32_t* zc, int32_t* xcd_remap,
                         int32_t* mixed, int32_t* streams);
 int oracle_layout_segments(oracle_handle* h, int32_t* n_segments, int32_t* x0);
+/* host-path (cable) cells: the scatter leaves them incident */
+int oracle_host_cells(oracle_handle* h, int32_t* n_host, int32_t* cells);
+int oracle_gather_host_cells(oracle_handle* h, double* V);
+int oracle_scatter_host_cells(oracle_handle* h, const double* V);
 int oracle_autotune(oracle_handle* h, int32_t first_timestep, int32_t steps_per_candidate, int32_t* steps_done);
 int oracle_set_small_to_zero(oracle_handle* h, int32_t n_small_to_zero, double small_to_zero_value);
 int oracle_read_kernel_times(oracle_handle* h, int64_t* launches, double* ms);

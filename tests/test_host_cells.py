@@ -1,0 +1,189 @@
+"""Host-path (cable) cells.
+
+The reference runs the cable junction inside the cell update of cells that
+carry a cable (scatter.F90:290 picks the junction, :452-462 applies it
+between the Thevenin node voltages and the port voltages).  Here such cells
+are scattered by the host: the device's scatter leaves them incident, the
+host reads their 12 incident voltages, runs its own scatter and writes the
+scattered voltages back before the connect.
+
+Pins: a host scatter that restates the free-space node (scatter.F90:240-272)
+reproduces the deck without host-path cells bit for bit, on the oracle (here)
+and on the engine (GPU); a host scatter that shorts the node's z voltage (a
+thin PEC wire along z) changes the fields, and the engine matches the oracle
+bit for bit in the plain form and within the precision form's tolerance.
+"""
+import numpy as np
+import pytest
+
+import problems
+from ggi_tlm_amd import capi
+from ggi_tlm_amd.capi import FORM_PLAIN_FP64, FORM_PRECISION_FP32, probe_series
+from ggi_tlm_amd.problem import build_tables
+
+Z0 = 376.73031346177
+
+
+def host_scatter(V, wire):
+    """Free-space SCN scatter of rows of 12 port voltages (ports 1..12 in the
+    reference's numbering, Mesh_Modules.F90:83-94); ``wire`` shorts Vz."""
+    p = [V[:, k].copy() for k in range(12)]
+    Vx = (p[4] + p[6] + p[8] + p[10]) / 2.0
+    Vy = (p[0] + p[2] + p[9] + p[11]) / 2.0
+    Vz = (p[1] + p[3] + p[5] + p[7]) / 2.0
+    if wire:
+        Vz = np.zeros_like(Vz)
+    Ix = (p[7] - p[11] - p[5] + p[9]) / (2.0 * Z0)
+    Iy = (p[10] - p[3] - p[8] + p[1]) / (2.0 * Z0)
+    Iz = (p[2] - p[6] - p[0] + p[4]) / (2.0 * Z0)
+    out = np.empty_like(V)
+    out[:, 4] = Vx - Z0 * Iz - p[6]
+    out[:, 6] = Vx + Z0 * Iz - p[4]
+    out[:, 8] = Vx + Z0 * Iy - p[10]
+    out[:, 10] = Vx - Z0 * Iy - p[8]
+    out[:, 9] = Vy - Z0 * Ix - p[11]
+    out[:, 11] = Vy + Z0 * Ix - p[9]
+    out[:, 0] = Vy + Z0 * Iz - p[2]
+    out[:, 2] = Vy - Z0 * Iz - p[0]
+    out[:, 1] = Vz - Z0 * Iy - p[3]
+    out[:, 3] = Vz + Z0 * Iy - p[1]
+    out[:, 5] = Vz + Z0 * Ix - p[7]
+    out[:, 7] = Vz - Z0 * Ix - p[5]
+    return out
+
+
+def wire_deck(x, y, steps=60, **kw):
+    """The 24^3 PML + sphere + plane-wave deck with a z-directed run of
+    host-path cells at (x, y, 8..16)."""
+    p = problems.pml_sphere(steps=steps, **kw)
+    p.host_cells = [(x, y, z) for z in range(8, 17)]
+    return p
+
+
+def run_split(h, t, steps, wire):
+    n = len(h.host_cells())
+    for ts in range(1, steps + 1):
+        h.excitation(ts)
+        h.scatter(ts)
+        if n:
+            h.scatter_host_cells(host_scatter(h.gather_host_cells(n), wire))
+        h.connect(ts)
+        h.outer_boundary()
+    return h.read_cell_outputs()[0]
+
+
+def test_host_cells_listed_in_scan_order(oracle_lib):
+    from oracle import oracle
+
+    t = build_tables(wire_deck(8, 8))
+    cells = oracle.handle(t).host_cells()
+    assert cells.tolist() == [[8, 8, z] for z in range(8, 17)]
+
+
+def test_free_space_host_scatter_is_the_free_space_cell(oracle_lib):
+    """Oracle: host cells given the free-space scatter == no host cells."""
+    from oracle import oracle
+
+    steps = 40
+    t0 = build_tables(problems.pml_sphere(steps=steps))
+    t1 = build_tables(wire_deck(8, 8, steps))
+    a = oracle.handle(t0, output_capacity=steps)
+    b = oracle.handle(t1, output_capacity=steps)
+    oa = run_split(a, t0, steps, wire=False)
+    ob = run_split(b, t1, steps, wire=False)
+    assert np.array_equal(oa, ob)
+    assert np.array_equal(a.download_links(), b.download_links())
+
+
+def test_wire_changes_the_fields(oracle_lib):
+    from oracle import oracle
+
+    steps = 40
+    t = build_tables(wire_deck(8, 8, steps))
+    free = run_split(oracle.handle(t, output_capacity=steps), t, steps, wire=False)
+    wire = run_split(oracle.handle(t, output_capacity=steps), t, steps, wire=True)
+    assert np.max(np.abs(free - wire)) > 1e-3 * np.max(np.abs(free))
+
+
+def test_host_cells_rejected_with_output_or_pml():
+    p = wire_deck(8, 8)
+    p.host_cells.append(p.outputs[0].cell)
+    with pytest.raises(ValueError):
+        build_tables(p)
+    p = wire_deck(8, 8)
+    p.host_cells.append((2, 2, 2))          # inside the PML
+    with pytest.raises(ValueError):
+        build_tables(p)
+
+
+def test_oracle_refuses_host_cells_with_pml(oracle_lib):
+    """The C ABI itself refuses a to_cable entry on a PML cell."""
+    from oracle import oracle
+
+    t = build_tables(wire_deck(8, 8))
+    cable = np.asarray(t.cell_to_cable).copy()
+    pml_special = int(np.flatnonzero(np.asarray(t.cell_to_pml))[0])
+    cable[pml_special] = 99
+    t.cell_to_cable = cable
+    with pytest.raises(capi.TlmError):
+        oracle.handle(t)
+
+
+@pytest.mark.gpu
+def test_engine_free_space_host_scatter_is_the_free_space_cell():
+    steps = 40
+    lib = capi.engine_library()
+    t0 = build_tables(problems.pml_sphere(steps=steps))
+    t1 = build_tables(wire_deck(8, 8, steps))
+    a = capi.Handle(lib, t0, FORM_PLAIN_FP64, steps)
+    b = capi.Handle(lib, t1, FORM_PLAIN_FP64, steps)
+    oa = run_split(a, t0, steps, wire=False)
+    ob = run_split(b, t1, steps, wire=False)
+    assert np.array_equal(oa, ob)
+    assert np.array_equal(a.download_links()[:, 1:-1, 1:-1], b.download_links()[:, 1:-1, 1:-1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("x,y", [(8, 8), (11, 13)])   # (11, 13): the run crosses the sphere
+def test_engine_wire_matches_oracle(x, y, oracle_lib):
+    from oracle import oracle
+
+    steps = 60
+    t = build_tables(wire_deck(x, y, steps))
+    lib = capi.engine_library()
+    o = oracle.handle(t, output_capacity=steps)
+    g = capi.Handle(lib, t, FORM_PLAIN_FP64, steps)
+    assert np.array_equal(o.host_cells(), g.host_cells())
+    so = run_split(o, t, steps, wire=True)
+    sg = run_split(g, t, steps, wire=True)
+    assert np.array_equal(so, sg)
+    assert np.array_equal(o.download_links()[:, 1:-1, 1:-1], g.download_links()[:, 1:-1, 1:-1])
+    # precision form: FP32 links, the host exchange in FP64
+    p = capi.Handle(lib, t, FORM_PRECISION_FP32, steps)
+    sp = probe_series(run_split(p, t, steps, wire=True), t)
+    so = probe_series(so, t)
+    floor = 1e-3 * max(np.linalg.norm(so[:, k]) for k in range(so.shape[1]))
+    for k in range(so.shape[1]):
+        rel = np.linalg.norm(sp[:, k] - so[:, k]) / max(np.linalg.norm(so[:, k]), floor)
+        print(f"probe {k}: relative L2 {rel:.3e}")
+        assert rel < 1e-5, (k, rel)
+
+
+@pytest.mark.gpu
+def test_engine_host_window_and_step_refusal():
+    lib = capi.engine_library()
+    t = build_tables(wire_deck(8, 8, 10))
+    g = capi.Handle(lib, t, FORM_PLAIN_FP64, 10)
+    n = len(g.host_cells())
+    assert n == 9
+    with pytest.raises(capi.TlmError):
+        g.step(1, 2)                       # fused steps cannot hand over host cells
+    g.excitation(1)
+    with pytest.raises(capi.TlmError):
+        g.gather_host_cells(n)             # before the scatter
+    g.scatter(1)
+    V = g.gather_host_cells(n)
+    g.scatter_host_cells(host_scatter(V, True))
+    g.connect(1)
+    with pytest.raises(capi.TlmError):
+        g.gather_host_cells(n)             # after the connect
