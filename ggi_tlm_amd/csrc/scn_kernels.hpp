@@ -647,6 +647,29 @@ __device__ inline void bulk_tile(const BulkMap& m, int& bx, int& byi, int& chunk
 
 // Incident voltages of a bulk cell from its own scattered values o[] and the
 // neighbours' facing ports (x-1, x+1, y-1, y+1, z-1, z+1 pairs).
+// A bulk cell's own link values.  The plain form's connect needs all twelve
+// ((own + incoming) - own).  The precision form needs them only on the outer
+// boundary; the x-facing planes are still read in full, since they prime the
+// lines of the x-shifted neighbour reads, but the y- and z-facing planes of
+// a row are left to the rows and planes beside it, which read them once.
+template <bool PLAIN, typename T, typename O>
+__device__ inline void bulk_own(const Dev& d, const T* __restrict__ Lin, long long P, long long c, int y, int z,
+                                O o[12]) {
+  if (PLAIN) {
+#pragma unroll
+    for (int p = 0; p < 12; p++) o[p] = (O)Lin[p * P + c];
+    return;
+  }
+  o[Vy_xmin] = (O)Lin[Vy_xmin * P + c]; o[Vz_xmin] = (O)Lin[Vz_xmin * P + c];
+  o[Vy_xmax] = (O)Lin[Vy_xmax * P + c]; o[Vz_xmax] = (O)Lin[Vz_xmax * P + c];
+  o[Vx_ymin] = o[Vz_ymin] = o[Vx_ymax] = o[Vz_ymax] = (O)0;
+  o[Vx_zmin] = o[Vy_zmin] = o[Vx_zmax] = o[Vy_zmax] = (O)0;
+  if (y == 0) { o[Vx_ymin] = (O)Lin[Vx_ymin * P + c]; o[Vz_ymin] = (O)Lin[Vz_ymin * P + c]; }
+  if (y == d.ny - 1) { o[Vx_ymax] = (O)Lin[Vx_ymax * P + c]; o[Vz_ymax] = (O)Lin[Vz_ymax * P + c]; }
+  if (z == 0) { o[Vx_zmin] = (O)Lin[Vx_zmin * P + c]; o[Vy_zmin] = (O)Lin[Vy_zmin * P + c]; }
+  if (z == d.nz - 1) { o[Vx_zmax] = (O)Lin[Vx_zmax * P + c]; o[Vy_zmax] = (O)Lin[Vy_zmax * P + c]; }
+}
+
 template <bool PLAIN>
 __device__ inline void bulk_incident(const Dev& d, const PullArgs& a, int x, int y, int z, const double o[12],
                                      double xa, double xb, double xc, double xd, double ya, double yb,
@@ -686,13 +709,11 @@ __device__ inline void bulk_update(const Dev& d, const T* __restrict__ Lin, T* _
   const long long sy = seg_width(d, sg), sz = sy * d.ny;
   double v[12];
   if (PULL) {
-    // Own values stream every line of the cell's planes in once; the
-    // neighbour reads of the same lines then hit the caches.  (Reading only
-    // the facing ports, which the precision form's ordinary faces need, was
-    // measured slower.)
+    // Own values first (bulk_own): they stream the cell's x-facing lines in
+    // once and the x-shifted neighbour reads of the same lines then hit the
+    // caches.  (Reading no own values at all was measured slower.)
     double o[12];
-#pragma unroll
-    for (int p = 0; p < 12; p++) o[p] = (double)Lin[p * P + c];
+    bulk_own<PLAIN>(d, Lin, P, c, y, z, o);
     double xa = 0, xb = 0, xc = 0, xd = 0, ya = 0, yb = 0, yc = 0, yd = 0, za = 0, zb = 0, zc_ = 0, zd = 0;
     if (x > 0) {
       const long long n = nb_xm(d, c, x, y, z, sg);
@@ -1576,8 +1597,11 @@ __device__ inline void bulk_update_n(const Dev& d, const T* __restrict__ Lin, T*
     if (c < 0) continue;
     int sg;
     cell_coords(d, c, x[b], y[b], z[b], sg);
+    if (PULL) bulk_own<PLAIN>(d, Lin, P, c, y[b], z[b], o[b]);
+    else {
 #pragma unroll
-    for (int p = 0; p < 12; p++) o[b][p] = Lin[p * P + c];
+      for (int p = 0; p < 12; p++) o[b][p] = Lin[p * P + c];
+    }
     if (PULL) {
       const long long sy = seg_width(d, sg), sz = sy * d.ny;
 #pragma unroll

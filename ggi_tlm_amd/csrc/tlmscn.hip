@@ -14,6 +14,14 @@
 // the caller always sees exactly the reference's V.
 #include "engine.hpp"
 
+// Cells of padding after each link plane (plane stride = storage cells +
+// TLM_PLANE_PAD): one 128-byte line keeps the stride off a power of two
+// (a 256^3 grid has 2^24 storage cells), which was 1% faster in the plain
+// form and neutral in the precision form.
+#ifndef TLM_PLANE_PAD
+#define TLM_PLANE_PAD 32
+#endif
+
 using namespace engine;
 
 namespace engine {
@@ -532,7 +540,7 @@ int finalize(tlm_handle* h) {
 
   // ---- device allocations -------------------------------------------------
   Dev& d = h->d;
-  h->P = S;
+  h->P = S + TLM_PLANE_PAD;
   d.nx = nx; d.ny = ny; d.nz = nz; d.N = N; d.P = h->P;
   d.nseg = nseg;
   for (int sg = 0; sg <= nseg; sg++) {
@@ -1610,7 +1618,7 @@ int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
   if ((rc = finalize(h))) return rc;
   const int64_t sl = (int64_t)link_size(h);
   const int64_t L = h->d.line_len;
-  const int64_t n_lines = h->P / L;
+  const int64_t n_lines = h->d.seg_base[h->d.nseg] / L;
   // filter words of a dispersive cell: 3 components x (carried words read
   // and written) for eps and mu, FP64
   auto filter_bytes = [&](int m) -> int64_t {
