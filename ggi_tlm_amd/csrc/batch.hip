@@ -25,7 +25,8 @@ namespace engine {
 // Device record of one resident solve.
 struct BatchSolve {
   Dev d;                     // the solve's tables (d.ring: its whole output series)
-  void* planes[2];
+  void* planes[3];           // link buffers (planes[2] unused without PML cells)
+  int nbuf;
   const int32_t* slot_a;     // mixed-line workgroup slots
   const int32_t* slot_b;
   long long n_slots;
@@ -34,7 +35,7 @@ struct BatchSolve {
   const int32_t* lds_slot_p;
   const int32_t* lds_slot_i;
   int n_pure_pml, n_pure_mat, n_pure_gen, n_pm, n_huy_coded;
-  int cur;                   // plane set holding the last step's result
+  int cur, prv, prv_mode;    // link buffer rotation, as in the single-solve handle
   int ts, first, last;       // next timestep, first and last timestep
   int stz_n;                 // small-to-zero period (0: off)
   double stz_value;
@@ -52,7 +53,7 @@ struct tlm_batch {
   struct Entry {
     tlm_handle* h = nullptr;
     int first = 0, last = 0, ts = 0;  // host copy of the device timestep
-    int cur0 = 0;
+    int cur0 = 0, prv0 = 1, nbuf = 2;
     double* ring = nullptr;
     int64_t bytes = 0;
   };
@@ -99,7 +100,7 @@ int bfail(tlm_batch* b, int code, const std::string& msg) {
 // link buffers, class tables, PML and filter states, output series.
 int64_t estimate_bytes(const tlm_handle* h, int n_steps) {
   const int64_t ls = link_size(h);
-  int64_t b = 2 * 12 * ((int64_t)h->N + 64 * TLM_MAX_SEGMENTS) * ls;  // link planes, segment padding
+  int64_t b = 3 * 12 * ((int64_t)h->N + 64 * TLM_MAX_SEGMENTS + TLM_PLANE_PAD) * ls;  // link planes, padding
   b += (int64_t)h->N / 8 + (int64_t)h->N / 8;            // mask, line table
   const int64_t n_pml = (int64_t)h->pml_cell_param.size();
   b += n_pml * (8 * (scn::PS_N + 3) + 12);               // PML state and lists
@@ -112,6 +113,29 @@ int64_t estimate_bytes(const tlm_handle* h, int n_steps) {
   return b + (1 << 20);
 }
 
+// The buffer a step writes (out_buffer of the single-solve handle).
+__host__ __device__ inline int batch_out(const BatchSolve& s) {
+  return s.nbuf == 3 ? 3 - s.cur - s.prv : s.cur ^ 1;
+}
+
+// Buffer rotation after k steps of a solve that entered with (cur0, prv0):
+// the first step reads its incident voltages as they are, the later ones pull.
+inline void batch_rotation(int nbuf, int cur0, int prv0, int k, int& cur, int& prv, int& mode) {
+  BatchSolve s;
+  s.nbuf = nbuf;
+  s.cur = cur0;
+  s.prv = prv0;
+  mode = 0;
+  for (int j = 0; j < k; j++) {
+    const int out = batch_out(s);
+    s.prv = s.cur;
+    s.cur = out;
+    mode = j > 0 ? 1 : 2;
+  }
+  cur = s.cur;
+  prv = s.prv;
+}
+
 template <typename T, bool PLAIN>
 __device__ inline void batch_pull_args(const BatchSolve& s, PullArgs& a, bool& pull, int& slot) {
   pull = s.ts > s.first;
@@ -119,6 +143,10 @@ __device__ inline void batch_pull_args(const BatchSolve& s, PullArgs& a, bool& p
   a.ts_connect = s.ts - 1;
   a.small = (s.stz_n > 0 && s.ts % s.stz_n == 0) ? s.stz_value : 0.0;
   a.fslot = -1;
+  a.prev = s.planes[s.prv];
+  a.prev_mode = s.prv_mode;
+  a.prev_apply_r = true;
+  a.prev_small = (s.stz_n > 0 && (s.ts - 1) % s.stz_n == 0) ? s.stz_value : 0.0;
   slot = s.ts - s.first;
 }
 
@@ -132,7 +160,7 @@ __global__ __launch_bounds__(256) void kb_lines(const BatchSolve* __restrict__ S
   int slot;
   batch_pull_args<T, PLAIN>(s, a, pull, slot);
   const T* Lin = (const T*)s.planes[s.cur];
-  T* Lout = (T*)s.planes[s.cur ^ 1];
+  T* Lout = (T*)s.planes[batch_out(s)];
   const long long c = (long long)e.y * 256 + threadIdx.x;
   if (pull) scn::bulk_line_cell<T, PLAIN, true>(s.d, Lin, Lout, a, c);
   else scn::bulk_line_cell<T, PLAIN, false>(s.d, Lin, Lout, a, c);
@@ -170,7 +198,7 @@ __global__ __launch_bounds__(256) void kb_mixed(const BatchSolve* __restrict__ S
   int slot;
   batch_pull_args<T, PLAIN>(s, a, pull, slot);
   const T* Lin = (const T*)s.planes[s.cur];
-  T* Lout = (T*)s.planes[s.cur ^ 1];
+  T* Lout = (T*)s.planes[batch_out(s)];
   const long long si = (long long)e.y * 256 + threadIdx.x;
   if (pull) mixed_lane<T, PLAIN, true>(s, Lin, Lout, a, slot, si);
   else mixed_lane<T, PLAIN, false>(s, Lin, Lout, a, slot, si);
@@ -188,7 +216,7 @@ __global__ __launch_bounds__(scn::kLdsCells) void kb_mixed_lds(const BatchSolve*
   int slot;
   batch_pull_args<T, PLAIN>(s, a, pull, slot);
   const T* Lin = (const T*)s.planes[s.cur];
-  T* Lout = (T*)s.planes[s.cur ^ 1];
+  T* Lout = (T*)s.planes[batch_out(s)];
   if (pull)
     scn::mixed_lds_body<T, PLAIN, true, CM>(s.d, Lin, Lout, a, e.y, s.lds_lines, s.lds_fd, s.lds_slot_p,
                                             s.lds_slot_i, s.ts, slot, sv);
@@ -212,7 +240,7 @@ __global__ __launch_bounds__(256) void kb_list(const BatchSolve* __restrict__ S,
   batch_pull_args<T, PLAIN>(s, a, pull, slot);
   const Dev& d = s.d;
   const T* Lin = (const T*)s.planes[s.cur];
-  T* Lout = (T*)s.planes[s.cur ^ 1];
+  T* Lout = (T*)s.planes[batch_out(s)];
   if (pull) {
     if (CLS == 0) scn::pml_update<T, PLAIN, true>(d, Lin, Lout, a, slot, i);
     else if (CLS == 1) scn::material_update<T, PLAIN, true>(d, Lin, Lout, a, i);
@@ -252,13 +280,16 @@ __global__ __launch_bounds__(256) void kb_dft(const BatchSolve* __restrict__ S, 
   s.dft[2 * i + 1] -= v * sn * s.dt;
 }
 
-// End of a batch step: flip the ping-pong parity and advance the timestep.
+// End of a batch step: rotate the link buffers and advance the timestep.
 __global__ void kb_advance(BatchSolve* S, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   BatchSolve& s = S[i];
   if (s.ts > s.last) return;
-  s.cur ^= 1;
+  const int out = batch_out(s);
+  s.prv_mode = s.ts > s.first ? 1 : 2;
+  s.prv = s.cur;
+  s.cur = out;
   s.ts += 1;
 }
 
@@ -326,6 +357,8 @@ int rebuild(tlm_batch* b) {
     s.d.ring = e.ring;
     s.planes[0] = h->planes[0];
     s.planes[1] = h->planes[1];
+    s.planes[2] = h->planes[2];
+    s.nbuf = h->n_bufs;
     s.slot_a = h->d_slot_cell;
     s.slot_b = h->d_slot_idx;
     s.n_slots = (long long)h->n_mixed_blocks * 256;
@@ -341,7 +374,7 @@ int rebuild(tlm_batch* b) {
     // the host copy of ts / parity is exact: every step advances every
     // unfinished solve by one
     s.ts = e.ts;
-    s.cur = e.cur0 ^ ((e.ts - e.first) & 1);
+    batch_rotation(e.nbuf, e.cur0, e.prv0, e.ts - e.first, s.cur, s.prv, s.prv_mode);
     s.first = e.first;
     s.last = e.last;
     s.stz_n = h->stz_n;
@@ -491,7 +524,7 @@ int tlm_batch_add(tlm_batch* b, tlm_handle* h, int32_t first_timestep, int32_t n
   if (h->form != b->form) return bfail(b, TLM_ERR_ARG, "solve and batch numerical forms differ");
   if (n_steps < 1 || first_timestep < 1 || first_timestep + n_steps - 1 > h->g.n_timesteps)
     return bfail(b, TLM_ERR_ARG, "timestep range out of range");
-  if (h->kind != Kind::INCIDENT || h->ring_used || h->host_steps)
+  if (h->kind != Kind::INCIDENT || h->ring_used || h->host_steps || h->prv_mode != 0)
     return bfail(b, TLM_ERR_STATE, "a batch takes solves that have not been stepped");
   for (auto& e : b->solves)
     if (e.h == h) return bfail(b, TLM_ERR_STATE, "solve is already in the batch");
@@ -508,6 +541,8 @@ int tlm_batch_add(tlm_batch* b, tlm_handle* h, int32_t first_timestep, int32_t n
   e.last = first_timestep + n_steps - 1;
   e.ts = first_timestep;
   e.cur0 = h->cur;
+  e.prv0 = h->prv;
+  e.nbuf = h->n_bufs;
   const size_t ring_bytes = (size_t)n_steps * h->d.n_out_rows * 6 * sizeof(double);
   if (ring_bytes) {
     void* p = nullptr;
@@ -555,7 +590,9 @@ int tlm_batch_collect(tlm_batch* b, tlm_handle** done, int32_t max_done, int32_t
     }
     if (e.ring) BHIP_TRY(b, hipFree(e.ring));
     // the handle continues as after tlm_step(first, n): last connect pending
-    h->cur = e.cur0 ^ (steps & 1);
+    batch_rotation(e.nbuf, e.cur0, e.prv0, steps, h->cur, h->prv, h->prv_mode);
+    h->prv_apply_r = true;
+    h->prv_small = (h->stz_n > 0 && e.last % h->stz_n == 0) ? h->stz_value : 0.0;
     h->kind = Kind::SCATTERED;
     h->pend_connect = true;
     h->pend_boundary = true;

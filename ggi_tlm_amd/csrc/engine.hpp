@@ -14,6 +14,14 @@
 
 #include "scn_kernels.hpp"
 
+// Cells of padding after each link plane (plane stride = storage cells +
+// TLM_PLANE_PAD): one 128-byte line keeps the stride off a power of two
+// (a 256^3 grid has 2^24 storage cells), which was 1% faster in the plain
+// form and neutral in the precision form.
+#ifndef TLM_PLANE_PAD
+#define TLM_PLANE_PAD 32
+#endif
+
 namespace engine {
 
 using scn::Dev;
@@ -62,11 +70,21 @@ struct tlm_handle {
   // device
   bool finalized = false;
   std::vector<DevBuf> bufs;
-  void* planes[2] = {nullptr, nullptr};
-  int cur = 0;
+  // Link buffers.  cur holds the state; prv is the buffer the last step's
+  // incident voltages came from (pulled, prv_mode 1, or read as they were,
+  // 2; 0 before the first step), which the PML cells pull again to recompute
+  // their previous incident voltages.  A step writes the third buffer.
+  // Without PML cells two buffers alternate.
+  void* planes[3] = {nullptr, nullptr, nullptr};
+  int n_bufs = 2;
+  int cur = 0, prv = 1;
+  int prv_mode = 0;
+  bool prv_apply_r = false;
+  double prv_small = 0.0;
   Dev d{};
   int n_gen = 0, n_pml = 0, n_mat = 0;
   int n_pure_pml = 0, n_pure_mat = 0, n_pure_gen = 0, n_mixed_blocks = 0;
+  int n_pml_fd_pure = 0, n_pml_fd_all = 0;  // PML cells with a special face
   int n_huy_coded = 0;  // descriptor entries in the kFdHuy form
   int n_pm = 0;         // PML cells holding a material
   int stz_n = 0;        // TLM_set_small_to_zero every stz_n steps (0: off)
@@ -151,6 +169,10 @@ inline int hip_ok(tlm_handle* h, hipError_t e) { return e == hipSuccess ? TLM_OK
     hipError_t e_ = (expr);                                \
     if (e_ != hipSuccess) return hip_fail((h), e_, #expr); \
   } while (0)
+
+// The link buffer the next step or connect writes: neither the one it reads
+// (cur) nor, with three buffers, the one the PML cells pull again (prv).
+inline int out_buffer(const tlm_handle* h) { return h->n_bufs == 3 ? 3 - h->cur - h->prv : h->cur ^ 1; }
 
 template <typename V>
 inline int dev_upload(tlm_handle* h, const std::vector<V>& v, const V** out) {

@@ -440,6 +440,15 @@ struct PullArgs {
   int ts_connect;  // timestep of the pending connect (face sources)
   double small;    // > 0: incident voltages below it are set to zero first
   int fslot;       // face output ring slot of the pending connect (-1: none)
+  // The previous step's incident voltages, which PML cells without special
+  // faces recompute instead of keeping them as state: prev_mode 0 none (the
+  // state is zero), 1 pulled from the link buffer prev (its outer boundary
+  // applied if prev_apply_r), 2 read from prev as they are; prev_small is
+  // that step's flush threshold.
+  const void* prev = nullptr;
+  int prev_mode = 0;
+  bool prev_apply_r = false;
+  double prev_small = 0.0;
 };
 
 // TLM_set_small_to_zero (TLM_SOLVER/TLM_set_small_to_zero.F90:34-68):
@@ -700,34 +709,93 @@ __device__ inline int xcd_block(int b, int nblocks, int remap) {
   return (b & 7) * (nblocks >> 3) + (b >> 3);
 }
 
+// Incident voltages of a cell with six ordinary faces (connect and outer
+// boundary), pulled from the link buffer L.  Own values first (bulk_own):
+// they stream the cell's x-facing lines in once and the x-shifted neighbour
+// reads of the same lines then hit the caches.  (Reading no own values at
+// all was measured slower.)
+template <typename T, bool PLAIN>
+__device__ inline void bulk_pull(const Dev& d, const T* __restrict__ L, const PullArgs& a, long long c, int x, int y,
+                                 int z, int sg, double v[12]) {
+  const long long P = d.P;
+  const long long sy = seg_width(d, sg), sz = sy * d.ny;
+  double o[12];
+  bulk_own<PLAIN>(d, L, P, c, y, z, o);
+  double xa = 0, xb = 0, xc = 0, xd = 0, ya = 0, yb = 0, yc = 0, yd = 0, za = 0, zb = 0, zc_ = 0, zd = 0;
+  if (x > 0) {
+    const long long n = nb_xm(d, c, x, y, z, sg);
+    xa = L[Vy_xmax * P + n]; xb = L[Vz_xmax * P + n];
+  }
+  if (x + 1 < d.nx) {
+    const long long n = nb_xp(d, c, x, y, z, sg);
+    xc = L[Vy_xmin * P + n]; xd = L[Vz_xmin * P + n];
+  }
+  if (y > 0) { ya = L[Vx_ymax * P + c - sy]; yb = L[Vz_ymax * P + c - sy]; }
+  if (y + 1 < d.ny) { yc = L[Vx_ymin * P + c + sy]; yd = L[Vz_ymin * P + c + sy]; }
+  if (z > 0) { za = L[Vx_zmax * P + c - sz]; zb = L[Vy_zmax * P + c - sz]; }
+  if (z + 1 < d.nz) { zc_ = L[Vx_zmin * P + c + sz]; zd = L[Vy_zmin * P + c + sz]; }
+  bulk_incident<PLAIN>(d, a, x, y, z, o, xa, xb, xc, xd, ya, yb, yc, yd, za, zb, zc_, zd, v);
+}
+
+// bulk_pull from two link buffers at once: the current step's incident
+// voltages from L (boundary as a says) in v and the previous step's from
+// a.prev (boundary as a.prev_apply_r says) in w.  Every load is issued
+// before either set is used, so the two pulls cost one memory round trip.
+template <typename T, bool PLAIN>
+__device__ inline void bulk_pull_pair(const Dev& d, const T* __restrict__ L, const PullArgs& a, long long c, int x,
+                                      int y, int z, int sg, double v[12], double w[12]) {
+  const T* __restrict__ Q = (const T*)a.prev;
+  const long long P = d.P;
+  const long long sy = seg_width(d, sg), sz = sy * d.ny;
+  T o[12], q[12];
+  bulk_own<PLAIN>(d, L, P, c, y, z, o);
+  bulk_own<PLAIN>(d, Q, P, c, y, z, q);
+  T m[12], r[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) m[k] = r[k] = (T)0;
+  if (x > 0) {
+    const long long n = nb_xm(d, c, x, y, z, sg);
+    m[0] = L[Vy_xmax * P + n]; m[1] = L[Vz_xmax * P + n];
+    r[0] = Q[Vy_xmax * P + n]; r[1] = Q[Vz_xmax * P + n];
+  }
+  if (x + 1 < d.nx) {
+    const long long n = nb_xp(d, c, x, y, z, sg);
+    m[2] = L[Vy_xmin * P + n]; m[3] = L[Vz_xmin * P + n];
+    r[2] = Q[Vy_xmin * P + n]; r[3] = Q[Vz_xmin * P + n];
+  }
+  if (y > 0) {
+    m[4] = L[Vx_ymax * P + c - sy]; m[5] = L[Vz_ymax * P + c - sy];
+    r[4] = Q[Vx_ymax * P + c - sy]; r[5] = Q[Vz_ymax * P + c - sy];
+  }
+  if (y + 1 < d.ny) {
+    m[6] = L[Vx_ymin * P + c + sy]; m[7] = L[Vz_ymin * P + c + sy];
+    r[6] = Q[Vx_ymin * P + c + sy]; r[7] = Q[Vz_ymin * P + c + sy];
+  }
+  if (z > 0) {
+    m[8] = L[Vx_zmax * P + c - sz]; m[9] = L[Vy_zmax * P + c - sz];
+    r[8] = Q[Vx_zmax * P + c - sz]; r[9] = Q[Vy_zmax * P + c - sz];
+  }
+  if (z + 1 < d.nz) {
+    m[10] = L[Vx_zmin * P + c + sz]; m[11] = L[Vy_zmin * P + c + sz];
+    r[10] = Q[Vx_zmin * P + c + sz]; r[11] = Q[Vy_zmin * P + c + sz];
+  }
+  double od[12], qd[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) { od[k] = (double)o[k]; qd[k] = (double)q[k]; }
+  bulk_incident<PLAIN>(d, a, x, y, z, od, m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], m[9], m[10], m[11], v);
+  PullArgs b{a.prev_apply_r, 0, 0.0, -1};
+  bulk_incident<PLAIN>(d, b, x, y, z, qd, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10], r[11], w);
+}
+
 // Bulk free-space cell update, one cell per lane: pull the six faces
 // (connect and outer boundary) and scatter.
 template <typename T, bool PLAIN, bool PULL>
 __device__ inline void bulk_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                    const PullArgs& a, long long c, int x, int y, int z, int sg) {
   const long long P = d.P;
-  const long long sy = seg_width(d, sg), sz = sy * d.ny;
   double v[12];
   if (PULL) {
-    // Own values first (bulk_own): they stream the cell's x-facing lines in
-    // once and the x-shifted neighbour reads of the same lines then hit the
-    // caches.  (Reading no own values at all was measured slower.)
-    double o[12];
-    bulk_own<PLAIN>(d, Lin, P, c, y, z, o);
-    double xa = 0, xb = 0, xc = 0, xd = 0, ya = 0, yb = 0, yc = 0, yd = 0, za = 0, zb = 0, zc_ = 0, zd = 0;
-    if (x > 0) {
-      const long long n = nb_xm(d, c, x, y, z, sg);
-      xa = Lin[Vy_xmax * P + n]; xb = Lin[Vz_xmax * P + n];
-    }
-    if (x + 1 < d.nx) {
-      const long long n = nb_xp(d, c, x, y, z, sg);
-      xc = Lin[Vy_xmin * P + n]; xd = Lin[Vz_xmin * P + n];
-    }
-    if (y > 0) { ya = Lin[Vx_ymax * P + c - sy]; yb = Lin[Vz_ymax * P + c - sy]; }
-    if (y + 1 < d.ny) { yc = Lin[Vx_ymin * P + c + sy]; yd = Lin[Vz_ymin * P + c + sy]; }
-    if (z > 0) { za = Lin[Vx_zmax * P + c - sz]; zb = Lin[Vy_zmax * P + c - sz]; }
-    if (z + 1 < d.nz) { zc_ = Lin[Vx_zmin * P + c + sz]; zd = Lin[Vy_zmin * P + c + sz]; }
-    bulk_incident<PLAIN>(d, a, x, y, z, o, xa, xb, xc, xd, ya, yb, yc, yd, za, zb, zc_, zd, v);
+    bulk_pull<T, PLAIN>(d, Lin, a, c, x, y, z, sg, v);
   } else {
 #pragma unroll
     for (int p = 0; p < 12; p++) v[p] = (double)Lin[p * P + c];
@@ -946,8 +1014,11 @@ __global__ __launch_bounds__(256) void k_bulk_z(Dev d, const T* __restrict__ Lin
 }
 
 // PML cells: scatter_PML.F90 stages 1-13.
+template <typename T, bool PLAIN>
+__device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int32_t cw, const PullArgs& a,
+                                const double* wprev, const double* st = nullptr);
 template <bool PLAIN>
-__device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int32_t cw);
+__device__ inline void pml_state_preload(const Dev& d, int i, double st[9]);
 
 template <typename T, bool PLAIN, bool PULL>
 __device__ inline void pml_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
@@ -955,6 +1026,20 @@ __device__ inline void pml_update(const Dev& d, const T* __restrict__ Lin, T* __
   const int32_t cw = d.pml_cell[i];
   const long long c = cw & kCellMask;
   double v[12];
+  if (PULL && !(cw & kFlagFd) && a.prev_mode == 1) {
+    // the common case: both pulls in one round trip
+    double st[9];
+    pml_state_preload<PLAIN>(d, i, st);
+    int x, y, z, sg;
+    cell_coords(d, c, x, y, z, sg);
+    double w[12];
+    bulk_pull_pair<T, PLAIN>(d, Lin, a, c, x, y, z, sg, v, w);
+    flush_small(v, a.small);
+    flush_small(w, a.prev_small);
+    pml_node<T, PLAIN>(d, v, slot, i, cw, a, w, st);
+    store_cell(Lout, d.P, c, v);
+    return;
+  }
   if (PULL) {
     int x, y, z, sg;
     cell_coords(d, c, x, y, z, sg);
@@ -965,7 +1050,7 @@ __device__ inline void pml_update(const Dev& d, const T* __restrict__ Lin, T* __
     for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
   }
   flush_small(v, a.small);
-  pml_node<PLAIN>(d, v, slot, i, cw);
+  pml_node<T, PLAIN>(d, v, slot, i, cw, a, nullptr);
   store_cell(Lout, d.P, c, v);
 }
 
@@ -974,10 +1059,53 @@ __device__ inline void pml_update(const Dev& d, const T* __restrict__ Lin, T* __
 // off the PML kernel).
 __device__ inline double state_ld(const double* p) { return __builtin_nontemporal_load(p); }
 
+// The previous step's incident voltages of PML cell c (six ordinary faces),
+// after that step's flush: what its pull produced then, recomputed from the
+// link buffer it pulled from (a.prev), which no kernel has written since.
+template <typename T, bool PLAIN>
+__device__ inline void pml_prev_incident(const Dev& d, const PullArgs& a, long long c, double w[12]) {
+  if (a.prev_mode == 0) {
+#pragma unroll
+    for (int p = 0; p < 12; p++) w[p] = 0.0;
+    return;
+  }
+  const T* __restrict__ L = (const T*)a.prev;
+  if (a.prev_mode == 1) {
+    int x, y, z, sg;
+    cell_coords(d, c, x, y, z, sg);
+    PullArgs b{a.prev_apply_r, 0, 0.0, -1};
+    bulk_pull<T, PLAIN>(d, L, b, c, x, y, z, sg, w);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 12; p++) w[p] = (double)L[p * d.P + c];
+  }
+  flush_small(w, a.prev_small);
+}
+
 // scatter_PML.F90:22-313 stages 1-13 on the incident voltages v of PML list
-// entry i (cw: its packed cell word), in place.
+// entry i (cw: its packed cell word), in place.  The previous step's scaled
+// incident voltages (last_Vi) and branch currents (last_I) are recomputed
+// from that step's incident voltages (pml_prev_incident) by the same
+// operations that produced them, so they equal the stored words bit for bit;
+// a cell with a special face keeps them as state words.  wprev: those
+// incident voltages when the caller pulled them already (flushed), else null.
+// The nine state words every PML cell carries (Vxt, Vyt, Vzt, Vyxt, Vzxt,
+// Vxyt, Vzyt, Vyzt, Vxzt), loaded ahead so that they are in flight with the
+// link loads.
 template <bool PLAIN>
-__device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int32_t cw) {
+__device__ inline void pml_state_preload(const Dev& d, int i, double st[9]) {
+  const long long n = d.n_pml_state;
+  const double* S = d.pml_state;
+  const int w[9] = {PLAIN ? PS_Vxt : PC_Vxt, PLAIN ? PS_Vyt : PC_Vyt, PLAIN ? PS_Vzt : PC_Vzt,
+                    PLAIN ? PS_Vyxt : PC_Vyxt, PLAIN ? PS_Vzxt : PC_Vzxt, PLAIN ? PS_Vxyt : PC_Vxyt,
+                    PLAIN ? PS_Vzyt : PC_Vzyt, PLAIN ? PS_Vyzt : PC_Vyzt, PLAIN ? PS_Vxzt : PC_Vxzt};
+#pragma unroll
+  for (int k = 0; k < 9; k++) st[k] = state_ld(&S[w[k] * n + i]);
+}
+
+template <typename T, bool PLAIN>
+__device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int32_t cw, const PullArgs& a,
+                                const double* wprev, const double* st) {
   const double Z0 = kZ0, dl = d.dl;
   const tlm_pml_parameter& Pp = d.pml_params[d.pml_par[i]];
   const double ax = Pp.ax, ay = Pp.ay, az = Pp.az;
@@ -996,11 +1124,57 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
   constexpr int wVyxt = PLAIN ? PS_Vyxt : PC_Vyxt, wVzxt = PLAIN ? PS_Vzxt : PC_Vzxt;
   constexpr int wVxyt = PLAIN ? PS_Vxyt : PC_Vxyt, wVzyt = PLAIN ? PS_Vzyt : PC_Vzyt;
   constexpr int wVyzt = PLAIN ? PS_Vyzt : PC_Vyzt, wVxzt = PLAIN ? PS_Vxzt : PC_Vxzt;
-  const double last_Ix = state_ld(&S[wIx * n + si]), last_Iy = state_ld(&S[wIy * n + si]), last_Iz = state_ld(&S[wIz * n + si]);
-  const double last_Vxt = state_ld(&S[wVxt * n + si]), last_Vyt = state_ld(&S[wVyt * n + si]), last_Vzt = state_ld(&S[wVzt * n + si]);
-  const double last_Vyxt = state_ld(&S[wVyxt * n + si]), last_Vzxt = state_ld(&S[wVzxt * n + si]);
-  const double last_Vxyt = state_ld(&S[wVxyt * n + si]), last_Vzyt = state_ld(&S[wVzyt * n + si]);
-  const double last_Vyzt = state_ld(&S[wVyzt * n + si]), last_Vxzt = state_ld(&S[wVxzt * n + si]);
+  // previous incident voltages: recomputed (rec) or state words
+  const bool rec = (cw & kFlagFd) == 0;
+  double last_Ix, last_Iy, last_Iz;
+  double lv[12];                               // plain: last_Vi
+  double Szx, Syx, Sxy, Szy, Syz, Sxz;         // precision: pair sums of last_Vi
+  if (rec) {
+    double w[12];
+    if (wprev) {
+#pragma unroll
+      for (int k = 0; k < 12; k++) w[k] = wprev[k];
+    } else {
+      pml_prev_incident<T, PLAIN>(d, a, cw & kCellMask, w);
+    }
+    w[Vynx] = w[Vynx] * exp_y; w[Vypx] = w[Vypx] * exp_y;
+    w[Vznx] = w[Vznx] * exp_z; w[Vzpx] = w[Vzpx] * exp_z;
+    w[Vxny] = w[Vxny] * exp_x; w[Vxpy] = w[Vxpy] * exp_x;
+    w[Vzny] = w[Vzny] * exp_z; w[Vzpy] = w[Vzpy] * exp_z;
+    w[Vxnz] = w[Vxnz] * exp_x; w[Vxpz] = w[Vxpz] * exp_x;
+    w[Vynz] = w[Vynz] * exp_y; w[Vypz] = w[Vypz] * exp_y;
+    last_Ix = (w[Vypz] - w[Vzpy] - w[Vynz] + w[Vzny]) / (2.0 * Z0);
+    last_Iy = (w[Vzpx] - w[Vxpz] - w[Vznx] + w[Vxnz]) / (2.0 * Z0);
+    last_Iz = (w[Vxpy] - w[Vypx] - w[Vxny] + w[Vynx]) / (2.0 * Z0);
+#pragma unroll
+    for (int k = 0; k < 12; k++) lv[k] = w[k];
+    Szx = w[Vznx] + w[Vzpx]; Syx = w[Vynx] + w[Vypx];
+    Sxy = w[Vxny] + w[Vxpy]; Szy = w[Vzny] + w[Vzpy];
+    Syz = w[Vynz] + w[Vypz]; Sxz = w[Vxnz] + w[Vxpz];
+  } else {
+    last_Ix = state_ld(&S[wIx * n + si]); last_Iy = state_ld(&S[wIy * n + si]); last_Iz = state_ld(&S[wIz * n + si]);
+    if (PLAIN) {
+#pragma unroll
+      for (int k = 0; k < 12; k++) lv[k] = state_ld(&S[(PS_Vi + k) * n + si]);
+      Szx = Syx = Sxy = Szy = Syz = Sxz = 0.0;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 12; k++) lv[k] = 0.0;
+      Szx = state_ld(&S[PC_Szx * n + si]); Syx = state_ld(&S[PC_Syx * n + si]);
+      Sxy = state_ld(&S[PC_Sxy * n + si]); Szy = state_ld(&S[PC_Szy * n + si]);
+      Syz = state_ld(&S[PC_Syz * n + si]); Sxz = state_ld(&S[PC_Sxz * n + si]);
+    }
+  }
+  // (loading these ahead of the previous step's pull was measured slower)
+  double st_[9];
+  if (!st) {
+    pml_state_preload<PLAIN>(d, i, st_);
+    st = st_;
+  }
+  const double last_Vxt = st[0], last_Vyt = st[1], last_Vzt = st[2];
+  const double last_Vyxt = st[3], last_Vzxt = st[4];
+  const double last_Vxyt = st[5], last_Vzyt = st[6];
+  const double last_Vyzt = st[7], last_Vxzt = st[8];
   // STAGE 3
   v[Vynx] = v[Vynx] * exp_y; v[Vypx] = v[Vypx] * exp_y;
   v[Vznx] = v[Vznx] * exp_z; v[Vzpx] = v[Vzpx] * exp_z;
@@ -1018,9 +1192,6 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
   // STAGE 5
   double Vx_sh, Vy_sh, Vz_sh;
   if (PLAIN) {
-    double lv[12];
-#pragma unroll
-    for (int k = 0; k < 12; k++) lv[k] = state_ld(&S[(PS_Vi + k) * n + si]);
     Vx_sh = ax * (-(lv[Vznx] + lv[Vzpx] + lv[Vynx] + lv[Vypx]) / 2.0
                   + Csy * ((v[Vznx] + v[Vzpx]) + (lv[Vznx] + lv[Vzpx]))
                   + Csz * ((v[Vynx] + v[Vypx]) + (lv[Vynx] + lv[Vypx]))
@@ -1034,9 +1205,6 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
                   + Csy * ((v[Vxnz] + v[Vxpz]) + (lv[Vxnz] + lv[Vxpz]))
                   + Csx_sy * last_Vzt);
   } else {
-    const double Szx = state_ld(&S[PC_Szx * n + si]), Syx = state_ld(&S[PC_Syx * n + si]);
-    const double Sxy = state_ld(&S[PC_Sxy * n + si]), Szy = state_ld(&S[PC_Szy * n + si]);
-    const double Syz = state_ld(&S[PC_Syz * n + si]), Sxz = state_ld(&S[PC_Sxz * n + si]);
     Vx_sh = ax * (-(Szx + Syx) * 0.5 + Csy * ((v[Vznx] + v[Vzpx]) + Szx) + Csz * ((v[Vynx] + v[Vypx]) + Syx)
                   + Csy_sz * last_Vxt);
     Vy_sh = ay * (-(Sxy + Szy) * 0.5 + Csz * ((v[Vxny] + v[Vxpy]) + Sxy) + Csx * ((v[Vzny] + v[Vzpy]) + Szy)
@@ -1072,14 +1240,16 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
   const double Vyzt = ax * Ix * Z0 + Vyz_se;
   const double Vzyt = ax * Ix * Z0 + Vzy_se;
   // STAGE 9
-  S[wIx * n + si] = Ix; S[wIy * n + si] = Iy; S[wIz * n + si] = Iz;
-  if (PLAIN) {
+  if (!rec) {
+    S[wIx * n + si] = Ix; S[wIy * n + si] = Iy; S[wIz * n + si] = Iz;
+    if (PLAIN) {
 #pragma unroll
-    for (int k = 0; k < 12; k++) S[(PS_Vi + k) * n + si] = v[k];
-  } else {
-    S[PC_Szx * n + si] = v[Vznx] + v[Vzpx]; S[PC_Syx * n + si] = v[Vynx] + v[Vypx];
-    S[PC_Sxy * n + si] = v[Vxny] + v[Vxpy]; S[PC_Szy * n + si] = v[Vzny] + v[Vzpy];
-    S[PC_Syz * n + si] = v[Vynz] + v[Vypz]; S[PC_Sxz * n + si] = v[Vxnz] + v[Vxpz];
+      for (int k = 0; k < 12; k++) S[(PS_Vi + k) * n + si] = v[k];
+    } else {
+      S[PC_Szx * n + si] = v[Vznx] + v[Vzpx]; S[PC_Syx * n + si] = v[Vynx] + v[Vypx];
+      S[PC_Sxy * n + si] = v[Vxny] + v[Vxpy]; S[PC_Szy * n + si] = v[Vzny] + v[Vzpy];
+      S[PC_Syz * n + si] = v[Vynz] + v[Vypz]; S[PC_Sxz * n + si] = v[Vxnz] + v[Vxpz];
+    }
   }
   S[wVxt * n + si] = Vxt; S[wVyt * n + si] = Vyt; S[wVzt * n + si] = Vzt;
   if (orow >= 0) {
@@ -1740,7 +1910,7 @@ __device__ inline void mixed_lds_body(const Dev& d, const T* __restrict__ Lin, T
       if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
     } else {
       const int i = slot_i[base + t];
-      if ((CM & 1) && cls == 1) pml_node<PLAIN>(d, v, slot, i, d.pml_cell[i]);
+      if ((CM & 1) && cls == 1) pml_node<T, PLAIN>(d, v, slot, i, d.pml_cell[i], a, nullptr);
       else if ((CM & 2) && cls == 2) material_node(d, v, i);
       else if ((CM & 4) && cls == 3) generic_node<PLAIN>(d, v, timestep, slot, i);
     }

@@ -14,14 +14,6 @@
 // the caller always sees exactly the reference's V.
 #include "engine.hpp"
 
-// Cells of padding after each link plane (plane stride = storage cells +
-// TLM_PLANE_PAD): one 128-byte line keeps the stride off a power of two
-// (a 256^3 grid has 2^24 storage cells), which was 1% faster in the plain
-// form and neutral in the precision form.
-#ifndef TLM_PLANE_PAD
-#define TLM_PLANE_PAD 32
-#endif
-
 using namespace engine;
 
 namespace engine {
@@ -476,6 +468,12 @@ int finalize(tlm_handle* h) {
     gen_exc.push_back(r.er); gen_out.push_back(r.orow); gen_fd.push_back(r.fdc);
   }
   h->n_pure_pml = n_pure_pml;
+  h->n_pml_fd_pure = h->n_pml_fd_all = 0;
+  for (size_t i = 0; i < pr.size(); i++)
+    if (pr[i].fdc >= 0) {
+      h->n_pml_fd_all++;
+      if ((int)i < n_pure_pml) h->n_pml_fd_pure++;
+    }
   h->n_pure_mat = n_pure_mat;
   h->n_pure_gen = n_pure_gen;
   h->n_mixed_cells = n_mixed_cells;
@@ -552,7 +550,8 @@ int finalize(tlm_handle* h) {
   d.R[0] = h->g.r_xmin; d.R[1] = h->g.r_xmax; d.R[2] = h->g.r_ymin;
   d.R[3] = h->g.r_ymax; d.R[4] = h->g.r_zmin; d.R[5] = h->g.r_zmax;
   const size_t ls = link_size(h);
-  for (int b = 0; b < 2; b++) {
+  h->n_bufs = pml_cell.empty() ? 2 : 3;
+  for (int b = 0; b < h->n_bufs; b++) {
     if ((rc = dev_alloc(h, 12 * (size_t)h->P * ls, &h->planes[b]))) return rc;
     HIP_TRY(h, hipMemsetAsync(h->planes[b], 0, 12 * (size_t)h->P * ls, h->stream));
   }
@@ -851,7 +850,7 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
   const Dev& d = h->d;
   if (pull) launch_huygens_values(h, a);
   const T* Lin = (const T*)h->planes[h->cur];
-  T* Lout = (T*)h->planes[h->cur ^ 1];
+  T* Lout = (T*)h->planes[out_buffer(h)];
   // The class kernels write cells disjoint from the bulk and PML kernels'
   // and read only the input buffer, so with two streams they run on the
   // second stream concurrently with those (joined before the next kernel
@@ -982,7 +981,7 @@ int launch_connect(tlm_handle* h, const PullArgs& a) {
   const Dev& d = h->d;
   launch_huygens_values(h, a);
   const T* Lin = (const T*)h->planes[h->cur];
-  T* Lout = (T*)h->planes[h->cur ^ 1];
+  T* Lout = (T*)h->planes[out_buffer(h)];
   dim3 bb(64, 4, 1);
   dim3 gb((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
   hipLaunchKernelGGL((scn::k_connect_bulk<T, PLAIN>), gb, bb, 0, h->stream, d, Lin, Lout, a);
@@ -1017,6 +1016,10 @@ int do_scatter(tlm_handle* h, int timestep) {
   int fslot = -1;
   if (pull && (rc = reserve_face_slot(h, h->pend_ts, &fslot))) return rc;
   PullArgs a{h->pend_boundary, h->pend_ts, small, fslot};
+  a.prev = h->planes[h->prv];
+  a.prev_mode = h->prv_mode;
+  a.prev_apply_r = h->prv_apply_r;
+  a.prev_small = h->prv_small;
   if (h->d.n_out_rows > 0 && h->ring_used == h->cap)
     if ((rc = drain_ring(h))) return rc;
   int slot = h->ring_used;
@@ -1034,7 +1037,12 @@ int do_scatter(tlm_handle* h, int timestep) {
                          slot, timestep, h->d_prow, h->d_pcomp, np, h->d_freq, nf, h->g.dt, h->d_dft);
     }
   }
-  h->cur ^= 1;
+  const int out = out_buffer(h);
+  h->prv = h->cur;
+  h->cur = out;
+  h->prv_mode = pull ? 1 : 2;
+  h->prv_apply_r = a.apply_r;
+  h->prv_small = small;
   h->kind = Kind::SCATTERED;
   h->pend_connect = false;
   h->pend_boundary = false;
@@ -1054,7 +1062,7 @@ int materialize(tlm_handle* h) {
   else rc = launch_connect<float, false>(h, a);
   if (rc) return rc;
   HIP_TRY(h, hipGetLastError());
-  h->cur ^= 1;
+  h->cur = out_buffer(h);  // prv stays: the next step still pulls it again
   h->kind = Kind::INCIDENT;
   h->pend_connect = false;
   h->pend_boundary = false;
@@ -1628,10 +1636,14 @@ int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
     int64_t words = (vm.eps_filter_exists ? vm.eps_order + 1 : 0) + (vm.mu_filter_exists ? vm.mu_order + 1 : 0);
     return 3 * 2 * 8 * words;
   };
-  // PML: links, FP64 state words read and written (24 plain, 18 precision),
-  // packed cell word and 16-bit parameter row
+  // PML: links, the previous step's incident voltages pulled again (12
+  // links), the nine FP64 state words read and written, packed cell word and
+  // 16-bit parameter row; a cell with a special face reads and writes all
+  // its state words (24 plain, 18 precision) instead of the second pull.
   const int64_t pml_words = h->form == TLM_FORM_PLAIN_FP64 ? scn::PS_N : scn::PC_N;
-  const int64_t pml_cell_bytes = 24 * sl + 2 * 8 * pml_words + 4 + 2;
+  const int64_t pml_rec_bytes = 36 * sl + 2 * 8 * 9 + 4 + 2;
+  const int64_t pml_fd_bytes = 24 * sl + 2 * 8 * pml_words + 4 + 2;
+  auto pml_bytes = [&](int64_t n, int64_t n_fd) { return (n - n_fd) * pml_rec_bytes + n_fd * pml_fd_bytes; };
   // generic: links, 7 int32 list fields, filter words
   std::vector<int32_t> gm(h->n_gen);
   if (h->n_gen) HIP_TRY(h, hipMemcpy(gm.data(), h->d.gen_mat, gm.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -1657,11 +1669,11 @@ int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
   // mixed lines: every cell with its class's bytes, plus two slot words
   cells[TLM_KCLASS_MIXED] = h->n_mixed_cells;
   bytes[TLM_KCLASS_MIXED] = (h->n_mixed_cells - n_special_mixed) * 24 * sl +
-                            (int64_t)(h->n_pml - h->n_pure_pml) * pml_cell_bytes +
+                            pml_bytes(h->n_pml - h->n_pure_pml, h->n_pml_fd_all - h->n_pml_fd_pure) +
                             mat_bytes(h->n_pure_mat, h->n_mat) + gen_bytes(h->n_pure_gen, h->n_gen) +
                             (int64_t)h->n_mixed_blocks * 256 * 8;
   cells[TLM_KCLASS_PML] = h->n_pure_pml;
-  bytes[TLM_KCLASS_PML] = (int64_t)h->n_pure_pml * pml_cell_bytes;
+  bytes[TLM_KCLASS_PML] = pml_bytes(h->n_pure_pml, h->n_pml_fd_pure);
   cells[TLM_KCLASS_GENERIC] = h->n_pure_gen;
   bytes[TLM_KCLASS_GENERIC] = gen_bytes(0, h->n_pure_gen);
   cells[TLM_KCLASS_MATERIAL] = h->n_pure_mat;
