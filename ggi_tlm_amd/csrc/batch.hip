@@ -4,7 +4,7 @@
 // whose device tables and link planes stay resident in HBM.  One step of
 // the batch is one launch per kernel class over the union of the solves'
 // work: a block table maps every workgroup to (solve, block of that solve),
-// and each solve's timestep, ping-pong parity and output slot live in a
+// and each solve's timestep, link buffer rotation and output slot live in a
 // device record, so the K-step launch sequence is captured once in a HIP
 // graph and replayed until the set of solves changes.  Solves that finish
 // inside a replay idle (their workgroups return at once) until collected.
