@@ -71,6 +71,41 @@ def test_batch_matches_alone(form, kernels):
         h.close()
 
 
+@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
+def test_batch_flush_and_continue_alone(form):
+    """The small-value flush inside a batch (PML cells recompute the previous
+    step's incident voltages with that step's flush), and a collected solve
+    continued on the single-solve path (the link-buffer rotation handed back):
+    the same bits as stepping alone throughout."""
+    lib = capi.engine_library()
+    decks = [("pml_sphere", build_tables(problems.pml_sphere(24, 60)), 30, 60),
+             ("special_mix", build_tables(problems.special_mix()), 17, 40)]
+    for name, t, n_batch, n_total in decks:
+        a = capi.Handle(lib, t, form, output_capacity=n_total)
+        a.set_small_to_zero(3, 1e-5)
+        a.step(1, n_total)
+        ref = results(a)
+        a.close()
+        h = capi.Handle(lib, t, form, output_capacity=n_total)
+        h.set_small_to_zero(3, 1e-5)
+        b = capi.Batch(form, 16 << 30)
+        b.set_graph_steps(4)
+        assert b.add(h, 1, n_batch)
+        got = []
+        for _ in range(n_batch):
+            b.run(1)
+            got += b.collect()
+            if got:
+                break
+        assert got and got[0] is h, name
+        b.close()
+        h.step(n_batch + 1, n_total - n_batch)
+        out, first, dft, links = results(h)
+        assert first == 1 and np.array_equal(out, ref[0]), name
+        assert np.array_equal(links, ref[3]), name
+        h.close()
+
+
 def test_batch_admission_by_budget():
     lib = capi.engine_library()
     t = build_tables(problems.pml_sphere(24, 40))
