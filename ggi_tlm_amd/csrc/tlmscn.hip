@@ -877,14 +877,14 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
       const int32_t* blocks = direct ? nullptr : (yband ? h->d_bulk_blocks_yband : h->d_bulk_blocks);
       const int nb = yband ? h->n_bulk_blocks_yband : (int)h->bulk_blocks.size();
       const int per = h->sched_by;  // storage blocks per workgroup
-      const int nwg = direct ? (int)((d.seg_base[d.nseg] + 255) / 256)
+      const int nwg = direct ? (int)((d.seg_base[d.nseg] + scn::kLineWg - 1) / scn::kLineWg)
                              : (yband ? 8 * ((nb / 8 + per - 1) / per) : (nb + per - 1) / per);
       if (direct) {
         if (pull)
-          hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, true>), dim3(nwg), dim3(256), 0, h->stream, d, Lin, Lout, a,
+          hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, true>), dim3(nwg), dim3(scn::kLineWg), 0, h->stream, d, Lin, Lout, a,
                              h->sched_xcd);
         else
-          hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, false>), dim3(nwg), dim3(256), 0, h->stream, d, Lin, Lout, a,
+          hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, false>), dim3(nwg), dim3(scn::kLineWg), 0, h->stream, d, Lin, Lout, a,
                              h->sched_xcd);
       } else if (nb) {
         if (pull)
