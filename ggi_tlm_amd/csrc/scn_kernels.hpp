@@ -7,13 +7,16 @@
 // holds its cells x-fastest with row width w_s = x1_s - x0_s from storage
 // index base_s: c = base_s + (x - x0_s) + w_s*(y + ny*z).  A PML slab or a
 // source plane thus fills 128-byte lines by itself.  One segment is the
-// reference's scan order c = x + nx*(y + ny*z).  Two buffers ping-pong: a
-// step reads the previous step's scattered voltages from one buffer, pulls
-// each cell's incident voltages across its six faces (the reference's
-// connect and outer_boundary, SRC/TLM_SOLVER/connect.F90:180-214 and
+// reference's scan order c = x + nx*(y + ny*z).  A step reads the previous
+// step's scattered voltages from one buffer, pulls each cell's incident
+// voltages across its six faces (the reference's connect and
+// outer_boundary, SRC/TLM_SOLVER/connect.F90:180-214 and
 // outer_boundary.F90:71-131), scatters (scatter.F90:216-585) and writes the
-// new scattered voltages to the other buffer.  Each link therefore crosses
+// new scattered voltages to another buffer.  Each link therefore crosses
 // HBM once per step in each direction; the neighbour reads hit the caches.
+// With PML cells three buffers rotate: the one the previous step pulled
+// from is kept, and PML cells pull it again for their previous incident
+// voltages instead of storing them.
 //
 // Every 128-byte line of a link plane is written whole by one kernel.  Lines
 // holding one class only run in that class's wavefront-uniform kernel:
@@ -59,12 +62,15 @@ enum {
 };
 
 // PML state words per cell (PML_module.F90:117-129), SoA: state[w*n + i].
+// A PML cell without special faces streams only the nine words Vxt, Vyt,
+// Vzt and the six series terms; it recomputes Ix, Iy, Iz and Vi (or the
+// pair sums) from the previous step's incident voltages (pml_node).
 // Plain form: the reference's 24 words, Vi(12) kept port by port.
 enum {
   PS_Vxt, PS_Vyt, PS_Vzt, PS_Ix, PS_Iy, PS_Iz,
   PS_Vi,  // 12 words
   PS_Vyxt = PS_Vi + 12, PS_Vzxt, PS_Vxyt, PS_Vzyt, PS_Vyzt, PS_Vxzt,
-  PS_N    // 24 words streamed by every PML cell
+  PS_N
 };
 // Precision form: the previous step's incident voltages Vi enter the update
 // only as the six pair sums (Vznx+Vzpx), (Vynx+Vypx), (Vxny+Vxpy),
