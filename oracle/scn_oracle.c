@@ -1286,6 +1286,9 @@ int oracle_outer_boundary(oracle_handle* h) {
 }
 
 int oracle_step(oracle_handle* h, int32_t first_timestep, int32_t n_steps) {
+  if (h->c_cable)
+    for (int32_t i = 0; i < h->n_special; i++)
+      if (h->c_cable[i]) return fail(h, TLM_ERR_STATE, "host-path cells need the split calls (tlm_scatter, tlm_connect)");
   for (int32_t t = first_timestep; t < first_timestep + n_steps; t++) {
     int rc;
     if ((rc = oracle_excitation(h, t))) return rc;

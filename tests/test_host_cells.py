@@ -116,6 +116,15 @@ def test_host_cells_rejected_with_output_or_pml():
         build_tables(p)
 
 
+def test_oracle_fused_step_refuses_host_cells(oracle_lib):
+    """As the engine: the fused step has no hand-off point."""
+    from oracle import oracle
+
+    h = oracle.handle(build_tables(wire_deck(8, 8, 10)), output_capacity=10)
+    with pytest.raises(capi.TlmError):
+        h.step(1, 2)
+
+
 def test_oracle_refuses_host_cells_with_pml(oracle_lib):
     """The C ABI itself refuses a to_cable entry on a PML cell."""
     from oracle import oracle
