@@ -334,12 +334,23 @@ void launch_batch_step(tlm_batch* b) {
   hipLaunchKernelGGL(kb_advance, dim3((n + 255) / 256), dim3(256), 0, st, S, n);
 }
 
+// Destroy the captured graph.  tlm_batch_run returns while its replays may
+// still execute, and a replay reads the graph's kernel nodes and argument
+// buffers, the device records and the block tables: the stream is drained
+// first, so that neither the graph nor the tables it names are released
+// (or, in rebuild, overwritten) while a replay is in flight.
+int drop_graph(tlm_batch* b) {
+  if (!b->graph) return TLM_OK;
+  BHIP_TRY(b, hipStreamSynchronize(b->stream));
+  (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  return TLM_OK;
+}
+
 // Rebuild the device records and block tables after the set of solves changed.
 int rebuild(tlm_batch* b) {
-  if (b->graph) {
-    (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-  }
+  int rc;
+  if ((rc = drop_graph(b))) return rc;
   const int n = (int)b->solves.size();
   if (n > b->d_cap) {
     if (b->d_solves) BHIP_TRY(b, hipFree(b->d_solves));
@@ -425,10 +436,8 @@ int rebuild(tlm_batch* b) {
 }
 
 int capture(tlm_batch* b, int k) {
-  if (b->graph) {
-    (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-  }
+  int rc;
+  if ((rc = drop_graph(b))) return rc;
   hipGraph_t g = nullptr;
   BHIP_TRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < k; i++) {
@@ -628,9 +637,9 @@ int tlm_batch_set_kernels(tlm_batch* b, int32_t mixed, int32_t streams) {
     return bfail(b, TLM_ERR_ARG, "mixed must be -1, 0 or 1 and streams 1 or 2");
   if (mixed != b->mixed) b->dirty = true;  // the mixed-line block table changes
   b->mixed = mixed;
-  if (streams != b->streams && b->graph) {
-    (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
+  if (streams != b->streams) {
+    int rc;
+    if ((rc = drop_graph(b))) return rc;
   }
   b->streams = streams;
   return TLM_OK;
