@@ -174,6 +174,62 @@ def test_zero_conductivity_pml_material_equals_material(oracle_lib, mu_r, sigma_
         assert np.linalg.norm(a[:, k] - b[:, k]) <= 1e-9 * np.linalg.norm(a[:, k]), k
 
 
+@pytest.mark.parametrize("reflection", [1e-3, 1e-6])
+def test_free_space_material_in_real_pml_equals_pml(oracle_lib, reflection):
+    """An eps_r = mu_r = 1, sigma = 0 block reaching into a real PML
+    (sigma != 0 on every axis pair) takes the PML-material update
+    (scatter_PML_material.F90:86-221 coefficients A..Q built from s_x, s_y,
+    s_z and the material's stubs) in its PML cells, the ordinary PML update
+    (scatter_PML.F90, pinned by TEM_PML_TEST) elsewhere.  With no stubs and
+    no conductivity the two-step recursions must reduce to the PML cell:
+    a swapped s_x/s_y or a dropped sigma term in any coefficient gives an
+    O(1) difference, rounding stays far below 1e-9."""
+    base = problems.pml_material(reflection=reflection)
+    base.materials = []
+    base.material_regions = []
+    mat = problems.pml_material(reflection=reflection)
+    mat.materials = [VolumeMaterial("dispersive", Sfilter.constant(1.0), Sfilter.constant(1.0))]
+    # the block runs through the PML corner regions of x, y and z
+    mat.material_regions = [(Box(1, 6, 1, 16, 1, 16), 1)]
+    a, b = _series(base, 300), _series(mat, 300)
+    t = build_tables(mat)
+    assert t.n_pml_cells > 0
+    for k in range(a.shape[1]):
+        assert np.linalg.norm(a[:, k]) > 0
+        assert np.linalg.norm(a[:, k] - b[:, k]) <= 1e-9 * np.linalg.norm(a[:, k]), k
+
+
+@pytest.mark.parametrize("sigma_e,sigma_m", [(0.0, 0.0), (0.02, 20.0)])
+def test_material_filled_pml_corners_stay_stable(oracle_lib, sigma_e, sigma_m):
+    """A box filled with a magnetic dielectric (eps_r = 3, mu_r = 2) up to
+    the outer wall, 6-cell PML on all faces: every edge and corner of the
+    PML holds the material, so the PML-material coefficients take all
+    their s_i*s_j*(stub) terms (scatter_PML_material.F90:86-221).  After the
+    pulse the link energy may only fall (static charge left by the source
+    stays; the conductivities drain it): over 6000 steps it never exceeds
+    its value at step 1000 by more than 0.1%.  A factor or index slip in
+    any coefficient, down to a second-order corner term, makes the
+    recursion grow without bound within those steps."""
+    dl = 0.01
+    dt = dl / (2 * C0)
+    n, npml = 24, 6
+    p = Problem(nx=n, ny=n, nz=n, dl=dl, n_timesteps=6000, reflection=(-1.0,) * 6,
+                functions=[ExcitationFunction("gaussian", (1.0, 3 * dt, 15 * dt))],
+                sources=[PointSource((npml + 4, npml + 5, npml + 6), "ez", 1)],
+                materials=[VolumeMaterial("dispersive", Sfilter.constant(3.0), Sfilter.constant(2.0),
+                                          sigma_e=sigma_e, sigma_m=sigma_m)],
+                material_regions=[(Box(1, n, 1, n, 1, n), 1)],
+                pml=PML((npml,) * 6, (1e-4,) * 6, 2))
+    h = oracle_handle(build_tables(p), 10)
+    e = []
+    for k in range(12):
+        h.step(1 + 500 * k, 500)
+        e.append(float(np.sum(links(h) ** 2)))
+    e = np.array(e)
+    assert np.all(np.isfinite(e)) and e[1] > 0
+    assert np.all(e[1:] <= 1.001 * e[1]), e / e[1]
+
+
 def test_pml_material_absorbs(oracle_lib):
     """A zero-mean pulse in a box whose PML (R = 1e-3) holds part of a lossy
     block: the fields stay bounded over 1200 steps and the link energy
