@@ -69,22 +69,62 @@ def measured_traffic(a, kernel, alg_bytes):
     return rec.get("traffic_bytes")
 
 
-def cpu_baseline(tables, steps):
-    """Oracle on the same grid for `steps` steps; per-step time from the
-    difference of two step counts removes set-up."""
+def host_cpu():
+    """CPU model and the cores this process may use (lscpu / affinity)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "usable_cores": usable, "os_cpu_count": os.cpu_count()}
+
+
+def _oracle_rate(tables, steps, threads):
+    """Gcell-updates/s of the oracle on `tables`: one untimed step (first
+    touch), then `steps` timed steps."""
     from oracle import oracle
 
-    h = oracle.handle(tables, output_capacity=2 * steps + 2)
-    h.step(1, 1)  # touch everything once
+    h = oracle.handle(tables, output_capacity=steps + 2, threads=threads)
+    h.step(1, 1)
     t0 = time.perf_counter()
     h.step(2, steps)
     t1 = time.perf_counter()
     h.close()
     N = tables.nx * tables.ny * tables.nz
-    rate = N * steps / (t1 - t0) / 1e9
-    return {"value": rate, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{tables.nx}x{tables.ny}x{tables.nz} grid of the same workload, {steps} steps "
-                      f"({t1 - t0:.1f} s), single-threaded C oracle (gcc -O2, FP64)"}
+    return N * steps / (t1 - t0) / 1e9, t1 - t0
+
+
+def cpu_baseline(tables, steps, steps_all=None):
+    """The CPU oracle (FP64 C restatement of the reference's loop, gcc -O2)
+    on the same grid: single-threaded (the reference's sequential build)
+    and over all usable cores as z-slabs (its MPI build's partition, as
+    OpenMP threads).  value is the all-core rate."""
+    from oracle import oracle
+
+    host = host_cpu()
+    threads = max(1, min(oracle.max_threads(), host["usable_cores"] or 1))
+    r1, t1 = _oracle_rate(tables, steps, 1)
+    steps_all = steps_all or max(steps, int(steps * min(threads, 8)))
+    rn, tn = _oracle_rate(tables, steps_all, threads) if threads > 1 else (r1, t1)
+    grid = f"{tables.nx}x{tables.ny}x{tables.nz}"
+    return {"value": rn, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{grid} grid of the same workload: {steps_all} steps over {threads} z-slab threads "
+                      f"({tn:.1f} s) and {steps} steps single-threaded ({t1:.1f} s); C oracle, gcc -O2, FP64",
+            "sequential": {"value": r1, "cores": 1},
+            "host": host}
+
+
+def dtype_of(form):
+    """Arithmetic and storage of a numerical form."""
+    return "f64" if form == capi.FORM_PLAIN_FP64 else "f32-links/f64-arith"
 
 
 def _build(problem):
@@ -152,7 +192,7 @@ def main_batch(a):
     line = {
         "metric": "solves/s", "value": round(value, 3), "unit": "solves/s", "n_gpus": 1,
         "steps": a.batch_steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": dtype_of(capi.FORM_PRECISION_FP32), "data": "synthetic",
         "config": {"workload": f"{len(tables)} shielding enclosures (PEC box, one aperture, 6-cell PML, "
                                f"Huygens plane wave, centre probe DFT at 100 f), {a.batch_steps} steps each",
                    "form": "precision: FP32 link planes", "graph_steps": k, "cells": int(sum(cells)),
@@ -204,7 +244,7 @@ def main_accuracy(a):
         print(f"[bench] accuracy {json.dumps(rec)}", file=sys.stderr, flush=True)
     line = {"metric": "relative L2, precision form vs FP64", "value": max(r["max_rel_l2_probes_above_5pct_of_peak"]
                                                                           for r in out),
-            "unit": "relative L2", "n_gpus": 1, "higher_is_better": False, "dtype": "f32 links / f64",
+            "unit": "relative L2", "n_gpus": 1, "higher_is_better": False, "dtype": dtype_of(capi.FORM_PRECISION_FP32),
             "data": "synthetic", "config": {"workloads": out}}
     print(json.dumps(line), flush=True)
 
@@ -213,23 +253,24 @@ def main_serve_batch(a):
     """The serving stream's requests (same seed) admitted at once and stepped
     to the end as one resident batch in the precision form: the GPU-bound
     part of the saturated stream, for kernel profiles (one graph capture per
-    chunk set, no arrivals)."""
+    chunk set, no arrivals).  The timed work counts each request's own
+    steps after the untimed first chunk."""
     rng = np.random.default_rng(a.seed)
     probs = [workloads.serving_request(rng) for _ in range(a.serve_requests)]
     tables = build_all(probs)
     lib = capi.engine_library()
-    b = capi.Batch(capi.FORM_PRECISION_FP32, 200 << 30, lib)
+    form = capi.FORM_PRECISION_FP32
+    b = capi.Batch(form, 200 << 30, lib)
     chunk = 512
     b.set_graph_steps(chunk)
     hs = []
     for t, p in zip(tables, probs):
-        h = capi.Handle(lib, t, capi.FORM_PRECISION_FP32, output_capacity=p.n_timesteps)
+        h = capi.Handle(lib, t, form, output_capacity=p.n_timesteps)
         h.release_tables()
         if not b.add(h, 1, p.n_timesteps):
             raise RuntimeError("the serving requests do not fit the batch budget")
         hs.append(h)
     steps = max(p.n_timesteps for p in probs)
-    cells = sum(t.nx * t.ny * t.nz * p.n_timesteps for t, p in zip(tables, probs))
     b.run(chunk)  # first chunk (graph capture) untimed
     b.synchronize()
     t0 = time.perf_counter()
@@ -247,19 +288,37 @@ def main_serve_batch(a):
         for h in got:
             h.close()
     b.close()
-    timed_cells = cells * (steps - chunk) / steps
+    timed_cells = sum(t.nx * t.ny * t.nz * max(0, p.n_timesteps - chunk) for t, p in zip(tables, probs))
     print(json.dumps({"metric": "Gcell-updates/s", "value": round(timed_cells / dt / 1e9, 3),
                       "unit": "Gcell-updates/s", "n_gpus": 1, "steps": steps - chunk, "higher_is_better": True,
-                      "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                      "scaling": "weak", "vs_baseline": None, "dtype": dtype_of(form), "data": "synthetic",
                       "config": {"workload": f"{len(tables)} serving requests (seed {a.seed}) stepped as one "
-                                             "resident batch", "form": "precision", "seconds": round(dt, 3)}}))
+                                             "resident batch", "form": "precision", "seconds": round(dt, 3),
+                                 "timed_cell_updates": int(timed_cells)}}))
+
+
+def request_errors(test, ref):
+    """Per request: the largest relative L2 over its probes of `test`
+    against `ref` (each probe by its own norm), and the same error over the
+    request's largest probe norm."""
+    own, peak = [], []
+    for a_, r in zip(test, ref):
+        so, sr = a_.outputs, r.outputs
+        norms = np.array([np.linalg.norm(sr[:, k]) for k in range(sr.shape[1])])
+        errs = np.array([np.linalg.norm(so[:, k] - sr[:, k]) for k in range(sr.shape[1])])
+        own.append(float(np.max(errs / np.where(norms > 0, norms, 1.0))))
+        peak.append(float(np.max(errs) / max(norms.max(), 1e-300)))
+    return own, peak
 
 
 def main_serve(a):
-    """Poisson stream of mixed requests served with continuous batching
-    (precision form) against the same stream served one request at a time
-    in the plain FP64 path; p50/p99 latency and solves/s for both, and the
-    largest probe-series relative L2 difference between the two."""
+    """Poisson stream of mixed requests served four ways on the same arrivals:
+    continuous batching in the precision form (the product), one request at
+    a time in plain FP64 (the north-star baseline), one at a time in the
+    precision form and batched in plain FP64 (same-form baselines that
+    separate the gain of batching from the gain of FP32 links).  p50/p99
+    latency and solves/s for each; per-request relative L2 of the batched
+    precision series against the one-at-a-time FP64 series."""
     from ggi_tlm_amd import serving
 
     rng = np.random.default_rng(a.seed)
@@ -269,29 +328,46 @@ def main_serve(a):
     print(f"[bench] serve: {len(tables)} requests, tables in {time.perf_counter() - t_setup:.1f} s",
           file=sys.stderr, flush=True)
     lib = capi.engine_library()
+    P32, P64 = capi.FORM_PRECISION_FP32, capi.FORM_PLAIN_FP64
     out = {}
     for rate in a.rates:
         arr = serving.poisson_arrivals(len(tables), rate, np.random.default_rng(a.seed + int(rate)))
-        reqs = [serving.Request(i, t, p.n_timesteps, float(arr[i])) for i, (t, p) in enumerate(zip(tables, probs))]
-        st_b = serving.run_batched(reqs, capi.FORM_PRECISION_FP32, chunk=a.chunk, lib=lib)
-        series_b = [r.outputs for r in reqs]
-        print(f"[bench] rate {rate}: batched {st_b}", file=sys.stderr, flush=True)
-        reqs_s = [serving.Request(i, t, p.n_timesteps, float(arr[i])) for i, (t, p) in enumerate(zip(tables, probs))]
-        st_s = serving.run_sequential(reqs_s, capi.FORM_PLAIN_FP64, lib)
-        print(f"[bench] rate {rate}: one at a time {st_s}", file=sys.stderr, flush=True)
-        err = 0.0
-        for r, sb in zip(reqs_s, series_b):
-            so = r.outputs
-            floor = 1e-3 * max(np.linalg.norm(so[:, k]) for k in range(so.shape[1]))
-            for k in range(so.shape[1]):
-                err = max(err, float(np.linalg.norm(sb[:, k] - so[:, k]) / max(np.linalg.norm(so[:, k]), floor)))
-        out[str(rate)] = {"batched_precision": st_b, "one_at_a_time_fp64": st_s,
-                          "speedup_solves_per_s": round(st_b["solves_per_s"] / st_s["solves_per_s"], 2),
-                          "max_probe_rel_l2_vs_fp64": err}
+
+        def stream():
+            return [serving.Request(i, t, p.n_timesteps, float(arr[i])) for i, (t, p) in
+                    enumerate(zip(tables, probs))]
+
+        runs = {}
+        for key, form, batched in (("batched_precision", P32, True), ("one_at_a_time_fp64", P64, False),
+                                   ("one_at_a_time_precision", P32, False), ("batched_fp64", P64, True)):
+            reqs = stream()
+            if batched:
+                st = serving.run_batched(reqs, form, chunk=a.chunk, lib=lib)
+            else:
+                st = serving.run_sequential(reqs, form, lib)
+            runs[key] = (st, reqs)
+            print(f"[bench] rate {rate}: {key} {st}", file=sys.stderr, flush=True)
+        ref = runs["one_at_a_time_fp64"][1]
+        own, peak = request_errors(runs["batched_precision"][1], ref)
+        same64 = all(np.array_equal(x.outputs, y.outputs) for x, y in zip(runs["batched_fp64"][1], ref))
+        sps = {k: v[0]["solves_per_s"] for k, v in runs.items()}
+        out[str(rate)] = {
+            **{k: v[0] for k, v in runs.items()},
+            "speedup_vs_one_at_a_time_fp64": round(sps["batched_precision"] / sps["one_at_a_time_fp64"], 2),
+            "speedup_same_form_precision": round(sps["batched_precision"] / sps["one_at_a_time_precision"], 2),
+            "speedup_same_form_fp64": round(sps["batched_fp64"] / sps["one_at_a_time_fp64"], 2),
+            "batched_fp64_bitwise_equal_to_one_at_a_time": bool(same64),
+            "precision_vs_fp64_rel_l2_per_request": {
+                "own_norm_max": float(f"{max(own):.3e}"), "own_norm_median": float(f"{np.median(own):.3e}"),
+                "requests_within_1e-5": int(sum(e <= 1e-5 for e in own)),
+                "peak_norm_max": float(f"{max(peak):.3e}"),
+                "requests_within_1e-5_of_peak": int(sum(e <= 1e-5 for e in peak)),
+                "own_norm": [float(f"{e:.2e}") for e in own]},
+        }
     top = out[str(max(a.rates))]
     line = {
         "metric": "solves/s", "value": top["batched_precision"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
-        "steps": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "steps": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype_of(P32),
         "data": "synthetic",
         "config": {"workload": f"{len(tables)} Poisson-arrival requests, cubic n log-U[24,160], steps U[1000,8000], "
                                "40% free / 35% sphere / 25% 2-pole blocks, 6-cell PML, 1-8 probes",
@@ -412,7 +488,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtype_of(form),
         "data": "synthetic",
         "config": {
             "workload": desc,

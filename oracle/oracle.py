@@ -23,14 +23,28 @@ def library():
     if _lib is None:
         _lib = Library(ORACLE_SO, "oracle_",
                        extra=[("zfilter_sequence", (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32,
-                                                              C.c_void_p, C.c_int32, C.c_void_p]))])
+                                                              C.c_void_p, C.c_int32, C.c_void_p])),
+                              ("set_threads", (C.c_int, [C.c_void_p, C.c_int32])),
+                              ("max_threads", (C.c_int, []))])
     return _lib
 
 
-def handle(tables, form=0, output_capacity=None):
+def handle(tables, form=0, output_capacity=None, threads=1):
+    """An oracle solve; threads > 1 runs scatter and connect over that many
+    z-slabs (identical results)."""
     from ggi_tlm_amd.capi import Handle
 
-    return Handle(library(), tables, form, output_capacity)
+    h = Handle(library(), tables, form, output_capacity)
+    if threads != 1:
+        rc = library().fn["set_threads"](h.h, int(threads))
+        if rc:
+            raise ValueError(f"oracle_set_threads({threads}) failed")
+    return h
+
+
+def max_threads() -> int:
+    """Threads the oracle's OpenMP runtime would use by default."""
+    return int(library().fn["max_threads"]())
 
 
 def build():

@@ -31,6 +31,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 /* SRC/CONSTANTS/constants.F90:27-39 */
 static const double PI = 3.141592653589793;
 static const double Z0 = 376.73031346177;
@@ -111,6 +115,12 @@ struct oracle_handle {
   double* dft; /* (point, freq, 2) */
   int cap, ring_count, ring_first;
   double* ring; /* cap blocks of n_out_rows*6 */
+  /* z-slab threading (the MPI build's z partition, general_mesh_subroutines.F90:868-972,
+   * as threads): the scan-order counters of scatter and connect at the start
+   * of each z-plane, so that planes can run in any order */
+  int threads;
+  int32_t *pl_out, *pl_exc;   /* output / excitation cell numbers before plane cz (nz+1) */
+  int32_t *pl_fexc, *pl_fout; /* excitation / output face numbers before plane cz (nz+1) */
 };
 
 /* PML_material_data_type words (PML_module.F90:131-156), without the
@@ -133,6 +143,7 @@ enum {
 };
 
 static int fail(oracle_handle* h, int code, const char* msg) {
+#pragma omp critical(oracle_error)
   snprintf(h->err, sizeof h->err, "%s", msg);
   return code;
 }
@@ -154,6 +165,7 @@ int oracle_create(const tlm_grid* g, int32_t form, oracle_handle** out) {
   h->V = (double*)calloc(n, sizeof(double));
   if (!h->V) { free(h); return TLM_ERR_NOMEM; }
   h->cap = 4096;
+  h->threads = 1;
   *out = h;
   return TLM_OK;
 }
@@ -169,6 +181,7 @@ void oracle_destroy(oracle_handle* h) {
   free(h->cell_exc_field); free(h->cell_exc_type); free(h->face_exc_field);
   free(h->face_exc_type); free(h->huy); free(h->cell_out); free(h->face_out); free(h->fring);
   free(h->pt_row); free(h->pt_comp); free(h->freq); free(h->dft); free(h->ring);
+  free(h->pl_out); free(h->pl_exc); free(h->pl_fexc); free(h->pl_fout);
   free(h);
 }
 
@@ -195,6 +208,8 @@ int oracle_set_cell_codes(oracle_handle* h, const int32_t* code, int32_t n_speci
     if (to_cable && to_cable[i] &&
         ((to_pml && to_pml[i]) || (to_excitation && to_excitation[i]) || (to_output && to_output[i])))
       return fail(h, TLM_ERR_UNSUPPORTED, "host-path cells carry no PML, excitation or output");
+  free(h->pl_out); free(h->pl_exc);
+  h->pl_out = h->pl_exc = NULL;
   h->cell_code = dup_i32(code, N);
   h->n_special = n_special;
   h->c_mat = dup_i32(to_material, n_special);
@@ -220,6 +235,8 @@ int oracle_set_face_codes(oracle_handle* h, const int32_t* code, int32_t n_speci
   size_t nf = (size_t)(nx - 1) * ny * nz + (size_t)nx * (ny - 1) * nz + (size_t)nx * ny * (nz - 1);
   for (int32_t i = 0; i < n_special; i++)
     if (to_cable && to_cable[i]) return fail(h, TLM_ERR_UNSUPPORTED, "cable faces stay on the host path");
+  free(h->pl_fexc); free(h->pl_fout);
+  h->pl_fexc = h->pl_fout = NULL;
   h->n_faces = (int)nf;
   h->face_code = dup_i32(code, nf);
   h->n_special_faces = n_special;
@@ -912,24 +929,53 @@ static void record_outputs(oracle_handle* h, int timestep) {
   }
 }
 
-int oracle_scatter(oracle_handle* h, int32_t timestep) {
+/* Scan-order counters of scatter() at the start of each z-plane: output
+ * cells (generic, PML and PML-material cells all number their rows) and
+ * excitation cells (counted by the Thevenin path only, scatter.F90:431). */
+static int cell_prefixes(oracle_handle* h) {
+  if (h->pl_out) return TLM_OK;
+  int nx = h->nx, ny = h->ny, nz = h->nz;
+  h->pl_out = (int32_t*)calloc((size_t)nz + 1, sizeof(int32_t));
+  h->pl_exc = (int32_t*)calloc((size_t)nz + 1, sizeof(int32_t));
+  if (!h->pl_out || !h->pl_exc) return fail(h, TLM_ERR_NOMEM, "plane counters");
+  int no = 0, ne = 0;
+  size_t c = 0;
+  for (int cz = 1; cz <= nz; cz++) {
+    h->pl_out[cz - 1] = no;
+    h->pl_exc[cz - 1] = ne;
+    for (int k = 0; k < nx * ny; k++, c++) {
+      int code = h->cell_code[c];
+      if (!code || h->c_cable[code - 1]) continue;
+      no += h->c_out[code - 1] != 0;
+      ne += h->c_pml[code - 1] == 0 && h->c_exc[code - 1] != 0;
+    }
+  }
+  h->pl_out[nz] = no;
+  h->pl_exc[nz] = ne;
+  return TLM_OK;
+}
+
+/* scatter() over the cells of z-plane cz (scatter.F90:216-585 restricted to
+ * one plane); the planes are independent, a plane's counters start from
+ * cell_prefixes. */
+static int scatter_plane(oracle_handle* h, int cz, int32_t timestep) {
   const double dl = h->g.dl;
   const double Z04 = Z0 * 4.0, Y04 = Y0 * 4.0;
-  int nx = h->nx, ny = h->ny, nz = h->nz;
-  int cell_number = 0, output_cell_number = 0, excitation_cell_number = 0;
-  if (!h->cell_code) return fail(h, TLM_ERR_STATE, "cell codes not set");
+  int nx = h->nx, ny = h->ny;
+  int cell_number = (cz - 1) * nx * ny;
+  int output_cell_number = h->pl_out[cz - 1], excitation_cell_number = h->pl_exc[cz - 1];
   /* TLM_set_small_to_zero.F90:34-68 at the top of a step whose number is a
    * multiple of n_small_to_zero (run_TLM.F90:236-238); excitation() does
-   * not touch V, so doing it here, before the scatter, is the same */
+   * not touch V, and the flush and the scatter are cell-local, so flushing
+   * this plane just before its scatter is the same */
   if (h->stz_n > 0 && timestep % h->stz_n == 0)
-    for (int cz = 1; cz <= nz; cz++)
-      for (int cy = 1; cy <= ny; cy++)
-        for (int cx = 1; cx <= nx; cx++) {
-          double* v = cellV(h, cx, cy, cz);
-          for (int i = 1; i <= 12; i++)
-            if (fabs(v[i]) < h->stz_value) v[i] = 0.0;
-        }
-  for (int cz = 1; cz <= nz; cz++)
+    for (int cy = 1; cy <= ny; cy++)
+      for (int cx = 1; cx <= nx; cx++) {
+        double* v = cellV(h, cx, cy, cz);
+        for (int i = 1; i <= 12; i++)
+          if (fabs(v[i]) < h->stz_value) v[i] = 0.0;
+      }
+  {
     for (int cy = 1; cy <= ny; cy++)
       for (int cx = 1; cx <= nx; cx++) {
         cell_number++;
@@ -957,8 +1003,7 @@ int oracle_scatter(oracle_handle* h, int32_t timestep) {
           if (material_type == TLM_VMAT_PEC || material_type == TLM_VMAT_PMC)
             return fail(h, TLM_ERR_ARG, "PEC/PMC volume material in PML");
           if (material_type == TLM_VMAT_DISPERSIVE) {
-            int rc = setup_pml_materials(h);
-            if (rc) return rc;
+            /* PML material cells were numbered before the planes ran */
             scatter_pml_material_cell(h, v, PML_cell, material_number, output_number, &output_cell_number);
             continue;
           }
@@ -1068,6 +1113,27 @@ int oracle_scatter(oracle_handle* h, int32_t timestep) {
         }
         scatter_ports(v, Vx, Vy, Vz, Ix, Iy, Iz);
       }
+  }
+  return TLM_OK;
+}
+
+int oracle_scatter(oracle_handle* h, int32_t timestep) {
+  if (!h->cell_code) return fail(h, TLM_ERR_STATE, "cell codes not set");
+  int rc;
+  if ((rc = cell_prefixes(h))) return rc;
+  /* allocate_PML_material_data before the planes run (it numbers the PML
+   * material cells in scan order) */
+  if (h->n_pml_cells > 0 && (rc = setup_pml_materials(h))) return rc;
+  int err = TLM_OK;
+#pragma omp parallel for schedule(static) num_threads(h->threads)
+  for (int cz = 1; cz <= h->nz; cz++) {
+    int r = scatter_plane(h, cz, timestep);
+    if (r) {
+#pragma omp atomic write
+      err = r;
+    }
+  }
+  if (err) return err;
   record_outputs(h, timestep);
   return TLM_OK;
 }
@@ -1177,11 +1243,46 @@ static int connect_special_face(oracle_handle* h, int axis, double* vmin_cell, d
   return TLM_OK;
 }
 
-int oracle_connect(oracle_handle* h, int32_t timestep) {
-  int nx = h->nx, ny = h->ny, nz = h->nz;
-  int face_number = 0, excitation_face_number = 0, output_face_number = 0;
-  if (!h->face_code) return fail(h, TLM_ERR_STATE, "face codes not set");
-  for (int cz = 1; cz <= nz; cz++)
+/* Faces of z-plane cz in connect()'s scan order: the x-face (cx != 1), the
+ * y-face (cy != 1) and the z-face (cz != 1) of every cell of the plane. */
+static long long faces_before_plane(const oracle_handle* h, int cz) {
+  const long long nx = h->nx, ny = h->ny;
+  const long long fxy = (nx - 1) * ny + nx * (ny - 1);
+  return (long long)(cz - 1) * fxy + (cz > 1 ? (long long)(cz - 2) * nx * ny : 0);
+}
+
+/* Scan-order excitation and output face numbers at the start of each
+ * z-plane (connect_special_face counts every special face carrying them). */
+static int face_prefixes(oracle_handle* h) {
+  if (h->pl_fexc) return TLM_OK;
+  int nz = h->nz;
+  h->pl_fexc = (int32_t*)calloc((size_t)nz + 1, sizeof(int32_t));
+  h->pl_fout = (int32_t*)calloc((size_t)nz + 1, sizeof(int32_t));
+  if (!h->pl_fexc || !h->pl_fout) return fail(h, TLM_ERR_NOMEM, "plane counters");
+  int ne = 0, no = 0;
+  for (int cz = 1; cz <= nz; cz++) {
+    h->pl_fexc[cz - 1] = ne;
+    h->pl_fout[cz - 1] = no;
+    for (long long f = faces_before_plane(h, cz); f < faces_before_plane(h, cz + 1); f++) {
+      int code = h->face_code[f];
+      if (!code) continue;
+      ne += h->f_exc[code - 1] != 0;
+      no += h->f_out[code - 1] != 0;
+    }
+  }
+  h->pl_fexc[nz] = ne;
+  h->pl_fout[nz] = no;
+  return TLM_OK;
+}
+
+/* connect() over the faces of z-plane cz (connect.F90:180-214): the x- and
+ * y-faces inside the plane and the z-faces to plane cz-1.  A face touches
+ * only its own two port pairs, so planes can run concurrently. */
+static int connect_plane(oracle_handle* h, int cz) {
+  int nx = h->nx, ny = h->ny;
+  long long face_number = faces_before_plane(h, cz);
+  int excitation_face_number = h->pl_fexc[cz - 1], output_face_number = h->pl_fout[cz - 1];
+  {
     for (int cy = 1; cy <= ny; cy++)
       for (int cx = 1; cx <= nx; cx++) {
         double* v = cellV(h, cx, cy, cz);
@@ -1234,6 +1335,24 @@ int oracle_connect(oracle_handle* h, int32_t timestep) {
           }
         }
       }
+  }
+  return TLM_OK;
+}
+
+int oracle_connect(oracle_handle* h, int32_t timestep) {
+  if (!h->face_code) return fail(h, TLM_ERR_STATE, "face codes not set");
+  int rc;
+  if ((rc = face_prefixes(h))) return rc;
+  int err = TLM_OK;
+#pragma omp parallel for schedule(static) num_threads(h->threads)
+  for (int cz = 1; cz <= h->nz; cz++) {
+    int r = connect_plane(h, cz);
+    if (r) {
+#pragma omp atomic write
+      err = r;
+    }
+  }
+  if (err) return err;
   /* face_output_points (output.F90:232-272): this connect's rows, one block
    * per step, kept for reading */
   if (h->n_face_out_rows) {
@@ -1300,6 +1419,23 @@ int oracle_step(oracle_handle* h, int32_t first_timestep, int32_t n_steps) {
 }
 
 int oracle_synchronize(oracle_handle* h) { (void)h; return TLM_OK; }
+
+/* Threads over z-slabs for scatter and connect (1: the sequential build).
+ * Every cell and face is computed by the same operations whatever the
+ * count, so the results are identical. */
+int oracle_set_threads(oracle_handle* h, int32_t threads) {
+  if (!h || threads < 1) return TLM_ERR_ARG;
+  h->threads = threads;
+  return TLM_OK;
+}
+
+int oracle_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 int oracle_set_output_capacity(oracle_handle* h, int32_t steps) {
   if (steps < 1) return fail(h, TLM_ERR_ARG, "capacity must be positive");

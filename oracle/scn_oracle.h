@@ -79,6 +79,9 @@ int oracle_kernel_bytes(oracle_handle* h, int64_t* cells, int64_t* bytes);
  * (row, side, component); n_rows = total_number_output_faces. */
 int oracle_read_face_outputs(oracle_handle* h, double* out, int32_t max_steps, int32_t* n_read,
                              int32_t* first_step);
+/* z-slab threads of scatter and connect; results do not depend on them */
+int oracle_set_threads(oracle_handle* h, int32_t threads);
+int oracle_max_threads(void);
 int oracle_zfilter_sequence(const double* a, const double* b, int32_t order, const double* G,
                             int32_t n, double* out);
 
