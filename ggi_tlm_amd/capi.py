@@ -52,6 +52,7 @@ SIGNATURES = {
     "set_dft_frequencies": (C.c_int, [_h, _i32, _p]),
     "upload_links": (C.c_int, [_h, _p]),
     "download_links": (C.c_int, [_h, _p]),
+    "download_links_noalloc": (C.c_int, [_h, _p]),
     "excitation": (C.c_int, [_h, _i32]),
     "scatter": (C.c_int, [_h, _i32]),
     "connect": (C.c_int, [_h, _i32]),
@@ -251,6 +252,15 @@ class Handle:
         if V is None:
             V = np.zeros((12, t.nx + 2, t.ny + 2, t.nz), dtype=np.float64, order="F")
         self._check(self.lib.fn["download_links"](self.h, _ptr(V)))
+        return V
+
+    def download_links_noalloc(self, V=None):
+        """download_links through the library's fixed staging buffer (no
+        device allocation); see tlm_download_links_noalloc."""
+        t = self.t
+        if V is None:
+            V = np.zeros((12, t.nx + 2, t.ny + 2, t.nz), dtype=np.float64, order="F")
+        self._check(self.lib.fn["download_links_noalloc"](self.h, _ptr(V)))
         return V
 
     def read_cell_outputs(self, max_steps=None):

@@ -45,6 +45,7 @@ struct tlm_handle {
   std::string err;
   int nx = 0, ny = 0, nz = 0;
   long long N = 0, P = 0;
+  int device = 0;  // the GPU current at tlm_create, which holds the handle's memory
   hipStream_t stream = nullptr;
 
   // host tables (reference layout)

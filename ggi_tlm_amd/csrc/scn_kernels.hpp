@@ -2044,4 +2044,21 @@ __global__ void k_planes_to_aos(Dev d, const T* __restrict__ L, double* __restri
   for (int p = 0; p < 12; p++) V[fi + p] = (double)L[p * d.P + c];
 }
 
+// k_planes_to_aos for the z-planes [z0, z1) only, into a slab array laid
+// out as V(12, 0:nx+1, 0:ny+1, z0:z1-1); one lane per cell of the slab.
+template <typename T>
+__global__ void k_planes_to_aos_slab(Dev d, const T* __restrict__ L, double* __restrict__ V, int z0, int z1) {
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long plane = (long long)d.nx * d.ny;
+  if (k >= plane * (z1 - z0)) return;
+  const int x = (int)(k % d.nx);
+  const int y = (int)((k / d.nx) % d.ny);
+  const int zl = (int)(k / plane);
+  int s;
+  const long long c = storage_index(d, x, y, z0 + zl, s);
+  const long long fi =
+      12 * ((long long)(x + 1) + (long long)(d.nx + 2) * ((long long)(y + 1) + (long long)(d.ny + 2) * zl));
+  for (int p = 0; p < 12; p++) V[fi + p] = (double)L[p * d.P + c];
+}
+
 }  // namespace scn

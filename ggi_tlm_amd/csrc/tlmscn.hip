@@ -12,6 +12,8 @@
 // and outer_boundary are recorded as pending and folded into the next
 // scatter's pull.  A download materialises the pending connect first, so
 // the caller always sees exactly the reference's V.
+#include <mutex>
+
 #include "engine.hpp"
 
 using namespace engine;
@@ -1075,6 +1077,39 @@ int materialize(tlm_handle* h) {
 // C ABI
 // ===========================================================================
 
+// Staging buffer of tlm_download_links_noalloc: a file-scope device array,
+// one copy per GPU for the life of the process.  Host code never passes the
+// array's name to a kernel or a copy (on the host it only stands for the
+// symbol); it uses the per-device address from hipGetSymbolAddress.
+constexpr size_t kStagingDoubles = TLM_NOALLOC_STAGING_DOUBLES;
+static __device__ __align__(256) double scratch[kStagingDoubles];
+
+namespace {
+constexpr int kMaxStagingDevices = 64;
+std::mutex g_staging_mu;
+double* g_staging[kMaxStagingDevices] = {};
+
+// The staging buffer's device address on the handle's GPU, looked up once
+// per device with that device current.
+int staging_buffer(tlm_handle* h, double** out) {
+  if (h->device < 0 || h->device >= kMaxStagingDevices)
+    return fail(h, TLM_ERR_UNSUPPORTED, "device number beyond the staging table");
+  std::lock_guard<std::mutex> lock(g_staging_mu);
+  if (!g_staging[h->device]) {
+    int prev = -1;
+    HIP_TRY(h, hipGetDevice(&prev));
+    if (prev != h->device) HIP_TRY(h, hipSetDevice(h->device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (prev != h->device) (void)hipSetDevice(prev);
+    if (e != hipSuccess) return hip_fail(h, e, "hipGetSymbolAddress(staging buffer)");
+    g_staging[h->device] = (double*)p;
+  }
+  *out = g_staging[h->device];
+  return TLM_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int tlm_create(const tlm_grid* g, int32_t form, tlm_handle** out) {
@@ -1092,7 +1127,8 @@ int tlm_create(const tlm_grid* g, int32_t form, tlm_handle** out) {
   h->nx = g->nx; h->ny = g->ny; h->nz = g->nz;
   h->N = (long long)g->nx * g->ny * g->nz;
   h->P = (h->N + 63) / 64 * 64;
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipGetDevice(&h->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return TLM_ERR_HIP;
   }
@@ -1314,6 +1350,42 @@ int tlm_download_links(tlm_handle* h, double* V) {
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return hip_fail(h, e, "download_links");
+  return TLM_OK;
+}
+
+int tlm_download_links_noalloc(tlm_handle* h, double* V) {
+  if (!h || !V) return TLM_ERR_ARG;
+  if (!h->finalized)
+    return fail(h, TLM_ERR_STATE, "the allocation-free download needs a handle whose tables are set up "
+                                  "(step, upload or tlm_host_cells first)");
+  const size_t plane = (size_t)12 * (h->nx + 2) * (h->ny + 2);
+  if (plane > kStagingDoubles)
+    return fail(h, TLM_ERR_ARG, "one z-plane of the link array (" + std::to_string(plane) +
+                                    " doubles) exceeds the allocation-free staging buffer (" +
+                                    std::to_string(kStagingDoubles) + " doubles); use tlm_download_links");
+  int rc;
+  if ((rc = materialize(h))) return rc;  // a pending connect, on the existing buffers
+  double* st = nullptr;
+  if ((rc = staging_buffer(h, &st))) return rc;
+  const int zc = (int)std::min<size_t>((size_t)h->nz, kStagingDoubles / plane);
+  for (int z0 = 0; z0 < h->nz; z0 += zc) {
+    const int z1 = std::min(h->nz, z0 + zc);
+    const size_t n = plane * (size_t)(z1 - z0);
+    // the caller's halo is kept: its slab goes in first, the kernel
+    // overwrites the interior cells
+    HIP_TRY(h, hipMemcpyAsync(st, V + plane * z0, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    const long long cells = (long long)h->nx * h->ny * (z1 - z0);
+    const dim3 g((unsigned)((cells + 255) / 256));
+    if (h->form == TLM_FORM_PLAIN_FP64)
+      hipLaunchKernelGGL(scn::k_planes_to_aos_slab<double>, g, dim3(256), 0, h->stream, h->d,
+                         (const double*)h->planes[h->cur], st, z0, z1);
+    else
+      hipLaunchKernelGGL(scn::k_planes_to_aos_slab<float>, g, dim3(256), 0, h->stream, h->d,
+                         (const float*)h->planes[h->cur], st, z0, z1);
+    HIP_TRY(h, hipGetLastError());
+    HIP_TRY(h, hipMemcpyAsync(V + plane * z0, st, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  }
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
   return TLM_OK;
 }
 

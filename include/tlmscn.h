@@ -206,6 +206,18 @@ int tlm_set_dft_frequencies(tlm_handle* h, int32_t n, const double* freq);
 int tlm_upload_links(tlm_handle* h, const double* V);
 int tlm_download_links(tlm_handle* h, double* V);
 
+/* tlm_download_links without any device allocation.  It has no counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The array is staged through a fixed device buffer of
+ * TLM_NOALLOC_STAGING_DOUBLES doubles (32 MiB) one z-slab at a time; a
+ * z-plane larger than that (12*(nx+2)*(ny+2) doubles) is refused with
+ * TLM_ERR_ARG.  The handle must be set up already (tables uploaded by a
+ * step, an upload or tlm_host_cells).  Every call on a GPU shares that GPU's
+ * staging buffer: calls that overlap on one GPU race. */
+#define TLM_NOALLOC_STAGING_DOUBLES (4 * 1024 * 1024)
+int tlm_download_links_noalloc(tlm_handle* h, double* V);
+
 /* ---- stepping (reference call order) ------------------------------------- */
 int tlm_excitation(tlm_handle* h, int32_t timestep);
 int tlm_scatter(tlm_handle* h, int32_t timestep);

@@ -411,6 +411,10 @@ int oracle_download_links(oracle_handle* h, double* V) {
   return TLM_OK;
 }
 
+/* The oracle holds no device memory: the allocation-free download is the
+ * download. */
+int oracle_download_links_noalloc(oracle_handle* h, double* V) { return oracle_download_links(h, V); }
+
 /* ---- excitation ---------------------------------------------------------- */
 
 /* get_interpolated_excitation_value, EXCITATION/excitation.F90:645-690.

@@ -49,6 +49,7 @@ int oracle_set_output_points(oracle_handle* h, int32_t n, const int32_t* row,
 int oracle_set_dft_frequencies(oracle_handle* h, int32_t n, const double* freq);
 int oracle_upload_links(oracle_handle* h, const double* V);
 int oracle_download_links(oracle_handle* h, double* V);
+int oracle_download_links_noalloc(oracle_handle* h, double* V);
 int oracle_excitation(oracle_handle* h, int32_t timestep);
 int oracle_scatter(oracle_handle* h, int32_t timestep);
 int oracle_connect(oracle_handle* h, int32_t timestep);
