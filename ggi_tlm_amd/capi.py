@@ -76,6 +76,9 @@ SIGNATURES = {
     "host_cells": (C.c_int, [_h, C.POINTER(_i32), _p]),
     "gather_host_cells": (C.c_int, [_h, _p]),
     "scatter_host_cells": (C.c_int, [_h, _p]),
+    "host_faces": (C.c_int, [_h, C.POINTER(_i32), _p]),
+    "gather_host_faces": (C.c_int, [_h, _p]),
+    "scatter_host_faces": (C.c_int, [_h, _p]),
 }
 
 LAYOUT_X_SEGMENTS = 0
@@ -328,6 +331,31 @@ class Handle:
         """Write the host's scattered voltages (n_host, 12) of the host-path cells back."""
         V = np.ascontiguousarray(V, dtype=np.float64)
         self._check(self.lib.fn["scatter_host_cells"](self.h, _ptr(V)))
+
+    def host_faces(self) -> np.ndarray:
+        """Host-path faces (thin layer, diode, spice, switch), (n, 4): the
+        1-based (x, y, z) of the cell on the face's + side and the axis 1..3
+        of its normal, in the connect's scan order."""
+        n = C.c_int32(0)
+        self._check(self.lib.fn["host_faces"](self.h, C.byref(n), None))
+        faces = np.zeros((n.value, 4), dtype=np.int32)
+        if n.value:
+            self._check(self.lib.fn["host_faces"](self.h, C.byref(n), _ptr(faces)))
+        return faces
+
+    def gather_host_faces(self, n_faces: int) -> np.ndarray:
+        """Scattered pulses (n, 4) beside each host-path face: a on the + side,
+        a on the - side, b on the + side, b on the - side; between scatter and
+        connect."""
+        V = np.zeros((n_faces, 4), dtype=np.float64)
+        self._check(self.lib.fn["gather_host_faces"](self.h, _ptr(V)))
+        return V
+
+    def scatter_host_faces(self, T: np.ndarray):
+        """The host model's face voltages (n, 4) = (Va_min, Va_max, Vb_min,
+        Vb_max) of each host-path face, for the coming connect."""
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        self._check(self.lib.fn["scatter_host_faces"](self.h, _ptr(T)))
 
     def set_streams(self, streams: int):
         """1: a step's kernels in turn on one stream; 2: class kernels concurrent."""

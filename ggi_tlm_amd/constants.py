@@ -36,6 +36,11 @@ FACES = {
 # volume / surface material types (SRC/TLM_MODULES/TLM_material_modules.F90:72-169)
 VMAT_PEC, VMAT_PMC, VMAT_DISPERSIVE = 1, 2, 3
 SMAT_PEC, SMAT_PMC, SMAT_FREE_SPACE = 1, 2, 4
+# host-path surface materials (their face models stay on the host)
+SMAT_DISPERSIVE, SMAT_ANISOTROPIC_DISPERSIVE, SMAT_DIODE, SMAT_SPICE, SMAT_SWITCH = 3, 5, 6, 7, 9
+SURFACE_MATERIALS = {"pec": SMAT_PEC, "pmc": SMAT_PMC, "free_space": SMAT_FREE_SPACE,
+                     "thin_layer": SMAT_DISPERSIVE, "anisotropic_thin_layer": SMAT_ANISOTROPIC_DISPERSIVE,
+                     "diode": SMAT_DIODE, "spice": SMAT_SPICE, "switch": SMAT_SWITCH}
 
 SOURCE_HARD, SOURCE_SOFT = 0, 1
 

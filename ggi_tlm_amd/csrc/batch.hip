@@ -544,6 +544,7 @@ int tlm_batch_add(tlm_batch* b, tlm_handle* h, int32_t first_timestep, int32_t n
   if ((rc = finalize(h))) return bfail(b, rc, std::string("finalize: ") + h->err);
   if (h->d.n_face_out_rows) return bfail(b, TLM_ERR_UNSUPPORTED, "face outputs are read on the single-solve path");
   if (h->n_host) return bfail(b, TLM_ERR_UNSUPPORTED, "host-path cells are exchanged on the single-solve path");
+  if (h->n_host_faces) return bfail(b, TLM_ERR_UNSUPPORTED, "host-path faces are exchanged on the single-solve path");
   tlm_batch::Entry e;
   e.h = h;
   e.first = first_timestep;

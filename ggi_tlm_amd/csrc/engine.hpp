@@ -96,6 +96,8 @@ struct tlm_handle {
   // pure bulk line, in storage order (XCD mappings 0, 1) and interleaved by
   // y-band (mapping 2; -1 entries idle)
   std::vector<int32_t> bulk_blocks;
+  const int4* d_bulk_chunks = nullptr;   // k_bulk_chunks records
+  int n_bulk_chunks = 0;
   const int32_t* d_bulk_blocks = nullptr;
   const int32_t* d_bulk_blocks_yband = nullptr;
   int n_bulk_blocks_yband = 0;
@@ -109,6 +111,13 @@ struct tlm_handle {
   const int32_t* d_host_cells = nullptr;
   std::vector<int32_t> host_xyz;
   double* d_host_buf = nullptr;   // 12 * n_host staging words
+  // host-path faces (thin layer, diode, spice and switch surface materials)
+  int n_host_faces = 0;
+  std::vector<int32_t> host_face_xyz;  // 4 per face: cx, cy, cz (+ side cell), axis (1-based)
+  const int32_t *d_hf_cp = nullptr, *d_hf_cm = nullptr, *d_hf_ax = nullptr;
+  double* d_hf_tot = nullptr;     // 4 face voltages per host face
+  double* d_hf_buf = nullptr;     // 4 gathered pulses per host face
+  bool hf_ready = false;          // the host wrote this step's face voltages
   int n_lds_var[2] = {0, 0};              // LDS workgroups per class-mask variant, in table order
   const int32_t* d_slot_cell = nullptr;  // mixed-line workgroup slots
   const int32_t* d_slot_idx = nullptr;

@@ -181,6 +181,8 @@ struct Dev {
   const int32_t* sf_mtype;   // per special face code (1-based index): surface material type
   const int32_t* sf_exc;     // excitation row (0-based) or -1
   const int32_t* sf_out;     // face output row (0-based) or -1
+  const int32_t* sf_host;    // host-path face number (0-based) or -1
+  const double* host_tot;    // 4 face voltages per host-path face, written by the host
   double* face_ring;         // slots of n_face_out_rows * 2 sides * 6 fields
   int n_face_out_rows;
   // excitation
@@ -366,9 +368,16 @@ __device__ inline void special_face(const Dev& d, int code, int axis, bool is_mi
   } else if (mtype == TLM_SMAT_PMC) {
     a_min = 2.0 * min_a; a_max = 2.0 * max_a;
     b_min = 2.0 * min_b; b_max = 2.0 * max_b;
-  } else {  // none or FREE_SPACE
+  } else if (mtype == 0 || mtype == TLM_SMAT_FREE_SPACE) {
     a_min = min_a + max_a; a_max = a_min;
     b_min = min_b + max_b; b_max = b_min;
+  } else {
+    // host-path surface material (thin layer, diode, spice port, switch):
+    // the face voltages its host model computed from this face's scattered
+    // pulses (connect_normal_to_x.F90:75-283)
+    const double* T = d.host_tot + 4 * (long long)d.sf_host[code];
+    a_min = T[0]; a_max = T[1];
+    b_min = T[2]; b_max = T[3];
   }
   int row = huy ? 0 : d.sf_exc[code];
   if (row >= 0) {
@@ -1720,20 +1729,36 @@ __device__ inline void bulk_line_cell(const Dev& d, const T* __restrict__ Lin, T
   bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z, sg);
 }
 
-// Workgroup g updates storage cells [kLineWg g, kLineWg g + kLineWg) (mode
-// 1: each XCD takes a contiguous range of them).
-// Cells (lanes) per workgroup of the direct line kernel: one wavefront per
-// workgroup measured 4.5% faster than 256 cells on the 256^3 workload
-// (0.238 against 0.249 ms; 128 cells: 0.252 ms).
-#ifndef TLM_LINE_WG
-#define TLM_LINE_WG 64
-#endif
-constexpr int kLineWg = TLM_LINE_WG;
+// Pure bulk 64-cell storage chunks, one wavefront each.  The chunk table
+// holds, per chunk with at least one pure bulk line, a record that every
+// lane of the wavefront reads at the same address (a scalar load):
+//   x = chunk << 4 | mask of its pure bulk lines (bit q: lanes q*L..q*L+L-1)
+//   y = x of its first cell | segment << 16,  z = y,  w = z of that cell.
+// A lane's coordinates follow from the first cell's by at most one row wrap
+// in a segment at least 64 cells wide; no per-cell table word or division
+// precedes the link loads.  (Reading each line's class from the line table
+// before the loads measured 7% slower: 0.240 against 0.224 ms on the 256^3
+// workload.)  mode 1: each XCD takes a contiguous range of table records.
+constexpr int kChunk = 64;
 template <typename T, bool PLAIN, bool PULL>
-__global__ __launch_bounds__(kLineWg) void k_bulk_lines(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                   PullArgs a, int mode) {
-  const int b = xcd_block(blockIdx.x, gridDim.x, mode);
-  bulk_line_cell<T, PLAIN, PULL>(d, Lin, Lout, a, (long long)b * kLineWg + threadIdx.x);
+__global__ __launch_bounds__(kChunk) void k_bulk_chunks(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                        PullArgs a, const int4* __restrict__ tab, int mode) {
+  const int4 e = tab[xcd_block(blockIdx.x, gridDim.x, mode)];
+  const int lane = threadIdx.x;
+  if (!((e.x >> (lane / d.line_len)) & 1)) return;
+  const long long c = (long long)(e.x >> 4) * kChunk + lane;
+  int sg = e.y >> 16, x = (e.y & 0xffff) + lane, y = e.z, z = e.w;
+  const int w = seg_width(d, sg);
+  if (w >= kChunk) {
+    if (x >= d.seg_x0[sg + 1]) {
+      x -= w;
+      if (++y == d.ny) { y = 0; z++; }
+    }
+    if (z >= d.nz) return;  // padding after the segment's last row
+  } else if (!cell_coords(d, c, x, y, z, sg)) {
+    return;
+  }
+  bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z, sg);
 }
 
 // blocks[0..n_blocks) lists the storage blocks to update (-1: none); a
@@ -2003,6 +2028,25 @@ __global__ void k_scatter_host(T* __restrict__ L, long long P, const int32_t* __
   if (k >= n) return;
   const long long c = cells[k];
   for (int p = 0; p < 12; p++) L[p * P + c] = (T)buf[12 * (long long)k + p];
+}
+
+// The four scattered pulses beside each host-path face (tlm_gather_host_faces):
+// (a on the + side cell cp, a on the - side cell cm, b on cp, b on cm).
+template <typename T>
+__global__ void k_gather_host_faces(const T* __restrict__ L, long long P, const int32_t* __restrict__ cp,
+                                    const int32_t* __restrict__ cm, const int32_t* __restrict__ ax, int n,
+                                    double* __restrict__ buf) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int a = ax[k];
+  const int pa = a == 0 ? Vy_xmin : a == 1 ? Vx_ymin : Vx_zmin;
+  const int pb = a == 0 ? Vz_xmin : a == 1 ? Vz_ymin : Vy_zmin;
+  const int qa = a == 0 ? Vy_xmax : a == 1 ? Vx_ymax : Vx_zmax;
+  const int qb = a == 0 ? Vz_xmax : a == 1 ? Vz_ymax : Vy_zmax;
+  buf[4 * (long long)k + 0] = (double)L[pa * P + cp[k]];
+  buf[4 * (long long)k + 1] = (double)L[qa * P + cm[k]];
+  buf[4 * (long long)k + 2] = (double)L[pb * P + cp[k]];
+  buf[4 * (long long)k + 3] = (double)L[qb * P + cm[k]];
 }
 
 // Probe DFT: acc(p,k) += v_p(n) exp(-j 2 pi f_k t_n) dt, t_n = (n-1) dt.

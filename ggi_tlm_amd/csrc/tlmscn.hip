@@ -54,6 +54,17 @@ int resolve_profile(tlm_handle* h) {
 }
 
 // Build every device table from the reference tables.
+// Surface material types whose face model stays on the host.
+static bool host_smat(int type) {
+  return type == TLM_SMAT_DISPERSIVE || type == TLM_SMAT_ANISOTROPIC_DISPERSIVE || type == TLM_SMAT_DIODE ||
+         type == TLM_SMAT_SPICE || type == TLM_SMAT_SWITCH;
+}
+
+// The chunk records keep the chunk number in 27 bits.
+static bool k_max_chunks_exceeded(long long storage_cells) {
+  return (storage_cells + scn::kChunk - 1) / scn::kChunk >= (1LL << 27);
+}
+
 int finalize(tlm_handle* h) {
   if (h->finalized) return TLM_OK;
   if (!h->cells_set) return fail(h, TLM_ERR_STATE, "tlm_set_cell_codes has not been called");
@@ -64,7 +75,7 @@ int finalize(tlm_handle* h) {
 
   // ---- special faces: locate each special face code in scan order -------
   // face_info[code] = (cell index of the face's min-side cell cx, axis)
-  std::vector<long long> sf_cell(h->n_special_faces + 1, -1);
+  std::vector<long long> sf_cell(h->n_special_faces + 1, -1), sf_fnum(h->n_special_faces + 1, -1);
   std::vector<int> sf_axis(h->n_special_faces + 1, -1);
   if (h->n_special_faces > 0) {
     long long fnum = 0;
@@ -81,6 +92,7 @@ int finalize(tlm_handle* h) {
               if (code < 1 || code > h->n_special_faces) return fail(h, TLM_ERR_ARG, "face code out of range");
               sf_cell[code] = c;
               sf_axis[code] = ax;
+              sf_fnum[code] = fnum - 1;
               found++;
             }
           }
@@ -106,6 +118,12 @@ int finalize(tlm_handle* h) {
     }
     if (er != h->n_face_rows) return fail(h, TLM_ERR_ARG, "face excitation rows do not match the face codes");
   }
+  // host-path faces in the connect's scan order
+  std::vector<int32_t> sf_host(h->n_special_faces + 1, -1), hf_codes;
+  for (int code = 1; code <= h->n_special_faces; code++)
+    if (host_smat(sf_mtype[code])) hf_codes.push_back(code);
+  std::sort(hf_codes.begin(), hf_codes.end(), [&](int a, int b) { return sf_fnum[a] < sf_fnum[b]; });
+  for (size_t k = 0; k < hf_codes.size(); k++) sf_host[hf_codes[k]] = (int32_t)k;
   // descriptors: per special cell, the special-face codes of its 6 faces
   // (xmin, xmax, ymin, ymax, zmin, zmax).  Key: cell index.
   struct FdEntry { long long cell; int side; int code; };
@@ -575,6 +593,27 @@ int finalize(tlm_handle* h) {
   d.line_len = L;
   if ((rc = dev_upload(h, line_info, &d.line_info))) return rc;
   {
+    // k_bulk_chunks records: 64-cell chunks holding a pure bulk line
+    std::vector<int4> ch;
+    const int lpc = scn::kChunk / L;
+    for (long long k = 0; k < (S + scn::kChunk - 1) / scn::kChunk; k++) {
+      int m = 0;
+      for (int q = 0; q < lpc; q++)
+        if (k * lpc + q < n_lines && line_info[k * lpc + q] == 0) m |= 1 << q;
+      if (!m) continue;
+      const long long c0 = k * scn::kChunk;
+      int sg = 0;
+      while (sg + 1 < nseg && c0 >= seg_base[sg + 1]) sg++;
+      const long long w = seg_x0[sg + 1] - seg_x0[sg];
+      const long long r = c0 - seg_base[sg], row = r / w;
+      ch.push_back(make_int4((int)(k << 4) | m, (int)(seg_x0[sg] + (r - row * w)) | (sg << 16), (int)(row % ny),
+                             (int)(row / ny)));
+    }
+    if (k_max_chunks_exceeded(S)) return fail(h, TLM_ERR_UNSUPPORTED, "storage too large for the chunk table");
+    h->n_bulk_chunks = (int)ch.size();
+    if ((rc = dev_upload(h, ch, &h->d_bulk_chunks))) return rc;
+  }
+  {
     // Workgroups of the bulk line kernel: the 256-cell storage blocks that
     // hold a pure bulk line.  The y-band table splits the blocks of every
     // z-plane of a segment into eight equal runs in storage order and gives
@@ -702,6 +741,35 @@ int finalize(tlm_handle* h) {
   if ((rc = dev_upload(h, sf_mtype, &d.sf_mtype))) return rc;
   if ((rc = dev_upload(h, sf_exc, &d.sf_exc))) return rc;
   if ((rc = dev_upload(h, sf_out, &d.sf_out))) return rc;
+  if ((rc = dev_upload(h, sf_host, &d.sf_host))) return rc;
+  {
+    h->n_host_faces = (int)hf_codes.size();
+    h->host_face_xyz.clear();
+    std::vector<int32_t> cp, cm, ax;
+    for (int code : hf_codes) {
+      const long long c = sf_cell[code];
+      const int a = sf_axis[code];
+      const long long off = a == 0 ? 1 : (a == 1 ? nx : (long long)nx * ny);
+      cp.push_back(sidx(c));
+      cm.push_back(sidx(c - off));
+      ax.push_back(a);
+      const long long r = c / nx;
+      h->host_face_xyz.push_back((int32_t)(c - r * nx) + 1);
+      h->host_face_xyz.push_back((int32_t)(r % ny) + 1);
+      h->host_face_xyz.push_back((int32_t)(r / ny) + 1);
+      h->host_face_xyz.push_back(a + 1);
+    }
+    if ((rc = dev_upload(h, cp, &h->d_hf_cp))) return rc;
+    if ((rc = dev_upload(h, cm, &h->d_hf_cm))) return rc;
+    if ((rc = dev_upload(h, ax, &h->d_hf_ax))) return rc;
+    void* p = nullptr;
+    if ((rc = dev_alloc(h, (size_t)4 * std::max(h->n_host_faces, 1) * sizeof(double), &p))) return rc;
+    h->d_hf_tot = (double*)p;
+    HIP_TRY(h, hipMemsetAsync(p, 0, (size_t)4 * std::max(h->n_host_faces, 1) * sizeof(double), h->stream));
+    if ((rc = dev_alloc(h, (size_t)4 * std::max(h->n_host_faces, 1) * sizeof(double), &p))) return rc;
+    h->d_hf_buf = (double*)p;
+    d.host_tot = h->d_hf_tot;
+  }
   d.n_face_out_rows = n_face_out_rows;
   if (n_face_out_rows) {
     void* p;
@@ -879,15 +947,15 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
       const int32_t* blocks = direct ? nullptr : (yband ? h->d_bulk_blocks_yband : h->d_bulk_blocks);
       const int nb = yband ? h->n_bulk_blocks_yband : (int)h->bulk_blocks.size();
       const int per = h->sched_by;  // storage blocks per workgroup
-      const int nwg = direct ? (int)((d.seg_base[d.nseg] + scn::kLineWg - 1) / scn::kLineWg)
-                             : (yband ? 8 * ((nb / 8 + per - 1) / per) : (nb + per - 1) / per);
+      const int nwg = direct ? h->n_bulk_chunks : (yband ? 8 * ((nb / 8 + per - 1) / per) : (nb + per - 1) / per);
       if (direct) {
-        if (pull)
-          hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, true>), dim3(nwg), dim3(scn::kLineWg), 0, h->stream, d, Lin, Lout, a,
-                             h->sched_xcd);
+        if (!nwg) {
+        } else if (pull)
+          hipLaunchKernelGGL((scn::k_bulk_chunks<T, PLAIN, true>), dim3(nwg), dim3(scn::kChunk), 0, h->stream, d, Lin,
+                             Lout, a, h->d_bulk_chunks, h->sched_xcd);
         else
-          hipLaunchKernelGGL((scn::k_bulk_lines<T, PLAIN, false>), dim3(nwg), dim3(scn::kLineWg), 0, h->stream, d, Lin, Lout, a,
-                             h->sched_xcd);
+          hipLaunchKernelGGL((scn::k_bulk_chunks<T, PLAIN, false>), dim3(nwg), dim3(scn::kChunk), 0, h->stream, d, Lin,
+                             Lout, a, h->d_bulk_chunks, h->sched_xcd);
       } else if (nb) {
         if (pull)
           hipLaunchKernelGGL((scn::k_bulk_blocks<T, PLAIN, true>), dim3(nwg), dim3(256), 0, h->stream, d, Lin, Lout, a,
@@ -1219,8 +1287,8 @@ int tlm_set_surface_materials(tlm_handle* h, int32_t n, const int32_t* type) {
   if (!h || n < 0 || (n > 0 && !type)) return TLM_ERR_ARG;
   if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
   for (int i = 0; i < n; i++)
-    if (type[i] != TLM_SMAT_PEC && type[i] != TLM_SMAT_PMC && type[i] != TLM_SMAT_FREE_SPACE)
-      return fail(h, TLM_ERR_UNSUPPORTED, "surface material type stays on the host path");
+    if (type[i] != TLM_SMAT_PEC && type[i] != TLM_SMAT_PMC && type[i] != TLM_SMAT_FREE_SPACE && !host_smat(type[i]))
+      return fail(h, TLM_ERR_UNSUPPORTED, "unknown surface material type");
   h->smat.assign(type, type + n);
   return TLM_OK;
 }
@@ -1409,6 +1477,9 @@ int tlm_connect(tlm_handle* h, int32_t timestep) {
   if (h->kind != Kind::SCATTERED || h->pend_connect)
     return fail(h, TLM_ERR_STATE, "connect must follow a scatter");
   if (timestep < 1 || timestep > h->g.n_timesteps) return fail(h, TLM_ERR_ARG, "timestep out of range");
+  if (h->n_host_faces && !h->hf_ready)
+    return fail(h, TLM_ERR_STATE, "host-path faces need tlm_scatter_host_faces between scatter and connect");
+  h->hf_ready = false;  // the pending connect owns them until the next pull
   h->pend_connect = true;
   h->pend_ts = timestep;
   return TLM_OK;
@@ -1480,6 +1551,47 @@ int tlm_scatter_host_cells(tlm_handle* h, const double* V) {
   return TLM_OK;
 }
 
+int tlm_host_faces(tlm_handle* h, int32_t* n_host_faces, int32_t* faces) {
+  if (!h || !n_host_faces) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  *n_host_faces = h->n_host_faces;
+  if (faces && h->n_host_faces) std::copy(h->host_face_xyz.begin(), h->host_face_xyz.end(), faces);
+  return TLM_OK;
+}
+
+int tlm_gather_host_faces(tlm_handle* h, double* V) {
+  if (!h || !V) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = host_window(h))) return rc;
+  if (!h->n_host_faces) return TLM_OK;
+  const dim3 g((h->n_host_faces + 255) / 256);
+  if (h->form == TLM_FORM_PLAIN_FP64)
+    hipLaunchKernelGGL(scn::k_gather_host_faces<double>, g, dim3(256), 0, h->stream, (const double*)h->planes[h->cur],
+                       h->P, h->d_hf_cp, h->d_hf_cm, h->d_hf_ax, h->n_host_faces, h->d_hf_buf);
+  else
+    hipLaunchKernelGGL(scn::k_gather_host_faces<float>, g, dim3(256), 0, h->stream, (const float*)h->planes[h->cur],
+                       h->P, h->d_hf_cp, h->d_hf_cm, h->d_hf_ax, h->n_host_faces, h->d_hf_buf);
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipMemcpyAsync(V, h->d_hf_buf, (size_t)4 * h->n_host_faces * sizeof(double), hipMemcpyDeviceToHost,
+                            h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return TLM_OK;
+}
+
+int tlm_scatter_host_faces(tlm_handle* h, const double* T) {
+  if (!h || !T) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = host_window(h))) return rc;
+  if (h->n_host_faces)
+    HIP_TRY(h, hipMemcpyAsync(h->d_hf_tot, T, (size_t)4 * h->n_host_faces * sizeof(double), hipMemcpyHostToDevice,
+                              h->stream));
+  // the caller's array may be reused at once: wait for the copy
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->hf_ready = true;
+  return TLM_OK;
+}
+
 int tlm_step(tlm_handle* h, int32_t first_timestep, int32_t n_steps) {
   if (!h) return TLM_ERR_ARG;
   if (n_steps < 0 || first_timestep < 1 || first_timestep + n_steps - 1 > h->g.n_timesteps)
@@ -1487,6 +1599,8 @@ int tlm_step(tlm_handle* h, int32_t first_timestep, int32_t n_steps) {
   int rc;
   if ((rc = finalize(h))) return rc;
   if (h->n_host) return fail(h, TLM_ERR_STATE, "host-path cells need the split calls (tlm_scatter, tlm_connect)");
+  if (h->n_host_faces)
+    return fail(h, TLM_ERR_STATE, "host-path faces need the split calls (tlm_scatter, tlm_connect)");
   for (int t = first_timestep; t < first_timestep + n_steps; t++) {
     if ((rc = do_scatter(h, t))) return rc;
     h->pend_connect = true;

@@ -162,6 +162,13 @@ module tlm_gpu_iface
       real(c_double), intent(inout) :: V(*)
     end function tlm_download_links
 
+    ! tlm_download_links without device allocation (fixed 32 MiB staging buffer)
+    integer(c_int) function tlm_download_links_noalloc(h, V) bind(c, name='tlm_download_links_noalloc')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(inout) :: V(*)
+    end function tlm_download_links_noalloc
+
     integer(c_int) function tlm_excitation(h, timestep) bind(c, name='tlm_excitation')
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
@@ -264,6 +271,32 @@ module tlm_gpu_iface
       type(c_ptr), value :: h
       real(c_double), intent(in) :: V(12, *)
     end function tlm_scatter_host_cells
+
+    ! Host-path faces (thin layer, diode, spice, switch): faces = c_null_ptr
+    ! asks for the count, else c_loc of 4*n entries (cx, cy, cz, axis).
+    integer(c_int) function tlm_host_faces(h, n_faces, faces) bind(c, name='tlm_host_faces')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: n_faces
+      type(c_ptr), value :: faces
+    end function tlm_host_faces
+
+    ! V(1:4, k): scattered pulses beside host face k (a on the + side cell,
+    ! a on the - side cell, b on +, b on -), between scatter and connect
+    integer(c_int) function tlm_gather_host_faces(h, V) bind(c, name='tlm_gather_host_faces')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: V(4, *)
+    end function tlm_gather_host_faces
+
+    ! T(1:4, k) = (Va_min, Va_max, Vb_min, Vb_max): the face voltages of the
+    ! surface model, e.g. Vy_min, Vy_max, Vz_min, Vz_max of
+    ! connect_normal_to_x.F90
+    integer(c_int) function tlm_scatter_host_faces(h, T) bind(c, name='tlm_scatter_host_faces')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: T(4, *)
+    end function tlm_scatter_host_faces
 
     integer(c_int) function tlm_set_dft_frequencies(h, n, f) bind(c, name='tlm_set_dft_frequencies')
       import :: c_int, c_int32_t, c_ptr, c_double

@@ -37,7 +37,7 @@ import numpy as np
 from .constants import (C0, CENTRE, COMPONENTS, EPS0, FACE_XMAX, FACE_XMIN,
                         FACE_YMAX, FACE_YMIN, FACE_ZMAX, FACE_ZMIN, FACES,
                         MAX_FILTER_ORDER, PI, SMAT_FREE_SPACE, SMAT_PEC,
-                        SMAT_PMC, SOURCE_HARD, SOURCE_SOFT, VMAT_DISPERSIVE, Z0)
+                        SMAT_PMC, SOURCE_HARD, SOURCE_SOFT, SURFACE_MATERIALS, VMAT_DISPERSIVE, Z0)
 from .filters import Sfilter, volume_material_row
 
 # ---------------------------------------------------------------------------
@@ -162,7 +162,9 @@ class Problem:
     pml: Optional[PML] = None
     materials: List[VolumeMaterial] = field(default_factory=list)
     material_regions: List[Tuple[object, int]] = field(default_factory=list)
-    surface_materials: List[str] = field(default_factory=list)   # 'pec'|'pmc'|'free_space'
+    # 'pec'|'pmc'|'free_space' (device) or a host-path type: 'thin_layer',
+    # 'anisotropic_thin_layer', 'diode', 'spice', 'switch'
+    surface_materials: List[str] = field(default_factory=list)
     surface_faces: List[Tuple[np.ndarray, int]] = field(default_factory=list)
     functions: List[ExcitationFunction] = field(default_factory=list)
     sources: List[PointSource] = field(default_factory=list)
@@ -608,7 +610,7 @@ def build_tables(p: Problem) -> Tables:
     f_out = np.zeros(fshape, dtype=bool)
     smat_types = []
     for s in p.surface_materials:
-        smat_types.append({"pec": SMAT_PEC, "pmc": SMAT_PMC, "free_space": SMAT_FREE_SPACE}[s])
+        smat_types.append(SURFACE_MATERIALS[s])
     for faces, mnum in p.surface_faces:
         for (cx, cy, cz, face) in np.asarray(faces, dtype=np.int64).reshape(-1, 4):
             t = _min_face(nx, ny, nz, (int(cx), int(cy), int(cz)), int(face))

@@ -55,12 +55,18 @@ extern "C" {
 #define TLM_VMAT_PMC 2
 #define TLM_VMAT_DISPERSIVE 3
 
-/* Surface material types (TLM_material_modules.F90:153-169).  Only PEC, PMC
- * and FREE_SPACE faces are updated on the device; the others (thin layers,
- * diodes, switches, spice ports) stay on the host path. */
+/* Surface material types (TLM_material_modules.F90:153-169).  PEC, PMC and
+ * FREE_SPACE faces are connected on the device; thin layers (DISPERSIVE,
+ * ANISOTROPIC_DISPERSIVE), diodes, spice ports and switches are host-path
+ * faces whose face voltages the host computes (see tlm_host_faces). */
 #define TLM_SMAT_PEC 1
 #define TLM_SMAT_PMC 2
+#define TLM_SMAT_DISPERSIVE 3
 #define TLM_SMAT_FREE_SPACE 4
+#define TLM_SMAT_ANISOTROPIC_DISPERSIVE 5
+#define TLM_SMAT_DIODE 6
+#define TLM_SMAT_SPICE 7
+#define TLM_SMAT_SWITCH 9
 
 typedef struct tlm_handle tlm_handle;
 
@@ -269,6 +275,32 @@ int tlm_read_dft(tlm_handle* h, double* re_im);
 int tlm_host_cells(tlm_handle* h, int32_t* n_host, int32_t* cells /* 3*n_host, 1-based, or NULL */);
 int tlm_gather_host_cells(tlm_handle* h, double* V);
 int tlm_scatter_host_cells(tlm_handle* h, const double* V);
+
+/* ---- host-path faces ------------------------------------------------------
+ * Faces of the host-path surface material types (thin layer, anisotropic
+ * thin layer, diode, spice port, switch: connect_normal_to_x.F90:75-283 and
+ * its y, z counterparts) keep their models on the host.  For such a face
+ * the host computes the four face voltages the reference's material branch
+ * produces from the four scattered pulses beside the face; the device then
+ * applies the face's excitation, completes the connection and records the
+ * face output exactly as for any special face (:286-361).  Between
+ * tlm_scatter and tlm_connect of a step:
+ *   tlm_gather_host_faces  V[4k+0..3] = scattered (a on the face's + side
+ *                          cell, a on the - side cell, b on the + side,
+ *                          b on the - side), where for a face normal to x
+ *                          (a, b) = (Vy, Vz): V(Vy_xmin,cx), V(Vy_xmax,cx-1),
+ *                          V(Vz_xmin,cx), V(Vz_xmax,cx-1); normal to y
+ *                          (Vx, Vz); normal to z (Vx, Vy);
+ *   tlm_scatter_host_faces T[4k+0..3] = (Va_min, Va_max, Vb_min, Vb_max),
+ *                          the reference's Vy_min, Vy_max, Vz_min, Vz_max
+ *                          (x-normal faces) and likewise for y and z.
+ * Rows follow the connect's scan order (cell x fastest; per cell its x-,
+ * y-, z-normal min face); tlm_host_faces returns (cx, cy, cz, axis 1..3) of
+ * the cell on the face's + side.  tlm_connect refuses a step whose host
+ * faces were not written; tlm_step refuses handles with host-path faces. */
+int tlm_host_faces(tlm_handle* h, int32_t* n_host_faces, int32_t* faces /* 4*n, 1-based, or NULL */);
+int tlm_gather_host_faces(tlm_handle* h, double* V);
+int tlm_scatter_host_faces(tlm_handle* h, const double* T);
 
 /* ---- schedule and measurement (no counterpart in the reference) ---------- */
 

@@ -118,6 +118,14 @@ struct oracle_handle {
   /* z-slab threading (the MPI build's z partition, general_mesh_subroutines.F90:868-972,
    * as threads): the scan-order counters of scatter and connect at the start
    * of each z-plane, so that planes can run in any order */
+  /* host-path faces (thin layer, diode, spice, switch surface materials):
+   * per special face code the host face number (1-based, 0 = device face),
+   * their count, and the four face voltages per host face that the host
+   * wrote for the coming connect */
+  int32_t* hf_of_code;
+  int n_host_faces;
+  double* hf_tot;
+  int hf_ready;
   int threads;
   int32_t *pl_out, *pl_exc;   /* output / excitation cell numbers before plane cz (nz+1) */
   int32_t *pl_fexc, *pl_fout; /* excitation / output face numbers before plane cz (nz+1) */
@@ -182,6 +190,7 @@ void oracle_destroy(oracle_handle* h) {
   free(h->face_exc_type); free(h->huy); free(h->cell_out); free(h->face_out); free(h->fring);
   free(h->pt_row); free(h->pt_comp); free(h->freq); free(h->dft); free(h->ring);
   free(h->pl_out); free(h->pl_exc); free(h->pl_fexc); free(h->pl_fout);
+  free(h->hf_of_code); free(h->hf_tot);
   free(h);
 }
 
@@ -237,6 +246,7 @@ int oracle_set_face_codes(oracle_handle* h, const int32_t* code, int32_t n_speci
     if (to_cable && to_cable[i]) return fail(h, TLM_ERR_UNSUPPORTED, "cable faces stay on the host path");
   free(h->pl_fexc); free(h->pl_fout);
   h->pl_fexc = h->pl_fout = NULL;
+  free(h->hf_of_code); h->hf_of_code = NULL;
   h->n_faces = (int)nf;
   h->face_code = dup_i32(code, nf);
   h->n_special_faces = n_special;
@@ -298,10 +308,18 @@ int oracle_set_volume_materials(oracle_handle* h, int32_t n, const tlm_volume_ma
   return alloc_filters(h);
 }
 
+/* Surface material types whose connect stays on the host
+ * (connect_normal_to_x.F90:75-283). */
+static int host_smat(int type) {
+  return type == TLM_SMAT_DISPERSIVE || type == TLM_SMAT_ANISOTROPIC_DISPERSIVE || type == TLM_SMAT_DIODE ||
+         type == TLM_SMAT_SPICE || type == TLM_SMAT_SWITCH;
+}
+
 int oracle_set_surface_materials(oracle_handle* h, int32_t n, const int32_t* type) {
   for (int i = 0; i < n; i++)
-    if (type[i] != TLM_SMAT_PEC && type[i] != TLM_SMAT_PMC && type[i] != TLM_SMAT_FREE_SPACE)
-      return fail(h, TLM_ERR_UNSUPPORTED, "surface material type stays on the host path");
+    if (type[i] != TLM_SMAT_PEC && type[i] != TLM_SMAT_PMC && type[i] != TLM_SMAT_FREE_SPACE && !host_smat(type[i]))
+      return fail(h, TLM_ERR_UNSUPPORTED, "unknown surface material type");
+  free(h->hf_of_code); h->hf_of_code = NULL;
   free(h->smat);
   h->n_smat = n;
   h->smat = dup_i32(type, n);
@@ -1174,6 +1192,12 @@ static int connect_special_face(oracle_handle* h, int axis, double* vmin_cell, d
   } else if (material_type == TLM_SMAT_PMC) {
     a_min = 2.0 * *A; a_max = 2.0 * *C;
     b_min = 2.0 * *B; b_max = 2.0 * *D;
+  } else if (host_smat(material_type)) {
+    /* the face voltages the host's material model computed from this
+     * face's scattered pulses (tlm_scatter_host_faces) */
+    const double* T = &h->hf_tot[(size_t)(h->hf_of_code[code - 1] - 1) * 4];
+    a_min = T[0]; a_max = T[1];
+    b_min = T[2]; b_max = T[3];
   } else {
     return fail(h, TLM_ERR_UNSUPPORTED, "surface material type stays on the host path");
   }
@@ -1244,6 +1268,81 @@ static int connect_special_face(oracle_handle* h, int axis, double* vmin_cell, d
       }
     }
   }
+  return TLM_OK;
+}
+
+/* Host-path faces in connect()'s scan order.  mode 0 numbers them (and
+ * counts), 1 lists (cx, cy, cz, axis), 2 gathers the four scattered pulses
+ * beside each face. */
+static void host_face_walk(oracle_handle* h, int mode, int32_t* faces, double* V4) {
+  static const int pa[3] = {Vy_xmin, Vx_ymin, Vx_zmin}, pb[3] = {Vz_xmin, Vz_ymin, Vy_zmin};
+  static const int qa[3] = {Vy_xmax, Vx_ymax, Vx_zmax}, qb[3] = {Vz_xmax, Vz_ymax, Vy_zmax};
+  long long face_number = 0;
+  int k = 0;
+  for (int cz = 1; cz <= h->nz; cz++)
+    for (int cy = 1; cy <= h->ny; cy++)
+      for (int cx = 1; cx <= h->nx; cx++)
+        for (int axis = 0; axis < 3; axis++) {
+          if ((axis == 0 && cx == 1) || (axis == 1 && cy == 1) || (axis == 2 && cz == 1)) continue;
+          int code = h->face_code[face_number++];
+          if (!code) continue;
+          int m = h->f_mat[code - 1];
+          int mn = m < 0 ? -m : m;
+          if (!mn || !host_smat(h->smat[mn - 1])) continue;
+          if (mode == 0) h->hf_of_code[code - 1] = k + 1;
+          if (mode == 1 && faces) {
+            faces[4 * k] = cx; faces[4 * k + 1] = cy; faces[4 * k + 2] = cz; faces[4 * k + 3] = axis + 1;
+          }
+          if (mode == 2) {
+            const double* v = cellV(h, cx, cy, cz);
+            const double* w = cellV(h, cx - (axis == 0), cy - (axis == 1), cz - (axis == 2));
+            V4[4 * k] = v[pa[axis]]; V4[4 * k + 1] = w[qa[axis]];
+            V4[4 * k + 2] = v[pb[axis]]; V4[4 * k + 3] = w[qb[axis]];
+          }
+          k++;
+        }
+  if (mode == 0) h->n_host_faces = k;
+}
+
+static int host_face_numbers(oracle_handle* h) {
+  if (h->hf_of_code) return TLM_OK;
+  if (!h->face_code) return fail(h, TLM_ERR_STATE, "face codes not set");
+  for (int i = 0; i < h->n_special_faces; i++) {
+    int m = h->f_mat[i] < 0 ? -h->f_mat[i] : h->f_mat[i];
+    if (m > h->n_smat) return fail(h, TLM_ERR_ARG, "surface material number out of range");
+  }
+  h->hf_of_code = (int32_t*)calloc((size_t)(h->n_special_faces ? h->n_special_faces : 1), sizeof(int32_t));
+  if (!h->hf_of_code) return fail(h, TLM_ERR_NOMEM, "host face numbers");
+  host_face_walk(h, 0, NULL, NULL);
+  free(h->hf_tot);
+  h->hf_tot = (double*)calloc((size_t)(h->n_host_faces ? h->n_host_faces : 1) * 4, sizeof(double));
+  h->hf_ready = 0;
+  return h->hf_tot ? TLM_OK : fail(h, TLM_ERR_NOMEM, "host face voltages");
+}
+
+int oracle_host_faces(oracle_handle* h, int32_t* n_host_faces, int32_t* faces) {
+  if (!h || !n_host_faces) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = host_face_numbers(h))) return rc;
+  *n_host_faces = h->n_host_faces;
+  if (faces) host_face_walk(h, 1, faces, NULL);
+  return TLM_OK;
+}
+
+int oracle_gather_host_faces(oracle_handle* h, double* V) {
+  if (!h || !V) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = host_face_numbers(h))) return rc;
+  host_face_walk(h, 2, NULL, V);
+  return TLM_OK;
+}
+
+int oracle_scatter_host_faces(oracle_handle* h, const double* T) {
+  if (!h || !T) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = host_face_numbers(h))) return rc;
+  memcpy(h->hf_tot, T, (size_t)h->n_host_faces * 4 * sizeof(double));
+  h->hf_ready = 1;
   return TLM_OK;
 }
 
@@ -1347,6 +1446,10 @@ int oracle_connect(oracle_handle* h, int32_t timestep) {
   if (!h->face_code) return fail(h, TLM_ERR_STATE, "face codes not set");
   int rc;
   if ((rc = face_prefixes(h))) return rc;
+  if ((rc = host_face_numbers(h))) return rc;
+  if (h->n_host_faces && !h->hf_ready)
+    return fail(h, TLM_ERR_STATE, "host-path faces need tlm_scatter_host_faces between scatter and connect");
+  h->hf_ready = 0;
   int err = TLM_OK;
 #pragma omp parallel for schedule(static) num_threads(h->threads)
   for (int cz = 1; cz <= h->nz; cz++) {
@@ -1409,6 +1512,11 @@ int oracle_outer_boundary(oracle_handle* h) {
 }
 
 int oracle_step(oracle_handle* h, int32_t first_timestep, int32_t n_steps) {
+  if (h->face_code && h->smat) {
+    int rc = host_face_numbers(h);
+    if (rc) return rc;
+    if (h->n_host_faces) return fail(h, TLM_ERR_STATE, "host-path faces need the split calls (tlm_scatter, tlm_connect)");
+  }
   if (h->c_cable)
     for (int32_t i = 0; i < h->n_special; i++)
       if (h->c_cable[i]) return fail(h, TLM_ERR_STATE, "host-path cells need the split calls (tlm_scatter, tlm_connect)");

@@ -72,6 +72,9 @@ int oracle_layout_segments(oracle_handle* h, int32_t* n_segments, int32_t* x0);
 int oracle_host_cells(oracle_handle* h, int32_t* n_host, int32_t* cells);
 int oracle_gather_host_cells(oracle_handle* h, double* V);
 int oracle_scatter_host_cells(oracle_handle* h, const double* V);
+int oracle_host_faces(oracle_handle* h, int32_t* n_host_faces, int32_t* faces);
+int oracle_gather_host_faces(oracle_handle* h, double* V);
+int oracle_scatter_host_faces(oracle_handle* h, const double* T);
 int oracle_autotune(oracle_handle* h, int32_t first_timestep, int32_t steps_per_candidate, int32_t* steps_done);
 int oracle_set_small_to_zero(oracle_handle* h, int32_t n_small_to_zero, double small_to_zero_value);
 int oracle_read_kernel_times(oracle_handle* h, int64_t* launches, double* ms);

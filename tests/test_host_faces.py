@@ -1,0 +1,140 @@
+"""Host-path faces (thin layer, diode, spice and switch surface materials):
+between scatter and connect the host reads the four scattered pulses beside
+each such face, runs its face model and writes the four face voltages
+(Va_min, Va_max, Vb_min, Vb_max) that the reference's material branch of
+connect_normal_to_{x,y,z}.F90 produces; excitation, completion and face
+outputs then proceed as for any special face.
+
+Pins: host models restating the device's own PEC, PMC and free-space faces
+reproduce the deck that has those faces on the device, bit for bit (oracle
+and engine), which fixes the hand-off's side and polarisation order; a
+lumped resistive sheet model runs identically on the oracle and the engine.
+"""
+import numpy as np
+import pytest
+
+import problems
+from ggi_tlm_amd import capi
+from ggi_tlm_amd.capi import FORM_PLAIN_FP64, FORM_PRECISION_FP32
+from ggi_tlm_amd.constants import Z0
+from ggi_tlm_amd.problem import build_tables
+
+STEPS = 50
+
+
+def device_deck():
+    """face_outputs(): PEC x-faces, PMC y-faces, free-space z-faces on the
+    device, face outputs on each."""
+    return problems.face_outputs(14, STEPS)
+
+
+def host_deck():
+    """The same faces as host-path surface materials."""
+    p = problems.face_outputs(14, STEPS)
+    p.surface_materials = ["thin_layer", "diode", "switch"]
+    return p
+
+
+def model_device(V, axes):
+    """Face voltages of the device's PEC (x-normal set), PMC (y-normal set)
+    and free-space (z-normal set) faces from the pulses
+    (a+, a-, b+, b-)."""
+    T = np.zeros_like(V)
+    ap, am, bp, bm = V[:, 0], V[:, 1], V[:, 2], V[:, 3]
+    pmc, free = axes == 2, axes == 3
+    T[pmc] = np.stack([2.0 * ap, 2.0 * am, 2.0 * bp, 2.0 * bm], axis=1)[pmc]
+    s_a, s_b = ap + am, bp + bm
+    T[free] = np.stack([s_a, s_a, s_b, s_b], axis=1)[free]
+    return T
+
+
+def model_sheet(V, axes, R=50.0):
+    """A resistive sheet R (ohm per square) in shunt across the face: both
+    lines see the node voltage (a+ + a-) 2R / (2R + Z0)."""
+    k = 2.0 * R / (2.0 * R + Z0)
+    s_a, s_b = (V[:, 0] + V[:, 1]) * k, (V[:, 2] + V[:, 3]) * k
+    return np.stack([s_a, s_a, s_b, s_b], axis=1)
+
+
+def run_host(h, model, steps=STEPS):
+    faces = h.host_faces()
+    axes = faces[:, 3]
+    for t in range(1, steps + 1):
+        h.excitation(t)
+        h.scatter(t)
+        V = h.gather_host_faces(len(faces))
+        h.scatter_host_faces(model(V, axes))
+        h.connect(t)
+        h.outer_boundary()
+    return results(h)
+
+
+def results(h):
+    return h.read_cell_outputs()[0], h.read_face_outputs()[0], h.download_links()[:, 1:-1, 1:-1]
+
+
+def same(a, b):
+    return all(x.shape == y.shape and np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def test_host_face_listing(oracle_lib):
+    from oracle import oracle
+
+    t = build_tables(host_deck())
+    h = oracle.handle(t, output_capacity=STEPS)
+    f = h.host_faces()
+    assert len(f) == 36 + 25 + 9
+    assert set(np.unique(f[:, 3])) == {1, 2, 3}
+    # scan order of the connect: z, then y, then x of the + side cell, then axis
+    key = ((f[:, 2] * 100 + f[:, 1]) * 100 + f[:, 0]) * 4 + f[:, 3]
+    assert np.all(np.diff(key) > 0)
+    assert np.all(f[f[:, 3] == 1, 0] == 4)     # PEC set: xmin faces of x = 4
+    assert np.all(f[f[:, 3] == 2, 1] == 10)    # PMC set: ymin faces of y = 10
+    assert np.all(f[f[:, 3] == 3, 2] == 12)    # free set: zmax faces of z = 11 (cell 12's zmin)
+
+
+def test_host_faces_need_their_voltages(oracle_lib):
+    from oracle import oracle
+
+    t = build_tables(host_deck())
+    h = oracle.handle(t, output_capacity=STEPS)
+    with pytest.raises(capi.TlmError):
+        h.step(1, 1)
+    h.excitation(1)
+    h.scatter(1)
+    with pytest.raises(capi.TlmError):
+        h.connect(1)
+
+
+def test_host_models_reproduce_device_faces_oracle(oracle_lib):
+    from oracle import oracle
+
+    td, th = build_tables(device_deck()), build_tables(host_deck())
+    d = oracle.handle(td, output_capacity=STEPS)
+    d.step(1, STEPS)
+    want = results(d)
+    got = run_host(oracle.handle(th, output_capacity=STEPS), model_device)
+    assert np.abs(want[1]).max() > 0
+    assert same(want, got)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
+def test_host_faces_engine_vs_oracle(oracle_lib, form):
+    from oracle import oracle
+
+    th, td = build_tables(host_deck()), build_tables(device_deck())
+    lib = capi.engine_library()
+    for model in (model_device, model_sheet):
+        print(f"[host faces] form={form} model={model.__name__}", flush=True)
+        o = run_host(oracle.handle(th, output_capacity=STEPS), model)
+        g = run_host(capi.Handle(lib, th, form, output_capacity=STEPS), model)
+        if form == FORM_PLAIN_FP64:
+            assert same(o, g), model.__name__
+        else:
+            for a, b in zip(o, g):
+                assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(a), model.__name__
+    # host models of the device's faces = the device's faces, on the engine
+    dev = capi.Handle(lib, td, form, output_capacity=STEPS)
+    dev.step(1, STEPS)
+    assert same(results(dev), run_host(capi.Handle(lib, th, form, output_capacity=STEPS), model_device))
