@@ -34,6 +34,7 @@ struct BatchSolve {
   const int32_t* lds_fd;
   const int32_t* lds_slot_p;
   const int32_t* lds_slot_i;
+  const int4* mat_recs;      // material-line records (two int4 each)
   int n_pure_pml, n_pure_mat, n_pure_gen, n_pm, n_huy_coded;
   int cur, prv, prv_mode;    // link buffer rotation, as in the single-solve handle
   int ts, first, last;       // next timestep, first and last timestep
@@ -67,7 +68,7 @@ struct tlm_batch {
   // block tables: lines, mixed (lane slots, or LDS-staged without PML
   // cells), LDS-staged with PML cells, pml, material, generic,
   // pml-material, huygens, dft
-  enum { T_LINES, T_MIXED, T_MIXED_P, T_PML, T_MAT, T_GEN, T_PM, T_HUY, T_DFT, T_N };
+  enum { T_LINES, T_MIXED, T_MIXED_P, T_PML, T_MAT, T_GEN, T_PM, T_HUY, T_DFT, T_MATL, T_N };
   int2* d_tab[T_N] = {};
   int n_tab[T_N] = {};
   size_t cap_tab[T_N] = {};
@@ -225,6 +226,24 @@ __global__ __launch_bounds__(scn::kLdsCells) void kb_mixed_lds(const BatchSolve*
                                              s.lds_slot_i, s.ts, slot, sv);
 }
 
+// Material lines of a solve, one wavefront per record (k_mat_lines).
+template <typename T, bool PLAIN>
+__global__ __launch_bounds__(scn::kChunk) void kb_mat_lines(const BatchSolve* __restrict__ S,
+                                                            const int2* __restrict__ tab) {
+  const int2 e = tab[blockIdx.x];
+  const BatchSolve& s = S[e.x];
+  if (s.ts > s.last) return;
+  PullArgs a;
+  bool pull;
+  int slot;
+  batch_pull_args<T, PLAIN>(s, a, pull, slot);
+  const T* Lin = (const T*)s.planes[s.cur];
+  T* Lout = (T*)s.planes[batch_out(s)];
+  const int4 r0 = s.mat_recs[2 * e.y], r1 = s.mat_recs[2 * e.y + 1];
+  if (pull) scn::mat_line_cell<T, PLAIN, true>(s.d, Lin, Lout, a, r0, r1, threadIdx.x);
+  else scn::mat_line_cell<T, PLAIN, false>(s.d, Lin, Lout, a, r0, r1, threadIdx.x);
+}
+
 // One class list of a solve: 0 PML, 1 material, 2 generic, 3 PML holding a material.
 template <typename T, bool PLAIN, int CLS>
 __global__ __launch_bounds__(256) void kb_list(const BatchSolve* __restrict__ S, const int2* __restrict__ tab) {
@@ -321,6 +340,7 @@ void launch_batch_step(tlm_batch* b) {
   } else {
     go(cs, tlm_batch::T_MIXED, kb_mixed<T, PLAIN>);
   }
+  go(cs, tlm_batch::T_MATL, kb_mat_lines<T, PLAIN>, scn::kChunk);
   go(cs, tlm_batch::T_MAT, kb_list<T, PLAIN, 1>);
   go(cs, tlm_batch::T_GEN, kb_list<T, PLAIN, 2>);
   go(cs, tlm_batch::T_PM, kb_list<T, PLAIN, 3>);
@@ -377,6 +397,7 @@ int rebuild(tlm_batch* b) {
     s.lds_fd = h->d_lds_fd;
     s.lds_slot_p = h->d_lds_slot_p;
     s.lds_slot_i = h->d_lds_slot_i;
+    s.mat_recs = h->d_mat_recs;
     s.n_pure_pml = h->n_pure_pml;
     s.n_pure_mat = h->n_pure_mat;
     s.n_pure_gen = h->n_pure_gen;
@@ -410,6 +431,7 @@ int rebuild(tlm_batch* b) {
     auto add = [&](int t, long long cnt) {
       for (long long k = 0; k < (cnt + 255) / 256; k++) tab[t].push_back(make_int2(i, (int)k));
     };
+    for (int k = 0; k < h->n_mat_recs; k++) tab[tlm_batch::T_MATL].push_back(make_int2(i, k));
     add(tlm_batch::T_PML, h->n_pure_pml);
     add(tlm_batch::T_MAT, h->n_pure_mat);
     add(tlm_batch::T_GEN, h->n_pure_gen);

@@ -1761,6 +1761,61 @@ __global__ __launch_bounds__(kChunk) void k_bulk_chunks(Dev d, const T* __restri
   bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z, sg);
 }
 
+// Material lines: lines holding free-space cells and material cells of one
+// material number (a dielectric body's interior and its surface rows).
+// One wavefront per 64-cell chunk and material number, from two records
+// read by all lanes at one address (scalar loads):
+//   e = (chunk << 4 | mask of the chunk's lines it writes,
+//        x of the chunk's first cell | segment << 16, y, z)
+//   f = (material number, lanes that are material cells (64 bits),
+//        filter slot of the first of them)
+// The material cells of a chunk hold consecutive filter slots (3 each), so
+// a lane's slot follows from the count of material lanes below it.  Every
+// cell has six ordinary faces: the bulk pull serves both kinds.  The node
+// updates are the reference's: scatter.F90:222-272 for free space, the
+// Thevenin node (:274-548) for the material cells.
+template <typename T, bool PLAIN, bool PULL>
+__device__ inline void mat_line_cell(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                     const PullArgs& a, int4 e, int4 f, int lane) {
+  if (!((e.x >> (lane / d.line_len)) & 1)) return;
+  const long long c = (long long)(e.x >> 4) * kChunk + lane;
+  int sg = e.y >> 16, x = (e.y & 0xffff) + lane, y = e.z, z = e.w;
+  const int w = seg_width(d, sg);
+  if (w >= kChunk) {
+    if (x >= d.seg_x0[sg + 1]) {
+      x -= w;
+      if (++y == d.ny) { y = 0; z++; }
+    }
+    if (z >= d.nz) return;  // padding after the segment's last row
+  } else if (!cell_coords(d, c, x, y, z, sg)) {
+    return;
+  }
+  const uint64_t lanes = (uint64_t)(uint32_t)f.y | ((uint64_t)(uint32_t)f.z << 32);
+  double v[12];
+  if (PULL) {
+    bulk_pull<T, PLAIN>(d, Lin, a, c, x, y, z, sg, v);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 12; p++) v[p] = (double)Lin[p * d.P + c];
+  }
+  flush_small(v, a.small);
+  if ((lanes >> lane) & 1u) {
+    const long long slot = f.w + 3 * (long long)__popcll(lanes & ((1ull << lane) - 1ull));
+    node_update<false>(d, v, f.x, slot, -1, -1, 0, 0);
+  } else if (PLAIN) {
+    scatter_free_plain(v);
+  } else {
+    scatter_free_fast(v);
+  }
+  store_cell(Lout, d.P, c, v);
+}
+
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(kChunk) void k_mat_lines(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                      PullArgs a, const int4* __restrict__ rec) {
+  mat_line_cell<T, PLAIN, PULL>(d, Lin, Lout, a, rec[2 * blockIdx.x], rec[2 * blockIdx.x + 1], threadIdx.x);
+}
+
 // blocks[0..n_blocks) lists the storage blocks to update (-1: none); a
 // workgroup takes per_wg of them in turn: consecutive entries (mode 0, 1),
 // or entries 8 apart from its XCD's column (mode 2, the y-band table).
