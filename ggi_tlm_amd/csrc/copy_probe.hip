@@ -18,6 +18,12 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a, floa
   for (; i < n; i += stride) b[i] = a[i];
 }
 
+// one float4 per lane, no loop: the practical streaming ceiling
+__global__ __launch_bounds__(256) void k_copy1(const float4* __restrict__ a, float4* __restrict__ b, long long n) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
 // twelve float planes of stride P in, twelve out, one cell per lane
 __global__ __launch_bounds__(256) void k_planes12(const float* __restrict__ a, float* __restrict__ b, long long P) {
   long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -63,8 +69,16 @@ int main(int argc, char** argv) {
   double per = time_ms(e0, e1, reps);
   printf("{\"kernel\": \"copy\", \"copy_bytes_read\": %lld, \"copy_bytes_written\": %lld, \"avg_ms\": %.5f, "
          "\"GB_s\": %.1f}\n", bytes, bytes, per, 2.0 * bytes / (per * 1e-3) / 1e9);
-  // twelve planes filling the buffer (plane stride = a twelfth of it)
-  const long long P = bytes / 4 / 12;
+  const int g1 = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(k_copy1, dim3(g1), dim3(256), 0, 0, a, b, n);
+  hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy1, dim3(g1), dim3(256), 0, 0, a, b, n);
+  hipEventRecord(e1, 0);
+  per = time_ms(e0, e1, reps);
+  printf("{\"kernel\": \"copy1\", \"avg_ms\": %.5f, \"GB_s\": %.1f}\n", per, 2.0 * bytes / (per * 1e-3) / 1e9);
+  // twelve planes of 2^24 cells (the 256^3 grid) with the engine's plane
+  // padding of 32 cells, when the buffer holds them; else a twelfth of it
+  long long P = 12LL * ((1LL << 24) + 32) * 4 <= bytes ? (1LL << 24) + 32 : bytes / 4 / 12 / 64 * 64;
   const int g = (int)((P + 255) / 256);
   hipLaunchKernelGGL(k_planes12, dim3(g), dim3(256), 0, 0, (const float*)a, (float*)b, P);
   hipEventRecord(e0, 0);
