@@ -124,7 +124,12 @@ def cpu_baseline(tables, steps, steps_all=None):
 
 def dtype_of(form):
     """Arithmetic and storage of a numerical form."""
-    return "f64" if form == capi.FORM_PLAIN_FP64 else "f32-links/f64-arith"
+    return {capi.FORM_PLAIN_FP64: "f64", capi.FORM_PRECISION_FP32: "f32-links/f64-arith",
+            capi.FORM_EXTENDED_FP32: "f32+i8-links/f64-arith"}[form]
+
+
+FORM_NAMES = {"plain": capi.FORM_PLAIN_FP64, "precision": capi.FORM_PRECISION_FP32,
+              "extended": capi.FORM_EXTENDED_FP32}
 
 
 def _build(problem):
@@ -204,10 +209,11 @@ def main_batch(a):
 
 
 def main_accuracy(a):
-    """Precision form against FP64 on the full-size workloads: relative L2 of
-    every probe series and of every probe's DFT (transfer function).  The
-    FP64 side is the plain form of the engine, which equals the CPU oracle
-    bit for bit on every parity deck (the oracle would take hours here)."""
+    """Precision and extended forms against FP64 on the full-size workloads:
+    relative L2 of every probe series and of every probe's DFT (transfer
+    function), each against its own FP64 norm.  The FP64 side is the plain
+    form of the engine, which equals the CPU oracle bit for bit on every
+    parity deck (the oracle would take hours here)."""
     out = []
     for name in a.acc_workloads:
         if name == "large":
@@ -217,7 +223,7 @@ def main_accuracy(a):
         t = build_tables(p)
         steps = t.n_timesteps
         res = {}
-        for form in (capi.FORM_PLAIN_FP64, capi.FORM_PRECISION_FP32):
+        for form in (capi.FORM_PLAIN_FP64, capi.FORM_PRECISION_FP32, capi.FORM_EXTENDED_FP32):
             h = capi.Handle(capi.engine_library(), t, form, output_capacity=steps)
             t0 = time.perf_counter()
             h.step(1, steps)
@@ -228,24 +234,33 @@ def main_accuracy(a):
             h.close()
             res[form] = (series, dft)
             print(f"[bench] accuracy {name}: form {form}, {steps} steps in {el:.1f} s", file=sys.stderr, flush=True)
-        (s64, d64), (s32, d32) = res[capi.FORM_PLAIN_FP64], res[capi.FORM_PRECISION_FP32]
+        s64, d64 = res[capi.FORM_PLAIN_FP64]
         norms = np.array([np.linalg.norm(s64[:, k]) for k in range(s64.shape[1])])
-        rel = [float(np.linalg.norm(s32[:, k] - s64[:, k]) / norms[k]) for k in range(s64.shape[1])]
-        rec = {"workload": name, "steps": steps, "probe_rel_l2": [float(f"{e:.3e}") for e in rel],
-               "probe_norm_rel_to_peak": [round(float(v), 4) for v in norms / norms.max()],
-               "max_rel_l2_probes_above_5pct_of_peak": float(f"{max(e for e, n in zip(rel, norms) if n >= 0.05 * norms.max()):.3e}")}
+        rec = {"workload": name, "steps": steps,
+               "probe_norm_rel_to_peak": [round(float(v), 4) for v in norms / norms.max()]}
         if d64 is not None:
             dn = np.array([np.linalg.norm(d64[k]) for k in range(d64.shape[0])])
-            drel = [float(np.linalg.norm(d32[k] - d64[k]) / dn[k]) for k in range(d64.shape[0])]
             rec["dft_frequencies"] = len(t.dft_frequencies)
-            rec["dft_rel_l2"] = [float(f"{e:.3e}") for e in drel]
             rec["dft_norm_rel_to_peak"] = [round(float(v), 4) for v in dn / dn.max()]
+        for fname, form in (("precision", capi.FORM_PRECISION_FP32), ("extended", capi.FORM_EXTENDED_FP32)):
+            s32, d32 = res[form]
+            rel = [float(np.linalg.norm(s32[:, k] - s64[:, k]) / norms[k]) for k in range(s64.shape[1])]
+            f = {"probe_rel_l2": [float(f"{e:.3e}") for e in rel], "max_probe_rel_l2": float(f"{max(rel):.3e}"),
+                 "max_rel_l2_probes_above_5pct_of_peak":
+                     float(f"{max(e for e, n in zip(rel, norms) if n >= 0.05 * norms.max()):.3e}")}
+            if d64 is not None:
+                drel = [float(np.linalg.norm(d32[k] - d64[k]) / dn[k]) for k in range(d64.shape[0])]
+                f["dft_rel_l2"] = [float(f"{e:.3e}") for e in drel]
+                f["max_dft_rel_l2"] = float(f"{max(drel):.3e}")
+            rec[fname] = f
         out.append(rec)
         print(f"[bench] accuracy {json.dumps(rec)}", file=sys.stderr, flush=True)
-    line = {"metric": "relative L2, precision form vs FP64", "value": max(r["max_rel_l2_probes_above_5pct_of_peak"]
-                                                                          for r in out),
-            "unit": "relative L2", "n_gpus": 1, "higher_is_better": False, "dtype": dtype_of(capi.FORM_PRECISION_FP32),
-            "data": "synthetic", "config": {"workloads": out}}
+    line = {"metric": "largest per-probe relative L2 vs FP64 (extended form; precision form in config)",
+            "value": max(r["extended"]["max_probe_rel_l2"] for r in out),
+            "unit": "relative L2", "n_gpus": 1, "higher_is_better": False, "dtype": dtype_of(capi.FORM_EXTENDED_FP32),
+            "data": "synthetic", "config": {"workloads": out,
+                                            "precision_form_max_probe_rel_l2":
+                                                max(r["precision"]["max_probe_rel_l2"] for r in out)}}
     print(json.dumps(line), flush=True)
 
 
@@ -312,13 +327,13 @@ def request_errors(test, ref):
 
 
 def main_serve(a):
-    """Poisson stream of mixed requests served four ways on the same arrivals:
-    continuous batching in the precision form (the product), one request at
-    a time in plain FP64 (the north-star baseline), one at a time in the
-    precision form and batched in plain FP64 (same-form baselines that
-    separate the gain of batching from the gain of FP32 links).  p50/p99
+    """Poisson stream of mixed requests served six ways on the same arrivals:
+    continuous batching in the precision and extended forms, one request at
+    a time in plain FP64 (the baseline), and the same-form baselines (one at
+    a time in the precision and extended forms, batched in plain FP64) that
+    separate the gain of batching from the gain of narrower links.  p50/p99
     latency and solves/s for each; per-request relative L2 of the batched
-    precision series against the one-at-a-time FP64 series."""
+    precision and extended series against the one-at-a-time FP64 series."""
     from ggi_tlm_amd import serving
 
     rng = np.random.default_rng(a.seed)
@@ -328,7 +343,7 @@ def main_serve(a):
     print(f"[bench] serve: {len(tables)} requests, tables in {time.perf_counter() - t_setup:.1f} s",
           file=sys.stderr, flush=True)
     lib = capi.engine_library()
-    P32, P64 = capi.FORM_PRECISION_FP32, capi.FORM_PLAIN_FP64
+    P32, P64, PX = capi.FORM_PRECISION_FP32, capi.FORM_PLAIN_FP64, capi.FORM_EXTENDED_FP32
     out = {}
     for rate in a.rates:
         arr = serving.poisson_arrivals(len(tables), rate, np.random.default_rng(a.seed + int(rate)))
@@ -339,7 +354,8 @@ def main_serve(a):
 
         runs = {}
         for key, form, batched in (("batched_precision", P32, True), ("one_at_a_time_fp64", P64, False),
-                                   ("one_at_a_time_precision", P32, False), ("batched_fp64", P64, True)):
+                                   ("one_at_a_time_precision", P32, False), ("batched_fp64", P64, True),
+                                   ("batched_extended", PX, True), ("one_at_a_time_extended", PX, False)):
             reqs = stream()
             if batched:
                 st = serving.run_batched(reqs, form, chunk=a.chunk, lib=lib)
@@ -348,21 +364,27 @@ def main_serve(a):
             runs[key] = (st, reqs)
             print(f"[bench] rate {rate}: {key} {st}", file=sys.stderr, flush=True)
         ref = runs["one_at_a_time_fp64"][1]
-        own, peak = request_errors(runs["batched_precision"][1], ref)
         same64 = all(np.array_equal(x.outputs, y.outputs) for x, y in zip(runs["batched_fp64"][1], ref))
         sps = {k: v[0]["solves_per_s"] for k, v in runs.items()}
+
+        def errors(key):
+            own, peak = request_errors(runs[key][1], ref)
+            return {"own_norm_max": float(f"{max(own):.3e}"), "own_norm_median": float(f"{np.median(own):.3e}"),
+                    "requests_within_1e-5": int(sum(e <= 1e-5 for e in own)),
+                    "peak_norm_max": float(f"{max(peak):.3e}"),
+                    "requests_within_1e-5_of_peak": int(sum(e <= 1e-5 for e in peak)),
+                    "own_norm": [float(f"{e:.2e}") for e in own]}
+
         out[str(rate)] = {
             **{k: v[0] for k, v in runs.items()},
             "speedup_vs_one_at_a_time_fp64": round(sps["batched_precision"] / sps["one_at_a_time_fp64"], 2),
+            "speedup_extended_vs_one_at_a_time_fp64": round(sps["batched_extended"] / sps["one_at_a_time_fp64"], 2),
             "speedup_same_form_precision": round(sps["batched_precision"] / sps["one_at_a_time_precision"], 2),
+            "speedup_same_form_extended": round(sps["batched_extended"] / sps["one_at_a_time_extended"], 2),
             "speedup_same_form_fp64": round(sps["batched_fp64"] / sps["one_at_a_time_fp64"], 2),
             "batched_fp64_bitwise_equal_to_one_at_a_time": bool(same64),
-            "precision_vs_fp64_rel_l2_per_request": {
-                "own_norm_max": float(f"{max(own):.3e}"), "own_norm_median": float(f"{np.median(own):.3e}"),
-                "requests_within_1e-5": int(sum(e <= 1e-5 for e in own)),
-                "peak_norm_max": float(f"{max(peak):.3e}"),
-                "requests_within_1e-5_of_peak": int(sum(e <= 1e-5 for e in peak)),
-                "own_norm": [float(f"{e:.2e}") for e in own]},
+            "precision_vs_fp64_rel_l2_per_request": errors("batched_precision"),
+            "extended_vs_fp64_rel_l2_per_request": errors("batched_extended"),
         }
     top = out[str(max(a.rates))]
     line = {
@@ -381,7 +403,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--form", choices=("precision", "plain"), default="precision")
+    ap.add_argument("--form", choices=tuple(FORM_NAMES), default="precision")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=30)
@@ -425,7 +447,7 @@ def main():
         desc = (f"{p.nx}^3 cells, 8-cell PML, eps_r=4 sigma=0.01 sphere r=60, Huygens plane wave, "
                 f"16 probes, {a.steps} timed steps")
     t = build_tables(p)
-    form = capi.FORM_PRECISION_FP32 if a.form == "precision" else capi.FORM_PLAIN_FP64
+    form = FORM_NAMES[a.form]
     h = capi.Handle(capi.engine_library(), t, form, output_capacity=1024)
     setup_s = time.perf_counter() - t_setup
     print(f"[bench] set-up {setup_s:.1f} s; cells {t.nx * t.ny * t.nz}, PML {t.n_pml_cells}, "
@@ -492,8 +514,11 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": desc,
-            "form": "precision: FP32 link planes, FP64 arithmetic and states" if form else "plain FP64",
-            "link_bytes": 4 if form else 8,
+            "form": {capi.FORM_PLAIN_FP64: "plain FP64",
+                     capi.FORM_PRECISION_FP32: "precision: FP32 link planes, FP64 arithmetic and states",
+                     capi.FORM_EXTENDED_FP32: "extended: FP32 link planes plus an 8-bit residual plane (32-bit "
+                                              "significand), FP64 arithmetic and states"}[form],
+            "link_bytes": {capi.FORM_PLAIN_FP64: 8, capi.FORM_PRECISION_FP32: 4, capi.FORM_EXTENDED_FP32: 5}[form],
             "cells": N,
             "pml_cells": t.n_pml_cells,
             "special_cells": t.n_special_cells,

@@ -20,6 +20,7 @@ ENGINE_SO = os.path.join(HERE, "libtlmscn.so")
 
 FORM_PLAIN_FP64 = 0
 FORM_PRECISION_FP32 = 1
+FORM_EXTENDED_FP32 = 2   # FP32 links plus an 8-bit residual plane (32-bit significand)
 
 
 class TlmGrid(C.Structure):

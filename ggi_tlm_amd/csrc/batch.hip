@@ -100,7 +100,7 @@ int bfail(tlm_batch* b, int code, const std::string& msg) {
 // Upper bound of a solve's device bytes before its tables are built: two
 // link buffers, class tables, PML and filter states, output series.
 int64_t estimate_bytes(const tlm_handle* h, int n_steps) {
-  const int64_t ls = link_size(h);
+  const int64_t ls = link_bytes(h);
   int64_t b = 3 * 12 * ((int64_t)h->N + 64 * TLM_MAX_SEGMENTS + TLM_PLANE_PAD) * ls;  // link planes, padding
   b += (int64_t)h->N / 8 + (int64_t)h->N / 8;            // mask, line table
   const int64_t n_pml = (int64_t)h->pml_cell_param.size();
@@ -312,7 +312,8 @@ __global__ void kb_advance(BatchSolve* S, int n) {
   s.ts += 1;
 }
 
-bool batch_lds(const tlm_batch* b) { return b->mixed != 0; }  // LDS-staged unless lane slots are asked for
+// LDS-staged unless lane slots are asked for (the extended form has the LDS-staged kernel only)
+bool batch_lds(const tlm_batch* b) { return b->mixed != 0 || b->form == TLM_FORM_EXTENDED_FP32; }
 
 // One batch step.  With two streams the class kernels (mixed lines,
 // materials, generic cells, PML cells holding a material) run on the side
@@ -337,7 +338,7 @@ void launch_batch_step(tlm_batch* b) {
   if (batch_lds(b)) {
     go(cs, tlm_batch::T_MIXED, kb_mixed_lds<T, PLAIN, scn::kMixedMasks[0]>, scn::kLdsCells);
     go(cs, tlm_batch::T_MIXED_P, kb_mixed_lds<T, PLAIN, scn::kMixedMasks[1]>, scn::kLdsCells);
-  } else {
+  } else if constexpr (!std::is_same<T, scn::xlink>::value) {
     go(cs, tlm_batch::T_MIXED, kb_mixed<T, PLAIN>);
   }
   go(cs, tlm_batch::T_MATL, kb_mat_lines<T, PLAIN>, scn::kChunk);
@@ -463,8 +464,7 @@ int capture(tlm_batch* b, int k) {
   hipGraph_t g = nullptr;
   BHIP_TRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < k; i++) {
-    if (b->form == TLM_FORM_PLAIN_FP64) launch_batch_step<double, true>(b);
-    else launch_batch_step<float, false>(b);
+    with_form(b->form, [&](auto t, auto plain) { launch_batch_step<decltype(t), decltype(plain)::value>(b); });
   }
   hipError_t e = hipStreamEndCapture(b->stream, &g);
   if (e != hipSuccess) return bfail(b, TLM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -495,8 +495,7 @@ int run_steps(tlm_batch* b, int steps) {
       BHIP_TRY(b, hipGraphLaunch(b->graph, b->stream));
     } else {
       for (int i = 0; i < k; i++) {
-        if (b->form == TLM_FORM_PLAIN_FP64) launch_batch_step<double, true>(b);
-        else launch_batch_step<float, false>(b);
+        with_form(b->form, [&](auto t, auto plain) { launch_batch_step<decltype(t), decltype(plain)::value>(b); });
       }
       BHIP_TRY(b, hipGetLastError());
     }
@@ -513,7 +512,7 @@ extern "C" {
 int tlm_batch_create(int32_t form, int64_t hbm_budget_bytes, tlm_batch** out) {
   if (!out) return TLM_ERR_ARG;
   *out = nullptr;
-  if (form != TLM_FORM_PLAIN_FP64 && form != TLM_FORM_PRECISION_FP32) return TLM_ERR_ARG;
+  if (!valid_form(form)) return TLM_ERR_ARG;
   if (hbm_budget_bytes <= 0) return TLM_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return TLM_ERR_HIP;
@@ -658,6 +657,8 @@ int tlm_batch_set_kernels(tlm_batch* b, int32_t mixed, int32_t streams) {
   if (!b) return TLM_ERR_ARG;
   if (mixed < -1 || mixed > 1 || (streams != 1 && streams != 2))
     return bfail(b, TLM_ERR_ARG, "mixed must be -1, 0 or 1 and streams 1 or 2");
+  if (mixed == 0 && b->form == TLM_FORM_EXTENDED_FP32)
+    return bfail(b, TLM_ERR_UNSUPPORTED, "the extended form runs the LDS-staged mixed-line kernel only");
   if (mixed != b->mixed) b->dirty = true;  // the mixed-line block table changes
   b->mixed = mixed;
   if (streams != b->streams) {

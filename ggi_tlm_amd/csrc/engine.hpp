@@ -10,6 +10,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "scn_kernels.hpp"
@@ -212,7 +213,23 @@ inline int dev_alloc(tlm_handle* h, size_t bytes, void** out) {
   return TLM_OK;
 }
 
+// Element size of a link plane (the 128-byte line holds 128 / link_size
+// cells) and the bytes one link takes in HBM (the extended form adds its
+// residual byte).
 inline size_t link_size(const tlm_handle* h) { return h->form == TLM_FORM_PLAIN_FP64 ? sizeof(double) : sizeof(float); }
+inline size_t link_bytes(const tlm_handle* h) { return link_size(h) + (h->form == TLM_FORM_EXTENDED_FP32 ? 1 : 0); }
+inline bool valid_form(int form) {
+  return form == TLM_FORM_PLAIN_FP64 || form == TLM_FORM_PRECISION_FP32 || form == TLM_FORM_EXTENDED_FP32;
+}
+
+// f(T{}, std::integral_constant<bool, PLAIN>{}) with the link storage type
+// of the form: double (plain), float (precision) or scn::xlink (extended).
+template <typename F>
+inline auto with_form(int form, F&& f) {
+  if (form == TLM_FORM_PLAIN_FP64) return f(double{}, std::true_type{});
+  if (form == TLM_FORM_EXTENDED_FP32) return f(scn::xlink{}, std::false_type{});
+  return f(float{}, std::false_type{});
+}
 
 // Defined in tlmscn.hip.
 int finalize(tlm_handle* h);

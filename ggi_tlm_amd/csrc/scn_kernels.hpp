@@ -332,9 +332,114 @@ __device__ inline void face_excitation(const Dev& d, int row, int timestep, doub
 // incident voltages (connect + outer boundary of the previous step)
 // ---------------------------------------------------------------------------
 
+// Extended link storage (TLM_FORM_EXTENDED_FP32): a buffer holds twelve
+// FP32 planes followed by twelve int8 planes.  Byte r of value i refines its
+// FP32 value h: V = h + r * u(h), u(h) = ulp(h) / 254.  h is V rounded to
+// nearest, so |V - h| <= ulp(h)/2 = 127 u(h) and r = rint((V - h)/u) never
+// leaves [-127, 127]; the stored value is within u/2 = ulp(h)/508 of V,
+// about 2^-32 relative (FP32 alone: 2^-24), in 5 bytes.  Every decoder
+// evaluates the same expression, so a stored link reads back with the same
+// bits in every kernel.  xlink has no arithmetic and no conversions: every
+// access to such a buffer goes through lk_get / lk_put, with the buffer's
+// base pointer and plane stride P.
+struct xlink {
+  float hi;
+};
+static_assert(sizeof(xlink) == 4, "xlink is the FP32 plane element");
+
+// Register type of a loaded link value: the storage type itself for FP64
+// and FP32 planes, FP64 for the extended form.
+template <typename T> struct LinkReg { typedef T type; };
+template <> struct LinkReg<xlink> { typedef double type; };
+
+// ulp(h) = 2^(e - 150) for the biased FP32 exponent e of h (e = 1 for zero
+// and subnormals, whose spacing is that of the smallest normal binade).
+__device__ inline int xl_exp(float h) {
+  const int e = (int)((__float_as_uint(h) >> 23) & 0xffu);
+  return e ? e : 1;
+}
+__device__ inline double xl_unit(float h) {
+  return __longlong_as_double((long long)(xl_exp(h) + 873) << 52) * (1.0 / 254.0);
+}
+
+// Split v into its nearest FP32 value and the residual byte (v - h is exact:
+// h is v rounded once; (v - h) * 254 / ulp(h) is exact, a product by 254 and
+// a power of two).
+__device__ inline void xl_split(double v, float& h, int8_t& r) {
+  h = (float)v;
+  const double inv = __longlong_as_double((long long)(1173 - xl_exp(h)) << 52) * 254.0;  // 1/u(h)
+  r = (int8_t)rint((v - (double)h) * inv);
+}
+
+// Residual byte of port p of cell c: the residual planes group four ports
+// per 32-bit word, word w = p / 4 of cell c at word index w * P + c, so a
+// cell's x-, y- and z-facing port residuals are one word each (loads and
+// stores of whole words; 64 lanes write whole 128-byte lines).
+__device__ inline long long xl_byte(long long P, int p, long long c) {
+  return (((long long)(p >> 2) * P + c) << 2) + (p & 3);
+}
+
+// Link value of port p of cell c.
+template <typename T>
+__device__ inline typename LinkReg<T>::type lk_get(const T* __restrict__ L, long long P, int p, long long c) {
+  return L[(long long)p * P + c];
+}
+__device__ inline double lk_get(const xlink* __restrict__ L, long long P, int p, long long c) {
+  const float h = reinterpret_cast<const float*>(L)[(long long)p * P + c];
+  const int8_t r = reinterpret_cast<const int8_t*>(L + 12 * P)[xl_byte(P, p, c)];
+  return (double)h + (double)r * xl_unit(h);
+}
+
+// Stores of scattered values (non-temporal: read again only next step) and
+// plain stores (uploads).
+template <typename T>
+__device__ inline void lk_put_nt(T* __restrict__ L, long long P, int p, long long c, double v) {
+  __builtin_nontemporal_store((T)v, &L[(long long)p * P + c]);
+}
+__device__ inline void lk_put_nt(xlink* __restrict__ L, long long P, int p, long long c, double v) {
+  float h;
+  int8_t r;
+  xl_split(v, h, r);
+  __builtin_nontemporal_store(h, reinterpret_cast<float*>(L) + (long long)p * P + c);
+  __builtin_nontemporal_store(r, reinterpret_cast<int8_t*>(L + 12 * P) + xl_byte(P, p, c));
+}
+template <typename T>
+__device__ inline void lk_put(T* __restrict__ L, long long P, int p, long long c, double v) {
+  L[(long long)p * P + c] = (T)v;
+}
+__device__ inline void lk_put(xlink* __restrict__ L, long long P, int p, long long c, double v) {
+  float h;
+  int8_t r;
+  xl_split(v, h, r);
+  reinterpret_cast<float*>(L)[(long long)p * P + c] = h;
+  reinterpret_cast<int8_t*>(L + 12 * P)[xl_byte(P, p, c)] = r;
+}
+
+// All twelve link values of cell c, stored non-temporally.  The extended
+// form assembles each residual word in registers and stores it whole.
+template <typename T>
+__device__ inline void store_links(T* __restrict__ L, long long P, long long c, const double v[12]) {
+#pragma unroll
+  for (int p = 0; p < 12; p++) lk_put_nt(L, P, p, c, v[p]);
+}
+__device__ inline void store_links(xlink* __restrict__ L, long long P, long long c, const double v[12]) {
+  uint32_t w[3] = {0u, 0u, 0u};
+#pragma unroll
+  for (int p = 0; p < 12; p++) {
+    float h;
+    int8_t r;
+    xl_split(v[p], h, r);
+    __builtin_nontemporal_store(h, reinterpret_cast<float*>(L) + (long long)p * P + c);
+    w[p >> 2] |= (uint32_t)(uint8_t)r << (8 * (p & 3));
+  }
+  uint32_t* R = reinterpret_cast<uint32_t*>(L + 12 * P);
+#pragma unroll
+  for (int k = 0; k < 3; k++) __builtin_nontemporal_store(w[k], R + (long long)k * P + c);
+}
+
 template <typename T>
 __device__ inline double ld(const T* __restrict__ L, long long P, int port, long long c) {
-  return (double)L[(long long)port * P + c];
+  return (double)lk_get(L, P, port, c);
 }
 
 // Ordinary face: S = V_min + V_max; incident = S - own
@@ -621,8 +726,7 @@ __device__ inline void scatter_free_f32(float v[12]) {
 // faster steps in both forms).
 template <typename T>
 __device__ inline void store_cell(T* __restrict__ L, long long P, long long c, const double v[12]) {
-#pragma unroll
-  for (int p = 0; p < 12; p++) __builtin_nontemporal_store((T)v[p], &L[(long long)p * P + c]);
+  store_links(L, P, c, v);
 }
 
 // evaluate_Zfilter / timeshift_Zfilter (FILTER/filter_subroutines.F90:510-575)
@@ -681,17 +785,17 @@ __device__ inline void bulk_own(const Dev& d, const T* __restrict__ Lin, long lo
                                 O o[12]) {
   if (PLAIN) {
 #pragma unroll
-    for (int p = 0; p < 12; p++) o[p] = (O)Lin[p * P + c];
+    for (int p = 0; p < 12; p++) o[p] = (O)lk_get(Lin, P, p, c);
     return;
   }
-  o[Vy_xmin] = (O)Lin[Vy_xmin * P + c]; o[Vz_xmin] = (O)Lin[Vz_xmin * P + c];
-  o[Vy_xmax] = (O)Lin[Vy_xmax * P + c]; o[Vz_xmax] = (O)Lin[Vz_xmax * P + c];
+  o[Vy_xmin] = (O)lk_get(Lin, P, Vy_xmin, c); o[Vz_xmin] = (O)lk_get(Lin, P, Vz_xmin, c);
+  o[Vy_xmax] = (O)lk_get(Lin, P, Vy_xmax, c); o[Vz_xmax] = (O)lk_get(Lin, P, Vz_xmax, c);
   o[Vx_ymin] = o[Vz_ymin] = o[Vx_ymax] = o[Vz_ymax] = (O)0;
   o[Vx_zmin] = o[Vy_zmin] = o[Vx_zmax] = o[Vy_zmax] = (O)0;
-  if (y == 0) { o[Vx_ymin] = (O)Lin[Vx_ymin * P + c]; o[Vz_ymin] = (O)Lin[Vz_ymin * P + c]; }
-  if (y == d.ny - 1) { o[Vx_ymax] = (O)Lin[Vx_ymax * P + c]; o[Vz_ymax] = (O)Lin[Vz_ymax * P + c]; }
-  if (z == 0) { o[Vx_zmin] = (O)Lin[Vx_zmin * P + c]; o[Vy_zmin] = (O)Lin[Vy_zmin * P + c]; }
-  if (z == d.nz - 1) { o[Vx_zmax] = (O)Lin[Vx_zmax * P + c]; o[Vy_zmax] = (O)Lin[Vy_zmax * P + c]; }
+  if (y == 0) { o[Vx_ymin] = (O)lk_get(Lin, P, Vx_ymin, c); o[Vz_ymin] = (O)lk_get(Lin, P, Vz_ymin, c); }
+  if (y == d.ny - 1) { o[Vx_ymax] = (O)lk_get(Lin, P, Vx_ymax, c); o[Vz_ymax] = (O)lk_get(Lin, P, Vz_ymax, c); }
+  if (z == 0) { o[Vx_zmin] = (O)lk_get(Lin, P, Vx_zmin, c); o[Vy_zmin] = (O)lk_get(Lin, P, Vy_zmin, c); }
+  if (z == d.nz - 1) { o[Vx_zmax] = (O)lk_get(Lin, P, Vx_zmax, c); o[Vy_zmax] = (O)lk_get(Lin, P, Vy_zmax, c); }
 }
 
 template <bool PLAIN>
@@ -739,16 +843,16 @@ __device__ inline void bulk_pull(const Dev& d, const T* __restrict__ L, const Pu
   double xa = 0, xb = 0, xc = 0, xd = 0, ya = 0, yb = 0, yc = 0, yd = 0, za = 0, zb = 0, zc_ = 0, zd = 0;
   if (x > 0) {
     const long long n = nb_xm(d, c, x, y, z, sg);
-    xa = L[Vy_xmax * P + n]; xb = L[Vz_xmax * P + n];
+    xa = lk_get(L, P, Vy_xmax, n); xb = lk_get(L, P, Vz_xmax, n);
   }
   if (x + 1 < d.nx) {
     const long long n = nb_xp(d, c, x, y, z, sg);
-    xc = L[Vy_xmin * P + n]; xd = L[Vz_xmin * P + n];
+    xc = lk_get(L, P, Vy_xmin, n); xd = lk_get(L, P, Vz_xmin, n);
   }
-  if (y > 0) { ya = L[Vx_ymax * P + c - sy]; yb = L[Vz_ymax * P + c - sy]; }
-  if (y + 1 < d.ny) { yc = L[Vx_ymin * P + c + sy]; yd = L[Vz_ymin * P + c + sy]; }
-  if (z > 0) { za = L[Vx_zmax * P + c - sz]; zb = L[Vy_zmax * P + c - sz]; }
-  if (z + 1 < d.nz) { zc_ = L[Vx_zmin * P + c + sz]; zd = L[Vy_zmin * P + c + sz]; }
+  if (y > 0) { ya = lk_get(L, P, Vx_ymax, c - sy); yb = lk_get(L, P, Vz_ymax, c - sy); }
+  if (y + 1 < d.ny) { yc = lk_get(L, P, Vx_ymin, c + sy); yd = lk_get(L, P, Vz_ymin, c + sy); }
+  if (z > 0) { za = lk_get(L, P, Vx_zmax, c - sz); zb = lk_get(L, P, Vy_zmax, c - sz); }
+  if (z + 1 < d.nz) { zc_ = lk_get(L, P, Vx_zmin, c + sz); zd = lk_get(L, P, Vy_zmin, c + sz); }
   bulk_incident<PLAIN>(d, a, x, y, z, o, xa, xb, xc, xd, ya, yb, yc, yd, za, zb, zc_, zd, v);
 }
 
@@ -762,37 +866,38 @@ __device__ inline void bulk_pull_pair(const Dev& d, const T* __restrict__ L, con
   const T* __restrict__ Q = (const T*)a.prev;
   const long long P = d.P;
   const long long sy = seg_width(d, sg), sz = sy * d.ny;
-  T o[12], q[12];
+  typedef typename LinkReg<T>::type R;
+  R o[12], q[12];
   bulk_own<PLAIN>(d, L, P, c, y, z, o);
   bulk_own<PLAIN>(d, Q, P, c, y, z, q);
-  T m[12], r[12];
+  R m[12], r[12];
 #pragma unroll
-  for (int k = 0; k < 12; k++) m[k] = r[k] = (T)0;
+  for (int k = 0; k < 12; k++) m[k] = r[k] = (R)0;
   if (x > 0) {
     const long long n = nb_xm(d, c, x, y, z, sg);
-    m[0] = L[Vy_xmax * P + n]; m[1] = L[Vz_xmax * P + n];
-    r[0] = Q[Vy_xmax * P + n]; r[1] = Q[Vz_xmax * P + n];
+    m[0] = lk_get(L, P, Vy_xmax, n); m[1] = lk_get(L, P, Vz_xmax, n);
+    r[0] = lk_get(Q, P, Vy_xmax, n); r[1] = lk_get(Q, P, Vz_xmax, n);
   }
   if (x + 1 < d.nx) {
     const long long n = nb_xp(d, c, x, y, z, sg);
-    m[2] = L[Vy_xmin * P + n]; m[3] = L[Vz_xmin * P + n];
-    r[2] = Q[Vy_xmin * P + n]; r[3] = Q[Vz_xmin * P + n];
+    m[2] = lk_get(L, P, Vy_xmin, n); m[3] = lk_get(L, P, Vz_xmin, n);
+    r[2] = lk_get(Q, P, Vy_xmin, n); r[3] = lk_get(Q, P, Vz_xmin, n);
   }
   if (y > 0) {
-    m[4] = L[Vx_ymax * P + c - sy]; m[5] = L[Vz_ymax * P + c - sy];
-    r[4] = Q[Vx_ymax * P + c - sy]; r[5] = Q[Vz_ymax * P + c - sy];
+    m[4] = lk_get(L, P, Vx_ymax, c - sy); m[5] = lk_get(L, P, Vz_ymax, c - sy);
+    r[4] = lk_get(Q, P, Vx_ymax, c - sy); r[5] = lk_get(Q, P, Vz_ymax, c - sy);
   }
   if (y + 1 < d.ny) {
-    m[6] = L[Vx_ymin * P + c + sy]; m[7] = L[Vz_ymin * P + c + sy];
-    r[6] = Q[Vx_ymin * P + c + sy]; r[7] = Q[Vz_ymin * P + c + sy];
+    m[6] = lk_get(L, P, Vx_ymin, c + sy); m[7] = lk_get(L, P, Vz_ymin, c + sy);
+    r[6] = lk_get(Q, P, Vx_ymin, c + sy); r[7] = lk_get(Q, P, Vz_ymin, c + sy);
   }
   if (z > 0) {
-    m[8] = L[Vx_zmax * P + c - sz]; m[9] = L[Vy_zmax * P + c - sz];
-    r[8] = Q[Vx_zmax * P + c - sz]; r[9] = Q[Vy_zmax * P + c - sz];
+    m[8] = lk_get(L, P, Vx_zmax, c - sz); m[9] = lk_get(L, P, Vy_zmax, c - sz);
+    r[8] = lk_get(Q, P, Vx_zmax, c - sz); r[9] = lk_get(Q, P, Vy_zmax, c - sz);
   }
   if (z + 1 < d.nz) {
-    m[10] = L[Vx_zmin * P + c + sz]; m[11] = L[Vy_zmin * P + c + sz];
-    r[10] = Q[Vx_zmin * P + c + sz]; r[11] = Q[Vy_zmin * P + c + sz];
+    m[10] = lk_get(L, P, Vx_zmin, c + sz); m[11] = lk_get(L, P, Vy_zmin, c + sz);
+    r[10] = lk_get(Q, P, Vx_zmin, c + sz); r[11] = lk_get(Q, P, Vy_zmin, c + sz);
   }
   double od[12], qd[12];
 #pragma unroll
@@ -813,7 +918,7 @@ __device__ inline void bulk_update(const Dev& d, const T* __restrict__ Lin, T* _
     bulk_pull<T, PLAIN>(d, Lin, a, c, x, y, z, sg, v);
   } else {
 #pragma unroll
-    for (int p = 0; p < 12; p++) v[p] = (double)Lin[p * P + c];
+    for (int p = 0; p < 12; p++) v[p] = (double)lk_get(Lin, P, p, c);
   }
   flush_small(v, a.small);
   if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
@@ -1092,7 +1197,7 @@ __device__ inline void pml_prev_incident(const Dev& d, const PullArgs& a, long l
     bulk_pull<T, PLAIN>(d, L, b, c, x, y, z, sg, w);
   } else {
 #pragma unroll
-    for (int p = 0; p < 12; p++) w[p] = (double)L[p * d.P + c];
+    for (int p = 0; p < 12; p++) w[p] = (double)lk_get(L, d.P, p, c);
   }
   flush_small(w, a.prev_small);
 }
@@ -1796,7 +1901,7 @@ __device__ inline void mat_line_cell(const Dev& d, const T* __restrict__ Lin, T*
     bulk_pull<T, PLAIN>(d, Lin, a, c, x, y, z, sg, v);
   } else {
 #pragma unroll
-    for (int p = 0; p < 12; p++) v[p] = (double)Lin[p * d.P + c];
+    for (int p = 0; p < 12; p++) v[p] = (double)lk_get(Lin, d.P, p, c);
   }
   flush_small(v, a.small);
   if ((lanes >> lane) & 1u) {
@@ -2013,8 +2118,10 @@ __device__ inline void mixed_lds_body(const Dev& d, const T* __restrict__ Lin, T
   }
   __syncthreads();
   if (mine) {
+    double v[12];
 #pragma unroll
-    for (int k = 0; k < 12; k++) __builtin_nontemporal_store((T)sv[k * C + t], &Lout[(long long)k * d.P + c]);
+    for (int k = 0; k < 12; k++) v[k] = sv[k * C + t];
+    store_links(Lout, d.P, c, v);
   }
 }
 
@@ -2073,7 +2180,7 @@ __global__ void k_gather_host(const T* __restrict__ L, long long P, const int32_
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const long long c = cells[k];
-  for (int p = 0; p < 12; p++) buf[12 * (long long)k + p] = (double)L[p * P + c];
+  for (int p = 0; p < 12; p++) buf[12 * (long long)k + p] = (double)lk_get(L, P, p, c);
 }
 
 template <typename T>
@@ -2082,7 +2189,7 @@ __global__ void k_scatter_host(T* __restrict__ L, long long P, const int32_t* __
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const long long c = cells[k];
-  for (int p = 0; p < 12; p++) L[p * P + c] = (T)buf[12 * (long long)k + p];
+  for (int p = 0; p < 12; p++) lk_put(L, P, p, c, buf[12 * (long long)k + p]);
 }
 
 // The four scattered pulses beside each host-path face (tlm_gather_host_faces):
@@ -2098,10 +2205,10 @@ __global__ void k_gather_host_faces(const T* __restrict__ L, long long P, const 
   const int pb = a == 0 ? Vz_xmin : a == 1 ? Vz_ymin : Vy_zmin;
   const int qa = a == 0 ? Vy_xmax : a == 1 ? Vx_ymax : Vx_zmax;
   const int qb = a == 0 ? Vz_xmax : a == 1 ? Vz_ymax : Vy_zmax;
-  buf[4 * (long long)k + 0] = (double)L[pa * P + cp[k]];
-  buf[4 * (long long)k + 1] = (double)L[qa * P + cm[k]];
-  buf[4 * (long long)k + 2] = (double)L[pb * P + cp[k]];
-  buf[4 * (long long)k + 3] = (double)L[qb * P + cm[k]];
+  buf[4 * (long long)k + 0] = (double)lk_get(L, P, pa, cp[k]);
+  buf[4 * (long long)k + 1] = (double)lk_get(L, P, qa, cm[k]);
+  buf[4 * (long long)k + 2] = (double)lk_get(L, P, pb, cp[k]);
+  buf[4 * (long long)k + 3] = (double)lk_get(L, P, qb, cm[k]);
 }
 
 // Probe DFT: acc(p,k) += v_p(n) exp(-j 2 pi f_k t_n) dt, t_n = (n-1) dt.
@@ -2129,7 +2236,7 @@ __global__ void k_aos_to_planes(Dev d, const double* __restrict__ V, T* __restri
   if (!cell_coords(d, c, x, y, z, s)) return;
   const long long fi =
       12 * ((long long)(x + 1) + (long long)(d.nx + 2) * ((long long)(y + 1) + (long long)(d.ny + 2) * z));
-  for (int p = 0; p < 12; p++) L[p * d.P + c] = (T)V[fi + p];
+  for (int p = 0; p < 12; p++) lk_put(L, d.P, p, c, V[fi + p]);
 }
 
 template <typename T>
@@ -2140,7 +2247,7 @@ __global__ void k_planes_to_aos(Dev d, const T* __restrict__ L, double* __restri
   if (!cell_coords(d, c, x, y, z, s)) return;
   const long long fi =
       12 * ((long long)(x + 1) + (long long)(d.nx + 2) * ((long long)(y + 1) + (long long)(d.ny + 2) * z));
-  for (int p = 0; p < 12; p++) V[fi + p] = (double)L[p * d.P + c];
+  for (int p = 0; p < 12; p++) V[fi + p] = (double)lk_get(L, d.P, p, c);
 }
 
 // k_planes_to_aos for the z-planes [z0, z1) only, into a slab array laid
@@ -2157,7 +2264,7 @@ __global__ void k_planes_to_aos_slab(Dev d, const T* __restrict__ L, double* __r
   const long long c = storage_index(d, x, y, z0 + zl, s);
   const long long fi =
       12 * ((long long)(x + 1) + (long long)(d.nx + 2) * ((long long)(y + 1) + (long long)(d.ny + 2) * zl));
-  for (int p = 0; p < 12; p++) V[fi + p] = (double)L[p * d.P + c];
+  for (int p = 0; p < 12; p++) V[fi + p] = (double)lk_get(L, d.P, p, c);
 }
 
 }  // namespace scn

@@ -648,7 +648,7 @@ int finalize(tlm_handle* h) {
   d.n_timesteps = h->g.n_timesteps;
   d.R[0] = h->g.r_xmin; d.R[1] = h->g.r_xmax; d.R[2] = h->g.r_ymin;
   d.R[3] = h->g.r_ymax; d.R[4] = h->g.r_zmin; d.R[5] = h->g.r_zmax;
-  const size_t ls = link_size(h);
+  const size_t ls = link_bytes(h);
   h->n_bufs = pml_cell.empty() ? 2 : 3;
   for (int b = 0; b < h->n_bufs; b++) {
     if ((rc = dev_alloc(h, 12 * (size_t)h->P * ls, &h->planes[b]))) return rc;
@@ -1011,8 +1011,11 @@ int reserve_face_slot(tlm_handle* h, int timestep, int* slot) {
 // chosen otherwise per handle; variant 0: 64 x rows tiles of one plane;
 // variant 1: z-marching blocks of zc planes; variant 2: 16-byte vector
 // accesses.
+bool ext_form(const tlm_handle* h) { return h->form == TLM_FORM_EXTENDED_FP32; }
+
 int sched_variant(const tlm_handle* h) {
   if (h->stz_n > 0) return 3;  // the line kernel applies the small-value flush
+  if (h->form == TLM_FORM_EXTENDED_FP32) return 3;  // the only bulk kernel built for it
   if (h->sched_variant >= 0) return h->sched_variant;
   return 3;
 }
@@ -1076,10 +1079,12 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
   }
   {
     ProfScope ps(h, TLM_KCLASS_BULK);
-    if (sched_variant(h) == 3) {
+    constexpr bool ext = std::is_same<T, scn::xlink>::value;
+    if (ext || sched_variant(h) == 3) {
       // one storage block per workgroup over the whole storage (measured
-      // faster than a block table), or the block tables
-      const bool yband = h->sched_xcd == 2, direct = !yband && h->sched_by == 1;
+      // faster than a block table), or the block tables; the extended form
+      // has the chunk kernel only (tlm_set_schedule refuses the others)
+      const bool yband = !ext && h->sched_xcd == 2, direct = ext || (!yband && h->sched_by == 1);
       const int32_t* blocks = direct ? nullptr : (yband ? h->d_bulk_blocks_yband : h->d_bulk_blocks);
       const int nb = yband ? h->n_bulk_blocks_yband : (int)h->bulk_blocks.size();
       const int per = h->sched_by;  // storage blocks per workgroup
@@ -1092,6 +1097,7 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
         else
           hipLaunchKernelGGL((scn::k_bulk_chunks<T, PLAIN, false>), dim3(nwg), dim3(scn::kChunk), 0, h->stream, d, Lin,
                              Lout, a, h->d_bulk_chunks, h->sched_xcd);
+      } else if constexpr (ext) {
       } else if (nb) {
         if (pull)
           hipLaunchKernelGGL((scn::k_bulk_blocks<T, PLAIN, true>), dim3(nwg), dim3(256), 0, h->stream, d, Lin, Lout, a,
@@ -1100,7 +1106,7 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
           hipLaunchKernelGGL((scn::k_bulk_blocks<T, PLAIN, false>), dim3(nwg), dim3(256), 0, h->stream, d, Lin, Lout, a,
                              blocks, nb, per, h->sched_xcd);
       }
-    } else {
+    } else if constexpr (!ext) {
       scn::BulkMap m = bulk_map(h);
       dim3 bb(64, m.by, 1);
       const int w = bulk_vec_width(h);
@@ -1117,7 +1123,7 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
       }
     }
   }
-  const int mixed = h->mixed_variant >= 0 ? h->mixed_variant : 1;  // LDS-staged: measured faster in both forms
+  const int mixed = ext_form(h) ? 1 : h->mixed_variant >= 0 ? h->mixed_variant : 1;  // LDS-staged: measured faster
   if (mixed == 1 && h->n_lds_wg) {
     ProfScope ps(h, TLM_KCLASS_MIXED);
     int wg0 = 0;
@@ -1137,10 +1143,11 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
       }
       wg0 += n;
     }
-  } else if (h->n_mixed_blocks) {
+  } else if (mixed == 0 && h->n_mixed_blocks) {
     ProfScope ps(h, TLM_KCLASS_MIXED);
     dim3 g(h->n_mixed_blocks);
-    if (pull)
+    if constexpr (std::is_same<T, scn::xlink>::value) {
+    } else if (pull)
       hipLaunchKernelGGL((scn::k_mixed_lines<T, PLAIN, true>), g, dim3(256), 0, cs, d, Lin, Lout, a,
                          h->d_slot_cell, h->d_slot_idx, (long long)h->n_mixed_blocks * 256, timestep, slot);
     else
@@ -1238,8 +1245,9 @@ int do_scatter(tlm_handle* h, int timestep) {
   if (h->d.n_out_rows > 0 && h->ring_used == h->cap)
     if ((rc = drain_ring(h))) return rc;
   int slot = h->ring_used;
-  if (h->form == TLM_FORM_PLAIN_FP64) rc = launch_scatter<double, true>(h, timestep, pull, a, slot);
-  else rc = launch_scatter<float, false>(h, timestep, pull, a, slot);
+  rc = with_form(h->form, [&](auto t, auto plain) {
+    return launch_scatter<decltype(t), decltype(plain)::value>(h, timestep, pull, a, slot);
+  });
   if (rc) return rc;
   HIP_TRY(h, hipGetLastError());
   if (h->d.n_out_rows > 0) {
@@ -1273,8 +1281,9 @@ int materialize(tlm_handle* h) {
   int fslot = -1;
   if ((rc = reserve_face_slot(h, h->pend_ts, &fslot))) return rc;
   PullArgs a{h->pend_boundary, h->pend_ts, 0.0, fslot};
-  if (h->form == TLM_FORM_PLAIN_FP64) rc = launch_connect<double, true>(h, a);
-  else rc = launch_connect<float, false>(h, a);
+  rc = with_form(h->form, [&](auto t, auto plain) {
+    return launch_connect<decltype(t), decltype(plain)::value>(h, a);
+  });
   if (rc) return rc;
   HIP_TRY(h, hipGetLastError());
   h->cur = out_buffer(h);  // prv stays: the next step still pulls it again
@@ -1330,7 +1339,7 @@ int tlm_create(const tlm_grid* g, int32_t form, tlm_handle** out) {
   *out = nullptr;
   if (g->nx < 1 || g->ny < 1 || g->nz < 1 || g->n_timesteps < 1 || !(g->dl > 0) || !(g->dt > 0))
     return TLM_ERR_ARG;
-  if (form != TLM_FORM_PLAIN_FP64 && form != TLM_FORM_PRECISION_FP32) return TLM_ERR_ARG;
+  if (!valid_form(form)) return TLM_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return TLM_ERR_HIP;
   tlm_handle* h = new (std::nothrow) tlm_handle();
@@ -1526,12 +1535,11 @@ int tlm_upload_links(tlm_handle* h, const double* V) {
   hipError_t e = hipMemcpyAsync(tmp, V, n * sizeof(double), hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess) {
     unsigned g = (unsigned)((h->P + 255) / 256);
-    if (h->form == TLM_FORM_PLAIN_FP64)
-      hipLaunchKernelGGL(scn::k_aos_to_planes<double>, dim3(g), dim3(256), 0, h->stream, h->d, (const double*)tmp,
-                         (double*)h->planes[h->cur]);
-    else
-      hipLaunchKernelGGL(scn::k_aos_to_planes<float>, dim3(g), dim3(256), 0, h->stream, h->d, (const double*)tmp,
-                         (float*)h->planes[h->cur]);
+    with_form(h->form, [&](auto t, auto) {
+      using T = decltype(t);
+      hipLaunchKernelGGL(scn::k_aos_to_planes<T>, dim3(g), dim3(256), 0, h->stream, h->d, (const double*)tmp,
+                         (T*)h->planes[h->cur]);
+    });
     e = hipStreamSynchronize(h->stream);
   }
   (void)hipFree(tmp);
@@ -1552,12 +1560,11 @@ int tlm_download_links(tlm_handle* h, double* V) {
   hipError_t e = hipMemcpyAsync(tmp, V, n * sizeof(double), hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess) {
     unsigned g = (unsigned)((h->P + 255) / 256);
-    if (h->form == TLM_FORM_PLAIN_FP64)
-      hipLaunchKernelGGL(scn::k_planes_to_aos<double>, dim3(g), dim3(256), 0, h->stream, h->d,
-                         (const double*)h->planes[h->cur], (double*)tmp);
-    else
-      hipLaunchKernelGGL(scn::k_planes_to_aos<float>, dim3(g), dim3(256), 0, h->stream, h->d,
-                         (const float*)h->planes[h->cur], (double*)tmp);
+    with_form(h->form, [&](auto t, auto) {
+      using T = decltype(t);
+      hipLaunchKernelGGL(scn::k_planes_to_aos<T>, dim3(g), dim3(256), 0, h->stream, h->d,
+                         (const T*)h->planes[h->cur], (double*)tmp);
+    });
     e = hipMemcpyAsync(V, tmp, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -1589,12 +1596,11 @@ int tlm_download_links_noalloc(tlm_handle* h, double* V) {
     HIP_TRY(h, hipMemcpyAsync(st, V + plane * z0, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
     const long long cells = (long long)h->nx * h->ny * (z1 - z0);
     const dim3 g((unsigned)((cells + 255) / 256));
-    if (h->form == TLM_FORM_PLAIN_FP64)
-      hipLaunchKernelGGL(scn::k_planes_to_aos_slab<double>, g, dim3(256), 0, h->stream, h->d,
-                         (const double*)h->planes[h->cur], st, z0, z1);
-    else
-      hipLaunchKernelGGL(scn::k_planes_to_aos_slab<float>, g, dim3(256), 0, h->stream, h->d,
-                         (const float*)h->planes[h->cur], st, z0, z1);
+    with_form(h->form, [&](auto t, auto) {
+      using T = decltype(t);
+      hipLaunchKernelGGL(scn::k_planes_to_aos_slab<T>, g, dim3(256), 0, h->stream, h->d,
+                         (const T*)h->planes[h->cur], st, z0, z1);
+    });
     HIP_TRY(h, hipGetLastError());
     HIP_TRY(h, hipMemcpyAsync(V + plane * z0, st, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   }
@@ -1663,12 +1669,11 @@ int tlm_gather_host_cells(tlm_handle* h, double* V) {
   if ((rc = host_window(h))) return rc;
   if (!h->n_host) return TLM_OK;
   const dim3 g((h->n_host + 255) / 256);
-  if (h->form == TLM_FORM_PLAIN_FP64)
-    hipLaunchKernelGGL(scn::k_gather_host<double>, g, dim3(256), 0, h->stream, (const double*)h->planes[h->cur], h->P,
+  with_form(h->form, [&](auto t, auto) {
+    using T = decltype(t);
+    hipLaunchKernelGGL(scn::k_gather_host<T>, g, dim3(256), 0, h->stream, (const T*)h->planes[h->cur], h->P,
                        h->d_host_cells, h->n_host, h->d_host_buf);
-  else
-    hipLaunchKernelGGL(scn::k_gather_host<float>, g, dim3(256), 0, h->stream, (const float*)h->planes[h->cur], h->P,
-                       h->d_host_cells, h->n_host, h->d_host_buf);
+  });
   HIP_TRY(h, hipGetLastError());
   HIP_TRY(h, hipMemcpyAsync(V, h->d_host_buf, (size_t)12 * h->n_host * sizeof(double), hipMemcpyDeviceToHost,
                             h->stream));
@@ -1684,12 +1689,11 @@ int tlm_scatter_host_cells(tlm_handle* h, const double* V) {
   HIP_TRY(h, hipMemcpyAsync(h->d_host_buf, V, (size_t)12 * h->n_host * sizeof(double), hipMemcpyHostToDevice,
                             h->stream));
   const dim3 g((h->n_host + 255) / 256);
-  if (h->form == TLM_FORM_PLAIN_FP64)
-    hipLaunchKernelGGL(scn::k_scatter_host<double>, g, dim3(256), 0, h->stream, (double*)h->planes[h->cur], h->P,
+  with_form(h->form, [&](auto t, auto) {
+    using T = decltype(t);
+    hipLaunchKernelGGL(scn::k_scatter_host<T>, g, dim3(256), 0, h->stream, (T*)h->planes[h->cur], h->P,
                        h->d_host_cells, h->n_host, h->d_host_buf);
-  else
-    hipLaunchKernelGGL(scn::k_scatter_host<float>, g, dim3(256), 0, h->stream, (float*)h->planes[h->cur], h->P,
-                       h->d_host_cells, h->n_host, h->d_host_buf);
+  });
   HIP_TRY(h, hipGetLastError());
   // the caller's array may be reused at once: wait for the copy
   HIP_TRY(h, hipStreamSynchronize(h->stream));
@@ -1711,12 +1715,11 @@ int tlm_gather_host_faces(tlm_handle* h, double* V) {
   if ((rc = host_window(h))) return rc;
   if (!h->n_host_faces) return TLM_OK;
   const dim3 g((h->n_host_faces + 255) / 256);
-  if (h->form == TLM_FORM_PLAIN_FP64)
-    hipLaunchKernelGGL(scn::k_gather_host_faces<double>, g, dim3(256), 0, h->stream, (const double*)h->planes[h->cur],
-                       h->P, h->d_hf_cp, h->d_hf_cm, h->d_hf_ax, h->n_host_faces, h->d_hf_buf);
-  else
-    hipLaunchKernelGGL(scn::k_gather_host_faces<float>, g, dim3(256), 0, h->stream, (const float*)h->planes[h->cur],
-                       h->P, h->d_hf_cp, h->d_hf_cm, h->d_hf_ax, h->n_host_faces, h->d_hf_buf);
+  with_form(h->form, [&](auto t, auto) {
+    using T = decltype(t);
+    hipLaunchKernelGGL(scn::k_gather_host_faces<T>, g, dim3(256), 0, h->stream, (const T*)h->planes[h->cur], h->P,
+                       h->d_hf_cp, h->d_hf_cm, h->d_hf_ax, h->n_host_faces, h->d_hf_buf);
+  });
   HIP_TRY(h, hipGetLastError());
   HIP_TRY(h, hipMemcpyAsync(V, h->d_hf_buf, (size_t)4 * h->n_host_faces * sizeof(double), hipMemcpyDeviceToHost,
                             h->stream));
@@ -1858,6 +1861,9 @@ int tlm_set_schedule(tlm_handle* h, int32_t variant, int32_t rows, int32_t zc, i
   if (variant != 3 && h->layout != TLM_LAYOUT_ROWS)
     return fail(h, TLM_ERR_ARG, "the tile schedules (variants 0-2) need TLM_LAYOUT_ROWS");
   if (xcd_remap < 0 || xcd_remap > 2 || (xcd_remap == 2 && variant != 3)) return fail(h, TLM_ERR_ARG, "bad XCD mapping");
+  if (ext_form(h) && (variant != 3 || rows != 1 || xcd_remap == 2 || mixed == 0))
+    return fail(h, TLM_ERR_UNSUPPORTED, "the extended form runs the chunk kernel (variant 3, rows 1, XCD mapping 0 or "
+                                        "1) and the LDS-staged mixed-line kernel only");
   h->mixed_variant = mixed;
   h->sched_variant = variant;
   h->sched_by = rows;
@@ -1874,8 +1880,9 @@ int tlm_autotune(tlm_handle* h, int32_t first_timestep, int32_t steps_per_candid
   std::vector<Cand> cand;
   for (int st = 2; st >= 1; st--)
     for (int xcd = 0; xcd < 3; xcd += 2)
-      for (int mixed = 1; mixed >= 0; mixed--) cand.push_back({3, 1, 1, xcd, mixed, st});
-  if (h->layout == TLM_LAYOUT_ROWS) {
+      for (int mixed = 1; mixed >= 0; mixed--)
+        if (!ext_form(h) || (xcd == 0 && mixed == 1)) cand.push_back({3, 1, 1, xcd, mixed, st});
+  if (h->layout == TLM_LAYOUT_ROWS && !ext_form(h)) {
     const Cand tiles[] = {{0, 1, 1, 0, 1, 2}, {0, 2, 1, 0, 1, 2}, {0, 4, 1, 0, 1, 2}, {2, 1, 1, 0, 1, 2},
                           {2, 4, 1, 0, 1, 2}, {1, 4, 8, 0, 1, 2}};
     cand.insert(cand.end(), std::begin(tiles), std::end(tiles));
@@ -1955,7 +1962,7 @@ int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
   if (!h || !cells || !bytes) return TLM_ERR_ARG;
   int rc;
   if ((rc = finalize(h))) return rc;
-  const int64_t sl = (int64_t)link_size(h);
+  const int64_t sl = (int64_t)link_bytes(h);
   const int64_t L = h->d.line_len;
   const int64_t n_lines = h->d.seg_base[h->d.nseg] / L;
   // filter words of a dispersive cell: 3 components x (carried words read

@@ -43,6 +43,7 @@ program tlm_gpu_cavity
   if (command_argument_count() .ge. 2) call get_command_argument(2, mode)
   form = TLM_FORM_PLAIN_FP64
   if (trim(mode) .eq. 'precision') form = TLM_FORM_PRECISION_FP32
+  if (trim(mode) .eq. 'extended') form = TLM_FORM_EXTENDED_FP32
 
   ! mesh (set_mesh_parameters.F90:62-69) and time step (set_TLM_solver_parameters.F90:94-96)
   mesh_min = (/ -0.5d0, -1.0d0, -1.5d0 /)

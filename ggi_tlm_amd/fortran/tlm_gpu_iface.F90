@@ -17,6 +17,7 @@ module tlm_gpu_iface
 
   integer(c_int32_t), parameter :: TLM_FORM_PLAIN_FP64 = 0
   integer(c_int32_t), parameter :: TLM_FORM_PRECISION_FP32 = 1
+  integer(c_int32_t), parameter :: TLM_FORM_EXTENDED_FP32 = 2
   integer, parameter :: TLM_MAX_FILTER_ORDER = 8
 
   ! TLM_general grid scalars (tlm_grid)

@@ -43,9 +43,14 @@ extern "C" {
 
 /* Numerical forms.  PLAIN: FP64 link planes, bit-faithful to the reference
  * sequential build's operation order.  PRECISION: FP32 link planes; filter
- * states, PML accumulators and the probe DFT stay FP64. */
+ * states, PML accumulators and the probe DFT stay FP64.  EXTENDED: the
+ * precision form's arithmetic and states with each link stored as FP32 plus
+ * an 8-bit residual plane (a 32-bit significand, 5 bytes per link): link
+ * rounding 256 times smaller than PRECISION's for 25% more link bytes.
+ * The batch (tlmbatch.h) takes every form. */
 #define TLM_FORM_PLAIN_FP64 0
 #define TLM_FORM_PRECISION_FP32 1
+#define TLM_FORM_EXTENDED_FP32 2
 
 /* Largest Zfilter order accepted (a and b polynomials of order <= this). */
 #define TLM_MAX_FILTER_ORDER 8

@@ -6,7 +6,7 @@ import pytest
 
 import problems
 from ggi_tlm_amd import capi
-from ggi_tlm_amd.capi import FORM_PLAIN_FP64, FORM_PRECISION_FP32
+from ggi_tlm_amd.capi import FORM_EXTENDED_FP32, FORM_PLAIN_FP64, FORM_PRECISION_FP32
 from ggi_tlm_amd.problem import build_tables
 
 pytestmark = pytest.mark.gpu
@@ -28,7 +28,7 @@ def results(h):
 
 
 @pytest.mark.parametrize("kernels", [(-1, 2), (1, 1), (0, 2)])
-@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
+@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32, FORM_EXTENDED_FP32])
 def test_batch_matches_alone(form, kernels):
     lib = capi.engine_library()
     tables = [(name, build_tables(f()), n) for name, f, n in DECKS]
@@ -40,6 +40,11 @@ def test_batch_matches_alone(form, kernels):
         h.close()
     b = capi.Batch(form, 16 << 30)
     b.set_graph_steps(8)
+    if form == FORM_EXTENDED_FP32 and kernels[0] == 0:
+        with pytest.raises(capi.TlmError, match="LDS-staged"):
+            b.set_kernels(*kernels)     # the extended form has the LDS-staged mixed kernel only
+        b.close()
+        return
     b.set_kernels(*kernels)
     handles = {name: capi.Handle(lib, t, form, output_capacity=n) for name, t, n in tables}
     steps = {name: n for name, _, n in tables}
@@ -71,7 +76,7 @@ def test_batch_matches_alone(form, kernels):
         h.close()
 
 
-@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
+@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32, FORM_EXTENDED_FP32])
 def test_batch_flush_and_continue_alone(form):
     """The small-value flush inside a batch (PML cells recompute the previous
     step's incident voltages with that step's flush), and a collected solve

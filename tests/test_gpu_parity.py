@@ -10,7 +10,7 @@ import pytest
 
 import problems
 from ggi_tlm_amd import capi
-from ggi_tlm_amd.capi import FORM_PLAIN_FP64, FORM_PRECISION_FP32, probe_series
+from ggi_tlm_amd.capi import FORM_EXTENDED_FP32, FORM_PLAIN_FP64, FORM_PRECISION_FP32, probe_series
 from ggi_tlm_amd.problem import build_tables
 
 pytestmark = pytest.mark.gpu
@@ -103,15 +103,19 @@ def test_upload_then_step(oracle_lib):
     assert np.array_equal(o.read_cell_outputs()[0], g.read_cell_outputs()[0])
 
 
-@pytest.mark.parametrize("name", ["free_cube", "pml_sphere", "pml_sphere_64", "dispersive_blocks", "special_mix",
-                                  "huygens_surface", "pml_material"])
-def test_precision_form(name, oracle_lib):
+REDUCED_DECKS = ["free_cube", "pml_sphere", "pml_sphere_64", "dispersive_blocks", "special_mix", "huygens_surface",
+                 "pml_material"]
+
+
+@pytest.mark.parametrize("form", [FORM_PRECISION_FP32, FORM_EXTENDED_FP32])
+@pytest.mark.parametrize("name", REDUCED_DECKS)
+def test_precision_form(name, form, oracle_lib):
     from oracle import oracle
 
     t = build_tables(CASES[name]())
     steps = min(t.n_timesteps, 400)
     o = oracle.handle(t, output_capacity=steps)
-    g = engine(t, FORM_PRECISION_FP32)
+    g = engine(t, form)
     so = probe_series(run(o, t, steps), t)
     sg = probe_series(run(g, t, steps), t)
     # A probe whose FP64 signal is zero by symmetry (only round-off) has no
@@ -121,8 +125,12 @@ def test_precision_form(name, oracle_lib):
     # PML cells holding a material advance two-step recursions whose
     # coefficients nearly cancel (4 V_n - 8 V_n-1 + 4 V_n-2 terms), so the FP32
     # rounding of the link values is amplified there: 1e-4 for that deck
-    # (measured 2.4e-5); 1e-5 everywhere else.
+    # (measured 2.4e-5); 1e-5 everywhere else.  The extended form rounds each
+    # stored link 256 times less (2^-33 against 2^-25 relative): 1e-6 and
+    # 1e-7.
     tol = 1e-4 if name == "pml_material" else 1e-5
+    if form == FORM_EXTENDED_FP32:
+        tol /= 100.0
     for k in range(so.shape[1]):
         den = max(np.linalg.norm(so[:, k]), floor)
         rel = np.linalg.norm(sg[:, k] - so[:, k]) / den
@@ -226,7 +234,7 @@ LAYOUT_DECKS = ["pml_sphere", "pml_sphere_64", "pml_material", "dispersive_block
                 "huygens_surface", "tem_pml", "free_cube_odd", "layered_slabs"]
 
 
-@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
+@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32, FORM_EXTENDED_FP32])
 @pytest.mark.parametrize("name", LAYOUT_DECKS)
 def test_layouts_bitwise(name, form):
     """The x-segment and row-major storage orders give the same bits in both
@@ -249,7 +257,7 @@ def test_layouts_bitwise(name, form):
         assert np.array_equal(a, b), name
 
 
-@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
+@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32, FORM_EXTENDED_FP32])
 def test_streams_bitwise(form):
     """The class kernels on a second stream (default) or in turn on one give
     the same bits."""
@@ -311,3 +319,32 @@ def test_layout_segment_cap():
     steps = t.n_timesteps
     assert np.array_equal(run(o, t, steps), run(g, t, steps))
     assert np.array_equal(o.download_links()[:, 1:-1, 1:-1], g.download_links()[:, 1:-1, 1:-1])
+
+
+def test_extended_links_round_trip():
+    """Extended-form link storage: uploaded FP64 values come back within
+    ulp/508 of their FP32 rounding h (a residual byte in units of ulp/254:
+    at most 2^-23/508 |h|), at least 128 times closer than the FP32 planes
+    of the precision form bring them back, and values that are FP32 numbers
+    come back exactly."""
+    t = build_tables(CASES["pml_sphere"]())
+    rng = np.random.default_rng(7)
+    shape = (12, t.nx + 2, t.ny + 2, t.nz)
+    mag = 10.0 ** rng.uniform(-30, 3, size=shape)
+    V = np.asfortranarray(rng.choice([-1.0, 1.0], size=shape) * mag)
+    V[:, :, :, 0] = V[:, :, :, 0].astype(np.float32)   # exact in FP32
+    V[0, :, :, 1] = 0.0
+    back = {}
+    for form in (FORM_PRECISION_FP32, FORM_EXTENDED_FP32):
+        g = engine(t, form)
+        g.upload_links(V)
+        back[form] = g.download_links()[:, 1:-1, 1:-1]
+        g.close()
+    ref = V[:, 1:-1, 1:-1]
+    ex, pr = back[FORM_EXTENDED_FP32], back[FORM_PRECISION_FP32]
+    err_x, err_p = np.abs(ex - ref), np.abs(pr - ref)
+    bound = 2.0 ** -23 / 508 * (1 + 2.0 ** -20) * np.abs(ref)
+    assert np.all(err_x <= bound), float(np.max(err_x / np.maximum(np.abs(ref), 1e-300)))
+    assert np.array_equal(ex[:, :, :, 0], ref[:, :, :, 0])
+    assert np.all(ex[0, :, :, 1] == 0.0)
+    assert np.linalg.norm(err_x) * 128 < np.linalg.norm(err_p)

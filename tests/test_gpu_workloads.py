@@ -11,10 +11,12 @@
   size for a few steps, with their x-segment layout and autotuned schedule,
   against the oracle's final link state and probe rows.
 
-Plain FP64: bit-identical.  Precision form (FP32 links): relative L2 per
-probe against the FP64 oracle below 1e-5 (a probe whose FP64 signal is
-negligible is measured against 1e-3 of the solve's largest probe norm), and
-the link state within 1e-5 relative L2.
+Plain FP64: bit-identical.  Precision form (FP32 links) and extended form
+(FP32 links plus a residual byte): relative L2 per probe against the FP64
+oracle below 1e-5 (a probe whose FP64 signal is negligible is measured
+against 1e-3 of the solve's largest probe norm), and the link state within
+1e-5 relative L2; on the serving requests the precision form is held to
+2e-4 and the extended form to 1e-5.
 """
 import os
 
@@ -22,7 +24,7 @@ import numpy as np
 import pytest
 
 from ggi_tlm_amd import capi, serving, workloads
-from ggi_tlm_amd.capi import FORM_PLAIN_FP64, FORM_PRECISION_FP32, probe_series
+from ggi_tlm_amd.capi import FORM_EXTENDED_FP32, FORM_PLAIN_FP64, FORM_PRECISION_FP32, probe_series
 from ggi_tlm_amd.problem import build_tables
 
 pytestmark = pytest.mark.gpu
@@ -32,8 +34,10 @@ TOL = 1e-5
 # Point-source requests ringing in small PML boxes with dielectric spheres
 # (eps_r up to 8) or two-pole blocks accumulate the FP32 rounding of the
 # links on their weak probes: measured up to 9.2e-5 after 400 steps on this
-# stream (request 11, three two-pole blocks).
-TOL_SERVE = 2e-4
+# stream (request 11, three two-pole blocks).  The extended form rounds each
+# link 256 times less and is held to the 1e-5 target.
+TOL_SERVE = {FORM_PRECISION_FP32: 2e-4, FORM_EXTENDED_FP32: 1e-5}
+FORMS = [FORM_PLAIN_FP64, FORM_PRECISION_FP32, FORM_EXTENDED_FP32]
 
 
 def oracle_solve(t, steps):
@@ -84,7 +88,7 @@ def engine_results(h):
     return out, dft, h.download_links()[:, 1:-1, 1:-1]
 
 
-@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
+@pytest.mark.parametrize("form", FORMS)
 def test_enclosure_batch_vs_oracle(form):
     steps = 500
     seeds = (0, 1, 2, 3)
@@ -123,7 +127,7 @@ def serving_decks(seed=11, count=12, steps=400, max_n=90):
     return out
 
 
-@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
+@pytest.mark.parametrize("form", FORMS)
 def test_serving_requests_vs_oracle(form):
     decks = serving_decks()
     assert any(m > 1 for *_, m in decks) and any(m == 0 for *_, m in decks)
@@ -144,11 +148,11 @@ def test_serving_requests_vs_oracle(form):
             floor = 1e-3 * max(norms)
             rel = max(np.linalg.norm(sg[:, k] - so[:, k]) / max(norms[k], floor) for k in range(so.shape[1]))
             print(f"[serve] request {r.rid} n={t.nx} worst relative L2 {rel:.2e}", flush=True)
-            assert rel < TOL_SERVE, (r.rid, rel)
+            assert rel < TOL_SERVE[form], (r.rid, rel)
 
 
 @pytest.mark.parametrize("name", ["single_large", "dispersive_load"])
-@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
+@pytest.mark.parametrize("form", FORMS)
 def test_full_size_vs_oracle(name, form):
     # 16 autotuner steps (8 candidates on the x-segment layout), then enough
     # steps for the sources' fields to reach the probes

@@ -15,7 +15,7 @@ import pytest
 
 import problems
 from ggi_tlm_amd import capi
-from ggi_tlm_amd.capi import FORM_PLAIN_FP64, FORM_PRECISION_FP32
+from ggi_tlm_amd.capi import FORM_EXTENDED_FP32, FORM_PLAIN_FP64, FORM_PRECISION_FP32
 from ggi_tlm_amd.constants import Z0
 from ggi_tlm_amd.problem import build_tables
 
@@ -119,7 +119,7 @@ def test_host_models_reproduce_device_faces_oracle(oracle_lib):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
+@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32, FORM_EXTENDED_FP32])
 def test_host_faces_engine_vs_oracle(oracle_lib, form):
     from oracle import oracle
 

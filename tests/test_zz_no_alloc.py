@@ -8,7 +8,7 @@ import pytest
 
 import problems
 from ggi_tlm_amd import capi
-from ggi_tlm_amd.capi import FORM_PLAIN_FP64, FORM_PRECISION_FP32
+from ggi_tlm_amd.capi import FORM_EXTENDED_FP32, FORM_PLAIN_FP64, FORM_PRECISION_FP32
 from ggi_tlm_amd.problem import ExcitationFunction, OutputPoint, PointSource, Problem, build_tables
 
 pytestmark = pytest.mark.gpu
@@ -30,7 +30,7 @@ def both(h, t, seed):
     return V, a, b
 
 
-@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32])
+@pytest.mark.parametrize("form", [FORM_PLAIN_FP64, FORM_PRECISION_FP32, FORM_EXTENDED_FP32])
 def test_noalloc_download_equals_download(form):
     lib = capi.engine_library()
     # first call: 80^3 (12*82*82*80 doubles, more than one staging slab);
