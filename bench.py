@@ -266,22 +266,25 @@ def main_accuracy(a):
 
 def main_serve_batch(a):
     """The serving stream's requests (same seed) admitted at once and stepped
-    to the end as one resident batch in the precision form: the GPU-bound
+    to the end as one resident batch in the form --form names: the GPU-bound
     part of the saturated stream, for kernel profiles (one graph capture per
     chunk set, no arrivals).  The timed work counts each request's own
-    steps after the untimed first chunk."""
+    steps after the untimed first chunk; the algorithmic bytes are each
+    solve's per-step class bytes over those steps."""
     rng = np.random.default_rng(a.seed)
     probs = [workloads.serving_request(rng) for _ in range(a.serve_requests)]
     tables = build_all(probs)
     lib = capi.engine_library()
-    form = capi.FORM_PRECISION_FP32
+    form = FORM_NAMES[a.form]
     b = capi.Batch(form, 200 << 30, lib)
     chunk = 512
     b.set_graph_steps(chunk)
     hs = []
+    step_bytes = []
     for t, p in zip(tables, probs):
         h = capi.Handle(lib, t, form, output_capacity=p.n_timesteps)
         h.release_tables()
+        step_bytes.append(sum(v[1] for v in h.kernel_bytes().values()))
         if not b.add(h, 1, p.n_timesteps):
             raise RuntimeError("the serving requests do not fit the batch budget")
         hs.append(h)
@@ -304,12 +307,15 @@ def main_serve_batch(a):
             h.close()
     b.close()
     timed_cells = sum(t.nx * t.ny * t.nz * max(0, p.n_timesteps - chunk) for t, p in zip(tables, probs))
+    timed_bytes = sum(sb * max(0, p.n_timesteps - chunk) for sb, p in zip(step_bytes, probs))
     print(json.dumps({"metric": "Gcell-updates/s", "value": round(timed_cells / dt / 1e9, 3),
                       "unit": "Gcell-updates/s", "n_gpus": 1, "steps": steps - chunk, "higher_is_better": True,
                       "scaling": "weak", "vs_baseline": None, "dtype": dtype_of(form), "data": "synthetic",
                       "config": {"workload": f"{len(tables)} serving requests (seed {a.seed}) stepped as one "
-                                             "resident batch", "form": "precision", "seconds": round(dt, 3),
-                                 "timed_cell_updates": int(timed_cells)}}))
+                                             "resident batch", "form": a.form, "seconds": round(dt, 3),
+                                 "timed_cell_updates": int(timed_cells),
+                                 "algorithmic_GB_s": round(timed_bytes / dt / 1e9, 1),
+                                 "frac_of_hbm": round(timed_bytes / dt / 1e9 / HBM_PEAK_GBS, 4)}}))
 
 
 def request_errors(test, ref):
