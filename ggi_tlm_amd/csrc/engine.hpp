@@ -53,7 +53,7 @@ struct tlm_handle {
   bool cells_set = false, faces_set = false;
   std::vector<int32_t> cell_code, c_mat, c_filt, c_pml, c_exc, c_out, c_cable;
   int n_special = 0;
-  std::vector<int32_t> face_code, f_mat, f_exc, f_out;
+  std::vector<int32_t> face_code, f_mat, f_exc, f_out, f_cable;
   int n_special_faces = 0;
   std::vector<tlm_volume_material> vmat;
   std::vector<int32_t> smat;

@@ -1415,9 +1415,11 @@ int tlm_set_face_codes(tlm_handle* h, const int32_t* code, int32_t n_special, co
   cp(to_material, h->f_mat);
   cp(to_excitation, h->f_exc);
   cp(to_output, h->f_out);
-  if (to_cable)
-    for (int i = 0; i < n_special; i++)
-      if (to_cable[i]) return fail(h, TLM_ERR_UNSUPPORTED, "cable faces stay on the host path");
+  // A cable face (to_cable != 0) is connected as its surface material or
+  // excitation makes it; the reference then calls face_cable_junction
+  // (connect_normal_to_x.F90:317-319), which reads and writes only the cable
+  // model's own state, so the host runs it after tlm_connect.
+  cp(to_cable, h->f_cable);
   h->faces_set = true;
   return TLM_OK;
 }

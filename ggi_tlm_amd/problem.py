@@ -174,6 +174,9 @@ class Problem:
     # cells whose scatter stays on the host path (cable junction cells,
     # cell_update_code_to_cable_cell_number != 0), 1-based (cx, cy, cz)
     host_cells: List[Tuple[int, int, int]] = field(default_factory=list)
+    # faces carrying a cable face junction (face_update_code_to_cable_number
+    # = position in this list + 1), (cx, cy, cz, face) as surface_faces
+    cable_faces: List[Tuple[int, int, int, int]] = field(default_factory=list)
 
     @property
     def dt(self) -> float:
@@ -628,6 +631,11 @@ def build_tables(p: Problem) -> Tables:
             t = _min_face(nx, ny, nz, (cx, cy, cz), face)
             if t is not None:
                 f_exc[t[2] - 1, t[1] - 1, t[0] - 1, t[3]] = True
+    f_cab = np.zeros(fshape, dtype=np.int32)
+    for k, (cx, cy, cz, face) in enumerate(p.cable_faces):
+        t = _min_face(nx, ny, nz, (int(cx), int(cy), int(cz)), int(face))
+        if t is not None:
+            f_cab[t[2] - 1, t[1] - 1, t[0] - 1, t[3]] = k + 1
     face_out_pts = []
     for o in p.outputs:
         if FACES[o.point] != CENTRE:
@@ -642,7 +650,9 @@ def build_tables(p: Problem) -> Tables:
     fm = f_mat.reshape(-1)[vflat]
     fe = f_exc.reshape(-1)[vflat]
     fo = f_out.reshape(-1)[vflat]
-    fspecial = (fm != 0) | fe | fo
+    fc = f_cab.reshape(-1)[vflat]
+    # set_update_codes.F90:383-791: material, cable, excitation or output
+    fspecial = (fm != 0) | (fc != 0) | fe | fo
     n_faces = int(vflat.sum())
     n_special_faces = int(fspecial.sum())
     face_code = np.zeros(n_faces, dtype=np.int32)
@@ -654,7 +664,7 @@ def build_tables(p: Problem) -> Tables:
     face_to_material[:, 2] = np.where(fm[fsp] != 0, axis_of[fsp] + 1, 0)
     face_to_exc = fe[fsp].astype(np.int32)
     face_to_out = fo[fsp].astype(np.int32)
-    face_to_cable = np.zeros(n_special_faces, dtype=np.int32)
+    face_to_cable = fc[fsp].astype(np.int32)
     # per-face ranks over the full [cz,cy,cx,axis] grid
     fexc_rank_full = np.zeros(fshape, dtype=np.int32).reshape(-1)
     fidx_valid = np.nonzero(vflat)[0]

@@ -159,7 +159,11 @@ int tlm_set_cell_codes(tlm_handle* h, const int32_t* cell_centre_update_code,
 
 /* mesh::face_update_code(1:number_of_face_codes) and face_update_code_to_*
  * (set_update_codes.F90:383-791).  to_material is the Fortran (n,3) array,
- * column-major; column 1 is the signed surface material number. */
+ * column-major; column 1 is the signed surface material number.  A face
+ * with a nonzero to_cable entry is connected as its material and excitation
+ * make it; its cable junction (connect_normal_to_x.F90:317-319,
+ * face_cable_junction) reads and writes only the cable model, and the host
+ * runs it after tlm_connect, in the connect's face order. */
 int tlm_set_face_codes(tlm_handle* h, const int32_t* face_update_code,
                        int32_t n_special_faces, const int32_t* to_material,
                        const int32_t* to_excitation, const int32_t* to_output,

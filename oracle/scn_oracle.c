@@ -242,8 +242,10 @@ int oracle_set_face_codes(oracle_handle* h, const int32_t* code, int32_t n_speci
                           const int32_t* to_output, const int32_t* to_cable) {
   int nx = h->nx, ny = h->ny, nz = h->nz;
   size_t nf = (size_t)(nx - 1) * ny * nz + (size_t)nx * (ny - 1) * nz + (size_t)nx * ny * (nz - 1);
-  for (int32_t i = 0; i < n_special; i++)
-    if (to_cable && to_cable[i]) return fail(h, TLM_ERR_UNSUPPORTED, "cable faces stay on the host path");
+  /* cable faces (to_cable != 0): connected as any special face; the cable
+     junction itself (connect_normal_to_x.F90:317-319, face_cable_junction)
+     touches only the cable model and runs on the host after the connect */
+  (void)to_cable;
   free(h->pl_fexc); free(h->pl_fout);
   h->pl_fexc = h->pl_fout = NULL;
   free(h->hf_of_code); h->hf_of_code = NULL;
