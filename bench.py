@@ -208,6 +208,45 @@ def main_batch(a):
     print(json.dumps(line), flush=True)
 
 
+def timed_solve(t, form, steps, warmup, tune, profile_steps):
+    """Autotune (untimed), warm up, time `steps` steps between two
+    synchronisations, then one profiled pass of per-class kernel durations
+    (HIP events on the engine's stream).  Returns the elapsed seconds, the
+    per-class figures, the step's algorithmic bytes, the schedule and the
+    x-segments."""
+    h = capi.Handle(capi.engine_library(), t, form, output_capacity=1024)
+    first = 1
+    if tune:
+        first += h.autotune(1, tune)
+    h.step(first, warmup)
+    first += warmup
+    h.synchronize()
+    t0 = time.perf_counter()
+    h.step(first, steps)
+    first += steps
+    h.synchronize()
+    elapsed = time.perf_counter() - t0
+    print(f"[bench] form {form}: timed {steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    h.set_profiling(True)
+    h.step(first, profile_steps)
+    kt = h.read_kernel_times()
+    h.set_profiling(False)
+    kb = h.kernel_bytes()
+    per_class = {}
+    step_bytes = 0
+    for k, (n_launch, ms) in kt.items():
+        cells, b = kb[k]
+        avg_ms = ms / n_launch if n_launch else 0.0
+        step_bytes += b
+        per_class[k] = {"cells": cells, "bytes_per_launch": b, "avg_ms": round(avg_ms, 5),
+                        "GB_s": round(b / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None}
+    print(f"[bench] kernels {json.dumps(per_class)}", file=sys.stderr, flush=True)
+    sched, segs = h.get_schedule(), h.layout_segments()
+    h.read_cell_outputs()
+    h.close()
+    return elapsed, per_class, step_bytes, sched, segs
+
+
 def main_accuracy(a):
     """Precision and extended forms against FP64 on the full-size workloads:
     relative L2 of every probe series and of every probe's DFT (transfer
@@ -414,6 +453,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extended", action="store_true", help="skip the extended-form figures of the default line")
     ap.add_argument("--mode", choices=("single", "batch", "serve", "serve-batch", "accuracy"), default="single")
     ap.add_argument("--acc-workloads", nargs="+", default=["large", "dispersive"])
     ap.add_argument("--acc-steps-large", type=int, default=4000)
@@ -454,50 +494,29 @@ def main():
                 f"16 probes, {a.steps} timed steps")
     t = build_tables(p)
     form = FORM_NAMES[a.form]
-    h = capi.Handle(capi.engine_library(), t, form, output_capacity=1024)
     setup_s = time.perf_counter() - t_setup
     print(f"[bench] set-up {setup_s:.1f} s; cells {t.nx * t.ny * t.nz}, PML {t.n_pml_cells}, "
           f"special {t.n_special_cells}, special faces {t.n_special_faces}", file=sys.stderr, flush=True)
-
-    # schedule autotuning and warm-up (untimed), then the timed region
-    first = 1
-    if tune_steps:
-        first += h.autotune(1, a.tune_steps)
-    h.step(first, a.warmup)
-    first += a.warmup
-    h.synchronize()
-    t0 = time.perf_counter()
-    h.step(first, a.steps)
-    first += a.steps
-    h.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
     N = t.nx * t.ny * t.nz
+    elapsed, per_class, step_bytes, sched, segs = timed_solve(t, form, a.steps, a.warmup, a.tune_steps,
+                                                              a.profile_steps)
     gcells = N * a.steps / elapsed / 1e9
-    print(f"[bench] timed {a.steps} steps in {elapsed:.3f} s: {gcells:.2f} Gcell/s", file=sys.stderr, flush=True)
-
-    # profiled pass: per-class kernel durations with HIP events on the engine stream
-    h.set_profiling(True)
-    h.step(first, a.profile_steps)
-    kt = h.read_kernel_times()
-    h.set_profiling(False)
-    kb = h.kernel_bytes()
-    per_class = {}
-    step_bytes = 0
-    for k, (n_launch, ms) in kt.items():
-        cells, b = kb[k]
-        avg_ms = ms / n_launch if n_launch else 0.0
-        step_bytes += b
-        per_class[k] = {"cells": cells, "bytes_per_launch": b, "avg_ms": round(avg_ms, 5),
-                        "GB_s": round(b / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None}
-    print(f"[bench] kernels {json.dumps(per_class)}", file=sys.stderr, flush=True)
     dom = max((k for k in per_class if k != "dft" and per_class[k]["cells"]),
               key=lambda k: per_class[k]["avg_ms"])
     achieved = per_class[dom]["GB_s"]
     traffic = measured_traffic(a, dom, per_class[dom]["bytes_per_launch"])
-    sched, segs = h.get_schedule(), h.layout_segments()
-    h.read_cell_outputs()
-    h.close()
+    # the same workload in the extended form (every probe within 1e-5 of FP64)
+    ext = None
+    if form == capi.FORM_PRECISION_FP32 and not a.no_extended:
+        ex_steps = min(a.steps, 2000)
+        e_el, e_pc, e_bytes, e_sched, _ = timed_solve(t, capi.FORM_EXTENDED_FP32, ex_steps, a.warmup,
+                                                       a.tune_steps, a.profile_steps)
+        ext = {"value": round(N * ex_steps / e_el / 1e9, 3), "ms_per_step": round(e_el / ex_steps * 1e3, 4),
+               "steps": ex_steps, "dtype": dtype_of(capi.FORM_EXTENDED_FP32), "link_bytes": 5,
+               "bulk_GB_s": e_pc["bulk"]["GB_s"],
+               "bulk_frac_of_hbm": round(e_pc["bulk"]["GB_s"] / HBM_PEAK_GBS, 4) if e_pc["bulk"]["GB_s"] else None,
+               "whole_step_GB_s": round(e_bytes / (e_el / ex_steps) / 1e9, 1), "kernels": e_pc,
+               "schedule": e_sched}
 
     cpu = None
     if not a.no_cpu:
@@ -533,6 +552,7 @@ def main():
             "kernels": per_class,
             "schedule": sched,
             "x_segments": segs,
+            "extended_form": ext,
         },
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic},
