@@ -15,9 +15,12 @@ from ggi_tlm_amd.problem import (PML, Box, ExcitationFunction, HuygensBox, Outpu
 
 
 def mesh_dl(lo, hi, n):
-    """set_mesh_parameters.F90:66-69: the mean of the three cell sizes."""
+    """set_mesh_parameters.F90:66-69: the mean of the three cell sizes, the
+    division by 3 taken as a product with 1/3 as the reference's
+    -funsafe-math-optimizations build evaluates it (0.1 for three cells of
+    0.1, not 0.10000000000000002)."""
     d = [(hi[i] - lo[i]) / float(n[i]) for i in range(3)]
-    return (d[0] + d[1] + d[2]) / 3.0
+    return (d[0] + d[1] + d[2]) * (1.0 / 3.0)
 
 
 # ---- reference decks ----------------------------------------------------------

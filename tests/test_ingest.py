@@ -1,0 +1,186 @@
+"""Standalone ingest (ggi_tlm_amd/ingest.py) against the reference's own
+test inputs (tests/golden/inputs/, copied from TEST_DATA by
+tests/golden/make_inputs.py).
+
+* The six golden decks' `.inp` files give the same tables, bit for bit, as
+  the decks built by hand in tests/problems.py, whose oracle runs match the
+  reference's golden outputs (test_oracle_golden.py).  MATERIAL_TEST2's two
+  points lie exactly on cell boundaries: the ingest flags them, and with
+  the cells the reference's model builder gave them (the `.mesh` point
+  records; pinned by the golden series) the tables match.
+* HUYGENS_SURFACE ingested with all its output points (centre and face)
+  runs on the oracle to the reference's golden centre-field series.
+* The model builder's own `.mesh` (TEST_DATA/HELIX_ANTENNA/helix.mesh)
+  reads to the counts the reference's run log (helix.info) states, and
+  survives a write/read round trip.
+* `.vmat` files read to the filters the hand-built decks use.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import problems
+from ggi_tlm_amd import ingest
+from ggi_tlm_amd.capi import probe_series
+from ggi_tlm_amd.filters import Sfilter
+from ggi_tlm_amd.problem import build_tables
+
+D = os.path.join(os.path.dirname(__file__), "golden", "inputs")
+MAT = os.path.join(D, "materials")
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_goldens.npz"))
+
+
+def same_tables(a, b):
+    diff = []
+    for k in vars(a):
+        x, y = getattr(a, k), getattr(b, k)
+        if isinstance(x, np.ndarray) or isinstance(y, np.ndarray):
+            ok = np.array_equal(np.asarray(x), np.asarray(y))
+        else:
+            ok = x == y
+        if not ok:
+            diff.append(k)
+    return diff
+
+
+@pytest.mark.parametrize("tok,want", [("3.0-3", 3.0e-3), ("-3.0e-3", -3.0e-3), ("1.0d2", 100.0), (".5", 0.5),
+                                      ("5.", 5.0), ("-2", -2.0), ("1E+3", 1000.0), ("2.5+1", 25.0)])
+def test_fortran_reals(tok, want):
+    assert ingest.fortran_real(tok) == want
+
+
+def test_fortran_real_rejects_words():
+    with pytest.raises(ingest.IngestError):
+        ingest.fortran_real("Ex")
+
+
+DECKS = {
+    "cavity_free_space": problems.cavity,
+    "cavity_epsr2": lambda: problems.cavity(Sfilter(1e8, [2.0], [1.0]), None),
+    "cavity_mur2": lambda: problems.cavity(None, Sfilter(1e8, [2.0], [1.0])),
+    "tem_pml": problems.tem_pml,
+}
+
+
+@pytest.mark.parametrize("name", sorted(DECKS))
+def test_reference_inputs_give_the_hand_built_tables(name):
+    ing = ingest.load(os.path.join(D, name + ".inp"), material_dir=MAT)
+    assert ing.unsupported == [] and ing.boundary_points == []
+    assert same_tables(build_tables(DECKS[name]()), build_tables(ing.problem)) == []
+
+
+def test_material_test2_boundary_points():
+    path = os.path.join(D, "material_test2.inp")
+    ing = ingest.load(path, material_dir=MAT)
+    assert ing.boundary_points == [1, 2]        # x = -1.2e-3 and +1.2e-3 on cell boundaries
+    ing = ingest.load(path, material_dir=MAT, point_cells={1: (18, 31, 31), 2: (42, 31, 31)})
+    assert same_tables(build_tables(problems.material_test2()), build_tables(ing.problem)) == []
+
+
+def test_huygens_surface_inp_runs_to_the_golden_series(oracle_lib):
+    from oracle import oracle
+
+    ing = ingest.load(os.path.join(D, "huygens_surface.inp"))
+    assert ing.unsupported == ["output_surface_list", "output_volume_list"]
+    p = ing.problem
+    hand = problems.huygens_surface()
+    centre = [o for o in p.outputs if o.point == "centre"]
+    assert [(o.cell, o.component) for o in centre] == [(o.cell, o.component) for o in hand.outputs]
+    assert len(p.outputs) == 30 and p.huygens == hand.huygens
+    # the same tables as the hand-built deck once the face outputs are left out
+    q = ingest.load(os.path.join(D, "huygens_surface.inp")).problem
+    q.outputs = centre
+    assert same_tables(build_tables(hand), build_tables(q)) == []
+    # all output points (face outputs make their faces special) on the oracle
+    t = build_tables(p)
+    h = oracle.handle(t, output_capacity=t.n_timesteps)
+    h.step(1, t.n_timesteps)
+    s = probe_series(h.read_cell_outputs()[0], t)     # one column per centre output point, in order
+    assert s.shape[1] == len(centre) == 6
+    for i, o in enumerate(centre):
+        r = GOLD[f"huygens_{o.component}_centre_tout"][:, 2]
+        err = np.max(np.abs(s[:, i] - r)) / np.max(np.abs(r))
+        assert err < 1e-6, (o.component, err)
+    fo = h.read_face_outputs()[0]
+    assert fo.shape[0] == t.n_timesteps and np.abs(fo).max() > 0
+
+
+def info_counts():
+    counts = {}
+    with open(os.path.join(D, "helix.info")) as f:
+        for line in f:
+            if "Total number of material cells=" in line:
+                counts["material_cells"] = int(line.split("=")[1])
+            elif "Total number of material faces=" in line:
+                counts["material_faces"] = int(line.split("=")[1])
+            elif "Number of cell_faces=" in line:
+                counts.setdefault("frequency_surface_faces", []).append(int(line.split("=")[-1]))
+            elif "Total number of cells=" in line:
+                counts["cells"] = int(line.split("=")[1])
+    return counts
+
+
+def test_helix_mesh_matches_the_reference_run_log(tmp_path):
+    m = ingest.read_mesh(os.path.join(D, "helix.mesh.gz"))
+    c = info_counts()
+    assert m.dl == 0.5 and m.n == (42, 42, 42) and m.mesh_min == (-10.5, -10.5, -10.5)
+    assert int(np.prod(m.n)) == c["cells"]
+    assert len(m.volumes) == 1 and len(m.volumes[0]) == c["material_cells"]
+    # helix.inp: PEC on surfaces 3-6, frequency output surfaces on surface 2
+    assert sum(len(m.surfaces[k - 1]["faces"]) for k in (3, 4, 5, 6)) == c["material_faces"]
+    assert c["frequency_surface_faces"] == [len(m.surfaces[1]["faces"])] * 2
+    assert [len(ln) for ln in m.lines] == [10, 10] and len(m.points) == 4
+    assert all(np.all((v[:, :3] >= 1) & (v[:, :3] <= 42)) for v in m.volumes)
+    # round trip through the writer
+    out = str(tmp_path / "helix.mesh")
+    ingest.write_mesh(m, out)
+    m2 = ingest.read_mesh(out)
+    assert (m2.dl, m2.n, m2.mesh_min, m2.mesh_max) == (m.dl, m.n, m.mesh_min, m.mesh_max)
+    for a, b in zip(m.volumes + m.lines, m2.volumes + m2.lines):
+        assert np.array_equal(a, b)
+    for a, b in zip(m.surfaces, m2.surfaces):
+        assert np.array_equal(a["faces"], b["faces"]) and a["bounds"] == b["bounds"]
+    assert m.points == m2.points
+
+
+def test_mesh_volumes_become_cell_regions(tmp_path):
+    """A deck meshed here and the same deck read back from a written `.mesh`
+    (volume cell lists, surface face lists, point cells) give the same
+    tables."""
+    path = os.path.join(D, "cavity_epsr2.inp")
+    a = ingest.load(path, material_dir=MAT).problem
+    lo = np.array(a.mesh_min)
+    cells = np.array([(i, j, k, 7) for k in range(1, 31) for j in range(1, 21) for i in range(1, 11)])
+    pts = []
+    for s, o in ((a.sources[0], None), (None, a.outputs[0])):
+        cell = (s or o).cell
+        pts.append({"xyz": (0.0, 0.0, 0.0), "cell": cell, "face": (*cell, 7)})
+    # the packets number the excitation point 1 and the output point 2
+    m = ingest.MeshFile(a.dl, (a.nx, a.ny, a.nz), tuple(lo), tuple(lo + a.dl * np.array([a.nx, a.ny, a.nz])),
+                        [cells], [], [], pts)
+    out = str(tmp_path / "cavity.mesh")
+    ingest.write_mesh(m, out)
+    b = ingest.load(path, mesh_path=out, material_dir=MAT).problem
+    assert same_tables(build_tables(a), build_tables(b)) == []
+
+
+def test_vmat_filters():
+    m = ingest.read_vmat(os.path.join(MAT, "epsr=2.vmat"))
+    assert (m.eps.wnorm, list(m.eps.a), list(m.eps.b)) == (1e8, [2.0], [1.0])
+    assert (list(m.mu.a), list(m.mu.b), m.sigma_e, m.sigma_m) == ([1.0], [1.0], 0.0, 0.0)
+    lor = ingest.read_vmat(os.path.join(MAT, "mur_lorentz.vmat"))
+    want = problems.material_test2().materials[0]
+    for f, g in ((lor.eps, want.eps), (lor.mu, want.mu)):
+        assert (f.wnorm, list(f.a), list(f.b)) == (g.wnorm, list(g.a), list(g.b))
+
+
+def test_unknown_geometry_needs_the_mesh_file(tmp_path):
+    p = tmp_path / "sphere.inp"
+    p.write_text("Mesh_outer_boundary_dimension\n0 1 0 1 0 1\n\nMesh_dimensions_in_cells\n10 10 10\n\n"
+                 "volume_list\n1\n1\nsphere\n0.3\n0 0 0\n0.5 0.5 0.5\n\n"
+                 "volume_material_list\n1\n1\nPEC\n1\n1\n\nSimulation_time\n1e-9\n")
+    deck = ingest.read_inp(str(p))
+    assert deck.volumes[1].kind == "sphere"
+    with pytest.raises(ingest.IngestError, match="mesh"):
+        ingest.problem_from(deck)
