@@ -55,7 +55,8 @@ struct tlm_batch {
     tlm_handle* h = nullptr;
     int first = 0, last = 0, ts = 0;  // host copy of the device timestep
     int cur0 = 0, prv0 = 1, nbuf = 2;
-    double* ring = nullptr;
+    double* ring = nullptr;   // cell output rows of every step
+    double* fring = nullptr;  // face output blocks of every connect the batch pulls
     int64_t bytes = 0;
   };
   int form = TLM_FORM_PRECISION_FP32;
@@ -111,6 +112,7 @@ int64_t estimate_bytes(const tlm_handle* h, int n_steps) {
   int64_t out_rows = 0;
   for (int32_t o : h->c_out) out_rows += o != 0;
   b += (int64_t)n_steps * out_rows * 6 * 8;              // output series
+  b += (int64_t)n_steps * (int64_t)h->f_out.size() * 12 * 8;  // face output series (bound)
   return b + (1 << 20);
 }
 
@@ -143,7 +145,9 @@ __device__ inline void batch_pull_args(const BatchSolve& s, PullArgs& a, bool& p
   a.apply_r = true;
   a.ts_connect = s.ts - 1;
   a.small = (s.stz_n > 0 && s.ts % s.stz_n == 0) ? s.stz_value : 0.0;
-  a.fslot = -1;
+  // the pending connect (of step ts-1) records its face outputs in its own
+  // slot of the solve's face series
+  a.fslot = (s.ts > s.first && s.d.n_face_out_rows) ? s.ts - 1 - s.first : -1;
   a.prev = s.planes[s.prv];
   a.prev_mode = s.prv_mode;
   a.prev_apply_r = true;
@@ -387,6 +391,7 @@ int rebuild(tlm_batch* b) {
     std::memset(&s, 0, sizeof s);
     s.d = h->d;
     s.d.ring = e.ring;
+    s.d.face_ring = e.fring;
     s.planes[0] = h->planes[0];
     s.planes[1] = h->planes[1];
     s.planes[2] = h->planes[2];
@@ -537,6 +542,8 @@ void tlm_batch_destroy(tlm_batch* b) {
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   for (auto& e : b->solves)
     if (e.ring) (void)hipFree(e.ring);
+  for (auto& e : b->solves)
+    if (e.fring) (void)hipFree(e.fring);
   if (b->d_solves) (void)hipFree(b->d_solves);
   for (auto* t : b->d_tab)
     if (t) (void)hipFree(t);
@@ -554,7 +561,8 @@ int tlm_batch_add(tlm_batch* b, tlm_handle* h, int32_t first_timestep, int32_t n
   if (h->form != b->form) return bfail(b, TLM_ERR_ARG, "solve and batch numerical forms differ");
   if (n_steps < 1 || first_timestep < 1 || first_timestep + n_steps - 1 > h->g.n_timesteps)
     return bfail(b, TLM_ERR_ARG, "timestep range out of range");
-  if (h->kind != Kind::INCIDENT || h->ring_used || h->host_steps || h->prv_mode != 0)
+  if (h->kind != Kind::INCIDENT || h->ring_used || h->host_steps || h->fring_used || h->host_fsteps ||
+      h->prv_mode != 0)
     return bfail(b, TLM_ERR_STATE, "a batch takes solves that have not been stepped");
   for (auto& e : b->solves)
     if (e.h == h) return bfail(b, TLM_ERR_STATE, "solve is already in the batch");
@@ -563,7 +571,6 @@ int tlm_batch_add(tlm_batch* b, tlm_handle* h, int32_t first_timestep, int32_t n
     return bfail(b, TLM_ERR_BUDGET, "HBM budget of the batch is exhausted");
   int rc;
   if ((rc = finalize(h))) return bfail(b, rc, std::string("finalize: ") + h->err);
-  if (h->d.n_face_out_rows) return bfail(b, TLM_ERR_UNSUPPORTED, "face outputs are read on the single-solve path");
   if (h->n_host) return bfail(b, TLM_ERR_UNSUPPORTED, "host-path cells are exchanged on the single-solve path");
   if (h->n_host_faces) return bfail(b, TLM_ERR_UNSUPPORTED, "host-path faces are exchanged on the single-solve path");
   tlm_batch::Entry e;
@@ -580,9 +587,20 @@ int tlm_batch_add(tlm_batch* b, tlm_handle* h, int32_t first_timestep, int32_t n
     if (hipMalloc(&p, ring_bytes) != hipSuccess) return bfail(b, TLM_ERR_NOMEM, "output series allocation failed");
     e.ring = (double*)p;
   }
+  // face outputs of the connects of steps first .. last-1 (the last one is
+  // pulled after collection, on the single-solve path)
+  const size_t fring_bytes = (size_t)(n_steps - 1) * h->d.n_face_out_rows * 12 * sizeof(double);
+  if (fring_bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, fring_bytes) != hipSuccess) {
+      if (e.ring) (void)hipFree(e.ring);
+      return bfail(b, TLM_ERR_NOMEM, "face output series allocation failed");
+    }
+    e.fring = (double*)p;
+  }
   // the handle's own stream may still be uploading its tables
   BHIP_TRY(b, hipStreamSynchronize(h->stream));
-  e.bytes = h->dev_bytes + (int64_t)ring_bytes;
+  e.bytes = h->dev_bytes + (int64_t)ring_bytes + (int64_t)fring_bytes;
   b->resident += e.bytes;
   b->solves.push_back(e);
   b->dirty = true;
@@ -620,6 +638,17 @@ int tlm_batch_collect(tlm_batch* b, tlm_handle** done, int32_t max_done, int32_t
       h->host_steps = steps;
     }
     if (e.ring) BHIP_TRY(b, hipFree(e.ring));
+    const size_t fw = (size_t)h->d.n_face_out_rows * 12;
+    if (fw && steps > 1) {
+      // connects first .. last-1; the last connect's block follows when the
+      // handle applies it (tlm_read_face_outputs, tlm_connect, tlm_step)
+      h->host_fout.resize((size_t)(steps - 1) * fw);
+      BHIP_TRY(b, hipMemcpy(h->host_fout.data(), e.fring, (size_t)(steps - 1) * fw * sizeof(double),
+                            hipMemcpyDeviceToHost));
+      h->host_ffirst = e.first;
+      h->host_fsteps = steps - 1;
+    }
+    if (e.fring) BHIP_TRY(b, hipFree(e.fring));
     // the handle continues as after tlm_step(first, n): last connect pending
     batch_rotation(e.nbuf, e.cur0, e.prv0, steps, h->cur, h->prv, h->prv_mode);
     h->prv_apply_r = true;
@@ -678,6 +707,7 @@ int tlm_batch_autotune(tlm_batch* b, int32_t steps_per_candidate, int32_t* best_
   for (int c = 0; c < (int)(sizeof cand / sizeof cand[0]); c++) {
     if (!unfinished(b)) break;
     const int k = cand[c].k;
+    if (cand[c].mixed == 0 && b->form == TLM_FORM_EXTENDED_FP32) continue;  // LDS-staged kernel only
     if ((rc = tlm_batch_set_kernels(b, cand[c].mixed, b->streams))) return rc;
     b->graph_steps = k;
     const int n = std::max(k, steps_per_candidate / k * k);

@@ -47,6 +47,18 @@ class Request:
         return self.finish - self.arrival
 
 
+def request_from_files(rid: int, inp_path: str, arrival: float = 0.0, mesh_path: Optional[str] = None,
+                       material_dir: Optional[str] = None, steps: Optional[int] = None) -> Request:
+    """A request given in the reference's own formats: the `.inp` file (and
+    the model builder's `.mesh`, when the geometry needs it), read by
+    ingest.load; steps defaults to the deck's own step count."""
+    from . import ingest
+    from .problem import build_tables
+
+    t = build_tables(ingest.load(inp_path, mesh_path, material_dir).problem)
+    return Request(rid, t, int(steps or t.n_timesteps), arrival)
+
+
 def _results(req: Request, h):
     out, _ = h.read_cell_outputs(req.steps)
     req.outputs = capi.probe_series(out, h.t) if len(out) else out

@@ -28,7 +28,9 @@ const char* tlm_batch_last_error(const tlm_batch* b);
 /* Admit a configured, not yet stepped solve for steps first_timestep ..
  * first_timestep+n_steps-1.  TLM_ERR_BUDGET: it does not fit now (keep it
  * queued and retry after tlm_batch_collect).  The handle must not be used
- * until collected.  Face outputs are not recorded in a batch. */
+ * until collected.  Host-path cells and faces are refused
+ * (TLM_ERR_UNSUPPORTED): the host exchanges them between the split calls
+ * of the single-solve path. */
 int tlm_batch_add(tlm_batch* b, tlm_handle* h, int32_t first_timestep, int32_t n_steps);
 /* Enqueue n_steps steps of every unfinished solve (solves that reach their
  * last step earlier stop there) and return at once: the host can build the
@@ -39,7 +41,9 @@ int tlm_batch_run(tlm_batch* b, int32_t n_steps);
 int tlm_batch_synchronize(tlm_batch* b);
 /* Wait for the enqueued steps, then hand back up to max_done finished
  * solves.  Each is then as after
- * tlm_step over its range: tlm_read_cell_outputs, tlm_read_dft and
+ * tlm_step over its range: tlm_read_cell_outputs, tlm_read_face_outputs
+ * (the batch recorded the face outputs of every connect it pulled; the
+ * last connect's follow when the handle applies it), tlm_read_dft and
  * tlm_download_links work on it. */
 int tlm_batch_collect(tlm_batch* b, tlm_handle** done, int32_t max_done, int32_t* n_done);
 int tlm_batch_info(const tlm_batch* b, int32_t* n_solves, int32_t* n_unfinished, int64_t* resident_bytes);
@@ -52,7 +56,8 @@ int tlm_batch_set_graph_steps(tlm_batch* b, int32_t k);
  * depend on either. */
 int tlm_batch_set_kernels(tlm_batch* b, int32_t mixed, int32_t streams);
 /* Time candidate graph lengths (4, 16, 64 steps) with each mixed-line
- * kernel on the current solves (these are real steps: *steps_done of them
+ * kernel of the form (the extended form has the LDS-staged one only) on
+ * the current solves (these are real steps: *steps_done of them
  * were run) and keep the fastest. */
 int tlm_batch_autotune(tlm_batch* b, int32_t steps_per_candidate, int32_t* best_k, int32_t* steps_done);
 

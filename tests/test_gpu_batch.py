@@ -19,12 +19,15 @@ DECKS = [
     ("pml_material", lambda: problems.pml_material(steps=120), 57),
     ("free_cube_odd", lambda: problems.free_cube(13, 41, r=0.3), 41),
     ("pml_sphere_64", lambda: problems.pml_sphere(64, 50, 8, radius=20.0), 21),
+    ("face_outputs", problems.face_outputs, 45),
+    ("huygens_face_outputs", problems.huygens_face_outputs, 70),
 ]
 
 
 def results(h):
     out, first = h.read_cell_outputs()
-    return out, first, h.read_dft(), h.download_links()[:, 1:-1, 1:-1]
+    fo, ffirst = h.read_face_outputs()
+    return out, first, h.read_dft(), h.download_links()[:, 1:-1, 1:-1], fo, ffirst
 
 
 @pytest.mark.parametrize("kernels", [(-1, 2), (1, 1), (0, 2)])
@@ -65,12 +68,17 @@ def test_batch_matches_alone(form, kernels):
             break
     assert set(done) == set(handles)
     for name in handles:
-        ao, af, ad, av = alone[name]
-        bo, bf, bd, bv = done[name]
+        ao, af, ad, av, afo, aff = alone[name]
+        bo, bf, bd, bv, bfo, bff = done[name]
         assert af == bf == 1 and ao.shape == bo.shape, name
         assert np.array_equal(ao, bo), name
         assert np.array_equal(ad, bd), name
         assert np.array_equal(av, bv), name
+        # face outputs of every connect, the last one applied after collection
+        assert aff == bff and afo.shape == bfo.shape, name
+        assert np.array_equal(afo, bfo), name
+        if "face" in name:
+            assert afo.shape[0] == steps[name] and np.abs(afo).max() > 0, name
     b.close()
     for h in handles.values():
         h.close()
@@ -80,11 +88,12 @@ def test_batch_matches_alone(form, kernels):
 def test_batch_flush_and_continue_alone(form):
     """The small-value flush inside a batch (PML cells recompute the previous
     step's incident voltages with that step's flush), and a collected solve
-    continued on the single-solve path (the link-buffer rotation handed back):
-    the same bits as stepping alone throughout."""
+    continued on the single-solve path (the link-buffer rotation and the
+    face-output series handed back): the same bits as stepping alone
+    throughout."""
     lib = capi.engine_library()
     decks = [("pml_sphere", build_tables(problems.pml_sphere(24, 60)), 30, 60),
-             ("special_mix", build_tables(problems.special_mix()), 17, 40)]
+             ("face_outputs", build_tables(problems.face_outputs(14, 40)), 17, 40)]
     for name, t, n_batch, n_total in decks:
         a = capi.Handle(lib, t, form, output_capacity=n_total)
         a.set_small_to_zero(3, 1e-5)
@@ -105,9 +114,11 @@ def test_batch_flush_and_continue_alone(form):
         assert got and got[0] is h, name
         b.close()
         h.step(n_batch + 1, n_total - n_batch)
-        out, first, dft, links = results(h)
+        out, first, dft, links, fo, ffirst = results(h)
         assert first == 1 and np.array_equal(out, ref[0]), name
         assert np.array_equal(links, ref[3]), name
+        # face outputs: the batch's connects, then the single-solve path's
+        assert ffirst == ref[5] and np.array_equal(fo, ref[4]), name
         h.close()
 
 

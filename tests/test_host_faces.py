@@ -197,8 +197,9 @@ def test_cable_faces_engine(oracle_lib, form):
         # a cable face is a special face: its connect evaluates (own + nb) - own,
         # exact in FP64 for FP32 values (plain and precision forms give the
         # same bits as the ordinary face's swap) but rounded for the extended
-        # form's 32-bit significands
+        # form's 32-bit significands; the next store turns that last-bit
+        # difference into one residual unit (2^-32 relative): 1.3e-10 measured
         for a, b in zip(results(dev), got):
-            assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(a)
+            assert np.linalg.norm(a - b) <= 1e-8 * np.linalg.norm(a)
     else:
         assert same(results(dev), got)

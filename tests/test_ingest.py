@@ -184,3 +184,24 @@ def test_unknown_geometry_needs_the_mesh_file(tmp_path):
     assert deck.volumes[1].kind == "sphere"
     with pytest.raises(ingest.IngestError, match="mesh"):
         ingest.problem_from(deck)
+
+
+@pytest.mark.gpu
+def test_serving_takes_reference_input_files(oracle_lib):
+    """Requests given as the reference's `.inp` files, served with continuous
+    batching, in the plain form: the same bits as the oracle."""
+    from oracle import oracle
+
+    from ggi_tlm_amd import capi, serving
+
+    reqs = [serving.request_from_files(0, os.path.join(D, "tem_pml.inp"), 0.0, steps=300),
+            serving.request_from_files(1, os.path.join(D, "cavity_free_space.inp"), 0.0, steps=200),
+            serving.request_from_files(2, os.path.join(D, "cavity_epsr2.inp"), 0.01, material_dir=MAT,
+                                       steps=400)]
+    st = serving.run_batched(reqs, capi.FORM_PLAIN_FP64, budget_bytes=4 << 30, chunk=32)
+    print(f"[ingest serve] {st}", flush=True)
+    for r in reqs:
+        h = oracle.handle(r.tables, output_capacity=r.steps)
+        h.step(1, r.steps)
+        want = probe_series(h.read_cell_outputs()[0], r.tables)
+        assert np.array_equal(r.outputs, want), r.rid
