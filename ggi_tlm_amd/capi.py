@@ -18,6 +18,11 @@ from .problem import Tables
 HERE = os.path.dirname(os.path.abspath(__file__))
 ENGINE_SO = os.path.join(HERE, "libtlmscn.so")
 
+# tlm_frequency_output (include/tlmscn.h)
+FREQUENCY_OUTPUT_DTYPE = np.dtype([("frequency", np.float64), ("kind", np.int32), ("field_component", np.int32),
+                                   ("first_timestep", np.int32), ("last_timestep", np.int32),
+                                   ("timestep_interval", np.int32), ("n_items", np.int32)])
+
 FORM_PLAIN_FP64 = 0
 FORM_PRECISION_FP32 = 1
 FORM_EXTENDED_FP32 = 2   # FP32 links plus an 8-bit residual plane (32-bit significand)
@@ -63,6 +68,8 @@ SIGNATURES = {
     "set_output_capacity": (C.c_int, [_h, _i32]),
     "read_cell_outputs": (C.c_int, [_h, _p, _i32, C.POINTER(_i32), C.POINTER(_i32)]),
     "read_dft": (C.c_int, [_h, _p]),
+    "set_frequency_outputs": (C.c_int, [_h, _i32, _p, _p, _p, _p]),
+    "read_frequency_output": (C.c_int, [_h, _i32, _p]),
     "set_schedule": (C.c_int, [_h, _i32, _i32, _i32, _i32, _i32]),
     "set_profiling": (C.c_int, [_h, _i32]),
     "set_small_to_zero": (C.c_int, [_h, _i32, C.c_double]),
@@ -224,6 +231,15 @@ class Handle:
         self._check(f["set_output_points"](self.h, len(t.output_rows),
                                             _ptr(self._arr(t.output_rows, i32)),
                                             _ptr(self._arr(t.output_components, i32))))
+        fos = getattr(t, "freq_outputs", None) or []
+        if fos:
+            outs = np.zeros(len(fos), dtype=FREQUENCY_OUTPUT_DTYPE)
+            for k, e in enumerate(fos):
+                outs[k] = (e["frequency"], e["kind"], e["component"], e["first_timestep"], e["last_timestep"],
+                           e["timestep_interval"], len(e["rows"]))
+            cat = lambda key: self._arr(np.concatenate([e[key] for e in fos]).astype(i32), i32)
+            self._check(f["set_frequency_outputs"](self.h, len(fos), _ptr(outs), _ptr(cat("rows")),
+                                                    _ptr(cat("sides")), _ptr(cat("normals"))))
 
     # ---- reference call sequence ---------------------------------------
     def excitation(self, timestep):
@@ -294,6 +310,13 @@ class Handle:
         out = np.zeros((max(npnt, 1), max(nfreq, 1), 2), dtype=np.float64)
         self._check(self.lib.fn["read_dft"](self.h, _ptr(out)))
         return out[:npnt, :nfreq, 0] + 1j * out[:npnt, :nfreq, 1]
+
+    def read_frequency_output(self, k: int):
+        """Accumulators of frequency output k (complex, one per item)."""
+        n = len(self.t.freq_outputs[k]["rows"])
+        out = np.zeros((max(n, 1), 2), dtype=np.float64)
+        self._check(self.lib.fn["read_frequency_output"](self.h, int(k), _ptr(out)))
+        return out[:n, 0] + 1j * out[:n, 1]
 
     def set_layout(self, layout: int):
         """Storage order of the link planes (before the first step)."""
@@ -402,7 +425,7 @@ class Handle:
         self.t = SimpleNamespace(n_output_cells=t.n_output_cells, n_output_faces=t.n_output_faces,
                                  n_timesteps=t.n_timesteps, output_rows=t.output_rows,
                                  output_components=t.output_components, dft_frequencies=t.dft_frequencies,
-                                 nx=t.nx, ny=t.ny, nz=t.nz)
+                                 nx=t.nx, ny=t.ny, nz=t.nz, freq_outputs=getattr(t, "freq_outputs", []))
         self._keep = []
 
     def close(self):

@@ -134,6 +134,14 @@ struct tlm_handle {
   int32_t* d_pcomp = nullptr;
   double* d_freq = nullptr;
   double* d_dft = nullptr;
+  // frequency-domain outputs (tlm_set_frequency_outputs): host copies, the
+  // device tables and one complex accumulator per item
+  std::vector<tlm_frequency_output> fo;
+  std::vector<int32_t> fo_rows, fo_sides, fo_normals;
+  const scn::FreqOut* d_fo = nullptr;
+  const scn::FreqItem* d_fo_items = nullptr;
+  double* d_fo_acc = nullptr;
+  int n_fo_items = 0, n_fo_vol = 0, n_fo_surf = 0;
   int64_t dev_bytes = 0;
 
   // bulk schedule (tlm_set_schedule)

@@ -2227,6 +2227,51 @@ static __global__ void k_dft(const double* __restrict__ ring, int n_out_rows, in
   acc[2 * i + 1] -= v * s * dt;
 }
 
+// Frequency-domain outputs (frequency_output_volumes.F90:309-386,
+// frequency_output_surfaces.F90:297-421): one lane per item of every output
+// of the launch's kind (1 volume, 2 surface) due at this timestep.
+// src: the step's cell-output rows (volumes) or the connect's face-output
+// block (surfaces), rows wide; acc gains value * exp(j y), y = -2 pi f time.
+struct FreqOut {
+  double frequency;
+  int kind, comp, first, last, interval;
+};
+struct FreqItem {
+  int out, row, side, normal;
+};
+
+static __global__ void k_freq_out(const FreqOut* __restrict__ outs, const FreqItem* __restrict__ items, int n,
+                                  int kind, const double* __restrict__ src, int rows, int timestep, double time,
+                                  double* __restrict__ acc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const FreqItem it = items[i];
+  const FreqOut o = outs[it.out];
+  if (o.kind != kind || timestep < o.first || timestep > o.last || (timestep - o.first) % o.interval != 0) return;
+  double value;
+  if (kind == 1) {
+    value = src[(long long)(it.row - 1) * 6 + (o.comp - 1)];
+  } else {
+    const double* f = src + ((long long)(it.row - 1) * 2 + (it.side - 1)) * 6;
+    if (o.comp <= 6) {
+      value = f[o.comp - 1];
+    } else {
+      // J = n x H with the face's outward normal (normx, normy, normz)
+      const double nx = it.normal == 1 ? 1.0 : it.normal == -1 ? -1.0 : 0.0;
+      const double ny = it.normal == 2 ? 1.0 : it.normal == -2 ? -1.0 : 0.0;
+      const double nz = it.normal == 3 ? 1.0 : it.normal == -3 ? -1.0 : 0.0;
+      if (o.comp == 7) value = ny * f[5] - nz * f[4];
+      else if (o.comp == 8) value = nz * f[3] - nx * f[5];
+      else value = nx * f[4] - ny * f[3];
+    }
+  }
+  const double y = -(2.0 * kPI * o.frequency * time);
+  double sy, cy;
+  sincos(y, &sy, &cy);
+  acc[2 * i] += value * cy;
+  acc[2 * i + 1] += value * sy;
+}
+
 // Fortran V(1:12,0:nx+1,0:ny+1,1:nz) <-> planes; thread = storage index.
 template <typename T>
 __global__ void k_aos_to_planes(Dev d, const double* __restrict__ V, T* __restrict__ L) {

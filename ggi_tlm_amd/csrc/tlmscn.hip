@@ -960,9 +960,61 @@ int finalize(tlm_handle* h) {
     HIP_TRY(h, hipMemsetAsync(p, 0, nb, h->stream));
     h->d_dft = (double*)p;
   }
+  if (!h->fo.empty()) {
+    // frequency outputs: items refer to this handle's cell and face output rows
+    std::vector<scn::FreqOut> outs;
+    std::vector<scn::FreqItem> items;
+    size_t k = 0;
+    for (size_t o = 0; o < h->fo.size(); o++) {
+      const tlm_frequency_output& f = h->fo[o];
+      outs.push_back({f.frequency, f.kind, f.field_component, f.first_timestep, f.last_timestep,
+                      f.timestep_interval});
+      const int nrows = f.kind == 1 ? d.n_out_rows : d.n_face_out_rows;
+      for (int i = 0; i < f.n_items; i++, k++) {
+        const int r = h->fo_rows[k];
+        if (r < 1 || r > nrows)
+          return fail(h, TLM_ERR_ARG, "frequency output " + std::to_string(o + 1) + ": output row out of range");
+        const int side = f.kind == 2 ? h->fo_sides[k] : 1, nrm = f.kind == 2 ? h->fo_normals[k] : 0;
+        if (f.kind == 2 && (side < 1 || side > 2 || nrm == 0 || nrm < -3 || nrm > 3))
+          return fail(h, TLM_ERR_ARG, "frequency output " + std::to_string(o + 1) + ": bad face side or normal");
+        items.push_back({(int)o, r, side, nrm});
+        (f.kind == 1 ? h->n_fo_vol : h->n_fo_surf)++;
+      }
+    }
+    h->n_fo_items = (int)items.size();
+    if ((rc = dev_upload(h, outs, &h->d_fo))) return rc;
+    if ((rc = dev_upload(h, items, &h->d_fo_items))) return rc;
+    void* p;
+    const size_t nb = (size_t)std::max(1, h->n_fo_items) * 2 * sizeof(double);
+    if ((rc = dev_alloc(h, nb, &p))) return rc;
+    HIP_TRY(h, hipMemsetAsync(p, 0, nb, h->stream));
+    h->d_fo_acc = (double*)p;
+  }
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   h->finalized = true;
   return TLM_OK;
+}
+
+// Frequency outputs of one kind (1: volumes over the cell-output rows of
+// ring slot `slot` at step `timestep`; 2: surfaces over the face-output
+// block `slot` of the connect of step `timestep`).
+void launch_freq_outputs(tlm_handle* h, int kind, int slot, int timestep) {
+  if (!(kind == 1 ? h->n_fo_vol : h->n_fo_surf) || slot < 0) return;
+  const double dt = h->g.dt;
+  const double* src;
+  int rows;
+  double time;
+  if (kind == 1) {
+    rows = h->d.n_out_rows;
+    src = h->d.ring + (size_t)slot * rows * 6;
+    time = (double)(timestep - 1) * dt;  // cell_output (run_TLM.F90:240)
+  } else {
+    rows = h->d.n_face_out_rows;
+    src = h->d.face_ring + (size_t)slot * rows * 12;
+    time = (double)(timestep - 1) * dt + dt / 2.0;  // face_output (run_TLM.F90:248)
+  }
+  hipLaunchKernelGGL(scn::k_freq_out, dim3((h->n_fo_items + 255) / 256), dim3(256), 0, h->stream, h->d_fo,
+                     h->d_fo_items, h->n_fo_items, kind, src, rows, timestep, time, h->d_fo_acc);
 }
 
 // Move filled ring slots to the host staging vector.
@@ -1259,7 +1311,10 @@ int do_scatter(tlm_handle* h, int timestep) {
       hipLaunchKernelGGL(scn::k_dft, dim3((np * nf + 255) / 256), dim3(256), 0, h->stream, h->d.ring, h->d.n_out_rows,
                          slot, timestep, h->d_prow, h->d_pcomp, np, h->d_freq, nf, h->g.dt, h->d_dft);
     }
+    launch_freq_outputs(h, 1, slot, timestep);
   }
+  // the pulled connect's face outputs
+  if (pull) launch_freq_outputs(h, 2, fslot, a.ts_connect);
   const int out = out_buffer(h);
   h->prv = h->cur;
   h->cur = out;
@@ -1285,6 +1340,7 @@ int materialize(tlm_handle* h) {
     return launch_connect<decltype(t), decltype(plain)::value>(h, a);
   });
   if (rc) return rc;
+  launch_freq_outputs(h, 2, fslot, a.ts_connect);
   HIP_TRY(h, hipGetLastError());
   h->cur = out_buffer(h);  // prv stays: the next step still pulls it again
   h->kind = Kind::INCIDENT;
@@ -1516,6 +1572,42 @@ int tlm_set_dft_frequencies(tlm_handle* h, int32_t n, const double* f) {
   if (!h || n < 0 || (n && !f)) return TLM_ERR_ARG;
   if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
   h->freq.assign(f, f + n);
+  return TLM_OK;
+}
+
+int tlm_set_frequency_outputs(tlm_handle* h, int32_t n, const tlm_frequency_output* outs, const int32_t* rows,
+                              const int32_t* sides, const int32_t* normals) {
+  if (!h || n < 0 || (n && !outs)) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  size_t items = 0;
+  for (int i = 0; i < n; i++) {
+    const tlm_frequency_output& f = outs[i];
+    if ((f.kind != 1 && f.kind != 2) || f.n_items < 0 || f.timestep_interval < 1 || f.field_component < 1 ||
+        f.field_component > (f.kind == 1 ? 6 : 9))
+      return fail(h, TLM_ERR_ARG, "frequency output " + std::to_string(i + 1) + ": bad kind, component or window");
+    items += (size_t)f.n_items;
+  }
+  if (items && (!rows || !sides || !normals)) return TLM_ERR_ARG;
+  h->fo.assign(outs, outs + n);
+  h->fo_rows.assign(rows, rows + items);
+  h->fo_sides.assign(sides, sides + items);
+  h->fo_normals.assign(normals, normals + items);
+  return TLM_OK;
+}
+
+int tlm_read_frequency_output(tlm_handle* h, int32_t k, double* re_im) {
+  if (!h || k < 0 || !re_im) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  if (k >= (int)h->fo.size()) return fail(h, TLM_ERR_ARG, "no such frequency output");
+  if ((rc = materialize(h))) return rc;  // a pending connect adds its face outputs
+  size_t off = 0;
+  for (int i = 0; i < k; i++) off += (size_t)h->fo[i].n_items;
+  const size_t n = (size_t)h->fo[k].n_items * 2;
+  if (n) {
+    HIP_TRY(h, hipMemcpyAsync(re_im, h->d_fo_acc + off * 2, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+  }
   return TLM_OK;
 }
 

@@ -22,8 +22,9 @@ the model builder meshes them: rectangular_block volumes (cells whose
 centre lies inside), rectangular_block, rectangle and x/y/zplane surfaces
 (the cell faces on them), and points (point_to_cell,
 general_mesh_subroutines.F90:345-383); rotated or other shapes need the
-model builder's `.mesh`.  Packets the engine has no path for (surface and
-volume field outputs, frequency-domain outputs, far field, modes, cables)
+model builder's `.mesh`.  Frequency output surfaces and volumes become
+device-accumulated frequency outputs.  Packets the engine has no path for
+(time-domain surface and volume field outputs, far field, modes, cables)
 are listed in `Deck.unsupported` and left out.
 """
 from __future__ import annotations
@@ -38,8 +39,8 @@ import numpy as np
 
 from .constants import CENTRE, FACE_XMAX, FACE_XMIN, FACE_YMAX, FACE_YMIN, FACE_ZMAX, FACE_ZMIN
 from .filters import Sfilter
-from .problem import (PML, Box, ExcitationFunction, HuygensBox, OutputPoint, PointSource, Problem,
-                      VolumeMaterial, point_to_cell)
+from .problem import (PML, Box, ExcitationFunction, FrequencyOutput, HuygensBox, OutputPoint, PointSource,
+                      Problem, VolumeMaterial, point_to_cell)
 
 _LOCATIONS = {"xmin": FACE_XMIN, "xmax": FACE_XMAX, "ymin": FACE_YMIN, "ymax": FACE_YMAX, "zmin": FACE_ZMIN,
               "zmax": FACE_ZMAX, "centre": CENTRE}
@@ -149,6 +150,7 @@ class Deck:
     simulation_time: Optional[float] = None
     reduced_c_factor: float = 1.0
     small_to_zero: Optional[Tuple[int, float]] = None
+    frequency_outputs: List[dict] = field(default_factory=list)
     unsupported: List[str] = field(default_factory=list)
 
 
@@ -317,6 +319,34 @@ def read_inp(path: str) -> Deck:
                 if comp not in _COMPONENTS or loc not in _LOCATIONS:
                     raise IngestError(f"output point {k}: {comp!r} {loc!r}")
                 d.output_points.append({"point": pt, "component": comp, "location": loc, **_time_information(r)})
+        elif line == "frequency_output_surface_list":
+            # read_frequency_output_surface.F90: surface, side, frequency,
+            # component (field or current), time window
+            n = r.ints(1)[0]
+            for k in range(1, n + 1):
+                if r.ints(1)[0] != k:
+                    raise IngestError("frequency output surfaces must be numbered 1..n in order")
+                surf, side = r.ints(1)[0], r.ints(1)[0]
+                freq = r.reals(1)[0]
+                comp = r.word()
+                if comp not in _COMPONENTS:
+                    raise IngestError(f"frequency output surface {k}: component {comp!r} has no path here")
+                d.frequency_outputs.append({"kind": "surface", "number": surf, "side": side, "frequency": freq,
+                                            "component": comp, **_time_information(r)})
+        elif line == "frequency_output_volume_list":
+            # read_frequency_output_volume.F90 (normal format): volume,
+            # frequency, field component, time window
+            n = r.ints(1)[0]
+            for k in range(1, n + 1):
+                if r.ints(1)[0] != k:
+                    raise IngestError("frequency output volumes must be numbered 1..n in order")
+                vol = r.ints(1)[0]
+                freq = r.reals(1)[0]
+                comp = r.word()
+                if comp not in _COMPONENTS[:6]:
+                    raise IngestError(f"frequency output volume {k}: component {comp!r}")
+                d.frequency_outputs.append({"kind": "volume", "number": vol, "frequency": freq, "component": comp,
+                                            **_time_information(r)})
         elif line == "simulation_time":
             d.simulation_time = r.reals(1)[0]
         elif line == "reduced_c_factor":
@@ -685,10 +715,23 @@ def problem_from(deck: Deck, mesh: Optional[MeshFile] = None, material_dir: Opti
     elif deck.pml is not None:
         t, rr, order = deck.pml
         pml = PML(tuple(int(round(x / dl)) for x in t), tuple(rr), order)
+    freq_outputs = []
+    for fo in deck.frequency_outputs:
+        if "first_time" in fo:
+            raise IngestError("frequency output windows given in time (output_time_information) have no path here")
+        window = {"first_timestep": fo["first_timestep"], "last_timestep": fo["last_timestep"],
+                  "timestep_interval": fo["timestep_interval"]}
+        if fo["kind"] == "volume":
+            freq_outputs.append(FrequencyOutput("volume", fo["frequency"], fo["component"],
+                                                region=volume_region(fo["number"]), **window))
+        else:
+            freq_outputs.append(FrequencyOutput("surface", fo["frequency"], fo["component"],
+                                                faces=surface_faces(fo["number"]), side=fo["side"], **window))
     p = Problem(nx=nx, ny=ny, nz=nz, dl=dl, simulation_time=deck.simulation_time, mesh_min=tuple(mesh_min),
                 reflection=tuple(deck.reflection), materials=materials, material_regions=regions,
                 surface_materials=smats, surface_faces=sfaces, functions=list(deck.functions), sources=sources,
-                huygens=huygens, outputs=outputs, pml=pml, reduced_c_factor=deck.reduced_c_factor)
+                huygens=huygens, outputs=outputs, pml=pml, reduced_c_factor=deck.reduced_c_factor,
+                frequency_outputs=freq_outputs)
     info = [{k: v for k, v in o.items() if k not in ("point", "component", "location")} for o in deck.output_points]
     return Ingested(p, list(deck.unsupported), info, sorted(boundary))
 

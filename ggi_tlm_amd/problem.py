@@ -149,6 +149,26 @@ class HuygensBox:
 
 
 @dataclass
+class FrequencyOutput:
+    """A frequency-domain output (frequency_output_volumes.F90:309-386,
+    frequency_output_surfaces.F90:297-421): every step in the window adds
+    value * exp(-j 2 pi f t) per cell (kind 'volume': region, a cell mask
+    object) or per face (kind 'surface': faces (cx, cy, cz, face) of the
+    surface mesh, side +1 the outward normal, -1 the other side of each
+    face).  component: 'ex'..'hz', and 'jx'..'jz' (n x H) for surfaces."""
+
+    kind: str
+    frequency: float
+    component: str
+    region: object = None
+    faces: Optional[list] = None
+    side: int = 1
+    first_timestep: int = 0
+    last_timestep: int = 1000000000
+    timestep_interval: int = 1
+
+
+@dataclass
 class Problem:
     nx: int
     ny: int
@@ -177,6 +197,7 @@ class Problem:
     # faces carrying a cable face junction (face_update_code_to_cable_number
     # = position in this list + 1), (cx, cy, cz, face) as surface_faces
     cable_faces: List[Tuple[int, int, int, int]] = field(default_factory=list)
+    frequency_outputs: List[FrequencyOutput] = field(default_factory=list)
 
     @property
     def dt(self) -> float:
@@ -359,6 +380,7 @@ class Tables:
     dft_frequencies: np.ndarray
     # handy extras (not part of the ABI)
     output_cells: np.ndarray = None   # (n_output_cells, 3) 1-based cx,cy,cz
+    freq_outputs: list = field(default_factory=list)  # frequency output volumes and surfaces
     n_pml_cells: int = 0
 
 
@@ -484,6 +506,11 @@ def build_tables(p: Problem) -> Tables:
     for o in p.outputs:
         if FACES[o.point] == CENTRE:
             local_out[o.cell[2] - 1, o.cell[1] - 1, o.cell[0] - 1] = True
+    # frequency output volumes make their cells output cells
+    # (set_outputs_in_mesh, output.F90:46-)
+    for fq in p.frequency_outputs:
+        if fq.kind == "volume":
+            local_out |= fq.region.mask(nx, ny, nz)
     if (pml_flag & local_exc).any():
         raise ValueError("Excitation intersects PML")
     local_host = np.zeros(shape, dtype=bool)
@@ -636,6 +663,26 @@ def build_tables(p: Problem) -> Tables:
         t = _min_face(nx, ny, nz, (int(cx), int(cy), int(cz)), int(face))
         if t is not None:
             f_cab[t[2] - 1, t[1] - 1, t[0] - 1, t[3]] = k + 1
+    # frequency output surfaces: each face, turned to the other side for
+    # side -1 (frequency_output_surfaces.F90:30-280), is an output face
+    fo_faces = []
+    for fq in p.frequency_outputs:
+        faces = []
+        if fq.kind == "surface":
+            other = {FACE_XMIN: (-1, 0, 0, FACE_XMAX), FACE_XMAX: (1, 0, 0, FACE_XMIN),
+                     FACE_YMIN: (0, -1, 0, FACE_YMAX), FACE_YMAX: (0, 1, 0, FACE_YMIN),
+                     FACE_ZMIN: (0, 0, -1, FACE_ZMAX), FACE_ZMAX: (0, 0, 1, FACE_ZMIN)}
+            for (cx, cy, cz, face) in fq.faces:
+                cx, cy, cz, face = int(cx), int(cy), int(cz), int(face)
+                if fq.side == -1:
+                    di, dj, dk, face = other[face]
+                    cx, cy, cz = cx + di, cy + dj, cz + dk
+                faces.append((cx, cy, cz, face))
+                t = _min_face(nx, ny, nz, (cx, cy, cz), face)
+                if t is None:
+                    raise ValueError("a frequency output surface face lies on the mesh boundary")
+                f_out[t[2] - 1, t[1] - 1, t[0] - 1, t[3]] = True
+        fo_faces.append(faces)
     face_out_pts = []
     for o in p.outputs:
         if FACES[o.point] != CENTRE:
@@ -752,6 +799,37 @@ def build_tables(p: Problem) -> Tables:
             face_out_pts.append((0 if t is None else int(fout_rank_full[face_flat(t)]),
                                  _comp(o.component)))
     out_cells = np.stack(np.nonzero(local_out)[::-1], axis=1) + 1 if flat_out.any() else np.zeros((0, 3), int)
+    # frequency outputs: per item the output row (cell row for volumes, face
+    # row for surfaces, 1-based), the side of the face whose fields it takes
+    # (1: the face is its cell's min face) and the face normal as a signed
+    # axis (J = n x H), in the reference's per-step loop order
+    normal_of = {FACE_XMIN: (1, 1), FACE_XMAX: (2, -1), FACE_YMIN: (1, 2), FACE_YMAX: (2, -2),
+                 FACE_ZMIN: (1, 3), FACE_ZMAX: (2, -3)}
+    freq_outputs = []
+    for fq, faces in zip(p.frequency_outputs, fo_faces):
+        e = {"kind": 1 if fq.kind == "volume" else 2, "frequency": float(fq.frequency),
+             "component": _comp(fq.component), "first_timestep": int(fq.first_timestep),
+             "last_timestep": int(fq.last_timestep), "timestep_interval": int(fq.timestep_interval)}
+        if fq.kind == "volume":
+            if e["component"] > 6:
+                raise ValueError("frequency output volumes take field components Ex..Hz")
+            m = fq.region.mask(nx, ny, nz).ravel()
+            e["rows"] = out_rank[np.nonzero(m)[0]].astype(np.int32)
+            e["sides"] = np.zeros(len(e["rows"]), dtype=np.int32)
+            e["normals"] = np.zeros(len(e["rows"]), dtype=np.int32)
+        else:
+            if e["component"] > 9:
+                raise ValueError("frequency output surfaces take Ex..Hz and Jx..Jz")
+            rows, sides, normals = [], [], []
+            for f in faces:
+                t = _min_face(nx, ny, nz, f[:3], f[3])
+                rows.append(int(fout_rank_full[face_flat(t)]))
+                sides.append(normal_of[f[3]][0])
+                normals.append(normal_of[f[3]][1])
+            e["rows"] = np.asarray(rows, dtype=np.int32)
+            e["sides"] = np.asarray(sides, dtype=np.int32)
+            e["normals"] = np.asarray(normals, dtype=np.int32)
+        freq_outputs.append(e)
 
     return Tables(
         nx=nx, ny=ny, nz=nz, dl=dl, dt=dt, n_timesteps=nts,
@@ -773,5 +851,5 @@ def build_tables(p: Problem) -> Tables:
         n_output_cells=int(flat_out.sum()), n_output_faces=len(out_faces_full),
         face_output_points=face_out_pts,
         dft_frequencies=np.asarray(p.dft_frequencies, dtype=np.float64),
-        output_cells=out_cells, n_pml_cells=n_pml_cells,
+        output_cells=out_cells, n_pml_cells=n_pml_cells, freq_outputs=freq_outputs,
     )

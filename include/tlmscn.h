@@ -214,6 +214,30 @@ int tlm_set_output_points(tlm_handle* h, int32_t n, const int32_t* row,
  * t_n = (n-1) dt, the cell-output time of step n, in FP64. */
 int tlm_set_dft_frequencies(tlm_handle* h, int32_t n, const double* freq);
 
+/* Frequency-domain outputs (SRC/OUTPUT/frequency_output_volumes.F90:309-386
+ * and frequency_output_surfaces.F90:297-421), accumulated on the device:
+ * every step in an output's window (first_timestep <= t <= last_timestep,
+ * (t - first_timestep) % timestep_interval == 0) adds value * exp(-j 2 pi f
+ * time) to each item, time = (t-1) dt for volumes (cell outputs) and
+ * (t-1) dt + dt/2 for surfaces (face outputs of the step's connect).
+ * kind 1: a volume; its items are cell output rows (1-based, as
+ * tlm_set_output_points), value = that row's field_component (1..6).
+ * kind 2: a surface; its items are face output rows (1-based), the side of
+ * the face whose fields they take (1: the cell that owns the face as its
+ * min face, 2: the other) and the outward face normal as a signed axis
+ * (+-1, +-2, +-3); field_component 1..6 takes that field, 7..9 the surface
+ * current J = n x H (Jx, Jy, Jz).  Items of all outputs are concatenated in
+ * rows/sides/normals (sides and normals are read for surfaces only). */
+typedef struct {
+  double frequency;
+  int32_t kind;
+  int32_t field_component;
+  int32_t first_timestep, last_timestep, timestep_interval;
+  int32_t n_items;
+} tlm_frequency_output;
+int tlm_set_frequency_outputs(tlm_handle* h, int32_t n, const tlm_frequency_output* outputs,
+                              const int32_t* rows, const int32_t* sides, const int32_t* normals);
+
 /* ---- link state ---------------------------------------------------------- */
 
 /* mesh::V(1:12, 0:nx+1, 0:ny+1, 1:nz), FP64, Fortran layout.  Only the
@@ -269,6 +293,10 @@ int tlm_read_face_outputs(tlm_handle* h, double* out, int32_t max_steps, int32_t
 
 /* DFT accumulators: re_im[(p*n_freq + k)*2 + {0,1}]. */
 int tlm_read_dft(tlm_handle* h, double* re_im);
+
+/* Accumulators of frequency output k (0-based): re_im[2*i + {0,1}] for its
+ * items i in order. */
+int tlm_read_frequency_output(tlm_handle* h, int32_t k, double* re_im);
 
 /* ---- host-path cells ------------------------------------------------------
  * Cable junction cells (scatter.F90:452-462, cell_centre_cable_junction)

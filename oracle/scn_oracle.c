@@ -113,6 +113,12 @@ struct oracle_handle {
   int n_freq;
   double* freq;
   double* dft; /* (point, freq, 2) */
+  /* frequency-domain outputs: per item its output, row, side, normal and
+     complex accumulator */
+  int n_fo, n_fo_items;
+  tlm_frequency_output* fo;
+  int32_t *fo_out, *fo_row, *fo_side, *fo_norm;
+  double* fo_acc;
   int cap, ring_count, ring_first;
   double* ring; /* cap blocks of n_out_rows*6 */
   /* z-slab threading (the MPI build's z partition, general_mesh_subroutines.F90:868-972,
@@ -189,6 +195,7 @@ void oracle_destroy(oracle_handle* h) {
   free(h->cell_exc_field); free(h->cell_exc_type); free(h->face_exc_field);
   free(h->face_exc_type); free(h->huy); free(h->cell_out); free(h->face_out); free(h->fring);
   free(h->pt_row); free(h->pt_comp); free(h->freq); free(h->dft); free(h->ring);
+  free(h->fo); free(h->fo_out); free(h->fo_row); free(h->fo_side); free(h->fo_norm); free(h->fo_acc);
   free(h->pl_out); free(h->pl_exc); free(h->pl_fexc); free(h->pl_fout);
   free(h->hf_of_code); free(h->hf_tot);
   free(h);
@@ -926,6 +933,39 @@ static void scatter_pml_material_cell(oracle_handle* h, double* v, int PML_cell,
   v[Vynz] = v[Vynz] * exp_yr; v[Vypz] = v[Vypz] * exp_yr;
 }
 
+/* frequency_output_volumes.F90:309-386 (kind 1, the cell outputs of the
+   step, time (t-1) dt) and frequency_output_surfaces.F90:297-421 (kind 2,
+   the face outputs of the step's connect, time (t-1) dt + dt/2, J = n x H
+   for components 7..9): value * exp(j y), y = -2 pi f time, added to each
+   item of every output due at this timestep (get_output_flag,
+   output.F90:564-). */
+static void accumulate_frequency_outputs(oracle_handle* h, int kind, const double* src, int timestep,
+                                         double time) {
+  for (int i = 0; i < h->n_fo_items; i++) {
+    const tlm_frequency_output* o = &h->fo[h->fo_out[i]];
+    if (o->kind != kind || timestep < o->first_timestep || timestep > o->last_timestep ||
+        (timestep - o->first_timestep) % o->timestep_interval != 0)
+      continue;
+    double value;
+    if (kind == 1) {
+      value = src[(size_t)(h->fo_row[i] - 1) * 6 + (o->field_component - 1)];
+    } else {
+      const double* f = &src[((size_t)(h->fo_row[i] - 1) * 2 + (h->fo_side[i] - 1)) * 6];
+      int nrm = h->fo_norm[i];
+      double normx = nrm == 1 ? 1.0 : nrm == -1 ? -1.0 : 0.0;
+      double normy = nrm == 2 ? 1.0 : nrm == -2 ? -1.0 : 0.0;
+      double normz = nrm == 3 ? 1.0 : nrm == -3 ? -1.0 : 0.0;
+      if (o->field_component <= 6) value = f[o->field_component - 1];
+      else if (o->field_component == 7) value = normy * f[5] - normz * f[4];
+      else if (o->field_component == 8) value = normz * f[3] - normx * f[5];
+      else value = normx * f[4] - normy * f[3];
+    }
+    double y = -(2.0 * PI * o->frequency * time);
+    h->fo_acc[2 * i] += value * cos(y);
+    h->fo_acc[2 * i + 1] += value * sin(y);
+  }
+}
+
 static void record_outputs(oracle_handle* h, int timestep) {
   /* cell_output(): copy this step's cell_output_field into the ring and
    * accumulate the probe DFT at time=(timestep-1)*dt (run_TLM.F90:240). */
@@ -951,6 +991,7 @@ static void record_outputs(oracle_handle* h, int timestep) {
       }
     }
   }
+  if (h->n_fo_items && rows) accumulate_frequency_outputs(h, 1, h->cell_out, timestep, (double)(timestep - 1) * h->g.dt);
 }
 
 /* Scan-order counters of scatter() at the start of each z-plane: output
@@ -1472,6 +1513,8 @@ int oracle_connect(oracle_handle* h, int32_t timestep) {
       memcpy(h->fring + (size_t)h->fring_count * w, h->face_out, w * sizeof(double));
       h->fring_count++;
     }
+    if (h->n_fo_items)
+      accumulate_frequency_outputs(h, 2, h->face_out, timestep, (double)(timestep - 1) * h->g.dt + h->g.dt / 2.0);
   }
   return TLM_OK;
 }
@@ -1572,6 +1615,50 @@ int oracle_read_cell_outputs(oracle_handle* h, double* out, int32_t max_steps,
     h->ring_first += n;
   }
   h->ring_count -= n;
+  return TLM_OK;
+}
+
+int oracle_set_frequency_outputs(oracle_handle* h, int32_t n, const tlm_frequency_output* outs, const int32_t* rows,
+                                 const int32_t* sides, const int32_t* normals) {
+  if (n < 0 || (n && !outs)) return TLM_ERR_ARG;
+  int items = 0;
+  for (int i = 0; i < n; i++) {
+    const tlm_frequency_output* f = &outs[i];
+    if ((f->kind != 1 && f->kind != 2) || f->n_items < 0 || f->timestep_interval < 1 || f->field_component < 1 ||
+        f->field_component > (f->kind == 1 ? 6 : 9))
+      return fail(h, TLM_ERR_ARG, "frequency output: bad kind, component or window");
+    items += f->n_items;
+  }
+  if (items && (!rows || !sides || !normals)) return TLM_ERR_ARG;
+  free(h->fo); free(h->fo_out); free(h->fo_row); free(h->fo_side); free(h->fo_norm); free(h->fo_acc);
+  h->n_fo = n;
+  h->n_fo_items = items;
+  h->fo = (tlm_frequency_output*)malloc((size_t)(n ? n : 1) * sizeof(tlm_frequency_output));
+  if (n) memcpy(h->fo, outs, (size_t)n * sizeof(tlm_frequency_output));
+  h->fo_out = (int32_t*)malloc((size_t)(items ? items : 1) * sizeof(int32_t));
+  h->fo_row = dup_i32(rows, (size_t)items);
+  h->fo_side = dup_i32(sides, (size_t)items);
+  h->fo_norm = dup_i32(normals, (size_t)items);
+  h->fo_acc = (double*)calloc((size_t)(items ? items : 1) * 2, sizeof(double));
+  int k = 0;
+  for (int o = 0; o < n; o++) {
+    int nrows = outs[o].kind == 1 ? h->n_out_rows : h->n_face_out_rows;
+    for (int i = 0; i < outs[o].n_items; i++, k++) {
+      h->fo_out[k] = o;
+      if (h->fo_row[k] < 1 || h->fo_row[k] > nrows) return fail(h, TLM_ERR_ARG, "frequency output: row out of range");
+      if (outs[o].kind == 2 && (h->fo_side[k] < 1 || h->fo_side[k] > 2 || h->fo_norm[k] == 0 ||
+                                h->fo_norm[k] < -3 || h->fo_norm[k] > 3))
+        return fail(h, TLM_ERR_ARG, "frequency output: bad face side or normal");
+    }
+  }
+  return TLM_OK;
+}
+
+int oracle_read_frequency_output(oracle_handle* h, int32_t k, double* re_im) {
+  if (k < 0 || k >= h->n_fo || !re_im) return TLM_ERR_ARG;
+  size_t off = 0;
+  for (int i = 0; i < k; i++) off += (size_t)h->fo[i].n_items;
+  memcpy(re_im, h->fo_acc + off * 2, (size_t)h->fo[k].n_items * 2 * sizeof(double));
   return TLM_OK;
 }
 

@@ -1,10 +1,10 @@
 /*
  * scn_oracle.h -- CPU oracle for the SCN scatter/connect hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This is a plain-C, single-threaded, FP64
- * restatement of GGI_TLM v2.1.7's per-timestep loop body, following the
- * Fortran operation order line by line so that its results can be compared
- * bit for bit with the device path.  Only tests/, the smoke check and the
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C FP64 restatement of
+ * GGI_TLM v2.1.7's per-timestep loop body, following the Fortran operation
+ * order line by line so that its results can be compared bit for bit with
+ * the device path (z-slab threads give the same bits as one thread).  Only tests/, the smoke check and the
  * CPU-baseline leg of bench.py load it; the product library never does.
  *
  * It exports the same entry points as include/tlmscn.h with the prefix
@@ -60,6 +60,9 @@ int oracle_set_output_capacity(oracle_handle* h, int32_t steps);
 int oracle_read_cell_outputs(oracle_handle* h, double* out, int32_t max_steps,
                              int32_t* n_read, int32_t* first_step);
 int oracle_read_dft(oracle_handle* h, double* re_im);
+int oracle_set_frequency_outputs(oracle_handle* h, int32_t n, const tlm_frequency_output* outputs,
+                                 const int32_t* rows, const int32_t* sides, const int32_t* normals);
+int oracle_read_frequency_output(oracle_handle* h, int32_t k, double* re_im);
 int oracle_set_schedule(oracle_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap,
                         int32_t mixed);
 int oracle_set_profiling(oracle_handle* h, int32_t enable);

@@ -17,9 +17,10 @@ FILES = {
     "MATERIAL_TEST2/PROBLEM_SPECIFICATION_FILES/mat_test.inp": "material_test2.inp",
     "HUYGENS_SURFACE/PROBLEM_SPECIFICATION_FILES/huygens_test.inp": "huygens_surface.inp",
     "TEM_PML_TEST/PROBLEM_SPECIFICATION_FILES/TEM_cell.inp": "tem_pml.inp",
+    "HELIX_ANTENNA/PROBLEM_SPECIFICATION_FILES/helix.inp": "helix.inp",
     "HELIX_ANTENNA/helix.info": "helix.info",
 }
-MATERIALS = ("epsr=2.vmat", "mur=2.vmat", "mur_lorentz.vmat")
+MATERIALS = ("epsr=2.vmat", "mur=2.vmat", "mur_lorentz.vmat", "epsr=1.5.vmat")
 
 
 def main():
@@ -29,7 +30,7 @@ def main():
     for m in MATERIALS:
         shutil.copyfile(os.path.join(SRC, "MATERIAL_DATA", m), os.path.join(OUT, "materials", m))
     with open(os.path.join(SRC, "HELIX_ANTENNA", "helix.mesh"), "rb") as f, \
-            gzip.open(os.path.join(OUT, "helix.mesh.gz"), "wb") as g:
+            gzip.GzipFile(os.path.join(OUT, "helix.mesh.gz"), "wb", mtime=0) as g:
         g.write(f.read())
 
 

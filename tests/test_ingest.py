@@ -165,6 +165,41 @@ def test_mesh_volumes_become_cell_regions(tmp_path):
     assert same_tables(build_tables(a), build_tables(b)) == []
 
 
+def test_frequency_output_packets(tmp_path):
+    """Frequency output surfaces and volumes (read_frequency_output_surface.F90:70-101,
+    read_frequency_output_volume.F90:120-140): helix.inp's two surfaces on the
+    model builder's mesh hold the face counts of the reference's run log, and
+    a cavity deck with appended packets gives the windows and regions they
+    name."""
+    ing = ingest.load(os.path.join(D, "helix.inp"), os.path.join(D, "helix.mesh.gz"), material_dir=MAT)
+    fo = ing.problem.frequency_outputs
+    assert [(o.kind, o.frequency, o.component, o.side) for o in fo] == [("surface", 1.3e7, "ex", -1),
+                                                                           ("surface", 1.3e7, "ez", -1)]
+    assert "frequency_output_surface_list" not in ing.unsupported
+    t = build_tables(ing.problem)
+    assert [len(e["rows"]) for e in t.freq_outputs] == info_counts()["frequency_surface_faces"]
+    assert [e["component"] for e in t.freq_outputs] == [1, 3]
+    text = open(os.path.join(D, "cavity_free_space.inp")).read()
+    text += ("\nsurface_list\n1\n1\nrectangular_block\n0.4 0.8 1.2\n0 0 0\n0 0 0\n\n"
+             "Frequency_output_surface_list\n1\n1\n1\n1\n2.0e8\nJy\n\n"
+             "Frequency_output_volume_list\n2\n1\n1\n3.0e8 frequency\nHz\n"
+             "output_timestep_information\n5\n40\n3\n2\n1\n2.5e8\nEx\n\n")
+    text += "volume_list\n1\n1\nrectangular_block\n0.2 0.4 0.6\n0 0 0\n0 0 0\n"
+    path = tmp_path / "cavity_fo.inp"
+    path.write_text(text)
+    ing = ingest.load(str(path))
+    s, v1, v2 = ing.problem.frequency_outputs
+    assert (s.kind, s.component, s.side, s.frequency, len(s.faces)) == ("surface", "jy", 1, 2.0e8,
+                                                                         2 * (8 * 12 + 4 * 12 + 4 * 8))
+    assert (v1.kind, v1.component, v1.first_timestep, v1.last_timestep, v1.timestep_interval) == \
+        ("volume", "hz", 5, 40, 3)
+    assert (v2.component, v2.frequency, v2.timestep_interval) == ("ex", 2.5e8, 1)
+    t = build_tables(ing.problem)
+    assert [len(e["rows"]) for e in t.freq_outputs] == [len(s.faces), 2 * 4 * 6, 2 * 4 * 6]
+    e = t.freq_outputs[1]
+    assert (e["first_timestep"], e["last_timestep"], e["timestep_interval"]) == (5, 40, 3)
+
+
 def test_vmat_filters():
     m = ingest.read_vmat(os.path.join(MAT, "epsr=2.vmat"))
     assert (m.eps.wnorm, list(m.eps.a), list(m.eps.b)) == (1e8, [2.0], [1.0])
