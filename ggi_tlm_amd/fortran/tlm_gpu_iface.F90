@@ -53,6 +53,14 @@ module tlm_gpu_iface
     real(c_double) :: offset
   end type tlm_huygens_face
 
+  ! one frequency output volume or surface (tlm_frequency_output)
+  type, bind(c) :: tlm_frequency_output
+    real(c_double) :: frequency
+    integer(c_int32_t) :: kind, field_component
+    integer(c_int32_t) :: first_timestep, last_timestep, timestep_interval
+    integer(c_int32_t) :: n_items
+  end type tlm_frequency_output
+
   interface
 
     integer(c_int) function tlm_create(grid, form, h) bind(c, name='tlm_create')
@@ -238,6 +246,44 @@ module tlm_gpu_iface
       integer(c_int32_t), intent(out) :: variant, rows, zc, xcd_remap, mixed, streams
     end function tlm_get_schedule
 
+    integer(c_int) function tlm_set_schedule(h, variant, rows, zc, xcd_remap, mixed) &
+                                               bind(c, name='tlm_set_schedule')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: variant, rows, zc, xcd_remap, mixed
+    end function tlm_set_schedule
+
+    integer(c_int) function tlm_synchronize(h) bind(c, name='tlm_synchronize')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function tlm_synchronize
+
+    integer(c_int64_t) function tlm_device_bytes(h) bind(c, name='tlm_device_bytes')
+      import :: c_int64_t, c_ptr
+      type(c_ptr), value :: h
+    end function tlm_device_bytes
+
+    integer(c_int) function tlm_set_profiling(h, enable) bind(c, name='tlm_set_profiling')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: enable
+    end function tlm_set_profiling
+
+    ! launches(TLM_N_KCLASS), ms(TLM_N_KCLASS)
+    integer(c_int) function tlm_read_kernel_times(h, launches, ms) bind(c, name='tlm_read_kernel_times')
+      import :: c_int, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int64_t), intent(inout) :: launches(*)
+      real(c_double), intent(inout) :: ms(*)
+    end function tlm_read_kernel_times
+
+    ! cells(TLM_N_KCLASS), bytes(TLM_N_KCLASS)
+    integer(c_int) function tlm_kernel_bytes(h, cells, bytes) bind(c, name='tlm_kernel_bytes')
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int64_t), intent(inout) :: cells(*), bytes(*)
+    end function tlm_kernel_bytes
+
     integer(c_int) function tlm_set_streams(h, streams) bind(c, name='tlm_set_streams')
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
@@ -311,6 +357,22 @@ module tlm_gpu_iface
       type(c_ptr), value :: h
       real(c_double), intent(inout) :: re_im(*)
     end function tlm_read_dft
+
+    integer(c_int) function tlm_set_frequency_outputs(h, n, outputs, rows, sides, normals) &
+                                               bind(c, name='tlm_set_frequency_outputs')
+      import :: c_int, c_int32_t, c_ptr, tlm_frequency_output
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n
+      type(tlm_frequency_output), intent(in) :: outputs(*)
+      integer(c_int32_t), intent(in) :: rows(*), sides(*), normals(*)
+    end function tlm_set_frequency_outputs
+
+    integer(c_int) function tlm_read_frequency_output(h, k, re_im) bind(c, name='tlm_read_frequency_output')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: k
+      real(c_double), intent(inout) :: re_im(*)
+    end function tlm_read_frequency_output
 
     integer(c_int) function tlm_autotune(h, first_timestep, steps_per_candidate, steps_done) &
                                                bind(c, name='tlm_autotune')

@@ -26,6 +26,14 @@ def test_header_matches_binding():
     assert set(batch_symbols()) == set(capi.BATCH_EXPORTED), set(batch_symbols()) ^ set(capi.BATCH_EXPORTED)
 
 
+def test_fortran_module_binds_every_single_solve_entry_point():
+    """tlm_gpu_iface.F90 declares every tlmscn.h entry point under its C name
+    (the batch ABI, tlmbatch.h, is for C and Python serving hosts)."""
+    txt = open(os.path.join(ROOT, "ggi_tlm_amd", "fortran", "tlm_gpu_iface.F90")).read()
+    bound = set(re.findall(r"name='(tlm_[a-z_]+)'", txt))
+    assert set(declared_symbols()) <= bound, set(declared_symbols()) - bound
+
+
 def test_engine_exports_batch_symbols():
     if not os.path.exists(capi.ENGINE_SO):
         pytest.fail("libtlmscn.so not built: run __graft_entry__.build()")
