@@ -23,6 +23,13 @@ FREQUENCY_OUTPUT_DTYPE = np.dtype([("frequency", np.float64), ("kind", np.int32)
                                    ("first_timestep", np.int32), ("last_timestep", np.int32),
                                    ("timestep_interval", np.int32), ("n_items", np.int32)])
 
+# tlm_far_field_surface and tlm_rcs_surface (include/tlmscn.h)
+FAR_FIELD_DTYPE = np.dtype([("frequency", np.float64), ("theta_min", np.float64), ("theta_step", np.float64),
+                            ("phi_min", np.float64), ("phi_step", np.float64), ("n_theta", np.int32),
+                            ("n_phi", np.int32), ("dim", np.int32), ("n_faces", np.int32)])
+RCS_DTYPE = np.dtype([("theta", np.float64), ("phi", np.float64), ("n_faces", np.int32), ("n_points", np.int32),
+                      ("point_offset", np.int32), ("points_per_cell", np.int32)])
+
 FORM_PLAIN_FP64 = 0
 FORM_PRECISION_FP32 = 1
 FORM_EXTENDED_FP32 = 2   # FP32 links plus an 8-bit residual plane (32-bit significand)
@@ -70,6 +77,10 @@ SIGNATURES = {
     "read_dft": (C.c_int, [_h, _p]),
     "set_frequency_outputs": (C.c_int, [_h, _i32, _p, _p, _p, _p]),
     "read_frequency_output": (C.c_int, [_h, _i32, _p]),
+    "set_far_field_surfaces": (C.c_int, [_h, _i32, _p, _p, _p, _p, _p]),
+    "read_far_field": (C.c_int, [_h, _i32, _p]),
+    "set_rcs_surface": (C.c_int, [_h, _p, _p, _p, _p, _p]),
+    "read_rcs": (C.c_int, [_h, _p]),
     "set_schedule": (C.c_int, [_h, _i32, _i32, _i32, _i32, _i32]),
     "set_profiling": (C.c_int, [_h, _i32]),
     "set_small_to_zero": (C.c_int, [_h, _i32, C.c_double]),
@@ -240,6 +251,25 @@ class Handle:
             cat = lambda key: self._arr(np.concatenate([e[key] for e in fos]).astype(i32), i32)
             self._check(f["set_frequency_outputs"](self.h, len(fos), _ptr(outs), _ptr(cat("rows")),
                                                     _ptr(cat("sides")), _ptr(cat("normals"))))
+        ffs = getattr(t, "far_fields", None) or []
+        if ffs:
+            surf = np.zeros(len(ffs), dtype=FAR_FIELD_DTYPE)
+            for k, e in enumerate(ffs):
+                surf[k] = (e["frequency"], e["theta_min"], e["theta_step"], e["phi_min"], e["phi_step"],
+                           e["n_theta"], e["n_phi"], e["dim"], len(e["rows"]))
+            cat = lambda key, dt: self._arr(np.concatenate([np.asarray(e[key], dtype=dt).ravel() for e in ffs]), dt)
+            self._check(f["set_far_field_surfaces"](self.h, len(ffs), _ptr(self._arr(surf, FAR_FIELD_DTYPE)),
+                                                     _ptr(cat("rows", i32)), _ptr(cat("sides", i32)),
+                                                     _ptr(cat("normals", i32)), _ptr(cat("centres", np.float64))))
+        rcs = getattr(t, "rcs", None)
+        if rcs is not None:
+            s = np.zeros(1, dtype=RCS_DTYPE)
+            s[0] = (rcs["theta"], rcs["phi"], len(rcs["rows"]), rcs["n_points"], rcs["point_offset"],
+                    rcs["points_per_cell"])
+            self._check(f["set_rcs_surface"](self.h, _ptr(self._arr(s, RCS_DTYPE)),
+                                              _ptr(self._arr(rcs["rows"], i32)), _ptr(self._arr(rcs["sides"], i32)),
+                                              _ptr(self._arr(rcs["normals"], i32)),
+                                              _ptr(self._arr(np.asarray(rcs["centres"]).ravel(), np.float64))))
 
     # ---- reference call sequence ---------------------------------------
     def excitation(self, timestep):
@@ -317,6 +347,22 @@ class Handle:
         out = np.zeros((max(n, 1), 2), dtype=np.float64)
         self._check(self.lib.fn["read_frequency_output"](self.h, int(k), _ptr(out)))
         return out[:n, 0] + 1j * out[:n, 1]
+
+    def read_far_field(self, k: int):
+        """Far field of far-field surface k at r = 1 m: complex E_theta and
+        E_phi, each of shape (n_theta, n_phi)."""
+        e = self.t.far_fields[k]
+        out = np.zeros((e["n_theta"] * e["n_phi"], 4), dtype=np.float64)
+        self._check(self.lib.fn["read_far_field"](self.h, int(k), _ptr(out)))
+        shape = (e["n_theta"], e["n_phi"])
+        return (out[:, 0] + 1j * out[:, 1]).reshape(shape), (out[:, 2] + 1j * out[:, 3]).reshape(shape)
+
+    def read_rcs(self):
+        """The RCS surface's far-field time series: array (n_points, 4) of
+        E_theta, E_phi, H_theta, H_phi, row p-1 for point p."""
+        out = np.zeros((self.t.rcs["n_points"], 4), dtype=np.float64)
+        self._check(self.lib.fn["read_rcs"](self.h, _ptr(out)))
+        return out
 
     def set_layout(self, layout: int):
         """Storage order of the link planes (before the first step)."""
@@ -425,7 +471,8 @@ class Handle:
         self.t = SimpleNamespace(n_output_cells=t.n_output_cells, n_output_faces=t.n_output_faces,
                                  n_timesteps=t.n_timesteps, output_rows=t.output_rows,
                                  output_components=t.output_components, dft_frequencies=t.dft_frequencies,
-                                 nx=t.nx, ny=t.ny, nz=t.nz, freq_outputs=getattr(t, "freq_outputs", []))
+                                 nx=t.nx, ny=t.ny, nz=t.nz, freq_outputs=getattr(t, "freq_outputs", []),
+                                 far_fields=getattr(t, "far_fields", []), rcs=getattr(t, "rcs", None))
         self._keep = []
 
     def close(self):

@@ -142,6 +142,24 @@ struct tlm_handle {
   const scn::FreqItem* d_fo_items = nullptr;
   double* d_fo_acc = nullptr;
   int n_fo_items = 0, n_fo_vol = 0, n_fo_surf = 0;
+  // far-field surfaces (tlm_set_far_field_surfaces): their J and M
+  // accumulators are frequency-output items after the caller's, from item
+  // ff_item0 on, six blocks of n_faces per surface
+  std::vector<tlm_far_field_surface> ff;
+  std::vector<int32_t> ff_rows, ff_sides, ff_normals;
+  std::vector<double> ff_centres;
+  std::vector<const double*> d_ff_centres;
+  double* d_ff_e = nullptr;  // transform output, sized for the largest surface
+  int ff_item0 = 0;
+  // RCS surface (tlm_set_rcs_surface): faces sorted by K.r, the far-field
+  // time series (4 per point)
+  bool rcs_on = false;
+  tlm_rcs_surface rcs{};
+  std::vector<int32_t> rcs_rows, rcs_sides, rcs_normals;
+  std::vector<double> rcs_centres;
+  std::vector<scn::RcsFace> rcs_faces;  // sorted, host copy
+  const scn::RcsFace* d_rcs_faces = nullptr;
+  double* d_rcs = nullptr;
   int64_t dev_bytes = 0;
 
   // bulk schedule (tlm_set_schedule)

@@ -2232,8 +2232,12 @@ static __global__ void k_dft(const double* __restrict__ ring, int n_out_rows, in
 // of the launch's kind (1 volume, 2 surface) due at this timestep.
 // src: the step's cell-output rows (volumes) or the connect's face-output
 // block (surfaces), rows wide; acc gains value * exp(j y), y = -2 pi f time.
+// One accumulated frequency output (kind 1: cell rows, 2: face rows).
+// comp 1..6: a field; 7..9: J = n x H; 10..12: M = -n x E.  Each due step
+// adds value * exp(-j 2 pi f time) * scale (scale: 1, or dt for far-field
+// surfaces).
 struct FreqOut {
-  double frequency;
+  double frequency, scale;
   int kind, comp, first, last, interval;
 };
 struct FreqItem {
@@ -2262,14 +2266,179 @@ static __global__ void k_freq_out(const FreqOut* __restrict__ outs, const FreqIt
       const double nz = it.normal == 3 ? 1.0 : it.normal == -3 ? -1.0 : 0.0;
       if (o.comp == 7) value = ny * f[5] - nz * f[4];
       else if (o.comp == 8) value = nz * f[3] - nx * f[5];
-      else value = nx * f[4] - ny * f[3];
+      else if (o.comp == 9) value = nx * f[4] - ny * f[3];
+      // M = -n x E
+      else if (o.comp == 10) value = -(ny * f[2] - nz * f[1]);
+      else if (o.comp == 11) value = -(nz * f[0] - nx * f[2]);
+      else value = -(nx * f[1] - ny * f[0]);
     }
   }
   const double y = -(2.0 * kPI * o.frequency * time);
   double sy, cy;
   sincos(y, &sy, &cy);
-  acc[2 * i] += value * cy;
-  acc[2 * i + 1] += value * sy;
+  acc[2 * i] += value * (cy * o.scale);
+  acc[2 * i + 1] += value * (sy * o.scale);
+}
+
+// Far-field transform of one surface (far_field_surfaces.F90:370-712):
+// one workgroup per angle sums the N and L integrals over the faces and
+// forms E_theta, E_phi at r = 1 m.  acc holds the surface's accumulators
+// component-major (Jx, Jy, Jz, Mx, My, Mz blocks of nf complex values).
+struct FarFieldArgs {
+  double beta, dA, theta_min, theta_step, phi_min, phi_step, prop2_re, prop2_im;
+  int n_theta, n_phi, dim, nf;
+};
+
+__device__ inline void cmul_acc(double& sr, double& si, double ar, double ai, double br, double bi, double s) {
+  // s += (a * b) * s_real, complex a, b
+  const double pr = ar * br - ai * bi, pi = ar * bi + ai * br;
+  sr += pr * s;
+  si += pi * s;
+}
+
+static __global__ void __launch_bounds__(256) k_far_field(FarFieldArgs s, const double* __restrict__ acc,
+                                                          const double* __restrict__ centres,
+                                                          double* __restrict__ e) {
+  __shared__ double red[8][256];
+  const int a = blockIdx.x, tid = threadIdx.x;
+  const int it = a / s.n_phi, ip = a - it * s.n_phi;
+  const double theta = s.theta_min + (double)it * s.theta_step;
+  const double phi = s.phi_min + (double)ip * s.phi_step;
+  const double ct = cos(theta), st = sin(theta), cp = cos(phi), sp = sin(phi);
+  const double r1 = st * cp, r2 = st * sp, r3 = ct;
+  const int nf = s.nf;
+  const double* J0 = acc;
+  const double* J1 = acc + 2 * (size_t)nf;
+  const double* J2 = acc + 4 * (size_t)nf;
+  const double* M0 = acc + 6 * (size_t)nf;
+  const double* M1 = acc + 8 * (size_t)nf;
+  const double* M2 = acc + 10 * (size_t)nf;
+  // 3D: N_theta, N_phi, L_theta, L_phi; 2D: N_z, N_phi, L_z, L_phi
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int f = tid; f < nf; f += 256) {
+    const double* rp = centres + 3 * (size_t)f;
+    const double x = s.dim == 2 ? r1 * rp[0] + r2 * rp[1] : r1 * rp[0] + r2 * rp[1] + r3 * rp[2];
+    double pim, pre;
+    sincos(s.beta * x, &pim, &pre);
+    const double jxr = J0[2 * f], jxi = J0[2 * f + 1], jyr = J1[2 * f], jyi = J1[2 * f + 1];
+    const double jzr = J2[2 * f], jzi = J2[2 * f + 1];
+    const double mxr = M0[2 * f], mxi = M0[2 * f + 1], myr = M1[2 * f], myi = M1[2 * f + 1];
+    const double mzr = M2[2 * f], mzi = M2[2 * f + 1];
+    if (s.dim == 2) {
+      cmul_acc(v[0], v[1], jzr, jzi, pre, pim, s.dA);
+      cmul_acc(v[2], v[3], -jxr * sp + jyr * cp, -jxi * sp + jyi * cp, pre, pim, s.dA);
+      cmul_acc(v[4], v[5], mzr, mzi, pre, pim, s.dA);
+      cmul_acc(v[6], v[7], -mxr * sp + myr * cp, -mxi * sp + myi * cp, pre, pim, s.dA);
+    } else {
+      cmul_acc(v[0], v[1], jxr * ct * cp + jyr * ct * sp - jzr * st, jxi * ct * cp + jyi * ct * sp - jzi * st, pre,
+               pim, s.dA);
+      cmul_acc(v[2], v[3], -jxr * sp + jyr * cp, -jxi * sp + jyi * cp, pre, pim, s.dA);
+      cmul_acc(v[4], v[5], mxr * ct * cp + myr * ct * sp - mzr * st, mxi * ct * cp + myi * ct * sp - mzi * st, pre,
+               pim, s.dA);
+      cmul_acc(v[6], v[7], -mxr * sp + myr * cp, -mxi * sp + myi * cp, pre, pim, s.dA);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; k++) red[k][tid] = v[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+#pragma unroll
+      for (int k = 0; k < 8; k++) red[k][tid] += red[k][tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const double nar = red[0][0], nai = red[1][0], npr = red[2][0], npi = red[3][0];
+    const double lar = red[4][0], lai = red[5][0], lpr = red[6][0], lpi = red[7][0];
+    const double p2r = s.prop2_re, p2i = s.prop2_im;
+    double qr, qi, ur, ui;
+    if (s.dim == 2) {
+      // E_theta = -prop2 (L_phi - Z0 N_z), E_phi = prop2 (L_z + Z0 N_phi)
+      qr = lpr - kZ0 * nar; qi = lpi - kZ0 * nai;
+      ur = lar + kZ0 * npr; ui = lai + kZ0 * npi;
+    } else {
+      // E_theta = -prop2 (L_phi + Z0 N_theta), E_phi = prop2 (L_theta - Z0 N_phi)
+      qr = lpr + kZ0 * nar; qi = lpi + kZ0 * nai;
+      ur = lar - kZ0 * npr; ui = lai - kZ0 * npi;
+    }
+    e[4 * (size_t)a] = -(p2r * qr - p2i * qi);
+    e[4 * (size_t)a + 1] = -(p2r * qi + p2i * qr);
+    e[4 * (size_t)a + 2] = p2r * ur - p2i * ui;
+    e[4 * (size_t)a + 3] = p2r * ui + p2i * ur;
+  }
+}
+
+// RCS surface (RCS_surfaces.F90:386-559).  Faces are sorted by d = K.r
+// ascending, so the far-field point of a face at one step,
+// rcs_point(d), never increases along them and the faces of one point are
+// a contiguous run.  One workgroup per point of this step's range sums its
+// run (binary search for its ends) and adds the sums to the point.
+struct RcsFace {
+  int row, side, normal, pad;
+  double d;
+};
+struct RcsArgs {
+  double ct, st, cp, sp, dA, time_c0, dl;
+  int points_per_cell, offset, nf, p_lo, rows;
+};
+
+__host__ __device__ inline int rcs_point(double d, double time_c0, int points_per_cell, double dl, int offset) {
+  const double dm = -d + time_c0;
+  return (int)((double)points_per_cell * dm / dl) - offset;
+}
+
+static __global__ void __launch_bounds__(256) k_rcs(RcsArgs a, const RcsFace* __restrict__ faces,
+                                                    const double* __restrict__ src, double* __restrict__ E) {
+  __shared__ double red[4][256];
+  const int tid = threadIdx.x, p = a.p_lo + blockIdx.x;
+  int lo = 0, hi = a.nf;
+  while (lo < hi) {  // first face with point <= p
+    const int mid = (lo + hi) >> 1;
+    if (rcs_point(faces[mid].d, a.time_c0, a.points_per_cell, a.dl, a.offset) <= p) hi = mid;
+    else lo = mid + 1;
+  }
+  const int i0 = lo;
+  hi = a.nf;
+  while (lo < hi) {  // first face with point < p
+    const int mid = (lo + hi) >> 1;
+    if (rcs_point(faces[mid].d, a.time_c0, a.points_per_cell, a.dl, a.offset) < p) hi = mid;
+    else lo = mid + 1;
+  }
+  const int i1 = lo;
+  const double q = 4.0 * kPI;
+  double v[4] = {0, 0, 0, 0};
+  for (int i = i0 + tid; i < i1; i += 256) {
+    const RcsFace fc = faces[i];
+    const double* f = src + ((size_t)(fc.row - 1) * 2 + (fc.side - 1)) * 6;
+    const double nx = fc.normal == 1 ? 1.0 : fc.normal == -1 ? -1.0 : 0.0;
+    const double ny = fc.normal == 2 ? 1.0 : fc.normal == -2 ? -1.0 : 0.0;
+    const double nz = fc.normal == 3 ? 1.0 : fc.normal == -3 ? -1.0 : 0.0;
+    const double jx = ny * f[5] - nz * f[4], jy = nz * f[3] - nx * f[5], jz = nx * f[4] - ny * f[3];
+    const double mx = -(ny * f[2] - nz * f[1]), my = -(nz * f[0] - nx * f[2]), mz = -(nx * f[1] - ny * f[0]);
+    const double nth = (jx * a.ct * a.cp + jy * a.ct * a.sp - jz * a.st) * a.dA / q;
+    const double nph = (-jx * a.sp + jy * a.cp) * a.dA / q;
+    const double lth = (mx * a.ct * a.cp + my * a.ct * a.sp - mz * a.st) * a.dA / q;
+    const double lph = (-mx * a.sp + my * a.cp) * a.dA / q;
+    v[0] += -(lph + kZ0 * nth);
+    v[1] += lth - kZ0 * nph;
+    v[2] += nph - lth / kZ0;
+    v[3] += -(nth + lph / kZ0);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) red[k][tid] = v[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) red[k][tid] += red[k][tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && i1 > i0) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) E[4 * (size_t)(p - 1) + k] += red[k][0];
+  }
 }
 
 // Fortran V(1:12,0:nx+1,0:ny+1,1:nz) <-> planes; thread = storage index.

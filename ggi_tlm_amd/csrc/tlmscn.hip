@@ -960,14 +960,14 @@ int finalize(tlm_handle* h) {
     HIP_TRY(h, hipMemsetAsync(p, 0, nb, h->stream));
     h->d_dft = (double*)p;
   }
-  if (!h->fo.empty()) {
+  if (!h->fo.empty() || !h->ff.empty()) {
     // frequency outputs: items refer to this handle's cell and face output rows
     std::vector<scn::FreqOut> outs;
     std::vector<scn::FreqItem> items;
     size_t k = 0;
     for (size_t o = 0; o < h->fo.size(); o++) {
       const tlm_frequency_output& f = h->fo[o];
-      outs.push_back({f.frequency, f.kind, f.field_component, f.first_timestep, f.last_timestep,
+      outs.push_back({f.frequency, 1.0, f.kind, f.field_component, f.first_timestep, f.last_timestep,
                       f.timestep_interval});
       const int nrows = f.kind == 1 ? d.n_out_rows : d.n_face_out_rows;
       for (int i = 0; i < f.n_items; i++, k++) {
@@ -981,6 +981,37 @@ int finalize(tlm_handle* h) {
         (f.kind == 1 ? h->n_fo_vol : h->n_fo_surf)++;
       }
     }
+    // far-field surfaces: J (components 7..9) and M (10..12) of every face
+    // at every step, times dt (face_output_far_field, far_field_surfaces.F90:
+    // 252-351)
+    h->ff_item0 = (int)items.size();
+    size_t kf = 0, most = 1;
+    h->d_ff_centres.clear();
+    for (size_t sfc = 0; sfc < h->ff.size(); sfc++) {
+      const tlm_far_field_surface& f = h->ff[sfc];
+      for (int c = 7; c <= 12; c++) {
+        const int o = (int)outs.size();
+        outs.push_back({f.frequency, h->g.dt, 2, c, 1, 2147483647, 1});
+        for (int i = 0; i < f.n_faces; i++) {
+          const int r = h->ff_rows[kf + i];
+          if (r < 1 || r > d.n_face_out_rows)
+            return fail(h, TLM_ERR_ARG, "far-field surface " + std::to_string(sfc + 1) + ": output row out of range");
+          items.push_back({o, r, h->ff_sides[kf + i], h->ff_normals[kf + i]});
+          h->n_fo_surf++;
+        }
+      }
+      std::vector<double> cen(h->ff_centres.begin() + 3 * kf, h->ff_centres.begin() + 3 * (kf + f.n_faces));
+      const double* dc;
+      if ((rc = dev_upload(h, cen, &dc))) return rc;
+      h->d_ff_centres.push_back(dc);
+      most = std::max(most, (size_t)f.n_theta * f.n_phi);
+      kf += f.n_faces;
+    }
+    if (!h->ff.empty()) {
+      void* pe;
+      if ((rc = dev_alloc(h, most * 4 * sizeof(double), &pe))) return rc;
+      h->d_ff_e = (double*)pe;
+    }
     h->n_fo_items = (int)items.size();
     if ((rc = dev_upload(h, outs, &h->d_fo))) return rc;
     if ((rc = dev_upload(h, items, &h->d_fo_items))) return rc;
@@ -990,9 +1021,58 @@ int finalize(tlm_handle* h) {
     HIP_TRY(h, hipMemsetAsync(p, 0, nb, h->stream));
     h->d_fo_acc = (double*)p;
   }
+  if (h->rcs_on) {
+    // faces sorted by d = K.r (initialise_rcs_surfaces, RCS_surfaces.F90:
+    // 159-367): K the unit vector of (theta, phi), r the face centre
+    const double kx = sin(h->rcs.theta) * cos(h->rcs.phi), ky = sin(h->rcs.theta) * sin(h->rcs.phi),
+                 kz = cos(h->rcs.theta);
+    h->rcs_faces.clear();
+    for (int i = 0; i < h->rcs.n_faces; i++) {
+      const int r = h->rcs_rows[i];
+      if (r < 1 || r > d.n_face_out_rows) return fail(h, TLM_ERR_ARG, "RCS surface: output row out of range");
+      const double* c = &h->rcs_centres[3 * (size_t)i];
+      h->rcs_faces.push_back({r, h->rcs_sides[i], h->rcs_normals[i], 0, kx * c[0] + ky * c[1] + kz * c[2]});
+    }
+    std::stable_sort(h->rcs_faces.begin(), h->rcs_faces.end(),
+                     [](const scn::RcsFace& a, const scn::RcsFace& b) { return a.d < b.d; });
+    if ((rc = dev_upload(h, h->rcs_faces, &h->d_rcs_faces))) return rc;
+    void* p;
+    const size_t nb = (size_t)h->rcs.n_points * 4 * sizeof(double);
+    if ((rc = dev_alloc(h, nb, &p))) return rc;
+    HIP_TRY(h, hipMemsetAsync(p, 0, nb, h->stream));
+    h->d_rcs = (double*)p;
+  }
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   h->finalized = true;
   return TLM_OK;
+}
+
+// The RCS surface's contributions of the face outputs in block `slot` of
+// the connect of step `timestep` (face_output_rcs, RCS_surfaces.F90:386-559).
+void launch_rcs(tlm_handle* h, int slot, int timestep) {
+  if (!h->rcs_on || slot < 0 || h->rcs_faces.empty()) return;
+  const double dt = h->g.dt, dl = h->g.dl;
+  const double time = (double)(timestep - 1) * dt + dt / 2.0;
+  scn::RcsArgs a;
+  a.ct = cos(h->rcs.theta);
+  a.st = sin(h->rcs.theta);
+  a.cp = cos(h->rcs.phi);
+  a.sp = sin(h->rcs.phi);
+  a.dA = dl * dl;
+  a.time_c0 = time * scn::kC0;
+  a.dl = dl;
+  a.points_per_cell = h->rcs.points_per_cell;
+  a.offset = h->rcs.point_offset;
+  a.nf = (int)h->rcs_faces.size();
+  a.rows = h->d.n_face_out_rows;
+  // points of the faces with the smallest and largest K.r
+  const int p_hi = scn::rcs_point(h->rcs_faces.front().d, a.time_c0, a.points_per_cell, dl, a.offset);
+  const int p_lo = scn::rcs_point(h->rcs_faces.back().d, a.time_c0, a.points_per_cell, dl, a.offset);
+  const int lo = std::max(p_lo, 1), hi = std::min(p_hi, h->rcs.n_points);
+  if (lo > hi) return;
+  a.p_lo = lo;
+  hipLaunchKernelGGL(scn::k_rcs, dim3(hi - lo + 1), dim3(256), 0, h->stream, a, h->d_rcs_faces,
+                     h->d.face_ring + (size_t)slot * h->d.n_face_out_rows * 12, h->d_rcs);
 }
 
 // Frequency outputs of one kind (1: volumes over the cell-output rows of
@@ -1314,7 +1394,10 @@ int do_scatter(tlm_handle* h, int timestep) {
     launch_freq_outputs(h, 1, slot, timestep);
   }
   // the pulled connect's face outputs
-  if (pull) launch_freq_outputs(h, 2, fslot, a.ts_connect);
+  if (pull) {
+    launch_freq_outputs(h, 2, fslot, a.ts_connect);
+    launch_rcs(h, fslot, a.ts_connect);
+  }
   const int out = out_buffer(h);
   h->prv = h->cur;
   h->cur = out;
@@ -1341,6 +1424,7 @@ int materialize(tlm_handle* h) {
   });
   if (rc) return rc;
   launch_freq_outputs(h, 2, fslot, a.ts_connect);
+  launch_rcs(h, fslot, a.ts_connect);
   HIP_TRY(h, hipGetLastError());
   h->cur = out_buffer(h);  // prv stays: the next step still pulls it again
   h->kind = Kind::INCIDENT;
@@ -1608,6 +1692,108 @@ int tlm_read_frequency_output(tlm_handle* h, int32_t k, double* re_im) {
     HIP_TRY(h, hipMemcpyAsync(re_im, h->d_fo_acc + off * 2, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
   }
+  return TLM_OK;
+}
+
+int tlm_set_far_field_surfaces(tlm_handle* h, int32_t n, const tlm_far_field_surface* surfaces, const int32_t* rows,
+                               const int32_t* sides, const int32_t* normals, const double* centres) {
+  if (!h || n < 0 || (n && !surfaces)) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  size_t faces = 0;
+  for (int i = 0; i < n; i++) {
+    const tlm_far_field_surface& f = surfaces[i];
+    if (f.n_faces < 0 || f.n_theta < 1 || f.n_phi < 1 || (f.dim != 2 && f.dim != 3) ||
+        (f.dim == 2 && f.n_theta != 1) || (double)f.n_theta * f.n_phi > 1e8)
+      return fail(h, TLM_ERR_ARG, "far-field surface " + std::to_string(i + 1) + ": bad dimension or angles");
+    faces += (size_t)f.n_faces;
+  }
+  if (faces && (!rows || !sides || !normals || !centres)) return TLM_ERR_ARG;
+  for (size_t i = 0; i < faces; i++)
+    if (sides[i] < 1 || sides[i] > 2 || normals[i] == 0 || normals[i] < -3 || normals[i] > 3)
+      return fail(h, TLM_ERR_ARG, "far-field surface: bad face side or normal");
+  h->ff.assign(surfaces, surfaces + n);
+  h->ff_rows.assign(rows, rows + faces);
+  h->ff_sides.assign(sides, sides + faces);
+  h->ff_normals.assign(normals, normals + faces);
+  h->ff_centres.assign(centres, centres + 3 * faces);
+  return TLM_OK;
+}
+
+int tlm_read_far_field(tlm_handle* h, int32_t k, double* e) {
+  if (!h || k < 0 || !e) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  if (k >= (int)h->ff.size()) return fail(h, TLM_ERR_ARG, "no such far-field surface");
+  if ((rc = materialize(h))) return rc;  // a pending connect adds its currents
+  const tlm_far_field_surface& f = h->ff[k];
+  size_t item = (size_t)h->ff_item0;
+  for (int i = 0; i < k; i++) item += 6 * (size_t)h->ff[i].n_faces;
+  scn::FarFieldArgs a;
+  // the reference forms j beta as j*cmplx(beta) (far_field_surfaces.F90:431,
+  // :561): a default-kind CMPLX, so beta enters rounded to a float
+  a.beta = (double)(float)(2.0 * scn::kPI * f.frequency / scn::kC0);
+  a.dA = f.dim == 2 ? h->g.dl : h->g.dl * h->g.dl;
+  a.theta_min = f.theta_min;
+  a.theta_step = f.theta_step;
+  a.phi_min = f.phi_min;
+  a.phi_step = f.phi_step;
+  a.n_theta = f.n_theta;
+  a.n_phi = f.n_phi;
+  a.dim = f.dim;
+  a.nf = f.n_faces;
+  // r0 = 1 m; exp(j beta r0) = (cos beta, sin beta)
+  const double cb = cos(a.beta), sb = sin(a.beta);
+  if (f.dim == 2) {
+    // sqrt(j beta / (8 pi r0)) exp(j beta r0); the principal root of j y is
+    // sqrt(y/2) (1 + j)
+    const double rt = sqrt(0.5 * (a.beta / (8.0 * scn::kPI)));
+    a.prop2_re = rt * cb - rt * sb;
+    a.prop2_im = rt * sb + rt * cb;
+  } else {
+    // j beta exp(j beta r0) / (4 pi r0)
+    a.prop2_re = (0.0 * cb - a.beta * sb) / (4.0 * scn::kPI);
+    a.prop2_im = (0.0 * sb + a.beta * cb) / (4.0 * scn::kPI);
+  }
+  const int n_angles = f.n_theta * f.n_phi;
+  hipLaunchKernelGGL(scn::k_far_field, dim3(n_angles), dim3(256), 0, h->stream, a, h->d_fo_acc + 2 * item,
+                     h->d_ff_centres[k], h->d_ff_e);
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipMemcpyAsync(e, h->d_ff_e, (size_t)n_angles * 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return TLM_OK;
+}
+
+int tlm_set_rcs_surface(tlm_handle* h, const tlm_rcs_surface* s, const int32_t* rows, const int32_t* sides,
+                        const int32_t* normals, const double* centres) {
+  if (!h) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  if (!s) {
+    h->rcs_on = false;
+    return TLM_OK;
+  }
+  if (s->n_faces < 0 || s->n_points < 1 || s->points_per_cell < 1 || (s->n_faces && (!rows || !sides || !normals || !centres)))
+    return fail(h, TLM_ERR_ARG, "RCS surface: bad faces or far-field points");
+  for (int i = 0; i < s->n_faces; i++)
+    if (sides[i] < 1 || sides[i] > 2 || normals[i] == 0 || normals[i] < -3 || normals[i] > 3)
+      return fail(h, TLM_ERR_ARG, "RCS surface: bad face side or normal");
+  h->rcs_on = true;
+  h->rcs = *s;
+  h->rcs_rows.assign(rows, rows + s->n_faces);
+  h->rcs_sides.assign(sides, sides + s->n_faces);
+  h->rcs_normals.assign(normals, normals + s->n_faces);
+  h->rcs_centres.assign(centres, centres + 3 * (size_t)s->n_faces);
+  return TLM_OK;
+}
+
+int tlm_read_rcs(tlm_handle* h, double* out) {
+  if (!h || !out) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  if (!h->rcs_on) return fail(h, TLM_ERR_ARG, "no RCS surface");
+  if ((rc = materialize(h))) return rc;
+  HIP_TRY(h, hipMemcpyAsync(out, h->d_rcs, (size_t)h->rcs.n_points * 4 * sizeof(double), hipMemcpyDeviceToHost,
+                            h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
   return TLM_OK;
 }
 

@@ -61,6 +61,21 @@ module tlm_gpu_iface
     integer(c_int32_t) :: n_items
   end type tlm_frequency_output
 
+  ! one far-field surface (tlm_far_field_surface); angles in radians
+  type, bind(c) :: tlm_far_field_surface
+    real(c_double) :: frequency
+    real(c_double) :: theta_min, theta_step, phi_min, phi_step
+    integer(c_int32_t) :: n_theta, n_phi
+    integer(c_int32_t) :: dim
+    integer(c_int32_t) :: n_faces
+  end type tlm_far_field_surface
+
+  ! the RCS surface (tlm_rcs_surface); angles in radians
+  type, bind(c) :: tlm_rcs_surface
+    real(c_double) :: theta, phi
+    integer(c_int32_t) :: n_faces, n_points, point_offset, points_per_cell
+  end type tlm_rcs_surface
+
   interface
 
     integer(c_int) function tlm_create(grid, form, h) bind(c, name='tlm_create')
@@ -373,6 +388,40 @@ module tlm_gpu_iface
       integer(c_int32_t), value :: k
       real(c_double), intent(inout) :: re_im(*)
     end function tlm_read_frequency_output
+
+    integer(c_int) function tlm_set_far_field_surfaces(h, n, surfaces, rows, sides, normals, centres) &
+                                               bind(c, name='tlm_set_far_field_surfaces')
+      import :: c_int, c_int32_t, c_ptr, c_double, tlm_far_field_surface
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n
+      type(tlm_far_field_surface), intent(in) :: surfaces(*)
+      integer(c_int32_t), intent(in) :: rows(*), sides(*), normals(*)
+      real(c_double), intent(in) :: centres(*)
+    end function tlm_set_far_field_surfaces
+
+    ! e(4, n_phi * n_theta): Re, Im E_theta, Re, Im E_phi per angle (phi fastest)
+    integer(c_int) function tlm_read_far_field(h, k, e) bind(c, name='tlm_read_far_field')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: k
+      real(c_double), intent(inout) :: e(*)
+    end function tlm_read_far_field
+
+    integer(c_int) function tlm_set_rcs_surface(h, surface, rows, sides, normals, centres) &
+                                               bind(c, name='tlm_set_rcs_surface')
+      import :: c_int, c_int32_t, c_ptr, c_double, tlm_rcs_surface
+      type(c_ptr), value :: h
+      type(tlm_rcs_surface), intent(in) :: surface
+      integer(c_int32_t), intent(in) :: rows(*), sides(*), normals(*)
+      real(c_double), intent(in) :: centres(*)
+    end function tlm_set_rcs_surface
+
+    ! out(4, n_points): E_theta, E_phi, H_theta, H_phi per far-field point
+    integer(c_int) function tlm_read_rcs(h, out) bind(c, name='tlm_read_rcs')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(inout) :: out(*)
+    end function tlm_read_rcs
 
     integer(c_int) function tlm_autotune(h, first_timestep, steps_per_candidate, steps_done) &
                                                bind(c, name='tlm_autotune')

@@ -22,14 +22,15 @@ the model builder meshes them: rectangular_block volumes (cells whose
 centre lies inside), rectangular_block, rectangle and x/y/zplane surfaces
 (the cell faces on them), and points (point_to_cell,
 general_mesh_subroutines.F90:345-383); rotated or other shapes need the
-model builder's `.mesh`.  Frequency output surfaces and volumes become
-device-accumulated frequency outputs.  Packets the engine has no path for
-(time-domain surface and volume field outputs, far field, modes, cables)
-are listed in `Deck.unsupported` and left out.
+model builder's `.mesh`.  Frequency output surfaces and volumes,
+far-field surfaces and the RCS surface become device-accumulated outputs.
+Packets the engine has no path for (time-domain surface and volume field
+outputs, modes, cables) are listed in `Deck.unsupported` and left out.
 """
 from __future__ import annotations
 
 import gzip
+import math
 import os
 import re
 from dataclasses import dataclass, field
@@ -39,8 +40,8 @@ import numpy as np
 
 from .constants import CENTRE, FACE_XMAX, FACE_XMIN, FACE_YMAX, FACE_YMIN, FACE_ZMAX, FACE_ZMIN
 from .filters import Sfilter
-from .problem import (PML, Box, ExcitationFunction, FrequencyOutput, HuygensBox, OutputPoint, PointSource,
-                      Problem, VolumeMaterial, point_to_cell)
+from .problem import (PML, Box, ExcitationFunction, FarFieldSurface, FrequencyOutput, HuygensBox, OutputPoint,
+                      PointSource, Problem, RCSSurface, VolumeMaterial, point_to_cell)
 
 _LOCATIONS = {"xmin": FACE_XMIN, "xmax": FACE_XMAX, "ymin": FACE_YMIN, "ymax": FACE_YMAX, "zmin": FACE_ZMIN,
               "zmax": FACE_ZMAX, "centre": CENTRE}
@@ -126,6 +127,7 @@ class Shape:
     params: List[float]
     rotation_deg: Tuple[float, float, float]
     translation: Tuple[float, float, float]
+    roughness: Optional[Tuple[float, ...]] = None   # surface_roughness parameters of a surface
 
 
 @dataclass
@@ -151,6 +153,8 @@ class Deck:
     reduced_c_factor: float = 1.0
     small_to_zero: Optional[Tuple[int, float]] = None
     frequency_outputs: List[dict] = field(default_factory=list)
+    far_field_surfaces: List[dict] = field(default_factory=list)
+    rcs_surface: Optional[dict] = None
     unsupported: List[str] = field(default_factory=list)
 
 
@@ -172,9 +176,17 @@ def _shapes(r: Records, kinds: Dict[str, int], what: str) -> Dict[int, Shape]:
             params = r.reals(kinds[kind])
         else:
             kind, params = "point", r.reals(3)
+        rough = None
+        if what == "surface":
+            # an optional surface_roughness record and its parameters (one
+            # for a sphere, two otherwise), read_surface_list.F90:242-262
+            if r.record()[:17].strip().lower() == "surface_roughness":
+                rough = tuple(r.reals(1 if kind == "sphere" else 2))
+            else:
+                r.backspace()
         rot = tuple(r.reals(3))
         trans = tuple(r.reals(3))
-        out[num] = Shape(kind, params, rot, trans)
+        out[num] = Shape(kind, params, rot, trans, rough)
     return out
 
 
@@ -208,7 +220,12 @@ def read_inp(path: str) -> Deck:
     plus the geometry packets a deck without a `.mesh` needs)."""
     with open(path, "rb") as f:
         text = f.read().decode("latin-1")
-    r = Records(text, os.path.basename(path))
+    return read_inp_text(text, os.path.basename(path))
+
+
+def read_inp_text(text: str, name: str = "input") -> Deck:
+    """read_inp of a `.inp` file's text."""
+    r = Records(text, name)
     d = Deck()
     while not r.eof():
         line = r.record().strip().lower()
@@ -347,6 +364,39 @@ def read_inp(path: str) -> Deck:
                     raise IngestError(f"frequency output volume {k}: component {comp!r}")
                 d.frequency_outputs.append({"kind": "volume", "number": vol, "frequency": freq, "component": comp,
                                             **_time_information(r)})
+        elif line == "far_field_surface_list":
+            # read_far_field_surface.F90:77-143: surface, side, frequency,
+            # theta and phi ranges (degrees), then an optional '2D' record
+            # and its axis (z only)
+            n = r.ints(1)[0]
+            for k in range(1, n + 1):
+                if r.ints(1)[0] != k:
+                    raise IngestError("far field surfaces must be numbered 1..n in order")
+                surf, side = r.ints(1)[0], r.ints(1)[0]
+                if side not in (1, -1):
+                    raise IngestError("far field surface side must be +1 or -1")
+                freq = r.reals(1)[0]
+                theta, phi = tuple(r.reals(3)), tuple(r.reals(3))
+                dim = 3
+                if not r.eof():
+                    if r.record()[:2] in ("2D", "2d"):
+                        if r.record()[:1] not in ("z", "Z"):
+                            raise IngestError("the 2D far-field transform takes the z axis only")
+                        dim = 2
+                    else:
+                        r.backspace()
+                d.far_field_surfaces.append({"number": surf, "side": side, "frequency": freq, "theta": theta,
+                                             "phi": phi, "dim": dim})
+        elif line == "rcs_surface":
+            # read_rcs_surface.F90:56-79: surface, side, fmin fmax fstep,
+            # theta, phi (degrees)
+            surf, side = r.ints(1)[0], r.ints(1)[0]
+            if side not in (1, -1):
+                raise IngestError("RCS surface side must be +1 or -1")
+            fmin, fmax, fstep = r.reals(3)
+            theta, phi = r.reals(1)[0], r.reals(1)[0]
+            d.rcs_surface = {"number": surf, "side": side, "fmin": fmin, "fmax": fmax, "fstep": fstep,
+                             "theta": theta, "phi": phi}
         elif line == "simulation_time":
             d.simulation_time = r.reals(1)[0]
         elif line == "reduced_c_factor":
@@ -513,19 +563,23 @@ def _extent(s: Shape) -> Tuple[np.ndarray, np.ndarray]:
     translation (rotations are not meshed here)."""
     if any(a != 0.0 for a in s.rotation_deg):
         raise IngestError(f"rotated {s.kind} is not meshed here: give the model builder's .mesh file")
+    if s.roughness is not None:
+        raise IngestError(f"a rough {s.kind} (random surface points) is not meshed here: "
+                          "give the model builder's .mesh file")
     t = np.array(s.translation)
     p = s.params
     if s.kind == "rectangular_block":
         h = np.array(p[:3]) / 2.0
         return t - h, t + h
-    if s.kind == "rectangular_block2":
-        return t + np.array(p[0::2][:3]), t + np.array(p[1::2][:3])
+    if s.kind == "rectangular_block2":      # corners (x1, y1, z1), (x2, y2, z2)
+        a, b = np.array(p[:3]), np.array(p[3:6])
+        return t + np.minimum(a, b), t + np.maximum(a, b)
     if s.kind == "rectangle":           # in the z = 0 plane of its frame
         h = np.array([p[0] / 2.0, p[1] / 2.0, 0.0])
         return t - h, t + h
-    if s.kind in ("xplane", "yplane", "zplane"):
-        lo, hi = np.array(p[0::2][:3]), np.array(p[1::2][:3])
-        return t + lo, t + hi
+    if s.kind in ("xplane", "yplane", "zplane"):  # corners (x1, y1, z1), (x2, y2, z2)
+        a, b = np.array(p[:3]), np.array(p[3:6])
+        return t + np.minimum(a, b), t + np.maximum(a, b)
     raise IngestError(f"a {s.kind} is not meshed here: give the model builder's .mesh file")
 
 
@@ -541,7 +595,9 @@ def _cells_inside(lo, hi, mesh_min, dl, n):
 
 
 def _grid_index(v, m0, dl):
-    return int(round((v - m0) / dl))
+    """The grid line nearest v, ties away from zero (Fortran NINT)."""
+    x = (v - m0) / dl
+    return int(math.floor(x + 0.5)) if x >= 0 else -int(math.floor(-x + 0.5))
 
 
 @dataclass
@@ -557,24 +613,79 @@ class CellList:
         return m
 
 
-def _plane_faces(lo, hi, mesh_min, dl, n):
-    """Cell faces on an axis-aligned rectangle (one extent of zero width):
-    (i, j, k, *max face) of the cell on the plane's low side."""
+def _plane_faces(lo, hi, mesh_min, dl, n, normal_sign):
+    """Cell faces on an axis-aligned rectangle (one extent of zero width) on
+    its nearest grid line, over the cells whose centre lines meet it.  The
+    model builder's mesher (mesh_triangle.F90:155-168) keeps each face on the
+    side its triangles' normal points away from: a normal along +axis gives
+    the *max face of the cell on the low side, along -axis the *min face of
+    the cell on the high side."""
     axis = [a for a in range(3) if hi[a] == lo[a]]
     if len(axis) != 1:
         raise IngestError("a rectangle or plane must be flat along exactly one axis")
     a = axis[0]
     g = _grid_index(lo[a], mesh_min[a], dl)
     rng = _cells_inside(lo, hi, mesh_min, dl, n)
-    face = (FACE_XMAX, FACE_YMAX, FACE_ZMAX)[a]
+    cell, face = (g, (FACE_XMAX, FACE_YMAX, FACE_ZMAX)[a]) if normal_sign > 0 else \
+        (g + 1, (FACE_XMIN, FACE_YMIN, FACE_ZMIN)[a])
     faces = []
     others = [b for b in range(3) if b != a]
     for u in range(rng[others[0]][0], rng[others[0]][1] + 1):
         for v in range(rng[others[1]][0], rng[others[1]][1] + 1):
             c = [0, 0, 0]
-            c[a], c[others[0]], c[others[1]] = g, u, v
+            c[a], c[others[0]], c[others[1]] = cell, u, v
             faces.append((c[0], c[1], c[2], face))
     return faces
+
+
+def _plane_normal_sign(s: Shape) -> int:
+    """Sign of the triangle normal along the plane's axis:
+    triangle_normal (functions.F90:241-278) is (v3 - v1) x (v2 - v1) of the
+    triangles the geometry builders make (build_surface_xplane.F90,
+    _yplane, _zplane: (p1, p3, p2) over the corners p1 = (x1, y1, z1)
+    .. p4; build_surface_rectangle.F90: (p1, p3, p2) over a grid in x, y)."""
+    p = s.params
+    if s.kind == "rectangle":
+        return -1
+    x1, y1, z1, x2, y2, z2 = p[:6]
+    if s.kind == "xplane":
+        v = (y2 - y1) * (z2 - z1)
+    elif s.kind == "yplane":
+        v = -((x2 - x1) * (z2 - z1))
+    else:
+        v = (x2 - x1) * (y2 - y1)
+    return 1 if v > 0 else -1
+
+
+def mesh_surface(s: Shape, mesh_min, dl: float, n) -> List[Tuple[int, int, int, int]]:
+    """Faces (i, j, k, face) of an unrotated rectangular block, rectangle
+    or x/y/z plane surface, as the model builder's mesher lists them."""
+    lo, hi = _extent(s)
+    if s.kind.startswith("rectangular_block"):
+        # walls on their nearest grid lines g0 (low) and g1 (high), over
+        # the cells whose centre lines meet them; the builder keeps each
+        # wall face on the outside cell, facing the block
+        # (build_surface_rectangular_block.F90:106-: its triangles'
+        # normals, by triangle_normal, point into the block)
+        (i0, i1), (j0, j1), (k0, k1) = _cells_inside(lo, hi, mesh_min, dl, n)
+        gx = [_grid_index(lo[0], mesh_min[0], dl), _grid_index(hi[0], mesh_min[0], dl)]
+        gy = [_grid_index(lo[1], mesh_min[1], dl), _grid_index(hi[1], mesh_min[1], dl)]
+        gz = [_grid_index(lo[2], mesh_min[2], dl), _grid_index(hi[2], mesh_min[2], dl)]
+        f = []
+        for k in range(k0, k1 + 1):
+            for j in range(j0, j1 + 1):
+                f += [(gx[0], j, k, FACE_XMAX), (gx[1] + 1, j, k, FACE_XMIN)]
+        for k in range(k0, k1 + 1):
+            for i in range(i0, i1 + 1):
+                f += [(i, gy[0], k, FACE_YMAX), (i, gy[1] + 1, k, FACE_YMIN)]
+        for j in range(j0, j1 + 1):
+            for i in range(i0, i1 + 1):
+                f += [(i, j, gz[0], FACE_ZMAX), (i, j, gz[1] + 1, FACE_ZMIN)]
+    else:
+        f = _plane_faces(lo, hi, mesh_min, dl, n, _plane_normal_sign(s))
+    # faces whose cell lies outside the mesh are dropped
+    # (cell_point_inside_mesh, mesh_triangle.F90:170)
+    return [c for c in f if all(1 <= c[a] <= n[a] for a in range(3))]
 
 
 # ---- deck -> problem --------------------------------------------------------------
@@ -643,22 +754,7 @@ def problem_from(deck: Deck, mesh: Optional[MeshFile] = None, material_dir: Opti
     def surface_faces(num):
         if mesh is not None:
             return [tuple(int(v) for v in f) for f in mesh.surfaces[num - 1]["faces"]]
-        s = deck.surfaces[num]
-        lo, hi = _extent(s)
-        if s.kind.startswith("rectangular_block"):
-            (i0, i1), (j0, j1), (k0, k1) = _cells_inside(lo, hi, mesh_min, dl, n)
-            f = []
-            for k in range(k0, k1 + 1):
-                for j in range(j0, j1 + 1):
-                    f += [(i0, j, k, FACE_XMIN), (i1, j, k, FACE_XMAX)]
-            for k in range(k0, k1 + 1):
-                for i in range(i0, i1 + 1):
-                    f += [(i, j0, k, FACE_YMIN), (i, j1, k, FACE_YMAX)]
-            for j in range(j0, j1 + 1):
-                for i in range(i0, i1 + 1):
-                    f += [(i, j, k0, FACE_ZMIN), (i, j, k1, FACE_ZMAX)]
-            return f
-        return _plane_faces(lo, hi, mesh_min, dl, n)
+        return mesh_surface(deck.surfaces[num], mesh_min, dl, n)
 
     def surface_box(num):
         """The surface's extent as a cell box (grid lines i0-1 .. i1)."""
@@ -727,11 +823,18 @@ def problem_from(deck: Deck, mesh: Optional[MeshFile] = None, material_dir: Opti
         else:
             freq_outputs.append(FrequencyOutput("surface", fo["frequency"], fo["component"],
                                                 faces=surface_faces(fo["number"]), side=fo["side"], **window))
+    far_fields = [FarFieldSurface(surface_faces(f["number"]), f["frequency"], theta=f["theta"], phi=f["phi"],
+                                  side=f["side"], dim=f["dim"]) for f in deck.far_field_surfaces]
+    rcs = None
+    if deck.rcs_surface is not None:
+        q = deck.rcs_surface
+        rcs = RCSSurface(surface_faces(q["number"]), q["fmin"], q["fmax"], q["fstep"], theta=q["theta"],
+                         phi=q["phi"], side=q["side"])
     p = Problem(nx=nx, ny=ny, nz=nz, dl=dl, simulation_time=deck.simulation_time, mesh_min=tuple(mesh_min),
                 reflection=tuple(deck.reflection), materials=materials, material_regions=regions,
                 surface_materials=smats, surface_faces=sfaces, functions=list(deck.functions), sources=sources,
                 huygens=huygens, outputs=outputs, pml=pml, reduced_c_factor=deck.reduced_c_factor,
-                frequency_outputs=freq_outputs)
+                frequency_outputs=freq_outputs, far_field_surfaces=far_fields, rcs_surface=rcs)
     info = [{k: v for k, v in o.items() if k not in ("point", "component", "location")} for o in deck.output_points]
     return Ingested(p, list(deck.unsupported), info, sorted(boundary))
 

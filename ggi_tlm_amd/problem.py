@@ -143,8 +143,9 @@ class HuygensBox:
     ephi: float
     side: int = 1
     wave_impedance: float = Z0
-    # explicit surface-mesh faces (cx, cy, cz, face) before the side flip;
-    # when given, the box only sets the surface extent used for offset_min
+    # explicit surface-mesh faces (cx, cy, cz, face) as the model builder's
+    # mesh lists them (side=-1 turns each to the other side); when given,
+    # the box only sets the surface extent used for offset_min
     faces: Optional[list] = None
 
 
@@ -166,6 +167,38 @@ class FrequencyOutput:
     first_timestep: int = 0
     last_timestep: int = 1000000000
     timestep_interval: int = 1
+
+
+@dataclass
+class FarFieldSurface:
+    """A far-field surface (read_far_field_surface.F90, far_field_surfaces.F90):
+    faces (cx, cy, cz, face) of the surface mesh and the side (+1 the
+    outward normal, -1 the other side of each face), the frequency, and the
+    angle ranges (min, max, step) in degrees as the input file gives them.
+    dim 2: the 2D transform in the theta = 90 degree plane (theta is not
+    read)."""
+
+    faces: list
+    frequency: float
+    theta: Tuple[float, float, float] = (0.0, 180.0, 1.0)
+    phi: Tuple[float, float, float] = (0.0, 0.0, 1.0)
+    side: int = 1
+    dim: int = 3
+
+
+@dataclass
+class RCSSurface:
+    """The RCS surface (read_rcs_surface.F90, RCS_surfaces.F90): faces and
+    side as for far-field surfaces, the frequency range (fmin, fmax, fstep)
+    of the RCS spectrum, and the monostatic direction in degrees."""
+
+    faces: list
+    fmin: float
+    fmax: float
+    fstep: float
+    theta: float = 0.0
+    phi: float = 0.0
+    side: int = 1
 
 
 @dataclass
@@ -198,6 +231,8 @@ class Problem:
     # = position in this list + 1), (cx, cy, cz, face) as surface_faces
     cable_faces: List[Tuple[int, int, int, int]] = field(default_factory=list)
     frequency_outputs: List[FrequencyOutput] = field(default_factory=list)
+    far_field_surfaces: List[FarFieldSurface] = field(default_factory=list)
+    rcs_surface: Optional[RCSSurface] = None
 
     @property
     def dt(self) -> float:
@@ -381,6 +416,8 @@ class Tables:
     # handy extras (not part of the ABI)
     output_cells: np.ndarray = None   # (n_output_cells, 3) 1-based cx,cy,cz
     freq_outputs: list = field(default_factory=list)  # frequency output volumes and surfaces
+    far_fields: list = field(default_factory=list)    # far-field surfaces
+    rcs: Optional[dict] = None                        # the RCS surface
     n_pml_cells: int = 0
 
 
@@ -422,27 +459,29 @@ def huygens_face_list(hb: HuygensBox, nx, ny, nz):
     faces = []
     if hb.faces is not None:
         faces = [tuple(int(v) for v in f) for f in hb.faces]
+    # a box's walls as the builder meshes a rectangular_block surface: the
+    # face of the cell just outside each wall, facing the box
     for k in (range(b.k0, b.k1 + 1) if hb.faces is None else ()):
         for j in range(b.j0, b.j1 + 1):
-            faces.append((b.i0, j, k, FACE_XMIN))
-            faces.append((b.i1, j, k, FACE_XMAX))
+            faces.append((b.i0 - 1, j, k, FACE_XMAX))
+            faces.append((b.i1 + 1, j, k, FACE_XMIN))
     for k in (range(b.k0, b.k1 + 1) if hb.faces is None else ()):
         for i in range(b.i0, b.i1 + 1):
-            faces.append((i, b.j0, k, FACE_YMIN))
-            faces.append((i, b.j1, k, FACE_YMAX))
+            faces.append((i, b.j0 - 1, k, FACE_YMAX))
+            faces.append((i, b.j1 + 1, k, FACE_YMIN))
     for j in (range(b.j0, b.j1 + 1) if hb.faces is None else ()):
         for i in range(b.i0, b.i1 + 1):
-            faces.append((i, j, b.k0, FACE_ZMIN))
-            faces.append((i, j, b.k1, FACE_ZMAX))
+            faces.append((i, j, b.k0 - 1, FACE_ZMAX))
+            faces.append((i, j, b.k1 + 1, FACE_ZMIN))
     out = []
-    # the box's own faces, flipped for side=+1 (see HuygensBox)
+    # side -1 turns each face to the other side (huygens_surface.F90:423-453)
     flip = {FACE_XMIN: (-1, 0, 0, FACE_XMAX), FACE_XMAX: (1, 0, 0, FACE_XMIN),
             FACE_YMIN: (0, -1, 0, FACE_YMAX), FACE_YMAX: (0, 1, 0, FACE_YMIN),
             FACE_ZMIN: (0, 0, -1, FACE_ZMAX), FACE_ZMAX: (0, 0, 1, FACE_ZMIN)}
     normal = {FACE_XMIN: (-1, 0, 0), FACE_XMAX: (1, 0, 0), FACE_YMIN: (0, -1, 0),
               FACE_YMAX: (0, 1, 0), FACE_ZMIN: (0, 0, -1), FACE_ZMAX: (0, 0, 1)}
     for (i, j, k, f) in faces:
-        if hb.side == 1:
+        if hb.side == -1:
             di, dj, dk, f2 = flip[f]
             i, j, k, f = i + di, j + dj, k + dk, f2
         out.append((i, j, k, f, normal[f]))
@@ -663,26 +702,32 @@ def build_tables(p: Problem) -> Tables:
         t = _min_face(nx, ny, nz, (int(cx), int(cy), int(cz)), int(face))
         if t is not None:
             f_cab[t[2] - 1, t[1] - 1, t[0] - 1, t[3]] = k + 1
-    # frequency output surfaces: each face, turned to the other side for
-    # side -1 (frequency_output_surfaces.F90:30-280), is an output face
-    fo_faces = []
-    for fq in p.frequency_outputs:
+    # output surfaces (frequency output, far-field and RCS surfaces): each
+    # face, turned to the other side for side -1
+    # (frequency_output_surfaces.F90:30-280, far_field_surfaces.F90:36-145,
+    # RCS_surfaces.F90:35-143), is an output face
+    other = {FACE_XMIN: (-1, 0, 0, FACE_XMAX), FACE_XMAX: (1, 0, 0, FACE_XMIN),
+             FACE_YMIN: (0, -1, 0, FACE_YMAX), FACE_YMAX: (0, 1, 0, FACE_YMIN),
+             FACE_ZMIN: (0, 0, -1, FACE_ZMAX), FACE_ZMAX: (0, 0, 1, FACE_ZMIN)}
+
+    def output_surface(surface_faces, side, what):
         faces = []
-        if fq.kind == "surface":
-            other = {FACE_XMIN: (-1, 0, 0, FACE_XMAX), FACE_XMAX: (1, 0, 0, FACE_XMIN),
-                     FACE_YMIN: (0, -1, 0, FACE_YMAX), FACE_YMAX: (0, 1, 0, FACE_YMIN),
-                     FACE_ZMIN: (0, 0, -1, FACE_ZMAX), FACE_ZMAX: (0, 0, 1, FACE_ZMIN)}
-            for (cx, cy, cz, face) in fq.faces:
-                cx, cy, cz, face = int(cx), int(cy), int(cz), int(face)
-                if fq.side == -1:
-                    di, dj, dk, face = other[face]
-                    cx, cy, cz = cx + di, cy + dj, cz + dk
-                faces.append((cx, cy, cz, face))
-                t = _min_face(nx, ny, nz, (cx, cy, cz), face)
-                if t is None:
-                    raise ValueError("a frequency output surface face lies on the mesh boundary")
-                f_out[t[2] - 1, t[1] - 1, t[0] - 1, t[3]] = True
-        fo_faces.append(faces)
+        for (cx, cy, cz, face) in surface_faces:
+            cx, cy, cz, face = int(cx), int(cy), int(cz), int(face)
+            if side == -1:
+                di, dj, dk, face = other[face]
+                cx, cy, cz = cx + di, cy + dj, cz + dk
+            faces.append((cx, cy, cz, face))
+            t = _min_face(nx, ny, nz, (cx, cy, cz), face)
+            if t is None:
+                raise ValueError(f"a {what} face lies on the mesh boundary")
+            f_out[t[2] - 1, t[1] - 1, t[0] - 1, t[3]] = True
+        return faces
+
+    fo_faces = [output_surface(fq.faces, fq.side, "frequency output surface") if fq.kind == "surface" else []
+                for fq in p.frequency_outputs]
+    ff_faces = [output_surface(s.faces, s.side, "far-field surface") for s in p.far_field_surfaces]
+    rcs_faces = output_surface(p.rcs_surface.faces, p.rcs_surface.side, "RCS surface") if p.rcs_surface else []
     face_out_pts = []
     for o in p.outputs:
         if FACES[o.point] != CENTRE:
@@ -831,6 +876,61 @@ def build_tables(p: Problem) -> Tables:
             e["normals"] = np.asarray(normals, dtype=np.int32)
         freq_outputs.append(e)
 
+    def surface_items(faces):
+        """Face output rows, sides, signed normals and centre coordinates
+        (get_cell_point_coordinate, general_mesh_subroutines.F90:84-137) of
+        output-surface faces."""
+        rows, sides, normals, centres = [], [], [], []
+        for (cx, cy, cz, face) in faces:
+            t = _min_face(nx, ny, nz, (cx, cy, cz), face)
+            rows.append(int(fout_rank_full[face_flat(t)]))
+            sides.append(normal_of[face][0])
+            normals.append(normal_of[face][1])
+            off = [-0.5, -0.5, -0.5]
+            if face in (FACE_XMIN, FACE_YMIN, FACE_ZMIN):
+                off[face - FACE_XMIN] = -1.0
+            elif face in (FACE_XMAX, FACE_YMAX, FACE_ZMAX):
+                off[face - FACE_XMAX] = 0.0
+            centres.append([p.mesh_min[a] + (cx, cy, cz)[a] * dl + dl * off[a] for a in range(3)])
+        return (np.asarray(rows, dtype=np.int32), np.asarray(sides, dtype=np.int32),
+                np.asarray(normals, dtype=np.int32), np.asarray(centres, dtype=np.float64).reshape(-1, 3))
+
+    # far-field surfaces: angles in radians and their counts as
+    # read_far_field_surface.F90 and write_far_field_surfaces
+    # (far_field_surfaces.F90:503-507) form them
+    rad = PI / 180.0
+    far_fields = []
+    for s, faces in zip(p.far_field_surfaces, ff_faces):
+        if s.dim == 2:
+            th = (PI / 2.0, PI / 2.0, 1.0)
+        else:
+            th = (rad * s.theta[0], rad * s.theta[1], rad * s.theta[2])
+        ph = (rad * s.phi[0], rad * s.phi[1], rad * s.phi[2])
+        rows, sides, normals, centres = surface_items(faces)
+        far_fields.append({"frequency": float(s.frequency), "dim": int(s.dim),
+                           "theta_min": th[0], "theta_step": th[2], "n_theta": int((th[1] - th[0]) / th[2]) + 1,
+                           "phi_min": ph[0], "phi_step": ph[2], "n_phi": int((ph[1] - ph[0]) / ph[2]) + 1,
+                           "rows": rows, "sides": sides, "normals": normals, "centres": centres})
+    # the RCS surface: its far-field points as initialise_rcs_surfaces
+    # (RCS_surfaces.F90:159-367) sizes them, n_rcs_points_per_cell = 2
+    # (TLM_output_modules.F90:388)
+    rcs = None
+    if p.rcs_surface is not None:
+        s = p.rcs_surface
+        theta, phi = rad * s.theta, rad * s.phi
+        rows, sides, normals, centres = surface_items(rcs_faces)
+        K = (1.0 * math.sin(theta) * math.cos(phi), 1.0 * math.sin(theta) * math.sin(phi), 1.0 * math.cos(theta))
+        d = [K[0] * c[0] + K[1] * c[1] + K[2] * c[2] for c in centres]
+        dmin, dmax = (min(d), max(d)) if d else (1e30, -1e30)
+        sim_time = p.simulation_time if p.simulation_time is not None else nts * dt
+        period = (dmax - dmin) / C0 + sim_time
+        distance = period * C0
+        ppc = 2
+        rcs = {"theta": theta, "phi": phi, "points_per_cell": ppc,
+               "n_points": int(ppc * distance / dl + 2), "point_offset": int(ppc * dmin / dl - 1),
+               "fmin": float(s.fmin), "fmax": float(s.fmax), "fstep": float(s.fstep),
+               "rows": rows, "sides": sides, "normals": normals, "centres": centres}
+
     return Tables(
         nx=nx, ny=ny, nz=nz, dl=dl, dt=dt, n_timesteps=nts,
         reduced_c_factor=p.reduced_c_factor, reflection=tuple(p.reflection),
@@ -852,4 +952,5 @@ def build_tables(p: Problem) -> Tables:
         face_output_points=face_out_pts,
         dft_frequencies=np.asarray(p.dft_frequencies, dtype=np.float64),
         output_cells=out_cells, n_pml_cells=n_pml_cells, freq_outputs=freq_outputs,
+        far_fields=far_fields, rcs=rcs,
     )

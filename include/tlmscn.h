@@ -238,6 +238,52 @@ typedef struct {
 int tlm_set_frequency_outputs(tlm_handle* h, int32_t n, const tlm_frequency_output* outputs,
                               const int32_t* rows, const int32_t* sides, const int32_t* normals);
 
+/* Far-field surfaces (SRC/OUTPUT/far_field_surfaces.F90).  The connect of
+ * every step adds to each face its surface currents J = n x H and
+ * M = -n x E times exp(-j 2 pi f time) dt, time = (t-1) dt + dt/2
+ * (face_output_far_field, :252-351), on the device.  tlm_read_far_field
+ * transforms them, on the device, to the far field at r = 1 m
+ * (write_far_field_surfaces, :370-712): E_theta and E_phi for the angles
+ * theta_min + i theta_step (i < n_theta) by phi_min + j phi_step (j < n_phi),
+ * theta-major.  dim 3: the 3D transform (face area dl^2); dim 2: the 2D
+ * transform in the plane theta = pi/2 (n_theta 1, face length dl).  Faces
+ * are given as for frequency output surfaces (face output rows, sides,
+ * signed outward normals; n_faces per surface, concatenated), with their
+ * centres: 3 coordinates per face in metres (get_cell_point_coordinate). */
+typedef struct {
+  double frequency;
+  double theta_min, theta_step, phi_min, phi_step; /* radians */
+  int32_t n_theta, n_phi;
+  int32_t dim;
+  int32_t n_faces;
+} tlm_far_field_surface;
+int tlm_set_far_field_surfaces(tlm_handle* h, int32_t n, const tlm_far_field_surface* surfaces,
+                               const int32_t* rows, const int32_t* sides, const int32_t* normals,
+                               const double* centres);
+/* e[4 a + {0, 1, 2, 3}] = Re E_theta, Im E_theta, Re E_phi, Im E_phi at angle
+ * a of far-field surface k (0-based). */
+int tlm_read_far_field(tlm_handle* h, int32_t k, double* e);
+
+/* RCS surface (SRC/OUTPUT/RCS_surfaces.F90:386-559): the connect of every
+ * step adds, per face, the time-domain far field of its currents in the
+ * monostatic direction (theta, phi) to far-field point
+ *   p = INT(points_per_cell (c0 time - K.r) / dl) - point_offset,
+ * K the unit vector of (theta, phi), r the face centre, time = (t-1) dt +
+ * dt/2, when 1 <= p <= n_points: E_theta, E_phi, H_theta, H_phi.  The caller
+ * sizes n_points and point_offset as initialise_rcs_surfaces does.  Faces
+ * as for far-field surfaces.  NULL clears the surface. */
+typedef struct {
+  double theta, phi; /* radians */
+  int32_t n_faces;
+  int32_t n_points;
+  int32_t point_offset;
+  int32_t points_per_cell;
+} tlm_rcs_surface;
+int tlm_set_rcs_surface(tlm_handle* h, const tlm_rcs_surface* surface, const int32_t* rows,
+                        const int32_t* sides, const int32_t* normals, const double* centres);
+/* out[4 (p-1) + {0, 1, 2, 3}] = E_theta, E_phi, H_theta, H_phi of point p. */
+int tlm_read_rcs(tlm_handle* h, double* out);
+
 /* ---- link state ---------------------------------------------------------- */
 
 /* mesh::V(1:12, 0:nx+1, 0:ny+1, 1:nz), FP64, Fortran layout.  Only the

@@ -119,6 +119,18 @@ struct oracle_handle {
   tlm_frequency_output* fo;
   int32_t *fo_out, *fo_row, *fo_side, *fo_norm;
   double* fo_acc;
+  /* far-field surfaces: per surface its faces (row, side, normal, centre)
+     and the accumulated J(face, 1:3), M(face, 1:3) as complex pairs */
+  int n_ff;
+  tlm_far_field_surface* ff;
+  int32_t *ff_row, *ff_side, *ff_norm;
+  double *ff_centre, *ff_J, *ff_M;
+  /* RCS surface: faces in the caller's order and the far-field series
+     E_theta, E_phi, H_theta, H_phi per point */
+  int rcs_on;
+  tlm_rcs_surface rcs;
+  int32_t *rcs_row, *rcs_side, *rcs_norm;
+  double *rcs_centre, *rcs_E;
   int cap, ring_count, ring_first;
   double* ring; /* cap blocks of n_out_rows*6 */
   /* z-slab threading (the MPI build's z partition, general_mesh_subroutines.F90:868-972,
@@ -196,6 +208,8 @@ void oracle_destroy(oracle_handle* h) {
   free(h->face_exc_type); free(h->huy); free(h->cell_out); free(h->face_out); free(h->fring);
   free(h->pt_row); free(h->pt_comp); free(h->freq); free(h->dft); free(h->ring);
   free(h->fo); free(h->fo_out); free(h->fo_row); free(h->fo_side); free(h->fo_norm); free(h->fo_acc);
+  free(h->ff); free(h->ff_row); free(h->ff_side); free(h->ff_norm); free(h->ff_centre); free(h->ff_J); free(h->ff_M);
+  free(h->rcs_row); free(h->rcs_side); free(h->rcs_norm); free(h->rcs_centre); free(h->rcs_E);
   free(h->pl_out); free(h->pl_exc); free(h->pl_fexc); free(h->pl_fout);
   free(h->hf_of_code); free(h->hf_tot);
   free(h);
@@ -966,6 +980,75 @@ static void accumulate_frequency_outputs(oracle_handle* h, int kind, const doubl
   }
 }
 
+/* face_output_far_field (far_field_surfaces.F90:252-351): every connect
+   adds ejwt (n x H) to J and -ejwt (n x E) to M of each face, ejwt =
+   exp(-j 2 pi f time) dt, time = (t-1) dt + dt/2, with the field of the
+   face's side (1 for a min face, 2 for a max face) and its outward normal. */
+static void far_field_currents(oracle_handle* h, const double* src, double time) {
+  size_t k = 0;
+  for (int s = 0; s < h->n_ff; s++) {
+    const tlm_far_field_surface* f = &h->ff[s];
+    double y = -(2.0 * PI * f->frequency * time);
+    double er = cos(y) * h->g.dt, ei = sin(y) * h->g.dt;
+    for (int i = 0; i < f->n_faces; i++, k++) {
+      const double* fld = &src[((size_t)(h->ff_row[k] - 1) * 2 + (h->ff_side[k] - 1)) * 6];
+      int nrm = h->ff_norm[k];
+      double normx = nrm == 1 ? 1.0 : nrm == -1 ? -1.0 : 0.0;
+      double normy = nrm == 2 ? 1.0 : nrm == -2 ? -1.0 : 0.0;
+      double normz = nrm == 3 ? 1.0 : nrm == -3 ? -1.0 : 0.0;
+      double jv[3] = {normy * fld[5] - normz * fld[4], normz * fld[3] - normx * fld[5], normx * fld[4] - normy * fld[3]};
+      double mv[3] = {normy * fld[2] - normz * fld[1], normz * fld[0] - normx * fld[2], normx * fld[1] - normy * fld[0]};
+      for (int c = 0; c < 3; c++) {
+        double* J = &h->ff_J[(k * 3 + c) * 2];
+        double* M = &h->ff_M[(k * 3 + c) * 2];
+        J[0] = J[0] + er * jv[c];
+        J[1] = J[1] + ei * jv[c];
+        M[0] = M[0] - er * mv[c];
+        M[1] = M[1] - ei * mv[c];
+      }
+    }
+  }
+}
+
+/* face_output_rcs (RCS_surfaces.F90:386-559): each face's far field in the
+   monostatic direction, added to the far-field point of its retarded
+   distance c0 time - K.r, faces in the caller's order. */
+static void rcs_currents(oracle_handle* h, const double* src, double time) {
+  const double dl = h->g.dl, dA = dl * dl;
+  double theta = h->rcs.theta, phi = h->rcs.phi;
+  double ct = cos(theta), st = sin(theta), cp = cos(phi), sp = sin(phi);
+  double K[3] = {1.0 * sin(theta) * cos(phi), 1.0 * sin(theta) * sin(phi), 1.0 * cos(theta)};
+  for (int i = 0; i < h->rcs.n_faces; i++) {
+    const double* fld = &src[((size_t)(h->rcs_row[i] - 1) * 2 + (h->rcs_side[i] - 1)) * 6];
+    int nrm = h->rcs_norm[i];
+    double normx = nrm == 1 ? 1.0 : nrm == -1 ? -1.0 : 0.0;
+    double normy = nrm == 2 ? 1.0 : nrm == -2 ? -1.0 : 0.0;
+    double normz = nrm == 3 ? 1.0 : nrm == -3 ? -1.0 : 0.0;
+    double Jsx = (normy * fld[5] - normz * fld[4]);
+    double Jsy = (normz * fld[3] - normx * fld[5]);
+    double Jsz = (normx * fld[4] - normy * fld[3]);
+    double Msx = -(normy * fld[2] - normz * fld[1]);
+    double Msy = -(normz * fld[0] - normx * fld[2]);
+    double Msz = -(normx * fld[1] - normy * fld[0]);
+    const double* r = &h->rcs_centre[3 * (size_t)i];
+    double dmrcs = K[0] * r[0] + K[1] * r[1] + K[2] * r[2];
+    dmrcs = -dmrcs + time * C0;
+    int point = (int)((double)h->rcs.points_per_cell * dmrcs / dl);
+    point = point - h->rcs.point_offset;
+    if (point >= 1 && point <= h->rcs.n_points) {
+      double Ntheta = (Jsx * ct * cp + Jsy * ct * sp - Jsz * st) * dA / (4.0 * PI);
+      double Nphi = (-Jsx * sp + Jsy * cp) * dA / (4.0 * PI);
+      double Ltheta = (Msx * ct * cp + Msy * ct * sp - Msz * st) * dA / (4.0 * PI);
+      double Lphi = (-Msx * sp + Msy * cp) * dA / (4.0 * PI);
+      double* E = &h->rcs_E[4 * (size_t)(point - 1)];
+      E[0] = E[0] + -(Lphi + Z0 * Ntheta);
+      E[1] = E[1] + (Ltheta - Z0 * Nphi);
+      E[2] = E[2] + (Nphi - Ltheta / Z0);
+      E[3] = E[3] + -(Ntheta + Lphi / Z0);
+    }
+  }
+}
+
 static void record_outputs(oracle_handle* h, int timestep) {
   /* cell_output(): copy this step's cell_output_field into the ring and
    * accumulate the probe DFT at time=(timestep-1)*dt (run_TLM.F90:240). */
@@ -1513,8 +1596,10 @@ int oracle_connect(oracle_handle* h, int32_t timestep) {
       memcpy(h->fring + (size_t)h->fring_count * w, h->face_out, w * sizeof(double));
       h->fring_count++;
     }
-    if (h->n_fo_items)
-      accumulate_frequency_outputs(h, 2, h->face_out, timestep, (double)(timestep - 1) * h->g.dt + h->g.dt / 2.0);
+    double time = (double)(timestep - 1) * h->g.dt + h->g.dt / 2.0;
+    if (h->n_fo_items) accumulate_frequency_outputs(h, 2, h->face_out, timestep, time);
+    if (h->n_ff) far_field_currents(h, h->face_out, time);
+    if (h->rcs_on) rcs_currents(h, h->face_out, time);
   }
   return TLM_OK;
 }
@@ -1659,6 +1744,146 @@ int oracle_read_frequency_output(oracle_handle* h, int32_t k, double* re_im) {
   size_t off = 0;
   for (int i = 0; i < k; i++) off += (size_t)h->fo[i].n_items;
   memcpy(re_im, h->fo_acc + off * 2, (size_t)h->fo[k].n_items * 2 * sizeof(double));
+  return TLM_OK;
+}
+
+int oracle_set_far_field_surfaces(oracle_handle* h, int32_t n, const tlm_far_field_surface* surfaces,
+                                  const int32_t* rows, const int32_t* sides, const int32_t* normals,
+                                  const double* centres) {
+  if (n < 0 || (n && !surfaces)) return TLM_ERR_ARG;
+  size_t faces = 0;
+  for (int i = 0; i < n; i++) {
+    const tlm_far_field_surface* f = &surfaces[i];
+    if (f->n_faces < 0 || f->n_theta < 1 || f->n_phi < 1 || (f->dim != 2 && f->dim != 3) ||
+        (f->dim == 2 && f->n_theta != 1))
+      return fail(h, TLM_ERR_ARG, "far-field surface: bad dimension or angles");
+    faces += (size_t)f->n_faces;
+  }
+  if (faces && (!rows || !sides || !normals || !centres)) return TLM_ERR_ARG;
+  for (size_t i = 0; i < faces; i++) {
+    if (rows[i] < 1 || rows[i] > h->n_face_out_rows) return fail(h, TLM_ERR_ARG, "far-field surface: row out of range");
+    if (sides[i] < 1 || sides[i] > 2 || normals[i] == 0 || normals[i] < -3 || normals[i] > 3)
+      return fail(h, TLM_ERR_ARG, "far-field surface: bad face side or normal");
+  }
+  free(h->ff); free(h->ff_row); free(h->ff_side); free(h->ff_norm); free(h->ff_centre); free(h->ff_J); free(h->ff_M);
+  h->n_ff = n;
+  h->ff = (tlm_far_field_surface*)malloc((size_t)(n ? n : 1) * sizeof(tlm_far_field_surface));
+  if (n) memcpy(h->ff, surfaces, (size_t)n * sizeof(tlm_far_field_surface));
+  h->ff_row = dup_i32(rows, faces);
+  h->ff_side = dup_i32(sides, faces);
+  h->ff_norm = dup_i32(normals, faces);
+  h->ff_centre = (double*)malloc((faces ? faces : 1) * 3 * sizeof(double));
+  if (faces) memcpy(h->ff_centre, centres, faces * 3 * sizeof(double));
+  h->ff_J = (double*)calloc((faces ? faces : 1) * 6, sizeof(double));
+  h->ff_M = (double*)calloc((faces ? faces : 1) * 6, sizeof(double));
+  return TLM_OK;
+}
+
+/* write_far_field_surfaces (far_field_surfaces.F90:370-712): the N and L
+   integrals over the faces for each angle, summed in face order, and the
+   far field at r0 = 1 m; complex arithmetic written out as the Fortran
+   complex*16 operations it restates. */
+int oracle_read_far_field(oracle_handle* h, int32_t k, double* e) {
+  if (k < 0 || k >= h->n_ff || !e) return TLM_ERR_ARG;
+  const tlm_far_field_surface* f = &h->ff[k];
+  size_t f0 = 0;
+  for (int i = 0; i < k; i++) f0 += (size_t)h->ff[i].n_faces;
+  double dA = f->dim == 2 ? h->g.dl : h->g.dl * h->g.dl;
+  /* jbeta = j*cmplx(beta) (:431, :561): CMPLX without a kind is a
+     single-precision complex, so beta enters rounded to a float */
+  double beta = (double)(float)(2.0 * PI * f->frequency / C0);
+  double r0 = 1.0;
+  double cb = cos(beta * r0), sb = sin(beta * r0); /* exp(j beta r0) */
+  double p2r, p2i;
+  if (f->dim == 2) {
+    /* sqrt(j beta / (8 pi r0)): the principal root of j y, y > 0, is
+       sqrt(y/2) (1 + j) */
+    double rt = sqrt(0.5 * (beta / (8.0 * PI * r0)));
+    p2r = rt * cb - rt * sb;
+    p2i = rt * sb + rt * cb;
+  } else {
+    p2r = (0.0 * cb - beta * sb) / (4.0 * PI * r0);
+    p2i = (0.0 * sb + beta * cb) / (4.0 * PI * r0);
+  }
+  int count = 0;
+  for (int it = 0; it < f->n_theta; it++) {
+    double theta = f->theta_min + (double)it * f->theta_step;
+    for (int ip = 0; ip < f->n_phi; ip++, count++) {
+      double phi = f->phi_min + (double)ip * f->phi_step;
+      double r[3] = {r0 * sin(theta) * cos(phi), r0 * sin(theta) * sin(phi), r0 * cos(theta)};
+      double ct = cos(theta), st = sin(theta), cp = cos(phi), sp = sin(phi);
+      double a[8] = {0, 0, 0, 0, 0, 0, 0, 0}; /* N_theta|N_z, N_phi, L_theta|L_z, L_phi */
+      for (int i = 0; i < f->n_faces; i++) {
+        size_t q = f0 + (size_t)i;
+        const double* rp = &h->ff_centre[3 * q];
+        double x = f->dim == 2 ? r[0] * rp[0] + r[1] * rp[1] : r[0] * rp[0] + r[1] * rp[1] + r[2] * rp[2];
+        double pr = cos(beta * x), pi = sin(beta * x); /* exp(j beta x) */
+        const double* J = &h->ff_J[q * 6];
+        const double* M = &h->ff_M[q * 6];
+        double t[8];
+        if (f->dim == 2) {
+          t[0] = J[4]; t[1] = J[5];
+          t[2] = -J[0] * sp + J[2] * cp; t[3] = -J[1] * sp + J[3] * cp;
+          t[4] = M[4]; t[5] = M[5];
+          t[6] = -M[0] * sp + M[2] * cp; t[7] = -M[1] * sp + M[3] * cp;
+        } else {
+          t[0] = J[0] * ct * cp + J[2] * ct * sp - J[4] * st; t[1] = J[1] * ct * cp + J[3] * ct * sp - J[5] * st;
+          t[2] = -J[0] * sp + J[2] * cp; t[3] = -J[1] * sp + J[3] * cp;
+          t[4] = M[0] * ct * cp + M[2] * ct * sp - M[4] * st; t[5] = M[1] * ct * cp + M[3] * ct * sp - M[5] * st;
+          t[6] = -M[0] * sp + M[2] * cp; t[7] = -M[1] * sp + M[3] * cp;
+        }
+        for (int c = 0; c < 4; c++) {
+          double zr = t[2 * c] * pr - t[2 * c + 1] * pi, zi = t[2 * c] * pi + t[2 * c + 1] * pr;
+          a[2 * c] = a[2 * c] + zr * dA;
+          a[2 * c + 1] = a[2 * c + 1] + zi * dA;
+        }
+      }
+      double qr, qi, ur, ui;
+      if (f->dim == 2) {
+        qr = a[6] - Z0 * a[0]; qi = a[7] - Z0 * a[1]; /* L_phi - Z0 N_z */
+        ur = a[4] + Z0 * a[2]; ui = a[5] + Z0 * a[3]; /* L_z + Z0 N_phi */
+      } else {
+        qr = a[6] + Z0 * a[0]; qi = a[7] + Z0 * a[1]; /* L_phi + Z0 N_theta */
+        ur = a[4] - Z0 * a[2]; ui = a[5] - Z0 * a[3]; /* L_theta - Z0 N_phi */
+      }
+      e[4 * count] = -(p2r * qr - p2i * qi);
+      e[4 * count + 1] = -(p2r * qi + p2i * qr);
+      e[4 * count + 2] = p2r * ur - p2i * ui;
+      e[4 * count + 3] = p2r * ui + p2i * ur;
+    }
+  }
+  return TLM_OK;
+}
+
+int oracle_set_rcs_surface(oracle_handle* h, const tlm_rcs_surface* s, const int32_t* rows, const int32_t* sides,
+                           const int32_t* normals, const double* centres) {
+  free(h->rcs_row); free(h->rcs_side); free(h->rcs_norm); free(h->rcs_centre); free(h->rcs_E);
+  h->rcs_row = h->rcs_side = h->rcs_norm = NULL;
+  h->rcs_centre = h->rcs_E = NULL;
+  h->rcs_on = 0;
+  if (!s) return TLM_OK;
+  if (s->n_faces < 0 || s->n_points < 1 || s->points_per_cell < 1 ||
+      (s->n_faces && (!rows || !sides || !normals || !centres)))
+    return fail(h, TLM_ERR_ARG, "RCS surface: bad faces or far-field points");
+  for (int i = 0; i < s->n_faces; i++) {
+    if (rows[i] < 1 || rows[i] > h->n_face_out_rows) return fail(h, TLM_ERR_ARG, "RCS surface: row out of range");
+    if (sides[i] < 1 || sides[i] > 2 || normals[i] == 0 || normals[i] < -3 || normals[i] > 3)
+      return fail(h, TLM_ERR_ARG, "RCS surface: bad face side or normal");
+  }
+  h->rcs_on = 1;
+  h->rcs = *s;
+  h->rcs_row = dup_i32(rows, (size_t)s->n_faces);
+  h->rcs_side = dup_i32(sides, (size_t)s->n_faces);
+  h->rcs_norm = dup_i32(normals, (size_t)s->n_faces);
+  h->rcs_centre = (double*)malloc((size_t)(s->n_faces ? s->n_faces : 1) * 3 * sizeof(double));
+  if (s->n_faces) memcpy(h->rcs_centre, centres, (size_t)s->n_faces * 3 * sizeof(double));
+  h->rcs_E = (double*)calloc((size_t)s->n_points * 4, sizeof(double));
+  return TLM_OK;
+}
+
+int oracle_read_rcs(oracle_handle* h, double* out) {
+  if (!h->rcs_on || !out) return TLM_ERR_ARG;
+  memcpy(out, h->rcs_E, (size_t)h->rcs.n_points * 4 * sizeof(double));
   return TLM_OK;
 }
 

@@ -63,6 +63,13 @@ int oracle_read_dft(oracle_handle* h, double* re_im);
 int oracle_set_frequency_outputs(oracle_handle* h, int32_t n, const tlm_frequency_output* outputs,
                                  const int32_t* rows, const int32_t* sides, const int32_t* normals);
 int oracle_read_frequency_output(oracle_handle* h, int32_t k, double* re_im);
+int oracle_set_far_field_surfaces(oracle_handle* h, int32_t n, const tlm_far_field_surface* surfaces,
+                                  const int32_t* rows, const int32_t* sides, const int32_t* normals,
+                                  const double* centres);
+int oracle_read_far_field(oracle_handle* h, int32_t k, double* e);
+int oracle_set_rcs_surface(oracle_handle* h, const tlm_rcs_surface* surface, const int32_t* rows,
+                           const int32_t* sides, const int32_t* normals, const double* centres);
+int oracle_read_rcs(oracle_handle* h, double* out);
 int oracle_set_schedule(oracle_handle* h, int32_t variant, int32_t rows, int32_t zc, int32_t xcd_remap,
                         int32_t mixed);
 int oracle_set_profiling(oracle_handle* h, int32_t enable);

@@ -1,6 +1,7 @@
 """Copy the reference's own test inputs used by tests/test_ingest.py into
 tests/golden/inputs/ (data files of TEST_DATA: problem specification
-files, material files and one model-builder mesh, gzip-compressed).
+files, material files, one model-builder mesh, and the RCS output files
+of one case, the larger ones gzip-compressed).
 
     python tests/golden/make_inputs.py     # in the container holding the reference
 """
@@ -18,6 +19,8 @@ FILES = {
     "HUYGENS_SURFACE/PROBLEM_SPECIFICATION_FILES/huygens_test.inp": "huygens_surface.inp",
     "TEM_PML_TEST/PROBLEM_SPECIFICATION_FILES/TEM_cell.inp": "tem_pml.inp",
     "HELIX_ANTENNA/PROBLEM_SPECIFICATION_FILES/helix.inp": "helix.inp",
+    "RCS_PEC_PLATE_ROUGH_SURFACE/PROBLEM_SPECIFICATION_FILES/plate.inp": "rcs_plate.inp",
+    "RCS_PEC_PLATE_ROUGH_SURFACE/PROBLEM_SPECIFICATION_FILES/plate.rcs.fout_ref": "rcs_plate.rcs.fout_ref",
     "HELIX_ANTENNA/helix.info": "helix.info",
 }
 MATERIALS = ("epsr=2.vmat", "mur=2.vmat", "mur_lorentz.vmat", "epsr=1.5.vmat")
@@ -29,6 +32,9 @@ def main():
         shutil.copyfile(os.path.join(SRC, src), os.path.join(OUT, dst))
     for m in MATERIALS:
         shutil.copyfile(os.path.join(SRC, "MATERIAL_DATA", m), os.path.join(OUT, "materials", m))
+    with open(os.path.join(SRC, "RCS_PEC_PLATE_ROUGH_SURFACE", "PROBLEM_SPECIFICATION_FILES", "plate.rcs.tout_ref"),
+              "rb") as f, gzip.GzipFile(os.path.join(OUT, "rcs_plate.rcs.tout_ref.gz"), "wb", mtime=0) as g:
+        g.write(f.read())
     with open(os.path.join(SRC, "HELIX_ANTENNA", "helix.mesh"), "rb") as f, \
             gzip.GzipFile(os.path.join(OUT, "helix.mesh.gz"), "wb", mtime=0) as g:
         g.write(f.read())

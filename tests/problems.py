@@ -81,7 +81,7 @@ def tem_pml():
                    pml=PML((0, 0, 0, 0, 0, 20), (1e-1, 1e-1, 1e-1, 1e-1, 1e-1, 1e-4), 2),
                    functions=[ExcitationFunction("gaussian", (1.0, 2e-9, 8e-9))],
                    huygens=HuygensBox(Box(1, 1, 1, 1, 2, 1), 1, 0.0, 90.0, 1.0, 0.0, side=1,
-                                      faces=[(1, 1, 1, FACE_ZMAX)]),
+                                      faces=[(1, 1, 2, FACE_ZMIN)]),
                    outputs=[OutputPoint(c1, "ey"), OutputPoint(c1, "hx"), OutputPoint(c2, "ey"),
                             OutputPoint(c2, "hx")])
 

@@ -144,6 +144,21 @@ def test_helix_mesh_matches_the_reference_run_log(tmp_path):
     assert m.points == m2.points
 
 
+def test_surfaces_meshed_as_the_model_builder_meshes_them():
+    """helix.inp's rectangular block, y plane and two z planes, meshed here
+    without the `.mesh` file, give exactly the face lists of the model
+    builder's helix.mesh: the walls' grid lines (ties of the block's
+    +-8.25 m walls rounded away from zero), the cells on each wall, and the
+    side each face is kept on (the outside cell of a block wall; the side
+    its plane's triangle normal points away from)."""
+    deck = ingest.read_inp(os.path.join(D, "helix.inp"))
+    m = ingest.read_mesh(os.path.join(D, "helix.mesh.gz"))
+    for k in (1, 2, 3, 4):
+        mine = sorted(ingest.mesh_surface(deck.surfaces[k], m.mesh_min, m.dl, m.n))
+        ref = sorted(tuple(int(v) for v in f) for f in m.surfaces[k - 1]["faces"])
+        assert mine == ref, k
+
+
 def test_mesh_volumes_become_cell_regions(tmp_path):
     """A deck meshed here and the same deck read back from a written `.mesh`
     (volume cell lists, surface face lists, point cells) give the same
@@ -198,6 +213,38 @@ def test_frequency_output_packets(tmp_path):
     assert [len(e["rows"]) for e in t.freq_outputs] == [len(s.faces), 2 * 4 * 6, 2 * 4 * 6]
     e = t.freq_outputs[1]
     assert (e["first_timestep"], e["last_timestep"], e["timestep_interval"]) == (5, 40, 3)
+
+
+def test_far_field_and_rcs_packets(tmp_path):
+    """Far_field_surface_list and RCS_surface (read_far_field_surface.F90,
+    read_rcs_surface.F90) of RCS_PEC_PLATE_ROUGH_SURFACE's deck.  Its plate
+    is a rough rectangle (random surface points): meshing it needs the
+    model builder's `.mesh`; with the roughness record taken out, the deck
+    builds to tables with the RCS surface and a 361-angle far-field cut."""
+    path = os.path.join(D, "rcs_plate.inp")
+    deck = ingest.read_inp(path)
+    assert deck.surfaces[1].roughness == (0.02, 0.025)
+    assert deck.rcs_surface == {"number": 3, "side": 1, "fmin": 0.3e9, "fmax": 3e9, "fstep": 0.1e8,
+                                "theta": 180.0, "phi": 0.0}
+    assert deck.far_field_surfaces == [{"number": 3, "side": 1, "frequency": 3e9, "theta": (-180.0, 180.0, 1.0),
+                                        "phi": (0.0, 0.0, 1.0), "dim": 3}]
+    with pytest.raises(ingest.IngestError, match="rough"):
+        ingest.problem_from(deck)
+    flat = tmp_path / "plate.inp"
+    flat.write_bytes(open(path, "rb").read().replace(b"surface_roughness \n0.02 0.025\n", b""))
+    t = build_tables(ingest.load(str(flat)).problem)
+    ff, rcs = t.far_fields[0], t.rcs
+    assert (ff["n_theta"], ff["n_phi"], ff["dim"], ff["frequency"]) == (361, 1, 3, 3e9)
+    assert len(ff["rows"]) == len(rcs["rows"]) == 6 * 29 * 29        # the 0.29 m block, dl = 0.01
+    assert np.all(np.abs(ff["centres"]) <= 0.145 + 1e-12) and np.abs(ff["centres"]).max() > 0.145 - 1e-12
+    # the far-field series has as many points as the reference's own run of
+    # this deck wrote records to plate.rcs.tout (initialise_rcs_surfaces'
+    # sizing from the block's K.r range and the 10 ns simulation time)
+    import gzip
+    with gzip.open(os.path.join(D, "rcs_plate.rcs.tout_ref.gz"), "rt") as f:
+        records = [line.split() for line in f if line.strip()]
+    assert [int(r[0]) for r in records] == list(range(1, len(records) + 1))
+    assert rcs["points_per_cell"] == 2 and rcs["n_points"] == len(records) == 659
 
 
 def test_vmat_filters():
