@@ -259,10 +259,17 @@ def test_engine_far_field_and_rcs_vs_oracle(oracle_lib, form):
     g = capi.Handle(capi.engine_library(), t, form, output_capacity=8)
     print(f"[far field] form={form} faces={len(t.far_fields[0]['rows'])} points={t.rcs['n_points']}", flush=True)
     g.step(1, steps)
-    tol = {FORM_PLAIN_FP64: 1e-12, FORM_PRECISION_FP32: 1e-5, FORM_EXTENDED_FP32: 1e-7}[form]
+    # measured: 1.4e-14 (plain), 6.5e-8 (precision), 1.9e-10 (extended)
+    tol = {FORM_PLAIN_FP64: 1e-12, FORM_PRECISION_FP32: 1e-6, FORM_EXTENDED_FP32: 1e-8}[form]
     for k in range(2):
         for want, got in zip(h.read_far_field(k), g.read_far_field(k)):
-            assert np.abs(got - want).max() <= tol * np.abs(want).max(), k
+            err = np.abs(got - want).max() / np.abs(want).max()
+            print(f"[far field] surface {k}: {err:.2e}", flush=True)
+            assert err <= tol, k
     want, got = h.read_rcs(), g.read_rcs()
-    for c in range(4):
-        assert np.abs(got[:, c] - want[:, c]).max() <= tol * np.abs(want[:, c]).max(), c
+    # E_theta, E_phi against the largest E; H_theta, H_phi against the
+    # largest H (the cross-polarised columns are rounding noise)
+    for cols in ((0, 1), (2, 3)):
+        err = np.abs(got[:, cols] - want[:, cols]).max() / np.abs(want[:, cols]).max()
+        print(f"[far field] rcs columns {cols}: {err:.2e}", flush=True)
+        assert err <= tol, cols
