@@ -30,6 +30,11 @@ FAR_FIELD_DTYPE = np.dtype([("frequency", np.float64), ("theta_min", np.float64)
 RCS_DTYPE = np.dtype([("theta", np.float64), ("phi", np.float64), ("n_faces", np.int32), ("n_points", np.int32),
                       ("point_offset", np.int32), ("points_per_cell", np.int32)])
 
+# tlm_field_output (include/tlmscn.h)
+FIELD_OUTPUT_DTYPE = np.dtype([("kind", np.int32), ("field_component", np.int32), ("both_sides", np.int32),
+                               ("first_timestep", np.int32), ("last_timestep", np.int32),
+                               ("timestep_interval", np.int32), ("n_items", np.int32)])
+
 FORM_PLAIN_FP64 = 0
 FORM_PRECISION_FP32 = 1
 FORM_EXTENDED_FP32 = 2   # FP32 links plus an 8-bit residual plane (32-bit significand)
@@ -77,6 +82,8 @@ SIGNATURES = {
     "read_dft": (C.c_int, [_h, _p]),
     "set_frequency_outputs": (C.c_int, [_h, _i32, _p, _p, _p, _p]),
     "read_frequency_output": (C.c_int, [_h, _i32, _p]),
+    "set_field_outputs": (C.c_int, [_h, _i32, _p, _p, _p, _p]),
+    "read_field_output": (C.c_int, [_h, _i32, _p, _i32, _p, _p]),
     "set_far_field_surfaces": (C.c_int, [_h, _i32, _p, _p, _p, _p, _p]),
     "read_far_field": (C.c_int, [_h, _i32, _p]),
     "set_rcs_surface": (C.c_int, [_h, _p, _p, _p, _p, _p]),
@@ -251,6 +258,15 @@ class Handle:
             cat = lambda key: self._arr(np.concatenate([e[key] for e in fos]).astype(i32), i32)
             self._check(f["set_frequency_outputs"](self.h, len(fos), _ptr(outs), _ptr(cat("rows")),
                                                     _ptr(cat("sides")), _ptr(cat("normals"))))
+        fds = getattr(t, "field_outputs", None) or []
+        if fds:
+            outs = np.zeros(len(fds), dtype=FIELD_OUTPUT_DTYPE)
+            for k, e in enumerate(fds):
+                outs[k] = (e["kind"], e["component"], e["both_sides"], e["first_timestep"], e["last_timestep"],
+                           e["timestep_interval"], len(e["rows"]))
+            cat = lambda key: self._arr(np.concatenate([np.asarray(e[key], dtype=i32) for e in fds]), i32)
+            self._check(f["set_field_outputs"](self.h, len(fds), _ptr(self._arr(outs, FIELD_OUTPUT_DTYPE)),
+                                                _ptr(cat("rows")), _ptr(cat("sides")), _ptr(cat("normals"))))
         ffs = getattr(t, "far_fields", None) or []
         if ffs:
             surf = np.zeros(len(ffs), dtype=FAR_FIELD_DTYPE)
@@ -347,6 +363,16 @@ class Handle:
         out = np.zeros((max(n, 1), 2), dtype=np.float64)
         self._check(self.lib.fn["read_frequency_output"](self.h, int(k), _ptr(out)))
         return out[:n, 0] + 1j * out[:n, 1]
+
+    def read_field_output(self, k: int, max_frames: int = None):
+        """The unread frames of field output k: (values (frames, items),
+        timestep of the first frame); reading consumes them."""
+        n = len(self.t.field_outputs[k]["rows"])
+        cap = int(max_frames if max_frames is not None else self.t.n_timesteps)
+        out = np.zeros((max(cap, 1), max(n, 1)), dtype=np.float64)
+        nf, first = C.c_int32(0), C.c_int32(0)
+        self._check(self.lib.fn["read_field_output"](self.h, int(k), _ptr(out), cap, C.byref(nf), C.byref(first)))
+        return out[: nf.value, :n], first.value
 
     def read_far_field(self, k: int):
         """Far field of far-field surface k at r = 1 m: complex E_theta and
@@ -472,7 +498,8 @@ class Handle:
                                  n_timesteps=t.n_timesteps, output_rows=t.output_rows,
                                  output_components=t.output_components, dft_frequencies=t.dft_frequencies,
                                  nx=t.nx, ny=t.ny, nz=t.nz, freq_outputs=getattr(t, "freq_outputs", []),
-                                 far_fields=getattr(t, "far_fields", []), rcs=getattr(t, "rcs", None))
+                                 far_fields=getattr(t, "far_fields", []), rcs=getattr(t, "rcs", None),
+                                 field_outputs=getattr(t, "field_outputs", []))
         self._keep = []
 
     def close(self):

@@ -573,9 +573,9 @@ int tlm_batch_add(tlm_batch* b, tlm_handle* h, int32_t first_timestep, int32_t n
   if ((rc = finalize(h))) return bfail(b, rc, std::string("finalize: ") + h->err);
   if (h->n_host) return bfail(b, TLM_ERR_UNSUPPORTED, "host-path cells are exchanged on the single-solve path");
   if (h->n_host_faces) return bfail(b, TLM_ERR_UNSUPPORTED, "host-path faces are exchanged on the single-solve path");
-  if (h->n_fo_items || h->rcs_on)
+  if (h->n_fo_items || h->rcs_on || !h->fd.empty())
     return bfail(b, TLM_ERR_UNSUPPORTED,
-                 "frequency, far-field and RCS outputs are accumulated on the single-solve path");
+                 "frequency, far-field, RCS and field outputs are recorded on the single-solve path");
   tlm_batch::Entry e;
   e.h = h;
   e.first = first_timestep;

@@ -142,6 +142,16 @@ struct tlm_handle {
   const scn::FreqItem* d_fo_items = nullptr;
   double* d_fo_acc = nullptr;
   int n_fo_items = 0, n_fo_vol = 0, n_fo_surf = 0;
+  // time-domain field outputs (tlm_set_field_outputs): per output its
+  // items, a device ring of fd_cap frames, the frames in it, the timestep
+  // of the oldest unread frame and the frames already moved to the host
+  std::vector<tlm_field_output> fd;
+  std::vector<int32_t> fd_rows, fd_sides, fd_normals;
+  std::vector<const scn::FreqItem*> d_fd_items;
+  std::vector<double*> d_fd_ring;
+  std::vector<int> fd_cap, fd_used, fd_first, fd_host_frames;
+  std::vector<std::vector<double>> fd_host;
+  int n_fd_vol = 0, n_fd_surf = 0;
   // far-field surfaces (tlm_set_far_field_surfaces): their J and M
   // accumulators are frequency-output items after the caller's, from item
   // ff_item0 on, six blocks of n_faces per surface

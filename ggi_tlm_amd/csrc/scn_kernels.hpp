@@ -2280,6 +2280,41 @@ static __global__ void k_freq_out(const FreqOut* __restrict__ outs, const FreqIt
   acc[2 * i + 1] += value * (sy * o.scale);
 }
 
+// One frame of a time-domain field output (output_surfaces.F90:660-720,
+// output_volumes.F90:535-545): one value per item.  kind 1: the cell row's
+// field `comp`; kind 2: the face's field (comp 1..6), J = n x H (7..9),
+// |E| (13), |H| (14) or |J| (15), with J taken as the jump between the two
+// sides of the face when `both`.
+static __global__ void k_field_out(const FreqItem* __restrict__ items, int n, int kind, int comp, int both,
+                                   const double* __restrict__ src, double* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const FreqItem it = items[i];
+  double value;
+  if (kind == 1) {
+    value = src[(long long)(it.row - 1) * 6 + (comp - 1)];
+  } else {
+    const double* f = src + ((long long)(it.row - 1) * 2 + (both ? 0 : it.side - 1)) * 6;
+    const double* g = src + ((long long)(it.row - 1) * 2 + 1) * 6;
+    const double nx = it.normal == 1 ? 1.0 : it.normal == -1 ? -1.0 : 0.0;
+    const double ny = it.normal == 2 ? 1.0 : it.normal == -2 ? -1.0 : 0.0;
+    const double nz = it.normal == 3 ? 1.0 : it.normal == -3 ? -1.0 : 0.0;
+    // the second side's H: zero unless both
+    const double g3 = both ? g[3] : 0.0, g4 = both ? g[4] : 0.0, g5 = both ? g[5] : 0.0;
+    const double jx = (ny * f[5] - nz * f[4]) - (ny * g5 - nz * g4);
+    const double jy = (nz * f[3] - nx * f[5]) - (nz * g3 - nx * g5);
+    const double jz = (nx * f[4] - ny * f[3]) - (nx * g4 - ny * g3);
+    if (comp <= 6) value = f[comp - 1];
+    else if (comp == 7) value = jx;
+    else if (comp == 8) value = jy;
+    else if (comp == 9) value = jz;
+    else if (comp == 13) value = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    else if (comp == 14) value = sqrt(f[3] * f[3] + f[4] * f[4] + f[5] * f[5]);
+    else value = sqrt(jx * jx + jy * jy + jz * jz);
+  }
+  dst[i] = value;
+}
+
 // Far-field transform of one surface (far_field_surfaces.F90:370-712):
 // one workgroup per angle sums the N and L integrals over the faces and
 // forms E_theta, E_phi at r = 1 m.  acc holds the surface's accumulators

@@ -1021,6 +1021,38 @@ int finalize(tlm_handle* h) {
     HIP_TRY(h, hipMemsetAsync(p, 0, nb, h->stream));
     h->d_fo_acc = (double*)p;
   }
+  if (!h->fd.empty()) {
+    // time-domain field outputs: per output its items and a ring of at
+    // most the output capacity of frames (fewer when its window has fewer)
+    size_t k = 0;
+    for (size_t o = 0; o < h->fd.size(); o++) {
+      const tlm_field_output& f = h->fd[o];
+      const int nrows = f.kind == 1 ? d.n_out_rows : d.n_face_out_rows;
+      std::vector<scn::FreqItem> items;
+      for (int i = 0; i < f.n_items; i++, k++) {
+        const int r = h->fd_rows[k];
+        if (r < 1 || r > nrows)
+          return fail(h, TLM_ERR_ARG, "field output " + std::to_string(o + 1) + ": output row out of range");
+        items.push_back({(int)o, r, f.kind == 2 ? h->fd_sides[k] : 1, f.kind == 2 ? h->fd_normals[k] : 0});
+      }
+      const scn::FreqItem* di;
+      if ((rc = dev_upload(h, items, &di))) return rc;
+      int due = 0;
+      for (int t = std::max(1, f.first_timestep); t <= std::min(h->g.n_timesteps, f.last_timestep) && due < h->cap; t++)
+        if ((t - f.first_timestep) % f.timestep_interval == 0) due++;
+      const int cap = std::max(1, due);
+      void* p;
+      if ((rc = dev_alloc(h, (size_t)cap * std::max(1, f.n_items) * sizeof(double), &p))) return rc;
+      h->d_fd_items.push_back(di);
+      h->d_fd_ring.push_back((double*)p);
+      h->fd_cap.push_back(cap);
+      (f.kind == 1 ? h->n_fd_vol : h->n_fd_surf)++;
+    }
+    h->fd_used.assign(h->fd.size(), 0);
+    h->fd_first.assign(h->fd.size(), 0);
+    h->fd_host_frames.assign(h->fd.size(), 0);
+    h->fd_host.assign(h->fd.size(), {});
+  }
   if (h->rcs_on) {
     // faces sorted by d = K.r (initialise_rcs_surfaces, RCS_surfaces.F90:
     // 159-367): K the unit vector of (theta, phi), r the face centre
@@ -1044,6 +1076,47 @@ int finalize(tlm_handle* h) {
   }
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   h->finalized = true;
+  return TLM_OK;
+}
+
+// Move field output o's frames from its device ring to the host.
+static int drain_field_output(tlm_handle* h, size_t o) {
+  const int n = h->fd_used[o];
+  if (!n) return TLM_OK;
+  const size_t items = (size_t)h->fd[o].n_items;
+  std::vector<double>& v = h->fd_host[o];
+  const size_t old = v.size();
+  v.resize(old + (size_t)n * items);
+  if (items) {
+    HIP_TRY(h, hipMemcpyAsync(v.data() + old, h->d_fd_ring[o], (size_t)n * items * sizeof(double),
+                              hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+  }
+  h->fd_host_frames[o] += n;
+  h->fd_used[o] = 0;
+  return TLM_OK;
+}
+
+// One frame of every field output of this kind due at `timestep`
+// (get_output_flag), from cell-output slot or face-output block `slot`.
+int launch_field_outputs(tlm_handle* h, int kind, int slot, int timestep) {
+  if (!(kind == 1 ? h->n_fd_vol : h->n_fd_surf) || slot < 0) return TLM_OK;
+  const double* src = kind == 1 ? h->d.ring + (size_t)slot * h->d.n_out_rows * 6
+                                : h->d.face_ring + (size_t)slot * h->d.n_face_out_rows * 12;
+  for (size_t o = 0; o < h->fd.size(); o++) {
+    const tlm_field_output& f = h->fd[o];
+    if (f.kind != kind || timestep < f.first_timestep || timestep > f.last_timestep ||
+        (timestep - f.first_timestep) % f.timestep_interval != 0)
+      continue;
+    int rc;
+    if (h->fd_used[o] == h->fd_cap[o] && (rc = drain_field_output(h, o))) return rc;
+    if (h->fd_used[o] == 0 && h->fd_host_frames[o] == 0) h->fd_first[o] = timestep;
+    if (f.n_items)
+      hipLaunchKernelGGL(scn::k_field_out, dim3((f.n_items + 255) / 256), dim3(256), 0, h->stream, h->d_fd_items[o],
+                         f.n_items, kind, f.field_component, f.both_sides, src,
+                         h->d_fd_ring[o] + (size_t)h->fd_used[o] * f.n_items);
+    h->fd_used[o]++;
+  }
   return TLM_OK;
 }
 
@@ -1392,11 +1465,13 @@ int do_scatter(tlm_handle* h, int timestep) {
                          slot, timestep, h->d_prow, h->d_pcomp, np, h->d_freq, nf, h->g.dt, h->d_dft);
     }
     launch_freq_outputs(h, 1, slot, timestep);
+    if ((rc = launch_field_outputs(h, 1, slot, timestep))) return rc;
   }
   // the pulled connect's face outputs
   if (pull) {
     launch_freq_outputs(h, 2, fslot, a.ts_connect);
     launch_rcs(h, fslot, a.ts_connect);
+    if ((rc = launch_field_outputs(h, 2, fslot, a.ts_connect))) return rc;
   }
   const int out = out_buffer(h);
   h->prv = h->cur;
@@ -1425,6 +1500,7 @@ int materialize(tlm_handle* h) {
   if (rc) return rc;
   launch_freq_outputs(h, 2, fslot, a.ts_connect);
   launch_rcs(h, fslot, a.ts_connect);
+  if ((rc = launch_field_outputs(h, 2, fslot, a.ts_connect))) return rc;
   HIP_TRY(h, hipGetLastError());
   h->cur = out_buffer(h);  // prv stays: the next step still pulls it again
   h->kind = Kind::INCIDENT;
@@ -1692,6 +1768,54 @@ int tlm_read_frequency_output(tlm_handle* h, int32_t k, double* re_im) {
     HIP_TRY(h, hipMemcpyAsync(re_im, h->d_fo_acc + off * 2, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
   }
+  return TLM_OK;
+}
+
+int tlm_set_field_outputs(tlm_handle* h, int32_t n, const tlm_field_output* outs, const int32_t* rows,
+                          const int32_t* sides, const int32_t* normals) {
+  if (!h || n < 0 || (n && !outs)) return TLM_ERR_ARG;
+  if (h->finalized) return fail(h, TLM_ERR_STATE, "tables are fixed after the first step");
+  size_t items = 0;
+  for (int i = 0; i < n; i++) {
+    const tlm_field_output& f = outs[i];
+    const int c = f.field_component;
+    const bool surf_c = (c >= 1 && c <= 9) || (c >= 13 && c <= 15);
+    const bool ok = (f.kind == 1 && c >= 1 && c <= 6 && !f.both_sides) ||
+                    (f.kind == 2 && surf_c && (!f.both_sides || (c >= 7 && c <= 9) || c == 15));
+    if (!ok || f.n_items < 0 || f.timestep_interval < 1 || (f.both_sides != 0 && f.both_sides != 1))
+      return fail(h, TLM_ERR_ARG, "field output " + std::to_string(i + 1) + ": bad kind, component or window");
+    items += (size_t)f.n_items;
+  }
+  if (items && (!rows || !sides || !normals)) return TLM_ERR_ARG;
+  for (size_t i = 0, k = 0; i < (size_t)n; i++)
+    for (int j = 0; j < outs[i].n_items; j++, k++)
+      if (outs[i].kind == 2 && (sides[k] < 1 || sides[k] > 2 || normals[k] == 0 || normals[k] < -3 || normals[k] > 3))
+        return fail(h, TLM_ERR_ARG, "field output " + std::to_string(i + 1) + ": bad face side or normal");
+  h->fd.assign(outs, outs + n);
+  h->fd_rows.assign(rows, rows + items);
+  h->fd_sides.assign(sides, sides + items);
+  h->fd_normals.assign(normals, normals + items);
+  return TLM_OK;
+}
+
+int tlm_read_field_output(tlm_handle* h, int32_t k, double* values, int32_t max_frames, int32_t* n_frames,
+                          int32_t* first_timestep) {
+  if (!h || k < 0 || max_frames < 0 || !n_frames || !first_timestep || (max_frames && !values)) return TLM_ERR_ARG;
+  int rc;
+  if ((rc = finalize(h))) return rc;
+  if (k >= (int)h->fd.size()) return fail(h, TLM_ERR_ARG, "no such field output");
+  if ((rc = materialize(h))) return rc;  // a pending connect records its face frame
+  if ((rc = drain_field_output(h, (size_t)k))) return rc;
+  const tlm_field_output& f = h->fd[k];
+  const int n = std::min(max_frames, h->fd_host_frames[k]);
+  const size_t items = (size_t)f.n_items;
+  std::vector<double>& v = h->fd_host[k];
+  if (n && items) std::memcpy(values, v.data(), (size_t)n * items * sizeof(double));
+  *n_frames = n;
+  *first_timestep = h->fd_first[k];
+  v.erase(v.begin(), v.begin() + (size_t)n * items);
+  h->fd_host_frames[k] -= n;
+  h->fd_first[k] += n * f.timestep_interval;
   return TLM_OK;
 }
 

@@ -61,6 +61,13 @@ module tlm_gpu_iface
     integer(c_int32_t) :: n_items
   end type tlm_frequency_output
 
+  ! one time-domain surface or volume field output (tlm_field_output)
+  type, bind(c) :: tlm_field_output
+    integer(c_int32_t) :: kind, field_component, both_sides
+    integer(c_int32_t) :: first_timestep, last_timestep, timestep_interval
+    integer(c_int32_t) :: n_items
+  end type tlm_field_output
+
   ! one far-field surface (tlm_far_field_surface); angles in radians
   type, bind(c) :: tlm_far_field_surface
     real(c_double) :: frequency
@@ -388,6 +395,25 @@ module tlm_gpu_iface
       integer(c_int32_t), value :: k
       real(c_double), intent(inout) :: re_im(*)
     end function tlm_read_frequency_output
+
+    integer(c_int) function tlm_set_field_outputs(h, n, outputs, rows, sides, normals) &
+                                               bind(c, name='tlm_set_field_outputs')
+      import :: c_int, c_int32_t, c_ptr, tlm_field_output
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n
+      type(tlm_field_output), intent(in) :: outputs(*)
+      integer(c_int32_t), intent(in) :: rows(*), sides(*), normals(*)
+    end function tlm_set_field_outputs
+
+    ! values(n_items, max_frames), oldest frame first; the frames read are consumed
+    integer(c_int) function tlm_read_field_output(h, k, values, max_frames, n_frames, first_timestep) &
+                                               bind(c, name='tlm_read_field_output')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: k, max_frames
+      real(c_double), intent(inout) :: values(*)
+      integer(c_int32_t), intent(out) :: n_frames, first_timestep
+    end function tlm_read_field_output
 
     integer(c_int) function tlm_set_far_field_surfaces(h, n, surfaces, rows, sides, normals, centres) &
                                                bind(c, name='tlm_set_far_field_surfaces')

@@ -22,10 +22,11 @@ the model builder meshes them: rectangular_block volumes (cells whose
 centre lies inside), rectangular_block, rectangle and x/y/zplane surfaces
 (the cell faces on them), and points (point_to_cell,
 general_mesh_subroutines.F90:345-383); rotated or other shapes need the
-model builder's `.mesh`.  Frequency output surfaces and volumes,
-far-field surfaces and the RCS surface become device-accumulated outputs.
-Packets the engine has no path for (time-domain surface and volume field
-outputs, modes, cables) are listed in `Deck.unsupported` and left out.
+model builder's `.mesh`.  Surface and volume field outputs, frequency
+output surfaces and volumes, far-field surfaces and the RCS surface become
+device-recorded outputs.  Packets the engine has no path for (modes,
+cables, volume averages and peaks) are listed in `Deck.unsupported` and left
+out.
 """
 from __future__ import annotations
 
@@ -40,8 +41,8 @@ import numpy as np
 
 from .constants import CENTRE, FACE_XMAX, FACE_XMIN, FACE_YMAX, FACE_YMIN, FACE_ZMAX, FACE_ZMIN
 from .filters import Sfilter
-from .problem import (PML, Box, ExcitationFunction, FarFieldSurface, FrequencyOutput, HuygensBox, OutputPoint,
-                      PointSource, Problem, RCSSurface, VolumeMaterial, point_to_cell)
+from .problem import (PML, Box, ExcitationFunction, FarFieldSurface, FieldOutput, FrequencyOutput, HuygensBox,
+                      OutputPoint, PointSource, Problem, RCSSurface, VolumeMaterial, point_to_cell)
 
 _LOCATIONS = {"xmin": FACE_XMIN, "xmax": FACE_XMAX, "ymin": FACE_YMIN, "ymax": FACE_YMAX, "zmin": FACE_ZMIN,
               "zmax": FACE_ZMAX, "centre": CENTRE}
@@ -155,6 +156,7 @@ class Deck:
     frequency_outputs: List[dict] = field(default_factory=list)
     far_field_surfaces: List[dict] = field(default_factory=list)
     rcs_surface: Optional[dict] = None
+    field_outputs: List[dict] = field(default_factory=list)
     unsupported: List[str] = field(default_factory=list)
 
 
@@ -364,6 +366,34 @@ def read_inp_text(text: str, name: str = "input") -> Deck:
                     raise IngestError(f"frequency output volume {k}: component {comp!r}")
                 d.frequency_outputs.append({"kind": "volume", "number": vol, "frequency": freq, "component": comp,
                                             **_time_information(r)})
+        elif line == "output_surface_list":
+            # read_output_surface_list.F90:74-131: surface, side (+1, -1, or
+            # 0 for both sides), component (its first two letters), window
+            n = r.ints(1)[0]
+            for k in range(1, n + 1):
+                if r.ints(1)[0] != k:
+                    raise IngestError("output surfaces must be numbered 1..n in order")
+                surf, side = r.ints(1)[0], r.ints(1)[0]
+                if side not in (1, -1, 0):
+                    raise IngestError("output surface side must be +1, -1 or 0")
+                comp = r.record().strip()[:2].lower()
+                if comp not in _COMPONENTS + ("em", "hm", "jm"):
+                    raise IngestError(f"output surface {k}: component {comp!r}")
+                if side == 0 and comp not in ("jx", "jy", "jz", "jm"):
+                    raise IngestError(f"output surface {k}: both sides take Jx, Jy, Jz or Jmagnitude")
+                d.field_outputs.append({"kind": "surface", "number": surf, "side": side, "component": comp,
+                                        **_time_information(r)})
+        elif line == "output_volume_list":
+            # read_output_volume_list.F90:70-90: volume, field component, window
+            n = r.ints(1)[0]
+            for k in range(1, n + 1):
+                if r.ints(1)[0] != k:
+                    raise IngestError("output volumes must be numbered 1..n in order")
+                vol = r.ints(1)[0]
+                comp = r.word()
+                if comp not in _COMPONENTS[:6]:
+                    raise IngestError(f"output volume {k}: component {comp!r}")
+                d.field_outputs.append({"kind": "volume", "number": vol, "component": comp, **_time_information(r)})
         elif line == "far_field_surface_list":
             # read_far_field_surface.F90:77-143: surface, side, frequency,
             # theta and phi ranges (degrees), then an optional '2D' record
@@ -823,6 +853,17 @@ def problem_from(deck: Deck, mesh: Optional[MeshFile] = None, material_dir: Opti
         else:
             freq_outputs.append(FrequencyOutput("surface", fo["frequency"], fo["component"],
                                                 faces=surface_faces(fo["number"]), side=fo["side"], **window))
+    field_outs = []
+    for fo in deck.field_outputs:
+        if "first_time" in fo:
+            raise IngestError("field output windows given in time (output_time_information) have no path here")
+        window = {"first_timestep": fo["first_timestep"], "last_timestep": fo["last_timestep"],
+                  "timestep_interval": fo["timestep_interval"]}
+        if fo["kind"] == "volume":
+            field_outs.append(FieldOutput("volume", fo["component"], region=volume_region(fo["number"]), **window))
+        else:
+            field_outs.append(FieldOutput("surface", fo["component"], faces=surface_faces(fo["number"]),
+                                          side=fo["side"], **window))
     far_fields = [FarFieldSurface(surface_faces(f["number"]), f["frequency"], theta=f["theta"], phi=f["phi"],
                                   side=f["side"], dim=f["dim"]) for f in deck.far_field_surfaces]
     rcs = None
@@ -834,7 +875,8 @@ def problem_from(deck: Deck, mesh: Optional[MeshFile] = None, material_dir: Opti
                 reflection=tuple(deck.reflection), materials=materials, material_regions=regions,
                 surface_materials=smats, surface_faces=sfaces, functions=list(deck.functions), sources=sources,
                 huygens=huygens, outputs=outputs, pml=pml, reduced_c_factor=deck.reduced_c_factor,
-                frequency_outputs=freq_outputs, far_field_surfaces=far_fields, rcs_surface=rcs)
+                frequency_outputs=freq_outputs, far_field_surfaces=far_fields, rcs_surface=rcs,
+                field_outputs=field_outs)
     info = [{k: v for k, v in o.items() if k not in ("point", "component", "location")} for o in deck.output_points]
     return Ingested(p, list(deck.unsupported), info, sorted(boundary))
 

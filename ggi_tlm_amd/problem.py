@@ -170,6 +170,27 @@ class FrequencyOutput:
 
 
 @dataclass
+class FieldOutput:
+    """A time-domain field output (read_output_surface_list.F90,
+    read_output_volume_list.F90; output_surfaces.F90, output_volumes.F90):
+    every step in the window records one value per cell (kind 'volume':
+    region, component 'ex'..'hz') or per face (kind 'surface': faces of the
+    surface mesh; side +1 the outward normal, -1 the other side, 0 both
+    sides, which takes the surface current as the jump across the face;
+    components 'ex'..'hz', 'jx'..'jz', 'em', 'hm', 'jm' for |E|, |H|,
+    |J|)."""
+
+    kind: str
+    component: str
+    region: object = None
+    faces: Optional[list] = None
+    side: int = 1
+    first_timestep: int = 0
+    last_timestep: int = 1000000000
+    timestep_interval: int = 1
+
+
+@dataclass
 class FarFieldSurface:
     """A far-field surface (read_far_field_surface.F90, far_field_surfaces.F90):
     faces (cx, cy, cz, face) of the surface mesh and the side (+1 the
@@ -233,6 +254,7 @@ class Problem:
     frequency_outputs: List[FrequencyOutput] = field(default_factory=list)
     far_field_surfaces: List[FarFieldSurface] = field(default_factory=list)
     rcs_surface: Optional[RCSSurface] = None
+    field_outputs: List[FieldOutput] = field(default_factory=list)
 
     @property
     def dt(self) -> float:
@@ -417,6 +439,7 @@ class Tables:
     output_cells: np.ndarray = None   # (n_output_cells, 3) 1-based cx,cy,cz
     freq_outputs: list = field(default_factory=list)  # frequency output volumes and surfaces
     far_fields: list = field(default_factory=list)    # far-field surfaces
+    field_outputs: list = field(default_factory=list)  # time-domain surface and volume field outputs
     rcs: Optional[dict] = None                        # the RCS surface
     n_pml_cells: int = 0
 
@@ -547,7 +570,7 @@ def build_tables(p: Problem) -> Tables:
             local_out[o.cell[2] - 1, o.cell[1] - 1, o.cell[0] - 1] = True
     # frequency output volumes make their cells output cells
     # (set_outputs_in_mesh, output.F90:46-)
-    for fq in p.frequency_outputs:
+    for fq in list(p.frequency_outputs) + list(p.field_outputs):
         if fq.kind == "volume":
             local_out |= fq.region.mask(nx, ny, nz)
     if (pml_flag & local_exc).any():
@@ -727,6 +750,10 @@ def build_tables(p: Problem) -> Tables:
     fo_faces = [output_surface(fq.faces, fq.side, "frequency output surface") if fq.kind == "surface" else []
                 for fq in p.frequency_outputs]
     ff_faces = [output_surface(s.faces, s.side, "far-field surface") for s in p.far_field_surfaces]
+    # field output surfaces: side 0 (both sides) turns the faces as -1 does
+    # (output_surfaces.F90:42-259)
+    fd_faces = [output_surface(fo.faces, 1 if fo.side == 1 else -1, "field output surface")
+                if fo.kind == "surface" else [] for fo in p.field_outputs]
     rcs_faces = output_surface(p.rcs_surface.faces, p.rcs_surface.side, "RCS surface") if p.rcs_surface else []
     face_out_pts = []
     for o in p.outputs:
@@ -895,6 +922,29 @@ def build_tables(p: Problem) -> Tables:
         return (np.asarray(rows, dtype=np.int32), np.asarray(sides, dtype=np.int32),
                 np.asarray(normals, dtype=np.int32), np.asarray(centres, dtype=np.float64).reshape(-1, 3))
 
+    field_outputs = []
+    extra = {"em": 13, "hm": 14, "jm": 15}
+    for fo, faces in zip(p.field_outputs, fd_faces):
+        comp = extra[fo.component.lower()] if fo.component.lower() in extra else _comp(fo.component)
+        e = {"kind": 1 if fo.kind == "volume" else 2, "component": comp, "both_sides": int(fo.side == 0),
+             "first_timestep": int(fo.first_timestep), "last_timestep": int(fo.last_timestep),
+             "timestep_interval": int(fo.timestep_interval)}
+        if fo.kind == "volume":
+            if comp > 6:
+                raise ValueError("output volumes take field components Ex..Hz")
+            m = fo.region.mask(nx, ny, nz).ravel()
+            e["rows"] = out_rank[np.nonzero(m)[0]].astype(np.int32)
+            e["sides"] = np.zeros(len(e["rows"]), dtype=np.int32)
+            e["normals"] = np.zeros(len(e["rows"]), dtype=np.int32)
+            e["cells"] = np.stack(np.unravel_index(np.nonzero(m)[0], (nz, ny, nx))[::-1], axis=1) + 1
+        else:
+            if fo.side == 0 and comp not in (7, 8, 9, 15):
+                raise ValueError("output on both sides of a surface takes Jx, Jy, Jz or Jm")
+            rows, sides, normals, _ = surface_items(faces)
+            e["rows"], e["sides"], e["normals"] = rows, sides, normals
+            e["faces"] = np.asarray(faces, dtype=np.int64).reshape(-1, 4)
+        field_outputs.append(e)
+
     # far-field surfaces: angles in radians and their counts as
     # read_far_field_surface.F90 and write_far_field_surfaces
     # (far_field_surfaces.F90:503-507) form them
@@ -952,5 +1002,5 @@ def build_tables(p: Problem) -> Tables:
         face_output_points=face_out_pts,
         dft_frequencies=np.asarray(p.dft_frequencies, dtype=np.float64),
         output_cells=out_cells, n_pml_cells=n_pml_cells, freq_outputs=freq_outputs,
-        far_fields=far_fields, rcs=rcs,
+        far_fields=far_fields, rcs=rcs, field_outputs=field_outputs,
     )

@@ -238,6 +238,34 @@ typedef struct {
 int tlm_set_frequency_outputs(tlm_handle* h, int32_t n, const tlm_frequency_output* outputs,
                               const int32_t* rows, const int32_t* sides, const int32_t* normals);
 
+/* Time-domain surface and volume field outputs (SRC/OUTPUT/output_surfaces.F90:
+ * 601-948, output_volumes.F90:496-761).  Every step in an output's window
+ * (get_output_flag) records one value per item, on the device, as a frame:
+ * kind 1: a volume; items are cell output rows, field_component 1..6.
+ * kind 2: a surface; items are face output rows with the side of the face
+ * whose fields they take and the face normal (signed axis), as for
+ * frequency output surfaces; field_component 1..6 a field, 7..9 the
+ * surface current J = n x H, 13 |E|, 14 |H|, 15 |J|.  both_sides 1 (the
+ * packet's side 0; J components and |J| only) takes J as the jump
+ * n x H(side 1) - n x H(side 2) of the face and the fields of side 1.
+ * Frames wait on the device, each output in a ring of the handle's output
+ * capacity, and move to the host when a ring fills or on read. */
+typedef struct {
+  int32_t kind;
+  int32_t field_component;
+  int32_t both_sides;
+  int32_t first_timestep, last_timestep, timestep_interval;
+  int32_t n_items;
+} tlm_field_output;
+int tlm_set_field_outputs(tlm_handle* h, int32_t n, const tlm_field_output* outputs, const int32_t* rows,
+                          const int32_t* sides, const int32_t* normals);
+/* The unread frames of field output k (0-based), oldest first, up to
+ * max_frames: values[f * n_items + i].  *n_frames frames are returned and
+ * consumed; *first_timestep is the timestep of the first (the others follow
+ * at the output's timestep interval). */
+int tlm_read_field_output(tlm_handle* h, int32_t k, double* values, int32_t max_frames, int32_t* n_frames,
+                          int32_t* first_timestep);
+
 /* Far-field surfaces (SRC/OUTPUT/far_field_surfaces.F90).  The connect of
  * every step adds to each face its surface currents J = n x H and
  * M = -n x E times exp(-j 2 pi f time) dt, time = (t-1) dt + dt/2

@@ -119,6 +119,13 @@ struct oracle_handle {
   tlm_frequency_output* fo;
   int32_t *fo_out, *fo_row, *fo_side, *fo_norm;
   double* fo_acc;
+  /* time-domain field outputs: per output its items and the recorded
+     frames not yet read (values, count, timestep of the first) */
+  int n_fd;
+  tlm_field_output* fd;
+  int32_t *fd_row, *fd_side, *fd_norm; /* items of all outputs, concatenated */
+  double** fd_frames;
+  int *fd_nframes, *fd_capframes, *fd_first;
   /* far-field surfaces: per surface its faces (row, side, normal, centre)
      and the accumulated J(face, 1:3), M(face, 1:3) as complex pairs */
   int n_ff;
@@ -209,6 +216,9 @@ void oracle_destroy(oracle_handle* h) {
   free(h->pt_row); free(h->pt_comp); free(h->freq); free(h->dft); free(h->ring);
   free(h->fo); free(h->fo_out); free(h->fo_row); free(h->fo_side); free(h->fo_norm); free(h->fo_acc);
   free(h->ff); free(h->ff_row); free(h->ff_side); free(h->ff_norm); free(h->ff_centre); free(h->ff_J); free(h->ff_M);
+  for (int i = 0; i < h->n_fd; i++) free(h->fd_frames[i]);
+  free(h->fd); free(h->fd_row); free(h->fd_side); free(h->fd_norm); free(h->fd_frames); free(h->fd_nframes);
+  free(h->fd_capframes); free(h->fd_first);
   free(h->rcs_row); free(h->rcs_side); free(h->rcs_norm); free(h->rcs_centre); free(h->rcs_E);
   free(h->pl_out); free(h->pl_exc); free(h->pl_fexc); free(h->pl_fout);
   free(h->hf_of_code); free(h->hf_tot);
@@ -980,6 +990,62 @@ static void accumulate_frequency_outputs(oracle_handle* h, int kind, const doubl
   }
 }
 
+/* face_output_surfaces (output_surfaces.F90:601-720) for kind 2 and
+   cell_output_volumes (output_volumes.F90:496-545) for kind 1: every output
+   due at this timestep (get_output_flag) records one value per item. */
+static void record_field_outputs(oracle_handle* h, int kind, const double* src, int timestep) {
+  size_t k = 0;
+  for (int o = 0; o < h->n_fd; o++) {
+    const tlm_field_output* f = &h->fd[o];
+    size_t k0 = k;
+    k += (size_t)f->n_items;
+    if (f->kind != kind || timestep < f->first_timestep || timestep > f->last_timestep ||
+        (timestep - f->first_timestep) % f->timestep_interval != 0)
+      continue;
+    if (h->fd_nframes[o] == h->fd_capframes[o]) {
+      h->fd_capframes[o] = h->fd_capframes[o] ? 2 * h->fd_capframes[o] : 16;
+      h->fd_frames[o] = (double*)realloc(h->fd_frames[o],
+                                         (size_t)h->fd_capframes[o] * (f->n_items ? f->n_items : 1) * sizeof(double));
+    }
+    if (h->fd_nframes[o] == 0) h->fd_first[o] = timestep;
+    double* out = h->fd_frames[o] + (size_t)h->fd_nframes[o] * f->n_items;
+    int c = f->field_component;
+    for (int i = 0; i < f->n_items; i++) {
+      size_t q = k0 + (size_t)i;
+      double value;
+      if (kind == 1) {
+        value = src[(size_t)(h->fd_row[q] - 1) * 6 + (c - 1)];
+      } else {
+        int nrm = h->fd_norm[q];
+        double normx = nrm == 1 ? 1.0 : nrm == -1 ? -1.0 : 0.0;
+        double normy = nrm == 2 ? 1.0 : nrm == -2 ? -1.0 : 0.0;
+        double normz = nrm == 3 ? 1.0 : nrm == -3 ? -1.0 : 0.0;
+        double field[6], field2[6];
+        const double* blk = &src[(size_t)(h->fd_row[q] - 1) * 12];
+        if (!f->both_sides) {
+          memcpy(field, blk + (size_t)(h->fd_side[q] - 1) * 6, sizeof(field));
+          for (int j = 0; j < 6; j++) field2[j] = 0.0;
+        } else {
+          memcpy(field, blk, sizeof(field));
+          memcpy(field2, blk + 6, sizeof(field2));
+        }
+        double Jsx = (normy * field[5] - normz * field[4]) - (normy * field2[5] - normz * field2[4]);
+        double Jsy = (normz * field[3] - normx * field[5]) - (normz * field2[3] - normx * field2[5]);
+        double Jsz = (normx * field[4] - normy * field[3]) - (normx * field2[4] - normy * field2[3]);
+        if (c <= 6) value = field[c - 1];
+        else if (c == 7) value = Jsx;
+        else if (c == 8) value = Jsy;
+        else if (c == 9) value = Jsz;
+        else if (c == 13) value = sqrt(field[0] * field[0] + field[1] * field[1] + field[2] * field[2]);
+        else if (c == 14) value = sqrt(field[3] * field[3] + field[4] * field[4] + field[5] * field[5]);
+        else value = sqrt(Jsx * Jsx + Jsy * Jsy + Jsz * Jsz);
+      }
+      out[i] = value;
+    }
+    h->fd_nframes[o]++;
+  }
+}
+
 /* face_output_far_field (far_field_surfaces.F90:252-351): every connect
    adds ejwt (n x H) to J and -ejwt (n x E) to M of each face, ejwt =
    exp(-j 2 pi f time) dt, time = (t-1) dt + dt/2, with the field of the
@@ -1075,6 +1141,7 @@ static void record_outputs(oracle_handle* h, int timestep) {
     }
   }
   if (h->n_fo_items && rows) accumulate_frequency_outputs(h, 1, h->cell_out, timestep, (double)(timestep - 1) * h->g.dt);
+  if (h->n_fd && rows) record_field_outputs(h, 1, h->cell_out, timestep);
 }
 
 /* Scan-order counters of scatter() at the start of each z-plane: output
@@ -1600,6 +1667,7 @@ int oracle_connect(oracle_handle* h, int32_t timestep) {
     if (h->n_fo_items) accumulate_frequency_outputs(h, 2, h->face_out, timestep, time);
     if (h->n_ff) far_field_currents(h, h->face_out, time);
     if (h->rcs_on) rcs_currents(h, h->face_out, time);
+    if (h->n_fd) record_field_outputs(h, 2, h->face_out, timestep);
   }
   return TLM_OK;
 }
@@ -1744,6 +1812,64 @@ int oracle_read_frequency_output(oracle_handle* h, int32_t k, double* re_im) {
   size_t off = 0;
   for (int i = 0; i < k; i++) off += (size_t)h->fo[i].n_items;
   memcpy(re_im, h->fo_acc + off * 2, (size_t)h->fo[k].n_items * 2 * sizeof(double));
+  return TLM_OK;
+}
+
+int oracle_set_field_outputs(oracle_handle* h, int32_t n, const tlm_field_output* outs, const int32_t* rows,
+                             const int32_t* sides, const int32_t* normals) {
+  if (n < 0 || (n && !outs)) return TLM_ERR_ARG;
+  size_t items = 0;
+  for (int i = 0; i < n; i++) {
+    const tlm_field_output* f = &outs[i];
+    int c = f->field_component;
+    int surf_c = (c >= 1 && c <= 9) || (c >= 13 && c <= 15);
+    int ok = (f->kind == 1 && c >= 1 && c <= 6 && !f->both_sides) ||
+             (f->kind == 2 && surf_c && (!f->both_sides || (c >= 7 && c <= 9) || c == 15));
+    if (!ok || f->n_items < 0 || f->timestep_interval < 1 || (f->both_sides != 0 && f->both_sides != 1))
+      return fail(h, TLM_ERR_ARG, "field output: bad kind, component or window");
+    items += (size_t)f->n_items;
+  }
+  if (items && (!rows || !sides || !normals)) return TLM_ERR_ARG;
+  size_t k = 0;
+  for (int i = 0; i < n; i++) {
+    int nrows = outs[i].kind == 1 ? h->n_out_rows : h->n_face_out_rows;
+    for (int j = 0; j < outs[i].n_items; j++, k++) {
+      if (rows[k] < 1 || rows[k] > nrows) return fail(h, TLM_ERR_ARG, "field output: row out of range");
+      if (outs[i].kind == 2 && (sides[k] < 1 || sides[k] > 2 || normals[k] == 0 || normals[k] < -3 || normals[k] > 3))
+        return fail(h, TLM_ERR_ARG, "field output: bad face side or normal");
+    }
+  }
+  for (int i = 0; i < h->n_fd; i++) free(h->fd_frames[i]);
+  free(h->fd); free(h->fd_row); free(h->fd_side); free(h->fd_norm); free(h->fd_frames); free(h->fd_nframes);
+  free(h->fd_capframes); free(h->fd_first);
+  h->n_fd = n;
+  h->fd = (tlm_field_output*)malloc((size_t)(n ? n : 1) * sizeof(tlm_field_output));
+  if (n) memcpy(h->fd, outs, (size_t)n * sizeof(tlm_field_output));
+  h->fd_row = dup_i32(rows, items);
+  h->fd_side = dup_i32(sides, items);
+  h->fd_norm = dup_i32(normals, items);
+  h->fd_frames = (double**)calloc((size_t)(n ? n : 1), sizeof(double*));
+  h->fd_nframes = (int*)calloc((size_t)(n ? n : 1), sizeof(int));
+  h->fd_capframes = (int*)calloc((size_t)(n ? n : 1), sizeof(int));
+  h->fd_first = (int*)calloc((size_t)(n ? n : 1), sizeof(int));
+  return TLM_OK;
+}
+
+int oracle_read_field_output(oracle_handle* h, int32_t k, double* values, int32_t max_frames, int32_t* n_frames,
+                             int32_t* first_timestep) {
+  if (k < 0 || k >= h->n_fd || max_frames < 0 || !n_frames || !first_timestep || (max_frames && !values))
+    return TLM_ERR_ARG;
+  const tlm_field_output* f = &h->fd[k];
+  int n = max_frames < h->fd_nframes[k] ? max_frames : h->fd_nframes[k];
+  size_t items = (size_t)f->n_items;
+  if (n && items) memcpy(values, h->fd_frames[k], (size_t)n * items * sizeof(double));
+  *n_frames = n;
+  *first_timestep = h->fd_first[k];
+  if (n && items)
+    memmove(h->fd_frames[k], h->fd_frames[k] + (size_t)n * items,
+            (size_t)(h->fd_nframes[k] - n) * items * sizeof(double));
+  h->fd_nframes[k] -= n;
+  h->fd_first[k] += n * f->timestep_interval;
   return TLM_OK;
 }
 

@@ -63,6 +63,10 @@ int oracle_read_dft(oracle_handle* h, double* re_im);
 int oracle_set_frequency_outputs(oracle_handle* h, int32_t n, const tlm_frequency_output* outputs,
                                  const int32_t* rows, const int32_t* sides, const int32_t* normals);
 int oracle_read_frequency_output(oracle_handle* h, int32_t k, double* re_im);
+int oracle_set_field_outputs(oracle_handle* h, int32_t n, const tlm_field_output* outputs, const int32_t* rows,
+                             const int32_t* sides, const int32_t* normals);
+int oracle_read_field_output(oracle_handle* h, int32_t k, double* values, int32_t max_frames, int32_t* n_frames,
+                             int32_t* first_timestep);
 int oracle_set_far_field_surfaces(oracle_handle* h, int32_t n, const tlm_far_field_surface* surfaces,
                                   const int32_t* rows, const int32_t* sides, const int32_t* normals,
                                   const double* centres);

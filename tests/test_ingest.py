@@ -82,8 +82,13 @@ def test_huygens_surface_inp_runs_to_the_golden_series(oracle_lib):
     from oracle import oracle
 
     ing = ingest.load(os.path.join(D, "huygens_surface.inp"))
-    assert ing.unsupported == ["output_surface_list", "output_volume_list"]
+    assert ing.unsupported == []
     p = ing.problem
+    # its three output surfaces (|E| on surfaces 2, 3, 4, side -1) and three
+    # output volumes (Hx, Hy, Hz of volume 1), every 4th step to 120
+    assert [(o.kind, o.component, o.side, o.first_timestep, o.last_timestep, o.timestep_interval)
+            for o in p.field_outputs] == [("surface", "em", -1, 0, 120, 4)] * 3 + \
+        [("volume", c, 1, 0, 120, 4) for c in ("hx", "hy", "hz")]
     hand = problems.huygens_surface()
     centre = [o for o in p.outputs if o.point == "centre"]
     assert [(o.cell, o.component) for o in centre] == [(o.cell, o.component) for o in hand.outputs]
@@ -91,6 +96,7 @@ def test_huygens_surface_inp_runs_to_the_golden_series(oracle_lib):
     # the same tables as the hand-built deck once the face outputs are left out
     q = ingest.load(os.path.join(D, "huygens_surface.inp")).problem
     q.outputs = centre
+    q.field_outputs = []
     assert same_tables(build_tables(hand), build_tables(q)) == []
     # all output points (face outputs make their faces special) on the oracle
     t = build_tables(p)
