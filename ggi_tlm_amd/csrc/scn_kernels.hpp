@@ -1752,6 +1752,94 @@ __device__ inline void node_update(const Dev& d, double v[12], int mnum, long lo
   scatter_ports(v, Vx, Vy, Vz, Ix, Iy, Iz);
 }
 
+// node_update (IO false) of a dispersive cell whose filters have order <= 1,
+// with the filter words in registers: e0/m0 hold each component's word 0
+// (loaded by the caller ahead of the link pull), and the new words 0 and 1
+// are stored at the end.  The operations are node_update's, zf_timeshift
+// and zf_evaluate's in the same order (the shift moves word 0 to word 1;
+// word 1 is never read before it is overwritten), so the results are the
+// same bits.
+__device__ inline void zf1_words(const double* a, const double* b, int order, double w1, double G, double& W0,
+                                 double& F0) {
+  W0 = G;
+  if (order >= 1) W0 = W0 - b[1] * w1;
+  W0 = W0 / b[0];
+  F0 = 0.0;
+  F0 = F0 + a[0] * W0;
+  if (order >= 1) F0 = F0 + a[1] * w1;
+}
+
+__device__ inline void mat_node_low(const Dev& d, double v[12], const tlm_volume_material& vm, long long f,
+                                    const double e0[3], const double m0[3]) {
+  const double Z04 = kZ0 * 4.0, Y04 = kY0 * 4.0;
+  double VE[3] = {(v[Vx_ymin] + v[Vx_ymax] + v[Vx_zmin] + v[Vx_zmax]) / 2.0,
+                  (v[Vy_xmin] + v[Vy_xmax] + v[Vy_zmin] + v[Vy_zmax]) / 2.0,
+                  (v[Vz_xmin] + v[Vz_xmax] + v[Vz_ymin] + v[Vz_ymax]) / 2.0};
+  double VH[3] = {2.0 * (v[Vz_ymax] - v[Vy_zmax] - v[Vz_ymin] + v[Vy_zmin]),
+                  2.0 * (v[Vx_zmax] - v[Vz_xmax] - v[Vx_zmin] + v[Vz_xmin]),
+                  2.0 * (v[Vy_xmax] - v[Vx_ymax] - v[Vy_xmin] + v[Vx_ymin])};
+  double Y[3] = {Y04, Y04, Y04}, Z[3] = {Z04, Z04, Z04}, Vo[3] = {0.0, 0.0, 0.0};
+  const long long nf = d.n_filt;
+  const bool fe = vm.eps_filter_exists, fm = vm.mu_filter_exists;
+  const int oe = vm.eps_order, om = vm.mu_order;
+  if (fe) {
+    const double Ys = vm.Ys_eps_f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double W0;
+      zf1_words(vm.eps_a, vm.eps_b, oe, e0[c], 0.0, W0, Vo[c]);
+      VE[c] = (VE[c] * Y[c] + Vo[c] * Ys) / (Y[c] + Ys);
+      Y[c] = Y[c] + Ys;
+    }
+  }
+  const double Ge = vm.Ge;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    VE[c] = (VE[c] * Y[c] + 0.0 * Ge) / (Y[c] + Ge);
+    Y[c] = Y[c] + Ge;
+  }
+  if (fm) {
+    const double Zs = vm.Zs_mu_f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double W0, Vs;
+      zf1_words(vm.mu_a, vm.mu_b, om, m0[c], 0.0, W0, Vs);
+      VH[c] = VH[c] + Vs;
+      Z[c] = Z[c] + Zs;
+    }
+  }
+  const double Rm = vm.Rm;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    VH[c] = VH[c] + 0.0;
+    Z[c] = Z[c] + Rm;
+  }
+  const double Vx = VE[0], Vy = VE[1], Vz = VE[2];
+  const double Ix = VH[0] / Z[0], Iy = VH[1] / Z[1], Iz = VH[2] / Z[2];
+  if (fe) {
+    const double Ysx = vm.Ys_eps_f;
+    const double G[3] = {(Vx - Vo[0]) * Ysx, (Vy - Vo[1]) * Ysx, (Vz - Vo[2]) * Ysx};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double W0, F0;
+      zf1_words(vm.eps_a, vm.eps_b, oe, e0[c], G[c], W0, F0);
+      d.eps_w[f + c] = W0;
+      if (oe >= 1) d.eps_w[nf + f + c] = e0[c];
+    }
+  }
+  if (fm) {
+    const double G[3] = {-Ix, -Iy, -Iz};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double W0, F0;
+      zf1_words(vm.mu_a, vm.mu_b, om, m0[c], G[c], W0, F0);
+      d.mu_w[f + c] = W0;
+      if (om >= 1) d.mu_w[nf + f + c] = m0[c];
+    }
+  }
+  scatter_ports(v, Vx, Vy, Vz, Ix, Iy, Iz);
+}
+
 // gen_code: the reference's update code, 0 for a free-space cell beside a
 // special face, negative for a host-path cell (left incident: the host
 // scatters it between tlm_scatter and tlm_connect).
@@ -1896,6 +1984,24 @@ __device__ inline void mat_line_cell(const Dev& d, const T* __restrict__ Lin, T*
     return;
   }
   const uint64_t lanes = (uint64_t)(uint32_t)f.y | ((uint64_t)(uint32_t)f.z << 32);
+  // a material lane's filter words (one per component and filter) are
+  // loaded here, so that they are in flight with the link pull
+  const bool matl = (lanes >> lane) & 1u;
+  const tlm_volume_material& vm = d.vmat[f.x > 0 ? f.x - 1 : 0];
+  const bool low = vm.type == TLM_VMAT_DISPERSIVE && (!vm.eps_filter_exists || vm.eps_order <= 1) &&
+                   (!vm.mu_filter_exists || vm.mu_order <= 1);
+  const long long slot = matl ? f.w + 3 * (long long)__popcll(lanes & ((1ull << lane) - 1ull)) : 0;
+  double e0[3] = {0.0, 0.0, 0.0}, m0[3] = {0.0, 0.0, 0.0};
+  if (matl && low) {
+    if (vm.eps_filter_exists) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) e0[k] = d.eps_w[slot + k];
+    }
+    if (vm.mu_filter_exists) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) m0[k] = d.mu_w[slot + k];
+    }
+  }
   double v[12];
   if (PULL) {
     bulk_pull<T, PLAIN>(d, Lin, a, c, x, y, z, sg, v);
@@ -1904,9 +2010,9 @@ __device__ inline void mat_line_cell(const Dev& d, const T* __restrict__ Lin, T*
     for (int p = 0; p < 12; p++) v[p] = (double)lk_get(Lin, d.P, p, c);
   }
   flush_small(v, a.small);
-  if ((lanes >> lane) & 1u) {
-    const long long slot = f.w + 3 * (long long)__popcll(lanes & ((1ull << lane) - 1ull));
-    node_update<false>(d, v, f.x, slot, -1, -1, 0, 0);
+  if (matl) {
+    if (low) mat_node_low(d, v, vm, slot, e0, m0);
+    else node_update<false>(d, v, f.x, slot, -1, -1, 0, 0);
   } else if (PLAIN) {
     scatter_free_plain(v);
   } else {
