@@ -590,9 +590,14 @@ __device__ inline void pull_cell(const Dev& d, const T* __restrict__ L, long lon
                                  int fdi, const PullArgs& a, double v[12]) {
   const long long P = d.P;
   const long long sy = seg_width(d, sg), sz = sy * d.ny;
-  double o[12];
+  // the cell's own values: the plain form's connect reads all twelve; the
+  // reduced forms need a side's two only on the outer boundary or at a
+  // special face (an ordinary face takes the neighbour's value as it is)
+  double o[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (PLAIN) {
 #pragma unroll
-  for (int p = 0; p < 12; p++) o[p] = ld(L, P, p, c);
+    for (int p = 0; p < 12; p++) o[p] = ld(L, P, p, c);
+  }
   int fc[6] = {0, 0, 0, 0, 0, 0};
   if (FD && fdi >= 0) {
 #pragma unroll
@@ -607,6 +612,10 @@ __device__ inline void pull_cell(const Dev& d, const T* __restrict__ L, long lon
 #pragma unroll
   for (int s = 0; s < 6; s++) {
     double oa = o[pa[s]], ob = o[pb[s]];
+    if (!PLAIN && (bnd[s] || (FD && fc[s] != 0))) {
+      oa = ld(L, P, pa[s], c);
+      ob = ld(L, P, pb[s], c);
+    }
     if (bnd[s]) {
       double R = a.apply_r ? d.R[s] : 1.0;
       v[pa[s]] = oa * R;
@@ -1844,13 +1853,16 @@ __device__ inline void mat_node_low(const Dev& d, double v[12], const tlm_volume
 // special face, negative for a host-path cell (left incident: the host
 // scatters it between tlm_scatter and tlm_connect).
 template <bool PLAIN>
-__device__ inline void generic_node(const Dev& d, double v[12], int timestep, int slot, int i) {
-  const int code = d.gen_code[i];
+__device__ inline void generic_node(const Dev& d, double v[12], int timestep, int slot, int i, int code) {
   if (code == 0) {
     if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
   } else if (code > 0) {
     node_update<true>(d, v, d.gen_mat[i], d.gen_filt[i], d.gen_exc[i], d.gen_out[i], timestep, slot);
   }
+}
+template <bool PLAIN>
+__device__ inline void generic_node(const Dev& d, double v[12], int timestep, int slot, int i) {
+  generic_node<PLAIN>(d, v, timestep, slot, i, d.gen_code[i]);
 }
 
 // Generic special cells: point sources, probes, cells touching a special
@@ -1860,6 +1872,7 @@ template <typename T, bool PLAIN, bool PULL>
 __device__ inline void generic_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                       const PullArgs& a, int timestep, int slot, int i) {
   const long long c = d.gen_cell[i];
+  const int code = d.gen_code[i];  // read with the list entry, not after the pull
   double v[12];
   if (PULL) {
     int x, y, z, sg;
@@ -1870,7 +1883,7 @@ __device__ inline void generic_update(const Dev& d, const T* __restrict__ Lin, T
     for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
   }
   flush_small(v, a.small);
-  generic_node<PLAIN>(d, v, timestep, slot, i);
+  generic_node<PLAIN>(d, v, timestep, slot, i, code);
   store_cell(Lout, d.P, c, v);
 }
 
