@@ -105,7 +105,7 @@ int64_t estimate_bytes(const tlm_handle* h, int n_steps) {
   int64_t b = 3 * 12 * ((int64_t)h->N + 64 * TLM_MAX_SEGMENTS + TLM_PLANE_PAD) * ls;  // link planes, padding
   b += (int64_t)h->N / 8 + (int64_t)h->N / 8;            // mask, line table
   const int64_t n_pml = (int64_t)h->pml_cell_param.size();
-  b += n_pml * (8 * (scn::PS_N + 3) + 12);               // PML state and lists
+  b += (n_pml + 64) * (8 * (scn::PS_N + 3) + 12);        // PML state (planes padded to 64) and lists
   b += (int64_t)h->n_special * 64;                       // class lists, slots
   b += (int64_t)h->n_special * 2 * 3 * 8 * scn::NW;      // filter states (bound)
   b += (int64_t)h->n_special_faces * 48;

@@ -72,16 +72,22 @@ enum {
   PS_Vyxt = PS_Vi + 12, PS_Vzxt, PS_Vxyt, PS_Vzyt, PS_Vyzt, PS_Vxzt,
   PS_N
 };
-// Precision form: the previous step's incident voltages Vi enter the update
-// only as the six pair sums (Vznx+Vzpx), (Vynx+Vypx), (Vxny+Vxpy),
-// (Vzny+Vzpy), (Vynz+Vypz), (Vxnz+Vxpz) (scatter_PML.F90:127-141), so those
-// are kept instead, in FP64: 18 words per cell.
+// Reduced forms (FP32 links): the previous step's incident voltages Vi enter
+// the update only as the six pair sums (Vznx+Vzpx), (Vynx+Vypx),
+// (Vxny+Vxpy), (Vzny+Vzpy), (Vynz+Vypz), (Vxnz+Vxpz)
+// (scatter_PML.F90:127-141), so a cell with a special face keeps those and
+// Ix, Iy, Iz in FP64: PC_N words per cell.
 enum {
-  PC_Vxt, PC_Vyt, PC_Vzt, PC_Ix, PC_Iy, PC_Iz,
+  PC_Ix, PC_Iy, PC_Iz,
   PC_Szx, PC_Syx, PC_Sxy, PC_Szy, PC_Syz, PC_Sxz,
-  PC_Vyxt, PC_Vzxt, PC_Vxyt, PC_Vzyt, PC_Vyzt, PC_Vxzt,
   PC_N
 };
+// The nine words every PML cell streams (Vxt, Vyt, Vzt and the six series
+// terms), in the reduced forms as 48-bit values: the double rounded to its
+// top 48 bits (36 significand bits, relative error <= 2^-37, far below the
+// FP32 links' 2^-24), stored as a u32 plane of the top 32 bits and a u16
+// plane of the next 16.  SoA: hi[w*ns + i], lo[w*ns + i], ns = n_pml_state.
+enum { SW_Vxt, SW_Vyt, SW_Vzt, SW_Vyxt, SW_Vzxt, SW_Vxyt, SW_Vzyt, SW_Vyzt, SW_Vxzt, SW_N };
 // Ixt,Iyt,Izt live in a separate array, touched only by output cells.
 
 // PML cells holding a non-dispersive material (scatter_PML_material.F90):
@@ -120,6 +126,7 @@ __device__ inline int32_t sorted_lookup(const int2* tab, int n, int32_t key) {
 }
 
 constexpr int kMaxSeg = TLM_MAX_SEGMENTS;
+constexpr int kChunk = 64;  // cells of a chunk record (one wavefront)
 
 struct Dev {
   int nx, ny, nz;
@@ -147,7 +154,9 @@ struct Dev {
   const int2* pml_fd_tab;        // (list position, face descriptor), sorted
   int n_pml_fd;
   const tlm_pml_parameter* pml_params;
-  double* pml_state;             // PS_N (plain) or PC_N (precision) words * n_pml_state
+  double* pml_state;             // PS_N (plain) or PC_N (reduced forms) words * n_pml_state
+  uint32_t* pml_hi;              // reduced forms: SW_N top-32-bit planes * n_pml_state
+  uint16_t* pml_lo;              // reduced forms: SW_N next-16-bit planes * n_pml_state
   double* pml_It;                // 3 * n_pml_state (Ixt,Iyt,Izt)
   // PML cells holding a material (storage order)
   int n_pm;
@@ -1221,15 +1230,44 @@ __device__ inline void pml_prev_incident(const Dev& d, const PullArgs& a, long l
 // The nine state words every PML cell carries (Vxt, Vyt, Vzt, Vyxt, Vzxt,
 // Vxyt, Vzyt, Vyzt, Vxzt), loaded ahead so that they are in flight with the
 // link loads.
+// A 48-bit state word (reduced forms): the top 32 bits and the next 16 of
+// the double's bit pattern; the low 16 bits read back as zero.  Word w of
+// list position i sits at plane w (a uniform base) plus i: the per-lane
+// part is a 32-bit byte offset shared by all nine words, so each access is
+// a scalar base plus one offset register rather than a 64-bit address.
+template <typename U>
+__device__ inline U* s48_at(U* plane, uint32_t byte_off) {
+  return (U*)((char*)plane + byte_off);
+}
+__device__ inline double s48_get(const Dev& d, int w, int i) {
+  const long long n = d.n_pml_state;
+  const uint64_t hi = __builtin_nontemporal_load(s48_at(d.pml_hi + w * n, (uint32_t)i * 4u));
+  const uint64_t lo = __builtin_nontemporal_load(s48_at(d.pml_lo + w * n, (uint32_t)i * 2u));
+  return __longlong_as_double((long long)((hi << 32) | (lo << 16)));
+}
+// Rounded to nearest on the magnitude (half away from zero): adding half a
+// unit of the dropped 16 bits carries into the kept bits and, at the top of
+// a binade, into the exponent, which is the correctly rounded value.
+__device__ inline void s48_put(const Dev& d, int w, int i, double x) {
+  const long long n = d.n_pml_state;
+  const uint64_t b = (uint64_t)__double_as_longlong(x) + 0x8000ull;
+  *s48_at(d.pml_hi + w * n, (uint32_t)i * 4u) = (uint32_t)(b >> 32);
+  *s48_at(d.pml_lo + w * n, (uint32_t)i * 2u) = (uint16_t)(b >> 16);
+}
+
 template <bool PLAIN>
 __device__ inline void pml_state_preload(const Dev& d, int i, double st[9]) {
   const long long n = d.n_pml_state;
-  const double* S = d.pml_state;
-  const int w[9] = {PLAIN ? PS_Vxt : PC_Vxt, PLAIN ? PS_Vyt : PC_Vyt, PLAIN ? PS_Vzt : PC_Vzt,
-                    PLAIN ? PS_Vyxt : PC_Vyxt, PLAIN ? PS_Vzxt : PC_Vzxt, PLAIN ? PS_Vxyt : PC_Vxyt,
-                    PLAIN ? PS_Vzyt : PC_Vzyt, PLAIN ? PS_Vyzt : PC_Vyzt, PLAIN ? PS_Vxzt : PC_Vxzt};
+  if (PLAIN) {
+    const double* S = d.pml_state;
+    const int w[9] = {PS_Vxt, PS_Vyt, PS_Vzt, PS_Vyxt, PS_Vzxt, PS_Vxyt, PS_Vzyt, PS_Vyzt, PS_Vxzt};
 #pragma unroll
-  for (int k = 0; k < 9; k++) st[k] = state_ld(&S[w[k] * n + i]);
+    for (int k = 0; k < 9; k++) st[k] = state_ld(&S[w[k] * n + i]);
+  } else {
+    // st[k] is word SW k
+#pragma unroll
+    for (int k = 0; k < 9; k++) st[k] = s48_get(d, k, i);
+  }
 }
 
 template <typename T, bool PLAIN>
@@ -1247,12 +1285,16 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
   double* S = d.pml_state;
   // PML port aliases (PML_module.F90:60-71), 0-based
   enum { Vxny = 0, Vxnz, Vxpy, Vxpz, Vynx, Vynz, Vypx, Vypz, Vznx, Vzny, Vzpx, Vzpy };
+  // The plain form keeps the reference's operations: I = curl / (2 Z0) and
+  // the products with Z0 in their source order.  The reduced forms (not
+  // bitwise) carry h = Z0 I = curl / 2 in place of every current, which
+  // folds the division by 2 Z0 and the multiplications by Z0 away.
+  auto cur = [&](double curl) { return PLAIN ? curl / (2.0 * Z0) : curl * 0.5; };
+  auto zl = [&](double x) { return PLAIN ? Z0 * x : x; };   // Z0 * (x)
+  auto zr = [&](double x) { return PLAIN ? x * Z0 : x; };   // (x) * Z0
+  auto zm = [&](double a) { return PLAIN ? a * Z0 : a; };   // (a * Z0) * I
   // STAGE 2 (state reads)
   constexpr int wIx = PLAIN ? PS_Ix : PC_Ix, wIy = PLAIN ? PS_Iy : PC_Iy, wIz = PLAIN ? PS_Iz : PC_Iz;
-  constexpr int wVxt = PLAIN ? PS_Vxt : PC_Vxt, wVyt = PLAIN ? PS_Vyt : PC_Vyt, wVzt = PLAIN ? PS_Vzt : PC_Vzt;
-  constexpr int wVyxt = PLAIN ? PS_Vyxt : PC_Vyxt, wVzxt = PLAIN ? PS_Vzxt : PC_Vzxt;
-  constexpr int wVxyt = PLAIN ? PS_Vxyt : PC_Vxyt, wVzyt = PLAIN ? PS_Vzyt : PC_Vzyt;
-  constexpr int wVyzt = PLAIN ? PS_Vyzt : PC_Vyzt, wVxzt = PLAIN ? PS_Vxzt : PC_Vxzt;
   // previous incident voltages: recomputed (rec) or state words
   const bool rec = (cw & kFlagFd) == 0;
   double last_Ix, last_Iy, last_Iz;
@@ -1272,9 +1314,9 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
     w[Vzny] = w[Vzny] * exp_z; w[Vzpy] = w[Vzpy] * exp_z;
     w[Vxnz] = w[Vxnz] * exp_x; w[Vxpz] = w[Vxpz] * exp_x;
     w[Vynz] = w[Vynz] * exp_y; w[Vypz] = w[Vypz] * exp_y;
-    last_Ix = (w[Vypz] - w[Vzpy] - w[Vynz] + w[Vzny]) / (2.0 * Z0);
-    last_Iy = (w[Vzpx] - w[Vxpz] - w[Vznx] + w[Vxnz]) / (2.0 * Z0);
-    last_Iz = (w[Vxpy] - w[Vypx] - w[Vxny] + w[Vynx]) / (2.0 * Z0);
+    last_Ix = cur(w[Vypz] - w[Vzpy] - w[Vynz] + w[Vzny]);
+    last_Iy = cur(w[Vzpx] - w[Vxpz] - w[Vznx] + w[Vxnz]);
+    last_Iz = cur(w[Vxpy] - w[Vypx] - w[Vxny] + w[Vynx]);
 #pragma unroll
     for (int k = 0; k < 12; k++) lv[k] = w[k];
     Szx = w[Vznx] + w[Vzpx]; Syx = w[Vynx] + w[Vypx];
@@ -1315,9 +1357,9 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
   const double Vx = (v[Vynx] + v[Vypx] + v[Vznx] + v[Vzpx]) / 2.0;
   const double Vy = (v[Vxny] + v[Vxpy] + v[Vzny] + v[Vzpy]) / 2.0;
   const double Vz = (v[Vxnz] + v[Vxpz] + v[Vynz] + v[Vypz]) / 2.0;
-  const double Ix = (v[Vypz] - v[Vzpy] - v[Vynz] + v[Vzny]) / (2.0 * Z0);
-  const double Iy = (v[Vzpx] - v[Vxpz] - v[Vznx] + v[Vxnz]) / (2.0 * Z0);
-  const double Iz = (v[Vxpy] - v[Vypx] - v[Vxny] + v[Vynx]) / (2.0 * Z0);
+  const double Ix = cur(v[Vypz] - v[Vzpy] - v[Vynz] + v[Vzny]);
+  const double Iy = cur(v[Vzpx] - v[Vxpz] - v[Vznx] + v[Vxnz]);
+  const double Iz = cur(v[Vxpy] - v[Vypx] - v[Vxny] + v[Vynx]);
   // STAGE 5
   double Vx_sh, Vy_sh, Vz_sh;
   if (PLAIN) {
@@ -1350,24 +1392,28 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
   double Ixt = 0.0, Iyt = 0.0, Izt = 0.0;
   if (orow >= 0) {
     const double lIxt = d.pml_It[0 * n + si], lIyt = d.pml_It[1 * n + si], lIzt = d.pml_It[2 * n + si];
-    Ixt = ax * (Ix - last_Ix + Csx2 * (Ix + last_Ix) + Csy_sz * lIxt);
-    Iyt = ay * (Iy - last_Iy + Csy2 * (Iy + last_Iy) + Csz_sx * lIyt);
-    Izt = az * (Iz - last_Iz + Csz2 * (Iz + last_Iz) + Csx_sy * lIzt);
+    // currents proper (the reduced forms carry Z0 * I)
+    const double ix = PLAIN ? Ix : Ix / Z0, iy = PLAIN ? Iy : Iy / Z0, iz = PLAIN ? Iz : Iz / Z0;
+    const double lix = PLAIN ? last_Ix : last_Ix / Z0, liy = PLAIN ? last_Iy : last_Iy / Z0;
+    const double liz = PLAIN ? last_Iz : last_Iz / Z0;
+    Ixt = ax * (ix - lix + Csx2 * (ix + lix) + Csy_sz * lIxt);
+    Iyt = ay * (iy - liy + Csy2 * (iy + liy) + Csz_sx * lIyt);
+    Izt = az * (iz - liz + Csz2 * (iz + liz) + Csx_sy * lIzt);
   }
   // STAGE 7
-  const double Vxy_se = az * (Z0 * (-last_Iz + Csy2 * (Iz + last_Iz)) + Csx_sy * last_Vxyt);
-  const double Vyx_se = az * (Z0 * (-last_Iz + Csx2 * (Iz + last_Iz)) + Csx_sy * last_Vyxt);
-  const double Vzx_se = ay * (Z0 * (-last_Iy + Csx2 * (Iy + last_Iy)) + Csz_sx * last_Vzxt);
-  const double Vxz_se = ay * (Z0 * (-last_Iy + Csz2 * (Iy + last_Iy)) + Csz_sx * last_Vxzt);
-  const double Vyz_se = ax * (Z0 * (-last_Ix + Csz2 * (Ix + last_Ix)) + Csy_sz * last_Vyzt);
-  const double Vzy_se = ax * (Z0 * (-last_Ix + Csy2 * (Ix + last_Ix)) + Csy_sz * last_Vzyt);
+  const double Vxy_se = az * (zl(-last_Iz + Csy2 * (Iz + last_Iz)) + Csx_sy * last_Vxyt);
+  const double Vyx_se = az * (zl(-last_Iz + Csx2 * (Iz + last_Iz)) + Csx_sy * last_Vyxt);
+  const double Vzx_se = ay * (zl(-last_Iy + Csx2 * (Iy + last_Iy)) + Csz_sx * last_Vzxt);
+  const double Vxz_se = ay * (zl(-last_Iy + Csz2 * (Iy + last_Iy)) + Csz_sx * last_Vxzt);
+  const double Vyz_se = ax * (zl(-last_Ix + Csz2 * (Ix + last_Ix)) + Csy_sz * last_Vyzt);
+  const double Vzy_se = ax * (zl(-last_Ix + Csy2 * (Ix + last_Ix)) + Csy_sz * last_Vzyt);
   // STAGE 8
-  const double Vxyt = az * Iz * Z0 + Vxy_se;
-  const double Vyxt = az * Iz * Z0 + Vyx_se;
-  const double Vzxt = ay * Iy * Z0 + Vzx_se;
-  const double Vxzt = ay * Iy * Z0 + Vxz_se;
-  const double Vyzt = ax * Ix * Z0 + Vyz_se;
-  const double Vzyt = ax * Ix * Z0 + Vzy_se;
+  const double Vxyt = zr(az * Iz) + Vxy_se;
+  const double Vyxt = zr(az * Iz) + Vyx_se;
+  const double Vzxt = zr(ay * Iy) + Vzx_se;
+  const double Vxzt = zr(ay * Iy) + Vxz_se;
+  const double Vyzt = zr(ax * Ix) + Vyz_se;
+  const double Vzyt = zr(ax * Ix) + Vzy_se;
   // STAGE 9
   if (!rec) {
     S[wIx * n + si] = Ix; S[wIy * n + si] = Iy; S[wIz * n + si] = Iz;
@@ -1380,13 +1426,23 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
       S[PC_Syz * n + si] = v[Vynz] + v[Vypz]; S[PC_Sxz * n + si] = v[Vxnz] + v[Vxpz];
     }
   }
-  S[wVxt * n + si] = Vxt; S[wVyt * n + si] = Vyt; S[wVzt * n + si] = Vzt;
+  if (PLAIN) {
+    S[PS_Vxt * n + si] = Vxt; S[PS_Vyt * n + si] = Vyt; S[PS_Vzt * n + si] = Vzt;
+  } else {
+    s48_put(d, SW_Vxt, i, Vxt); s48_put(d, SW_Vyt, i, Vyt); s48_put(d, SW_Vzt, i, Vzt);
+  }
   if (orow >= 0) {
     d.pml_It[0 * n + si] = Ixt; d.pml_It[1 * n + si] = Iyt; d.pml_It[2 * n + si] = Izt;
   }
-  S[wVxyt * n + si] = Vxyt; S[wVyxt * n + si] = Vyxt;
-  S[wVzxt * n + si] = Vzxt; S[wVxzt * n + si] = Vxzt;
-  S[wVzyt * n + si] = Vzyt; S[wVyzt * n + si] = Vyzt;
+  if (PLAIN) {
+    S[PS_Vxyt * n + si] = Vxyt; S[PS_Vyxt * n + si] = Vyxt;
+    S[PS_Vzxt * n + si] = Vzxt; S[PS_Vxzt * n + si] = Vxzt;
+    S[PS_Vzyt * n + si] = Vzyt; S[PS_Vyzt * n + si] = Vyzt;
+  } else {
+    s48_put(d, SW_Vxyt, i, Vxyt); s48_put(d, SW_Vyxt, i, Vyxt);
+    s48_put(d, SW_Vzxt, i, Vzxt); s48_put(d, SW_Vxzt, i, Vxzt);
+    s48_put(d, SW_Vzyt, i, Vzyt); s48_put(d, SW_Vyzt, i, Vyzt);
+  }
   // STAGE 10
   const double Vynx_P = Vx_sh - Vyx_se, Vypx_P = Vx_sh + Vyx_se;
   const double Vznx_P = Vx_sh + Vzx_se, Vzpx_P = Vx_sh - Vzx_se;
@@ -1397,23 +1453,23 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
   // STAGE 11
   double M, m;
   M = v[Vypx]; m = v[Vynx];
-  v[Vynx] = ax * Vx - az * Z0 * Iz - M + Vynx_P;
-  v[Vypx] = ax * Vx + az * Z0 * Iz - m + Vypx_P;
+  v[Vynx] = ax * Vx - zm(az) * Iz - M + Vynx_P;
+  v[Vypx] = ax * Vx + zm(az) * Iz - m + Vypx_P;
   M = v[Vzpx]; m = v[Vznx];
-  v[Vznx] = ax * Vx + ay * Z0 * Iy - M + Vznx_P;
-  v[Vzpx] = ax * Vx - ay * Z0 * Iy - m + Vzpx_P;
+  v[Vznx] = ax * Vx + zm(ay) * Iy - M + Vznx_P;
+  v[Vzpx] = ax * Vx - zm(ay) * Iy - m + Vzpx_P;
   M = v[Vzpy]; m = v[Vzny];
-  v[Vzny] = ay * Vy - ax * Z0 * Ix - M + Vzny_P;
-  v[Vzpy] = ay * Vy + ax * Z0 * Ix - m + Vzpy_P;
+  v[Vzny] = ay * Vy - zm(ax) * Ix - M + Vzny_P;
+  v[Vzpy] = ay * Vy + zm(ax) * Ix - m + Vzpy_P;
   M = v[Vxpy]; m = v[Vxny];
-  v[Vxny] = ay * Vy + az * Z0 * Iz - M + Vxny_P;
-  v[Vxpy] = ay * Vy - az * Z0 * Iz - m + Vxpy_P;
+  v[Vxny] = ay * Vy + zm(az) * Iz - M + Vxny_P;
+  v[Vxpy] = ay * Vy - zm(az) * Iz - m + Vxpy_P;
   M = v[Vxpz]; m = v[Vxnz];
-  v[Vxnz] = az * Vz - ay * Z0 * Iy - M + Vxnz_P;
-  v[Vxpz] = az * Vz + ay * Z0 * Iy - m + Vxpz_P;
+  v[Vxnz] = az * Vz - zm(ay) * Iy - M + Vxnz_P;
+  v[Vxpz] = az * Vz + zm(ay) * Iy - m + Vxpz_P;
   M = v[Vypz]; m = v[Vynz];
-  v[Vynz] = az * Vz + ax * Z0 * Ix - M + Vynz_P;
-  v[Vypz] = az * Vz - ax * Z0 * Ix - m + Vypz_P;
+  v[Vynz] = az * Vz + zm(ax) * Ix - M + Vynz_P;
+  v[Vypz] = az * Vz - zm(ax) * Ix - m + Vypz_P;
   // STAGE 12
   if (orow >= 0) {
     double* fo = d.ring + ((long long)slot * d.n_out_rows + orow) * 6;
@@ -1427,6 +1483,63 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
   v[Vzny] = v[Vzny] * exp_z; v[Vzpy] = v[Vzpy] * exp_z;
   v[Vxnz] = v[Vxnz] * exp_x; v[Vxpz] = v[Vxpz] * exp_x;
   v[Vynz] = v[Vynz] * exp_y; v[Vypz] = v[Vypz] * exp_y;
+}
+
+// PML cells of pure PML lines from per-chunk records (two int4 each, read
+// by all lanes at one address):
+//   e = (chunk << 4 | mask of the chunk's pure PML lines,
+//        x of the chunk's first cell | segment << 16, y, z)
+//   f = (list position of the first of those cells, 1 if any of them has an
+//        output row or a special face, 0, 0)
+// The pure-line cells of the PML list are in storage order, so a lane's
+// list position follows from the lines of the mask below its own.  The
+// common step (pull, previous buffer resident, no flagged cell) takes its
+// cell and coordinates from the record: no list word and no division for a
+// segment at least 64 cells wide; every other case runs pml_update on the
+// lane's list position.  Padding lanes (after a segment's last row, in the
+// last line of a segment) have no list position: they leave as soon as
+// their coordinates show it, before any other access.
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(kChunk) void k_pml_chunks(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                       PullArgs a, int slot, const int4* __restrict__ rec) {
+  const int4 e = rec[2 * blockIdx.x], f = rec[2 * blockIdx.x + 1];
+  const int lane = threadIdx.x;
+  const int L = d.line_len;
+  const int q = lane / L;
+  if (!((e.x >> q) & 1)) return;
+  const int i = f.x + L * __popc((unsigned)(e.x & ((1 << q) - 1))) + (lane - q * L);
+  // coordinates first: a lane on the padding after a segment's last row
+  // has no list position and leaves before any access
+  const long long c = (long long)(e.x >> 4) * kChunk + lane;
+  int sg = e.y >> 16, x = (e.y & 0xffff) + lane, y = e.z, z = e.w;
+  const int w = seg_width(d, sg);
+  if (w >= kChunk) {
+    if (x >= d.seg_x0[sg + 1]) {
+      x -= w;
+      if (++y == d.ny) { y = 0; z++; }
+    }
+    if (z >= d.nz) return;  // padding after the segment's last row
+  } else {
+    // narrow segment (a PML slab a few cells thick): rows of w cells
+    const unsigned r = (unsigned)(x - d.seg_x0[sg]), row = r / (unsigned)w;
+    x = d.seg_x0[sg] + (int)(r - row * (unsigned)w);
+    const unsigned yy = (unsigned)y + row, dz = yy / (unsigned)d.ny;
+    y = (int)(yy - dz * (unsigned)d.ny);
+    z += (int)dz;
+    if (z >= d.nz) return;
+  }
+  if (!PULL || f.y || a.prev_mode != 1) {
+    pml_update<T, PLAIN, PULL>(d, Lin, Lout, a, slot, i);
+    return;
+  }
+  double st[9];
+  pml_state_preload<PLAIN>(d, i, st);
+  double v[12], wv[12];
+  bulk_pull_pair<T, PLAIN>(d, Lin, a, c, x, y, z, sg, v, wv);
+  flush_small(v, a.small);
+  flush_small(wv, a.prev_small);
+  pml_node<T, PLAIN>(d, v, slot, i, (int32_t)c, a, wv, st);
+  store_cell(Lout, d.P, c, v);
 }
 
 template <typename T, bool PLAIN, bool PULL>
@@ -1945,7 +2058,6 @@ __device__ inline void bulk_line_cell(const Dev& d, const T* __restrict__ Lin, T
 // precedes the link loads.  (Reading each line's class from the line table
 // before the loads measured 7% slower: 0.240 against 0.224 ms on the 256^3
 // workload.)  mode 1: each XCD takes a contiguous range of table records.
-constexpr int kChunk = 64;
 template <typename T, bool PLAIN, bool PULL>
 __global__ __launch_bounds__(kChunk) void k_bulk_chunks(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                                         PullArgs a, const int4* __restrict__ tab, int mode) {

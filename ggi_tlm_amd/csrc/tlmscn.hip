@@ -691,6 +691,48 @@ int finalize(tlm_handle* h) {
     if (k_max_chunks_exceeded(S)) return fail(h, TLM_ERR_UNSUPPORTED, "storage too large for the chunk table");
     h->n_bulk_chunks = (int)ch.size();
     if ((rc = dev_upload(h, ch, &h->d_bulk_chunks))) return rc;
+    // k_pml_chunks records (two int4 each): per 64-cell chunk holding a pure
+    // PML line, its pure PML lines, the chunk's first-cell coordinates, the
+    // list position of the first of their cells and whether any of them has
+    // an output row or a special face.  The pure part of the PML list is in
+    // storage order, so the cells of a chunk's lines hold consecutive
+    // positions.
+    std::vector<int32_t> line_pos(n_lines, -1);
+    std::vector<uint8_t> line_flag(n_lines, 0);
+    for (int i = 0; i < n_pure_pml; i++) {
+      const long long ln = pr[i].c / L;
+      if (line_pos[ln] < 0) line_pos[ln] = i;
+      if (pr[i].orow >= 0 || pr[i].fdc >= 0) line_flag[ln] = 1;
+    }
+    std::vector<int4> pc;
+    long long covered = 0;
+    for (long long k = 0; k < (S + scn::kChunk - 1) / scn::kChunk; k++) {
+      int m = 0, first = -1, flag = 0, expect = -1;
+      for (int q = 0; q < lpc && k * lpc + q < n_lines; q++) {
+        const long long ln = k * lpc + q;
+        if (line_pos[ln] < 0) continue;
+        if (first < 0) first = expect = line_pos[ln];
+        if (line_pos[ln] != expect) return fail(h, TLM_ERR_ARG, "PML list positions of a chunk are not consecutive");
+        long long nc = 0;
+        for (long long c = ln * L; c < (ln + 1) * L; c++) nc += scls[c] != 5;
+        expect += (int)nc;
+        covered += nc;
+        m |= 1 << q;
+        flag |= line_flag[ln];
+      }
+      if (!m) continue;
+      const long long c0 = k * scn::kChunk;
+      int sg = 0;
+      while (sg + 1 < nseg && c0 >= seg_base[sg + 1]) sg++;
+      const long long w = seg_x0[sg + 1] - seg_x0[sg];
+      const long long r = c0 - seg_base[sg], row = r / w;
+      pc.push_back(make_int4((int)(k << 4) | m, (int)(seg_x0[sg] + (r - row * w)) | (sg << 16), (int)(row % ny),
+                             (int)(row / ny)));
+      pc.push_back(make_int4(first, flag, 0, 0));
+    }
+    if (covered != n_pure_pml) return fail(h, TLM_ERR_ARG, "PML chunk records do not cover the pure PML lines");
+    h->n_pml_chunks = (int)(pc.size() / 2);
+    if ((rc = dev_upload(h, pc, &h->d_pml_chunks))) return rc;
     // k_mat_lines records (two int4 each): per 64-cell chunk and material
     // number, its material lines, the chunk's first-cell coordinates, the
     // material, the lanes that are material cells and their first filter
@@ -800,13 +842,24 @@ int finalize(tlm_handle* h) {
     h->n_mixed_blocks = (int)(slot_a.size() / 256);
   }
   if ((rc = dev_upload(h, h->pml_par, &d.pml_params))) return rc;
-  d.n_pml_state = std::max(d.n_pml, 1);
+  // state planes start on 128-byte lines: a multiple of 64 list positions
+  d.n_pml_state = (std::max(d.n_pml, 1) + 63) / 64 * 64;
   {
     void* p;
-    const size_t words = h->form == TLM_FORM_PLAIN_FP64 ? scn::PS_N : scn::PC_N;
+    const bool plain = h->form == TLM_FORM_PLAIN_FP64;
+    const size_t words = plain ? scn::PS_N : scn::PC_N;
     if ((rc = dev_alloc(h, words * d.n_pml_state * sizeof(double), &p))) return rc;
     HIP_TRY(h, hipMemsetAsync(p, 0, words * d.n_pml_state * sizeof(double), h->stream));
     d.pml_state = (double*)p;
+    d.pml_hi = nullptr;
+    d.pml_lo = nullptr;
+    if (!plain) {
+      const size_t nw = (size_t)scn::SW_N * d.n_pml_state;
+      if ((rc = dev_alloc(h, nw * 6, &p))) return rc;
+      HIP_TRY(h, hipMemsetAsync(p, 0, nw * 6, h->stream));
+      d.pml_hi = (uint32_t*)p;
+      d.pml_lo = (uint16_t*)((char*)p + nw * 4);
+    }
     if ((rc = dev_alloc(h, (size_t)3 * d.n_pml_state * sizeof(double), &p))) return rc;
     HIP_TRY(h, hipMemsetAsync(p, 0, (size_t)3 * d.n_pml_state * sizeof(double), h->stream));
     d.pml_It = (double*)p;
@@ -1390,11 +1443,15 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
     if (pull) hipLaunchKernelGGL((scn::k_pml_material<T, PLAIN, true>), g, dim3(256), 0, cs, d, Lin, Lout, a, slot);
     else hipLaunchKernelGGL((scn::k_pml_material<T, PLAIN, false>), g, dim3(256), 0, cs, d, Lin, Lout, a, slot);
   }
-  if (h->n_pure_pml) {
+  if (h->n_pml_chunks) {
     ProfScope ps(h, TLM_KCLASS_PML);
-    dim3 g((h->n_pure_pml + 255) / 256);
-    if (pull) hipLaunchKernelGGL((scn::k_pml<T, PLAIN, true>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot, h->n_pure_pml);
-    else hipLaunchKernelGGL((scn::k_pml<T, PLAIN, false>), g, dim3(256), 0, h->stream, d, Lin, Lout, a, slot, h->n_pure_pml);
+    dim3 g(h->n_pml_chunks);
+    if (pull)
+      hipLaunchKernelGGL((scn::k_pml_chunks<T, PLAIN, true>), g, dim3(scn::kChunk), 0, h->stream, d, Lin, Lout, a, slot,
+                         h->d_pml_chunks);
+    else
+      hipLaunchKernelGGL((scn::k_pml_chunks<T, PLAIN, false>), g, dim3(scn::kChunk), 0, h->stream, d, Lin, Lout, a,
+                         slot, h->d_pml_chunks);
   }
   if (fork) {
     HIP_TRY(h, hipEventRecord(h->ev_join, h->side));
@@ -2373,12 +2430,16 @@ int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
   // and written) for eps and mu, FP64
   auto filter_bytes = [&](int m) -> int64_t { return filter_bytes_of(h, m); };
   // PML: links, the previous step's incident voltages pulled again (12
-  // links), the nine FP64 state words read and written, packed cell word and
-  // 16-bit parameter row; a cell with a special face reads and writes all
-  // its state words (24 plain, 18 precision) instead of the second pull.
-  const int64_t pml_words = h->form == TLM_FORM_PLAIN_FP64 ? scn::PS_N : scn::PC_N;
-  const int64_t pml_rec_bytes = 36 * sl + 2 * 8 * 9 + 4 + 2;
-  const int64_t pml_fd_bytes = 24 * sl + 2 * 8 * pml_words + 4 + 2;
+  // links), the nine streamed state words read and written (FP64 plain,
+  // 48-bit in the reduced forms), packed cell word and 16-bit parameter row;
+  // a cell with a special face reads and writes all its state words (24
+  // FP64 plain; 9 FP64 and the nine streamed ones in the reduced forms)
+  // instead of the second pull.
+  const bool plain = h->form == TLM_FORM_PLAIN_FP64;
+  const int64_t sw_bytes = plain ? 8 : 6;  // a streamed word: FP64, or 48 bits in the reduced forms
+  const int64_t pml_rec_bytes = 36 * sl + 2 * sw_bytes * scn::SW_N + 4 + 2;
+  const int64_t pml_fd_bytes =
+      24 * sl + 2 * (plain ? 8 * scn::PS_N : 8 * scn::PC_N + sw_bytes * scn::SW_N) + 4 + 2;
   auto pml_bytes = [&](int64_t n, int64_t n_fd) { return (n - n_fd) * pml_rec_bytes + n_fd * pml_fd_bytes; };
   // generic: links, 7 int32 list fields, filter words
   std::vector<int32_t> gm(h->n_gen);
@@ -2413,8 +2474,11 @@ int tlm_kernel_bytes(tlm_handle* h, int64_t* cells, int64_t* bytes) {
                             pml_bytes(h->n_pml - h->n_pure_pml, h->n_pml_fd_all - h->n_pml_fd_pure) +
                             mat_bytes(h->n_pure_mat, h->n_mat) + gen_bytes(h->n_pure_gen, h->n_gen) +
                             (int64_t)h->n_mixed_blocks * 256 * 8;
+  // pure PML lines run from chunk records: no cell word, a 32-byte record
+  // per chunk
   cells[TLM_KCLASS_PML] = h->n_pure_pml;
-  bytes[TLM_KCLASS_PML] = pml_bytes(h->n_pure_pml, h->n_pml_fd_pure);
+  bytes[TLM_KCLASS_PML] = pml_bytes(h->n_pure_pml, h->n_pml_fd_pure) - 4 * (int64_t)h->n_pure_pml +
+                          32 * (int64_t)h->n_pml_chunks;
   cells[TLM_KCLASS_GENERIC] = h->n_pure_gen;
   bytes[TLM_KCLASS_GENERIC] = gen_bytes(0, h->n_pure_gen);
   // material class: pure material lines from the list, material lines from
