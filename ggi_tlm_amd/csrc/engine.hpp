@@ -103,8 +103,8 @@ struct tlm_handle {
   int64_t matline_filter_bytes = 0;
   const int4* d_bulk_chunks = nullptr;   // k_bulk_chunks records
   int n_bulk_chunks = 0;
-  const int4* d_pml_chunks = nullptr;    // k_pml_chunks records (two int4 each)
-  int n_pml_chunks = 0;
+  const int4* d_pml_chunks = nullptr;    // k_pml_chunks records (two int4 each): unflagged chunks, then flagged
+  int n_pml_chunks = 0, n_pml_chunks_fast = 0;
   const int32_t* d_bulk_blocks = nullptr;
   const int32_t* d_bulk_blocks_yband = nullptr;
   int n_bulk_blocks_yband = 0;

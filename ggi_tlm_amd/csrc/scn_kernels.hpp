@@ -399,6 +399,21 @@ __device__ inline double lk_get(const xlink* __restrict__ L, long long P, int p,
   return (double)h + (double)r * xl_unit(h);
 }
 
+// Link value of port p of cell c0 + k, with c0 the same in every lane of the
+// wavefront and k the lane's own offset (0 <= k < 2^28): the address is a
+// scalar base plus one 32-bit offset register that every such access of the
+// lane shares, instead of a 64-bit address per access.
+template <typename T>
+__device__ inline typename LinkReg<T>::type lk_get_u(const T* __restrict__ L, long long P, int p, long long c0,
+                                                     uint32_t k) {
+  return *(const T*)((const char*)(L + ((long long)p * P + c0)) + k * (uint32_t)sizeof(T));
+}
+__device__ inline double lk_get_u(const xlink* __restrict__ L, long long P, int p, long long c0, uint32_t k) {
+  const float h = *(const float*)((const char*)(reinterpret_cast<const float*>(L) + ((long long)p * P + c0)) + k * 4u);
+  const int8_t r = *(const int8_t*)(reinterpret_cast<const int8_t*>(L + 12 * P) + xl_byte(P, p, c0) + k * 4u);
+  return (double)h + (double)r * xl_unit(h);
+}
+
 // Stores of scattered values (non-temporal: read again only next step) and
 // plain stores (uploads).
 template <typename T>
@@ -444,6 +459,30 @@ __device__ inline void store_links(xlink* __restrict__ L, long long P, long long
   uint32_t* R = reinterpret_cast<uint32_t*>(L + 12 * P);
 #pragma unroll
   for (int k = 0; k < 3; k++) __builtin_nontemporal_store(w[k], R + (long long)k * P + c);
+}
+
+// store_links for cell c0 + k (c0 the same in every lane, as lk_get_u).
+template <typename T>
+__device__ inline void store_links_u(T* __restrict__ L, long long P, long long c0, uint32_t k, const double v[12]) {
+#pragma unroll
+  for (int p = 0; p < 12; p++)
+    __builtin_nontemporal_store((T)v[p], (T*)((char*)(L + ((long long)p * P + c0)) + k * (uint32_t)sizeof(T)));
+}
+__device__ inline void store_links_u(xlink* __restrict__ L, long long P, long long c0, uint32_t k,
+                                     const double v[12]) {
+  uint32_t w[3] = {0u, 0u, 0u};
+#pragma unroll
+  for (int p = 0; p < 12; p++) {
+    float h;
+    int8_t r;
+    xl_split(v[p], h, r);
+    __builtin_nontemporal_store(h, (float*)((char*)(reinterpret_cast<float*>(L) + ((long long)p * P + c0)) + k * 4u));
+    w[p >> 2] |= (uint32_t)(uint8_t)r << (8 * (p & 3));
+  }
+  uint32_t* R = reinterpret_cast<uint32_t*>(L + 12 * P);
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+    __builtin_nontemporal_store(w[j], (uint32_t*)((char*)(R + ((long long)j * P + c0)) + k * 4u));
 }
 
 template <typename T>
@@ -920,6 +959,93 @@ __device__ inline void bulk_pull_pair(const Dev& d, const T* __restrict__ L, con
   double od[12], qd[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) { od[k] = (double)o[k]; qd[k] = (double)q[k]; }
+  bulk_incident<PLAIN>(d, a, x, y, z, od, m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], m[9], m[10], m[11], v);
+  PullArgs b{a.prev_apply_r, 0, 0.0, -1};
+  bulk_incident<PLAIN>(d, b, x, y, z, qd, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10], r[11], w);
+}
+
+// bulk_pull_pair for lane k of a chunk whose first cell c0 is the same in
+// every lane (cell c = c0 + k): the own, y-, z- and in-segment x-neighbour
+// values are read at a scalar base plus the lane's one offset register;
+// only a lane at a segment's first or last column computes its x-neighbour
+// index (nb_xm, nb_xp).  Same loads, same values as bulk_pull_pair.
+template <typename T, bool PLAIN>
+__device__ inline void chunk_pull_pair(const Dev& d, const T* __restrict__ L, const PullArgs& a, long long c0,
+                                       uint32_t k, int x, int y, int z, int sg, double v[12], double w[12]) {
+  const T* __restrict__ Q = (const T*)a.prev;
+  const long long P = d.P;
+  const long long c = c0 + k;
+  const long long sy = seg_width(d, sg), sz = sy * d.ny;
+  typedef typename LinkReg<T>::type R;
+  R o[12], q[12];
+  if (PLAIN) {
+#pragma unroll
+    for (int p = 0; p < 12; p++) { o[p] = lk_get_u(L, P, p, c0, k); q[p] = lk_get_u(Q, P, p, c0, k); }
+  } else {
+#pragma unroll
+    for (int p = 0; p < 12; p++) o[p] = q[p] = (R)0;
+    const int xp[4] = {Vy_xmin, Vz_xmin, Vy_xmax, Vz_xmax};
+#pragma unroll
+    for (int j = 0; j < 4; j++) { o[xp[j]] = lk_get_u(L, P, xp[j], c0, k); q[xp[j]] = lk_get_u(Q, P, xp[j], c0, k); }
+    if (y == 0) {
+      o[Vx_ymin] = lk_get_u(L, P, Vx_ymin, c0, k); o[Vz_ymin] = lk_get_u(L, P, Vz_ymin, c0, k);
+      q[Vx_ymin] = lk_get_u(Q, P, Vx_ymin, c0, k); q[Vz_ymin] = lk_get_u(Q, P, Vz_ymin, c0, k);
+    }
+    if (y == d.ny - 1) {
+      o[Vx_ymax] = lk_get_u(L, P, Vx_ymax, c0, k); o[Vz_ymax] = lk_get_u(L, P, Vz_ymax, c0, k);
+      q[Vx_ymax] = lk_get_u(Q, P, Vx_ymax, c0, k); q[Vz_ymax] = lk_get_u(Q, P, Vz_ymax, c0, k);
+    }
+    if (z == 0) {
+      o[Vx_zmin] = lk_get_u(L, P, Vx_zmin, c0, k); o[Vy_zmin] = lk_get_u(L, P, Vy_zmin, c0, k);
+      q[Vx_zmin] = lk_get_u(Q, P, Vx_zmin, c0, k); q[Vy_zmin] = lk_get_u(Q, P, Vy_zmin, c0, k);
+    }
+    if (z == d.nz - 1) {
+      o[Vx_zmax] = lk_get_u(L, P, Vx_zmax, c0, k); o[Vy_zmax] = lk_get_u(L, P, Vy_zmax, c0, k);
+      q[Vx_zmax] = lk_get_u(Q, P, Vx_zmax, c0, k); q[Vy_zmax] = lk_get_u(Q, P, Vy_zmax, c0, k);
+    }
+  }
+  R m[12], r[12];
+#pragma unroll
+  for (int j = 0; j < 12; j++) m[j] = r[j] = (R)0;
+  if (x > 0) {
+    if (x > d.seg_x0[sg]) {
+      m[0] = lk_get_u(L, P, Vy_xmax, c0 - 1, k); m[1] = lk_get_u(L, P, Vz_xmax, c0 - 1, k);
+      r[0] = lk_get_u(Q, P, Vy_xmax, c0 - 1, k); r[1] = lk_get_u(Q, P, Vz_xmax, c0 - 1, k);
+    } else {
+      const long long n = nb_xm(d, c, x, y, z, sg);
+      m[0] = lk_get(L, P, Vy_xmax, n); m[1] = lk_get(L, P, Vz_xmax, n);
+      r[0] = lk_get(Q, P, Vy_xmax, n); r[1] = lk_get(Q, P, Vz_xmax, n);
+    }
+  }
+  if (x + 1 < d.nx) {
+    if (x + 1 < d.seg_x0[sg + 1]) {
+      m[2] = lk_get_u(L, P, Vy_xmin, c0 + 1, k); m[3] = lk_get_u(L, P, Vz_xmin, c0 + 1, k);
+      r[2] = lk_get_u(Q, P, Vy_xmin, c0 + 1, k); r[3] = lk_get_u(Q, P, Vz_xmin, c0 + 1, k);
+    } else {
+      const long long n = nb_xp(d, c, x, y, z, sg);
+      m[2] = lk_get(L, P, Vy_xmin, n); m[3] = lk_get(L, P, Vz_xmin, n);
+      r[2] = lk_get(Q, P, Vy_xmin, n); r[3] = lk_get(Q, P, Vz_xmin, n);
+    }
+  }
+  if (y > 0) {
+    m[4] = lk_get_u(L, P, Vx_ymax, c0 - sy, k); m[5] = lk_get_u(L, P, Vz_ymax, c0 - sy, k);
+    r[4] = lk_get_u(Q, P, Vx_ymax, c0 - sy, k); r[5] = lk_get_u(Q, P, Vz_ymax, c0 - sy, k);
+  }
+  if (y + 1 < d.ny) {
+    m[6] = lk_get_u(L, P, Vx_ymin, c0 + sy, k); m[7] = lk_get_u(L, P, Vz_ymin, c0 + sy, k);
+    r[6] = lk_get_u(Q, P, Vx_ymin, c0 + sy, k); r[7] = lk_get_u(Q, P, Vz_ymin, c0 + sy, k);
+  }
+  if (z > 0) {
+    m[8] = lk_get_u(L, P, Vx_zmax, c0 - sz, k); m[9] = lk_get_u(L, P, Vy_zmax, c0 - sz, k);
+    r[8] = lk_get_u(Q, P, Vx_zmax, c0 - sz, k); r[9] = lk_get_u(Q, P, Vy_zmax, c0 - sz, k);
+  }
+  if (z + 1 < d.nz) {
+    m[10] = lk_get_u(L, P, Vx_zmin, c0 + sz, k); m[11] = lk_get_u(L, P, Vy_zmin, c0 + sz, k);
+    r[10] = lk_get_u(Q, P, Vx_zmin, c0 + sz, k); r[11] = lk_get_u(Q, P, Vy_zmin, c0 + sz, k);
+  }
+  double od[12], qd[12];
+#pragma unroll
+  for (int j = 0; j < 12; j++) { od[j] = (double)o[j]; qd[j] = (double)q[j]; }
   bulk_incident<PLAIN>(d, a, x, y, z, od, m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], m[9], m[10], m[11], v);
   PullArgs b{a.prev_apply_r, 0, 0.0, -1};
   bulk_incident<PLAIN>(d, b, x, y, z, qd, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10], r[11], w);
@@ -1492,14 +1618,17 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
 //   f = (list position of the first of those cells, 1 if any of them has an
 //        output row or a special face, 0, 0)
 // The pure-line cells of the PML list are in storage order, so a lane's
-// list position follows from the lines of the mask below its own.  The
-// common step (pull, previous buffer resident, no flagged cell) takes its
-// cell and coordinates from the record: no list word and no division for a
-// segment at least 64 cells wide; every other case runs pml_update on the
-// lane's list position.  Padding lanes (after a segment's last row, in the
-// last line of a segment) have no list position: they leave as soon as
-// their coordinates show it, before any other access.
-template <typename T, bool PLAIN, bool PULL>
+// list position follows from the lines of the mask below its own.  Records
+// come in two lists: chunks without flagged cells, and chunks with one.
+// FAST = true runs the common step (pull, previous buffer resident) on the
+// first list: cell and coordinates from the record, no list word, no
+// division for a segment at least 64 cells wide, and no special-face or
+// output path compiled in (126 against 149 registers: four waves per SIMD
+// instead of three).  FAST = false runs pml_update on each lane's list
+// position (the flagged chunks, and every chunk in other steps).  Padding
+// lanes (after a segment's last row, in the last line of a segment) have no
+// list position: they leave as soon as their coordinates show it.
+template <typename T, bool PLAIN, bool PULL, bool FAST>
 __global__ __launch_bounds__(kChunk) void k_pml_chunks(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                                        PullArgs a, int slot, const int4* __restrict__ rec) {
   const int4 e = rec[2 * blockIdx.x], f = rec[2 * blockIdx.x + 1];
@@ -1508,8 +1637,6 @@ __global__ __launch_bounds__(kChunk) void k_pml_chunks(Dev d, const T* __restric
   const int q = lane / L;
   if (!((e.x >> q) & 1)) return;
   const int i = f.x + L * __popc((unsigned)(e.x & ((1 << q) - 1))) + (lane - q * L);
-  // coordinates first: a lane on the padding after a segment's last row
-  // has no list position and leaves before any access
   const long long c = (long long)(e.x >> 4) * kChunk + lane;
   int sg = e.y >> 16, x = (e.y & 0xffff) + lane, y = e.z, z = e.w;
   const int w = seg_width(d, sg);
@@ -1528,18 +1655,20 @@ __global__ __launch_bounds__(kChunk) void k_pml_chunks(Dev d, const T* __restric
     z += (int)dz;
     if (z >= d.nz) return;
   }
-  if (!PULL || f.y || a.prev_mode != 1) {
+  if (!FAST) {
     pml_update<T, PLAIN, PULL>(d, Lin, Lout, a, slot, i);
     return;
   }
   double st[9];
   pml_state_preload<PLAIN>(d, i, st);
   double v[12], wv[12];
-  bulk_pull_pair<T, PLAIN>(d, Lin, a, c, x, y, z, sg, v, wv);
+  const long long c0 = (long long)(e.x >> 4) * kChunk;
+  chunk_pull_pair<T, PLAIN>(d, Lin, a, c0, (uint32_t)lane, x, y, z, sg, v, wv);
   flush_small(v, a.small);
   flush_small(wv, a.prev_small);
-  pml_node<T, PLAIN>(d, v, slot, i, (int32_t)c, a, wv, st);
-  store_cell(Lout, d.P, c, v);
+  // a cell word without flags: no output row, no special face
+  pml_node<T, PLAIN>(d, v, slot, i, (int32_t)c & kCellMask, a, wv, st);
+  store_links_u(Lout, d.P, c0, (uint32_t)lane, v);
 }
 
 template <typename T, bool PLAIN, bool PULL>

@@ -704,7 +704,7 @@ int finalize(tlm_handle* h) {
       if (line_pos[ln] < 0) line_pos[ln] = i;
       if (pr[i].orow >= 0 || pr[i].fdc >= 0) line_flag[ln] = 1;
     }
-    std::vector<int4> pc;
+    std::vector<int4> pc, pcf;  // unflagged, flagged
     long long covered = 0;
     for (long long k = 0; k < (S + scn::kChunk - 1) / scn::kChunk; k++) {
       int m = 0, first = -1, flag = 0, expect = -1;
@@ -726,11 +726,14 @@ int finalize(tlm_handle* h) {
       while (sg + 1 < nseg && c0 >= seg_base[sg + 1]) sg++;
       const long long w = seg_x0[sg + 1] - seg_x0[sg];
       const long long r = c0 - seg_base[sg], row = r / w;
-      pc.push_back(make_int4((int)(k << 4) | m, (int)(seg_x0[sg] + (r - row * w)) | (sg << 16), (int)(row % ny),
+      auto& to = flag ? pcf : pc;
+      to.push_back(make_int4((int)(k << 4) | m, (int)(seg_x0[sg] + (r - row * w)) | (sg << 16), (int)(row % ny),
                              (int)(row / ny)));
-      pc.push_back(make_int4(first, flag, 0, 0));
+      to.push_back(make_int4(first, flag, 0, 0));
     }
     if (covered != n_pure_pml) return fail(h, TLM_ERR_ARG, "PML chunk records do not cover the pure PML lines");
+    h->n_pml_chunks_fast = (int)(pc.size() / 2);
+    pc.insert(pc.end(), pcf.begin(), pcf.end());
     h->n_pml_chunks = (int)(pc.size() / 2);
     if ((rc = dev_upload(h, pc, &h->d_pml_chunks))) return rc;
     // k_mat_lines records (two int4 each): per 64-cell chunk and material
@@ -1444,14 +1447,21 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
     else hipLaunchKernelGGL((scn::k_pml_material<T, PLAIN, false>), g, dim3(256), 0, cs, d, Lin, Lout, a, slot);
   }
   if (h->n_pml_chunks) {
+    // the common step runs the unflagged chunks on the fast kernel and the
+    // flagged ones on the general one; other steps run every chunk on the
+    // general kernel
     ProfScope ps(h, TLM_KCLASS_PML);
-    dim3 g(h->n_pml_chunks);
-    if (pull)
-      hipLaunchKernelGGL((scn::k_pml_chunks<T, PLAIN, true>), g, dim3(scn::kChunk), 0, h->stream, d, Lin, Lout, a, slot,
-                         h->d_pml_chunks);
-    else
-      hipLaunchKernelGGL((scn::k_pml_chunks<T, PLAIN, false>), g, dim3(scn::kChunk), 0, h->stream, d, Lin, Lout, a,
-                         slot, h->d_pml_chunks);
+    const int nf = pull && a.prev_mode == 1 ? h->n_pml_chunks_fast : 0;
+    if (nf)
+      hipLaunchKernelGGL((scn::k_pml_chunks<T, PLAIN, true, true>), dim3(nf), dim3(scn::kChunk), 0, h->stream, d, Lin,
+                         Lout, a, slot, h->d_pml_chunks);
+    const int ng = h->n_pml_chunks - nf;
+    if (ng && pull)
+      hipLaunchKernelGGL((scn::k_pml_chunks<T, PLAIN, true, false>), dim3(ng), dim3(scn::kChunk), 0, h->stream, d, Lin,
+                         Lout, a, slot, h->d_pml_chunks + 2 * nf);
+    else if (ng)
+      hipLaunchKernelGGL((scn::k_pml_chunks<T, PLAIN, false, false>), dim3(ng), dim3(scn::kChunk), 0, h->stream, d,
+                         Lin, Lout, a, slot, h->d_pml_chunks + 2 * nf);
   }
   if (fork) {
     HIP_TRY(h, hipEventRecord(h->ev_join, h->side));
