@@ -98,7 +98,7 @@ struct tlm_handle {
   // y-band (mapping 2; -1 entries idle)
   std::vector<int32_t> bulk_blocks;
   const int4* d_mat_recs = nullptr;      // k_mat_lines records (two int4 each)
-  int n_mat_recs = 0;
+  int n_mat_recs = 0, n_mat_recs_low = 0;  // the first n_mat_recs_low: materials of filter order <= 1
   long long n_matline_cells = 0, n_matline_mat = 0;  // cells of material lines, their material cells
   int64_t matline_filter_bytes = 0;
   const int4* d_bulk_chunks = nullptr;   // k_bulk_chunks records

@@ -964,15 +964,16 @@ __device__ inline void bulk_pull_pair(const Dev& d, const T* __restrict__ L, con
   bulk_incident<PLAIN>(d, b, x, y, z, qd, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10], r[11], w);
 }
 
-// bulk_pull_pair for lane k of a chunk whose first cell c0 is the same in
-// every lane (cell c = c0 + k): the own, y-, z- and in-segment x-neighbour
-// values are read at a scalar base plus the lane's one offset register;
-// only a lane at a segment's first or last column computes its x-neighbour
-// index (nb_xm, nb_xp).  Same loads, same values as bulk_pull_pair.
-template <typename T, bool PLAIN>
-__device__ inline void chunk_pull_pair(const Dev& d, const T* __restrict__ L, const PullArgs& a, long long c0,
+// bulk_pull (PAIR false) or bulk_pull_pair (PAIR true) for lane k of a
+// chunk whose first cell c0 is the same in every lane (cell c = c0 + k): the
+// own, y-, z- and in-segment x-neighbour values are read at a scalar base
+// plus the lane's one offset register; only a lane at a segment's first or
+// last column computes its x-neighbour index (nb_xm, nb_xp).  Same loads,
+// same values as bulk_pull and bulk_pull_pair.
+template <typename T, bool PLAIN, bool PAIR>
+__device__ inline void chunk_pull_impl(const Dev& d, const T* __restrict__ L, const PullArgs& a, long long c0,
                                        uint32_t k, int x, int y, int z, int sg, double v[12], double w[12]) {
-  const T* __restrict__ Q = (const T*)a.prev;
+  const T* __restrict__ Q = PAIR ? (const T*)a.prev : L;
   const long long P = d.P;
   const long long c = c0 + k;
   const long long sy = seg_width(d, sg), sz = sy * d.ny;
@@ -980,28 +981,34 @@ __device__ inline void chunk_pull_pair(const Dev& d, const T* __restrict__ L, co
   R o[12], q[12];
   if (PLAIN) {
 #pragma unroll
-    for (int p = 0; p < 12; p++) { o[p] = lk_get_u(L, P, p, c0, k); q[p] = lk_get_u(Q, P, p, c0, k); }
+    for (int p = 0; p < 12; p++) {
+      o[p] = lk_get_u(L, P, p, c0, k);
+      if (PAIR) q[p] = lk_get_u(Q, P, p, c0, k);
+    }
   } else {
 #pragma unroll
     for (int p = 0; p < 12; p++) o[p] = q[p] = (R)0;
     const int xp[4] = {Vy_xmin, Vz_xmin, Vy_xmax, Vz_xmax};
 #pragma unroll
-    for (int j = 0; j < 4; j++) { o[xp[j]] = lk_get_u(L, P, xp[j], c0, k); q[xp[j]] = lk_get_u(Q, P, xp[j], c0, k); }
+    for (int j = 0; j < 4; j++) {
+      o[xp[j]] = lk_get_u(L, P, xp[j], c0, k);
+      if (PAIR) q[xp[j]] = lk_get_u(Q, P, xp[j], c0, k);
+    }
     if (y == 0) {
       o[Vx_ymin] = lk_get_u(L, P, Vx_ymin, c0, k); o[Vz_ymin] = lk_get_u(L, P, Vz_ymin, c0, k);
-      q[Vx_ymin] = lk_get_u(Q, P, Vx_ymin, c0, k); q[Vz_ymin] = lk_get_u(Q, P, Vz_ymin, c0, k);
+      if (PAIR) { q[Vx_ymin] = lk_get_u(Q, P, Vx_ymin, c0, k); q[Vz_ymin] = lk_get_u(Q, P, Vz_ymin, c0, k); }
     }
     if (y == d.ny - 1) {
       o[Vx_ymax] = lk_get_u(L, P, Vx_ymax, c0, k); o[Vz_ymax] = lk_get_u(L, P, Vz_ymax, c0, k);
-      q[Vx_ymax] = lk_get_u(Q, P, Vx_ymax, c0, k); q[Vz_ymax] = lk_get_u(Q, P, Vz_ymax, c0, k);
+      if (PAIR) { q[Vx_ymax] = lk_get_u(Q, P, Vx_ymax, c0, k); q[Vz_ymax] = lk_get_u(Q, P, Vz_ymax, c0, k); }
     }
     if (z == 0) {
       o[Vx_zmin] = lk_get_u(L, P, Vx_zmin, c0, k); o[Vy_zmin] = lk_get_u(L, P, Vy_zmin, c0, k);
-      q[Vx_zmin] = lk_get_u(Q, P, Vx_zmin, c0, k); q[Vy_zmin] = lk_get_u(Q, P, Vy_zmin, c0, k);
+      if (PAIR) { q[Vx_zmin] = lk_get_u(Q, P, Vx_zmin, c0, k); q[Vy_zmin] = lk_get_u(Q, P, Vy_zmin, c0, k); }
     }
     if (z == d.nz - 1) {
       o[Vx_zmax] = lk_get_u(L, P, Vx_zmax, c0, k); o[Vy_zmax] = lk_get_u(L, P, Vy_zmax, c0, k);
-      q[Vx_zmax] = lk_get_u(Q, P, Vx_zmax, c0, k); q[Vy_zmax] = lk_get_u(Q, P, Vy_zmax, c0, k);
+      if (PAIR) { q[Vx_zmax] = lk_get_u(Q, P, Vx_zmax, c0, k); q[Vy_zmax] = lk_get_u(Q, P, Vy_zmax, c0, k); }
     }
   }
   R m[12], r[12];
@@ -1010,45 +1017,58 @@ __device__ inline void chunk_pull_pair(const Dev& d, const T* __restrict__ L, co
   if (x > 0) {
     if (x > d.seg_x0[sg]) {
       m[0] = lk_get_u(L, P, Vy_xmax, c0 - 1, k); m[1] = lk_get_u(L, P, Vz_xmax, c0 - 1, k);
-      r[0] = lk_get_u(Q, P, Vy_xmax, c0 - 1, k); r[1] = lk_get_u(Q, P, Vz_xmax, c0 - 1, k);
+      if (PAIR) { r[0] = lk_get_u(Q, P, Vy_xmax, c0 - 1, k); r[1] = lk_get_u(Q, P, Vz_xmax, c0 - 1, k); }
     } else {
       const long long n = nb_xm(d, c, x, y, z, sg);
       m[0] = lk_get(L, P, Vy_xmax, n); m[1] = lk_get(L, P, Vz_xmax, n);
-      r[0] = lk_get(Q, P, Vy_xmax, n); r[1] = lk_get(Q, P, Vz_xmax, n);
+      if (PAIR) { r[0] = lk_get(Q, P, Vy_xmax, n); r[1] = lk_get(Q, P, Vz_xmax, n); }
     }
   }
   if (x + 1 < d.nx) {
     if (x + 1 < d.seg_x0[sg + 1]) {
       m[2] = lk_get_u(L, P, Vy_xmin, c0 + 1, k); m[3] = lk_get_u(L, P, Vz_xmin, c0 + 1, k);
-      r[2] = lk_get_u(Q, P, Vy_xmin, c0 + 1, k); r[3] = lk_get_u(Q, P, Vz_xmin, c0 + 1, k);
+      if (PAIR) { r[2] = lk_get_u(Q, P, Vy_xmin, c0 + 1, k); r[3] = lk_get_u(Q, P, Vz_xmin, c0 + 1, k); }
     } else {
       const long long n = nb_xp(d, c, x, y, z, sg);
       m[2] = lk_get(L, P, Vy_xmin, n); m[3] = lk_get(L, P, Vz_xmin, n);
-      r[2] = lk_get(Q, P, Vy_xmin, n); r[3] = lk_get(Q, P, Vz_xmin, n);
+      if (PAIR) { r[2] = lk_get(Q, P, Vy_xmin, n); r[3] = lk_get(Q, P, Vz_xmin, n); }
     }
   }
   if (y > 0) {
     m[4] = lk_get_u(L, P, Vx_ymax, c0 - sy, k); m[5] = lk_get_u(L, P, Vz_ymax, c0 - sy, k);
-    r[4] = lk_get_u(Q, P, Vx_ymax, c0 - sy, k); r[5] = lk_get_u(Q, P, Vz_ymax, c0 - sy, k);
+    if (PAIR) { r[4] = lk_get_u(Q, P, Vx_ymax, c0 - sy, k); r[5] = lk_get_u(Q, P, Vz_ymax, c0 - sy, k); }
   }
   if (y + 1 < d.ny) {
     m[6] = lk_get_u(L, P, Vx_ymin, c0 + sy, k); m[7] = lk_get_u(L, P, Vz_ymin, c0 + sy, k);
-    r[6] = lk_get_u(Q, P, Vx_ymin, c0 + sy, k); r[7] = lk_get_u(Q, P, Vz_ymin, c0 + sy, k);
+    if (PAIR) { r[6] = lk_get_u(Q, P, Vx_ymin, c0 + sy, k); r[7] = lk_get_u(Q, P, Vz_ymin, c0 + sy, k); }
   }
   if (z > 0) {
     m[8] = lk_get_u(L, P, Vx_zmax, c0 - sz, k); m[9] = lk_get_u(L, P, Vy_zmax, c0 - sz, k);
-    r[8] = lk_get_u(Q, P, Vx_zmax, c0 - sz, k); r[9] = lk_get_u(Q, P, Vy_zmax, c0 - sz, k);
+    if (PAIR) { r[8] = lk_get_u(Q, P, Vx_zmax, c0 - sz, k); r[9] = lk_get_u(Q, P, Vy_zmax, c0 - sz, k); }
   }
   if (z + 1 < d.nz) {
     m[10] = lk_get_u(L, P, Vx_zmin, c0 + sz, k); m[11] = lk_get_u(L, P, Vy_zmin, c0 + sz, k);
-    r[10] = lk_get_u(Q, P, Vx_zmin, c0 + sz, k); r[11] = lk_get_u(Q, P, Vy_zmin, c0 + sz, k);
+    if (PAIR) { r[10] = lk_get_u(Q, P, Vx_zmin, c0 + sz, k); r[11] = lk_get_u(Q, P, Vy_zmin, c0 + sz, k); }
   }
   double od[12], qd[12];
 #pragma unroll
   for (int j = 0; j < 12; j++) { od[j] = (double)o[j]; qd[j] = (double)q[j]; }
   bulk_incident<PLAIN>(d, a, x, y, z, od, m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], m[9], m[10], m[11], v);
-  PullArgs b{a.prev_apply_r, 0, 0.0, -1};
-  bulk_incident<PLAIN>(d, b, x, y, z, qd, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10], r[11], w);
+  if (PAIR) {
+    PullArgs b{a.prev_apply_r, 0, 0.0, -1};
+    bulk_incident<PLAIN>(d, b, x, y, z, qd, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10], r[11],
+                         w);
+  }
+}
+template <typename T, bool PLAIN>
+__device__ inline void chunk_pull_pair(const Dev& d, const T* __restrict__ L, const PullArgs& a, long long c0,
+                                       uint32_t k, int x, int y, int z, int sg, double v[12], double w[12]) {
+  chunk_pull_impl<T, PLAIN, true>(d, L, a, c0, k, x, y, z, sg, v, w);
+}
+template <typename T, bool PLAIN>
+__device__ inline void chunk_pull(const Dev& d, const T* __restrict__ L, const PullArgs& a, long long c0, uint32_t k,
+                                  int x, int y, int z, int sg, double v[12]) {
+  chunk_pull_impl<T, PLAIN, false>(d, L, a, c0, k, x, y, z, sg, v, nullptr);
 }
 
 // Bulk free-space cell update, one cell per lane: pull the six faces
@@ -2205,7 +2225,18 @@ __global__ __launch_bounds__(kChunk) void k_bulk_chunks(Dev d, const T* __restri
   } else if (!cell_coords(d, c, x, y, z, sg)) {
     return;
   }
-  bulk_update<T, PLAIN, PULL>(d, Lin, Lout, a, c, x, y, z, sg);
+  // bulk_update with the chunk's addressing
+  const long long c0 = (long long)(e.x >> 4) * kChunk;
+  double v[12];
+  if (PULL) {
+    chunk_pull<T, PLAIN>(d, Lin, a, c0, (uint32_t)lane, x, y, z, sg, v);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 12; p++) v[p] = (double)lk_get_u(Lin, d.P, p, c0, (uint32_t)lane);
+  }
+  flush_small(v, a.small);
+  if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
+  store_links_u(Lout, d.P, c0, (uint32_t)lane, v);
 }
 
 // Material lines: lines holding free-space cells and material cells of one
@@ -2221,7 +2252,10 @@ __global__ __launch_bounds__(kChunk) void k_bulk_chunks(Dev d, const T* __restri
 // cell has six ordinary faces: the bulk pull serves both kinds.  The node
 // updates are the reference's: scatter.F90:222-272 for free space, the
 // Thevenin node (:274-548) for the material cells.
-template <typename T, bool PLAIN, bool PULL>
+// LOW = true: records of materials whose filters have order <= 1 only
+// (mat_node_low; the general node update is not compiled in: 78 against
+// 100 registers).
+template <typename T, bool PLAIN, bool PULL, bool LOW = false>
 __device__ inline void mat_line_cell(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                      const PullArgs& a, int4 e, int4 f, int lane) {
   if (!((e.x >> (lane / d.line_len)) & 1)) return;
@@ -2242,8 +2276,8 @@ __device__ inline void mat_line_cell(const Dev& d, const T* __restrict__ Lin, T*
   // loaded here, so that they are in flight with the link pull
   const bool matl = (lanes >> lane) & 1u;
   const tlm_volume_material& vm = d.vmat[f.x > 0 ? f.x - 1 : 0];
-  const bool low = vm.type == TLM_VMAT_DISPERSIVE && (!vm.eps_filter_exists || vm.eps_order <= 1) &&
-                   (!vm.mu_filter_exists || vm.mu_order <= 1);
+  const bool low = LOW || (vm.type == TLM_VMAT_DISPERSIVE && (!vm.eps_filter_exists || vm.eps_order <= 1) &&
+                           (!vm.mu_filter_exists || vm.mu_order <= 1));
   const long long slot = matl ? f.w + 3 * (long long)__popcll(lanes & ((1ull << lane) - 1ull)) : 0;
   double e0[3] = {0.0, 0.0, 0.0}, m0[3] = {0.0, 0.0, 0.0};
   if (matl && low) {
@@ -2257,28 +2291,29 @@ __device__ inline void mat_line_cell(const Dev& d, const T* __restrict__ Lin, T*
     }
   }
   double v[12];
+  const long long c0 = (long long)(e.x >> 4) * kChunk;
   if (PULL) {
-    bulk_pull<T, PLAIN>(d, Lin, a, c, x, y, z, sg, v);
+    chunk_pull<T, PLAIN>(d, Lin, a, c0, (uint32_t)lane, x, y, z, sg, v);
   } else {
 #pragma unroll
-    for (int p = 0; p < 12; p++) v[p] = (double)lk_get(Lin, d.P, p, c);
+    for (int p = 0; p < 12; p++) v[p] = (double)lk_get_u(Lin, d.P, p, c0, (uint32_t)lane);
   }
   flush_small(v, a.small);
   if (matl) {
-    if (low) mat_node_low(d, v, vm, slot, e0, m0);
+    if (LOW || low) mat_node_low(d, v, vm, slot, e0, m0);
     else node_update<false>(d, v, f.x, slot, -1, -1, 0, 0);
   } else if (PLAIN) {
     scatter_free_plain(v);
   } else {
     scatter_free_fast(v);
   }
-  store_cell(Lout, d.P, c, v);
+  store_links_u(Lout, d.P, c0, (uint32_t)lane, v);
 }
 
-template <typename T, bool PLAIN, bool PULL>
+template <typename T, bool PLAIN, bool PULL, bool LOW>
 __global__ __launch_bounds__(kChunk) void k_mat_lines(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                                       PullArgs a, const int4* __restrict__ rec) {
-  mat_line_cell<T, PLAIN, PULL>(d, Lin, Lout, a, rec[2 * blockIdx.x], rec[2 * blockIdx.x + 1], threadIdx.x);
+  mat_line_cell<T, PLAIN, PULL, LOW>(d, Lin, Lout, a, rec[2 * blockIdx.x], rec[2 * blockIdx.x + 1], threadIdx.x);
 }
 
 // blocks[0..n_blocks) lists the storage blocks to update (-1: none); a

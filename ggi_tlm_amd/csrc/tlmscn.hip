@@ -791,6 +791,24 @@ int finalize(tlm_handle* h) {
         mrec.push_back(make_int4(mq.first, (int)(uint32_t)lanes, (int)(uint32_t)(lanes >> 32), slot0));
       }
     }
+    // records of materials whose filters have order <= 1 first (the
+    // kernel variant without the general node update), the others after
+    {
+      auto is_low = [&](int m) {
+        const tlm_volume_material& vm = h->vmat[m - 1];
+        return vm.type == TLM_VMAT_DISPERSIVE && (!vm.eps_filter_exists || vm.eps_order <= 1) &&
+               (!vm.mu_filter_exists || vm.mu_order <= 1);
+      };
+      std::vector<int4> lo, hi;
+      for (size_t k = 0; k < mrec.size(); k += 2) {
+        auto& to = is_low(mrec[k + 1].x) ? lo : hi;
+        to.push_back(mrec[k]);
+        to.push_back(mrec[k + 1]);
+      }
+      h->n_mat_recs_low = (int)(lo.size() / 2);
+      lo.insert(lo.end(), hi.begin(), hi.end());
+      mrec.swap(lo);
+    }
     h->n_mat_recs = (int)(mrec.size() / 2);
     if ((rc = dev_upload(h, mrec, &h->d_mat_recs))) return rc;
   }
@@ -1416,13 +1434,21 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
                          h->d_slot_cell, h->d_slot_idx, (long long)h->n_mixed_blocks * 256, timestep, slot);
   }
   if (h->n_mat_recs) {
+    // order <= 1 materials' records, then the others
     ProfScope ps(h, TLM_KCLASS_MATERIAL);
-    if (pull)
-      hipLaunchKernelGGL((scn::k_mat_lines<T, PLAIN, true>), dim3(h->n_mat_recs), dim3(scn::kChunk), 0, cs, d, Lin,
-                         Lout, a, h->d_mat_recs);
-    else
-      hipLaunchKernelGGL((scn::k_mat_lines<T, PLAIN, false>), dim3(h->n_mat_recs), dim3(scn::kChunk), 0, cs, d, Lin,
-                         Lout, a, h->d_mat_recs);
+    const int nl = h->n_mat_recs_low, nh = h->n_mat_recs - nl;
+    if (nl && pull)
+      hipLaunchKernelGGL((scn::k_mat_lines<T, PLAIN, true, true>), dim3(nl), dim3(scn::kChunk), 0, cs, d, Lin, Lout, a,
+                         h->d_mat_recs);
+    else if (nl)
+      hipLaunchKernelGGL((scn::k_mat_lines<T, PLAIN, false, true>), dim3(nl), dim3(scn::kChunk), 0, cs, d, Lin, Lout,
+                         a, h->d_mat_recs);
+    if (nh && pull)
+      hipLaunchKernelGGL((scn::k_mat_lines<T, PLAIN, true, false>), dim3(nh), dim3(scn::kChunk), 0, cs, d, Lin, Lout,
+                         a, h->d_mat_recs + 2 * nl);
+    else if (nh)
+      hipLaunchKernelGGL((scn::k_mat_lines<T, PLAIN, false, false>), dim3(nh), dim3(scn::kChunk), 0, cs, d, Lin, Lout,
+                         a, h->d_mat_recs + 2 * nl);
   }
   if (h->n_pure_mat) {
     ProfScope ps(h, TLM_KCLASS_MATERIAL);
