@@ -86,6 +86,7 @@ struct tlm_handle {
   Dev d{};
   int n_gen = 0, n_pml = 0, n_mat = 0;
   int n_pure_pml = 0, n_pure_mat = 0, n_pure_gen = 0, n_mixed_blocks = 0;
+  int n_free_gen = 0;  // the first n_free_gen generic list entries: code 0 (free space beside a special face)
   int n_pml_fd_pure = 0, n_pml_fd_all = 0;  // PML cells with a special face
   int n_huy_coded = 0;  // descriptor entries in the kFdHuy form
   int n_pm = 0;         // PML cells holding a material

@@ -2130,11 +2130,13 @@ __device__ inline void generic_node(const Dev& d, double v[12], int timestep, in
 // Generic special cells: point sources, probes, cells touching a special
 // face; the Thevenin node, or the free-space branch for code 0 cells that
 // only touch a special face.
-template <typename T, bool PLAIN, bool PULL>
+// FREE = true: a cell of code 0 (free space beside a special face), whose
+// node is the free-space scatter; the Thevenin node is not compiled in.
+template <typename T, bool PLAIN, bool PULL, bool FREE = false>
 __device__ inline void generic_update(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
                                       const PullArgs& a, int timestep, int slot, int i) {
   const long long c = d.gen_cell[i];
-  const int code = d.gen_code[i];  // read with the list entry, not after the pull
+  const int code = FREE ? 0 : d.gen_code[i];  // read with the list entry, not after the pull
   double v[12];
   if (PULL) {
     int x, y, z, sg;
@@ -2145,15 +2147,20 @@ __device__ inline void generic_update(const Dev& d, const T* __restrict__ Lin, T
     for (int p = 0; p < 12; p++) v[p] = ld(Lin, d.P, p, c);
   }
   flush_small(v, a.small);
-  generic_node<PLAIN>(d, v, timestep, slot, i, code);
+  if (FREE) {
+    if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
+  } else {
+    generic_node<PLAIN>(d, v, timestep, slot, i, code);
+  }
   store_cell(Lout, d.P, c, v);
 }
 
-template <typename T, bool PLAIN, bool PULL>
+// Generic list entries [i0, i1).
+template <typename T, bool PLAIN, bool PULL, bool FREE>
 __global__ __launch_bounds__(256) void k_generic(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                PullArgs a, int timestep, int slot, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) generic_update<T, PLAIN, PULL>(d, Lin, Lout, a, timestep, slot, i);
+                                                PullArgs a, int timestep, int slot, int i0, int i1) {
+  const int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < i1) generic_update<T, PLAIN, PULL, FREE>(d, Lin, Lout, a, timestep, slot, i);
 }
 
 // Material cells (dispersive, PEC, PMC) without sources, probes or special

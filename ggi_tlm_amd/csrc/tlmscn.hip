@@ -396,6 +396,11 @@ int finalize(tlm_handle* h) {
   while (n_pure_pml < (int)pr.size() && pure_of(pr[n_pure_pml].c)) n_pure_pml++;
   while (n_pure_mat < (int)mr.size() && pure_of(mr[n_pure_mat].c)) n_pure_mat++;
   while (n_pure_gen < (int)gr.size() && pure_of(gr[n_pure_gen].c)) n_pure_gen++;
+  // pure generic cells with code 0 (free-space cells beside a special face)
+  // first: the generic kernel's variant without the Thevenin node runs them
+  std::stable_partition(gr.begin(), gr.begin() + n_pure_gen, [](const GenRec& r) { return r.code == 0; });
+  int n_free_gen = 0;
+  while (n_free_gen < n_pure_gen && gr[n_free_gen].code == 0) n_free_gen++;
   // Mixed lines: workgroups of 256 lanes, each owning whole lines.  A
   // special cell takes one lane, bulk cells kMixedBulkPerLane per lane; a
   // group takes consecutive mixed lines (storage order) while its lanes fit.
@@ -575,6 +580,7 @@ int finalize(tlm_handle* h) {
     }
   h->n_pure_mat = n_pure_mat;
   h->n_pure_gen = n_pure_gen;
+  h->n_free_gen = n_free_gen;
   h->n_mixed_cells = n_mixed_cells;
   if (exc_rank != h->n_cell_rows)
     return fail(h, TLM_ERR_ARG, "cell excitation rows do not match the cell codes");
@@ -1457,14 +1463,22 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
     else hipLaunchKernelGGL((scn::k_material<T, PLAIN, false>), g, dim3(256), 0, cs, d, Lin, Lout, a, h->n_pure_mat);
   }
   if (h->n_pure_gen) {
+    // code-0 cells (free space beside a special face) on the variant without
+    // the Thevenin node, then the others
     ProfScope ps(h, TLM_KCLASS_GENERIC);
-    dim3 g((h->n_pure_gen + 255) / 256);
-    if (pull)
-      hipLaunchKernelGGL((scn::k_generic<T, PLAIN, true>), g, dim3(256), 0, cs, d, Lin, Lout, a, timestep, slot,
-                         h->n_pure_gen);
-    else
-      hipLaunchKernelGGL((scn::k_generic<T, PLAIN, false>), g, dim3(256), 0, cs, d, Lin, Lout, a, timestep, slot,
-                         h->n_pure_gen);
+    const int nf = h->n_free_gen, nr = h->n_pure_gen - nf;
+    if (nf && pull)
+      hipLaunchKernelGGL((scn::k_generic<T, PLAIN, true, true>), dim3((nf + 255) / 256), dim3(256), 0, cs, d, Lin, Lout,
+                         a, timestep, slot, 0, nf);
+    else if (nf)
+      hipLaunchKernelGGL((scn::k_generic<T, PLAIN, false, true>), dim3((nf + 255) / 256), dim3(256), 0, cs, d, Lin,
+                         Lout, a, timestep, slot, 0, nf);
+    if (nr && pull)
+      hipLaunchKernelGGL((scn::k_generic<T, PLAIN, true, false>), dim3((nr + 255) / 256), dim3(256), 0, cs, d, Lin,
+                         Lout, a, timestep, slot, nf, h->n_pure_gen);
+    else if (nr)
+      hipLaunchKernelGGL((scn::k_generic<T, PLAIN, false, false>), dim3((nr + 255) / 256), dim3(256), 0, cs, d, Lin,
+                         Lout, a, timestep, slot, nf, h->n_pure_gen);
   }
   if (h->n_pm) {
     ProfScope ps(h, TLM_KCLASS_PML_MATERIAL);
