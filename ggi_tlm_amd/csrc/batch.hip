@@ -34,8 +34,10 @@ struct BatchSolve {
   const int32_t* lds_fd;
   const int32_t* lds_slot_p;
   const int32_t* lds_slot_i;
-  const int4* mat_recs;      // material-line records (two int4 each)
-  int n_pure_pml, n_pure_mat, n_pure_gen, n_pm, n_huy_coded;
+  const int4* mat_recs;      // material-line records (two int4 each; order <= 1 materials first)
+  const int4* pml_chunks;    // PML chunk records (two int4 each; unflagged chunks first)
+  const int4* bulk_chunks;   // bulk chunk records
+  int n_pure_pml, n_pure_mat, n_pure_gen, n_free_gen, n_pm, n_huy_coded;
   int cur, prv, prv_mode;    // link buffer rotation, as in the single-solve handle
   int ts, first, last;       // next timestep, first and last timestep
   int stz_n;                 // small-to-zero period (0: off)
@@ -69,7 +71,11 @@ struct tlm_batch {
   // block tables: lines, mixed (lane slots, or LDS-staged without PML
   // cells), LDS-staged with PML cells, pml, material, generic,
   // pml-material, huygens, dft
-  enum { T_LINES, T_MIXED, T_MIXED_P, T_PML, T_MAT, T_GEN, T_PM, T_HUY, T_DFT, T_MATL, T_N };
+  // lines (bulk chunk records), mixed, mixed with PML cells, PML chunks
+  // without / with flagged cells, material list, generic code-0 cells and
+  // the other generic cells, pml-material, huygens, dft, material lines of
+  // order <= 1 materials and of the others
+  enum { T_LINES, T_MIXED, T_MIXED_P, T_PML, T_PMLG, T_MAT, T_GENF, T_GEN, T_PM, T_HUY, T_DFT, T_MATL, T_MATLH, T_N };
   int2* d_tab[T_N] = {};
   int n_tab[T_N] = {};
   size_t cap_tab[T_N] = {};
@@ -155,8 +161,10 @@ __device__ inline void batch_pull_args(const BatchSolve& s, PullArgs& a, bool& p
   slot = s.ts - s.first;
 }
 
+// Pure bulk lines of a solve, one wavefront per chunk record (k_bulk_chunks).
 template <typename T, bool PLAIN>
-__global__ __launch_bounds__(256) void kb_lines(const BatchSolve* __restrict__ S, const int2* __restrict__ tab) {
+__global__ __launch_bounds__(scn::kChunk) void kb_lines(const BatchSolve* __restrict__ S,
+                                                        const int2* __restrict__ tab) {
   const int2 e = tab[blockIdx.x];
   const BatchSolve& s = S[e.x];
   if (s.ts > s.last) return;
@@ -166,9 +174,28 @@ __global__ __launch_bounds__(256) void kb_lines(const BatchSolve* __restrict__ S
   batch_pull_args<T, PLAIN>(s, a, pull, slot);
   const T* Lin = (const T*)s.planes[s.cur];
   T* Lout = (T*)s.planes[batch_out(s)];
-  const long long c = (long long)e.y * 256 + threadIdx.x;
-  if (pull) scn::bulk_line_cell<T, PLAIN, true>(s.d, Lin, Lout, a, c);
-  else scn::bulk_line_cell<T, PLAIN, false>(s.d, Lin, Lout, a, c);
+  const int4 r = s.bulk_chunks[e.y];
+  if (pull) scn::bulk_chunk_cell<T, PLAIN, true>(s.d, Lin, Lout, a, r, threadIdx.x);
+  else scn::bulk_chunk_cell<T, PLAIN, false>(s.d, Lin, Lout, a, r, threadIdx.x);
+}
+
+// PML cells of pure PML lines of a solve, one wavefront per chunk record
+// (k_pml_chunks; FAST: the records without flagged cells).
+template <typename T, bool PLAIN, bool FAST>
+__global__ __launch_bounds__(scn::kChunk) void kb_pml_chunks(const BatchSolve* __restrict__ S,
+                                                             const int2* __restrict__ tab) {
+  const int2 e = tab[blockIdx.x];
+  const BatchSolve& s = S[e.x];
+  if (s.ts > s.last) return;
+  PullArgs a;
+  bool pull;
+  int slot;
+  batch_pull_args<T, PLAIN>(s, a, pull, slot);
+  const T* Lin = (const T*)s.planes[s.cur];
+  T* Lout = (T*)s.planes[batch_out(s)];
+  const int4 r0 = s.pml_chunks[2 * e.y], r1 = s.pml_chunks[2 * e.y + 1];
+  if (pull) scn::pml_chunk_cell<T, PLAIN, true, FAST>(s.d, Lin, Lout, a, slot, r0, r1, threadIdx.x);
+  else scn::pml_chunk_cell<T, PLAIN, false, FAST>(s.d, Lin, Lout, a, slot, r0, r1, threadIdx.x);
 }
 
 template <typename T, bool PLAIN, bool PULL>
@@ -230,8 +257,9 @@ __global__ __launch_bounds__(scn::kLdsCells) void kb_mixed_lds(const BatchSolve*
                                              s.lds_slot_i, s.ts, slot, sv);
 }
 
-// Material lines of a solve, one wavefront per record (k_mat_lines).
-template <typename T, bool PLAIN>
+// Material lines of a solve, one wavefront per record (k_mat_lines; LOW:
+// the records of order <= 1 materials).
+template <typename T, bool PLAIN, bool LOW>
 __global__ __launch_bounds__(scn::kChunk) void kb_mat_lines(const BatchSolve* __restrict__ S,
                                                             const int2* __restrict__ tab) {
   const int2 e = tab[blockIdx.x];
@@ -244,18 +272,20 @@ __global__ __launch_bounds__(scn::kChunk) void kb_mat_lines(const BatchSolve* __
   const T* Lin = (const T*)s.planes[s.cur];
   T* Lout = (T*)s.planes[batch_out(s)];
   const int4 r0 = s.mat_recs[2 * e.y], r1 = s.mat_recs[2 * e.y + 1];
-  if (pull) scn::mat_line_cell<T, PLAIN, true>(s.d, Lin, Lout, a, r0, r1, threadIdx.x);
-  else scn::mat_line_cell<T, PLAIN, false>(s.d, Lin, Lout, a, r0, r1, threadIdx.x);
+  if (pull) scn::mat_line_cell<T, PLAIN, true, LOW>(s.d, Lin, Lout, a, r0, r1, threadIdx.x);
+  else scn::mat_line_cell<T, PLAIN, false, LOW>(s.d, Lin, Lout, a, r0, r1, threadIdx.x);
 }
 
-// One class list of a solve: 0 PML, 1 material, 2 generic, 3 PML holding a material.
+// One class list of a solve: 1 material, 2 generic cells other than code 0,
+// 3 PML holding a material, 4 generic code-0 cells (free space beside a
+// special face).
 template <typename T, bool PLAIN, int CLS>
 __global__ __launch_bounds__(256) void kb_list(const BatchSolve* __restrict__ S, const int2* __restrict__ tab) {
   const int2 e = tab[blockIdx.x];
   const BatchSolve& s = S[e.x];
   if (s.ts > s.last) return;
-  const int i = e.y * 256 + threadIdx.x;
-  const int n = CLS == 0 ? s.n_pure_pml : CLS == 1 ? s.n_pure_mat : CLS == 2 ? s.n_pure_gen : s.n_pm;
+  const int i = (CLS == 2 ? s.n_free_gen : 0) + e.y * 256 + threadIdx.x;
+  const int n = CLS == 1 ? s.n_pure_mat : CLS == 2 ? s.n_pure_gen : CLS == 4 ? s.n_free_gen : s.n_pm;
   if (i >= n) return;
   PullArgs a;
   bool pull;
@@ -265,14 +295,14 @@ __global__ __launch_bounds__(256) void kb_list(const BatchSolve* __restrict__ S,
   const T* Lin = (const T*)s.planes[s.cur];
   T* Lout = (T*)s.planes[batch_out(s)];
   if (pull) {
-    if (CLS == 0) scn::pml_update<T, PLAIN, true>(d, Lin, Lout, a, slot, i);
-    else if (CLS == 1) scn::material_update<T, PLAIN, true>(d, Lin, Lout, a, i);
+    if (CLS == 1) scn::material_update<T, PLAIN, true>(d, Lin, Lout, a, i);
     else if (CLS == 2) scn::generic_update<T, PLAIN, true>(d, Lin, Lout, a, s.ts, slot, i);
+    else if (CLS == 4) scn::generic_update<T, PLAIN, true, true>(d, Lin, Lout, a, s.ts, slot, i);
     else scn::pml_material_update<T, PLAIN, true>(d, Lin, Lout, a, slot, i);
   } else {
-    if (CLS == 0) scn::pml_update<T, PLAIN, false>(d, Lin, Lout, a, slot, i);
-    else if (CLS == 1) scn::material_update<T, PLAIN, false>(d, Lin, Lout, a, i);
+    if (CLS == 1) scn::material_update<T, PLAIN, false>(d, Lin, Lout, a, i);
     else if (CLS == 2) scn::generic_update<T, PLAIN, false>(d, Lin, Lout, a, s.ts, slot, i);
+    else if (CLS == 4) scn::generic_update<T, PLAIN, false, true>(d, Lin, Lout, a, s.ts, slot, i);
     else scn::pml_material_update<T, PLAIN, false>(d, Lin, Lout, a, slot, i);
   }
 }
@@ -338,18 +368,21 @@ void launch_batch_step(tlm_batch* b) {
     (void)hipEventRecord(b->ev_fork, st);
     (void)hipStreamWaitEvent(b->side, b->ev_fork, 0);
   }
-  go(st, tlm_batch::T_LINES, kb_lines<T, PLAIN>);
+  go(st, tlm_batch::T_LINES, kb_lines<T, PLAIN>, scn::kChunk);
   if (batch_lds(b)) {
     go(cs, tlm_batch::T_MIXED, kb_mixed_lds<T, PLAIN, scn::kMixedMasks[0]>, scn::kLdsCells);
     go(cs, tlm_batch::T_MIXED_P, kb_mixed_lds<T, PLAIN, scn::kMixedMasks[1]>, scn::kLdsCells);
   } else if constexpr (!std::is_same<T, scn::xlink>::value) {
     go(cs, tlm_batch::T_MIXED, kb_mixed<T, PLAIN>);
   }
-  go(cs, tlm_batch::T_MATL, kb_mat_lines<T, PLAIN>, scn::kChunk);
+  go(cs, tlm_batch::T_MATL, kb_mat_lines<T, PLAIN, true>, scn::kChunk);
+  go(cs, tlm_batch::T_MATLH, kb_mat_lines<T, PLAIN, false>, scn::kChunk);
   go(cs, tlm_batch::T_MAT, kb_list<T, PLAIN, 1>);
+  go(cs, tlm_batch::T_GENF, kb_list<T, PLAIN, 4>);
   go(cs, tlm_batch::T_GEN, kb_list<T, PLAIN, 2>);
   go(cs, tlm_batch::T_PM, kb_list<T, PLAIN, 3>);
-  go(st, tlm_batch::T_PML, kb_list<T, PLAIN, 0>);
+  go(st, tlm_batch::T_PML, kb_pml_chunks<T, PLAIN, true>, scn::kChunk);
+  go(st, tlm_batch::T_PMLG, kb_pml_chunks<T, PLAIN, false>, scn::kChunk);
   if (fork) {
     (void)hipEventRecord(b->ev_join, b->side);
     (void)hipStreamWaitEvent(st, b->ev_join, 0);
@@ -404,9 +437,12 @@ int rebuild(tlm_batch* b) {
     s.lds_slot_p = h->d_lds_slot_p;
     s.lds_slot_i = h->d_lds_slot_i;
     s.mat_recs = h->d_mat_recs;
+    s.pml_chunks = h->d_pml_chunks;
+    s.bulk_chunks = h->d_bulk_chunks;
     s.n_pure_pml = h->n_pure_pml;
     s.n_pure_mat = h->n_pure_mat;
     s.n_pure_gen = h->n_pure_gen;
+    s.n_free_gen = h->n_free_gen;
     s.n_pm = h->n_pm;
     s.n_huy_coded = h->n_huy_coded;
     // the host copy of ts / parity is exact: every step advances every
@@ -425,7 +461,7 @@ int rebuild(tlm_batch* b) {
     s.dft = h->d_dft;
     s.dt = h->g.dt;
     if (e.ts > e.last) continue;  // finished, awaiting collection
-    for (int32_t k : h->bulk_blocks) tab[tlm_batch::T_LINES].push_back(make_int2(i, k));
+    for (int k = 0; k < h->n_bulk_chunks; k++) tab[tlm_batch::T_LINES].push_back(make_int2(i, k));
     if (batch_lds(b)) {
       static const int lt[2] = {tlm_batch::T_MIXED, tlm_batch::T_MIXED_P};
       int wg = 0;
@@ -437,10 +473,13 @@ int rebuild(tlm_batch* b) {
     auto add = [&](int t, long long cnt) {
       for (long long k = 0; k < (cnt + 255) / 256; k++) tab[t].push_back(make_int2(i, (int)k));
     };
-    for (int k = 0; k < h->n_mat_recs; k++) tab[tlm_batch::T_MATL].push_back(make_int2(i, k));
-    add(tlm_batch::T_PML, h->n_pure_pml);
+    for (int k = 0; k < h->n_mat_recs; k++)
+      tab[k < h->n_mat_recs_low ? tlm_batch::T_MATL : tlm_batch::T_MATLH].push_back(make_int2(i, k));
+    for (int k = 0; k < h->n_pml_chunks; k++)
+      tab[k < h->n_pml_chunks_fast ? tlm_batch::T_PML : tlm_batch::T_PMLG].push_back(make_int2(i, k));
     add(tlm_batch::T_MAT, h->n_pure_mat);
-    add(tlm_batch::T_GEN, h->n_pure_gen);
+    add(tlm_batch::T_GENF, h->n_free_gen);
+    add(tlm_batch::T_GEN, h->n_pure_gen - h->n_free_gen);
     add(tlm_batch::T_PM, h->n_pm);
     if (h->n_huy_coded) add(tlm_batch::T_HUY, h->d.n_huy);
     if (s.d.n_out_rows && s.np && s.nf) add(tlm_batch::T_DFT, (long long)s.np * s.nf);

@@ -1631,6 +1631,48 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
   v[Vynz] = v[Vynz] * exp_y; v[Vypz] = v[Vypz] * exp_y;
 }
 
+// A PML cell without output row or special face, list position i, cell
+// c0 + k of a chunk (c0 the same in every lane) at (x, y, z) of segment sg:
+// pml_update's operations with the chunk's addressing.
+template <typename T, bool PLAIN, bool PULL>
+__device__ inline void pml_fast_cell(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                     const PullArgs& a, int slot, int i, long long c0, uint32_t k, int x, int y,
+                                     int z, int sg) {
+  double st[9];
+  pml_state_preload<PLAIN>(d, i, st);
+  double v[12], wv[12];
+  if (PULL && a.prev_mode == 1) {
+    // the common step: both pulls in one round trip
+    chunk_pull_pair<T, PLAIN>(d, Lin, a, c0, k, x, y, z, sg, v, wv);
+  } else {
+    // the first steps: this step's incident voltages pulled (or, without a
+    // pending connect, the buffer's own values), and the previous step's as
+    // pml_prev_incident gives them (none, pulled, or the buffer's own values)
+    if (PULL) {
+      chunk_pull<T, PLAIN>(d, Lin, a, c0, k, x, y, z, sg, v);
+    } else {
+#pragma unroll
+      for (int p = 0; p < 12; p++) v[p] = (double)lk_get_u(Lin, d.P, p, c0, k);
+    }
+    const T* __restrict__ Q = (const T*)a.prev;
+    if (a.prev_mode == 1) {
+      PullArgs b{a.prev_apply_r, 0, 0.0, -1};
+      chunk_pull<T, PLAIN>(d, Q, b, c0, k, x, y, z, sg, wv);
+    } else if (a.prev_mode == 2) {
+#pragma unroll
+      for (int p = 0; p < 12; p++) wv[p] = (double)lk_get_u(Q, d.P, p, c0, k);
+    } else {
+#pragma unroll
+      for (int p = 0; p < 12; p++) wv[p] = 0.0;
+    }
+  }
+  flush_small(v, a.small);
+  flush_small(wv, a.prev_small);
+  // a cell word without flags: no output row, no special face
+  pml_node<T, PLAIN>(d, v, slot, i, (int32_t)(c0 + k) & kCellMask, a, wv, st);
+  store_links_u(Lout, d.P, c0, k, v);
+}
+
 // PML cells of pure PML lines from per-chunk records (two int4 each, read
 // by all lanes at one address):
 //   e = (chunk << 4 | mask of the chunk's pure PML lines,
@@ -1640,19 +1682,18 @@ __device__ inline void pml_node(const Dev& d, double v[12], int slot, int i, int
 // The pure-line cells of the PML list are in storage order, so a lane's
 // list position follows from the lines of the mask below its own.  Records
 // come in two lists: chunks without flagged cells, and chunks with one.
-// FAST = true runs the common step (pull, previous buffer resident) on the
-// first list: cell and coordinates from the record, no list word, no
-// division for a segment at least 64 cells wide, and no special-face or
-// output path compiled in (126 against 149 registers: four waves per SIMD
-// instead of three).  FAST = false runs pml_update on each lane's list
-// position (the flagged chunks, and every chunk in other steps).  Padding
+// FAST = true runs the first list: cell and coordinates from the record, no
+// list word, no division for a segment at least 64 cells wide, and no
+// special-face or output path compiled in (126 against 149 registers: four
+// waves per SIMD instead of three).  Its incident voltages come as
+// pml_update's do in every step kind (pending connect or not; previous
+// buffer pulled, read as it is, or absent).  FAST = false runs pml_update
+// on each lane's list position (the flagged chunks).  Padding
 // lanes (after a segment's last row, in the last line of a segment) have no
 // list position: they leave as soon as their coordinates show it.
 template <typename T, bool PLAIN, bool PULL, bool FAST>
-__global__ __launch_bounds__(kChunk) void k_pml_chunks(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                       PullArgs a, int slot, const int4* __restrict__ rec) {
-  const int4 e = rec[2 * blockIdx.x], f = rec[2 * blockIdx.x + 1];
-  const int lane = threadIdx.x;
+__device__ inline void pml_chunk_cell(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                      const PullArgs& a, int slot, int4 e, int4 f, int lane) {
   const int L = d.line_len;
   const int q = lane / L;
   if (!((e.x >> q) & 1)) return;
@@ -1679,16 +1720,15 @@ __global__ __launch_bounds__(kChunk) void k_pml_chunks(Dev d, const T* __restric
     pml_update<T, PLAIN, PULL>(d, Lin, Lout, a, slot, i);
     return;
   }
-  double st[9];
-  pml_state_preload<PLAIN>(d, i, st);
-  double v[12], wv[12];
-  const long long c0 = (long long)(e.x >> 4) * kChunk;
-  chunk_pull_pair<T, PLAIN>(d, Lin, a, c0, (uint32_t)lane, x, y, z, sg, v, wv);
-  flush_small(v, a.small);
-  flush_small(wv, a.prev_small);
-  // a cell word without flags: no output row, no special face
-  pml_node<T, PLAIN>(d, v, slot, i, (int32_t)c & kCellMask, a, wv, st);
-  store_links_u(Lout, d.P, c0, (uint32_t)lane, v);
+  pml_fast_cell<T, PLAIN, PULL>(d, Lin, Lout, a, slot, i, (long long)(e.x >> 4) * kChunk, (uint32_t)lane, x, y, z,
+                                sg);
+}
+
+template <typename T, bool PLAIN, bool PULL, bool FAST>
+__global__ __launch_bounds__(kChunk) void k_pml_chunks(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                       PullArgs a, int slot, const int4* __restrict__ rec) {
+  pml_chunk_cell<T, PLAIN, PULL, FAST>(d, Lin, Lout, a, slot, rec[2 * blockIdx.x], rec[2 * blockIdx.x + 1],
+                                       threadIdx.x);
 }
 
 template <typename T, bool PLAIN, bool PULL>
@@ -2214,11 +2254,26 @@ __device__ inline void bulk_line_cell(const Dev& d, const T* __restrict__ Lin, T
 // precedes the link loads.  (Reading each line's class from the line table
 // before the loads measured 7% slower: 0.240 against 0.224 ms on the 256^3
 // workload.)  mode 1: each XCD takes a contiguous range of table records.
+// bulk_update of cell c0 + k of a chunk (c0 the same in every lane) with
+// the chunk's addressing.
 template <typename T, bool PLAIN, bool PULL>
-__global__ __launch_bounds__(kChunk) void k_bulk_chunks(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                                        PullArgs a, const int4* __restrict__ tab, int mode) {
-  const int4 e = tab[xcd_block(blockIdx.x, gridDim.x, mode)];
-  const int lane = threadIdx.x;
+__device__ inline void bulk_fast_cell(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                      const PullArgs& a, long long c0, uint32_t k, int x, int y, int z, int sg) {
+  double v[12];
+  if (PULL) {
+    chunk_pull<T, PLAIN>(d, Lin, a, c0, k, x, y, z, sg, v);
+  } else {
+#pragma unroll
+    for (int p = 0; p < 12; p++) v[p] = (double)lk_get_u(Lin, d.P, p, c0, k);
+  }
+  flush_small(v, a.small);
+  if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
+  store_links_u(Lout, d.P, c0, k, v);
+}
+
+template <typename T, bool PLAIN, bool PULL>
+__device__ inline void bulk_chunk_cell(const Dev& d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                       const PullArgs& a, int4 e, int lane) {
   if (!((e.x >> (lane / d.line_len)) & 1)) return;
   const long long c = (long long)(e.x >> 4) * kChunk + lane;
   int sg = e.y >> 16, x = (e.y & 0xffff) + lane, y = e.z, z = e.w;
@@ -2232,18 +2287,13 @@ __global__ __launch_bounds__(kChunk) void k_bulk_chunks(Dev d, const T* __restri
   } else if (!cell_coords(d, c, x, y, z, sg)) {
     return;
   }
-  // bulk_update with the chunk's addressing
-  const long long c0 = (long long)(e.x >> 4) * kChunk;
-  double v[12];
-  if (PULL) {
-    chunk_pull<T, PLAIN>(d, Lin, a, c0, (uint32_t)lane, x, y, z, sg, v);
-  } else {
-#pragma unroll
-    for (int p = 0; p < 12; p++) v[p] = (double)lk_get_u(Lin, d.P, p, c0, (uint32_t)lane);
-  }
-  flush_small(v, a.small);
-  if (PLAIN) scatter_free_plain(v); else scatter_free_fast(v);
-  store_links_u(Lout, d.P, c0, (uint32_t)lane, v);
+  bulk_fast_cell<T, PLAIN, PULL>(d, Lin, Lout, a, (long long)(e.x >> 4) * kChunk, (uint32_t)lane, x, y, z, sg);
+}
+
+template <typename T, bool PLAIN, bool PULL>
+__global__ __launch_bounds__(kChunk) void k_bulk_chunks(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
+                                                        PullArgs a, const int4* __restrict__ tab, int mode) {
+  bulk_chunk_cell<T, PLAIN, PULL>(d, Lin, Lout, a, tab[xcd_block(blockIdx.x, gridDim.x, mode)], threadIdx.x);
 }
 
 // Material lines: lines holding free-space cells and material cells of one

@@ -1487,13 +1487,14 @@ int launch_scatter(tlm_handle* h, int timestep, bool pull, const PullArgs& a, in
     else hipLaunchKernelGGL((scn::k_pml_material<T, PLAIN, false>), g, dim3(256), 0, cs, d, Lin, Lout, a, slot);
   }
   if (h->n_pml_chunks) {
-    // the common step runs the unflagged chunks on the fast kernel and the
-    // flagged ones on the general one; other steps run every chunk on the
-    // general kernel
+    // unflagged chunks on the fast kernel, flagged ones on the general one
     ProfScope ps(h, TLM_KCLASS_PML);
-    const int nf = pull && a.prev_mode == 1 ? h->n_pml_chunks_fast : 0;
-    if (nf)
+    const int nf = h->n_pml_chunks_fast;
+    if (nf && pull)
       hipLaunchKernelGGL((scn::k_pml_chunks<T, PLAIN, true, true>), dim3(nf), dim3(scn::kChunk), 0, h->stream, d, Lin,
+                         Lout, a, slot, h->d_pml_chunks);
+    else if (nf)
+      hipLaunchKernelGGL((scn::k_pml_chunks<T, PLAIN, false, true>), dim3(nf), dim3(scn::kChunk), 0, h->stream, d, Lin,
                          Lout, a, slot, h->d_pml_chunks);
     const int ng = h->n_pml_chunks - nf;
     if (ng && pull)
