@@ -475,12 +475,11 @@ int tlm_get_schedule(const tlm_handle* h, int32_t* variant, int32_t* rows, int32
                      int32_t* mixed, int32_t* streams);
 
 /* Streams of a step (no counterpart in the reference): 1 = every kernel of
- * a step on the handle's stream in turn; 2 (default) = the special-cell
- * class kernels (mixed lines, material, generic, PML cells holding a
- * material) on a second stream, concurrent with the bulk and PML kernels
- * (they write disjoint cells and read only the previous step's buffer),
- * joined before the step ends.  Profiling serialises them.  Results do not
- * depend on it. */
+ * a step on the handle's stream in turn; 2 (default) = the material,
+ * generic and PML-material class kernels on a second stream, concurrent
+ * with the bulk, PML and mixed-line kernels (they write disjoint cells and
+ * read only the previous step's buffer), joined before the step ends.
+ * Profiling serialises them.  Results do not depend on it. */
 int tlm_set_streams(tlm_handle* h, int32_t streams);
 
 /* Kernel classes timed by the profiling hooks. */
