@@ -349,11 +349,13 @@ __global__ void kb_advance(BatchSolve* S, int n) {
 // LDS-staged unless lane slots are asked for (the extended form has the LDS-staged kernel only)
 bool batch_lds(const tlm_batch* b) { return b->mixed != 0 || b->form == TLM_FORM_EXTENDED_FP32; }
 
-// One batch step.  With two streams the class kernels (mixed lines,
-// materials, generic cells, PML cells holding a material) run on the side
-// stream concurrently with the line and PML kernels: they write disjoint
-// cells and read only the previous step's buffers; the probe DFT and the
-// advance follow the join.
+// One batch step.  With two streams the PML, material, generic and
+// PML-material kernels run on the side stream concurrently with the bulk
+// and mixed-line kernels: they write disjoint cells and read only the
+// previous step's buffers; the probe DFT and the advance follow the join.
+// (PML on the side stream and mixed lines on the main one after the bulk
+// lines measured 2.7% faster on the serving requests than the other way
+// round; the single solve keeps PML on its main stream.)
 template <typename T, bool PLAIN>
 void launch_batch_step(tlm_batch* b) {
   const hipStream_t st = b->stream;
@@ -370,19 +372,19 @@ void launch_batch_step(tlm_batch* b) {
   }
   go(st, tlm_batch::T_LINES, kb_lines<T, PLAIN>, scn::kChunk);
   if (batch_lds(b)) {
-    go(cs, tlm_batch::T_MIXED, kb_mixed_lds<T, PLAIN, scn::kMixedMasks[0]>, scn::kLdsCells);
-    go(cs, tlm_batch::T_MIXED_P, kb_mixed_lds<T, PLAIN, scn::kMixedMasks[1]>, scn::kLdsCells);
+    go(st, tlm_batch::T_MIXED, kb_mixed_lds<T, PLAIN, scn::kMixedMasks[0]>, scn::kLdsCells);
+    go(st, tlm_batch::T_MIXED_P, kb_mixed_lds<T, PLAIN, scn::kMixedMasks[1]>, scn::kLdsCells);
   } else if constexpr (!std::is_same<T, scn::xlink>::value) {
-    go(cs, tlm_batch::T_MIXED, kb_mixed<T, PLAIN>);
+    go(st, tlm_batch::T_MIXED, kb_mixed<T, PLAIN>);
   }
+  go(cs, tlm_batch::T_PML, kb_pml_chunks<T, PLAIN, true>, scn::kChunk);
+  go(cs, tlm_batch::T_PMLG, kb_pml_chunks<T, PLAIN, false>, scn::kChunk);
   go(cs, tlm_batch::T_MATL, kb_mat_lines<T, PLAIN, true>, scn::kChunk);
   go(cs, tlm_batch::T_MATLH, kb_mat_lines<T, PLAIN, false>, scn::kChunk);
   go(cs, tlm_batch::T_MAT, kb_list<T, PLAIN, 1>);
   go(cs, tlm_batch::T_GENF, kb_list<T, PLAIN, 4>);
   go(cs, tlm_batch::T_GEN, kb_list<T, PLAIN, 2>);
   go(cs, tlm_batch::T_PM, kb_list<T, PLAIN, 3>);
-  go(st, tlm_batch::T_PML, kb_pml_chunks<T, PLAIN, true>, scn::kChunk);
-  go(st, tlm_batch::T_PMLG, kb_pml_chunks<T, PLAIN, false>, scn::kChunk);
   if (fork) {
     (void)hipEventRecord(b->ev_join, b->side);
     (void)hipStreamWaitEvent(st, b->ev_join, 0);

@@ -50,10 +50,10 @@ int tlm_batch_info(const tlm_batch* b, int32_t* n_solves, int32_t* n_unfinished,
 /* Steps per graph replay (default 32). */
 int tlm_batch_set_graph_steps(tlm_batch* b, int32_t k);
 /* Kernels of a batch step: mixed = 1 LDS-staged mixed-line kernel, 0 lane
- * slots, -1 the default (LDS-staged); streams = 2
- * (default) runs the class kernels on a second stream concurrently with
- * the line and PML kernels, 1 runs every kernel in turn.  Results do not
- * depend on either. */
+ * slots, -1 the default (LDS-staged); streams = 2 (default) runs the PML,
+ * material, generic and PML-material kernels on a second stream
+ * concurrently with the bulk-line and mixed-line kernels, 1 runs every
+ * kernel in turn.  Results do not depend on either. */
 int tlm_batch_set_kernels(tlm_batch* b, int32_t mixed, int32_t streams);
 /* Time candidate graph lengths (4, 16, 64 steps) with each mixed-line
  * kernel of the form (the extended form has the LDS-staged one only) on
