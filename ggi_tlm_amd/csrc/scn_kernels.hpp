@@ -20,9 +20,11 @@
 //
 // Every 128-byte line of a link plane is written whole by one kernel.  Lines
 // holding one class only run in that class's wavefront-uniform kernel:
-//   bulk     free-space cells with six ordinary faces (line kernel)
-//   pml      PML cells (scatter_PML.F90), FP64 state in SoA arrays
-//   material dispersive/PEC/PMC cells without sources, probes or special faces
+//   bulk     free-space cells with six ordinary faces (chunk records)
+//   pml      PML cells (scatter_PML.F90) from chunk records, state in SoA
+//            arrays (FP64, or 48-bit streamed words in the reduced forms)
+//   material dispersive/PEC/PMC cells without sources, probes or special
+//            faces (material-line records, or a list)
 //   generic  point sources, probes and cells touching a special face
 //            (PEC/PMC/free-space surface, face sources), the Thevenin path of
 //            scatter.F90
@@ -32,8 +34,9 @@
 // own.
 //
 // Template parameters: T is the link storage type (double: plain form,
-// float: precision form); PULL selects between reading incident voltages
-// directly (state uploaded after a connect) and pulling them from the
+// float: precision form, xlink: extended form); PULL selects between
+// reading incident voltages directly (state uploaded after a connect) and
+// pulling them from the
 // scattered planes.  In the plain form every expression keeps the
 // reference's operation order and the file is compiled with
 // -ffp-contract=off, so results are bit-identical to the reference's
@@ -1731,12 +1734,6 @@ __global__ __launch_bounds__(kChunk) void k_pml_chunks(Dev d, const T* __restric
                                        threadIdx.x);
 }
 
-template <typename T, bool PLAIN, bool PULL>
-__global__ __launch_bounds__(256) void k_pml(Dev d, const T* __restrict__ Lin, T* __restrict__ Lout,
-                                            PullArgs a, int slot, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) pml_update<T, PLAIN, PULL>(d, Lin, Lout, a, slot, i);
-}
 
 // PML cell holding a non-dispersive material, scatter_PML_material.F90:22-534
 // (two-step recursions for the stretched node voltages and currents, a
