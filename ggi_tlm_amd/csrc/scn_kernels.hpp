@@ -36,11 +36,10 @@
 // Template parameters: T is the link storage type (double: plain form,
 // float: precision form, xlink: extended form); PULL selects between
 // reading incident voltages directly (state uploaded after a connect) and
-// pulling them from the
-// scattered planes.  In the plain form every expression keeps the
-// reference's operation order and the file is compiled with
-// -ffp-contract=off, so results are bit-identical to the reference's
-// source-order FP64 arithmetic.
+// pulling them from the scattered planes.  In the plain form every
+// expression keeps the reference's operation order and the file is
+// compiled with -ffp-contract=off, so results are bit-identical to the
+// reference's source-order FP64 arithmetic.
 #pragma once
 
 #include <hip/hip_runtime.h>
